@@ -1,0 +1,121 @@
+/*
+ * CPU oracle for the panel-scoring path: TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of eschalign's score-only Gotoh DP and of the
+ * selection that consumes its scores, used as the checker for the HIP
+ * library. Nothing in the product links or calls this; only tests/,
+ * __graft_entry__.smoke() and bench.py's CPU-baseline leg load it.
+ *
+ * Every function cites the reference file:line it restates (paths are
+ * relative to the eschalign repository root; lib/ = crates/eschalign/src/).
+ *
+ * Pinning (what checks each function against something other than itself):
+ *   eo_score          192 parasail 1.3.4 results (tests/golden/align_golden.json),
+ *                     closed forms from lib/scalar.rs:378-430, brute-force
+ *                     enumeration of every alignment for n, m <= 5.
+ *   eo_fits_i16       the reference's own bounds test (lib/simd/mod.rs:1136-1140)
+ *                     and the closed-form bound.
+ *   eo_simd_*         equality with eo_score on the reference's own random
+ *                     suites (xorshift64* inputs reproduced bit for bit).
+ *   eo_select         lib/mapper.rs:571-589, 709-739 closed-form cases, and a
+ *                     sort-based restatement in the tests.
+ *   eo_pair_cands     lib/mapper/pair.rs:450-556 cases (score 120 paralog
+ *                     choice, min-score, strands) and a brute-force sort.
+ *   eo_adapter_*      lib/adapter.rs:116-197 known-answer tests.
+ *   eo_encode etc.    lib/alphabet.rs:86-105 known answers.
+ */
+#ifndef ESCH_ORACLE_H
+#define ESCH_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EO_WILDCARD 4
+#define EO_NEG (INT32_MIN / 4)
+
+/* lib/alphabet.rs:29-55 — letter -> code {A0,C1,G2,T/U3, everything else 4}. */
+void eo_encode(const uint8_t *letters, size_t n, uint8_t *out);
+/* lib/alphabet.rs:58-64 — reverse complement of codes, wildcard kept. */
+void eo_revcomp_codes(const uint8_t *codes, size_t n, uint8_t *out);
+
+/* lib/scoring.rs:70-100 — 0 when valid, else which rule failed (1..5). */
+int eo_scoring_validate(int32_t ma, int32_t mi, int32_t o, int32_t e);
+/* lib/scoring.rs:105-111 */
+int32_t eo_substitution(int32_t ma, int32_t mi, uint8_t a, uint8_t b);
+
+/* lib/scalar.rs:120-163 — the score-only DP (local=1: Smith-Waterman). */
+int32_t eo_score(const uint8_t *q, size_t n, const uint8_t *r, size_t m,
+                 int32_t ma, int32_t mi, int32_t o, int32_t e, int local);
+
+/* lib/simd/mod.rs:222-227 */
+int eo_fits_i16(size_t n, size_t max_ref_len, int32_t ma, int32_t mi,
+                int32_t o, int32_t e);
+
+/* Every query against every reference with eo_score, row-major
+ * out[q * n_refs + r], spread over `threads` POSIX threads. */
+void eo_score_matrix(const uint8_t *qcodes, const uint64_t *qoff,
+                     const int32_t *qlen, size_t nq, const uint8_t *rcodes,
+                     const uint64_t *roff, const int32_t *rlen, size_t nr,
+                     int32_t ma, int32_t mi, int32_t o, int32_t e, int local,
+                     int threads, int32_t *out);
+
+/* lib/mapper.rs:418-482 — one read's score-stage selection.
+ * fwd: n_refs scores; rev: n_refs scores or NULL (forward strand only).
+ * Candidates are taken in tie order (ref r forward, ref r reverse, r+1 ...).
+ * max_ties < 0 means "all" (None). Writes up to `cap` taken ties as
+ * (ref << 1) | reverse into ties. Returns the number taken. */
+int32_t eo_select(const int32_t *fwd, const int32_t *rev, size_t n_refs,
+                  int32_t min_score, int64_t max_ties, int32_t *best,
+                  int32_t *n_tied, int32_t *has_subopt, int32_t *subopt,
+                  int32_t *ties, size_t cap);
+
+/* lib/mapper/pair.rs:171-231 — eligible candidates of one pair, sorted by
+ * (-sum, ref, template strand). rows[m*2 + rv] are mate m's n_refs scores
+ * as sequenced (rv=0) or reverse-complemented (rv=1); a NULL row marks an
+ * empty mate (no candidates). Writes cand = (ref << 1) | template_reverse,
+ * sums and per-mate scores; returns the count (<= 2 * n_refs). */
+int32_t eo_pair_cands(const int32_t *const rows[4], size_t n_refs,
+                      int both_strands, int32_t min_score, int32_t *cand,
+                      int32_t *sum, int32_t *s0, int32_t *s1);
+
+/* lib/adapter.rs:68-91 — adapter bases at the 3' end of a read (letters). */
+size_t eo_adapter_find(const uint8_t *read, size_t n, const uint8_t *adapter,
+                       size_t alen, size_t min_overlap);
+/* lib/adapter.rs:95-110 */
+void eo_adapter_detect_pair(const uint8_t *r1, size_t l1, const uint8_t *r2,
+                            size_t l2, const uint8_t *a1, size_t la1,
+                            const uint8_t *a2, size_t la2, size_t min_overlap,
+                            size_t out[2]);
+
+/* ---- SIMD restatement (the CPU baseline; esch_simd.c) ---- */
+
+/* Which kernels the host CPU supports: bit 0 AVX2, bit 1 AVX-512BW. */
+int eo_simd_supported(void);
+
+typedef struct eo_profile eo_profile;
+/* lib/simd/mod.rs:286-331 (+ tprof :255-269): refs longest-first, stable,
+ * cut into groups of `lanes` (16 = AVX2, 32 = AVX-512BW). */
+eo_profile *eo_profile_build(const uint8_t *rcodes, const uint64_t *roff,
+                             const int32_t *rlen, size_t nr, int32_t ma,
+                             int32_t mi, int32_t o, int32_t e, int lanes);
+void eo_profile_free(eo_profile *p);
+/* lib/mapper.rs:216-222 + 242-252 score_all: row-major below
+ * transposed_min (<= 0: never transposed), transposed at and above, scalar
+ * when !fits_i16. kernel: -1 dispatch, 0 force row-major, 1 force transposed. */
+void eo_simd_score_all(const eo_profile *p, const uint8_t *q, size_t n,
+                       int local, int transposed_min, int kernel, int32_t *out);
+/* Batch driver for the CPU baseline: reads dealt to `threads` workers in
+ * chunks of 64 (crates/eschalign-cli/src/align.rs:1470), every query
+ * scored with eo_simd_score_all; out row-major [q * n_refs + r]. */
+void eo_simd_score_batch(const eo_profile *p, const uint8_t *qcodes,
+                         const uint64_t *qoff, const int32_t *qlen, size_t nq,
+                         int local, int threads, int32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
