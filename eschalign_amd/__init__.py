@@ -1,0 +1,581 @@
+"""eschalign_amd: MI355X-native panel scoring for eschalign.
+
+The host-side mirror of eschalign's score-backend interface, over the C ABI
+of libeschalign_amd.so (include/eschalign_amd.h). Names and semantics follow
+the reference so that code written against it reads the same:
+
+    Scoring, Mode          lib/scoring.rs:27-165
+    encode, reverse_complement_codes   lib/alphabet.rs:29-64
+    Panel                  lib/panel.rs:10-82
+    ScoreMatrix            lib/mapper.rs:94-139
+    GpuScorer, GpuError, device_visible, MAX_READ_LEN, MAX_REF_LEN
+                           lib/cuda/mod.rs:86-749
+
+There is no CPU fallback in this package: every score comes from the gfx950
+kernels, and a missing library raises at import of the scorer.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+__all__ = [
+    "Scoring", "Mode", "Panel", "ScoreMatrix", "GpuScorer", "GpuError", "DeviceBatch",
+    "ReadSelect", "PairSelect", "encode", "reverse_complement_codes", "device_visible",
+    "fits_i16", "MAX_READ_LEN", "MAX_REF_LEN", "lib_path", "load_library",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+lib_path = os.path.join(_HERE, "libeschalign_amd.so")
+
+MAX_READ_LEN = 4096
+MAX_REF_LEN = 16384
+
+# esch_status
+OK, UNAVAILABLE, DRIVER, CODE_OBJECT, REFERENCE_TOO_LONG, INVALID_ARG = 0, 1, 2, 3, 4, 5
+SCORING_INVALID, EMPTY_PANEL, EMPTY_REFERENCE, CAPACITY = 6, 7, 8, 9
+_STATUS = {
+    UNAVAILABLE: "Unavailable", DRIVER: "Driver", CODE_OBJECT: "Compile",
+    REFERENCE_TOO_LONG: "ReferenceTooLong", INVALID_ARG: "InvalidArg",
+    SCORING_INVALID: "ScoringInvalid", EMPTY_PANEL: "EmptyPanel",
+    EMPTY_REFERENCE: "EmptyReference", CAPACITY: "Capacity",
+}
+
+RUN_SCORES16, RUN_SCORES32, RUN_SELECT, RUN_PAIRS, RUN_PLAIN = 1, 2, 4, 8, 16
+RUN_NO_GRAPH, RUN_PAIR_BOTH = 32, 64
+
+# Every symbol the header declares; tests check the library exports them.
+EXPORTS = [
+    "esch_last_error", "esch_device_visible", "esch_scoring_validate", "esch_fits_i16",
+    "esch_scorer_new", "esch_scorer_free", "esch_scores_len", "esch_n_refs", "esch_prepare",
+    "esch_prepared_free", "esch_prepared_len", "esch_prepared_n_scored", "esch_submit",
+    "esch_finish", "esch_score_batch", "esch_device_name", "esch_geometry", "esch_registers",
+    "esch_local_bytes", "esch_score_batch_i32", "esch_select_batch", "esch_select_pairs",
+    "esch_batch_create", "esch_batch_free", "esch_batch_footprint", "esch_batch_load",
+    "esch_batch_load_concat", "esch_batch_run", "esch_batch_wait", "esch_batch_timing",
+    "esch_batch_stats", "esch_batch_read_scores32", "esch_batch_read_scores16",
+    "esch_batch_read_select", "esch_batch_read_pairs",
+]
+
+
+class GpuError(Exception):
+    """lib/cuda/mod.rs:144-182: why the scorer could not be built or run."""
+
+    def __init__(self, status: int, message: str):
+        self.status = status
+        self.kind = _STATUS.get(status, f"status {status}")
+        super().__init__(f"{self.kind}: {message}")
+
+
+_lib = None
+
+
+def load_library():
+    """Load libeschalign_amd.so; raises when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(lib_path):
+        raise ImportError(
+            f"{lib_path} is missing: build the HIP library first "
+            "(python -c 'import __graft_entry__ as g; g.build()')")
+    L = C.CDLL(lib_path)
+    vp, sz, i32, i64 = C.c_void_p, C.c_size_t, C.c_int32, C.c_int64
+    L.esch_last_error.restype = C.c_char_p
+    L.esch_device_visible.restype = C.c_int
+    L.esch_scoring_validate.argtypes = [i32] * 4
+    L.esch_fits_i16.argtypes = [sz, sz, i32, i32, i32, i32]
+    L.esch_scorer_new.argtypes = [C.c_int, vp, vp, sz, i32, i32, i32, i32, C.c_int, C.POINTER(vp)]
+    L.esch_scorer_free.argtypes = [vp]
+    L.esch_scorer_free.restype = None
+    L.esch_scores_len.argtypes = [vp, sz]
+    L.esch_n_refs.argtypes = [vp]
+    L.esch_n_refs.restype = sz
+    L.esch_prepare.argtypes = [vp, vp, vp, sz]
+    L.esch_prepare.restype = vp
+    L.esch_prepared_free.argtypes = [vp]
+    L.esch_prepared_free.restype = None
+    L.esch_prepared_len.argtypes = [vp]
+    L.esch_prepared_len.restype = sz
+    L.esch_prepared_n_scored.argtypes = [vp]
+    L.esch_prepared_n_scored.restype = sz
+    L.esch_submit.argtypes = [vp, vp, C.POINTER(vp)]
+    L.esch_finish.argtypes = [vp, vp, vp, vp]
+    L.esch_score_batch.argtypes = [vp, vp, vp, sz, vp, vp]
+    L.esch_device_name.argtypes = [vp, C.c_char_p, sz]
+    L.esch_geometry.argtypes = [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]
+    L.esch_geometry.restype = None
+    L.esch_registers.argtypes = [vp]
+    L.esch_local_bytes.argtypes = [vp]
+    L.esch_score_batch_i32.argtypes = [vp, vp, vp, sz, vp, vp]
+    L.esch_select_batch.argtypes = [vp, vp, vp, sz, C.c_int, i32, i64, sz, vp, vp]
+    L.esch_select_pairs.argtypes = [vp, vp, vp, vp, vp, sz, C.c_int, i32, sz, vp, vp]
+    L.esch_batch_create.argtypes = [vp, sz, sz, C.POINTER(vp)]
+    L.esch_batch_free.argtypes = [vp]
+    L.esch_batch_free.restype = None
+    L.esch_batch_footprint.argtypes = [vp, sz, sz]
+    L.esch_batch_footprint.restype = sz
+    L.esch_batch_load.argtypes = [vp, vp, vp, sz, C.c_int, C.c_int]
+    L.esch_batch_load_concat.argtypes = [vp, vp, vp, vp, sz, C.c_int, C.c_int]
+    L.esch_batch_run.argtypes = [vp, C.c_int, i32, i64, sz]
+    L.esch_batch_wait.argtypes = [vp]
+    L.esch_batch_timing.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.esch_batch_stats.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                   C.POINTER(C.c_uint64)]
+    L.esch_batch_read_scores32.argtypes = [vp, vp, vp]
+    L.esch_batch_read_scores16.argtypes = [vp, vp, vp]
+    L.esch_batch_read_select.argtypes = [vp, vp, vp]
+    L.esch_batch_read_pairs.argtypes = [vp, vp, vp]
+    _lib = L
+    return L
+
+
+def _check(st: int):
+    if st != OK:
+        raise GpuError(st, load_library().esch_last_error().decode(errors="replace"))
+
+
+def device_visible() -> bool:
+    """cuda::device_visible (lib/cuda/mod.rs:130-140)."""
+    return bool(load_library().esch_device_visible())
+
+
+# ---------------------------------------------------------------- alphabet
+
+_TABLE = np.full(256, 4, dtype=np.uint8)
+for _ch, _c in ((b"A", 0), (b"C", 1), (b"G", 2), (b"T", 3), (b"U", 3)):
+    _TABLE[_ch[0]] = _c
+    _TABLE[_ch.lower()[0]] = _c
+WILDCARD = 4
+
+
+def encode(letters) -> np.ndarray:
+    """Letters -> codes (lib/alphabet.rs:29-55): A/C/G/T(U) 0..3, else 4."""
+    if isinstance(letters, str):
+        letters = letters.encode()
+    if len(letters) == 0:
+        return np.zeros(0, np.uint8)
+    return _TABLE[np.frombuffer(bytes(letters), dtype=np.uint8)]
+
+
+def reverse_complement_codes(codes) -> np.ndarray:
+    """lib/alphabet.rs:58-64: reversed, 3 - c, wildcard kept."""
+    c = np.asarray(codes, dtype=np.uint8)[::-1].copy()
+    m = c != WILDCARD
+    c[m] = 3 - c[m]
+    return c
+
+
+# ---------------------------------------------------------------- scoring
+
+@dataclass(frozen=True)
+class Scoring:
+    """lib/scoring.rs:27-137. Default 2,-1,-10,-1."""
+
+    match_score: int = 2
+    mismatch: int = -1
+    gap_open: int = -10
+    gap_extend: int = -1
+
+    def __post_init__(self):
+        st = load_library().esch_scoring_validate(self.match_score, self.mismatch,
+                                                  self.gap_open, self.gap_extend)
+        if st != OK:
+            raise ValueError(load_library().esch_last_error().decode())
+
+    @classmethod
+    def parse(cls, s: str) -> "Scoring":
+        parts = [p.strip() for p in s.split(",")]
+        if len(parts) != 4:
+            raise ValueError(f"scoring {s!r} is not MATCH,MISMATCH,GAP_OPEN,GAP_EXTEND")
+        try:
+            v = [int(p) for p in parts]
+        except ValueError:
+            raise ValueError(f"scoring {s!r} is not MATCH,MISMATCH,GAP_OPEN,GAP_EXTEND") from None
+        return cls(*v)
+
+    def substitution(self, a: int, b: int) -> int:
+        return self.match_score if (a == b or a == WILDCARD or b == WILDCARD) else self.mismatch
+
+    def gap(self, length: int) -> int:
+        return self.gap_open + (length - 1) * self.gap_extend
+
+    def tuple(self):
+        return (self.match_score, self.mismatch, self.gap_open, self.gap_extend)
+
+    def __str__(self):
+        return ",".join(map(str, self.tuple()))
+
+
+class Mode:
+    """lib/scoring.rs:151-190."""
+
+    LOCAL = 0
+    SEMIGLOBAL = 1
+
+    @staticmethod
+    def parse(s: str) -> int:
+        if s == "local":
+            return Mode.LOCAL
+        if s in ("semiglobal", "semi-global", "overlap"):
+            return Mode.SEMIGLOBAL
+        raise ValueError(f"unknown mode {s!r}, expected local|semiglobal")
+
+    @staticmethod
+    def name(m: int) -> str:
+        return "local" if m == Mode.LOCAL else "semiglobal"
+
+
+def fits_i16(n: int, max_ref_len: int, scoring: Scoring) -> bool:
+    """lib/simd/mod.rs:222-227."""
+    return bool(load_library().esch_fits_i16(n, max_ref_len, *scoring.tuple()))
+
+
+# ---------------------------------------------------------------- panel
+
+class Panel:
+    """lib/panel.rs:10-82: references in the order given (= tie order)."""
+
+    def __init__(self, refs):
+        self.names: List[str] = []
+        self.seqs: List[bytes] = []
+        self.codes: List[np.ndarray] = []
+        for item in refs:
+            if isinstance(item, (tuple, list)) and len(item) == 2 and isinstance(item[0], str):
+                name, seq = item
+            else:
+                name, seq = f"r{len(self.names)}", item
+            if isinstance(seq, np.ndarray):
+                codes = np.ascontiguousarray(seq, dtype=np.uint8)
+                seq = bytes(b"ACGTN"[c] for c in codes)
+            else:
+                seq = seq.encode() if isinstance(seq, str) else bytes(seq)
+                codes = encode(seq)
+            if len(seq) == 0:
+                raise ValueError(f"reference {name!r} has no bases")
+            self.names.append(name)
+            self.seqs.append(seq)
+            self.codes.append(codes)
+        if not self.names:
+            raise ValueError("the reference panel is empty")
+
+    def __len__(self):
+        return len(self.names)
+
+    def max_len(self) -> int:
+        return max(len(c) for c in self.codes)
+
+    def total_len(self) -> int:
+        return int(sum(len(c) for c in self.codes))
+
+
+# ---------------------------------------------------------------- score matrix
+
+class ScoreMatrix:
+    """lib/mapper.rs:94-139: row-major i16 scores + which rows are present."""
+
+    def __init__(self, n_refs: int, scores: np.ndarray, scored: np.ndarray):
+        scores = np.asarray(scores)
+        scored = np.asarray(scored, dtype=bool)
+        if scores.size != scored.size * n_refs:
+            raise ValueError("row-major matrix")
+        self._n_refs = n_refs
+        self.scores = scores.reshape(scored.size, n_refs) if n_refs else scores.reshape(scored.size, 0)
+        self.scored = scored
+
+    from_parts = classmethod(lambda cls, n_refs, scores, scored: cls(n_refs, scores, scored))
+
+    def __len__(self):
+        return int(self.scored.size)
+
+    def is_empty(self):
+        return self.scored.size == 0
+
+    def n_refs(self):
+        return self._n_refs
+
+    def n_scored(self):
+        return int(self.scored.sum())
+
+    def row(self, k: int) -> Optional[np.ndarray]:
+        return self.scores[k] if self.scored[k] else None
+
+
+# ---------------------------------------------------------------- selection records
+
+@dataclass
+class ReadSelect:
+    """Score stage of Aligner::map_reads_scored (lib/mapper.rs:444-472)."""
+
+    best: Optional[int]
+    n_tied: int
+    suboptimal: Optional[int]
+    take: List[tuple]  # (ref, reverse), tie order, truncated to max_ties + 1
+
+
+@dataclass
+class PairSelect:
+    """Top tier of lib/mapper/pair.rs:185-231."""
+
+    n_cand: int
+    top_sum: Optional[int]
+    top: List[tuple]  # (ref, template_reverse)
+    n_top: int
+    next_sum: Optional[int]
+
+
+def _ptr_array(seqs):
+    """(keepalive, char** array, size_t* lens) for a list of code arrays."""
+    arrs = [np.ascontiguousarray(s, dtype=np.uint8) for s in seqs]
+    n = len(arrs)
+    ptrs = (C.c_void_p * max(n, 1))()
+    lens = (C.c_size_t * max(n, 1))()
+    for i, a in enumerate(arrs):
+        ptrs[i] = a.ctypes.data if a.size else None
+        lens[i] = a.size
+    return arrs, ptrs, lens
+
+
+# ---------------------------------------------------------------- scorer
+
+class PreparedBatch:
+    def __init__(self, ptr, n):
+        self._ptr, self.n = ptr, n
+
+    def __del__(self):
+        if getattr(self, "_ptr", None) and _lib is not None:
+            _lib.esch_prepared_free(self._ptr)
+
+
+class PendingBatch:
+    def __init__(self, ptr, n):
+        self._ptr, self.n = ptr, n
+
+
+class GpuScorer:
+    """lib/cuda/mod.rs:328-709: a panel resident on the device under one
+    scoring and one mode."""
+
+    def __init__(self, panel: Panel, scoring: Scoring = Scoring(), mode: int = Mode.LOCAL,
+                 ordinal: int = 0):
+        L = load_library()
+        self.panel = panel
+        self.scoring = scoring
+        self.mode = mode
+        keep, ptrs, lens = _ptr_array(panel.codes)
+        out = C.c_void_p()
+        _check(L.esch_scorer_new(ordinal, ptrs, lens, len(panel.codes), *scoring.tuple(), mode,
+                                 C.byref(out)))
+        self._h = out
+        self.n_refs = len(panel)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.esch_scorer_free(h)
+            self._h = None
+
+    close = __del__
+
+    # diagnostics (lib/cuda/mod.rs:446-488)
+    def device_name(self) -> str:
+        buf = C.create_string_buffer(256)
+        _check(_lib.esch_device_name(self._h, buf, 256))
+        return buf.value.decode()
+
+    def geometry(self):
+        a, b, c = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        _lib.esch_geometry(self._h, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
+    def registers(self) -> int:
+        return _lib.esch_registers(self._h)
+
+    def local_bytes(self) -> int:
+        return _lib.esch_local_bytes(self._h)
+
+    def scores_len(self, n: int) -> bool:
+        return bool(_lib.esch_scores_len(self._h, n))
+
+    # prepare / submit / finish (lib/cuda/mod.rs:524-708)
+    def prepare(self, queries: Sequence[np.ndarray]) -> PreparedBatch:
+        keep, ptrs, lens = _ptr_array(queries)
+        p = _lib.esch_prepare(self._h, ptrs, lens, len(keep))
+        if not p:
+            raise GpuError(INVALID_ARG, _lib.esch_last_error().decode())
+        return PreparedBatch(p, len(keep))
+
+    def submit(self, prepared: PreparedBatch) -> PendingBatch:
+        ptr = prepared._ptr
+        prepared._ptr = None  # consumed
+        out = C.c_void_p()
+        _check(_lib.esch_submit(self._h, ptr, C.byref(out)))
+        return PendingBatch(out, prepared.n)
+
+    def finish(self, pending: PendingBatch) -> ScoreMatrix:
+        n = pending.n
+        scores = np.zeros(max(n, 1) * max(self.n_refs, 1), np.int16)
+        scored = np.zeros(max(n, 1), np.uint8)
+        ptr = pending._ptr
+        pending._ptr = None
+        _check(_lib.esch_finish(self._h, ptr, scores.ctypes.data, scored.ctypes.data))
+        return ScoreMatrix(self.n_refs, scores[: n * self.n_refs], scored[:n].astype(bool))
+
+    def score_batch(self, queries: Sequence[np.ndarray]) -> ScoreMatrix:
+        return self.finish(self.submit(self.prepare(queries)))
+
+    # extensions
+    def score_batch_i32(self, queries: Sequence[np.ndarray]):
+        keep, ptrs, lens = _ptr_array(queries)
+        n = len(keep)
+        scores = np.zeros(max(n, 1) * self.n_refs, np.int32)
+        scored = np.zeros(max(n, 1), np.uint8)
+        _check(_lib.esch_score_batch_i32(self._h, ptrs, lens, n, scores.ctypes.data,
+                                         scored.ctypes.data))
+        return scores[: n * self.n_refs].reshape(n, self.n_refs), scored[:n].astype(bool)
+
+    def select_batch(self, reads: Sequence[np.ndarray], both_strands=False, min_score=0,
+                     max_ties=None, tie_cap=None) -> List[ReadSelect]:
+        keep, ptrs, lens = _ptr_array(reads)
+        n = len(keep)
+        cap = tie_cap or 2 * self.n_refs
+        recs = np.zeros((max(n, 1), 5), np.int32)
+        ties = np.zeros(max(n, 1) * cap, np.int32)
+        mt = -1 if max_ties is None else int(max_ties)
+        _check(_lib.esch_select_batch(self._h, ptrs, lens, n, int(both_strands), min_score, mt,
+                                      cap, recs.ctypes.data, ties.ctypes.data))
+        return [_read_select(recs[k], ties[k * cap:(k + 1) * cap]) for k in range(n)]
+
+    def select_pairs(self, mates1, mates2, both_strands=False, min_score=0, top_cap=None):
+        k1, p1, l1 = _ptr_array(mates1)
+        k2, p2, l2 = _ptr_array(mates2)
+        n = len(k1)
+        if len(k2) != n:
+            raise ValueError("mates1 and mates2 differ in length")
+        cap = top_cap or 2 * self.n_refs
+        recs = np.zeros((max(n, 1), 5), np.int32)
+        top = np.zeros(max(n, 1) * cap, np.int32)
+        _check(_lib.esch_select_pairs(self._h, p1, l1, p2, l2, n, int(both_strands), min_score,
+                                      cap, recs.ctypes.data, top.ctypes.data))
+        return [_pair_select(recs[k], top[k * cap:(k + 1) * cap]) for k in range(n)]
+
+    def batch(self, max_reads: int, max_bases: int) -> "DeviceBatch":
+        return DeviceBatch(self, max_reads, max_bases)
+
+    def batch_footprint(self, max_reads: int, max_bases: int) -> int:
+        return int(_lib.esch_batch_footprint(self._h, max_reads, max_bases))
+
+
+def _read_select(rec, ties) -> ReadSelect:
+    best, nt, so, flags, ntake = (int(v) for v in rec)
+    if not flags & 1:
+        return ReadSelect(None, 0, None, [])
+    take = [(int(t) >> 1, bool(t & 1)) for t in ties[:min(ntake, len(ties))]]
+    return ReadSelect(best, nt, so if flags & 2 else None, take)
+
+
+def _pair_select(rec, top) -> PairSelect:
+    ncand, tsum, ntop, nsum, flags = (int(v) for v in rec)
+    members = [(int(t) >> 1, bool(t & 1)) for t in top[:min(ntop, len(top))]]
+    return PairSelect(ncand, tsum if ncand else None, members, ntop,
+                      nsum if flags & 2 else None)
+
+
+class DeviceBatch:
+    """A device-resident batch: upload once, run the captured pipeline, read
+    back what is needed (include/eschalign_amd.h, esch_batch_*)."""
+
+    def __init__(self, scorer: GpuScorer, max_reads: int, max_bases: int):
+        self.scorer = scorer
+        out = C.c_void_p()
+        _check(_lib.esch_batch_create(scorer._h, max_reads, max_bases, C.byref(out)))
+        self._h = out
+        self.n_reads = 0
+        self.n_rows = 0
+        self.pairs = False
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.esch_batch_free(h)
+            self._h = None
+
+    close = __del__
+
+    def load(self, reads, both_strands=False, pairs=False):
+        keep, ptrs, lens = _ptr_array(reads)
+        _check(_lib.esch_batch_load(self._h, ptrs, lens, len(keep), int(both_strands), int(pairs)))
+        self._set(len(keep), both_strands, pairs)
+
+    def load_concat(self, codes: np.ndarray, offs: np.ndarray, lens: np.ndarray,
+                    both_strands=False, pairs=False):
+        codes = np.ascontiguousarray(codes, np.uint8)
+        offs = np.ascontiguousarray(offs, np.uint64)
+        lens = np.ascontiguousarray(lens, np.int32)
+        _check(_lib.esch_batch_load_concat(self._h, codes.ctypes.data, offs.ctypes.data,
+                                           lens.ctypes.data, len(lens), int(both_strands),
+                                           int(pairs)))
+        self._set(len(lens), both_strands, pairs)
+
+    def _set(self, n, both, pairs):
+        self.n_reads = n
+        self.n_rows = 2 * n if both else n
+        self.pairs = bool(pairs)
+
+    def run(self, what: int, min_score=0, max_ties=None, tie_cap=1):
+        mt = -1 if max_ties is None else int(max_ties)
+        self.tie_cap = max(1, int(tie_cap))
+        _check(_lib.esch_batch_run(self._h, what, min_score, mt, self.tie_cap))
+        self.what = what
+
+    def wait(self):
+        _check(_lib.esch_batch_wait(self._h))
+
+    def timing(self):
+        dp, tot = C.c_float(), C.c_float()
+        _check(_lib.esch_batch_timing(self._h, C.byref(dp), C.byref(tot)))
+        return dp.value, tot.value
+
+    def stats(self):
+        a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(_lib.esch_batch_stats(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def scores32(self):
+        nr = self.scorer.n_refs
+        s = np.zeros(max(self.n_rows, 1) * nr, np.int32)
+        ok = np.zeros(max(self.n_rows, 1), np.uint8)
+        _check(_lib.esch_batch_read_scores32(self._h, s.ctypes.data, ok.ctypes.data))
+        return s[: self.n_rows * nr].reshape(self.n_rows, nr), ok[: self.n_rows].astype(bool)
+
+    def scores16(self):
+        nr = self.scorer.n_refs
+        s = np.zeros(max(self.n_rows, 1) * nr, np.int16)
+        ok = np.zeros(max(self.n_rows, 1), np.uint8)
+        _check(_lib.esch_batch_read_scores16(self._h, s.ctypes.data, ok.ctypes.data))
+        return s[: self.n_rows * nr].reshape(self.n_rows, nr), ok[: self.n_rows].astype(bool)
+
+    def select(self) -> List[ReadSelect]:
+        n, cap = self.n_reads, self.tie_cap
+        recs = np.zeros((max(n, 1), 5), np.int32)
+        ties = np.zeros(max(n, 1) * cap, np.int32)
+        _check(_lib.esch_batch_read_select(self._h, recs.ctypes.data, ties.ctypes.data))
+        return [_read_select(recs[k], ties[k * cap:(k + 1) * cap]) for k in range(n)]
+
+    def select_raw(self):
+        n, cap = self.n_reads, self.tie_cap
+        recs = np.zeros((max(n, 1), 5), np.int32)
+        ties = np.zeros(max(n, 1) * cap, np.int32)
+        _check(_lib.esch_batch_read_select(self._h, recs.ctypes.data, ties.ctypes.data))
+        return recs[:n], ties[: n * cap].reshape(n, cap)
+
+    def pairs_select(self) -> List[PairSelect]:
+        n, cap = self.n_reads // 2, self.tie_cap
+        recs = np.zeros((max(n, 1), 5), np.int32)
+        top = np.zeros(max(n, 1) * cap, np.int32)
+        _check(_lib.esch_batch_read_pairs(self._h, recs.ctypes.data, top.ctypes.data))
+        return [_pair_select(recs[k], top[k * cap:(k + 1) * cap]) for k in range(n)]

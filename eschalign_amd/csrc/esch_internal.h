@@ -1,0 +1,107 @@
+// Internal layout shared by the gfx950 kernels and the host library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace esch {
+
+// Rows of the read held in registers per DP stripe.
+constexpr int kStripe = 16;
+// Query code of a padding row placed above a read (substitution score 0,
+// which reproduces the DP's zero top border exactly).
+constexpr uint8_t kPreCode = 5;
+// Reads per wave in the batched kernel: 64 lanes x 2 packed int16 halves.
+constexpr int kChunk = 128;
+// Stripe-count bins of the length bucketing; a longer read goes to the
+// int32 kernel.
+constexpr int kMaxStripes = 4096;
+constexpr int kMaxLen16 = kMaxStripes * kStripe;
+// Largest |value| bound under which the packed int16 recurrence is exact:
+// every real cell, E/F and intermediate sum lies in
+// [-(k*M + |o| + |e|), k*M] with k = min(read, ref) and M = max(|ma|, |mi|),
+// and -32768 (saturating) is the only -inf, so a bound below 32767 keeps
+// every real value away from both clamps.
+constexpr int64_t kBound16 = 32000;
+// -inf of the int32 kernel, the oracle's i32::MIN / 4 (lib/scalar.rs:42).
+constexpr int32_t kNeg32 = INT32_MIN / 4;
+
+struct Scoring {
+    int32_t ma, mi, o, e;
+    int32_t local;
+};
+
+// A panel resident on the device: 2-bit codes (16 per word, A0 C1 G2 T3,
+// wildcards stored as 0) plus a wildcard bitmap (32 per word), per
+// reference offsets in words.
+struct DevPanel {
+    const uint32_t *ref2;
+    const uint32_t *refN;
+    const uint32_t *w2off;
+    const uint32_t *nmoff;
+    const int32_t *rlen;
+    const int32_t *rorder;  // panel indices, longest first (stable)
+    int32_t n_refs;
+    int32_t max_len;
+    int64_t total_len;
+};
+
+// Device-resident batch descriptor: every kernel reads its sizes from here,
+// so one captured graph replays for any batch that fits the buffers.
+struct DevDesc {
+    int32_t n_queries;   // rows scored (reads, then reverse complements)
+    int32_t n_reads;     // reads as loaded (mates interleaved when pairs)
+    int32_t both;        // reverse complements appended
+    int32_t pairs;
+    int32_t dropin;      // classify by scores_len (int16 rows only)
+    int32_t force32;     // score everything with the int32 kernel
+    int32_t min_score;
+    int32_t tie_cap;
+    int64_t max_ties;
+    int32_t n16, n32;    // queries on the int16 and int32 kernels
+    uint32_t next16, next32a, next32b;  // work-queue cursors
+    int32_t tstrands;    // pairs: template strands considered (1 or 2)
+    uint32_t pad_[2];
+    uint64_t cells;      // sum of len x panel bases over scored queries
+};
+
+struct KArgs {
+    DevPanel panel;
+    Scoring sc;
+    DevDesc *desc;
+    const uint8_t *qcodes;
+    const uint64_t *qoff;
+    const int32_t *qlen;
+    uint8_t *cls;        // 0 unscored, 1 int16 kernel, 2 int32 kernel
+    int32_t *hist;       // [kMaxStripes + 1]
+    int32_t *bin_start;  // [kMaxStripes + 1]
+    int32_t *bin_fill;   // [kMaxStripes + 1]
+    int32_t *sorted16;   // int16-kernel queries, longest first
+    int32_t *list32;     // int32-kernel queries
+    int32_t *out32;      // [n_queries * n_refs] or null
+    int16_t *out16;      // [n_queries * n_refs] or null
+    uint2 *scratch16;    // [grid16 * 4 waves * max_len * 64]
+    int2 *scratch32;     // [max_len * grid32 threads]
+    int32_t *recs;       // select / pair records, 5 int32 each
+    int32_t *ties;       // [n * tie_cap]
+    uint8_t *rc_codes;   // where reverse complements are written
+};
+
+// Host-side launchers (kernels.hip); every kernel runs on `st`.
+struct Launch {
+    int grid16;   // persistent workgroups of k_dp16 (4 waves each)
+    int grid32;   // persistent blocks of k_dp32
+    int block32;  // threads per k_dp32 block
+    int grid_aux; // blocks of the bucketing / selection kernels
+};
+hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st);
+hipError_t launch_dp16(const KArgs &a, const Launch &g, bool local, hipStream_t st);
+hipError_t launch_dp32(const KArgs &a, const Launch &g, bool local, int which, hipStream_t st);
+hipError_t launch_select(const KArgs &a, const Launch &g, hipStream_t st);
+hipError_t launch_pairs(const KArgs &a, const Launch &g, hipStream_t st);
+hipError_t dp16_attributes(bool local, hipFuncAttributes *attr);
+hipError_t dp32_attributes(bool local, hipFuncAttributes *attr);
+hipError_t dp16_occupancy(bool local, int *blocks_per_cu);
+hipError_t dp32_occupancy(bool local, int block, int *blocks_per_cu);
+
+}  // namespace esch

@@ -1,0 +1,702 @@
+// gfx950 kernels of the panel-scoring path.
+//
+//   k_dp16   batched packed-int16 score DP (the hot path)
+//   k_dp32   plain one-thread-per-pair int32 score DP (baseline + rescue)
+//   k_zero, k_revcomp, k_classify, k_scan, k_scatter   length bucketing
+//   k_select, k_pairs   fused best / tie reductions
+//
+// The recurrence is eschalign's score-only Gotoh DP (lib/scalar.rs:120-163):
+//   E[i][j] = max(H[i][j-1] + o, E[i][j-1] + e)
+//   F[i][j] = max(H[i-1][j] + o, F[i-1][j] + e)
+//   H[i][j] = max(H[i-1][j-1] + s(q_i, r_j), E, F [, 0 local])
+// with zero borders and E/F = -inf on them; local mode takes the best cell
+// anywhere, semiglobal the best of the last row and the last column.
+#include "esch_internal.h"
+
+namespace esch {
+
+typedef short v2s __attribute__((ext_vector_type(2)));
+// Panel words are read-only for a kernel's lifetime: read them through the
+// constant address space so uniform indices become scalar loads.
+typedef const __attribute__((address_space(4))) uint32_t *cwords;
+
+__device__ __forceinline__ v2s adds(v2s a, v2s b) { return __builtin_elementwise_add_sat(a, b); }
+__device__ __forceinline__ v2s vmax(v2s a, v2s b) { return __builtin_elementwise_max(a, b); }
+__device__ __forceinline__ v2s splat(int v) { return v2s{(short)v, (short)v}; }
+__device__ __forceinline__ uint32_t as_u32(v2s v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ v2s as_v2s(uint32_t v) { return __builtin_bit_cast(v2s, v); }
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// ---------------------------------------------------------------------------
+// k_dp16 — batched packed-int16 DP.
+//
+// Layout. A workgroup of four waves scores one work item: a chunk of 128
+// reads of the length-bucketed order (two reads per lane, packed as the low
+// and high int16 halves of every register) against four panel references
+// of neighbouring length, one reference per wave. Within a wave the
+// reference is uniform, so its code per column is a scalar; the read side
+// of the substitution comes from a query profile in LDS, shared by the four
+// waves: plane c holds s(q_r, c) - o for the 16 rows of the current stripe
+// (plane 4, a wildcard reference base, the best of the four). Selecting a
+// plane is a uniform LDS offset, so a column is four ds_read_b128 and a
+// branch-free body of 8 packed VALU ops per row (4 saturating adds, 4 max)
+// in semiglobal mode and 10 in local mode, for two cells.
+//
+// Stripes. The read is walked in stripes of 16 rows held in registers
+// (Ho = H + o and E per row); the stripe's bottom row (Ho, F) per column is
+// the only state that leaves registers, to a per-wave scratch the next
+// stripe reads back (coalesced: scratch[j * 64 + lane]).
+//
+// Bottom alignment. Every read of a chunk is padded at the TOP to the
+// chunk's stripe count with rows whose substitution score is 0. Starting
+// from the zero border such rows keep H = 0 and E = F = o, which is exactly
+// what the first real row would see from the border (H = 0, F = -inf:
+// max(0 + o, -inf + e) = o), so padding changes no real cell, and the
+// read's last row is always register 15 of the last stripe.
+//
+// Saturation. Adds clamp to int16; a query is sent here only when its
+// value bound (kBound16) keeps every real value off both clamps, so the
+// result equals the int32 recurrence exactly. Others go to k_dp32.
+// ---------------------------------------------------------------------------
+
+constexpr int kWaves16 = 4;  // waves (references) per k_dp16 workgroup
+
+__device__ __forceinline__ int prof_val(int x, int c, const Scoring &sc) {
+    // s(x, c) - o for read code x against reference code c; padding rows 0 - o.
+    if (x == kPreCode) return -sc.o;
+    return ((x == c || x == 4) ? sc.ma : sc.mi) - sc.o;
+}
+
+__device__ __forceinline__ uint32_t pack2(int lo, int hi) {
+    return (uint32_t)(uint16_t)(int16_t)lo | ((uint32_t)(uint16_t)(int16_t)hi << 16);
+}
+
+template <bool LOCAL>
+__device__ __forceinline__ void column16(const v2s (&sc)[kStripe], v2s (&Ho)[kStripe],
+                                         v2s (&E)[kStripe], v2s &hou, v2s &f, v2s hod, v2s vo,
+                                         v2s ve, v2s &best) {
+#pragma unroll
+    for (int r = 0; r < kStripe; ++r) {
+        E[r] = vmax(Ho[r], adds(E[r], ve));
+        f = vmax(hou, adds(f, ve));
+        v2s h = vmax(vmax(adds(hod, sc[r]), E[r]), f);
+        if constexpr (LOCAL) {
+            h = vmax(h, splat(0));
+            best = vmax(best, h);
+        }
+        hod = Ho[r];
+        Ho[r] = adds(h, vo);
+        hou = Ho[r];
+    }
+}
+
+template <bool LOCAL>
+__global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
+    // prof[(c * 4 + quad) * 64 + lane] = rows 4 quad .. 4 quad + 3 of plane c
+    __shared__ uint4 prof[5 * 4 * 64];
+    __shared__ uint32_t s_item;
+    const int lane = threadIdx.x & 63;
+    const int wv = uni(threadIdx.x >> 6);
+    const DevPanel &P = a.panel;
+    const int n_refs = P.n_refs;
+    const int n_groups = (n_refs + kWaves16 - 1) / kWaves16;
+    const int n16 = a.desc->n16;
+    const int n_chunks = (n16 + kChunk - 1) / kChunk;
+    const uint32_t n_items = (uint32_t)n_chunks * (uint32_t)n_groups;
+    uint2 *scr = a.scratch16 + ((size_t)blockIdx.x * kWaves16 + wv) * (size_t)P.max_len * 64;
+    const v2s vo = splat(a.sc.o), ve = splat(a.sc.e), NEG = splat(-32768);
+
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_item = atomicAdd(&a.desc->next16, 1u);
+        __syncthreads();
+        const uint32_t t = s_item;
+        if (t >= n_items) break;
+        const int k = (int)(t / (uint32_t)n_groups);
+        const int g = (int)(t - (uint32_t)k * (uint32_t)n_groups);
+        const int slot = g * kWaves16 + wv;
+        const bool active = slot < n_refs;
+        const int ref = active ? uni(P.rorder[slot]) : 0;
+
+        const int base = k * kChunk + 2 * lane;
+        const int qa = base < n16 ? a.sorted16[base] : -1;
+        const int qb = base + 1 < n16 ? a.sorted16[base + 1] : -1;
+        const int nA = qa >= 0 ? a.qlen[qa] : 0;
+        const int nB = qb >= 0 ? a.qlen[qb] : 0;
+        // Longest first: the chunk's first read sets its stripe count.
+        const int nmax = uni(a.qlen[a.sorted16[k * kChunk]]);
+        const int S = (nmax + kStripe - 1) / kStripe;
+        const int pA = kStripe * S - nA, pB = kStripe * S - nB;
+        const uint8_t *qA = a.qcodes + (qa >= 0 ? a.qoff[qa] : 0);
+        const uint8_t *qB = a.qcodes + (qb >= 0 ? a.qoff[qb] : 0);
+
+        const int m = active ? uni(P.rlen[ref]) : 0;
+        const cwords w2 = (cwords)(P.ref2 + uni((int)P.w2off[ref]));
+        const cwords wn = (cwords)(P.refN + uni((int)P.nmoff[ref]));
+        // Reference code of column j: 2-bit code, or 4 when the wildcard bit is set.
+        auto code = [&](int j) -> int {
+            const uint32_t c2 = (w2[j >> 4] >> (2 * (j & 15))) & 3u;
+            return ((wn[j >> 5] >> (j & 31)) & 1u) ? 4 : (int)c2;
+        };
+
+        v2s best = LOCAL ? splat(0) : NEG;  // local: H; semiglobal: Ho = H + o
+
+        for (int s = 0; s < S; ++s) {
+            // This stripe's profile: wave wv builds rows 4 wv .. 4 wv + 3.
+            __syncthreads();
+            {
+                uint32_t pl[5][4];
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) {
+                    const int i = kStripe * s + 4 * wv + rr;
+                    const int xa = (i < pA) ? kPreCode : qA[i - pA];
+                    const int xb = (i < pB) ? kPreCode : qB[i - pB];
+                    int bestA = -32768, bestB = -32768;
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        const int va = prof_val(xa, c, a.sc), vb = prof_val(xb, c, a.sc);
+                        bestA = max(bestA, va);
+                        bestB = max(bestB, vb);
+                        pl[c][rr] = pack2(va, vb);
+                    }
+                    pl[4][rr] = pack2(bestA, bestB);
+                }
+#pragma unroll
+                for (int c = 0; c < 5; ++c)
+                    prof[(c * 4 + wv) * 64 + lane] = make_uint4(pl[c][0], pl[c][1], pl[c][2], pl[c][3]);
+            }
+            __syncthreads();
+            if (!active) continue;
+
+            v2s Ho[kStripe], E[kStripe];
+#pragma unroll
+            for (int r = 0; r < kStripe; ++r) {
+                Ho[r] = vo;  // H = 0 on the left border
+                E[r] = NEG;
+            }
+            v2s hdt = vo;  // Ho of the row above, column j - 1 (border: H = 0)
+            const bool first = s == 0, last = s + 1 == S;
+
+            // The stripe boundary of column j + 1 is fetched while column j
+            // computes.
+            uint2 bnext = first ? make_uint2(0u, 0u) : scr[lane];
+            for (int j = 0; j < m; ++j) {
+                const uint2 b = bnext;
+                if (!first && j + 1 < m) bnext = scr[(size_t)(j + 1) * 64 + lane];
+                const int c = code(j);
+                const uint4 *pc = prof + c * 4 * 64 + lane;
+                v2s sc[kStripe];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint4 v = pc[q * 64];
+                    sc[4 * q + 0] = as_v2s(v.x);
+                    sc[4 * q + 1] = as_v2s(v.y);
+                    sc[4 * q + 2] = as_v2s(v.z);
+                    sc[4 * q + 3] = as_v2s(v.w);
+                }
+                v2s hou = first ? vo : as_v2s(b.x);
+                v2s f = first ? NEG : as_v2s(b.y);
+                const v2s hod = hdt;
+                hdt = hou;
+                column16<LOCAL>(sc, Ho, E, hou, f, hod, vo, ve, best);
+                if (!last) scr[(size_t)j * 64 + lane] = make_uint2(as_u32(hou), as_u32(f));
+                if (!LOCAL && last) best = vmax(best, hou);  // last row, Ho space
+            }
+            if (!LOCAL) {
+                // Last column: every real row of this stripe (padding rows out).
+#pragma unroll
+                for (int r = 0; r < kStripe; ++r) {
+                    const int i = kStripe * s + r;
+                    v2s v = Ho[r];
+                    if (i < pA) v.x = (short)-32768;
+                    if (i < pB) v.y = (short)-32768;
+                    best = vmax(best, v);
+                }
+            }
+        }
+        if (!active) continue;
+        int bA = best.x, bB = best.y;
+        if (!LOCAL) {
+            bA -= a.sc.o;
+            bB -= a.sc.o;
+        }
+        if (qa >= 0) {
+            const size_t o = (size_t)qa * n_refs + ref;
+            if (a.out32) a.out32[o] = bA;
+            if (a.out16) a.out16[o] = (int16_t)bA;
+        }
+        if (qb >= 0) {
+            const size_t o = (size_t)qb * n_refs + ref;
+            if (a.out32) a.out32[o] = bB;
+            if (a.out16) a.out16[o] = (int16_t)bB;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_dp32 — plain one-thread-per-pair int32 DP (the baseline kernel, and the
+// exact rescue for queries outside the int16 bound).
+//
+// A block scores one read against up to blockDim.x references, one per
+// thread; the read is uniform across the block (bottom-aligned in 16-row
+// register stripes exactly as above), each thread walks its own reference
+// to its own length. The substitution is the 4-bit mask intersection
+// (A=1, C=2, G=4, T=8, wildcard 0xF) with padding rows scoring 0. The
+// stripe boundary (H, F) lives in scratch[j * total_threads + thread].
+// ---------------------------------------------------------------------------
+
+template <bool LOCAL>
+__global__ __launch_bounds__(256) void k_dp32(KArgs a, int which) {
+    const DevPanel &P = a.panel;
+    const int n_refs = P.n_refs;
+    const int G = blockDim.x;
+    const int groups = (n_refs + G - 1) / G;
+    const int count = uni(which == 0 ? a.desc->n16 : a.desc->n32);
+    const int *list = which == 0 ? a.sorted16 : a.list32;
+    uint32_t *cursor = which == 0 ? &a.desc->next32a : &a.desc->next32b;
+    const uint32_t n_items = (uint32_t)count * (uint32_t)groups;
+    const size_t total = (size_t)gridDim.x * G;
+    const size_t gtid = (size_t)blockIdx.x * G + threadIdx.x;
+    int2 *scr = a.scratch32 + gtid;
+    const int32_t ma = a.sc.ma, mi = a.sc.mi, o = a.sc.o, e = a.sc.e;
+    __shared__ uint32_t s_item;
+
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_item = atomicAdd(cursor, 1u);
+        __syncthreads();
+        const uint32_t t = s_item;
+        if (t >= n_items) break;
+        const int qi = (int)(t / (uint32_t)groups);
+        const int g = (int)(t - (uint32_t)qi * (uint32_t)groups);
+        const int q = list[qi];
+        const int n = a.qlen[q];
+        const int S = (n + kStripe - 1) / kStripe;
+        const int p = kStripe * S - n;
+        const uint8_t *qc = a.qcodes + a.qoff[q];
+        const int ref = g * G + threadIdx.x;
+        if (ref >= n_refs) continue;
+        const int m = P.rlen[ref];
+        const uint32_t *w2 = P.ref2 + P.w2off[ref];
+        const uint32_t *wn = P.refN + P.nmoff[ref];
+        int32_t best = LOCAL ? 0 : kNeg32;
+
+        for (int s = 0; s < S; ++s) {
+            uint32_t qm[kStripe];
+            int32_t sma[kStripe], smi[kStripe];
+#pragma unroll
+            for (int r = 0; r < kStripe; ++r) {
+                const int i = kStripe * s + r;
+                if (i < p) {
+                    qm[r] = 0;
+                    sma[r] = 0;
+                    smi[r] = 0;
+                } else {
+                    const int x = qc[i - p];
+                    qm[r] = x == 4 ? 0xFu : (1u << x);
+                    sma[r] = ma;
+                    smi[r] = mi;
+                }
+            }
+            int32_t H[kStripe], E[kStripe];
+#pragma unroll
+            for (int r = 0; r < kStripe; ++r) {
+                H[r] = 0;
+                E[r] = kNeg32;
+            }
+            int32_t hdt = 0;
+            const bool first = s == 0, last = s + 1 == S;
+            for (int j = 0; j < m; ++j) {
+                const uint32_t c2 = (w2[j >> 4] >> (2 * (j & 15))) & 3u;
+                const uint32_t rm = ((wn[j >> 5] >> (j & 31)) & 1u) ? 0xFu : (1u << c2);
+                int32_t hup, f;
+                if (first) {
+                    hup = 0;
+                    f = kNeg32;
+                } else {
+                    const int2 b = scr[(size_t)j * total];
+                    hup = b.x;
+                    f = b.y;
+                }
+                int32_t hdiag = hdt;
+                hdt = hup;
+#pragma unroll
+                for (int r = 0; r < kStripe; ++r) {
+                    const int32_t hleft = H[r];
+                    const int32_t ev = max(hleft + o, E[r] + e);
+                    E[r] = ev;
+                    f = max(hup + o, f + e);
+                    const int32_t sv = (qm[r] & rm) ? sma[r] : smi[r];
+                    int32_t h = max(max(hdiag + sv, ev), f);
+                    if (LOCAL) {
+                        h = max(h, 0);
+                        best = max(best, h);
+                    }
+                    hdiag = hleft;
+                    H[r] = h;
+                    hup = h;
+                }
+                if (!last) scr[(size_t)j * total] = make_int2(hup, f);
+                if (!LOCAL && last) best = max(best, hup);
+            }
+            if (!LOCAL) {
+#pragma unroll
+                for (int r = 0; r < kStripe; ++r)
+                    if (kStripe * s + r >= p) best = max(best, H[r]);
+            }
+        }
+        const size_t oidx = (size_t)q * n_refs + ref;
+        if (a.out32) a.out32[oidx] = best;
+        if (a.out16) a.out16[oidx] = (int16_t)best;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Length bucketing: classify -> scan -> scatter. Reads go to the int16
+// kernel in stripe-count bins, longest first; chunk k of 128 consecutive
+// reads then shares one stripe count (its first read's).
+// ---------------------------------------------------------------------------
+
+__global__ void k_zero(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= kMaxStripes) {
+        a.hist[i] = 0;
+        a.bin_fill[i] = 0;
+    }
+    if (i == 0) {
+        a.desc->n16 = 0;
+        a.desc->n32 = 0;
+        a.desc->next16 = 0;
+        a.desc->next32a = 0;
+        a.desc->next32b = 0;
+        a.desc->cells = 0;
+    }
+}
+
+// Reverse complements of the loaded reads (lib/alphabet.rs:58-64) into the
+// rows n_reads .. 2 * n_reads; one wave per read.
+__global__ void k_revcomp(KArgs a) {
+    const int nr = a.desc->n_reads;
+    if (!a.desc->both) return;
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int waves = (gridDim.x * blockDim.x) >> 6;
+    for (int q = wave; q < nr; q += waves) {
+        const int n = a.qlen[q];
+        const uint8_t *src = a.qcodes + a.qoff[q];
+        uint8_t *dst = a.rc_codes + a.qoff[nr + q];
+        for (int i = lane; i < n; i += 64) {
+            const uint8_t c = src[n - 1 - i];
+            dst[i] = c == 4 ? 4 : (uint8_t)(3 - c);
+        }
+    }
+}
+
+__device__ __forceinline__ bool fits_i16_dev(int64_t n, int64_t L, const Scoring &sc) {
+    // lib/simd/mod.rs:222-227
+    const int64_t k = n < L ? n : L;
+    const int64_t M = max(abs(sc.ma), abs(sc.mi));
+    return k * M + abs(sc.o) + abs(sc.e) < 15000 && L < 15000;
+}
+
+__global__ void k_classify(KArgs a) {
+    const int nq = a.desc->n_queries;
+    const int dropin = a.desc->dropin, force32 = a.desc->force32;
+    const int64_t L = a.panel.max_len;
+    const int64_t M = max(abs(a.sc.ma), abs(a.sc.mi));
+    uint64_t cells = 0;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+        const int n = a.qlen[q];
+        uint8_t c = 0;
+        if (n > 0) {
+            if (dropin) {
+                // GpuScorer::scores_len (lib/cuda/mod.rs:506-508).
+                c = (n <= 4096 && fits_i16_dev(n, L, a.sc)) ? 1 : 0;
+            } else {
+                const int64_t k = n < L ? n : L;
+                const bool b16 = k * M + abs(a.sc.o) + abs(a.sc.e) < kBound16 && n <= kMaxLen16;
+                c = (b16 && !force32) ? 1 : 2;
+            }
+        }
+        a.cls[q] = c;
+        if (c == 1) {
+            atomicAdd(&a.hist[(n + kStripe - 1) / kStripe], 1);
+        } else if (c == 2) {
+            const int pos = atomicAdd(&a.desc->n32, 1);
+            a.list32[pos] = q;
+        } else if (a.out16 || a.out32) {
+            // Unscored rows read back as zeros (ScoreMatrix::from_parts).
+            const int nr = a.panel.n_refs;
+            for (int r = 0; r < nr; ++r) {
+                if (a.out16) a.out16[(size_t)q * nr + r] = 0;
+                if (a.out32) a.out32[(size_t)q * nr + r] = 0;
+            }
+        }
+        if (c) cells += (uint64_t)n;
+    }
+    // one atomic per wave for the cell count
+    for (int off = 32; off > 0; off >>= 1) cells += __shfl_down(cells, off, 64);
+    if ((threadIdx.x & 63) == 0 && cells) atomicAdd((unsigned long long *)&a.desc->cells,
+                                                    (unsigned long long)(cells * (uint64_t)a.panel.total_len));
+}
+
+// Exclusive prefix over the stripe bins in DESCENDING stripe count, one
+// block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_scan(KArgs a) {
+    __shared__ int32_t part[1024];
+    constexpr int NB = kMaxStripes + 1;
+    constexpr int PER = (NB + 1023) / 1024;
+    const int t = threadIdx.x;
+    int32_t local[PER];
+    int32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int rb = t * PER + k;          // rank in descending order
+        const int b = kMaxStripes - rb;      // bin
+        local[k] = (rb < NB) ? a.hist[b] : 0;
+        sum += local[k];
+    }
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int32_t v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int32_t run = part[t] - sum;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int rb = t * PER + k;
+        if (rb < NB) a.bin_start[kMaxStripes - rb] = run;
+        run += local[k];
+    }
+    if (t == 1023) a.desc->n16 = part[1023];
+}
+
+__global__ void k_scatter(KArgs a) {
+    const int nq = a.desc->n_queries;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+        if (a.cls[q] != 1) continue;
+        const int b = (a.qlen[q] + kStripe - 1) / kStripe;
+        const int pos = a.bin_start[b] + atomicAdd(&a.bin_fill[b], 1);
+        a.sorted16[pos] = q;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused selection (lib/mapper.rs:444-472): one wave per read. Candidate k of
+// the tie order is reference k/2 on strand k%2 with both strands, else
+// reference k forward. best = max; ties = every candidate equal to it, in
+// order; suboptimal = the best candidate that is not a tie.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int wave_max(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_sum(int v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_select(KArgs a) {
+    const DevDesc &d = *a.desc;
+    const int n_reads = d.n_reads, both = d.both, nr = a.panel.n_refs;
+    const int cap = d.tie_cap;
+    const int64_t max_ties = d.max_ties;
+    const int min_score = d.min_score;
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int waves = (gridDim.x * blockDim.x) >> 6;
+    const int nc = nr * (both ? 2 : 1);
+    for (int rd = wave; rd < n_reads; rd += waves) {
+        int32_t *rec = a.recs + (size_t)rd * 5;
+        if (a.qlen[rd] == 0) {
+            if (lane < 5) rec[lane] = 0;
+            continue;
+        }
+        const int32_t *fwd = a.out32 + (size_t)rd * nr;
+        const int32_t *rev = a.out32 + (size_t)(n_reads + rd) * nr;
+        auto cand = [&](int k) -> int {
+            if (!both) return fwd[k];
+            return (k & 1) ? rev[k >> 1] : fwd[k >> 1];
+        };
+        int b = INT32_MIN;
+        for (int k = lane; k < nc; k += 64) b = max(b, cand(k));
+        b = wave_max(b);
+        const int64_t take_lim = max_ties < 0 ? INT64_MAX : max_ties + 1;
+        const int lim = (int)min((int64_t)cap, take_lim);
+        int nt = 0, so = INT32_MIN, hs = 0, base = 0;
+        for (int c0 = 0; c0 < nc; c0 += 64) {
+            const int k = c0 + lane;
+            int s = 0;
+            bool tie = false;
+            if (k < nc) {
+                s = cand(k);
+                tie = s == b;
+                if (!tie) {
+                    so = max(so, s);
+                    hs = 1;
+                }
+            }
+            const uint64_t mask = __ballot(tie);
+            const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+            if (tie && base + rank < lim) {
+                const int r = both ? (k >> 1) : k;
+                const int rv = both ? (k & 1) : 0;
+                a.ties[(size_t)rd * cap + base + rank] = (r << 1) | rv;
+            }
+            base += __popcll(mask);
+        }
+        nt = base;
+        so = wave_max(so);
+        hs = wave_max(hs);
+        if (lane == 0) {
+            const int64_t take = b < min_score ? 0 : min((int64_t)nt, take_lim);
+            rec[0] = b;
+            rec[1] = nt;
+            rec[2] = hs ? so : 0;
+            rec[3] = 1 | (hs ? 2 : 0);
+            rec[4] = (int32_t)take;
+        }
+    }
+}
+
+// Pair candidates (lib/mapper/pair.rs:185-231): one wave per pair. Mates are
+// interleaved (row 2p = mate 1, 2p + 1 = mate 2), their reverse complements
+// in the second half. On template strand t mate 1 is reversed iff t, mate 2
+// iff !t; a candidate is eligible when both mates reach min_score, and the
+// top tier is every eligible candidate with the best sum, in (ref, strand)
+// order.
+__global__ __launch_bounds__(256) void k_pairs(KArgs a) {
+    const DevDesc &d = *a.desc;
+    const int n_pairs = d.n_reads / 2;
+    const int strands = d.tstrands;
+    const int nr = a.panel.n_refs, cap = d.tie_cap, min_score = d.min_score;
+    const int n_reads = d.n_reads;
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int waves = (gridDim.x * blockDim.x) >> 6;
+    const int nc = nr * strands;
+    for (int p = wave; p < n_pairs; p += waves) {
+        int32_t *rec = a.recs + (size_t)p * 5;
+        if (a.qlen[2 * p] == 0 || a.qlen[2 * p + 1] == 0) {
+            if (lane < 5) rec[lane] = 0;
+            continue;
+        }
+        const int32_t *m1f = a.out32 + (size_t)(2 * p) * nr;
+        const int32_t *m2f = a.out32 + (size_t)(2 * p + 1) * nr;
+        const int32_t *m1r = a.out32 + (size_t)(n_reads + 2 * p) * nr;
+        const int32_t *m2r = a.out32 + (size_t)(n_reads + 2 * p + 1) * nr;
+        auto cand = [&](int k, bool &ok) -> int {
+            const int r = strands == 2 ? (k >> 1) : k;
+            const int t = strands == 2 ? (k & 1) : 0;
+            const int s1 = t ? m1r[r] : m1f[r];
+            const int s2 = t ? m2f[r] : m2r[r];
+            ok = s1 >= min_score && s2 >= min_score;
+            return s1 + s2;
+        };
+        int b = INT32_MIN, cnt = 0;
+        for (int k = lane; k < nc; k += 64) {
+            bool ok;
+            const int s = cand(k, ok);
+            if (ok) {
+                b = max(b, s);
+                cnt++;
+            }
+        }
+        b = wave_max(b);
+        cnt = wave_sum(cnt);
+        int nxt = INT32_MIN, hn = 0, base = 0;
+        for (int c0 = 0; c0 < nc; c0 += 64) {
+            const int k = c0 + lane;
+            bool top = false;
+            if (k < nc) {
+                bool ok;
+                const int s = cand(k, ok);
+                if (ok) {
+                    top = s == b;
+                    if (!top) {
+                        nxt = max(nxt, s);
+                        hn = 1;
+                    }
+                }
+            }
+            const uint64_t mask = __ballot(top);
+            const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+            if (top && base + rank < cap) {
+                const int r = strands == 2 ? (k >> 1) : k;
+                const int t = strands == 2 ? (k & 1) : 0;
+                a.ties[(size_t)p * cap + base + rank] = (r << 1) | t;
+            }
+            base += __popcll(mask);
+        }
+        nxt = wave_max(nxt);
+        hn = wave_max(hn);
+        if (lane == 0) {
+            rec[0] = cnt;
+            rec[1] = cnt ? b : 0;
+            rec[2] = base;
+            rec[3] = hn ? nxt : 0;
+            rec[4] = 1 | (hn ? 2 : 0);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers.
+// ---------------------------------------------------------------------------
+
+hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero, dim3((kMaxStripes + 1 + 255) / 256), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_revcomp, dim3(g.grid_aux), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_classify, dim3(g.grid_aux), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, a);
+    hipLaunchKernelGGL(k_scatter, dim3(g.grid_aux), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_dp16(const KArgs &a, const Launch &g, bool local, hipStream_t st) {
+    if (local) hipLaunchKernelGGL(k_dp16<true>, dim3(g.grid16), dim3(64 * kWaves16), 0, st, a);
+    else hipLaunchKernelGGL(k_dp16<false>, dim3(g.grid16), dim3(64 * kWaves16), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_dp32(const KArgs &a, const Launch &g, bool local, int which, hipStream_t st) {
+    if (local) hipLaunchKernelGGL(k_dp32<true>, dim3(g.grid32), dim3(g.block32), 0, st, a, which);
+    else hipLaunchKernelGGL(k_dp32<false>, dim3(g.grid32), dim3(g.block32), 0, st, a, which);
+    return hipGetLastError();
+}
+
+hipError_t launch_select(const KArgs &a, const Launch &g, hipStream_t st) {
+    hipLaunchKernelGGL(k_select, dim3(g.grid_aux), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pairs(const KArgs &a, const Launch &g, hipStream_t st) {
+    hipLaunchKernelGGL(k_pairs, dim3(g.grid_aux), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t dp16_attributes(bool local, hipFuncAttributes *attr) {
+    return local ? hipFuncGetAttributes(attr, reinterpret_cast<const void *>(&k_dp16<true>))
+                 : hipFuncGetAttributes(attr, reinterpret_cast<const void *>(&k_dp16<false>));
+}
+
+hipError_t dp32_attributes(bool local, hipFuncAttributes *attr) {
+    return local ? hipFuncGetAttributes(attr, reinterpret_cast<const void *>(&k_dp32<true>))
+                 : hipFuncGetAttributes(attr, reinterpret_cast<const void *>(&k_dp32<false>));
+}
+
+hipError_t dp16_occupancy(bool local, int *blocks) {
+    return local ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_dp16<true>, 64 * kWaves16, 0)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_dp16<false>, 64 * kWaves16, 0);
+}
+
+hipError_t dp32_occupancy(bool local, int block, int *blocks) {
+    return local ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_dp32<true>, block, 0)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_dp32<false>, block, 0);
+}
+
+}  // namespace esch

@@ -1,0 +1,818 @@
+// Host side of the C ABI (include/eschalign_amd.h): the panel resident on
+// the device, device batches with their own stream, scratch and captured
+// graphs, and the drop-in prepare / submit / finish of GpuScorer
+// (lib/cuda/mod.rs:328-749).
+#include "../../include/eschalign_amd.h"
+#include "esch_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+using namespace esch;
+
+namespace {
+
+thread_local std::string g_err;
+
+esch_status fail(esch_status st, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return st;
+}
+
+#define HIPC(call)                                                                     \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(ESCH_DRIVER, "%s failed: %s", #call, hipGetErrorString(e_));   \
+    } while (0)
+
+int32_t iabs(int32_t v) { return v < 0 ? -v : v; }
+
+// lib/scoring.rs:70-100
+esch_status validate(int32_t ma, int32_t mi, int32_t o, int32_t e) {
+    if (ma <= 0) return fail(ESCH_SCORING_INVALID, "invalid scoring: match must be positive, got %d", ma);
+    if (mi >= ma)
+        return fail(ESCH_SCORING_INVALID, "invalid scoring: mismatch (%d) must score below match (%d)", mi, ma);
+    if (o > 0 || e > 0)
+        return fail(ESCH_SCORING_INVALID,
+                    "invalid scoring: gap scores are penalties and must be zero or negative, got open %d extend %d",
+                    o, e);
+    for (int32_t v : {ma, mi, o, e})
+        if (iabs(v) > 1000)
+            return fail(ESCH_SCORING_INVALID, "invalid scoring: |%d| exceeds the supported magnitude 1000", v);
+    return ESCH_OK;
+}
+
+bool fits_i16(size_t n, size_t L, int32_t ma, int32_t mi, int32_t o, int32_t e) {
+    const int64_t k = (int64_t)std::min(n, L);
+    const int64_t M = std::max(iabs(ma), iabs(mi));
+    return k * M + iabs(o) + iabs(e) < 15000 && L < 15000;
+}
+
+}  // namespace
+
+struct esch_scorer {
+    int ordinal = 0;
+    std::string device_name;
+    int n_cu = 0;
+    Scoring sc{};
+    esch_mode mode = ESCH_LOCAL;
+    size_t n_refs = 0, max_len = 0;
+    int64_t total_len = 0;
+    uint32_t *d_ref2 = nullptr, *d_refN = nullptr, *d_w2off = nullptr, *d_nmoff = nullptr;
+    int32_t *d_rlen = nullptr, *d_rorder = nullptr;
+    DevPanel panel{};
+    Launch geo{};
+    int regs16 = 0, local16 = 0;
+    std::mutex pool_mu;
+    std::vector<esch_batch *> pool;  // idle batches for the drop-in API
+};
+
+struct esch_batch {
+    esch_scorer *s = nullptr;
+    size_t max_reads = 0, max_bases = 0;  // capacity before strand expansion
+    size_t tie_cap_alloc = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+    // device
+    uint8_t *d_codes = nullptr;
+    uint64_t *d_qoff = nullptr;
+    int32_t *d_qlen = nullptr;
+    uint8_t *d_cls = nullptr;
+    int32_t *d_hist = nullptr, *d_bin_start = nullptr, *d_bin_fill = nullptr;
+    int32_t *d_sorted16 = nullptr, *d_list32 = nullptr;
+    int32_t *d_out32 = nullptr;
+    int16_t *d_out16 = nullptr;
+    uint2 *d_scr16 = nullptr;
+    int2 *d_scr32 = nullptr;
+    int32_t *d_recs = nullptr, *d_ties = nullptr;
+    DevDesc *d_desc = nullptr;
+    // pinned staging
+    uint8_t *h_codes = nullptr;
+    uint64_t *h_qoff = nullptr;
+    int32_t *h_qlen = nullptr;
+    DevDesc *h_desc = nullptr;
+    // last load / run
+    size_t n_reads = 0, n_rows = 0;
+    int both = 0, pairs = 0;
+    int last_what = 0;
+    bool timed_dp = false;
+    std::map<int, hipGraphExec_t> graphs;
+};
+
+struct esch_prepared {
+    size_t n = 0;
+    std::vector<uint8_t> codes;
+    std::vector<uint64_t> off;
+    std::vector<int32_t> len;
+    std::vector<uint8_t> scored;
+};
+
+struct esch_pending {
+    esch_batch *b = nullptr;
+    size_t n = 0;
+    std::vector<uint8_t> scored;
+};
+
+extern "C" {
+
+const char *esch_last_error(void) { return g_err.c_str(); }
+
+int esch_device_visible(void) {
+    static std::once_flag once;
+    static int visible = 0;
+    std::call_once(once, [] {
+        int n = 0;
+        visible = (hipGetDeviceCount(&n) == hipSuccess && n > 0) ? 1 : 0;
+    });
+    return visible;
+}
+
+esch_status esch_scoring_validate(int32_t ma, int32_t mi, int32_t o, int32_t e) {
+    return validate(ma, mi, o, e);
+}
+
+int esch_fits_i16(size_t n, size_t L, int32_t ma, int32_t mi, int32_t o, int32_t e) {
+    return fits_i16(n, L, ma, mi, o, e) ? 1 : 0;
+}
+
+esch_status esch_scorer_new(int ordinal, const uint8_t *const *refs, const size_t *ref_lens,
+                            size_t n_refs, int32_t ma, int32_t mi, int32_t o, int32_t e,
+                            esch_mode mode, esch_scorer **out) {
+    if (!out) return fail(ESCH_INVALID_ARG, "out is null");
+    *out = nullptr;
+    if (mode != ESCH_LOCAL && mode != ESCH_SEMIGLOBAL) return fail(ESCH_INVALID_ARG, "unknown mode %d", (int)mode);
+    if (esch_status st = validate(ma, mi, o, e)) return st;
+    if (n_refs == 0) return fail(ESCH_EMPTY_PANEL, "the reference panel is empty");
+    if (!refs || !ref_lens) return fail(ESCH_INVALID_ARG, "refs or ref_lens is null");
+    size_t max_len = 0;
+    int64_t total = 0;
+    for (size_t i = 0; i < n_refs; ++i) {
+        if (ref_lens[i] == 0) return fail(ESCH_EMPTY_REFERENCE, "reference %zu has no bases", i);
+        if (!refs[i]) return fail(ESCH_INVALID_ARG, "reference %zu is null", i);
+        for (size_t j = 0; j < ref_lens[i]; ++j)
+            if (refs[i][j] > 4) return fail(ESCH_INVALID_ARG, "reference %zu carries code %d at %zu (codes are 0..=4)", i, refs[i][j], j);
+        max_len = std::max(max_len, ref_lens[i]);
+        total += (int64_t)ref_lens[i];
+    }
+    if (max_len > ESCH_MAX_REF_LEN)
+        return fail(ESCH_REFERENCE_TOO_LONG,
+                    "the panel's longest reference (%zu nt) exceeds the GPU kernel's %d nt", max_len,
+                    ESCH_MAX_REF_LEN);
+    if (!esch_device_visible()) return fail(ESCH_UNAVAILABLE, "no HIP device is visible");
+    int ndev = 0;
+    HIPC(hipGetDeviceCount(&ndev));
+    if (ordinal < 0 || ordinal >= ndev) return fail(ESCH_UNAVAILABLE, "no HIP device %d (%d visible)", ordinal, ndev);
+    HIPC(hipSetDevice(ordinal));
+    hipDeviceProp_t prop;
+    HIPC(hipGetDeviceProperties(&prop, ordinal));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(ESCH_CODE_OBJECT, "device %d is %s; this library carries gfx950 code objects only", ordinal,
+                    prop.gcnArchName);
+
+    std::unique_ptr<esch_scorer> s(new esch_scorer);
+    s->ordinal = ordinal;
+    s->device_name = prop.name;
+    s->n_cu = prop.multiProcessorCount;
+    s->sc = Scoring{ma, mi, o, e, mode == ESCH_LOCAL ? 1 : 0};
+    s->mode = mode;
+    s->n_refs = n_refs;
+    s->max_len = max_len;
+    s->total_len = total;
+
+    // 2-bit codes, 16 per word (a wildcard stores 0 and sets its bit in the
+    // wildcard bitmap, 32 per word); every reference starts on a new word.
+    std::vector<uint32_t> ref2, refN, w2off(n_refs), nmoff(n_refs);
+    std::vector<int32_t> rlen(n_refs);
+    for (size_t i = 0; i < n_refs; ++i) {
+        const size_t L = ref_lens[i];
+        w2off[i] = (uint32_t)ref2.size();
+        nmoff[i] = (uint32_t)refN.size();
+        rlen[i] = (int32_t)L;
+        ref2.resize(ref2.size() + (L + 15) / 16 + 1, 0u);
+        refN.resize(refN.size() + (L + 31) / 32 + 1, 0u);
+        for (size_t j = 0; j < L; ++j) {
+            const uint8_t c = refs[i][j];
+            if (c == 4) refN[nmoff[i] + j / 32] |= 1u << (j % 32);
+            else ref2[w2off[i] + j / 16] |= (uint32_t)c << (2 * (j % 16));
+        }
+    }
+    HIPC(hipMalloc(&s->d_ref2, ref2.size() * 4));
+    HIPC(hipMalloc(&s->d_refN, refN.size() * 4));
+    HIPC(hipMalloc(&s->d_w2off, n_refs * 4));
+    HIPC(hipMalloc(&s->d_nmoff, n_refs * 4));
+    HIPC(hipMalloc(&s->d_rlen, n_refs * 4));
+    HIPC(hipMemcpy(s->d_ref2, ref2.data(), ref2.size() * 4, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(s->d_refN, refN.data(), refN.size() * 4, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(s->d_w2off, w2off.data(), n_refs * 4, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(s->d_nmoff, nmoff.data(), n_refs * 4, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(s->d_rlen, rlen.data(), n_refs * 4, hipMemcpyHostToDevice));
+    // Longest first, stable (as lib/simd/mod.rs:299-300 orders its lane
+    // groups): the four references of one workgroup then have nearly equal
+    // lengths and finish their stripes together.
+    std::vector<int32_t> order(n_refs);
+    for (size_t i = 0; i < n_refs; ++i) order[i] = (int32_t)i;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return rlen[x] > rlen[y]; });
+    HIPC(hipMalloc(&s->d_rorder, n_refs * 4));
+    HIPC(hipMemcpy(s->d_rorder, order.data(), n_refs * 4, hipMemcpyHostToDevice));
+    s->panel = DevPanel{s->d_ref2, s->d_refN, s->d_w2off, s->d_nmoff, s->d_rlen, s->d_rorder,
+                        (int32_t)n_refs, (int32_t)max_len, total};
+
+    // Launch geometry: persistent grids sized by occupancy.
+    const bool local = mode == ESCH_LOCAL;
+    hipFuncAttributes attr;
+    hipError_t ae = dp16_attributes(local, &attr);
+    if (ae != hipSuccess)
+        return fail(ESCH_CODE_OBJECT, "no gfx950 code object for the score kernel: %s", hipGetErrorString(ae));
+    s->regs16 = attr.numRegs;
+    s->local16 = (int)attr.localSizeBytes;
+    int b16 = 0, b32 = 0;
+    HIPC(dp16_occupancy(local, &b16));
+    int block32 = (int)std::min<size_t>(256, ((n_refs + 63) / 64) * 64);
+    HIPC(dp32_occupancy(local, block32, &b32));
+    s->geo.grid16 = std::max(1, b16) * s->n_cu;
+    s->geo.block32 = block32;
+    s->geo.grid32 = std::max(1, b32) * s->n_cu;
+    s->geo.grid_aux = 4 * s->n_cu;
+    *out = s.release();
+    return ESCH_OK;
+}
+
+void esch_batch_free(esch_batch *b);
+
+void esch_scorer_free(esch_scorer *s) {
+    if (!s) return;
+    for (esch_batch *b : s->pool) esch_batch_free(b);
+    (void)hipFree(s->d_ref2);
+    (void)hipFree(s->d_refN);
+    (void)hipFree(s->d_w2off);
+    (void)hipFree(s->d_nmoff);
+    (void)hipFree(s->d_rlen);
+    (void)hipFree(s->d_rorder);
+    delete s;
+}
+
+int esch_scores_len(const esch_scorer *s, size_t len) {
+    if (!s) return 0;
+    return (len > 0 && len <= ESCH_MAX_READ_LEN &&
+            fits_i16(len, s->max_len, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e))
+               ? 1
+               : 0;
+}
+
+size_t esch_n_refs(const esch_scorer *s) { return s ? s->n_refs : 0; }
+
+esch_status esch_device_name(const esch_scorer *s, char *buf, size_t cap) {
+    if (!s || !buf || cap == 0) return fail(ESCH_INVALID_ARG, "null argument");
+    snprintf(buf, cap, "%s", s->device_name.c_str());
+    return ESCH_OK;
+}
+
+void esch_geometry(const esch_scorer *s, size_t *lanes, size_t *groups, size_t *blocks) {
+    if (!s) return;
+    if (lanes) *lanes = 256;
+    if (groups) *groups = 4;
+    if (blocks) *blocks = (size_t)s->geo.grid16;
+}
+
+int esch_registers(const esch_scorer *s) { return s ? s->regs16 : 0; }
+int esch_local_bytes(const esch_scorer *s) { return s ? s->local16 : 0; }
+
+// ------------------------------------------------------------------ batches
+
+size_t esch_batch_footprint(const esch_scorer *s, size_t max_reads, size_t max_bases) {
+    if (!s) return 0;
+    const size_t rows = 2 * std::max<size_t>(max_reads, 1);
+    const size_t bases = 2 * std::max<size_t>(max_bases, 1);
+    size_t f = bases + rows * (8 + 4 + 1 + 4 + 4) + 3 * 4 * (kMaxStripes + 1) + sizeof(DevDesc);
+    f += rows * s->n_refs * (4 + 2);                                     // out32 + out16
+    f += (size_t)s->geo.grid16 * 4 * s->max_len * 64 * sizeof(uint2);        // scratch16
+    f += (size_t)s->geo.grid32 * s->geo.block32 * s->max_len * sizeof(int2);  // scratch32
+    f += rows * 5 * 4;                                                   // records
+    return f;
+}
+
+esch_status esch_batch_create(esch_scorer *s, size_t max_reads, size_t max_bases, esch_batch **out) {
+    if (!s || !out) return fail(ESCH_INVALID_ARG, "null argument");
+    *out = nullptr;
+    HIPC(hipSetDevice(s->ordinal));
+    std::unique_ptr<esch_batch> b(new esch_batch);
+    b->s = s;
+    b->max_reads = std::max<size_t>(max_reads, 1);
+    b->max_bases = std::max<size_t>(max_bases, 1);
+    const size_t rows = 2 * b->max_reads, bases = 2 * b->max_bases;
+    HIPC(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    HIPC(hipEventCreate(&b->ev0));
+    HIPC(hipEventCreate(&b->ev1));
+    HIPC(hipEventCreate(&b->ev2));
+    HIPC(hipEventCreate(&b->ev3));
+    HIPC(hipMalloc(&b->d_codes, bases));
+    HIPC(hipMalloc(&b->d_qoff, rows * 8));
+    HIPC(hipMalloc(&b->d_qlen, rows * 4));
+    HIPC(hipMalloc(&b->d_cls, rows));
+    HIPC(hipMalloc(&b->d_hist, 4 * (kMaxStripes + 1)));
+    HIPC(hipMalloc(&b->d_bin_start, 4 * (kMaxStripes + 1)));
+    HIPC(hipMalloc(&b->d_bin_fill, 4 * (kMaxStripes + 1)));
+    HIPC(hipMalloc(&b->d_sorted16, rows * 4));
+    HIPC(hipMalloc(&b->d_list32, rows * 4));
+    HIPC(hipMalloc(&b->d_out32, rows * s->n_refs * 4));
+    HIPC(hipMalloc(&b->d_out16, rows * s->n_refs * 2));
+    HIPC(hipMalloc(&b->d_scr16, (size_t)s->geo.grid16 * 4 * s->max_len * 64 * sizeof(uint2)));
+    HIPC(hipMalloc(&b->d_scr32, (size_t)s->geo.grid32 * s->geo.block32 * s->max_len * sizeof(int2)));
+    HIPC(hipMalloc(&b->d_recs, rows * 5 * 4));
+    HIPC(hipMalloc(&b->d_desc, sizeof(DevDesc)));
+    HIPC(hipHostMalloc(&b->h_codes, bases, hipHostMallocDefault));
+    HIPC(hipHostMalloc(&b->h_qoff, rows * 8, hipHostMallocDefault));
+    HIPC(hipHostMalloc(&b->h_qlen, rows * 4, hipHostMallocDefault));
+    HIPC(hipHostMalloc(&b->h_desc, sizeof(DevDesc), hipHostMallocDefault));
+    memset(b->h_desc, 0, sizeof(DevDesc));
+    *out = b.release();
+    return ESCH_OK;
+}
+
+void esch_batch_free(esch_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->s->ordinal);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+    (void)hipFree(b->d_codes);
+    (void)hipFree(b->d_qoff);
+    (void)hipFree(b->d_qlen);
+    (void)hipFree(b->d_cls);
+    (void)hipFree(b->d_hist);
+    (void)hipFree(b->d_bin_start);
+    (void)hipFree(b->d_bin_fill);
+    (void)hipFree(b->d_sorted16);
+    (void)hipFree(b->d_list32);
+    (void)hipFree(b->d_out32);
+    (void)hipFree(b->d_out16);
+    (void)hipFree(b->d_scr16);
+    (void)hipFree(b->d_scr32);
+    (void)hipFree(b->d_recs);
+    (void)hipFree(b->d_ties);
+    (void)hipFree(b->d_desc);
+    (void)hipHostFree(b->h_codes);
+    (void)hipHostFree(b->h_qoff);
+    (void)hipHostFree(b->h_qlen);
+    (void)hipHostFree(b->h_desc);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->ev2) (void)hipEventDestroy(b->ev2);
+    if (b->ev3) (void)hipEventDestroy(b->ev3);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+namespace {
+
+// Stage reads (and, for both strands, their reverse-complement rows, whose
+// codes the device writes) into the pinned buffers, then upload.
+esch_status batch_stage(esch_batch *b, size_t n, int both, int pairs,
+                        const std::function<void(size_t, const uint8_t *&, size_t &)> &get) {
+    if (pairs && (n % 2)) return fail(ESCH_INVALID_ARG, "pairs need an even number of mates");
+    size_t total = 0;
+    for (size_t k = 0; k < n; ++k) {
+        const uint8_t *p;
+        size_t len;
+        get(k, p, len);
+        if (len > (size_t)INT32_MAX) return fail(ESCH_INVALID_ARG, "read %zu is too long", k);
+        total += len;
+    }
+    if (n > b->max_reads || total > b->max_bases)
+        return fail(ESCH_CAPACITY, "batch holds %zu reads / %zu bases; got %zu / %zu", b->max_reads,
+                    b->max_bases, n, total);
+    HIPC(hipStreamSynchronize(b->stream));  // staging buffers may still feed a copy
+    size_t at = 0;
+    for (size_t k = 0; k < n; ++k) {
+        const uint8_t *p;
+        size_t len;
+        get(k, p, len);
+        for (size_t j = 0; j < len; ++j)
+            if (p[j] > 4) return fail(ESCH_INVALID_ARG, "read %zu carries code %d (codes are 0..=4)", k, p[j]);
+        if (len) memcpy(b->h_codes + at, p, len);
+        b->h_qoff[k] = at;
+        b->h_qlen[k] = (int32_t)len;
+        at += len;
+    }
+    const size_t rows = both ? 2 * n : n;
+    if (both) {
+        for (size_t k = 0; k < n; ++k) {
+            b->h_qoff[n + k] = total + b->h_qoff[k];
+            b->h_qlen[n + k] = b->h_qlen[k];
+        }
+    }
+    b->n_reads = n;
+    b->n_rows = rows;
+    b->both = both;
+    b->pairs = pairs;
+    if (total) HIPC(hipMemcpyAsync(b->d_codes, b->h_codes, total, hipMemcpyHostToDevice, b->stream));
+    if (rows) {
+        HIPC(hipMemcpyAsync(b->d_qoff, b->h_qoff, rows * 8, hipMemcpyHostToDevice, b->stream));
+        HIPC(hipMemcpyAsync(b->d_qlen, b->h_qlen, rows * 4, hipMemcpyHostToDevice, b->stream));
+    }
+    return ESCH_OK;
+}
+
+KArgs make_args(esch_batch *b, int what) {
+    esch_scorer *s = b->s;
+    KArgs a{};
+    a.panel = s->panel;
+    a.sc = s->sc;
+    a.desc = b->d_desc;
+    a.qcodes = b->d_codes;
+    a.qoff = b->d_qoff;
+    a.qlen = b->d_qlen;
+    a.cls = b->d_cls;
+    a.hist = b->d_hist;
+    a.bin_start = b->d_bin_start;
+    a.bin_fill = b->d_bin_fill;
+    a.sorted16 = b->d_sorted16;
+    a.list32 = b->d_list32;
+    a.out32 = (what & ESCH_RUN_SCORES16) ? nullptr : b->d_out32;
+    a.out16 = (what & ESCH_RUN_SCORES16) ? b->d_out16 : nullptr;
+    a.scratch16 = b->d_scr16;
+    a.scratch32 = b->d_scr32;
+    a.recs = b->d_recs;
+    a.ties = b->d_ties;
+    a.rc_codes = b->d_codes;
+    return a;
+}
+
+esch_status enqueue(esch_batch *b, int what, bool timed) {
+    esch_scorer *s = b->s;
+    const KArgs a = make_args(b, what);
+    const bool local = s->sc.local != 0;
+    hipStream_t st = b->stream;
+    HIPC(launch_prep(a, s->geo, st));
+    if (timed) HIPC(hipEventRecord(b->ev1, st));
+    if (what & ESCH_RUN_PLAIN) {
+        HIPC(launch_dp32(a, s->geo, local, 0, st));
+        HIPC(launch_dp32(a, s->geo, local, 1, st));
+    } else {
+        HIPC(launch_dp16(a, s->geo, local, st));
+        HIPC(launch_dp32(a, s->geo, local, 1, st));
+    }
+    if (timed) HIPC(hipEventRecord(b->ev2, st));
+    if (what & ESCH_RUN_SELECT) HIPC(launch_select(a, s->geo, st));
+    if (what & ESCH_RUN_PAIRS) HIPC(launch_pairs(a, s->geo, st));
+    return ESCH_OK;
+}
+
+}  // namespace
+
+esch_status esch_batch_load(esch_batch *b, const uint8_t *const *reads, const size_t *lens, size_t n,
+                            int both, int pairs) {
+    if (!b || (n && (!reads || !lens))) return fail(ESCH_INVALID_ARG, "null argument");
+    HIPC(hipSetDevice(b->s->ordinal));
+    return batch_stage(b, n, both ? 1 : 0, pairs ? 1 : 0, [&](size_t k, const uint8_t *&p, size_t &len) {
+        p = reads[k];
+        len = lens[k];
+    });
+}
+
+esch_status esch_batch_load_concat(esch_batch *b, const uint8_t *codes, const uint64_t *offs,
+                                   const int32_t *lens, size_t n, int both, int pairs) {
+    if (!b || (n && (!codes || !offs || !lens))) return fail(ESCH_INVALID_ARG, "null argument");
+    HIPC(hipSetDevice(b->s->ordinal));
+    return batch_stage(b, n, both ? 1 : 0, pairs ? 1 : 0, [&](size_t k, const uint8_t *&p, size_t &len) {
+        p = codes + offs[k];
+        len = (size_t)(lens[k] < 0 ? 0 : lens[k]);
+    });
+}
+
+esch_status esch_batch_run(esch_batch *b, int what, int32_t min_score, int64_t max_ties, size_t tie_cap) {
+    if (!b) return fail(ESCH_INVALID_ARG, "null batch");
+    esch_scorer *s = b->s;
+    HIPC(hipSetDevice(s->ordinal));
+    if ((what & ESCH_RUN_PAIRS) && !b->pairs) return fail(ESCH_INVALID_ARG, "batch was not loaded as pairs");
+    if ((what & (ESCH_RUN_SELECT | ESCH_RUN_PAIRS)) && (what & ESCH_RUN_SCORES16))
+        return fail(ESCH_INVALID_ARG, "selection reads the int32 matrix");
+    if (what & (ESCH_RUN_SELECT | ESCH_RUN_PAIRS)) {
+        if (tie_cap == 0) tie_cap = 1;
+        if (tie_cap > b->tie_cap_alloc) {
+            HIPC(hipStreamSynchronize(b->stream));
+            (void)hipFree(b->d_ties);
+            b->d_ties = nullptr;
+            HIPC(hipMalloc(&b->d_ties, std::max<size_t>(b->max_reads, 1) * tie_cap * 4));
+            b->tie_cap_alloc = tie_cap;
+            for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+            b->graphs.clear();
+        }
+    }
+    DevDesc &d = *b->h_desc;
+    HIPC(hipStreamSynchronize(b->stream));  // the pinned descriptor may still feed a copy
+    d.n_queries = (int32_t)b->n_rows;
+    d.n_reads = (int32_t)b->n_reads;
+    d.both = b->both;
+    d.pairs = b->pairs;
+    d.dropin = (what & ESCH_RUN_SCORES16) ? 1 : 0;
+    d.force32 = (what & ESCH_RUN_PLAIN) ? 1 : 0;
+    d.min_score = min_score;
+    d.tie_cap = (int32_t)std::max<size_t>(tie_cap, 1);
+    d.max_ties = max_ties;
+    d.tstrands = (what & ESCH_RUN_PAIR_BOTH) ? 2 : 1;
+    HIPC(hipMemcpyAsync(b->d_desc, b->h_desc, sizeof(DevDesc), hipMemcpyHostToDevice, b->stream));
+    b->last_what = what;
+    const bool graph = !(what & ESCH_RUN_NO_GRAPH);
+    // the template-strand count is read from the descriptor, not baked in
+    const int key = what & ~(ESCH_RUN_NO_GRAPH | ESCH_RUN_PAIR_BOTH);
+    HIPC(hipEventRecord(b->ev0, b->stream));
+    if (!graph) {
+        b->timed_dp = true;
+        if (esch_status st = enqueue(b, key, true)) return st;
+    } else {
+        b->timed_dp = false;
+        auto it = b->graphs.find(key);
+        if (it == b->graphs.end()) {
+            hipGraph_t g;
+            HIPC(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            esch_status st = enqueue(b, key, false);
+            hipError_t ce = hipStreamEndCapture(b->stream, &g);
+            if (st) return st;
+            if (ce != hipSuccess) return fail(ESCH_DRIVER, "hipStreamEndCapture: %s", hipGetErrorString(ce));
+            hipGraphExec_t ge;
+            hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) return fail(ESCH_DRIVER, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+            it = b->graphs.emplace(key, ge).first;
+        }
+        HIPC(hipGraphLaunch(it->second, b->stream));
+    }
+    HIPC(hipEventRecord(b->ev3, b->stream));
+    return ESCH_OK;
+}
+
+esch_status esch_batch_wait(esch_batch *b) {
+    if (!b) return fail(ESCH_INVALID_ARG, "null batch");
+    HIPC(hipSetDevice(b->s->ordinal));
+    HIPC(hipStreamSynchronize(b->stream));
+    return ESCH_OK;
+}
+
+esch_status esch_batch_timing(const esch_batch *b, float *dp_ms, float *total_ms) {
+    if (!b) return fail(ESCH_INVALID_ARG, "null batch");
+    if (dp_ms) {
+        *dp_ms = -1.0f;
+        if (b->timed_dp) HIPC(hipEventElapsedTime(dp_ms, b->ev1, b->ev2));
+    }
+    if (total_ms) HIPC(hipEventElapsedTime(total_ms, b->ev0, b->ev3));
+    return ESCH_OK;
+}
+
+esch_status esch_batch_stats(const esch_batch *b, uint64_t *cells, uint64_t *n16, uint64_t *n32) {
+    if (!b) return fail(ESCH_INVALID_ARG, "null batch");
+    DevDesc d;
+    HIPC(hipMemcpy(&d, b->d_desc, sizeof d, hipMemcpyDeviceToHost));
+    if (cells) *cells = d.cells;
+    if (n16) *n16 = (uint64_t)d.n16;
+    if (n32) *n32 = (uint64_t)d.n32;
+    return ESCH_OK;
+}
+
+esch_status esch_batch_read_scores32(esch_batch *b, int32_t *scores, uint8_t *scored) {
+    if (!b) return fail(ESCH_INVALID_ARG, "null batch");
+    if (b->last_what & ESCH_RUN_SCORES16) return fail(ESCH_INVALID_ARG, "last run wrote the int16 matrix");
+    HIPC(hipSetDevice(b->s->ordinal));
+    HIPC(hipStreamSynchronize(b->stream));
+    const size_t rows = b->n_rows, nr = b->s->n_refs;
+    if (scores && rows) HIPC(hipMemcpy(scores, b->d_out32, rows * nr * 4, hipMemcpyDeviceToHost));
+    if (scored && rows) {
+        HIPC(hipMemcpy(scored, b->d_cls, rows, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < rows; ++k) scored[k] = scored[k] ? 1 : 0;
+    }
+    return ESCH_OK;
+}
+
+esch_status esch_batch_read_scores16(esch_batch *b, int16_t *scores, uint8_t *scored) {
+    if (!b) return fail(ESCH_INVALID_ARG, "null batch");
+    if (!(b->last_what & ESCH_RUN_SCORES16)) return fail(ESCH_INVALID_ARG, "last run wrote the int32 matrix");
+    HIPC(hipSetDevice(b->s->ordinal));
+    HIPC(hipStreamSynchronize(b->stream));
+    const size_t rows = b->n_rows, nr = b->s->n_refs;
+    if (scores && rows) HIPC(hipMemcpy(scores, b->d_out16, rows * nr * 2, hipMemcpyDeviceToHost));
+    if (scored && rows) {
+        HIPC(hipMemcpy(scored, b->d_cls, rows, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < rows; ++k) scored[k] = scored[k] ? 1 : 0;
+    }
+    return ESCH_OK;
+}
+
+esch_status esch_batch_read_select(esch_batch *b, esch_read_select *out, int32_t *ties) {
+    if (!b) return fail(ESCH_INVALID_ARG, "null batch");
+    if (!(b->last_what & ESCH_RUN_SELECT)) return fail(ESCH_INVALID_ARG, "last run did not select");
+    HIPC(hipSetDevice(b->s->ordinal));
+    HIPC(hipStreamSynchronize(b->stream));
+    const size_t n = b->n_reads;
+    if (out && n) HIPC(hipMemcpy(out, b->d_recs, n * sizeof(esch_read_select), hipMemcpyDeviceToHost));
+    if (ties && n) HIPC(hipMemcpy(ties, b->d_ties, n * b->h_desc->tie_cap * 4, hipMemcpyDeviceToHost));
+    return ESCH_OK;
+}
+
+esch_status esch_batch_read_pairs(esch_batch *b, esch_pair_select *out, int32_t *top) {
+    if (!b) return fail(ESCH_INVALID_ARG, "null batch");
+    if (!(b->last_what & ESCH_RUN_PAIRS)) return fail(ESCH_INVALID_ARG, "last run did not select pairs");
+    HIPC(hipSetDevice(b->s->ordinal));
+    HIPC(hipStreamSynchronize(b->stream));
+    const size_t n = b->n_reads / 2;
+    if (out && n) HIPC(hipMemcpy(out, b->d_recs, n * sizeof(esch_pair_select), hipMemcpyDeviceToHost));
+    if (top && n) HIPC(hipMemcpy(top, b->d_ties, n * b->h_desc->tie_cap * 4, hipMemcpyDeviceToHost));
+    return ESCH_OK;
+}
+
+// ------------------------------------------------------------------ drop-in
+
+esch_prepared *esch_prepare(const esch_scorer *s, const uint8_t *const *queries, const size_t *lens, size_t n) {
+    if (!s || (n && (!queries || !lens))) {
+        fail(ESCH_INVALID_ARG, "null argument");
+        return nullptr;
+    }
+    std::unique_ptr<esch_prepared> p(new esch_prepared);
+    p->n = n;
+    p->off.resize(n);
+    p->len.resize(n);
+    p->scored.resize(n);
+    size_t total = 0;
+    for (size_t k = 0; k < n; ++k) {
+        p->scored[k] = (uint8_t)esch_scores_len(s, lens[k]);
+        // unscored queries travel as empty rows (lib/cuda/mod.rs:555-563)
+        const size_t L = p->scored[k] ? lens[k] : 0;
+        p->off[k] = total;
+        p->len[k] = (int32_t)L;
+        total += L;
+    }
+    p->codes.resize(std::max<size_t>(total, 1));
+    for (size_t k = 0; k < n; ++k) {
+        if (!p->len[k]) continue;
+        for (size_t j = 0; j < lens[k]; ++j) {
+            if (queries[k][j] > 4) {
+                fail(ESCH_INVALID_ARG, "query %zu carries code %d (codes are 0..=4)", k, queries[k][j]);
+                return nullptr;
+            }
+        }
+        memcpy(p->codes.data() + p->off[k], queries[k], lens[k]);
+    }
+    return p.release();
+}
+
+void esch_prepared_free(esch_prepared *p) { delete p; }
+size_t esch_prepared_len(const esch_prepared *p) { return p ? p->n : 0; }
+size_t esch_prepared_n_scored(const esch_prepared *p) {
+    if (!p) return 0;
+    size_t c = 0;
+    for (uint8_t v : p->scored) c += v;
+    return c;
+}
+
+namespace {
+
+esch_status take_batch(esch_scorer *s, size_t n, size_t bases, esch_batch **out) {
+    {
+        std::lock_guard<std::mutex> g(s->pool_mu);
+        for (size_t i = 0; i < s->pool.size(); ++i) {
+            esch_batch *b = s->pool[i];
+            if (b->max_reads >= n && b->max_bases >= bases) {
+                s->pool.erase(s->pool.begin() + (long)i);
+                *out = b;
+                return ESCH_OK;
+            }
+        }
+    }
+    // grow geometrically so a stream of similar batches reuses one buffer set
+    size_t cap_n = 1024, cap_b = 1 << 16;
+    while (cap_n < n) cap_n *= 2;
+    while (cap_b < bases) cap_b *= 2;
+    return esch_batch_create(s, cap_n, cap_b, out);
+}
+
+void give_batch(esch_scorer *s, esch_batch *b) {
+    std::lock_guard<std::mutex> g(s->pool_mu);
+    s->pool.push_back(b);
+    if (s->pool.size() > 4) {  // keep a few idle buffer sets, free the oldest
+        esch_batch *old = s->pool.front();
+        s->pool.erase(s->pool.begin());
+        esch_batch_free(old);
+    }
+}
+
+}  // namespace
+
+esch_status esch_submit(esch_scorer *s, esch_prepared *p, esch_pending **out) {
+    std::unique_ptr<esch_prepared> prep(p);
+    if (!s || !p || !out) return fail(ESCH_INVALID_ARG, "null argument");
+    *out = nullptr;
+    HIPC(hipSetDevice(s->ordinal));
+    std::unique_ptr<esch_pending> pend(new esch_pending);
+    pend->n = p->n;
+    pend->scored = p->scored;
+    size_t total = 0;
+    for (int32_t L : p->len) total += (size_t)L;
+    esch_batch *b = nullptr;
+    if (esch_status st = take_batch(s, p->n, total, &b)) return st;
+    pend->b = b;
+    esch_status st = esch_batch_load_concat(b, p->codes.data(), p->off.data(), p->len.data(), p->n, 0, 0);
+    if (!st) st = esch_batch_run(b, ESCH_RUN_SCORES16, 0, -1, 1);
+    if (st) {
+        give_batch(s, b);
+        return st;
+    }
+    *out = pend.release();
+    return ESCH_OK;
+}
+
+esch_status esch_finish(esch_scorer *s, esch_pending *p, int16_t *scores, uint8_t *scored) {
+    std::unique_ptr<esch_pending> pend(p);
+    if (!s || !p) return fail(ESCH_INVALID_ARG, "null argument");
+    if (p->n && (!scores || !scored)) {
+        give_batch(s, p->b);
+        return fail(ESCH_INVALID_ARG, "null output");
+    }
+    esch_status st = ESCH_OK;
+    if (p->n) st = esch_batch_read_scores16(p->b, scores, nullptr);
+    else st = esch_batch_wait(p->b);
+    give_batch(s, p->b);
+    if (st) return st;
+    if (p->n) memcpy(scored, p->scored.data(), p->n);
+    return ESCH_OK;
+}
+
+esch_status esch_score_batch(esch_scorer *s, const uint8_t *const *queries, const size_t *lens, size_t n,
+                             int16_t *scores, uint8_t *scored) {
+    esch_prepared *p = esch_prepare(s, queries, lens, n);
+    if (!p) return ESCH_INVALID_ARG;
+    esch_pending *pend = nullptr;
+    if (esch_status st = esch_submit(s, p, &pend)) return st;
+    return esch_finish(s, pend, scores, scored);
+}
+
+esch_status esch_score_batch_i32(esch_scorer *s, const uint8_t *const *queries, const size_t *lens, size_t n,
+                                 int32_t *scores, uint8_t *scored) {
+    if (!s || (n && (!queries || !lens || !scores || !scored))) return fail(ESCH_INVALID_ARG, "null argument");
+    size_t total = 0;
+    for (size_t k = 0; k < n; ++k) total += lens[k];
+    esch_batch *b = nullptr;
+    if (esch_status st = take_batch(s, n, total, &b)) return st;
+    esch_status st = esch_batch_load(b, queries, lens, n, 0, 0);
+    if (!st) st = esch_batch_run(b, ESCH_RUN_SCORES32, 0, -1, 1);
+    if (!st) st = esch_batch_read_scores32(b, scores, scored);
+    give_batch(s, b);
+    return st;
+}
+
+esch_status esch_select_batch(esch_scorer *s, const uint8_t *const *reads, const size_t *lens, size_t n,
+                              int both, int32_t min_score, int64_t max_ties, size_t tie_cap,
+                              esch_read_select *out, int32_t *ties) {
+    if (!s || (n && (!reads || !lens || !out))) return fail(ESCH_INVALID_ARG, "null argument");
+    size_t total = 0;
+    for (size_t k = 0; k < n; ++k) total += lens[k];
+    esch_batch *b = nullptr;
+    if (esch_status st = take_batch(s, n, total, &b)) return st;
+    esch_status st = esch_batch_load(b, reads, lens, n, both, 0);
+    if (!st) st = esch_batch_run(b, ESCH_RUN_SCORES32 | ESCH_RUN_SELECT, min_score, max_ties, tie_cap);
+    if (!st) st = esch_batch_read_select(b, out, ties);
+    give_batch(s, b);
+    return st;
+}
+
+esch_status esch_select_pairs(esch_scorer *s, const uint8_t *const *mates1, const size_t *lens1,
+                              const uint8_t *const *mates2, const size_t *lens2, size_t n, int both_strands,
+                              int32_t min_score, size_t top_cap, esch_pair_select *out, int32_t *top) {
+    if (!s || (n && (!mates1 || !lens1 || !mates2 || !lens2 || !out)))
+        return fail(ESCH_INVALID_ARG, "null argument");
+    std::vector<const uint8_t *> ptrs(2 * n);
+    std::vector<size_t> lens(2 * n);
+    size_t total = 0;
+    for (size_t k = 0; k < n; ++k) {
+        ptrs[2 * k] = mates1[k];
+        ptrs[2 * k + 1] = mates2[k];
+        lens[2 * k] = lens1[k];
+        lens[2 * k + 1] = lens2[k];
+        total += lens1[k] + lens2[k];
+    }
+    esch_batch *b = nullptr;
+    if (esch_status st = take_batch(s, 2 * n, total, &b)) return st;
+    esch_status st = esch_batch_load(b, ptrs.data(), lens.data(), 2 * n, 1, 1);
+    const int what = ESCH_RUN_SCORES32 | ESCH_RUN_PAIRS | (both_strands ? ESCH_RUN_PAIR_BOTH : 0);
+    if (!st) st = esch_batch_run(b, what, min_score, -1, top_cap);
+    if (!st) st = esch_batch_read_pairs(b, out, top);
+    give_batch(s, b);
+    return st;
+}
+
+}  // extern "C"

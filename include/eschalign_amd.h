@@ -1,0 +1,222 @@
+/*
+ * eschalign_amd — MI355X (gfx950) panel scoring behind a C ABI.
+ *
+ * Drop-in for eschalign's GPU score backend: every entry point below is what
+ * the reference's `GpuScorer` / `ScoreMatrix` boundary would bind through a
+ * thin FFI crate. Citations are relative to the eschalign repository root
+ * (lib/ = crates/eschalign/src/).
+ *
+ * Conventions (the reference's, lib/cuda/mod.rs:516-519, lib/alphabet.rs):
+ *   - sequences are alphabet CODES 0..=4 (A,C,G,T/U, wildcard), not letters;
+ *   - scores row-major [query * n_refs + ref], references in panel order;
+ *   - no exceptions cross the ABI: every call returns an esch_status, and
+ *     esch_last_error() gives the text of the calling thread's last error.
+ *
+ * Threading (lib/cuda/mod.rs:328-341, crates/eschalign-cli/src/align.rs:2101):
+ * esch_prepare is thread-safe and touches no device state; esch_submit /
+ * esch_score_batch on one scorer must not run concurrently (one thread per
+ * scorer, as the reference's &mut self). Device batches (esch_batch_*) are
+ * independent of each other and may run on several threads at once.
+ */
+#ifndef ESCHALIGN_AMD_H
+#define ESCHALIGN_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* GpuError (lib/cuda/mod.rs:144-153) plus the crate's AlignError variants
+ * that the boundary can raise (lib/error.rs:7-28). */
+typedef enum {
+    ESCH_OK = 0,
+    ESCH_UNAVAILABLE = 1,           /* no HIP runtime / no device       GpuError::Unavailable */
+    ESCH_DRIVER = 2,                /* a HIP call failed                 GpuError::Driver */
+    ESCH_CODE_OBJECT = 3,           /* no gfx950 code object for device  GpuError::Compile */
+    ESCH_REFERENCE_TOO_LONG = 4,    /* panel reference > ESCH_MAX_REF_LEN GpuError::ReferenceTooLong */
+    ESCH_INVALID_ARG = 5,           /* null pointer, bad size, bad mode */
+    ESCH_SCORING_INVALID = 6,       /* AlignError::ScoringInvalid (lib/scoring.rs:70-100) */
+    ESCH_EMPTY_PANEL = 7,           /* AlignError::EmptyPanel */
+    ESCH_EMPTY_REFERENCE = 8,       /* AlignError::EmptyReference */
+    ESCH_CAPACITY = 9               /* a device batch is smaller than the input */
+} esch_status;
+
+typedef enum { ESCH_LOCAL = 0, ESCH_SEMIGLOBAL = 1 } esch_mode; /* lib/scoring.rs:151-165 */
+
+/* Longest read the drop-in score matrix scores (lib/cuda/mod.rs:90-91,
+ * the same scheduling bound as the reference). */
+#define ESCH_MAX_READ_LEN 4096
+/* Longest panel reference this library accepts. The reference's CUDA
+ * kernel stops at 3,072 nt (shared memory, lib/cuda/mod.rs:94-95); here a
+ * reference is streamed through scalar registers and its DP boundary lives
+ * in device scratch sized by the panel, so the bound is the i16 profile
+ * index of the reference's own SIMD layout (lens: Vec<i16>). */
+#define ESCH_MAX_REF_LEN 16384
+
+typedef struct esch_scorer esch_scorer;     /* GpuScorer        lib/cuda/mod.rs:328-341 */
+typedef struct esch_prepared esch_prepared; /* PreparedBatch    lib/cuda/mod.rs:711-720 */
+typedef struct esch_pending esch_pending;   /* PendingBatch     lib/cuda/mod.rs:730-736 */
+typedef struct esch_batch esch_batch;       /* device-resident batch (no counterpart; see below) */
+
+/* Text of the calling thread's last error ("" if none). */
+const char *esch_last_error(void);
+
+/* cuda::device_visible() — lib/cuda/mod.rs:130-140. Cached per process;
+ * 0 when the HIP runtime sees no device. */
+int esch_device_visible(void);
+
+/* Validate a scoring scheme exactly as Scoring::validate
+ * (lib/scoring.rs:70-100): ESCH_OK or ESCH_SCORING_INVALID. */
+esch_status esch_scoring_validate(int32_t match, int32_t mismatch, int32_t gap_open,
+                                  int32_t gap_extend);
+
+/* simd::fits_i16 — lib/simd/mod.rs:222-227. */
+int esch_fits_i16(size_t n, size_t max_ref_len, int32_t match, int32_t mismatch,
+                  int32_t gap_open, int32_t gap_extend);
+
+/* GpuScorer::new_on_device(ordinal, &Panel, Scoring, Mode) —
+ * lib/cuda/mod.rs:350-443 (with Panel::new's refusals, lib/panel.rs:32-57).
+ * Uploads the panel once (2-bit codes + wildcard bitmap) and sizes the
+ * kernels for this device. refs[i] points at ref_lens[i] codes. */
+esch_status esch_scorer_new(int ordinal, const uint8_t *const *refs, const size_t *ref_lens,
+                            size_t n_refs, int32_t match, int32_t mismatch, int32_t gap_open,
+                            int32_t gap_extend, esch_mode mode, esch_scorer **out);
+/* Drop for GpuScorer. NULL is a no-op. */
+void esch_scorer_free(esch_scorer *s);
+
+/* GpuScorer::scores_len — lib/cuda/mod.rs:506-508:
+ * 0 < len <= ESCH_MAX_READ_LEN && fits_i16(len, panel max_len). */
+int esch_scores_len(const esch_scorer *s, size_t len);
+size_t esch_n_refs(const esch_scorer *s);
+
+/* GpuScorer::prepare — lib/cuda/mod.rs:533-576. Host only, thread-safe,
+ * may overlap a running kernel. Copies the queries; the caller keeps
+ * ownership of its buffers. Returns NULL (and sets the error) on bad args. */
+esch_prepared *esch_prepare(const esch_scorer *s, const uint8_t *const *queries,
+                            const size_t *lens, size_t n);
+void esch_prepared_free(esch_prepared *p);
+/* Rows of the prepared batch and how many of them will be scored. */
+size_t esch_prepared_len(const esch_prepared *p);
+size_t esch_prepared_n_scored(const esch_prepared *p);
+
+/* GpuScorer::submit — lib/cuda/mod.rs:581-699. Consumes `p` (also on
+ * error). Uploads and queues the kernels on the scorer's stream; returns
+ * without waiting. */
+esch_status esch_submit(esch_scorer *s, esch_prepared *p, esch_pending **out);
+
+/* GpuScorer::finish + ScoreMatrix::from_parts — lib/cuda/mod.rs:702-708,
+ * lib/mapper.rs:107-115. Waits, then writes scores[n * n_refs] (int16,
+ * rows of unscored queries zero) and scored[n] (1 = row present, 0 = the
+ * caller scores it on the CPU, ScoreMatrix::row -> None). Consumes `p`. */
+esch_status esch_finish(esch_scorer *s, esch_pending *p, int16_t *scores, uint8_t *scored);
+
+/* GpuScorer::score_batch — lib/cuda/mod.rs:524-528: prepare + submit +
+ * finish. n == 0 is valid (an empty matrix). */
+esch_status esch_score_batch(esch_scorer *s, const uint8_t *const *queries, const size_t *lens,
+                             size_t n, int16_t *scores, uint8_t *scored);
+
+/* Diagnostics — lib/cuda/mod.rs:446-488. geometry = (threads per block of
+ * the batched kernel, references per work item, persistent blocks). */
+esch_status esch_device_name(const esch_scorer *s, char *buf, size_t cap);
+void esch_geometry(const esch_scorer *s, size_t *lanes, size_t *groups, size_t *blocks);
+int esch_registers(const esch_scorer *s);   /* VGPRs of the batched kernel */
+int esch_local_bytes(const esch_scorer *s); /* private (spill) bytes of the batched kernel */
+
+/* ---------------------------------------------------------------------
+ * Extensions beyond the reference's i16 ScoreMatrix (no counterpart in the
+ * reference; they serve the same callers, Aligner::map_reads_scored and
+ * map_pairs_scored, lib/mapper.rs:400-405, lib/mapper/pair.rs:157-162).
+ * --------------------------------------------------------------------- */
+
+/* Every query scored, int32 out[n * n_refs]; a query outside the packed
+ * int16 bound is rescored by the int32 kernel, so nothing is left for the
+ * CPU except empty queries (row of zeros, scored[k] = 0). */
+esch_status esch_score_batch_i32(esch_scorer *s, const uint8_t *const *queries,
+                                 const size_t *lens, size_t n, int32_t *scores, uint8_t *scored);
+
+/* One read's score-stage selection, fused on device: the first half of
+ * Aligner::map_reads_scored (lib/mapper.rs:444-472). */
+typedef struct {
+    int32_t best;      /* best score over every (ref, strand); 0 when empty */
+    int32_t n_tied;    /* size of the tie set */
+    int32_t subopt;    /* best score outside the tie set (valid when has_subopt) */
+    int32_t flags;     /* bit 0: read non-empty (best present); bit 1: has_subopt */
+    int32_t n_take;    /* ties to trace: min(n_tied, max_ties + 1), 0 below min_score */
+} esch_read_select;
+
+/* Reads (codes) scored forward and, with both_strands, reverse-complemented
+ * on device; the tie set in reference order, forward before reverse.
+ * max_ties < 0 = all (None). ties[k * tie_cap + i] = (ref << 1) | reverse
+ * for i < min(n_take, tie_cap). */
+esch_status esch_select_batch(esch_scorer *s, const uint8_t *const *reads, const size_t *lens,
+                              size_t n, int both_strands, int32_t min_score, int64_t max_ties,
+                              size_t tie_cap, esch_read_select *out, int32_t *ties);
+
+/* One pair's candidates (lib/mapper/pair.rs:185-231), fused on device:
+ * the top tier of eligible (ref, template strand) candidates by summed mate
+ * score, both mates >= min_score. */
+typedef struct {
+    int32_t n_cand;     /* eligible candidates */
+    int32_t top_sum;    /* best summed score (valid when n_cand > 0) */
+    int32_t n_top;      /* candidates in the top tier */
+    int32_t next_sum;   /* the next lower sum (valid when flags bit 1) */
+    int32_t flags;      /* bit 0: both mates non-empty; bit 1: a second tier exists */
+} esch_pair_select;
+
+/* mates1[k], mates2[k]: codes of pair k's mates (adapters already
+ * trimmed). top[k * top_cap + i] = (ref << 1) | template_reverse. */
+esch_status esch_select_pairs(esch_scorer *s, const uint8_t *const *mates1, const size_t *lens1,
+                              const uint8_t *const *mates2, const size_t *lens2, size_t n,
+                              int both_strands, int32_t min_score, size_t top_cap,
+                              esch_pair_select *out, int32_t *top);
+
+/* ---------------------------------------------------------------------
+ * Device-resident batches: inputs uploaded once, every kernel reading its
+ * sizes from device memory, the launch sequence captured in a hipGraph and
+ * replayed. Used by the serving loop and the benchmark (no counterpart in
+ * the reference, whose batches always come from host memory).
+ * --------------------------------------------------------------------- */
+
+enum {
+    ESCH_RUN_SCORES16 = 1,   /* int16 matrix, drop-in rows only (scores_len) */
+    ESCH_RUN_SCORES32 = 2,   /* int32 matrix, every non-empty query */
+    ESCH_RUN_SELECT = 4,     /* fused unpaired selection (needs SCORES32 rows) */
+    ESCH_RUN_PAIRS = 8,      /* fused pair selection */
+    ESCH_RUN_PLAIN = 16,     /* score with the plain one-thread-per-pair int32 kernel */
+    ESCH_RUN_NO_GRAPH = 32,  /* launch eagerly instead of replaying a captured graph */
+    ESCH_RUN_PAIR_BOTH = 64  /* pairs: consider the reverse template strand too */
+};
+
+esch_status esch_batch_create(esch_scorer *s, size_t max_queries, size_t max_bases,
+                              esch_batch **out);
+void esch_batch_free(esch_batch *b);
+/* Device bytes held by a batch of this capacity for this scorer. */
+size_t esch_batch_footprint(const esch_scorer *s, size_t max_queries, size_t max_bases);
+/* Upload reads (codes). both_strands appends their reverse complements on
+ * device; pairs = 1 treats reads as mate1, mate2 interleaved. */
+esch_status esch_batch_load(esch_batch *b, const uint8_t *const *reads, const size_t *lens,
+                            size_t n, int both_strands, int pairs);
+/* Same, from one concatenated code buffer and per-read offsets. */
+esch_status esch_batch_load_concat(esch_batch *b, const uint8_t *codes, const uint64_t *offs,
+                                   const int32_t *lens, size_t n, int both_strands, int pairs);
+/* Queue the device pipeline (async on the batch's own stream). */
+esch_status esch_batch_run(esch_batch *b, int what, int32_t min_score, int64_t max_ties,
+                           size_t tie_cap);
+esch_status esch_batch_wait(esch_batch *b);
+/* Milliseconds of the last run: the DP kernel alone (HIP events on its
+ * stream) and the whole device pipeline. */
+esch_status esch_batch_timing(const esch_batch *b, float *dp_ms, float *total_ms);
+/* Cells of the last run (sum over scored queries of len x panel bases) and
+ * how many queries went to the int16 and int32 kernels. */
+esch_status esch_batch_stats(const esch_batch *b, uint64_t *cells, uint64_t *n16, uint64_t *n32);
+esch_status esch_batch_read_scores32(esch_batch *b, int32_t *scores, uint8_t *scored);
+esch_status esch_batch_read_scores16(esch_batch *b, int16_t *scores, uint8_t *scored);
+esch_status esch_batch_read_select(esch_batch *b, esch_read_select *out, int32_t *ties);
+esch_status esch_batch_read_pairs(esch_batch *b, esch_pair_select *out, int32_t *top);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

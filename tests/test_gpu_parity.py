@@ -1,0 +1,308 @@
+"""The gfx950 kernels against the CPU oracle, by equality (runs on an MI355X).
+
+A port of the reference's crates/eschalign/tests/gpu_parity.rs — the same
+seeded panels and reads (xorshift64*, seeds 0x5eed_0101..0104), the same five
+schemes (two of them with a positive mismatch or free gaps), every read
+length 1..40 around the 16-row stripe, reverse complements in the same
+batch, a 300-reference panel, the real tRNA fixture reads on both
+47-reference panels, unscored rows and refusal — plus the parts of this
+build that have no CUDA counterpart: the int32 rescue, the batched kernel
+over many length bins, the plain kernel, fused selection and graph replay.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import eschalign_amd as ea
+from oracle import pyoracle as O
+import refgen as G
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def codes_of(reads):
+    return [G.encode(r) for r in reads]
+
+
+def oracle(queries, panel_codes, scoring, mode):
+    return O.score_matrix(queries, panel_codes, scoring, mode == ea.Mode.LOCAL, THREADS)
+
+
+def check(panel_letters, reads, scoring, mode, what):
+    """Every query (and its reverse complement) against every reference in
+    one batch, drop-in i16 matrix and int32 extension both equal to the
+    oracle (gpu_parity.rs:128-186)."""
+    panel = ea.Panel(panel_letters)
+    gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+    queries = codes_of(reads)
+    queries += [ea.reverse_complement_codes(q) for q in queries]
+    want = oracle(queries, panel.codes, scoring, mode)
+    m = gpu.score_batch(queries)
+    assert len(m) == len(queries) and m.n_refs() == len(panel)
+    for k, q in enumerate(queries):
+        row = m.row(k)
+        assert row is not None, f"{what}: query {k} ({len(q)} nt) left unscored"
+        bad = np.nonzero(row.astype(np.int32) != want[k])[0]
+        assert bad.size == 0, (f"{what} {ea.Mode.name(mode)} {scoring}: query {k} ({len(q)} nt) "
+                               f"vs ref {bad[0]}: {row[bad[0]]} != {want[k][bad[0]]}")
+    s32, ok = gpu.score_batch_i32(queries)
+    assert ok.all()
+    assert np.array_equal(s32, want), what
+    return len(queries) * len(panel)
+
+
+def test_gpu_scores_equal_scalar():
+    """gpu_parity.rs:207-250."""
+    total = 0
+    rng = G.Rng(0x5EED_0101)
+    small = G.random_panel(rng, 37, 20, 280)
+    reads = G.reads_for(rng, small, 60)
+    for scoring in G.SCHEMES_GPU:
+        for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
+            total += check(small, reads, scoring, mode, "random 37-ref panel")
+    wide = G.random_panel(rng, 300, 20, 600)
+    reads = G.reads_for(rng, wide, 6)
+    for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
+        total += check(wide, reads, (2, -1, -10, -1), mode, "random 300-ref panel")
+    fixture = G.fixture_reads()
+    assert len(fixture) >= 5
+    for fasta in ("trna_reference.fa", "trna_reference_ambiguous.fa"):
+        panel = G.fixture_panel(fasta)
+        assert len(panel) == 47
+        for scoring in G.SCHEMES_GPU:
+            for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
+                total += check(panel, fixture, scoring, mode, fasta)
+    print(f"[gpu_parity] {total} (query, reference) pairs equal to the oracle")
+
+
+def test_gpu_leaves_what_it_cannot_score_to_the_cpu():
+    """gpu_parity.rs:356-378."""
+    rng = G.Rng(0x5EED_0102)
+    panel = ea.Panel(G.random_panel(rng, 20, 50, 100))
+    scoring = (2, -1, -10, -1)
+    gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), ea.Mode.LOCAL)
+    long = G.encode(G.random_seq(rng, ea.MAX_READ_LEN + 1, 0.0))
+    ok = G.encode(G.random_seq(rng, 120, 0.0))
+    edge = G.encode(G.random_seq(rng, ea.MAX_READ_LEN, 0.0))
+    queries = [ok, np.zeros(0, np.uint8), long, edge, ok]
+    m = gpu.score_batch(queries)
+    assert m.row(1) is None, "empty read"
+    assert m.row(2) is None, "read over MAX_READ_LEN"
+    want = oracle(queries, panel.codes, scoring, ea.Mode.LOCAL)
+    for k in (0, 3, 4):
+        assert np.array_equal(m.row(k).astype(np.int32), want[k]), k
+    assert m.scores[1].sum() == 0 and m.scores[2].sum() == 0
+    assert gpu.score_batch([]).is_empty()
+    # the int32 extension scores the long read too
+    s32, sc = gpu.score_batch_i32(queries)
+    assert list(sc) == [True, False, True, True, True]
+    for k in (0, 2, 3, 4):
+        assert np.array_equal(s32[k], want[k]), k
+
+
+def test_gpu_refuses_a_reference_it_cannot_hold():
+    """gpu_parity.rs:380-389, at this library's bound."""
+    with pytest.raises(ea.GpuError) as e:
+        ea.GpuScorer(ea.Panel([b"A" * (ea.MAX_REF_LEN + 1)]), ea.Scoring(), ea.Mode.LOCAL)
+    assert e.value.kind == "ReferenceTooLong"
+
+
+def test_prepare_submit_finish_overlap():
+    """Two batches in flight at once (prepare k+1 while k runs,
+    crates/eschalign-cli/src/align.rs:2879-2887) give the same rows."""
+    rng = G.Rng(0x5EED_0201)
+    panel_l = G.random_panel(rng, 23, 60, 120)
+    panel = ea.Panel(panel_l)
+    gpu = ea.GpuScorer(panel, ea.Scoring(), ea.Mode.SEMIGLOBAL)
+    b1 = codes_of(G.reads_for(rng, panel_l, 50))
+    b2 = codes_of(G.reads_for(rng, panel_l, 70))
+    p1 = gpu.submit(gpu.prepare(b1))
+    p2 = gpu.submit(gpu.prepare(b2))
+    m2 = gpu.finish(p2)
+    m1 = gpu.finish(p1)
+    for batch, m in ((b1, m1), (b2, m2)):
+        want = oracle(batch, panel.codes, (2, -1, -10, -1), ea.Mode.SEMIGLOBAL)
+        assert np.array_equal(m.scores.astype(np.int32), want)
+
+
+def test_int32_rescue_outside_the_int16_bound():
+    """Queries whose DP values can reach the int16 clamps are scored by the
+    int32 kernel and still equal the oracle; the drop-in matrix leaves them
+    to the CPU exactly as scores_len says."""
+    rng = G.Rng(0x5EED_0301)
+    panel_l = G.random_panel(rng, 9, 200, 200)
+    panel = ea.Panel(panel_l)
+    scoring = (1000, -1000, -1000, -1000)
+    reads = [G.random_seq(rng, 20, 0.0), panel_l[0][:150], G.mutate(rng, panel_l[3]),
+             G.random_seq(rng, 300, 0.05)]
+    queries = codes_of(reads)
+    for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
+        gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+        want = oracle(queries, panel.codes, scoring, mode)
+        s32, ok = gpu.score_batch_i32(queries)
+        assert ok.all()
+        assert np.array_equal(s32, want)
+        m = gpu.score_batch(queries)
+        for k, q in enumerate(queries):
+            assert (m.row(k) is not None) == gpu.scores_len(len(q))
+            assert gpu.scores_len(len(q)) == O.fits_i16(len(q), panel.max_len(), scoring)
+
+
+def test_batched_kernel_many_bins_and_chunks():
+    """Hundreds of reads of every length 1..1000 (chunks that straddle
+    length bins, padding rows in several stripes), both modes, three schemes,
+    a panel with wildcards and ragged reference lengths."""
+    rng = G.Rng(0x5EED_0401)
+    panel_l = G.random_panel(rng, 41, 30, 400)
+    panel = ea.Panel(panel_l)
+    reads = []
+    for t in range(700):
+        length = 1 + rng.below(1000) if t % 3 else 1 + rng.below(40)
+        if t % 2:
+            r = panel_l[rng.below(len(panel_l))]
+            read = G.mutate(rng, (r * (1 + length // max(1, len(r))))[:length])
+        else:
+            read = G.random_seq(rng, length, 0.02)
+        reads.append(read if read else b"A")
+    queries = codes_of(reads)
+    for scoring in ((2, -1, -10, -1), (1, -1, -2, -1), (3, 1, -5, -2)):
+        for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
+            gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+            want = oracle(queries, panel.codes, scoring, mode)
+            s32, ok = gpu.score_batch_i32(queries)
+            assert ok.all()
+            bad = np.argwhere(s32 != want)
+            assert bad.size == 0, (scoring, mode, bad[:5], len(queries[bad[0][0]]))
+
+
+def test_plain_kernel_equals_batched_kernel_and_oracle():
+    rng = G.Rng(0x5EED_0501)
+    panel_l = G.random_panel(rng, 70, 100, 100)
+    panel = ea.Panel(panel_l)
+    reads = codes_of(G.reads_for(rng, panel_l, 300))
+    for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
+        gpu = ea.GpuScorer(panel, ea.Scoring(1, -1, -2, -1), mode)
+        b = gpu.batch(len(reads), sum(len(r) for r in reads))
+        b.load(reads, both_strands=True)
+        b.run(ea.RUN_SCORES32 | ea.RUN_PLAIN)
+        plain, _ = b.scores32()
+        b.run(ea.RUN_SCORES32)
+        batched, _ = b.scores32()
+        assert np.array_equal(plain, batched)
+        rows = reads + [ea.reverse_complement_codes(q) for q in reads]
+        assert np.array_equal(batched, oracle(rows, panel.codes, (1, -1, -2, -1), mode))
+
+
+def test_select_equals_oracle():
+    """The fused reduction against lib/mapper.rs:444-472 (oracle eo_select),
+    both strands, forward only, max_ties and min_score."""
+    rng = G.Rng(0x5EED_0601)
+    # paralog-rich panel so that ties are common
+    seeds = G.random_panel(rng, 12, 60, 60)
+    panel_l = []
+    for s in seeds:
+        panel_l += [s, G.mutate(rng, s), s]
+    panel = ea.Panel(panel_l)
+    reads = codes_of(G.reads_for(rng, panel_l, 200)) + [np.zeros(0, np.uint8)]
+    scoring = (2, -1, -10, -1)
+    for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
+        gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+        fwd = oracle(reads, panel.codes, scoring, mode)
+        rev = oracle([ea.reverse_complement_codes(q) for q in reads], panel.codes, scoring, mode)
+        for both, min_score, max_ties in ((True, 0, None), (False, 0, None), (True, 40, 1),
+                                          (True, -5, 0)):
+            got = gpu.select_batch(reads, both_strands=both, min_score=min_score,
+                                   max_ties=max_ties)
+            for k, q in enumerate(reads):
+                if len(q) == 0:
+                    assert got[k].best is None and got[k].take == []
+                    continue
+                want = O.select(fwd[k], rev[k] if both else None, min_score, max_ties)
+                g = got[k]
+                assert (g.best, g.n_tied, g.suboptimal, g.take) == want, (k, both, min_score)
+
+
+def test_select_pairs_equals_oracle():
+    """Top tier of summed mate scores (lib/mapper/pair.rs:185-231)."""
+    rng = G.Rng(0x5EED_0701)
+    base = G.random_panel(rng, 10, 70, 25)
+    panel_l = []
+    for s in base:
+        t = bytearray(s)
+        t[len(t) // 2] = ord("A") if t[len(t) // 2] != ord("A") else ord("C")
+        panel_l += [s, bytes(t)]
+    panel = ea.Panel(panel_l)
+    scoring = (2, -1, -10, -1)
+    m1, m2 = [], []
+    for k in range(150):
+        r = panel_l[rng.below(len(panel_l))]
+        a = G.mutate(rng, r[: len(r) // 2 + 10])
+        b = G.mutate(rng, r[len(r) // 2 - 10:])
+        b = bytes(b"ACGTN"[c] for c in ea.reverse_complement_codes(G.encode(b)))
+        if k % 5 == 4:
+            a, b = b, a
+        m1.append(G.encode(a))
+        m2.append(G.encode(b) if k % 17 else np.zeros(0, np.uint8))
+    for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
+        gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+        rows = {}
+        for name, mates in (("m1", m1), ("m2", m2)):
+            rows[name] = (oracle(mates, panel.codes, scoring, mode),
+                          oracle([ea.reverse_complement_codes(q) for q in mates], panel.codes,
+                                 scoring, mode))
+        for both, min_score in ((False, 0), (True, 0), (True, 30)):
+            got = gpu.select_pairs(m1, m2, both_strands=both, min_score=min_score)
+            for k in range(len(m1)):
+                g = got[k]
+                if len(m1[k]) == 0 or len(m2[k]) == 0:
+                    assert g.n_cand == 0 and g.top == []
+                    continue
+                r = [[rows["m1"][0][k], rows["m1"][1][k]], [rows["m2"][0][k], rows["m2"][1][k]]]
+                cands = O.pair_cands(r, min_score, both)
+                assert g.n_cand == len(cands), k
+                if not cands:
+                    assert g.top == []
+                    continue
+                top = cands[0][0]
+                tier = [(c[1], c[2]) for c in cands if c[0] == top]
+                below = [c[0] for c in cands if c[0] != top]
+                assert g.top_sum == top and g.top == tier and g.n_top == len(tier), k
+                assert g.next_sum == (below[0] if below else None), k
+
+
+def test_graph_replay_matches_eager():
+    """A captured pipeline replayed for batches of different sizes, with
+    eager work between replays, equals the eager launch every time."""
+    rng = G.Rng(0x5EED_0801)
+    panel_l = G.random_panel(rng, 30, 80, 80)
+    panel = ea.Panel(panel_l)
+    gpu = ea.GpuScorer(panel, ea.Scoring(), ea.Mode.LOCAL)
+    b = gpu.batch(400, 400 * 400)
+    for rep in range(4):
+        reads = codes_of(G.reads_for(rng, panel_l, 100 + 80 * rep))
+        b.load(reads, both_strands=bool(rep % 2))
+        b.run(ea.RUN_SCORES32 | ea.RUN_NO_GRAPH)
+        eager, _ = b.scores32()
+        b.run(ea.RUN_SCORES32)
+        graph, _ = b.scores32()
+        b.run(ea.RUN_SCORES32)
+        graph2, _ = b.scores32()
+        assert np.array_equal(eager, graph) and np.array_equal(graph, graph2), rep
+        rows = reads + ([ea.reverse_complement_codes(q) for q in reads] if rep % 2 else [])
+        assert np.array_equal(graph, oracle(rows, panel.codes, (2, -1, -10, -1), ea.Mode.LOCAL))
+
+
+def test_result_is_the_same_alone_and_in_a_mix():
+    """Batch composition changes chunking and bins, never a score."""
+    rng = G.Rng(0x5EED_0901)
+    panel_l = G.random_panel(rng, 17, 100, 60)
+    panel = ea.Panel(panel_l)
+    gpu = ea.GpuScorer(panel, ea.Scoring(1, -1, -2, -1), ea.Mode.SEMIGLOBAL)
+    reads = codes_of(G.reads_for(rng, panel_l, 260))
+    s_all, _ = gpu.score_batch_i32(reads)
+    for k in (0, 7, 55, 199, len(reads) - 1):
+        alone, _ = gpu.score_batch_i32([reads[k]])
+        assert np.array_equal(alone[0], s_all[k])

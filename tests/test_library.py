@@ -1,0 +1,103 @@
+"""The C-ABI library without a GPU: it loads, exports every symbol the
+header declares, and its host-side logic (validation, refusals, the
+scores_len bound) matches the reference's rules. No compute calls."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import eschalign_amd as ea
+from oracle import pyoracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "eschalign_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(esch_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = ea.load_library()
+    names = declared_functions()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(L, n), f"{n} declared in the header but not exported"
+    assert sorted(ea.EXPORTS) == names
+
+
+def test_scoring_validation_matches_reference_rules():
+    for s in [(2, -1, -10, -1), (1, -1, -2, -1), (3, 1, -5, -2), (2, -1, 0, 0), (1000, -1000, -1000, -1000)]:
+        assert ea.load_library().esch_scoring_validate(*s) == 0
+        assert O.validate(*s) == 0
+    for s in [(0, -1, -10, -1), (2, 3, -10, -1), (2, -1, 10, -1), (2, -1, -10, 1), (1001, -1, -1, -1)]:
+        assert ea.load_library().esch_scoring_validate(*s) == ea.SCORING_INVALID
+        assert O.validate(*s) != 0
+        with pytest.raises(ValueError):
+            ea.Scoring(*s)
+    assert str(ea.Scoring.parse("2,-1,-10,-1")) == "2,-1,-10,-1"
+    for bad in ("2,-1,-10", "2,-1,-10,-x"):
+        with pytest.raises(ValueError):
+            ea.Scoring.parse(bad)
+    assert ea.Scoring().gap(1) == -10 and ea.Scoring().gap(3) == -12
+
+
+def test_fits_i16_matches_oracle():
+    rng = np.random.default_rng(3)
+    for _ in range(500):
+        n = int(rng.integers(0, 20000))
+        L = int(rng.integers(1, 20000))
+        s = (int(rng.integers(1, 30)), int(rng.integers(-30, 1)), int(rng.integers(-50, 1)),
+             int(rng.integers(-50, 1)))
+        assert ea.fits_i16(n, L, ea.Scoring(*s)) == O.fits_i16(n, L, s)
+
+
+def test_encode_matches_oracle():
+    every = bytes(range(256))
+    assert np.array_equal(ea.encode(every), O.encode(every))
+    c = ea.encode(b"ACGTNacgtu")
+    assert np.array_equal(ea.reverse_complement_codes(c), O.revcomp(c))
+
+
+def test_refusals_before_any_device_work():
+    """Panel::new and GpuScorer::new refusals are host-side: empty panel,
+    empty reference, invalid scoring, a reference over the bound, and codes
+    outside 0..=4 — the same answer with or without a device."""
+    L = ea.load_library()
+    out = C.c_void_p()
+
+    def new(refs, scoring=(2, -1, -10, -1), mode=0):
+        arrs = [np.ascontiguousarray(r, np.uint8) for r in refs]
+        ptrs = (C.c_void_p * max(1, len(arrs)))(*[a.ctypes.data for a in arrs])
+        lens = (C.c_size_t * max(1, len(arrs)))(*[a.size for a in arrs])
+        return L.esch_scorer_new(0, ptrs, lens, len(arrs), *scoring, mode, C.byref(out))
+
+    assert new([]) == ea.EMPTY_PANEL
+    assert new([np.zeros(0, np.uint8)]) == ea.EMPTY_REFERENCE
+    assert new([np.zeros(10, np.uint8)], (2, 3, -1, -1)) == ea.SCORING_INVALID
+    assert new([np.zeros(ea.MAX_REF_LEN + 1, np.uint8)]) == ea.REFERENCE_TOO_LONG
+    assert new([np.full(10, 7, np.uint8)]) == ea.INVALID_ARG
+    assert new([np.zeros(10, np.uint8)], mode=9) == ea.INVALID_ARG
+    assert b"reference" in L.esch_last_error() or b"mode" in L.esch_last_error()
+
+
+def test_no_device_is_an_error_not_a_crash():
+    """lib/cuda/mod.rs:356-366: a missing device is GpuError::Unavailable."""
+    if ea.device_visible():
+        pytest.skip("a device is visible here")
+    with pytest.raises(ea.GpuError) as e:
+        ea.GpuScorer(ea.Panel([b"ACGT" * 10]), ea.Scoring(), ea.Mode.LOCAL)
+    assert e.value.kind == "Unavailable"
+
+
+def test_score_matrix_semantics():
+    """lib/mapper.rs:94-139: rows of unscored queries are None."""
+    m = ea.ScoreMatrix(3, np.arange(9, dtype=np.int16), np.array([1, 0, 1], bool))
+    assert len(m) == 3 and m.n_refs() == 3 and m.n_scored() == 2
+    assert list(m.row(0)) == [0, 1, 2] and m.row(1) is None
+    with pytest.raises(ValueError):
+        ea.ScoreMatrix(4, np.zeros(9, np.int16), np.zeros(3, bool))
