@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Benchmark of the panel-scoring hot path on one MI355X.
+
+Workload (the batched uneven-length configuration): the 164-reference panel
+(uniform ACGT, lengths 140-160), 1M reads with lognormal lengths (median 90,
+sigma 0.6, clipped 20-1000), 90% drawn from the panel on a random strand
+through 5% substitution / 3% insertion / 3% deletion errors, 10% random;
+both strands, semiglobal mode, scoring 1,-1,-2,-1. Synthetic data.
+
+A step is one pass of the device pipeline over the batch, already resident
+in HBM: length bucketing, the batched packed-int16 DP (int32 rescue where
+needed) and the fused best / tie selection. value = reads per second.
+
+Also reported: the DP kernel's time from HIP events on its stream, GCUPS and
+the fraction of the int16 VALU lane-op roofline; the plain one-thread-per-
+pair int32 kernel on the same batch; and the reference's CPU kernel (the
+oracle's AVX-512/AVX2 restatement) on a bounded sample on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+# MI355X: 256 CUs at 2.4 GHz; one packed int16 VALU op does 2 lane-ops in
+# each of 64 lanes, 4 SIMDs x 16 lanes per clock per CU -> 128 lane-ops/clk/CU.
+N_CU, CLOCK_HZ = 256, 2.4e9
+PEAK_I16 = N_CU * CLOCK_HZ * 128           # lane-ops/s
+PEAK_I32 = N_CU * CLOCK_HZ * 64
+OPS_PER_CELL = {"local": 10, "semiglobal": 8}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(panel_codes, codes, off, lens, scoring, local, gpu_check):
+    """The oracle's SIMD restatement of the reference CPU kernel, both
+    strands, threads = host cores (at most 16), on a bounded sample sized
+    to about 10-20 s of CPU work; its scores are checked against the GPU's
+    on the same sample."""
+    from oracle import pyoracle as O
+    import eschalign_amd as ea
+
+    sup = O.simd_supported()
+    lanes = 32 if sup & 2 else (16 if sup & 1 else 0)
+    threads = min(16, os.cpu_count() or 1)
+    if not lanes:
+        return None
+    prof = O.Profile(panel_codes, scoring, lanes)
+
+    def run(n):
+        qs = [codes[int(off[k]):int(off[k]) + int(lens[k])] for k in range(n)]
+        qs += [ea.reverse_complement_codes(q) for q in qs]
+        qc, qo, ql = O.concat(qs)
+        t = time.perf_counter()
+        out = prof.score_batch(qc, qo, ql, local, threads)
+        return time.perf_counter() - t, qs, out
+
+    dt, _, _ = run(2000)
+    n = int(min(len(lens), max(2000, 2000 * 12.0 / max(dt, 1e-3))))
+    dt, qs, out = run(n)
+    cells = sum(len(q) for q in qs) * sum(len(p) for p in panel_codes)
+    gpu_check(qs, out)
+    return {
+        "value": round(n / dt, 1),
+        "unit": "reads/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"first {n} reads of the workload, both strands ({2 * n} queries x "
+                  f"{len(panel_codes)} refs), {'AVX-512BW 32' if lanes == 32 else 'AVX2 16'}-lane "
+                  f"inter-sequence i16 kernel restated from the reference, {threads} threads",
+        "cells_per_s": round(cells / dt, 1),
+        "seconds": round(dt, 2),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--reads", type=int, default=1_000_000)
+    ap.add_argument("--mode", default="semiglobal", choices=["local", "semiglobal"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-kernel comparison")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(f"bench.py measures one GPU only (the scoring path shards by reads with no "
+              f"collective); --gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import eschalign_amd as ea
+    from eschalign_amd import synth
+
+    scoring = (1, -1, -2, -1)
+    mode = ea.Mode.parse(args.mode)
+    local = mode == ea.Mode.LOCAL
+    t0 = time.perf_counter()
+    panel_codes = synth.panel(seed=0x0123456789ABCDEF, n_refs=164, lo=140, hi=160)
+    codes, off, lens = synth.read_arrays(panel_codes, seed=0x5EED_2002, n=args.reads,
+                                         length=("lognormal", 90, 0.6, 20, 1000))
+    log(f"[bench] workload: {args.reads} reads, mean {lens.mean():.1f} nt, panel "
+        f"{len(panel_codes)} refs / {sum(len(p) for p in panel_codes)} bases "
+        f"({time.perf_counter() - t0:.1f} s to generate)")
+
+    panel = ea.Panel(panel_codes)
+    gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+    batch = gpu.batch(args.reads, int(lens.sum()))
+    batch.load_concat(codes, off, lens, both_strands=True)
+    batch.wait()
+    what = ea.RUN_SCORES32 | ea.RUN_SELECT | ea.RUN_NO_GRAPH
+    tie_cap = 8
+
+    for _ in range(args.warmup):
+        batch.run(what, tie_cap=tie_cap)
+        batch.wait()
+    batch.wait()
+    dp_ms, total_ms = [], []
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run(what, tie_cap=tie_cap)
+        batch.wait()  # every step's events are read before the next is queued
+        d, tt = batch.timing()
+        dp_ms.append(d)
+        total_ms.append(tt)
+    batch.wait()
+    wall = time.perf_counter() - t
+    cells, n16, n32 = batch.stats()
+    ms_step = wall / args.steps * 1e3
+    value = args.reads / (wall / args.steps)
+    dp = float(np.mean(dp_ms)) / 1e3
+    gcups = cells / dp / 1e9
+    ops = OPS_PER_CELL[args.mode]
+    achieved = cells * ops / dp
+    log(f"[bench] {value:,.0f} reads/s, {ms_step:.1f} ms/step, DP {dp * 1e3:.1f} ms "
+        f"({gcups:,.0f} GCUPS), {n16} int16 / {n32} int32 queries")
+
+    plain = None
+    if not args.no_plain:
+        batch.run(ea.RUN_SCORES32 | ea.RUN_SELECT | ea.RUN_PLAIN | ea.RUN_NO_GRAPH, tie_cap=tie_cap)
+        batch.wait()
+        t = time.perf_counter()
+        batch.run(ea.RUN_SCORES32 | ea.RUN_SELECT | ea.RUN_PLAIN | ea.RUN_NO_GRAPH, tie_cap=tie_cap)
+        batch.wait()
+        pw = time.perf_counter() - t
+        pdp, _ = batch.timing()
+        plain = {"reads_per_s": round(args.reads / pw, 1),
+                 "dp_ms": round(pdp, 3),
+                 "gcups": round(cells / (pdp / 1e3) / 1e9, 1),
+                 "speedup": round(value / (args.reads / pw), 2)}
+        log(f"[bench] plain int32 kernel: {plain}")
+
+    cpu = None
+    if not args.no_cpu:
+        def gpu_check(qs, cpu_scores):
+            got, ok = gpu.score_batch_i32(qs)
+            assert ok.all() and np.array_equal(got, cpu_scores), "GPU != CPU kernel on the sample"
+        cpu = cpu_baseline(panel_codes, codes, off, lens, scoring, local, gpu_check)
+        log(f"[bench] CPU baseline: {cpu}")
+
+    line = {
+        "metric": "reads/s + GCUPS, bit-exact scores vs CPU; % VALU roofline; p50/p99 latency",
+        "value": round(value, 1),
+        "unit": "reads/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "i16x2 (int32 rescue)",
+        "data": "synthetic",
+        "config": {"workload": "batched uneven lengths: 164-ref panel (140-160 nt), "
+                               f"{args.reads} reads lognormal(median 90, sigma 0.6) clipped 20-1000, "
+                               f"both strands, {args.mode}, scoring 1,-1,-2,-1",
+                   "reads": args.reads, "queries": 2 * args.reads, "n_refs": len(panel),
+                   "parallelism": "single GPU"},
+        "gcups": round(gcups, 1),
+        "dp_ms": round(dp * 1e3, 3),
+        "cells": int(cells),
+        "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3),
+                     "peak": round(PEAK_I16 / 1e12, 3), "unit": "Tlane-op/s (int16)",
+                     "frac": round(achieved / PEAK_I16, 4), "traffic": None,
+                     "ops_per_cell": ops},
+        "plain_kernel": plain,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

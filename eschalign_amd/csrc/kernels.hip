@@ -80,13 +80,22 @@ __device__ __forceinline__ void column16(const v2s (&sc)[kStripe], v2s (&Ho)[kSt
         E[r] = vmax(Ho[r], adds(E[r], ve));
         f = vmax(hou, adds(f, ve));
         v2s h = vmax(vmax(adds(hod, sc[r]), E[r]), f);
-        if constexpr (LOCAL) {
-            h = vmax(h, splat(0));
-            best = vmax(best, h);
-        }
+        if constexpr (LOCAL) h = vmax(h, splat(0));
         hod = Ho[r];
         Ho[r] = adds(h, vo);
         hou = Ho[r];
+    }
+    if constexpr (LOCAL) {
+        // The column's best cell, in Ho = H + o space, as a depth-4 tree
+        // (same 16 max ops as a running max, without a serial chain).
+        v2s t[kStripe / 2];
+#pragma unroll
+        for (int r = 0; r < kStripe / 2; ++r) t[r] = vmax(Ho[2 * r], Ho[2 * r + 1]);
+#pragma unroll
+        for (int w = kStripe / 4; w >= 1; w /= 2)
+#pragma unroll
+            for (int r = 0; r < w; ++r) t[r] = vmax(t[2 * r], t[2 * r + 1]);
+        best = vmax(best, t[0]);
     }
 }
 
@@ -139,7 +148,7 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
             return ((wn[j >> 5] >> (j & 31)) & 1u) ? 4 : (int)c2;
         };
 
-        v2s best = LOCAL ? splat(0) : NEG;  // local: H; semiglobal: Ho = H + o
+        v2s best = LOCAL ? vo : NEG;  // Ho = H + o space (local starts at H = 0)
 
         for (int s = 0; s < S; ++s) {
             // This stripe's profile: wave wv builds rows 4 wv .. 4 wv + 3.
@@ -215,11 +224,7 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
             }
         }
         if (!active) continue;
-        int bA = best.x, bB = best.y;
-        if (!LOCAL) {
-            bA -= a.sc.o;
-            bB -= a.sc.o;
-        }
+        const int bA = best.x - a.sc.o, bB = best.y - a.sc.o;
         if (qa >= 0) {
             const size_t o = (size_t)qa * n_refs + ref;
             if (a.out32) a.out32[o] = bA;
