@@ -189,7 +189,7 @@ esch_status esch_scorer_new(int ordinal, const uint8_t *const *refs, const size_
 
     std::unique_ptr<esch_scorer> s(new esch_scorer);
     s->ordinal = ordinal;
-    s->device_name = prop.name;
+    s->device_name = prop.name[0] ? prop.name : prop.gcnArchName;
     s->n_cu = prop.multiProcessorCount;
     s->sc = Scoring{ma, mi, o, e, mode == ESCH_LOCAL ? 1 : 0};
     s->mode = mode;
