@@ -132,6 +132,12 @@ def main():
 
     panel = ea.Panel(panel_codes)
     gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+    # autotune the batched kernel on a sample of the workload's own reads
+    n_tune = min(args.reads, 100_000)
+    tune_reads = [codes[int(off[k]):int(off[k]) + int(lens[k])] for k in range(n_tune)]
+    chosen, tune_ms = gpu.autotune(tune_reads, both_strands=True)
+    log(f"[bench] autotune on {n_tune} reads: variant DP ms {tune_ms} -> variant {chosen} "
+        f"({gpu.registers()} VGPRs)")
     batch = gpu.batch(args.reads, int(lens.sum()))
     batch.load_concat(codes, off, lens, both_strands=True)
     batch.wait()
@@ -210,6 +216,8 @@ def main():
                      "peak": round(PEAK_I16 / 1e12, 3), "unit": "Tlane-op/s (int16)",
                      "frac": round(achieved / PEAK_I16, 4), "traffic": None,
                      "ops_per_cell": ops},
+        "kernel_variant": chosen,
+        "autotune_ms": [round(x, 3) for x in tune_ms],
         "plain_kernel": plain,
         "cpu_baseline": cpu,
     }

@@ -58,7 +58,8 @@ EXPORTS = [
     "esch_batch_create", "esch_batch_free", "esch_batch_footprint", "esch_batch_load",
     "esch_batch_load_concat", "esch_batch_run", "esch_batch_wait", "esch_batch_timing",
     "esch_batch_stats", "esch_batch_read_scores32", "esch_batch_read_scores16",
-    "esch_batch_read_select", "esch_batch_read_pairs",
+    "esch_batch_read_select", "esch_batch_read_pairs", "esch_n_variants", "esch_get_variant",
+    "esch_set_variant", "esch_autotune",
 ]
 
 
@@ -130,6 +131,10 @@ def load_library():
     L.esch_batch_read_scores16.argtypes = [vp, vp, vp]
     L.esch_batch_read_select.argtypes = [vp, vp, vp]
     L.esch_batch_read_pairs.argtypes = [vp, vp, vp]
+    L.esch_n_variants.restype = C.c_int
+    L.esch_get_variant.argtypes = [vp]
+    L.esch_set_variant.argtypes = [vp, C.c_int]
+    L.esch_autotune.argtypes = [vp, vp, vp, sz, C.c_int, vp, C.POINTER(C.c_int)]
     _lib = L
     return L
 
@@ -462,6 +467,26 @@ class GpuScorer:
         _check(_lib.esch_select_pairs(self._h, p1, l1, p2, l2, n, int(both_strands), min_score,
                                       cap, recs.ctypes.data, top.ctypes.data))
         return [_pair_select(recs[k], top[k * cap:(k + 1) * cap]) for k in range(n)]
+
+    # kernel variants and the autotuner
+    @property
+    def variant(self) -> int:
+        return _lib.esch_get_variant(self._h)
+
+    @variant.setter
+    def variant(self, v: int):
+        _check(_lib.esch_set_variant(self._h, int(v)))
+
+    def autotune(self, reads: Sequence[np.ndarray], both_strands=True):
+        """Time every batched-kernel variant on `reads`, keep the fastest;
+        returns (chosen, [ms per variant])."""
+        keep, ptrs, lens = _ptr_array(reads)
+        nv = _lib.esch_n_variants()
+        ms = np.zeros(nv, np.float32)
+        chosen = C.c_int()
+        _check(_lib.esch_autotune(self._h, ptrs, lens, len(keep), int(both_strands),
+                                  ms.ctypes.data, C.byref(chosen)))
+        return chosen.value, [float(x) for x in ms]
 
     def batch(self, max_reads: int, max_bases: int) -> "DeviceBatch":
         return DeviceBatch(self, max_reads, max_bases)

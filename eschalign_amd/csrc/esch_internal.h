@@ -13,10 +13,9 @@ constexpr int kStripe = 16;
 constexpr uint8_t kPreCode = 5;
 // Reads per wave in the batched kernel: 64 lanes x 2 packed int16 halves.
 constexpr int kChunk = 128;
-// Stripe-count bins of the length bucketing; a longer read goes to the
-// int32 kernel.
-constexpr int kMaxStripes = 4096;
-constexpr int kMaxLen16 = kMaxStripes * kStripe;
+// Longest read of the int16 kernel (length-bucketing bins 0 .. kMaxLen16);
+// a longer read goes to the int32 kernel.
+constexpr int kMaxLen16 = 65536;
 // Largest |value| bound under which the packed int16 recurrence is exact:
 // every real cell, E/F and intermediate sum lies in
 // [-(k*M + |o| + |e|), k*M] with k = min(read, ref) and M = max(|ma|, |mi|),
@@ -73,9 +72,9 @@ struct KArgs {
     const uint64_t *qoff;
     const int32_t *qlen;
     uint8_t *cls;        // 0 unscored, 1 int16 kernel, 2 int32 kernel
-    int32_t *hist;       // [kMaxStripes + 1]
-    int32_t *bin_start;  // [kMaxStripes + 1]
-    int32_t *bin_fill;   // [kMaxStripes + 1]
+    int32_t *hist;       // [kMaxLen16 + 1] reads per length
+    int32_t *bin_start;  // [kMaxLen16 + 1]
+    int32_t *bin_fill;   // [kMaxLen16 + 1]
     int32_t *sorted16;   // int16-kernel queries, longest first
     int32_t *list32;     // int32-kernel queries
     int32_t *out32;      // [n_queries * n_refs] or null
@@ -93,15 +92,17 @@ struct Launch {
     int grid32;   // persistent blocks of k_dp32
     int block32;  // threads per k_dp32 block
     int grid_aux; // blocks of the bucketing / selection kernels
+    int variant16;  // k_dp16 variant: 0 profile loaded per column, 1 prefetched
 };
+constexpr int kDp16Variants = 2;
 hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st);
 hipError_t launch_dp16(const KArgs &a, const Launch &g, bool local, hipStream_t st);
 hipError_t launch_dp32(const KArgs &a, const Launch &g, bool local, int which, hipStream_t st);
 hipError_t launch_select(const KArgs &a, const Launch &g, hipStream_t st);
 hipError_t launch_pairs(const KArgs &a, const Launch &g, hipStream_t st);
-hipError_t dp16_attributes(bool local, hipFuncAttributes *attr);
+hipError_t dp16_attributes(bool local, int variant, hipFuncAttributes *attr);
 hipError_t dp32_attributes(bool local, hipFuncAttributes *attr);
-hipError_t dp16_occupancy(bool local, int *blocks_per_cu);
+hipError_t dp16_occupancy(bool local, int variant, int *blocks_per_cu);
 hipError_t dp32_occupancy(bool local, int block, int *blocks_per_cu);
 
 }  // namespace esch

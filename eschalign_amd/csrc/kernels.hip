@@ -71,12 +71,11 @@ __device__ __forceinline__ uint32_t pack2(int lo, int hi) {
     return (uint32_t)(uint16_t)(int16_t)lo | ((uint32_t)(uint16_t)(int16_t)hi << 16);
 }
 
-template <bool LOCAL>
-__device__ __forceinline__ void column16(const v2s (&sc)[kStripe], v2s (&Ho)[kStripe],
-                                         v2s (&E)[kStripe], v2s &hou, v2s &f, v2s hod, v2s vo,
-                                         v2s ve, v2s &best) {
+template <int RR, bool LOCAL>
+__device__ __forceinline__ void column16(const v2s (&sc)[RR], v2s (&Ho)[RR], v2s (&E)[RR],
+                                         v2s &hou, v2s &f, v2s hod, v2s vo, v2s ve, v2s &best) {
 #pragma unroll
-    for (int r = 0; r < kStripe; ++r) {
+    for (int r = 0; r < RR; ++r) {
         E[r] = vmax(Ho[r], adds(E[r], ve));
         f = vmax(hou, adds(f, ve));
         v2s h = vmax(vmax(adds(hod, sc[r]), E[r]), f);
@@ -88,18 +87,110 @@ __device__ __forceinline__ void column16(const v2s (&sc)[kStripe], v2s (&Ho)[kSt
     if constexpr (LOCAL) {
         // The column's best cell, in Ho = H + o space, as a depth-4 tree
         // (same 16 max ops as a running max, without a serial chain).
-        v2s t[kStripe / 2];
+        v2s t[RR / 2];
 #pragma unroll
-        for (int r = 0; r < kStripe / 2; ++r) t[r] = vmax(Ho[2 * r], Ho[2 * r + 1]);
+        for (int r = 0; r < RR / 2; ++r) t[r] = vmax(Ho[2 * r], Ho[2 * r + 1]);
+        int w = RR / 2;
 #pragma unroll
-        for (int w = kStripe / 4; w >= 1; w /= 2)
+        while (w > 1) {
 #pragma unroll
-            for (int r = 0; r < w; ++r) t[r] = vmax(t[2 * r], t[2 * r + 1]);
+            for (int r = 0; r < w / 2; ++r) t[r] = vmax(t[2 * r], t[2 * r + 1]);
+            if (w & 1) t[0] = vmax(t[0], t[w - 1]);
+            w /= 2;
+        }
         best = vmax(best, t[0]);
     }
 }
 
-template <bool LOCAL>
+// One stripe of RR rows (RR = 16, or 4/8/12 for a chunk's first stripe)
+// against the wave's reference, columns 0 .. m-1. row0 is the stripe's first
+// row in the chunk's padded row numbering, pA / pB the padding rows of the
+// lane's two reads. The loop is unrolled by two with ping-pong registers so
+// that column j + 1's profile plane (LDS) and column j + 2's boundary (HBM)
+// are in flight while column j computes.
+template <int RR, bool LOCAL, bool PF>
+__device__ __forceinline__ void stripe16(const uint4 *__restrict__ prof, uint2 *__restrict__ scr,
+                                         const int lane, const cwords w2, const cwords wn,
+                                         const int m, const bool first, const bool last,
+                                         const int row0, const int pA, const int pB, const v2s vo,
+                                         const v2s ve, v2s &best) {
+    constexpr int NQ = RR / 4;
+    const v2s NEG = splat(-32768);
+    v2s Ho[RR], E[RR];
+#pragma unroll
+    for (int r = 0; r < RR; ++r) {
+        Ho[r] = vo;  // H = 0 on the left border
+        E[r] = NEG;
+    }
+    v2s hdt = vo;  // Ho of the row above at column j - 1 (border: H = 0)
+    auto code = [&](int j) -> int {
+        const uint32_t c2 = (w2[j >> 4] >> (2 * (j & 15))) & 3u;
+        return ((wn[j >> 5] >> (j & 31)) & 1u) ? 4 : (int)c2;
+    };
+    auto plane = [&](int c, v2s (&sc)[RR]) {
+        const uint4 *pc = prof + c * 4 * 64 + lane;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const uint4 v = pc[q * 64];
+            sc[4 * q + 0] = as_v2s(v.x);
+            sc[4 * q + 1] = as_v2s(v.y);
+            sc[4 * q + 2] = as_v2s(v.z);
+            sc[4 * q + 3] = as_v2s(v.w);
+        }
+    };
+    const uint2 border = make_uint2(as_u32(vo), as_u32(NEG));
+    auto bnd = [&](int j) -> uint2 { return (first || j >= m) ? border : scr[(size_t)j * 64 + lane]; };
+    auto col = [&](int j, const v2s (&sc)[RR], uint2 b) {
+        v2s hou = as_v2s(b.x), f = as_v2s(b.y);
+        const v2s hod = hdt;
+        hdt = hou;
+        column16<RR, LOCAL>(sc, Ho, E, hou, f, hod, vo, ve, best);
+        if (!last) scr[(size_t)j * 64 + lane] = make_uint2(as_u32(hou), as_u32(f));
+        if (!LOCAL && last) best = vmax(best, hou);  // last row, Ho space
+    };
+
+    if constexpr (PF) {
+        v2s sa[RR], sb[RR];
+        plane(code(0), sa);
+        uint2 ba = bnd(0), bb = bnd(1);
+        int j = 0;
+        for (; j + 1 < m; j += 2) {
+            plane(code(j + 1), sb);
+            const uint2 bc = bnd(j + 2);
+            col(j, sa, ba);
+            if (j + 2 < m) plane(code(j + 2), sa);
+            const uint2 bd = bnd(j + 3);
+            col(j + 1, sb, bb);
+            ba = bc;
+            bb = bd;
+        }
+        if (j < m) col(j, sa, ba);
+    } else {
+        // Profile plane loaded in the column that uses it (fewer registers,
+        // one more resident wave); the boundary still one column ahead.
+        uint2 bn = bnd(0);
+        for (int j = 0; j < m; ++j) {
+            const uint2 b = bn;
+            bn = bnd(j + 1);
+            v2s sc[RR];
+            plane(code(j), sc);
+            col(j, sc, b);
+        }
+    }
+
+    if (!LOCAL) {
+        // Last column: every real row of this stripe (padding rows out).
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
+            v2s v = Ho[r];
+            if (row0 + r < pA) v.x = (short)-32768;
+            if (row0 + r < pB) v.y = (short)-32768;
+            best = vmax(best, v);
+        }
+    }
+}
+
+template <bool LOCAL, bool PF>
 __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
     // prof[(c * 4 + quad) * 64 + lane] = rows 4 quad .. 4 quad + 3 of plane c
     __shared__ uint4 prof[5 * 4 * 64];
@@ -132,32 +223,32 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
         const int qb = base + 1 < n16 ? a.sorted16[base + 1] : -1;
         const int nA = qa >= 0 ? a.qlen[qa] : 0;
         const int nB = qb >= 0 ? a.qlen[qb] : 0;
-        // Longest first: the chunk's first read sets its stripe count.
+        // Longest first: the chunk's first read sets its row plan, S stripes
+        // of which the first has RR rows (a multiple of 4) and the rest 16.
         const int nmax = uni(a.qlen[a.sorted16[k * kChunk]]);
         const int S = (nmax + kStripe - 1) / kStripe;
-        const int pA = kStripe * S - nA, pB = kStripe * S - nB;
+        const int RR = ((nmax - kStripe * (S - 1) + 3) / 4) * 4;
+        const int T = RR + kStripe * (S - 1);
+        const int pA = T - nA, pB = T - nB;
         const uint8_t *qA = a.qcodes + (qa >= 0 ? a.qoff[qa] : 0);
         const uint8_t *qB = a.qcodes + (qb >= 0 ? a.qoff[qb] : 0);
 
         const int m = active ? uni(P.rlen[ref]) : 0;
         const cwords w2 = (cwords)(P.ref2 + uni((int)P.w2off[ref]));
         const cwords wn = (cwords)(P.refN + uni((int)P.nmoff[ref]));
-        // Reference code of column j: 2-bit code, or 4 when the wildcard bit is set.
-        auto code = [&](int j) -> int {
-            const uint32_t c2 = (w2[j >> 4] >> (2 * (j & 15))) & 3u;
-            return ((wn[j >> 5] >> (j & 31)) & 1u) ? 4 : (int)c2;
-        };
 
         v2s best = LOCAL ? vo : NEG;  // Ho = H + o space (local starts at H = 0)
 
         for (int s = 0; s < S; ++s) {
-            // This stripe's profile: wave wv builds rows 4 wv .. 4 wv + 3.
+            const int row0 = s == 0 ? 0 : RR + kStripe * (s - 1);
+            const int height = s == 0 ? RR : kStripe;
+            // This stripe's profile: wave wv builds rows row0 + 4 wv .. + 3.
             __syncthreads();
-            {
+            if (4 * wv < height) {
                 uint32_t pl[5][4];
 #pragma unroll
                 for (int rr = 0; rr < 4; ++rr) {
-                    const int i = kStripe * s + 4 * wv + rr;
+                    const int i = row0 + 4 * wv + rr;
                     const int xa = (i < pA) ? kPreCode : qA[i - pA];
                     const int xb = (i < pB) ? kPreCode : qB[i - pB];
                     int bestA = -32768, bestB = -32768;
@@ -176,52 +267,19 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
             }
             __syncthreads();
             if (!active) continue;
-
-            v2s Ho[kStripe], E[kStripe];
-#pragma unroll
-            for (int r = 0; r < kStripe; ++r) {
-                Ho[r] = vo;  // H = 0 on the left border
-                E[r] = NEG;
-            }
-            v2s hdt = vo;  // Ho of the row above, column j - 1 (border: H = 0)
             const bool first = s == 0, last = s + 1 == S;
-
-            // The stripe boundary of column j + 1 is fetched while column j
-            // computes.
-            uint2 bnext = first ? make_uint2(0u, 0u) : scr[lane];
-            for (int j = 0; j < m; ++j) {
-                const uint2 b = bnext;
-                if (!first && j + 1 < m) bnext = scr[(size_t)(j + 1) * 64 + lane];
-                const int c = code(j);
-                const uint4 *pc = prof + c * 4 * 64 + lane;
-                v2s sc[kStripe];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const uint4 v = pc[q * 64];
-                    sc[4 * q + 0] = as_v2s(v.x);
-                    sc[4 * q + 1] = as_v2s(v.y);
-                    sc[4 * q + 2] = as_v2s(v.z);
-                    sc[4 * q + 3] = as_v2s(v.w);
-                }
-                v2s hou = first ? vo : as_v2s(b.x);
-                v2s f = first ? NEG : as_v2s(b.y);
-                const v2s hod = hdt;
-                hdt = hou;
-                column16<LOCAL>(sc, Ho, E, hou, f, hod, vo, ve, best);
-                if (!last) scr[(size_t)j * 64 + lane] = make_uint2(as_u32(hou), as_u32(f));
-                if (!LOCAL && last) best = vmax(best, hou);  // last row, Ho space
+            if (s > 0) {
+                stripe16<16, LOCAL, PF>(prof, scr, lane, w2, wn, m, false, last, row0, pA, pB, vo, ve, best);
+            } else if (RR == 16) {
+                stripe16<16, LOCAL, PF>(prof, scr, lane, w2, wn, m, true, last, row0, pA, pB, vo, ve, best);
+            } else if (RR == 12) {
+                stripe16<12, LOCAL, PF>(prof, scr, lane, w2, wn, m, true, last, row0, pA, pB, vo, ve, best);
+            } else if (RR == 8) {
+                stripe16<8, LOCAL, PF>(prof, scr, lane, w2, wn, m, true, last, row0, pA, pB, vo, ve, best);
+            } else {
+                stripe16<4, LOCAL, PF>(prof, scr, lane, w2, wn, m, true, last, row0, pA, pB, vo, ve, best);
             }
-            if (!LOCAL) {
-                // Last column: every real row of this stripe (padding rows out).
-#pragma unroll
-                for (int r = 0; r < kStripe; ++r) {
-                    const int i = kStripe * s + r;
-                    v2s v = Ho[r];
-                    if (i < pA) v.x = (short)-32768;
-                    if (i < pB) v.y = (short)-32768;
-                    best = vmax(best, v);
-                }
-            }
+            (void)first;
         }
         if (!active) continue;
         const int bA = best.x - a.sc.o, bB = best.y - a.sc.o;
@@ -364,7 +422,7 @@ __global__ __launch_bounds__(256) void k_dp32(KArgs a, int which) {
 
 __global__ void k_zero(KArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i <= kMaxStripes) {
+    if (i <= kMaxLen16) {
         a.hist[i] = 0;
         a.bin_fill[i] = 0;
     }
@@ -425,7 +483,7 @@ __global__ void k_classify(KArgs a) {
         }
         a.cls[q] = c;
         if (c == 1) {
-            atomicAdd(&a.hist[(n + kStripe - 1) / kStripe], 1);
+            atomicAdd(&a.hist[n], 1);
         } else if (c == 2) {
             const int pos = atomicAdd(&a.desc->n32, 1);
             a.list32[pos] = q;
@@ -445,21 +503,18 @@ __global__ void k_classify(KArgs a) {
                                                     (unsigned long long)(cells * (uint64_t)a.panel.total_len));
 }
 
-// Exclusive prefix over the stripe bins in DESCENDING stripe count, one
-// block of 1024 threads.
+// Exclusive prefix over the length bins in DESCENDING length, one block of
+// 1024 threads: per-thread range sums, a block scan of the sums, then each
+// thread writes its range.
 __global__ __launch_bounds__(1024) void k_scan(KArgs a) {
     __shared__ int32_t part[1024];
-    constexpr int NB = kMaxStripes + 1;
+    constexpr int NB = kMaxLen16 + 1;
     constexpr int PER = (NB + 1023) / 1024;
     const int t = threadIdx.x;
-    int32_t local[PER];
     int32_t sum = 0;
-#pragma unroll
     for (int k = 0; k < PER; ++k) {
-        const int rb = t * PER + k;          // rank in descending order
-        const int b = kMaxStripes - rb;      // bin
-        local[k] = (rb < NB) ? a.hist[b] : 0;
-        sum += local[k];
+        const int rb = t * PER + k;  // rank in descending order
+        if (rb < NB) sum += a.hist[kMaxLen16 - rb];
     }
     part[t] = sum;
     __syncthreads();
@@ -470,11 +525,13 @@ __global__ __launch_bounds__(1024) void k_scan(KArgs a) {
         __syncthreads();
     }
     int32_t run = part[t] - sum;
-#pragma unroll
     for (int k = 0; k < PER; ++k) {
         const int rb = t * PER + k;
-        if (rb < NB) a.bin_start[kMaxStripes - rb] = run;
-        run += local[k];
+        if (rb < NB) {
+            const int b = kMaxLen16 - rb;
+            a.bin_start[b] = run;
+            run += a.hist[b];
+        }
     }
     if (t == 1023) a.desc->n16 = part[1023];
 }
@@ -483,7 +540,7 @@ __global__ void k_scatter(KArgs a) {
     const int nq = a.desc->n_queries;
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
         if (a.cls[q] != 1) continue;
-        const int b = (a.qlen[q] + kStripe - 1) / kStripe;
+        const int b = a.qlen[q];
         const int pos = a.bin_start[b] + atomicAdd(&a.bin_fill[b], 1);
         a.sorted16[pos] = q;
     }
@@ -654,7 +711,7 @@ __global__ __launch_bounds__(256) void k_pairs(KArgs a) {
 // ---------------------------------------------------------------------------
 
 hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st) {
-    hipLaunchKernelGGL(k_zero, dim3((kMaxStripes + 1 + 255) / 256), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_zero, dim3((kMaxLen16 + 1 + 255) / 256), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_revcomp, dim3(g.grid_aux), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_classify, dim3(g.grid_aux), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, a);
@@ -662,9 +719,22 @@ hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st) {
     return hipGetLastError();
 }
 
+// Kernel variants of k_dp16, chosen per scorer (set by the autotuner).
+template <bool LOCAL>
+static const void *dp16_fn(int variant) {
+    return variant == 1 ? reinterpret_cast<const void *>(&k_dp16<LOCAL, true>)
+                        : reinterpret_cast<const void *>(&k_dp16<LOCAL, false>);
+}
+
 hipError_t launch_dp16(const KArgs &a, const Launch &g, bool local, hipStream_t st) {
-    if (local) hipLaunchKernelGGL(k_dp16<true>, dim3(g.grid16), dim3(64 * kWaves16), 0, st, a);
-    else hipLaunchKernelGGL(k_dp16<false>, dim3(g.grid16), dim3(64 * kWaves16), 0, st, a);
+    const int v = g.variant16;
+    if (local) {
+        if (v == 1) hipLaunchKernelGGL((k_dp16<true, true>), dim3(g.grid16), dim3(64 * kWaves16), 0, st, a);
+        else hipLaunchKernelGGL((k_dp16<true, false>), dim3(g.grid16), dim3(64 * kWaves16), 0, st, a);
+    } else {
+        if (v == 1) hipLaunchKernelGGL((k_dp16<false, true>), dim3(g.grid16), dim3(64 * kWaves16), 0, st, a);
+        else hipLaunchKernelGGL((k_dp16<false, false>), dim3(g.grid16), dim3(64 * kWaves16), 0, st, a);
+    }
     return hipGetLastError();
 }
 
@@ -684,9 +754,8 @@ hipError_t launch_pairs(const KArgs &a, const Launch &g, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t dp16_attributes(bool local, hipFuncAttributes *attr) {
-    return local ? hipFuncGetAttributes(attr, reinterpret_cast<const void *>(&k_dp16<true>))
-                 : hipFuncGetAttributes(attr, reinterpret_cast<const void *>(&k_dp16<false>));
+hipError_t dp16_attributes(bool local, int variant, hipFuncAttributes *attr) {
+    return hipFuncGetAttributes(attr, local ? dp16_fn<true>(variant) : dp16_fn<false>(variant));
 }
 
 hipError_t dp32_attributes(bool local, hipFuncAttributes *attr) {
@@ -694,9 +763,9 @@ hipError_t dp32_attributes(bool local, hipFuncAttributes *attr) {
                  : hipFuncGetAttributes(attr, reinterpret_cast<const void *>(&k_dp32<false>));
 }
 
-hipError_t dp16_occupancy(bool local, int *blocks) {
-    return local ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_dp16<true>, 64 * kWaves16, 0)
-                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_dp16<false>, 64 * kWaves16, 0);
+hipError_t dp16_occupancy(bool local, int variant, int *blocks) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        blocks, local ? dp16_fn<true>(variant) : dp16_fn<false>(variant), 64 * kWaves16, 0);
 }
 
 hipError_t dp32_occupancy(bool local, int block, int *blocks) {

@@ -80,6 +80,9 @@ struct esch_scorer {
     int32_t *d_rlen = nullptr, *d_rorder = nullptr;
     DevPanel panel{};
     Launch geo{};
+    int grid16_v[kDp16Variants] = {};   // persistent grid of each k_dp16 variant
+    int regs16_v[kDp16Variants] = {}, local16_v[kDp16Variants] = {};
+    int max_grid16 = 0;                 // scratch is sized for the largest
     int regs16 = 0, local16 = 0;
     std::mutex pool_mu;
     std::vector<esch_batch *> pool;  // idle batches for the drop-in API
@@ -237,17 +240,25 @@ esch_status esch_scorer_new(int ordinal, const uint8_t *const *refs, const size_
 
     // Launch geometry: persistent grids sized by occupancy.
     const bool local = mode == ESCH_LOCAL;
-    hipFuncAttributes attr;
-    hipError_t ae = dp16_attributes(local, &attr);
-    if (ae != hipSuccess)
-        return fail(ESCH_CODE_OBJECT, "no gfx950 code object for the score kernel: %s", hipGetErrorString(ae));
-    s->regs16 = attr.numRegs;
-    s->local16 = (int)attr.localSizeBytes;
-    int b16 = 0, b32 = 0;
-    HIPC(dp16_occupancy(local, &b16));
+    for (int v = 0; v < kDp16Variants; ++v) {
+        hipFuncAttributes attr;
+        hipError_t ae = dp16_attributes(local, v, &attr);
+        if (ae != hipSuccess)
+            return fail(ESCH_CODE_OBJECT, "no gfx950 code object for the score kernel: %s", hipGetErrorString(ae));
+        s->regs16_v[v] = attr.numRegs;
+        s->local16_v[v] = (int)attr.localSizeBytes;
+        int b16 = 0;
+        HIPC(dp16_occupancy(local, v, &b16));
+        s->grid16_v[v] = std::max(1, b16) * s->n_cu;
+        s->max_grid16 = std::max(s->max_grid16, s->grid16_v[v]);
+    }
+    int b32 = 0;
     int block32 = (int)std::min<size_t>(256, ((n_refs + 63) / 64) * 64);
     HIPC(dp32_occupancy(local, block32, &b32));
-    s->geo.grid16 = std::max(1, b16) * s->n_cu;
+    s->geo.variant16 = 0;
+    s->geo.grid16 = s->grid16_v[0];
+    s->regs16 = s->regs16_v[0];
+    s->local16 = s->local16_v[0];
     s->geo.block32 = block32;
     s->geo.grid32 = std::max(1, b32) * s->n_cu;
     s->geo.grid_aux = 4 * s->n_cu;
@@ -301,9 +312,9 @@ size_t esch_batch_footprint(const esch_scorer *s, size_t max_reads, size_t max_b
     if (!s) return 0;
     const size_t rows = 2 * std::max<size_t>(max_reads, 1);
     const size_t bases = 2 * std::max<size_t>(max_bases, 1);
-    size_t f = bases + rows * (8 + 4 + 1 + 4 + 4) + 3 * 4 * (kMaxStripes + 1) + sizeof(DevDesc);
+    size_t f = bases + rows * (8 + 4 + 1 + 4 + 4) + 3 * 4 * (kMaxLen16 + 1) + sizeof(DevDesc);
     f += rows * s->n_refs * (4 + 2);                                     // out32 + out16
-    f += (size_t)s->geo.grid16 * 4 * s->max_len * 64 * sizeof(uint2);        // scratch16
+    f += (size_t)s->max_grid16 * 4 * s->max_len * 64 * sizeof(uint2);        // scratch16
     f += (size_t)s->geo.grid32 * s->geo.block32 * s->max_len * sizeof(int2);  // scratch32
     f += rows * 5 * 4;                                                   // records
     return f;
@@ -327,14 +338,14 @@ esch_status esch_batch_create(esch_scorer *s, size_t max_reads, size_t max_bases
     HIPC(hipMalloc(&b->d_qoff, rows * 8));
     HIPC(hipMalloc(&b->d_qlen, rows * 4));
     HIPC(hipMalloc(&b->d_cls, rows));
-    HIPC(hipMalloc(&b->d_hist, 4 * (kMaxStripes + 1)));
-    HIPC(hipMalloc(&b->d_bin_start, 4 * (kMaxStripes + 1)));
-    HIPC(hipMalloc(&b->d_bin_fill, 4 * (kMaxStripes + 1)));
+    HIPC(hipMalloc(&b->d_hist, 4 * (kMaxLen16 + 1)));
+    HIPC(hipMalloc(&b->d_bin_start, 4 * (kMaxLen16 + 1)));
+    HIPC(hipMalloc(&b->d_bin_fill, 4 * (kMaxLen16 + 1)));
     HIPC(hipMalloc(&b->d_sorted16, rows * 4));
     HIPC(hipMalloc(&b->d_list32, rows * 4));
     HIPC(hipMalloc(&b->d_out32, rows * s->n_refs * 4));
     HIPC(hipMalloc(&b->d_out16, rows * s->n_refs * 2));
-    HIPC(hipMalloc(&b->d_scr16, (size_t)s->geo.grid16 * 4 * s->max_len * 64 * sizeof(uint2)));
+    HIPC(hipMalloc(&b->d_scr16, (size_t)s->max_grid16 * 4 * s->max_len * 64 * sizeof(uint2)));
     HIPC(hipMalloc(&b->d_scr32, (size_t)s->geo.grid32 * s->geo.block32 * s->max_len * sizeof(int2)));
     HIPC(hipMalloc(&b->d_recs, rows * 5 * 4));
     HIPC(hipMalloc(&b->d_desc, sizeof(DevDesc)));
@@ -531,8 +542,9 @@ esch_status esch_batch_run(esch_batch *b, int what, int32_t min_score, int64_t m
     HIPC(hipMemcpyAsync(b->d_desc, b->h_desc, sizeof(DevDesc), hipMemcpyHostToDevice, b->stream));
     b->last_what = what;
     const bool graph = !(what & ESCH_RUN_NO_GRAPH);
-    // the template-strand count is read from the descriptor, not baked in
-    const int key = what & ~(ESCH_RUN_NO_GRAPH | ESCH_RUN_PAIR_BOTH);
+    // the template-strand count is read from the descriptor, not baked in;
+    // the kernel variant is baked into the captured launches
+    const int key = (what & ~(ESCH_RUN_NO_GRAPH | ESCH_RUN_PAIR_BOTH)) | (s->geo.variant16 << 16);
     HIPC(hipEventRecord(b->ev0, b->stream));
     if (!graph) {
         b->timed_dp = true;
@@ -759,6 +771,51 @@ esch_status esch_score_batch(esch_scorer *s, const uint8_t *const *queries, cons
     esch_pending *pend = nullptr;
     if (esch_status st = esch_submit(s, p, &pend)) return st;
     return esch_finish(s, pend, scores, scored);
+}
+
+esch_status esch_set_variant(esch_scorer *s, int variant) {
+    if (!s || variant < 0 || variant >= kDp16Variants)
+        return fail(ESCH_INVALID_ARG, "variant %d outside 0..%d", variant, kDp16Variants - 1);
+    s->geo.variant16 = variant;
+    s->geo.grid16 = s->grid16_v[variant];
+    s->regs16 = s->regs16_v[variant];
+    s->local16 = s->local16_v[variant];
+    return ESCH_OK;
+}
+
+int esch_get_variant(const esch_scorer *s) { return s ? s->geo.variant16 : -1; }
+int esch_n_variants(void) { return kDp16Variants; }
+
+esch_status esch_autotune(esch_scorer *s, const uint8_t *const *reads, const size_t *lens, size_t n,
+                          int both, float *ms, int *chosen) {
+    if (!s || (n && (!reads || !lens))) return fail(ESCH_INVALID_ARG, "null argument");
+    size_t total = 0;
+    for (size_t k = 0; k < n; ++k) total += lens[k];
+    esch_batch *b = nullptr;
+    if (esch_status st = take_batch(s, n, total, &b)) return st;
+    esch_status st = esch_batch_load(b, reads, lens, n, both, 0);
+    int best_v = s->geo.variant16;
+    float best_ms = 0.0f;
+    for (int v = 0; !st && v < kDp16Variants; ++v) {
+        st = esch_set_variant(s, v);
+        // one untimed pass, then the timed one
+        for (int rep = 0; !st && rep < 2; ++rep) {
+            st = esch_batch_run(b, ESCH_RUN_SCORES32 | ESCH_RUN_NO_GRAPH, 0, -1, 1);
+            if (!st) st = esch_batch_wait(b);
+        }
+        float dp = 0.0f;
+        if (!st) st = esch_batch_timing(b, &dp, nullptr);
+        if (st) break;
+        if (ms) ms[v] = dp;
+        if (v == 0 || dp < best_ms) {
+            best_ms = dp;
+            best_v = v;
+        }
+    }
+    give_batch(s, b);
+    if (st) return st;
+    if (chosen) *chosen = best_v;
+    return esch_set_variant(s, best_v);
 }
 
 esch_status esch_score_batch_i32(esch_scorer *s, const uint8_t *const *queries, const size_t *lens, size_t n,

@@ -172,6 +172,18 @@ esch_status esch_select_pairs(esch_scorer *s, const uint8_t *const *mates1, cons
                               int both_strands, int32_t min_score, size_t top_cap,
                               esch_pair_select *out, int32_t *top);
 
+/* Kernel variants of the batched DP (register tiling / software pipelining
+ * of the profile loads), and the autotuner that picks one by timing each
+ * on a sample of the caller's reads (no counterpart in the reference; the
+ * scores are identical whichever variant runs). */
+int esch_n_variants(void);
+int esch_get_variant(const esch_scorer *s);
+esch_status esch_set_variant(esch_scorer *s, int variant);
+/* ms[v] (may be NULL, esch_n_variants() entries) receives each variant's DP
+ * time on the sample; the fastest is selected and returned in *chosen. */
+esch_status esch_autotune(esch_scorer *s, const uint8_t *const *reads, const size_t *lens, size_t n,
+                          int both_strands, float *ms, int *chosen);
+
 /* ---------------------------------------------------------------------
  * Device-resident batches: inputs uploaded once, every kernel reading its
  * sizes from device memory, the launch sequence captured in a hipGraph and

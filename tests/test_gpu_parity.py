@@ -306,3 +306,25 @@ def test_result_is_the_same_alone_and_in_a_mix():
     for k in (0, 7, 55, 199, len(reads) - 1):
         alone, _ = gpu.score_batch_i32([reads[k]])
         assert np.array_equal(alone[0], s_all[k])
+
+
+def test_every_kernel_variant_equals_oracle_and_autotune_picks_one():
+    """Every batched-kernel variant (the autotuner's choices) gives the
+    oracle's scores: both modes, lengths 1..900 (first-stripe heights
+    4/8/12/16, several stripes), a panel with wildcards."""
+    rng = G.Rng(0x5EED_0A01)
+    panel_l = G.random_panel(rng, 13, 40, 300)
+    panel = ea.Panel(panel_l)
+    reads = [G.random_seq(rng, 1 + rng.below(900), 0.03) for _ in range(300)]
+    reads += [G.random_seq(rng, n, 0.0) for n in range(1, 34)]
+    queries = codes_of(reads)
+    for scoring in ((2, -1, -10, -1), (3, 1, -5, -2)):
+        for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
+            gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+            want = oracle(queries, panel.codes, scoring, mode)
+            for v in range(ea.load_library().esch_n_variants()):
+                gpu.variant = v
+                s32, ok = gpu.score_batch_i32(queries)
+                assert ok.all() and np.array_equal(s32, want), (scoring, mode, v)
+            chosen, ms = gpu.autotune(queries[:120])
+            assert gpu.variant == chosen and all(t > 0 for t in ms)
