@@ -59,7 +59,7 @@ EXPORTS = [
     "esch_batch_load_concat", "esch_batch_run", "esch_batch_wait", "esch_batch_timing",
     "esch_batch_stats", "esch_batch_read_scores32", "esch_batch_read_scores16",
     "esch_batch_read_select", "esch_batch_read_pairs", "esch_n_variants", "esch_get_variant",
-    "esch_set_variant", "esch_autotune",
+    "esch_set_variant", "esch_autotune", "esch_adapter_find", "esch_adapter_detect_pair",
 ]
 
 
@@ -135,6 +135,11 @@ def load_library():
     L.esch_get_variant.argtypes = [vp]
     L.esch_set_variant.argtypes = [vp, C.c_int]
     L.esch_autotune.argtypes = [vp, vp, vp, sz, C.c_int, vp, C.POINTER(C.c_int)]
+    L.esch_adapter_find.argtypes = [C.c_char_p, sz, C.c_char_p, sz, sz]
+    L.esch_adapter_find.restype = sz
+    L.esch_adapter_detect_pair.argtypes = [C.c_char_p, sz, C.c_char_p, sz, C.c_char_p, sz,
+                                           C.c_char_p, sz, sz, C.POINTER(sz * 2)]
+    L.esch_adapter_detect_pair.restype = None
     _lib = L
     return L
 
@@ -276,6 +281,29 @@ class Panel:
 
     def total_len(self) -> int:
         return int(sum(len(c) for c in self.codes))
+
+
+# ---------------------------------------------------------------- adapters
+
+TRUSEQ_R1 = b"AGATCGGAAGAGCACACGTCTGAACTCCAGTCA"  # lib/adapter.rs:40
+TRUSEQ_R2 = b"AGATCGGAAGAGCGTCGTGTAGGGAAAGAGTGT"  # lib/adapter.rs:42
+DEFAULT_MIN_OVERLAP = 5                            # lib/adapter.rs:44
+
+
+def adapter_find(read: bytes, adapter: bytes = TRUSEQ_R1, min_overlap: int = DEFAULT_MIN_OVERLAP) -> int:
+    """adapter::find (lib/adapter.rs:68-91), letters in."""
+    return int(load_library().esch_adapter_find(bytes(read), len(read), bytes(adapter),
+                                                 len(adapter), min_overlap))
+
+
+def adapter_detect_pair(r1: bytes, r2: bytes, a1: bytes = TRUSEQ_R1, a2: bytes = TRUSEQ_R2,
+                        min_overlap: int = DEFAULT_MIN_OVERLAP):
+    """adapter::detect_pair (lib/adapter.rs:95-110): [ad1, ad2]."""
+    out = (C.c_size_t * 2)()
+    load_library().esch_adapter_detect_pair(bytes(r1), len(r1), bytes(r2), len(r2), bytes(a1),
+                                            len(a1), bytes(a2), len(a2), min_overlap,
+                                            C.byref(out))
+    return [int(out[0]), int(out[1])]
 
 
 # ---------------------------------------------------------------- score matrix

@@ -172,6 +172,16 @@ esch_status esch_select_pairs(esch_scorer *s, const uint8_t *const *mates1, cons
                               int both_strands, int32_t min_score, size_t top_cap,
                               esch_pair_select *out, int32_t *top);
 
+/* adapter::find — lib/adapter.rs:68-91. Letters (not codes): how many bases
+ * at the 3' end of `read` are the adapter's prefix (0 = none). The pre-step
+ * that trims mates before scoring (crates/eschalign-cli/src/align.rs:2844-2862). */
+size_t esch_adapter_find(const uint8_t *read, size_t n, const uint8_t *adapter, size_t alen,
+                         size_t min_overlap);
+/* adapter::detect_pair — lib/adapter.rs:95-110: out = [ad1, ad2]. */
+void esch_adapter_detect_pair(const uint8_t *r1, size_t l1, const uint8_t *r2, size_t l2,
+                              const uint8_t *a1, size_t la1, const uint8_t *a2, size_t la2,
+                              size_t min_overlap, size_t out[2]);
+
 /* Kernel variants of the batched DP (register tiling / software pipelining
  * of the profile loads), and the autotuner that picks one by timing each
  * on a sample of the caller's reads (no counterpart in the reference; the

@@ -101,3 +101,55 @@ def test_score_matrix_semantics():
     assert list(m.row(0)) == [0, 1, 2] and m.row(1) is None
     with pytest.raises(ValueError):
         ea.ScoreMatrix(4, np.zeros(9, np.int16), np.zeros(3, bool))
+
+
+R1 = ea.TRUSEQ_R1
+R2 = ea.TRUSEQ_R2
+
+
+def test_adapter_find_known_answers():
+    """lib/adapter.rs:116-164 through the C ABI."""
+    f = ea.adapter_find
+    assert f(b"ACGTACGTAC" + R1[:20], R1, 5) == 20
+    assert f(b"ACGTACGTAC" + R1[:6], R1, 5) == 6
+    assert f(b"ACGTACGTAC" + R1 + b"TTTT", R1, 5) == len(R1) + 4
+    a = bytearray(R1[:20])
+    a[5] = ord("T")
+    a[12] = ord("T")
+    assert f(b"G" * 12 + bytes(a), R1, 5) == 20
+    a[15] = ord("C")
+    assert f(b"G" * 12 + bytes(a), R1, 5) == 0
+    assert f(b"G" * 12 + b"AGATCGGAT", R1, 5) == 0
+    assert f(b"G" * 12 + b"AGAT", R1, 5) == 0
+    assert f(b"G" * 12 + b"AGAT", R1, 4) == 4
+    assert f(b"AG", R1, 5) == 0 and f(b"", R1, 5) == 0
+    assert f(b"G" * 12 + b"AGATNGGAAGAG", R1, 5) == 12
+
+
+def test_adapter_detect_pair_known_answers():
+    """lib/adapter.rs:166-197."""
+    insert = b"GGGGCCCCGGGGCCCCGGGG"
+    a, b = insert + R1[:12], insert + R2[:12]
+    shifted = lambda ad, extra: insert + b"CC" + ad[:extra]
+    d = ea.adapter_detect_pair
+    assert d(a, b) == [12, 12]
+    assert d(a, shifted(R2, 9)) == [12, 0]
+    assert d(shifted(R1, 7), b) == [0, 12]
+    assert d(insert + R1[:8], shifted(R2, 8)) == [0, 0]
+    assert d(a, insert) == [12, 0]
+
+
+def test_adapter_equals_oracle_random():
+    rng = np.random.default_rng(5)
+    for _ in range(2000):
+        n = int(rng.integers(0, 60))
+        read = bytes(rng.choice(list(b"ACGTN"), n, p=[0.24, 0.24, 0.24, 0.24, 0.04]))
+        if n and rng.random() < 0.7:
+            k = int(rng.integers(0, min(n, len(R1)) + 1))
+            tail = bytearray(R1[:k])
+            for i in range(len(tail)):
+                if rng.random() < 0.08:
+                    tail[i] = ord("ACGT"[int(rng.integers(0, 4))])
+            read = read[: n - k] + bytes(tail)
+        mo = int(rng.integers(1, 8))
+        assert ea.adapter_find(read, R1, mo) == O.adapter_find(read, R1, mo)
