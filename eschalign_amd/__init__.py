@@ -37,12 +37,12 @@ MAX_REF_LEN = 16384
 
 # esch_status
 OK, UNAVAILABLE, DRIVER, CODE_OBJECT, REFERENCE_TOO_LONG, INVALID_ARG = 0, 1, 2, 3, 4, 5
-SCORING_INVALID, EMPTY_PANEL, EMPTY_REFERENCE, CAPACITY = 6, 7, 8, 9
+SCORING_INVALID, EMPTY_PANEL, EMPTY_REFERENCE, CAPACITY, TIMEOUT = 6, 7, 8, 9, 10
 _STATUS = {
     UNAVAILABLE: "Unavailable", DRIVER: "Driver", CODE_OBJECT: "Compile",
     REFERENCE_TOO_LONG: "ReferenceTooLong", INVALID_ARG: "InvalidArg",
     SCORING_INVALID: "ScoringInvalid", EMPTY_PANEL: "EmptyPanel",
-    EMPTY_REFERENCE: "EmptyReference", CAPACITY: "Capacity",
+    EMPTY_REFERENCE: "EmptyReference", CAPACITY: "Capacity", TIMEOUT: "Timeout",
 }
 
 RUN_SCORES16, RUN_SCORES32, RUN_SELECT, RUN_PAIRS, RUN_PLAIN = 1, 2, 4, 8, 16
@@ -60,7 +60,22 @@ EXPORTS = [
     "esch_batch_stats", "esch_batch_read_scores32", "esch_batch_read_scores16",
     "esch_batch_read_select", "esch_batch_read_pairs", "esch_n_variants", "esch_get_variant",
     "esch_set_variant", "esch_autotune", "esch_adapter_find", "esch_adapter_detect_pair",
+    "esch_batch_ready", "esch_server_new", "esch_server_free", "esch_server_add_panel",
+    "esch_server_scorer", "esch_server_submit", "esch_server_wait", "esch_server_result",
+    "esch_server_stats",
 ]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("ordinal", C.c_int), ("hbm_budget", C.c_size_t),
+                ("max_batch_reads", C.c_size_t), ("max_batch_bases", C.c_size_t),
+                ("max_slots", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("completed", C.c_uint64), ("batches", C.c_uint64),
+                ("queued_units", C.c_size_t), ("hbm_in_use", C.c_size_t),
+                ("hbm_peak", C.c_size_t), ("slots", C.c_size_t), ("busy_slots", C.c_size_t)]
 
 
 class GpuError(Exception):
@@ -140,6 +155,19 @@ def load_library():
     L.esch_adapter_detect_pair.argtypes = [C.c_char_p, sz, C.c_char_p, sz, C.c_char_p, sz,
                                            C.c_char_p, sz, sz, C.POINTER(sz * 2)]
     L.esch_adapter_detect_pair.restype = None
+    L.esch_batch_ready.argtypes = [vp]
+    L.esch_server_new.argtypes = [C.POINTER(ServerConfig), C.POINTER(vp)]
+    L.esch_server_free.argtypes = [vp]
+    L.esch_server_free.restype = None
+    L.esch_server_add_panel.argtypes = [vp, vp, vp, sz, i32, i32, i32, i32, C.c_int,
+                                        C.POINTER(C.c_int)]
+    L.esch_server_scorer.argtypes = [vp, C.c_int]
+    L.esch_server_scorer.restype = vp
+    L.esch_server_submit.argtypes = [vp, C.c_int, vp, vp, sz, C.c_int, i32, i64, sz,
+                                     C.POINTER(C.c_uint64)]
+    L.esch_server_wait.argtypes = [vp, C.c_uint64, i64]
+    L.esch_server_result.argtypes = [vp, C.c_uint64, vp, vp, C.POINTER(C.c_double)]
+    L.esch_server_stats.argtypes = [vp, C.POINTER(ServerStats)]
     _lib = L
     return L
 
@@ -632,3 +660,75 @@ class DeviceBatch:
         top = np.zeros(max(n, 1) * cap, np.int32)
         _check(_lib.esch_batch_read_pairs(self._h, recs.ctypes.data, top.ctypes.data))
         return [_pair_select(recs[k], top[k * cap:(k + 1) * cap]) for k in range(n)]
+
+
+class Server:
+    """Continuous-batching request loop over device-resident panels
+    (include/eschalign_amd.h, esch_server_*)."""
+
+    def __init__(self, hbm_budget=32 << 30, max_batch_reads=50_000, max_batch_bases=8_000_000,
+                 max_slots=4, ordinal=0):
+        L = load_library()
+        cfg = ServerConfig(ordinal, hbm_budget, max_batch_reads, max_batch_bases, max_slots)
+        out = C.c_void_p()
+        _check(L.esch_server_new(C.byref(cfg), C.byref(out)))
+        self._h = out
+        self._caps = {}
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.esch_server_free(h)
+            self._h = None
+
+    __del__ = close
+
+    def add_panel(self, panel: Panel, scoring: Scoring = Scoring(), mode: int = Mode.LOCAL) -> int:
+        keep, ptrs, lens = _ptr_array(panel.codes)
+        pid = C.c_int()
+        _check(_lib.esch_server_add_panel(self._h, ptrs, lens, len(keep), *scoring.tuple(), mode,
+                                          C.byref(pid)))
+        self._caps.setdefault("n_refs", {})[pid.value] = len(panel)
+        return pid.value
+
+    def autotune(self, panel_id: int, reads, both_strands=True):
+        h = _lib.esch_server_scorer(self._h, panel_id)
+        keep, ptrs, lens = _ptr_array(reads)
+        nv = _lib.esch_n_variants()
+        ms = np.zeros(nv, np.float32)
+        chosen = C.c_int()
+        _check(_lib.esch_autotune(h, ptrs, lens, len(keep), int(both_strands), ms.ctypes.data,
+                                  C.byref(chosen)))
+        return chosen.value, [float(x) for x in ms]
+
+    def submit(self, panel_id: int, reads, both_strands=True, min_score=0, max_ties=None,
+               tie_cap=8) -> int:
+        keep, ptrs, lens = _ptr_array(reads)
+        rid = C.c_uint64()
+        mt = -1 if max_ties is None else int(max_ties)
+        _check(_lib.esch_server_submit(self._h, panel_id, ptrs, lens, len(keep), int(both_strands),
+                                       min_score, mt, tie_cap, C.byref(rid)))
+        self._caps[rid.value] = (len(keep), tie_cap)
+        return rid.value
+
+    def wait(self, rid: int, timeout_ms: int = -1) -> bool:
+        st = _lib.esch_server_wait(self._h, rid, timeout_ms)
+        if st == TIMEOUT:
+            return False
+        _check(st)
+        return True
+
+    def result(self, rid: int):
+        """(list of ReadSelect, latency in seconds) of a finished request."""
+        n, cap = self._caps.pop(rid)
+        recs = np.zeros((max(n, 1), 5), np.int32)
+        ties = np.zeros(max(n, 1) * cap, np.int32)
+        lat = C.c_double()
+        _check(_lib.esch_server_result(self._h, rid, recs.ctypes.data, ties.ctypes.data,
+                                       C.byref(lat)))
+        return [_read_select(recs[k], ties[k * cap:(k + 1) * cap]) for k in range(n)], lat.value
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(_lib.esch_server_stats(self._h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}

@@ -578,6 +578,11 @@ esch_status esch_batch_wait(esch_batch *b) {
     return ESCH_OK;
 }
 
+int esch_batch_ready(esch_batch *b) {
+    if (!b) return 0;
+    return hipStreamQuery(b->stream) == hipSuccess ? 1 : 0;
+}
+
 esch_status esch_batch_timing(const esch_batch *b, float *dp_ms, float *total_ms) {
     if (!b) return fail(ESCH_INVALID_ARG, "null batch");
     if (dp_ms) {

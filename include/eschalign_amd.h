@@ -40,7 +40,8 @@ typedef enum {
     ESCH_SCORING_INVALID = 6,       /* AlignError::ScoringInvalid (lib/scoring.rs:70-100) */
     ESCH_EMPTY_PANEL = 7,           /* AlignError::EmptyPanel */
     ESCH_EMPTY_REFERENCE = 8,       /* AlignError::EmptyReference */
-    ESCH_CAPACITY = 9               /* a device batch is smaller than the input */
+    ESCH_CAPACITY = 9,              /* a device batch is smaller than the input */
+    ESCH_TIMEOUT = 10               /* a served request is not finished yet */
 } esch_status;
 
 typedef enum { ESCH_LOCAL = 0, ESCH_SEMIGLOBAL = 1 } esch_mode; /* lib/scoring.rs:151-165 */
@@ -227,6 +228,8 @@ esch_status esch_batch_load_concat(esch_batch *b, const uint8_t *codes, const ui
 esch_status esch_batch_run(esch_batch *b, int what, int32_t min_score, int64_t max_ties,
                            size_t tie_cap);
 esch_status esch_batch_wait(esch_batch *b);
+/* 1 when the batch's queued work has finished (non-blocking), else 0. */
+int esch_batch_ready(esch_batch *b);
 /* Milliseconds of the last run: the DP kernel alone (HIP events on its
  * stream) and the whole device pipeline. */
 esch_status esch_batch_timing(const esch_batch *b, float *dp_ms, float *total_ms);
@@ -237,6 +240,52 @@ esch_status esch_batch_read_scores32(esch_batch *b, int32_t *scores, uint8_t *sc
 esch_status esch_batch_read_scores16(esch_batch *b, int16_t *scores, uint8_t *scored);
 esch_status esch_batch_read_select(esch_batch *b, esch_read_select *out, int32_t *ties);
 esch_status esch_batch_read_pairs(esch_batch *b, esch_pair_select *out, int32_t *top);
+
+/* ---------------------------------------------------------------------
+ * Serving: device-resident panels and a continuous-batching request loop
+ * (no counterpart in the reference). A scheduler thread packs queued
+ * requests that share a panel and options into device batches as slots
+ * free up, keeps up to max_slots batches in flight on their own streams,
+ * and creates a slot only while the summed footprint of all slots stays
+ * within hbm_budget. Every result equals the request run alone.
+ * --------------------------------------------------------------------- */
+typedef struct esch_server esch_server;
+typedef struct {
+    int ordinal;             /* device */
+    size_t hbm_budget;       /* bytes of device memory for batch slots */
+    size_t max_batch_reads;  /* reads per device batch (before strands) */
+    size_t max_batch_bases;  /* bases per device batch */
+    int max_slots;           /* batches in flight at once */
+} esch_server_config;
+typedef struct {
+    uint64_t completed;      /* requests finished */
+    uint64_t batches;        /* device batches launched */
+    size_t queued_units;     /* request parts waiting for a slot */
+    size_t hbm_in_use;       /* bytes held by slots now */
+    size_t hbm_peak;         /* most bytes ever held by slots */
+    size_t slots;            /* slots allocated */
+    size_t busy_slots;       /* slots with a batch in flight */
+} esch_server_stats_t;
+
+esch_status esch_server_new(const esch_server_config *cfg, esch_server **out);
+void esch_server_free(esch_server *s);
+/* Make a panel resident (a scorer under one scoring and mode); returns its id. */
+esch_status esch_server_add_panel(esch_server *s, const uint8_t *const *refs, const size_t *ref_lens,
+                                  size_t n_refs, int32_t match, int32_t mismatch, int32_t gap_open,
+                                  int32_t gap_extend, esch_mode mode, int *panel_id);
+/* The scorer of a resident panel (owned by the server), e.g. to autotune it. */
+esch_scorer *esch_server_scorer(esch_server *s, int panel_id);
+/* Queue reads (codes, copied) for fused selection against a panel; returns at once. */
+esch_status esch_server_submit(esch_server *s, int panel_id, const uint8_t *const *reads,
+                               const size_t *lens, size_t n, int both_strands, int32_t min_score,
+                               int64_t max_ties, size_t tie_cap, uint64_t *request_id);
+/* Block until the request finishes (timeout_ms < 0: no limit); ESCH_TIMEOUT if not yet. */
+esch_status esch_server_wait(esch_server *s, uint64_t request_id, int64_t timeout_ms);
+/* Copy a finished request's records (n) and ties (n * tie_cap) and its latency
+ * from submit to completion; releases the request. */
+esch_status esch_server_result(esch_server *s, uint64_t request_id, esch_read_select *out, int32_t *ties,
+                               double *latency_s);
+esch_status esch_server_stats(esch_server *s, esch_server_stats_t *out);
 
 #ifdef __cplusplus
 }

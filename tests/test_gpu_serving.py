@@ -1,0 +1,76 @@
+"""The serving loop on an MI355X: continuous batching over three resident
+panels, admission by HBM footprint, requests arriving while others run.
+Every result must equal the same request run alone (and, on a sample, the
+CPU oracle)."""
+import os
+import random
+import time
+
+import numpy as np
+import pytest
+
+import eschalign_amd as ea
+from eschalign_amd import synth
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def test_served_results_equal_alone_results():
+    scoring = ea.Scoring(1, -1, -2, -1)
+    mode = ea.Mode.SEMIGLOBAL
+    panels = [synth.panel(seed=11 + k, n_refs=n, lo=140, hi=400) for k, n in enumerate((40, 90, 160))]
+    # a budget of about two slots of the largest panel: admission has to wait
+    probe = ea.GpuScorer(ea.Panel(panels[2]), scoring, mode)
+    slot = probe.batch_footprint(4000, 600_000)
+    del probe
+    budget = int(2.5 * slot)
+    server = ea.Server(hbm_budget=budget, max_batch_reads=4000, max_batch_bases=600_000, max_slots=3)
+    ids = [server.add_panel(ea.Panel(p), scoring, mode) for p in panels]
+    alone = [ea.GpuScorer(ea.Panel(p), scoring, mode) for p in panels]
+
+    rng = random.Random(1234)
+    requests = []
+    for r in range(48):
+        pid = rng.randrange(3)
+        n = int(round(np.exp(rng.uniform(0, np.log(3000)))))
+        reads = synth.reads(panels[pid], seed=1000 + r, n=n, length=("lognormal", 90, 0.6, 20, 500))
+        opts = rng.choice([dict(min_score=0, max_ties=None), dict(min_score=20, max_ties=2),
+                           dict(min_score=-5, max_ties=0)])
+        requests.append((pid, reads, opts))
+
+    t0 = time.perf_counter()
+    submitted = []
+    for pid, reads, opts in requests:
+        time.sleep(rng.expovariate(1 / 0.004))   # Poisson arrivals, mean gap 4 ms
+        rid = server.submit(ids[pid], reads, both_strands=True, tie_cap=8, **opts)
+        submitted.append(rid)
+    lat = []
+    results = []
+    for rid in submitted:
+        assert server.wait(rid, timeout_ms=120_000)
+        res, l = server.result(rid)
+        results.append(res)
+        lat.append(l)
+    wall = time.perf_counter() - t0
+    st = server.stats()
+    assert st["completed"] == len(requests)
+    assert st["hbm_peak"] <= budget
+    assert st["batches"] >= 1
+    print(f"[serving] {len(requests)} requests in {wall:.2f} s, {st['batches']} batches, "
+          f"p50 {np.percentile(lat, 50) * 1e3:.1f} ms, p99 {np.percentile(lat, 99) * 1e3:.1f} ms, "
+          f"peak HBM {st['hbm_peak'] / 2**30:.2f} GiB of {budget / 2**30:.2f}")
+
+    for (pid, reads, opts), res in zip(requests, results):
+        want = alone[pid].select_batch(reads, both_strands=True, tie_cap=8, **opts)
+        assert res == want
+    # a sample against the CPU oracle
+    pid, reads, opts = requests[0]
+    rows = reads[:50] + [ea.reverse_complement_codes(q) for q in reads[:50]]
+    sm = O.score_matrix(rows, panels[pid], scoring.tuple(), False, min(16, os.cpu_count() or 1))
+    n = len(reads[:50])
+    for k in range(n):
+        b, nt, so, take = O.select(sm[k], sm[n + k], opts["min_score"], opts["max_ties"])
+        r = results[0][k]
+        assert (r.best, r.n_tied, r.suboptimal, r.take) == (b, nt, so, take[:8])
+    server.close()
