@@ -95,6 +95,65 @@ def calibrate_copy(nbytes):
     print(json.dumps({"calibrate_copy_bytes": nbytes}), flush=True)
 
 
+def serving(rate, duration, seed=0x5E4E):
+    """The serving configuration: three resident panels (164, 1k and 10k
+    random references of 140-1000 nt), Poisson arrivals at `rate` requests/s
+    for `duration` s, each request 1-50k reads (log-uniform) of the batched
+    workload's lengths against a uniformly chosen panel, both strands,
+    semiglobal 1,-1,-2,-1, fused selection. Reports latency from submit to
+    result and reads/s over the run."""
+    import threading
+    import eschalign_amd as ea
+    from eschalign_amd import synth
+
+    rng = np.random.default_rng(seed)
+    panels = [synth.panel(seed=seed + k, n_refs=n, lo=140, hi=1000)
+              for k, n in enumerate((164, 1000, 10000))]
+    scoring = ea.Scoring(1, -1, -2, -1)
+    server = ea.Server(hbm_budget=96 << 30, max_batch_reads=50_000, max_batch_bases=8_000_000,
+                       max_slots=4)
+    ids = [server.add_panel(ea.Panel(p), scoring, ea.Mode.SEMIGLOBAL) for p in panels]
+    for pid, p in zip(ids, panels):
+        server.autotune(pid, synth.reads(p, seed=seed + 7, n=5000))
+    n_req = max(1, int(rng.poisson(rate * duration)))
+    arrivals = np.sort(rng.uniform(0, duration, n_req))
+    reqs = []
+    for r in range(n_req):
+        n = int(round(np.exp(rng.uniform(0, np.log(50_000)))))
+        pid = int(rng.integers(0, 3))
+        reqs.append((pid, synth.reads(panels[pid], seed=seed + 100 + r, n=n)))
+    done = [None] * n_req
+    t0 = time.perf_counter()
+
+    def feeder():
+        for r, (pid, reads) in enumerate(reqs):
+            dt = arrivals[r] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            done[r] = server.submit(ids[pid], reads, both_strands=True, tie_cap=8)
+
+    th = threading.Thread(target=feeder)
+    th.start()
+    lat = []
+    for r in range(n_req):
+        while done[r] is None:
+            time.sleep(0.0005)
+        server.wait(done[r])
+        _, l = server.result(done[r])
+        lat.append(l)
+    th.join()
+    wall = time.perf_counter() - t0
+    st = server.stats()
+    server.close()
+    total = sum(len(x[1]) for x in reqs)
+    return {"requests": n_req, "rate_per_s": rate, "reads": total,
+            "reads_per_s": round(total / wall, 1), "wall_s": round(wall, 2),
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+            "batches": int(st["batches"]), "hbm_peak_gib": round(st["hbm_peak"] / 2**30, 2),
+            "panels": "164 / 1,000 / 10,000 refs of 140-1000 nt"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +163,9 @@ def main():
     ap.add_argument("--mode", default="semiglobal", choices=["local", "semiglobal"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-kernel comparison")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving run")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="serving arrivals per second")
+    ap.add_argument("--serve-seconds", type=float, default=12.0, help="serving arrival window")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (scales rocprof's "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -191,6 +253,11 @@ def main():
         cpu = cpu_baseline(panel_codes, codes, off, lens, scoring, local, gpu_check)
         log(f"[bench] CPU baseline: {cpu}")
 
+    serve = None
+    if not args.no_serving:
+        serve = serving(args.serve_rate, args.serve_seconds)
+        log(f"[bench] serving: {serve}")
+
     line = {
         "metric": "reads/s + GCUPS, bit-exact scores vs CPU; % VALU roofline; p50/p99 latency",
         "value": round(value, 1),
@@ -220,6 +287,7 @@ def main():
         "autotune_ms": [round(x, 3) for x in tune_ms],
         "plain_kernel": plain,
         "cpu_baseline": cpu,
+        "serving": serve,
     }
     print(json.dumps(line), flush=True)
 
