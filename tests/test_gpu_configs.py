@@ -1,0 +1,98 @@
+"""The benchmark configurations as parity cases on an MI355X: each one's
+synthetic workload (at a size the CPU checker finishes in seconds) scored on
+the device and compared with the CPU kernels, by equality.
+
+The checker for the large panels is the oracle's AVX-512/AVX2 restatement of
+the reference's CPU kernel, itself pinned to the scalar oracle in
+tests/test_oracle.py; the scalar oracle checks a sample directly."""
+import os
+
+import numpy as np
+import pytest
+
+import eschalign_amd as ea
+from eschalign_amd import synth
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def simd_rows(panel_codes, queries, scoring, local):
+    sup = O.simd_supported()
+    lanes = 32 if sup & 2 else 16
+    prof = O.Profile(panel_codes, scoring, lanes)
+    qc, qo, ql = O.concat(queries)
+    return prof.score_batch(qc, qo, ql, local, THREADS)
+
+
+def test_config1_plain_cpu_reference_path():
+    """164 refs of 140-160 nt, 10k reads of 60-150 nt, local 2,-1,-10,-1,
+    forward strand only: every score equals the CPU kernel; a sample equals
+    the scalar oracle."""
+    panel = synth.panel(seed=0x0123456789ABCDEF, n_refs=164, lo=140, hi=160)
+    reads = synth.reads(panel, seed=0xC0F1, n=10_000, length=("uniform", 60, 150))
+    scoring = (2, -1, -10, -1)
+    gpu = ea.GpuScorer(ea.Panel(panel), ea.Scoring(*scoring), ea.Mode.LOCAL)
+    got, ok = gpu.score_batch_i32(reads)
+    assert ok.all()
+    assert np.array_equal(got, simd_rows(panel, reads, scoring, True))
+    assert np.array_equal(got[:200], O.score_matrix(reads[:200], panel, scoring, True, THREADS))
+    m = gpu.score_batch(reads)          # the drop-in i16 matrix agrees
+    assert m.n_scored() == len(reads) and np.array_equal(m.scores.astype(np.int32), got)
+
+
+def test_config3_large_panel_near_ties():
+    """A large panel of seeds plus 4 copies each at 2% substitutions (ties
+    and near-ties common), refs 200-1000 nt, reads 100-1000 nt, local
+    2,-1,-10,-1: scores equal the CPU kernel and the fused tie sets equal
+    the oracle's selection on them."""
+    panel = synth.panel_with_copies(seed=0xC0F3, n_seeds=120, copies=4, lo=200, hi=1000)
+    reads = synth.reads(panel, seed=0xC0F3 + 1, n=300, length=("uniform", 100, 1000))
+    scoring = (2, -1, -10, -1)
+    gpu = ea.GpuScorer(ea.Panel(panel), ea.Scoring(*scoring), ea.Mode.LOCAL)
+    got, ok = gpu.score_batch_i32(reads)
+    want = simd_rows(panel, reads, scoring, True)
+    assert ok.all() and np.array_equal(got, want)
+    sel = gpu.select_batch(reads, both_strands=False, tie_cap=64)
+    ties_seen = 0
+    for k in range(len(reads)):
+        b, nt, so, take = O.select(want[k], None, 0, None)
+        ties_seen += nt > 1
+        assert (sel[k].best, sel[k].n_tied, sel[k].suboptimal, sel[k].take) == (b, nt, so, take[:64])
+    assert ties_seen > 0, "the paralog panel should produce ties"
+
+
+def test_config4_paired_with_adapter_read_through():
+    """500 refs of 70-95 nt, mates of 150 nt from inserts N(120, 30) reading
+    into the TruSeq adapters: mates trimmed by the library's adapter
+    detection (checked against the oracle), scored on both orientations,
+    and the device's top pair tier equal to the oracle's candidates."""
+    panel = synth.panel(seed=0xC0F4, n_refs=500, lo=70, hi=95)
+    codes, off, lens = synth.pair_arrays(panel, seed=0xC0F4 + 1, n_pairs=400)
+    letters = [bytes(b"ACGTN"[c] for c in codes[int(o):int(o) + int(n)]) for o, n in zip(off, lens)]
+    m1, m2 = [], []
+    for k in range(0, len(letters), 2):
+        a, b = letters[k], letters[k + 1]
+        ad = ea.adapter_detect_pair(a, b)
+        assert ad == O.adapter_detect_pair(a, b, ea.TRUSEQ_R1, ea.TRUSEQ_R2)
+        m1.append(ea.encode(a[:len(a) - ad[0]]))
+        m2.append(ea.encode(b[:len(b) - ad[1]]))
+    scoring = (2, -1, -10, -1)
+    gpu = ea.GpuScorer(ea.Panel(panel), ea.Scoring(*scoring), ea.Mode.SEMIGLOBAL)
+    rows = {}
+    for name, mates in (("m1", m1), ("m2", m2)):
+        rc = [ea.reverse_complement_codes(q) for q in mates]
+        rows[name] = (simd_rows(panel, mates, scoring, False), simd_rows(panel, rc, scoring, False))
+    for both in (False, True):
+        got = gpu.select_pairs(m1, m2, both_strands=both, min_score=0, top_cap=32)
+        for k in range(len(m1)):
+            r = [[rows["m1"][0][k], rows["m1"][1][k]], [rows["m2"][0][k], rows["m2"][1][k]]]
+            cands = O.pair_cands(r, 0, both)
+            g = got[k]
+            assert g.n_cand == len(cands)
+            if not cands:
+                continue
+            top = cands[0][0]
+            tier = [(c[1], c[2]) for c in cands if c[0] == top]
+            assert g.top_sum == top and g.n_top == len(tier) and g.top == tier[:32], k
