@@ -62,7 +62,8 @@ EXPORTS = [
     "esch_set_variant", "esch_autotune", "esch_adapter_find", "esch_adapter_detect_pair",
     "esch_batch_ready", "esch_server_new", "esch_server_free", "esch_server_add_panel",
     "esch_server_scorer", "esch_server_submit", "esch_server_wait", "esch_server_result",
-    "esch_server_stats",
+    "esch_server_stats", "esch_set_diag_min_len", "esch_get_diag_min_len",
+    "esch_batch_stats_diag",
 ]
 
 
@@ -168,6 +169,10 @@ def load_library():
     L.esch_server_wait.argtypes = [vp, C.c_uint64, i64]
     L.esch_server_result.argtypes = [vp, C.c_uint64, vp, vp, C.POINTER(C.c_double)]
     L.esch_server_stats.argtypes = [vp, C.POINTER(ServerStats)]
+    L.esch_set_diag_min_len.argtypes = [vp, i32]
+    L.esch_get_diag_min_len.argtypes = [vp]
+    L.esch_get_diag_min_len.restype = i32
+    L.esch_batch_stats_diag.argtypes = [vp, C.POINTER(C.c_uint64)]
     _lib = L
     return L
 
@@ -533,6 +538,15 @@ class GpuScorer:
     def variant(self, v: int):
         _check(_lib.esch_set_variant(self._h, int(v)))
 
+    @property
+    def diag_min_len(self) -> int:
+        """Reads this long take the anti-diagonal kernel (0: none)."""
+        return _lib.esch_get_diag_min_len(self._h)
+
+    @diag_min_len.setter
+    def diag_min_len(self, v: int):
+        _check(_lib.esch_set_diag_min_len(self._h, int(v)))
+
     def autotune(self, reads: Sequence[np.ndarray], both_strands=True):
         """Time every batched-kernel variant on `reads`, keep the fastest;
         returns (chosen, [ms per variant])."""
@@ -625,6 +639,11 @@ class DeviceBatch:
         a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
         _check(_lib.esch_batch_stats(self._h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
+
+    def n_diag(self) -> int:
+        v = C.c_uint64()
+        _check(_lib.esch_batch_stats_diag(self._h, C.byref(v)))
+        return v.value
 
     def scores32(self):
         nr = self.scorer.n_refs

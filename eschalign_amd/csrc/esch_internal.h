@@ -60,7 +60,10 @@ struct DevDesc {
     int32_t n16, n32;    // queries on the int16 and int32 kernels
     uint32_t next16, next32a, next32b;  // work-queue cursors
     int32_t tstrands;    // pairs: template strands considered (1 or 2)
-    uint32_t pad_[2];
+    int32_t diag_min_len;  // reads this long go to the anti-diagonal kernel (0: none)
+    int32_t nd;          // queries on the anti-diagonal kernel
+    uint32_t nextD;      // its work-queue cursor
+    uint32_t pad_[3];
     uint64_t cells;      // sum of len x panel bases over scored queries
 };
 
@@ -84,7 +87,11 @@ struct KArgs {
     int32_t *recs;       // select / pair records, 5 int32 each
     int32_t *ties;       // [n * tie_cap]
     uint8_t *rc_codes;   // where reverse complements are written
+    int32_t *listD;      // anti-diagonal-kernel queries
+    uint2 *scratchD;     // [grid_diag * 4 rows * max_len]
 };
+// Rows per lane of the anti-diagonal kernel (a pass covers 16 * kDiagRows).
+constexpr int kDiagRows = 8;
 
 // Host-side launchers (kernels.hip); every kernel runs on `st`.
 struct Launch {
@@ -93,12 +100,15 @@ struct Launch {
     int block32;  // threads per k_dp32 block
     int grid_aux; // blocks of the bucketing / selection kernels
     int variant16;  // k_dp16 variant: 0 profile loaded per column, 1 prefetched
+    int grid_diag;  // persistent waves of k_diag16
 };
 constexpr int kDp16Variants = 2;
 hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st);
 hipError_t launch_dp16(const KArgs &a, const Launch &g, bool local, hipStream_t st);
 hipError_t launch_dp32(const KArgs &a, const Launch &g, bool local, int which, hipStream_t st);
 hipError_t launch_select(const KArgs &a, const Launch &g, hipStream_t st);
+hipError_t launch_diag(const KArgs &a, const Launch &g, bool local, hipStream_t st);
+hipError_t diag_occupancy(bool local, int *blocks_per_cu);
 hipError_t launch_pairs(const KArgs &a, const Launch &g, hipStream_t st);
 hipError_t dp16_attributes(bool local, int variant, hipFuncAttributes *attr);
 hipError_t dp32_attributes(bool local, hipFuncAttributes *attr);

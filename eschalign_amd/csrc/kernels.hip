@@ -297,6 +297,193 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// k_diag16 — anti-diagonal (intra-task) packed-int16 DP for long reads.
+//
+// For reads too few per length to fill a 128-read chunk. A row of 16 lanes
+// (one DPP row) scores ONE read against TWO references packed in the int16
+// halves; a 64-lane wave runs four such tasks over 8 length-adjacent
+// references of the same read. Lane l of a row holds R consecutive rows of
+// the read in registers (a pass covers 16 R rows; longer reads take several
+// passes, the last row of a pass handed to the next through scratch) and
+// processes column j at step j + l: the wavefront sweeps the anti-diagonal.
+// At every step each lane takes from lane l - 1, with a `row_shr:1` DPP
+// move, the bottom row's (Ho, F) of the column it now computes and that
+// column's reference-code selector; lane 0 takes the previous pass's
+// boundary (or the border) and the next column's codes as the DPP `old`
+// operand. The substitution is one v_perm_b32 per row: the row's profile
+// s(q_r, c) - o for c = 0..3 sits in two registers as four int16, and the
+// selector picks code cA's int16 into the low half and cB's into the high.
+// A wildcard reference base (rare) takes a slow path. The recurrence and
+// the Ho = H + o bookkeeping are k_dp16's: 9 packed VALU ops per row per
+// two cells.
+// ---------------------------------------------------------------------------
+
+constexpr int kDiagRefs = 8;  // references per work item: 4 rows x 2 halves
+
+__device__ __forceinline__ uint32_t half_sel(int c) {
+    // v_perm selector bytes of int16 entry c (0..3) of {tab1:tab0}; 0x0C0C = zero
+    return c < 4 ? (uint32_t)(2 * c) | ((uint32_t)(2 * c + 1) << 8) : 0x0C0Cu;
+}
+
+template <int R, bool LOCAL>
+__global__ __launch_bounds__(64) void k_diag16(KArgs a) {
+    const int lane = threadIdx.x;
+    const int grp = lane >> 4;
+    const int l = lane & 15;
+    const DevPanel &P = a.panel;
+    const int n_refs = P.n_refs;
+    const int n_rg = (n_refs + kDiagRefs - 1) / kDiagRefs;
+    const int nd = uni(a.desc->nd);
+    const uint32_t n_items = (uint32_t)nd * (uint32_t)n_rg;
+    uint2 *scr = a.scratchD + ((size_t)blockIdx.x * 4 + grp) * (size_t)P.max_len;
+    const v2s vo = splat(a.sc.o), ve = splat(a.sc.e), NEG = splat(-32768);
+    const int nval_real = a.sc.ma - a.sc.o, nval_pre = -a.sc.o;
+    constexpr int RP = 16 * R;
+
+    for (;;) {
+        uint32_t t0 = 0;
+        if (lane == 0) t0 = atomicAdd(&a.desc->nextD, 1u);
+        t0 = (uint32_t)uni((int)t0);
+        if (t0 >= n_items) break;
+        const int qi = (int)(t0 / (uint32_t)n_rg);
+        const int g = (int)(t0 - (uint32_t)qi * (uint32_t)n_rg);
+        const int q = uni(a.listD[qi]);
+        const int n = uni(a.qlen[q]);
+        const uint8_t *qc = a.qcodes + a.qoff[q];
+        const int slotA = g * kDiagRefs + 2 * grp, slotB = slotA + 1;
+        const int refA = slotA < n_refs ? P.rorder[slotA] : -1;
+        const int refB = slotB < n_refs ? P.rorder[slotB] : -1;
+        const int mA = refA >= 0 ? P.rlen[refA] : 0;
+        const int mB = refB >= 0 ? P.rlen[refB] : 0;
+        // longest first: the item's first reference is its longest
+        const int M = uni(P.rlen[P.rorder[g * kDiagRefs]]);
+        const uint32_t *w2A = P.ref2 + (refA >= 0 ? P.w2off[refA] : 0);
+        const uint32_t *wnA = P.refN + (refA >= 0 ? P.nmoff[refA] : 0);
+        const uint32_t *w2B = P.ref2 + (refB >= 0 ? P.w2off[refB] : 0);
+        const uint32_t *wnB = P.refN + (refB >= 0 ? P.nmoff[refB] : 0);
+        const int Pn = (n + RP - 1) / RP;
+        const int pre = Pn * RP - n;
+        v2s best = LOCAL ? vo : NEG;  // Ho space
+
+        for (int p = 0; p < Pn; ++p) {
+            // this pass's rows of lane l: p RP + l R + r
+            uint32_t tab0[R], tab1[R];
+            uint32_t prebits = 0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = p * RP + l * R + r;
+                const int x = i < pre ? (int)kPreCode : (int)qc[i - pre];
+                prebits |= (i < pre ? 1u : 0u) << r;
+                const int v0 = prof_val(x, 0, a.sc), v1 = prof_val(x, 1, a.sc);
+                const int v2 = prof_val(x, 2, a.sc), v3 = prof_val(x, 3, a.sc);
+                tab0[r] = pack2(v0, v1);
+                tab1[r] = pack2(v2, v3);
+            }
+            v2s Ho[R], E[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                Ho[r] = vo;
+                E[r] = NEG;
+            }
+            const bool lastpass = p + 1 == Pn;
+            uint32_t bot_h = as_u32(vo), bot_f = as_u32(NEG), sel_prev = 0, nb_prev = 0;
+            uint32_t in_prev = as_u32(vo);  // Ho above-left (column -1: border)
+            uint32_t wA = 0, wB = 0, nA = 0, nB = 0;
+            for (int t = 0; t < M + 15; ++t) {
+                // lane 0's inputs for column t: codes of both references
+                // each reference's words are read only inside its own length
+                if ((t & 15) == 0) {
+                    if (t < mA) wA = w2A[t >> 4];
+                    if (t < mB) wB = w2B[t >> 4];
+                }
+                if ((t & 31) == 0) {
+                    if (t < mA) nA = wnA[t >> 5];
+                    if (t < mB) nB = wnB[t >> 5];
+                }
+                uint32_t sel0 = 0x0C0C0C0Cu, nb0 = 0;
+                if (t < M) {
+                    const int cA = t < mA ? (int)((wA >> (2 * (t & 15))) & 3u) : 4;
+                    const int cB = t < mB ? (int)((wB >> (2 * (t & 15))) & 3u) : 4;
+                    const bool NA = t < mA && ((nA >> (t & 31)) & 1u);
+                    const bool NB = t < mB && ((nB >> (t & 31)) & 1u);
+                    sel0 = half_sel(NA ? 4 : cA) | (half_sel(NB ? 4 : cB) << 16);
+                    nb0 = (NA ? 1u : 0u) | (NB ? 2u : 0u);
+                }
+                uint2 b0 = make_uint2(as_u32(vo), as_u32(NEG));
+                if (p > 0 && t < M) b0 = scr[t];
+                // shift the wavefront one lane down the row
+                const uint32_t hin = (uint32_t)__builtin_amdgcn_update_dpp((int)b0.x, (int)bot_h, 0x111, 0xF, 0xF, false);
+                const uint32_t fin = (uint32_t)__builtin_amdgcn_update_dpp((int)b0.y, (int)bot_f, 0x111, 0xF, 0xF, false);
+                const uint32_t sel = (uint32_t)__builtin_amdgcn_update_dpp((int)sel0, (int)sel_prev, 0x111, 0xF, 0xF, false);
+                const uint32_t nb = (uint32_t)__builtin_amdgcn_update_dpp((int)nb0, (int)nb_prev, 0x111, 0xF, 0xF, false);
+                const uint32_t hdiag = in_prev;
+                in_prev = hin;
+                sel_prev = sel;
+                nb_prev = nb;
+                const int j = t - l;
+                if (j >= 0 && j < M) {
+                    v2s sc[R];
+#pragma unroll
+                    for (int r = 0; r < R; ++r) sc[r] = as_v2s(__builtin_amdgcn_perm(tab1[r], tab0[r], sel));
+                    if (nb) {
+                        // wildcard reference base: a match for every real row
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            const short nv = (short)(((prebits >> r) & 1u) ? nval_pre : nval_real);
+                            if (nb & 1u) sc[r].x = nv;
+                            if (nb & 2u) sc[r].y = nv;
+                        }
+                    }
+                    v2s hou = as_v2s(hin), f = as_v2s(fin);
+                    column16<R, LOCAL>(sc, Ho, E, hou, f, as_v2s(hdiag), vo, ve, best);
+                    bot_h = as_u32(hou);
+                    bot_f = as_u32(f);
+                    if (l == 15 && !lastpass) scr[j] = make_uint2(bot_h, bot_f);
+                    if (!LOCAL) {
+                        if (lastpass && l == 15) {
+                            // the read's last row (register R - 1 of lane 15, last pass)
+                            v2s v = hou;
+                            if (j >= mA) v.x = (short)-32768;
+                            if (j >= mB) v.y = (short)-32768;
+                            best = vmax(best, v);
+                        }
+                        if (j == mA - 1 || j == mB - 1) {
+                            // a reference's last column: every real row
+#pragma unroll
+                            for (int r = 0; r < R; ++r) {
+                                v2s v = Ho[r];
+                                const bool real = !((prebits >> r) & 1u);
+                                if (!(real && j == mA - 1)) v.x = (short)-32768;
+                                if (!(real && j == mB - 1)) v.y = (short)-32768;
+                                best = vmax(best, v);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        // the row's best over its 16 lanes
+        int bx = best.x, by = best.y;
+        for (int off = 8; off > 0; off >>= 1) {
+            bx = max(bx, __shfl_xor(bx, off, 16));
+            by = max(by, __shfl_xor(by, off, 16));
+        }
+        if (l == 0) {
+            if (refA >= 0) {
+                const size_t o = (size_t)q * n_refs + refA;
+                if (a.out32) a.out32[o] = bx - a.sc.o;
+                if (a.out16) a.out16[o] = (int16_t)(bx - a.sc.o);
+            }
+            if (refB >= 0) {
+                const size_t o = (size_t)q * n_refs + refB;
+                if (a.out32) a.out32[o] = by - a.sc.o;
+                if (a.out16) a.out16[o] = (int16_t)(by - a.sc.o);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // k_dp32 — plain one-thread-per-pair int32 DP (the baseline kernel, and the
 // exact rescue for queries outside the int16 bound).
 //
@@ -429,6 +616,8 @@ __global__ void k_zero(KArgs a) {
     if (i == 0) {
         a.desc->n16 = 0;
         a.desc->n32 = 0;
+        a.desc->nd = 0;
+        a.desc->nextD = 0;
         a.desc->next16 = 0;
         a.desc->next32a = 0;
         a.desc->next32b = 0;
@@ -481,8 +670,12 @@ __global__ void k_classify(KArgs a) {
                 c = (b16 && !force32) ? 1 : 2;
             }
         }
+        if (c == 1 && a.desc->diag_min_len > 0 && n >= a.desc->diag_min_len) c = 3;
         a.cls[q] = c;
-        if (c == 1) {
+        if (c == 3) {
+            const int pos = atomicAdd(&a.desc->nd, 1);
+            a.listD[pos] = q;
+        } else if (c == 1) {
             atomicAdd(&a.hist[n], 1);
         } else if (c == 2) {
             const int pos = atomicAdd(&a.desc->n32, 1);
@@ -742,6 +935,17 @@ hipError_t launch_dp32(const KArgs &a, const Launch &g, bool local, int which, h
     if (local) hipLaunchKernelGGL(k_dp32<true>, dim3(g.grid32), dim3(g.block32), 0, st, a, which);
     else hipLaunchKernelGGL(k_dp32<false>, dim3(g.grid32), dim3(g.block32), 0, st, a, which);
     return hipGetLastError();
+}
+
+hipError_t launch_diag(const KArgs &a, const Launch &g, bool local, hipStream_t st) {
+    if (local) hipLaunchKernelGGL((k_diag16<kDiagRows, true>), dim3(g.grid_diag), dim3(64), 0, st, a);
+    else hipLaunchKernelGGL((k_diag16<kDiagRows, false>), dim3(g.grid_diag), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t diag_occupancy(bool local, int *blocks) {
+    return local ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_diag16<kDiagRows, true>, 64, 0)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_diag16<kDiagRows, false>, 64, 0);
 }
 
 hipError_t launch_select(const KArgs &a, const Launch &g, hipStream_t st) {

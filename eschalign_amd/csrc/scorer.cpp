@@ -84,6 +84,7 @@ struct esch_scorer {
     int regs16_v[kDp16Variants] = {}, local16_v[kDp16Variants] = {};
     int max_grid16 = 0;                 // scratch is sized for the largest
     int regs16 = 0, local16 = 0;
+    int diag_min_len = 0;               // reads this long take k_diag16 (0: never)
     std::mutex pool_mu;
     std::vector<esch_batch *> pool;  // idle batches for the drop-in API
 };
@@ -106,6 +107,8 @@ struct esch_batch {
     uint2 *d_scr16 = nullptr;
     int2 *d_scr32 = nullptr;
     int32_t *d_recs = nullptr, *d_ties = nullptr;
+    int32_t *d_listD = nullptr;
+    uint2 *d_scrD = nullptr;
     DevDesc *d_desc = nullptr;
     // pinned staging
     uint8_t *h_codes = nullptr;
@@ -262,6 +265,9 @@ esch_status esch_scorer_new(int ordinal, const uint8_t *const *refs, const size_
     s->geo.block32 = block32;
     s->geo.grid32 = std::max(1, b32) * s->n_cu;
     s->geo.grid_aux = 4 * s->n_cu;
+    int bd = 0;
+    HIPC(diag_occupancy(local, &bd));
+    s->geo.grid_diag = std::max(1, bd) * s->n_cu;
     *out = s.release();
     return ESCH_OK;
 }
@@ -317,6 +323,7 @@ size_t esch_batch_footprint(const esch_scorer *s, size_t max_reads, size_t max_b
     f += (size_t)s->max_grid16 * 4 * s->max_len * 64 * sizeof(uint2);        // scratch16
     f += (size_t)s->geo.grid32 * s->geo.block32 * s->max_len * sizeof(int2);  // scratch32
     f += rows * 5 * 4;                                                   // records
+    f += rows * 4 + (size_t)s->geo.grid_diag * 4 * s->max_len * sizeof(uint2);  // anti-diagonal
     return f;
 }
 
@@ -348,6 +355,8 @@ esch_status esch_batch_create(esch_scorer *s, size_t max_reads, size_t max_bases
     HIPC(hipMalloc(&b->d_scr16, (size_t)s->max_grid16 * 4 * s->max_len * 64 * sizeof(uint2)));
     HIPC(hipMalloc(&b->d_scr32, (size_t)s->geo.grid32 * s->geo.block32 * s->max_len * sizeof(int2)));
     HIPC(hipMalloc(&b->d_recs, rows * 5 * 4));
+    HIPC(hipMalloc(&b->d_listD, rows * 4));
+    HIPC(hipMalloc(&b->d_scrD, (size_t)s->geo.grid_diag * 4 * s->max_len * sizeof(uint2)));
     HIPC(hipMalloc(&b->d_desc, sizeof(DevDesc)));
     HIPC(hipHostMalloc(&b->h_codes, bases, hipHostMallocDefault));
     HIPC(hipHostMalloc(&b->h_qoff, rows * 8, hipHostMallocDefault));
@@ -378,6 +387,8 @@ void esch_batch_free(esch_batch *b) {
     (void)hipFree(b->d_scr32);
     (void)hipFree(b->d_recs);
     (void)hipFree(b->d_ties);
+    (void)hipFree(b->d_listD);
+    (void)hipFree(b->d_scrD);
     (void)hipFree(b->d_desc);
     (void)hipHostFree(b->h_codes);
     (void)hipHostFree(b->h_qoff);
@@ -463,6 +474,8 @@ KArgs make_args(esch_batch *b, int what) {
     a.recs = b->d_recs;
     a.ties = b->d_ties;
     a.rc_codes = b->d_codes;
+    a.listD = b->d_listD;
+    a.scratchD = b->d_scrD;
     return a;
 }
 
@@ -478,6 +491,7 @@ esch_status enqueue(esch_batch *b, int what, bool timed) {
         HIPC(launch_dp32(a, s->geo, local, 1, st));
     } else {
         HIPC(launch_dp16(a, s->geo, local, st));
+        HIPC(launch_diag(a, s->geo, local, st));
         HIPC(launch_dp32(a, s->geo, local, 1, st));
     }
     if (timed) HIPC(hipEventRecord(b->ev2, st));
@@ -539,6 +553,7 @@ esch_status esch_batch_run(esch_batch *b, int what, int32_t min_score, int64_t m
     d.tie_cap = (int32_t)std::max<size_t>(tie_cap, 1);
     d.max_ties = max_ties;
     d.tstrands = (what & ESCH_RUN_PAIR_BOTH) ? 2 : 1;
+    d.diag_min_len = (what & (ESCH_RUN_PLAIN | ESCH_RUN_SCORES16)) ? 0 : s->diag_min_len;
     HIPC(hipMemcpyAsync(b->d_desc, b->h_desc, sizeof(DevDesc), hipMemcpyHostToDevice, b->stream));
     b->last_what = what;
     const bool graph = !(what & ESCH_RUN_NO_GRAPH);
@@ -598,8 +613,16 @@ esch_status esch_batch_stats(const esch_batch *b, uint64_t *cells, uint64_t *n16
     DevDesc d;
     HIPC(hipMemcpy(&d, b->d_desc, sizeof d, hipMemcpyDeviceToHost));
     if (cells) *cells = d.cells;
-    if (n16) *n16 = (uint64_t)d.n16;
+    if (n16) *n16 = (uint64_t)d.n16 + (uint64_t)d.nd;
     if (n32) *n32 = (uint64_t)d.n32;
+    return ESCH_OK;
+}
+
+esch_status esch_batch_stats_diag(const esch_batch *b, uint64_t *n_diag) {
+    if (!b || !n_diag) return fail(ESCH_INVALID_ARG, "null argument");
+    DevDesc d;
+    HIPC(hipMemcpy(&d, b->d_desc, sizeof d, hipMemcpyDeviceToHost));
+    *n_diag = (uint64_t)d.nd;
     return ESCH_OK;
 }
 
@@ -789,6 +812,14 @@ esch_status esch_set_variant(esch_scorer *s, int variant) {
 }
 
 int esch_get_variant(const esch_scorer *s) { return s ? s->geo.variant16 : -1; }
+
+esch_status esch_set_diag_min_len(esch_scorer *s, int32_t min_len) {
+    if (!s || min_len < 0) return fail(ESCH_INVALID_ARG, "bad anti-diagonal threshold");
+    s->diag_min_len = min_len;
+    return ESCH_OK;
+}
+
+int32_t esch_get_diag_min_len(const esch_scorer *s) { return s ? s->diag_min_len : -1; }
 int esch_n_variants(void) { return kDp16Variants; }
 
 esch_status esch_autotune(esch_scorer *s, const uint8_t *const *reads, const size_t *lens, size_t n,

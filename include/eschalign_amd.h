@@ -173,6 +173,14 @@ esch_status esch_select_pairs(esch_scorer *s, const uint8_t *const *mates1, cons
                               int both_strands, int32_t min_score, size_t top_cap,
                               esch_pair_select *out, int32_t *top);
 
+/* Reads at least min_len long are scored by the anti-diagonal wavefront
+ * kernel (one read against two references per 16-lane DPP row) instead of
+ * the length-bucketed inter-sequence kernel; 0 (the default) sends none.
+ * For batches whose long reads are too few per length to fill 128-read
+ * chunks. Identical scores either way. */
+esch_status esch_set_diag_min_len(esch_scorer *s, int32_t min_len);
+int32_t esch_get_diag_min_len(const esch_scorer *s);
+
 /* adapter::find — lib/adapter.rs:68-91. Letters (not codes): how many bases
  * at the 3' end of `read` are the adapter's prefix (0 = none). The pre-step
  * that trims mates before scoring (crates/eschalign-cli/src/align.rs:2844-2862). */
@@ -236,6 +244,8 @@ esch_status esch_batch_timing(const esch_batch *b, float *dp_ms, float *total_ms
 /* Cells of the last run (sum over scored queries of len x panel bases) and
  * how many queries went to the int16 and int32 kernels. */
 esch_status esch_batch_stats(const esch_batch *b, uint64_t *cells, uint64_t *n16, uint64_t *n32);
+/* How many queries of the last run took the anti-diagonal kernel. */
+esch_status esch_batch_stats_diag(const esch_batch *b, uint64_t *n_diag);
 esch_status esch_batch_read_scores32(esch_batch *b, int32_t *scores, uint8_t *scored);
 esch_status esch_batch_read_scores16(esch_batch *b, int16_t *scores, uint8_t *scored);
 esch_status esch_batch_read_select(esch_batch *b, esch_read_select *out, int32_t *ties);

@@ -328,3 +328,31 @@ def test_every_kernel_variant_equals_oracle_and_autotune_picks_one():
                 assert ok.all() and np.array_equal(s32, want), (scoring, mode, v)
             chosen, ms = gpu.autotune(queries[:120])
             assert gpu.variant == chosen and all(t > 0 for t in ms)
+
+
+def test_anti_diagonal_kernel_equals_oracle():
+    """The anti-diagonal wavefront kernel, forced for every read (threshold 1)
+    and for long reads only: lengths 1..3000 (one to several 128-row passes),
+    ragged reference pairs, wildcards in reads and references, five schemes,
+    both modes, against the oracle; and the routing counts it."""
+    rng = G.Rng(0x5EED_0B01)
+    panel_l = G.random_panel(rng, 21, 30, 700)
+    panel = ea.Panel(panel_l)
+    reads = [G.random_seq(rng, 1 + rng.below(40), 0.05) for _ in range(40)]
+    reads += [G.chimeric_read(rng, panel_l, 100 + rng.below(2900)) for _ in range(40)]
+    queries = codes_of(reads)
+    for scoring in G.SCHEMES_GPU:
+        for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
+            gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+            want = oracle(queries, panel.codes, scoring, mode)
+            for thr in (1, 1000):
+                gpu.diag_min_len = thr
+                b = gpu.batch(len(queries), sum(len(q) for q in queries))
+                b.load(queries)
+                b.run(ea.RUN_SCORES32)
+                got, ok = b.scores32()
+                assert ok.all()
+                bad = np.argwhere(got != want)
+                assert bad.size == 0, (scoring, mode, thr, bad[:4], len(queries[bad[0][0]]))
+                assert b.n_diag() == sum(len(q) >= thr for q in queries)
+                b.close()
