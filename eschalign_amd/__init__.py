@@ -63,7 +63,7 @@ EXPORTS = [
     "esch_batch_ready", "esch_server_new", "esch_server_free", "esch_server_add_panel",
     "esch_server_scorer", "esch_server_submit", "esch_server_wait", "esch_server_result",
     "esch_server_stats", "esch_set_diag_min_len", "esch_get_diag_min_len",
-    "esch_batch_stats_diag",
+    "esch_batch_stats_diag", "esch_set_diag_ratio", "esch_get_diag_ratio",
 ]
 
 
@@ -173,6 +173,8 @@ def load_library():
     L.esch_get_diag_min_len.argtypes = [vp]
     L.esch_get_diag_min_len.restype = i32
     L.esch_batch_stats_diag.argtypes = [vp, C.POINTER(C.c_uint64)]
+    L.esch_set_diag_ratio.argtypes = [vp, C.c_int32]
+    L.esch_get_diag_ratio.argtypes = [vp]
     _lib = L
     return L
 
@@ -547,9 +549,21 @@ class GpuScorer:
     def diag_min_len(self, v: int):
         _check(_lib.esch_set_diag_min_len(self._h, int(v)))
 
+    @property
+    def diag_ratio(self) -> float:
+        """Adaptive routing per length bucket: reads of length L take the
+        anti-diagonal kernel when L > ratio * L' + 64 (L' the read 127 ranks
+        later) or when they lead a mostly empty chunk; 0.0 turns it off."""
+        return _lib.esch_get_diag_ratio(self._h) / 16.0
+
+    @diag_ratio.setter
+    def diag_ratio(self, ratio: float):
+        _check(_lib.esch_set_diag_ratio(self._h, int(round(ratio * 16))))
+
     def autotune(self, reads: Sequence[np.ndarray], both_strands=True):
-        """Time every batched-kernel variant on `reads`, keep the fastest;
-        returns (chosen, [ms per variant])."""
+        """Time every batched-kernel variant on `reads`, keep the fastest,
+        then sweep the per-length-bucket routing ratio (unless it was set);
+        returns (chosen, [ms per variant]); `diag_ratio` reads the ratio."""
         keep, ptrs, lens = _ptr_array(reads)
         nv = _lib.esch_n_variants()
         ms = np.zeros(nv, np.float32)

@@ -63,7 +63,9 @@ struct DevDesc {
     int32_t diag_min_len;  // reads this long go to the anti-diagonal kernel (0: none)
     int32_t nd;          // queries on the anti-diagonal kernel
     uint32_t nextD;      // its work-queue cursor
-    uint32_t pad_[3];
+    int32_t diag_ratio;  // adaptive routing to the anti-diagonal kernel, x16 (0: off)
+    int32_t blocks16;    // resident blocks of the inter-sequence kernel's grid
+    uint32_t pad_[1];
     uint64_t cells;      // sum of len x panel bases over scored queries
 };
 
@@ -92,6 +94,7 @@ struct KArgs {
 };
 // Rows per lane of the anti-diagonal kernel (a pass covers 16 * kDiagRows).
 constexpr int kDiagRows = 8;
+constexpr int kDiagRatioDefault = 32;  // 2.0 in sixteenths
 
 // Host-side launchers (kernels.hip); every kernel runs on `st`.
 struct Launch {

@@ -697,27 +697,47 @@ __global__ void k_classify(KArgs a) {
 }
 
 // Exclusive prefix over the length bins in DESCENDING length, one block of
-// 1024 threads: per-thread range sums, a block scan of the sums, then each
-// thread writes its range.
+// 1024 threads, with the adaptive routing to the anti-diagonal kernel.
+//
+// rank(L) = reads longer than L. A read of length L would share its chunk
+// with the reads down to rank(L) + 127, of length L'. When L > rho L' + 64
+// (rho = diag_ratio / 16, the per-cell cost of the anti-diagonal kernel
+// relative to the inter-sequence one, 2 by default) the chunk would be
+// mostly padding, so the whole length bin goes to k_diag16 instead
+// (bin_start = -1). And when the whole batch makes fewer inter-sequence
+// work items (chunks x groups of 4 references) than that kernel keeps
+// resident blocks, it cannot fill the device, so every bin of length >= 512
+// goes to k_diag16 too, whose items are (read, 8 references). The remaining
+// bins are compacted into sorted16 order.
+__device__ int32_t block_exclusive_scan(int32_t v, int32_t *part, int32_t *total) {
+    const int t = threadIdx.x;
+    part[t] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int32_t x = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += x;
+        __syncthreads();
+    }
+    const int32_t incl = part[t];
+    *total = part[1023];
+    __syncthreads();
+    return incl - v;
+}
+
 __global__ __launch_bounds__(1024) void k_scan(KArgs a) {
     __shared__ int32_t part[1024];
     constexpr int NB = kMaxLen16 + 1;
     constexpr int PER = (NB + 1023) / 1024;
     const int t = threadIdx.x;
+    // pass 1: rank(L) = reads longer than L, into bin_start
     int32_t sum = 0;
     for (int k = 0; k < PER; ++k) {
-        const int rb = t * PER + k;  // rank in descending order
+        const int rb = t * PER + k;
         if (rb < NB) sum += a.hist[kMaxLen16 - rb];
     }
-    part[t] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int32_t v = t >= off ? part[t - off] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    int32_t run = part[t] - sum;
+    int32_t total;
+    int32_t run = block_exclusive_scan(sum, part, &total);
     for (int k = 0; k < PER; ++k) {
         const int rb = t * PER + k;
         if (rb < NB) {
@@ -726,7 +746,62 @@ __global__ __launch_bounds__(1024) void k_scan(KArgs a) {
             run += a.hist[b];
         }
     }
-    if (t == 1023) a.desc->n16 = part[1023];
+    __syncthreads();
+    __threadfence_block();
+    // pass 2: routing flags (into bin_fill, zeroed again below)
+    const int ratio = a.desc->diag_ratio;
+    const bool adaptive = ratio > 0;
+    const int64_t items16 = (int64_t)((total + kChunk - 1) / kChunk) * ((a.panel.n_refs + 3) / 4);
+    const bool starved = items16 < (int64_t)a.desc->blocks16;
+    int32_t kept = 0;
+    for (int k = 0; k < PER; ++k) {
+        const int rb = t * PER + k;
+        if (rb >= NB) continue;
+        const int L = kMaxLen16 - rb;
+        const int cnt = a.hist[L];
+        int flag = 0;
+        if (adaptive && cnt > 0 && L >= 128) {
+            const int r = a.bin_start[L];
+            int32_t L2;
+            if (r + kChunk - 1 >= total) {
+                // the shortest read: rank(L2) <= total - 1, smallest such L2
+                int lo = 0, hi = L;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (a.bin_start[mid] <= total - 1) hi = mid; else lo = mid + 1;
+                }
+                L2 = lo;
+            } else {
+                const int x = r + kChunk - 1;
+                int lo = 0, hi = L;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (a.bin_start[mid] <= x) hi = mid; else lo = mid + 1;
+                }
+                L2 = lo;
+            }
+            flag = (16 * L > ratio * L2 + 1024) || (starved && L >= 512);
+        }
+        a.bin_fill[L] = flag;
+        if (!flag) kept += cnt;
+    }
+    __syncthreads();
+    // pass 3: compacted prefix over the kept bins; flagged bins get -1
+    int32_t n16;
+    run = block_exclusive_scan(kept, part, &n16);
+    for (int k = 0; k < PER; ++k) {
+        const int rb = t * PER + k;
+        if (rb >= NB) continue;
+        const int b = kMaxLen16 - rb;
+        if (a.bin_fill[b]) {
+            a.bin_start[b] = -1;
+        } else {
+            a.bin_start[b] = run;
+            run += a.hist[b];
+        }
+        a.bin_fill[b] = 0;
+    }
+    if (t == 0) a.desc->n16 = n16;
 }
 
 __global__ void k_scatter(KArgs a) {
@@ -734,7 +809,14 @@ __global__ void k_scatter(KArgs a) {
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
         if (a.cls[q] != 1) continue;
         const int b = a.qlen[q];
-        const int pos = a.bin_start[b] + atomicAdd(&a.bin_fill[b], 1);
+        const int start = a.bin_start[b];
+        if (start < 0) {
+            // routed to the anti-diagonal kernel
+            a.cls[q] = 3;
+            a.listD[atomicAdd(&a.desc->nd, 1)] = q;
+            continue;
+        }
+        const int pos = start + atomicAdd(&a.bin_fill[b], 1);
         a.sorted16[pos] = q;
     }
 }

@@ -85,6 +85,8 @@ struct esch_scorer {
     int max_grid16 = 0;                 // scratch is sized for the largest
     int regs16 = 0, local16 = 0;
     int diag_min_len = 0;               // reads this long take k_diag16 (0: never)
+    int diag_ratio = kDiagRatioDefault; // padding-heavy length bins take k_diag16 (0: never)
+    bool diag_ratio_tunable = true;     // false once the caller set the ratio
     std::mutex pool_mu;
     std::vector<esch_batch *> pool;  // idle batches for the drop-in API
 };
@@ -554,6 +556,8 @@ esch_status esch_batch_run(esch_batch *b, int what, int32_t min_score, int64_t m
     d.max_ties = max_ties;
     d.tstrands = (what & ESCH_RUN_PAIR_BOTH) ? 2 : 1;
     d.diag_min_len = (what & (ESCH_RUN_PLAIN | ESCH_RUN_SCORES16)) ? 0 : s->diag_min_len;
+    d.diag_ratio = (what & ESCH_RUN_PLAIN) ? 0 : s->diag_ratio;
+    d.blocks16 = (int32_t)s->geo.grid16;
     HIPC(hipMemcpyAsync(b->d_desc, b->h_desc, sizeof(DevDesc), hipMemcpyHostToDevice, b->stream));
     b->last_what = what;
     const bool graph = !(what & ESCH_RUN_NO_GRAPH);
@@ -820,6 +824,17 @@ esch_status esch_set_diag_min_len(esch_scorer *s, int32_t min_len) {
 }
 
 int32_t esch_get_diag_min_len(const esch_scorer *s) { return s ? s->diag_min_len : -1; }
+
+esch_status esch_set_diag_ratio(esch_scorer *s, int32_t ratio_x16) {
+    if (!s) return fail(ESCH_INVALID_ARG, "null scorer");
+    if (ratio_x16 != 0 && (ratio_x16 < 16 || ratio_x16 > 1024))
+        return fail(ESCH_INVALID_ARG, "diag ratio must be 0 or 16..1024 (sixteenths)");
+    s->diag_ratio = ratio_x16;
+    s->diag_ratio_tunable = false;
+    return ESCH_OK;
+}
+
+int32_t esch_get_diag_ratio(const esch_scorer *s) { return s ? s->diag_ratio : -1; }
 int esch_n_variants(void) { return kDp16Variants; }
 
 esch_status esch_autotune(esch_scorer *s, const uint8_t *const *reads, const size_t *lens, size_t n,
@@ -830,17 +845,21 @@ esch_status esch_autotune(esch_scorer *s, const uint8_t *const *reads, const siz
     esch_batch *b = nullptr;
     if (esch_status st = take_batch(s, n, total, &b)) return st;
     esch_status st = esch_batch_load(b, reads, lens, n, both, 0);
+    // one untimed pass, then the timed one; DP time of the whole DP stage
+    auto timed = [&](float *out) -> esch_status {
+        for (int rep = 0; rep < 2; ++rep) {
+            if (esch_status e = esch_batch_run(b, ESCH_RUN_SCORES32 | ESCH_RUN_NO_GRAPH, 0, -1, 1)) return e;
+            if (esch_status e = esch_batch_wait(b)) return e;
+        }
+        return esch_batch_timing(b, out, nullptr);
+    };
+    // 1. the inter-sequence kernel's variants, routing as configured
     int best_v = s->geo.variant16;
     float best_ms = 0.0f;
     for (int v = 0; !st && v < kDp16Variants; ++v) {
         st = esch_set_variant(s, v);
-        // one untimed pass, then the timed one
-        for (int rep = 0; !st && rep < 2; ++rep) {
-            st = esch_batch_run(b, ESCH_RUN_SCORES32 | ESCH_RUN_NO_GRAPH, 0, -1, 1);
-            if (!st) st = esch_batch_wait(b);
-        }
         float dp = 0.0f;
-        if (!st) st = esch_batch_timing(b, &dp, nullptr);
+        if (!st) st = timed(&dp);
         if (st) break;
         if (ms) ms[v] = dp;
         if (v == 0 || dp < best_ms) {
@@ -848,10 +867,38 @@ esch_status esch_autotune(esch_scorer *s, const uint8_t *const *reads, const siz
             best_v = v;
         }
     }
+    if (!st) st = esch_set_variant(s, best_v);
+    // 2. the per-length-bucket routing between the inter-sequence and the
+    // anti-diagonal kernel: the ratio rho is the anti-diagonal kernel's
+    // per-cell cost relative to the inter-sequence one, a property of the
+    // two kernels on this panel rather than of the read distribution, so a
+    // sample measures it. Another ratio replaces the default only when it
+    // is faster by more than 2% (a sample without long tails times the same
+    // under every ratio and keeps the default).
+    if (!st && s->diag_ratio_tunable) {
+        static const int32_t kRatios[] = {0, 24, 32, 48, 64};
+        const int32_t base = kDiagRatioDefault;
+        float base_ms = 0.0f;
+        s->diag_ratio = base;
+        st = timed(&base_ms);
+        int32_t best_r = base;
+        float best_r_ms = base_ms;
+        for (int32_t r : kRatios) {
+            if (st || r == base) continue;
+            s->diag_ratio = r;
+            float dp = 0.0f;
+            st = timed(&dp);
+            if (!st && dp < best_r_ms && dp < 0.98f * base_ms) {
+                best_r_ms = dp;
+                best_r = r;
+            }
+        }
+        s->diag_ratio = best_r;
+    }
     give_batch(s, b);
     if (st) return st;
     if (chosen) *chosen = best_v;
-    return esch_set_variant(s, best_v);
+    return ESCH_OK;
 }
 
 esch_status esch_score_batch_i32(esch_scorer *s, const uint8_t *const *queries, const size_t *lens, size_t n,

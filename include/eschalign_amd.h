@@ -180,6 +180,16 @@ esch_status esch_select_pairs(esch_scorer *s, const uint8_t *const *mates1, cons
  * chunks. Identical scores either way. */
 esch_status esch_set_diag_min_len(esch_scorer *s, int32_t min_len);
 int32_t esch_get_diag_min_len(const esch_scorer *s);
+/* Adaptive routing per length bucket (on by default): after bucketing, the
+ * reads of length L take the anti-diagonal kernel when L > rho * L' + 64,
+ * L' the length of the read 127 ranks later (the chunk they would lead),
+ * or when L >= 512 and the batch is too small to fill the device with
+ * inter-sequence work items (128-read chunks x 4 references).
+ * rho = ratio_x16 / 16, default 32 (2.0); 0 turns the routing off;
+ * otherwise 16..1024. esch_autotune sweeps it. Identical scores either
+ * way. */
+esch_status esch_set_diag_ratio(esch_scorer *s, int32_t ratio_x16);
+int32_t esch_get_diag_ratio(const esch_scorer *s);
 
 /* adapter::find — lib/adapter.rs:68-91. Letters (not codes): how many bases
  * at the 3' end of `read` are the adapter's prefix (0 = none). The pre-step
@@ -199,7 +209,11 @@ int esch_n_variants(void);
 int esch_get_variant(const esch_scorer *s);
 esch_status esch_set_variant(esch_scorer *s, int variant);
 /* ms[v] (may be NULL, esch_n_variants() entries) receives each variant's DP
- * time on the sample; the fastest is selected and returned in *chosen. */
+ * time on the sample; the fastest is selected and returned in *chosen.
+ * Then, unless esch_set_diag_ratio fixed it, the per-length-bucket routing
+ * ratio is swept over {off, 1.5, 2, 3, 4} on the same sample; another one
+ * replaces the default 2.0 only when more than 2% faster
+ * (esch_get_diag_ratio reads the result). */
 esch_status esch_autotune(esch_scorer *s, const uint8_t *const *reads, const size_t *lens, size_t n,
                           int both_strands, float *ms, int *chosen);
 

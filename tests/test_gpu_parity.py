@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 import eschalign_amd as ea
+from eschalign_amd import synth
 from oracle import pyoracle as O
 import refgen as G
 
@@ -344,6 +345,7 @@ def test_anti_diagonal_kernel_equals_oracle():
     for scoring in G.SCHEMES_GPU:
         for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
             gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+            gpu.diag_ratio = 0      # only the fixed threshold routes here
             want = oracle(queries, panel.codes, scoring, mode)
             for thr in (1, 1000):
                 gpu.diag_min_len = thr
@@ -356,3 +358,112 @@ def test_anti_diagonal_kernel_equals_oracle():
                 assert bad.size == 0, (scoring, mode, thr, bad[:4], len(queries[bad[0][0]]))
                 assert b.n_diag() == sum(len(q) >= thr for q in queries)
                 b.close()
+
+
+def _routed(gpu, queries, ratio):
+    gpu.diag_ratio = ratio
+    b = gpu.batch(len(queries), sum(len(q) for q in queries))
+    b.load(queries)
+    b.run(ea.RUN_SCORES32)
+    got, ok = b.scores32()
+    nd = b.n_diag()
+    b.close()
+    return got, ok, nd
+
+
+def routed_count(lens, ratio, n_refs, blocks16):
+    """How many reads the per-length-bucket rule routes: a length bin L
+    (>= 128) goes when L > ratio * L' + 64, L' the length of the read 127
+    ranks later (or the shortest read), or when L >= 512 and the batch's
+    inter-sequence work items (128-read chunks x groups of 4 references)
+    are fewer than that kernel's resident blocks."""
+    srt = sorted(lens, reverse=True)
+    total = len(srt)
+    starved = -(-total // 128) * -(-n_refs // 4) < blocks16
+    count = 0
+    for L in set(lens):
+        if L < 128:
+            continue
+        r = sum(1 for x in srt if x > L)
+        L2 = srt[min(r + 127, total - 1)]
+        if 16 * L > round(ratio * 16) * L2 + 1024 or (starved and L >= 512):
+            count += lens.count(L)
+    return count
+
+
+def test_adaptive_routing_per_length_bucket():
+    """The per-length-bucket routing between the inter-sequence and the
+    anti-diagonal kernel: a long tail over many short reads (each long read
+    is over twice the read 127 ranks later), a handful of long reads only
+    (too few chunks to fill the device: all routed) and a small uniform
+    600-1400 batch (starved: the >= 512 bins go). Every routing gives the
+    oracle's scores, and the routed count is the rule's."""
+    rng = G.Rng(0x5EED_0B02)
+    panel_l = G.random_panel(rng, 24, 150, 400)
+    panel = ea.Panel(panel_l)
+    scoring = (1, -1, -2, -1)
+    short = [G.chimeric_read(rng, panel_l, 50 + rng.below(100)) for _ in range(600)]
+    tail = [G.chimeric_read(rng, panel_l, 1500 + rng.below(2500)) for _ in range(12)]
+    few = [G.chimeric_read(rng, panel_l, 2000 + rng.below(3000)) for _ in range(10)]
+    small = [G.chimeric_read(rng, panel_l, 300 + rng.below(1100)) for _ in range(200)]
+    cases = [("tail", codes_of(short + tail)), ("few", codes_of(few)), ("small", codes_of(small))]
+    for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
+        gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+        assert gpu.diag_ratio == 2.0
+        blocks16 = gpu.geometry()[2]
+        for name, queries in cases:
+            lens = [len(q) for q in queries]
+            want = oracle(queries, panel.codes, scoring, mode)
+            off, ok0, nd0 = _routed(gpu, queries, 0)
+            assert ok0.all() and nd0 == 0
+            assert np.array_equal(off, want), (name, mode)
+            for ratio in (2.0, 1.5, 4.0):
+                got, ok, nd = _routed(gpu, queries, ratio)
+                assert ok.all()
+                bad = np.argwhere(got != want)
+                assert bad.size == 0, (name, mode, ratio, bad[:4])
+                assert nd == routed_count(lens, ratio, len(panel_l), blocks16), (name, ratio, nd)
+        assert routed_count([len(q) for q in cases[0][1]], 2.0, len(panel_l), blocks16) == len(tail)
+        assert routed_count([len(q) for q in cases[1][1]], 2.0, len(panel_l), blocks16) == len(few)
+
+
+def test_adaptive_routing_dense_batch_keeps_the_chunks():
+    """A batch large enough to fill the device with 128-read chunks of
+    near-equal lengths keeps them on the inter-sequence kernel and routes
+    only its 10 long outliers; at this size the routed scores equal the
+    unrouted ones (each path pinned to the oracle above)."""
+    panel = synth.panel(seed=0x5EED_0B04, n_refs=164, lo=140, hi=160)
+    reads = synth.reads(panel, seed=0x5EED_0B05, n=20_000, length=("uniform", 600, 900))
+    reads += synth.reads(panel, seed=0x5EED_0B06, n=10, length=("uniform", 2000, 3000))
+    gpu = ea.GpuScorer(ea.Panel(panel), ea.Scoring(1, -1, -2, -1), ea.Mode.SEMIGLOBAL)
+    blocks16 = gpu.geometry()[2]
+    lens = [len(q) for q in reads]
+    assert -(-len(reads) // 128) * -(-164 // 4) >= blocks16
+    off, ok0, _ = _routed(gpu, reads, 0)
+    for ratio in (2.0, 1.0):
+        got, ok, nd = _routed(gpu, reads, ratio)
+        assert ok.all() and ok0.all() and np.array_equal(got, off), ratio
+        assert nd == routed_count(lens, ratio, 164, blocks16), (ratio, nd)
+    assert routed_count(lens, 2.0, 164, blocks16) == 10
+
+
+def test_autotune_sweeps_the_routing_ratio():
+    """The autotuner times the variants, then the routing ratios, on a
+    sample; it leaves an explicitly set ratio alone."""
+    rng = G.Rng(0x5EED_0B03)
+    panel_l = G.random_panel(rng, 40, 150, 400)
+    reads = [G.chimeric_read(rng, panel_l, 50 + rng.below(100)) for _ in range(3000)]
+    reads += [G.chimeric_read(rng, panel_l, 2000 + rng.below(2000)) for _ in range(8)]
+    queries = codes_of(reads)
+    panel = ea.Panel(panel_l)
+    gpu = ea.GpuScorer(panel, ea.Scoring(1, -1, -2, -1), ea.Mode.LOCAL)
+    chosen, ms = gpu.autotune(queries, both_strands=False)
+    assert chosen in range(len(ms)) and all(m > 0 for m in ms)
+    assert gpu.diag_ratio in (0.0, 1.5, 2.0, 3.0, 4.0)
+    gpu.diag_ratio = 3.0
+    gpu.autotune(queries, both_strands=False)
+    assert gpu.diag_ratio == 3.0
+    with pytest.raises(ea.GpuError):
+        gpu.diag_ratio = 0.5
+    got, ok, _ = _routed(gpu, queries, 3.0)
+    assert ok.all() and np.array_equal(got, oracle(queries, panel.codes, (1, -1, -2, -1), ea.Mode.LOCAL))
