@@ -164,6 +164,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-kernel comparison")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving run")
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="batched-kernel variant to run (default: autotune on a sample)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="serving arrivals per second")
     ap.add_argument("--serve-seconds", type=float, default=12.0, help="serving arrival window")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -194,12 +196,16 @@ def main():
 
     panel = ea.Panel(panel_codes)
     gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
-    # autotune the batched kernel on a sample of the workload's own reads
-    n_tune = min(args.reads, 100_000)
-    tune_reads = [codes[int(off[k]):int(off[k]) + int(lens[k])] for k in range(n_tune)]
-    chosen, tune_ms = gpu.autotune(tune_reads, both_strands=True)
-    log(f"[bench] autotune on {n_tune} reads: variant DP ms {tune_ms} -> variant {chosen} "
-        f"({gpu.registers()} VGPRs)")
+    if args.variant >= 0:
+        gpu.variant = args.variant
+        chosen, tune_ms = args.variant, []
+    else:
+        # autotune the batched kernel on a sample of the workload's own reads
+        n_tune = min(args.reads, 100_000)
+        tune_reads = [codes[int(off[k]):int(off[k]) + int(lens[k])] for k in range(n_tune)]
+        chosen, tune_ms = gpu.autotune(tune_reads, both_strands=True)
+        log(f"[bench] autotune on {n_tune} reads: variant DP ms {tune_ms} -> variant {chosen} "
+            f"({gpu.registers()} VGPRs), routing ratio {gpu.diag_ratio}")
     batch = gpu.batch(args.reads, int(lens.sum()))
     batch.load_concat(codes, off, lens, both_strands=True)
     batch.wait()

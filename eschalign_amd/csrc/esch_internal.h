@@ -84,7 +84,7 @@ struct KArgs {
     int32_t *list32;     // int32-kernel queries
     int32_t *out32;      // [n_queries * n_refs] or null
     int16_t *out16;      // [n_queries * n_refs] or null
-    uint2 *scratch16;    // [grid16 * 4 waves * max_len * 64]
+    uint2 *scratch16;    // [grid16 * 4 waves * (max_len + kScrPad) * 64]
     int2 *scratch32;     // [max_len * grid32 threads]
     int32_t *recs;       // select / pair records, 5 int32 each
     int32_t *ties;       // [n * tie_cap]
@@ -94,6 +94,7 @@ struct KArgs {
 };
 // Rows per lane of the anti-diagonal kernel (a pass covers 16 * kDiagRows).
 constexpr int kDiagRows = 8;
+constexpr int kScrPad = 8;  // scratch columns past max_len per wave (prefetch overrun)
 constexpr int kDiagRatioDefault = 32;  // 2.0 in sixteenths
 
 // Host-side launchers (kernels.hip); every kernel runs on `st`.
@@ -102,10 +103,11 @@ struct Launch {
     int grid32;   // persistent blocks of k_dp32
     int block32;  // threads per k_dp32 block
     int grid_aux; // blocks of the bucketing / selection kernels
-    int variant16;  // k_dp16 variant: 0 profile loaded per column, 1 prefetched
+    int variant16;  // k_dp16 variant: 0 profile per column, 1 ping-pong, 2 deep prefetch
     int grid_diag;  // persistent waves of k_diag16
 };
-constexpr int kDp16Variants = 2;
+constexpr int kDp16Variants = 3;
+constexpr int kDp16Default = 2;
 hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st);
 hipError_t launch_dp16(const KArgs &a, const Launch &g, bool local, hipStream_t st);
 hipError_t launch_dp32(const KArgs &a, const Launch &g, bool local, int which, hipStream_t st);

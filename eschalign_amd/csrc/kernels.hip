@@ -190,7 +190,112 @@ __device__ __forceinline__ void stripe16(const uint4 *__restrict__ prof, uint2 *
     }
 }
 
-template <bool LOCAL, bool PF>
+// The same stripe with every memory access of a column pair issued at the
+// start of the pair: the boundary loads of the next pair and the deferred
+// stores of the previous one, then a scheduling barrier, then the two
+// columns. gfx9-class hardware counts loads and stores on one counter, so
+// with both in flight any wait for a loaded value drains the counter; here
+// that wait falls at the end of the pair, two columns after the requests.
+// No data-dependent branch surrounds a memory access: the stores of the
+// pair before column 0 go to two padding columns in front of the wave's
+// scratch, loads past the last column read padding behind it (columns up
+// to m + 1), the profile prefetch reads at most the reference's padding
+// word (column m), and whether the stripe is the first (border instead of
+// scratch) or the last (no stores) is a template argument.
+template <int RR, bool LOCAL, bool FIRST, bool LAST>
+__device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 *__restrict__ scr,
+                                          const int lane, const cwords w2, const cwords wn,
+                                          const int m, const int row0, const int pA, const int pB,
+                                          const v2s vo, const v2s ve, v2s &best) {
+    constexpr int NQ = RR / 4;
+    const v2s NEG = splat(-32768);
+    v2s Ho[RR], E[RR];
+#pragma unroll
+    for (int r = 0; r < RR; ++r) {
+        Ho[r] = vo;
+        E[r] = NEG;
+    }
+    v2s hdt = vo;
+    auto code = [&](int j) -> int {
+        const uint32_t c2 = (w2[j >> 4] >> (2 * (j & 15))) & 3u;
+        return ((wn[j >> 5] >> (j & 31)) & 1u) ? 4 : (int)c2;
+    };
+    auto plane = [&](int c, v2s (&sc)[RR]) {
+        const uint4 *pc = prof + c * 4 * 64 + lane;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const uint4 v = pc[q * 64];
+            sc[4 * q + 0] = as_v2s(v.x);
+            sc[4 * q + 1] = as_v2s(v.y);
+            sc[4 * q + 2] = as_v2s(v.z);
+            sc[4 * q + 3] = as_v2s(v.w);
+        }
+    };
+    uint2 *const s0 = scr + 2 * 64 + lane;  // column 0 (two padding columns in front)
+    const uint2 border = make_uint2(as_u32(vo), as_u32(NEG));
+    auto bnd = [&](int j) -> uint2 {
+        if constexpr (FIRST) return border;
+        else return s0[(size_t)j * 64];
+    };
+    auto col = [&](const v2s (&sc)[RR], uint2 b) -> uint2 {
+        v2s hou = as_v2s(b.x), f = as_v2s(b.y);
+        const v2s hod = hdt;
+        hdt = hou;
+        column16<RR, LOCAL>(sc, Ho, E, hou, f, hod, vo, ve, best);
+        if constexpr (!LOCAL && LAST) best = vmax(best, hou);
+        return make_uint2(as_u32(hou), as_u32(f));
+    };
+
+    v2s sa[RR], sb[RR];
+    plane(code(0), sa);
+    uint2 r0 = bnd(0), r1 = bnd(1);
+    uint2 o0 = border, o1 = border;
+    int j = 0;
+    for (; j + 1 < m; j += 2) {
+        const uint2 q0 = bnd(j + 2), q1 = bnd(j + 3);
+        if constexpr (!LAST) {
+            s0[(ptrdiff_t)(j - 2) * 64] = o0;
+            s0[(ptrdiff_t)(j - 1) * 64] = o1;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        plane(code(j + 1), sb);
+        o0 = col(sa, r0);
+        plane(code(j + 2), sa);
+        o1 = col(sb, r1);
+        __builtin_amdgcn_sched_barrier(0);  // the copies (and their wait) stay at the end
+        r0 = q0;
+        r1 = q1;
+    }
+    if constexpr (!LAST) {
+        s0[(ptrdiff_t)(j - 2) * 64] = o0;
+        s0[(ptrdiff_t)(j - 1) * 64] = o1;
+    }
+    if (j < m) {
+        o0 = col(sa, r0);
+        if constexpr (!LAST) s0[(size_t)j * 64] = o0;
+    }
+
+    if constexpr (!LOCAL) {
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
+            v2s v = Ho[r];
+            if (row0 + r < pA) v.x = (short)-32768;
+            if (row0 + r < pB) v.y = (short)-32768;
+            best = vmax(best, v);
+        }
+    }
+}
+
+template <int RR, bool LOCAL, bool FIRST>
+__device__ __forceinline__ void stripe16d_any(const uint4 *prof, uint2 *scr, const int lane,
+                                              const cwords w2, const cwords wn, const int m,
+                                              const bool last, const int row0, const int pA,
+                                              const int pB, const v2s vo, const v2s ve, v2s &best) {
+    if (last) stripe16d<RR, LOCAL, FIRST, true>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best);
+    else stripe16d<RR, LOCAL, FIRST, false>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best);
+}
+
+template <bool LOCAL, int PF>
 __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
     // prof[(c * 4 + quad) * 64 + lane] = rows 4 quad .. 4 quad + 3 of plane c
     __shared__ uint4 prof[5 * 4 * 64];
@@ -203,7 +308,7 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
     const int n16 = a.desc->n16;
     const int n_chunks = (n16 + kChunk - 1) / kChunk;
     const uint32_t n_items = (uint32_t)n_chunks * (uint32_t)n_groups;
-    uint2 *scr = a.scratch16 + ((size_t)blockIdx.x * kWaves16 + wv) * (size_t)P.max_len * 64;
+    uint2 *scr = a.scratch16 + ((size_t)blockIdx.x * kWaves16 + wv) * (size_t)(P.max_len + kScrPad) * 64;
     const v2s vo = splat(a.sc.o), ve = splat(a.sc.e), NEG = splat(-32768);
 
     for (;;) {
@@ -268,16 +373,31 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
             __syncthreads();
             if (!active) continue;
             const bool first = s == 0, last = s + 1 == S;
-            if (s > 0) {
-                stripe16<16, LOCAL, PF>(prof, scr, lane, w2, wn, m, false, last, row0, pA, pB, vo, ve, best);
-            } else if (RR == 16) {
-                stripe16<16, LOCAL, PF>(prof, scr, lane, w2, wn, m, true, last, row0, pA, pB, vo, ve, best);
-            } else if (RR == 12) {
-                stripe16<12, LOCAL, PF>(prof, scr, lane, w2, wn, m, true, last, row0, pA, pB, vo, ve, best);
-            } else if (RR == 8) {
-                stripe16<8, LOCAL, PF>(prof, scr, lane, w2, wn, m, true, last, row0, pA, pB, vo, ve, best);
+            if constexpr (PF == 2) {
+                if (s > 0) {
+                    stripe16d_any<16, LOCAL, false>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
+                } else if (RR == 16) {
+                    stripe16d_any<16, LOCAL, true>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
+                } else if (RR == 12) {
+                    stripe16d_any<12, LOCAL, true>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
+                } else if (RR == 8) {
+                    stripe16d_any<8, LOCAL, true>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
+                } else {
+                    stripe16d_any<4, LOCAL, true>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
+                }
             } else {
-                stripe16<4, LOCAL, PF>(prof, scr, lane, w2, wn, m, true, last, row0, pA, pB, vo, ve, best);
+                constexpr bool PP = PF == 1;
+                if (s > 0) {
+                    stripe16<16, LOCAL, PP>(prof, scr, lane, w2, wn, m, false, last, row0, pA, pB, vo, ve, best);
+                } else if (RR == 16) {
+                    stripe16<16, LOCAL, PP>(prof, scr, lane, w2, wn, m, true, last, row0, pA, pB, vo, ve, best);
+                } else if (RR == 12) {
+                    stripe16<12, LOCAL, PP>(prof, scr, lane, w2, wn, m, true, last, row0, pA, pB, vo, ve, best);
+                } else if (RR == 8) {
+                    stripe16<8, LOCAL, PP>(prof, scr, lane, w2, wn, m, true, last, row0, pA, pB, vo, ve, best);
+                } else {
+                    stripe16<4, LOCAL, PP>(prof, scr, lane, w2, wn, m, true, last, row0, pA, pB, vo, ve, best);
+                }
             }
             (void)first;
         }
@@ -997,18 +1117,22 @@ hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st) {
 // Kernel variants of k_dp16, chosen per scorer (set by the autotuner).
 template <bool LOCAL>
 static const void *dp16_fn(int variant) {
-    return variant == 1 ? reinterpret_cast<const void *>(&k_dp16<LOCAL, true>)
-                        : reinterpret_cast<const void *>(&k_dp16<LOCAL, false>);
+    if (variant == 2) return reinterpret_cast<const void *>(&k_dp16<LOCAL, 2>);
+    if (variant == 1) return reinterpret_cast<const void *>(&k_dp16<LOCAL, 1>);
+    return reinterpret_cast<const void *>(&k_dp16<LOCAL, 0>);
 }
 
 hipError_t launch_dp16(const KArgs &a, const Launch &g, bool local, hipStream_t st) {
     const int v = g.variant16;
+    const dim3 grid(g.grid16), block(64 * kWaves16);
     if (local) {
-        if (v == 1) hipLaunchKernelGGL((k_dp16<true, true>), dim3(g.grid16), dim3(64 * kWaves16), 0, st, a);
-        else hipLaunchKernelGGL((k_dp16<true, false>), dim3(g.grid16), dim3(64 * kWaves16), 0, st, a);
+        if (v == 2) hipLaunchKernelGGL((k_dp16<true, 2>), grid, block, 0, st, a);
+        else if (v == 1) hipLaunchKernelGGL((k_dp16<true, 1>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((k_dp16<true, 0>), grid, block, 0, st, a);
     } else {
-        if (v == 1) hipLaunchKernelGGL((k_dp16<false, true>), dim3(g.grid16), dim3(64 * kWaves16), 0, st, a);
-        else hipLaunchKernelGGL((k_dp16<false, false>), dim3(g.grid16), dim3(64 * kWaves16), 0, st, a);
+        if (v == 2) hipLaunchKernelGGL((k_dp16<false, 2>), grid, block, 0, st, a);
+        else if (v == 1) hipLaunchKernelGGL((k_dp16<false, 1>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((k_dp16<false, 0>), grid, block, 0, st, a);
     }
     return hipGetLastError();
 }

@@ -260,10 +260,12 @@ esch_status esch_scorer_new(int ordinal, const uint8_t *const *refs, const size_
     int b32 = 0;
     int block32 = (int)std::min<size_t>(256, ((n_refs + 63) / 64) * 64);
     HIPC(dp32_occupancy(local, block32, &b32));
-    s->geo.variant16 = 0;
-    s->geo.grid16 = s->grid16_v[0];
-    s->regs16 = s->regs16_v[0];
-    s->local16 = s->local16_v[0];
+    // default: the deep-prefetch variant (the fastest on the measured
+    // workloads); esch_autotune re-decides on the caller's reads
+    s->geo.variant16 = kDp16Default;
+    s->geo.grid16 = s->grid16_v[kDp16Default];
+    s->regs16 = s->regs16_v[kDp16Default];
+    s->local16 = s->local16_v[kDp16Default];
     s->geo.block32 = block32;
     s->geo.grid32 = std::max(1, b32) * s->n_cu;
     s->geo.grid_aux = 4 * s->n_cu;
@@ -322,7 +324,7 @@ size_t esch_batch_footprint(const esch_scorer *s, size_t max_reads, size_t max_b
     const size_t bases = 2 * std::max<size_t>(max_bases, 1);
     size_t f = bases + rows * (8 + 4 + 1 + 4 + 4) + 3 * 4 * (kMaxLen16 + 1) + sizeof(DevDesc);
     f += rows * s->n_refs * (4 + 2);                                     // out32 + out16
-    f += (size_t)s->max_grid16 * 4 * s->max_len * 64 * sizeof(uint2);        // scratch16
+    f += (size_t)s->max_grid16 * 4 * (s->max_len + kScrPad) * 64 * sizeof(uint2);  // scratch16
     f += (size_t)s->geo.grid32 * s->geo.block32 * s->max_len * sizeof(int2);  // scratch32
     f += rows * 5 * 4;                                                   // records
     f += rows * 4 + (size_t)s->geo.grid_diag * 4 * s->max_len * sizeof(uint2);  // anti-diagonal
@@ -354,7 +356,7 @@ esch_status esch_batch_create(esch_scorer *s, size_t max_reads, size_t max_bases
     HIPC(hipMalloc(&b->d_list32, rows * 4));
     HIPC(hipMalloc(&b->d_out32, rows * s->n_refs * 4));
     HIPC(hipMalloc(&b->d_out16, rows * s->n_refs * 2));
-    HIPC(hipMalloc(&b->d_scr16, (size_t)s->max_grid16 * 4 * s->max_len * 64 * sizeof(uint2)));
+    HIPC(hipMalloc(&b->d_scr16, (size_t)s->max_grid16 * 4 * (s->max_len + kScrPad) * 64 * sizeof(uint2)));
     HIPC(hipMalloc(&b->d_scr32, (size_t)s->geo.grid32 * s->geo.block32 * s->max_len * sizeof(int2)));
     HIPC(hipMalloc(&b->d_recs, rows * 5 * 4));
     HIPC(hipMalloc(&b->d_listD, rows * 4));

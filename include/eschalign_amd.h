@@ -201,10 +201,14 @@ void esch_adapter_detect_pair(const uint8_t *r1, size_t l1, const uint8_t *r2, s
                               const uint8_t *a1, size_t la1, const uint8_t *a2, size_t la2,
                               size_t min_overlap, size_t out[2]);
 
-/* Kernel variants of the batched DP (register tiling / software pipelining
- * of the profile loads), and the autotuner that picks one by timing each
- * on a sample of the caller's reads (no counterpart in the reference; the
- * scores are identical whichever variant runs). */
+/* Kernel variants of the batched DP, and the autotuner that picks one by
+ * timing each on a sample of the caller's reads (no counterpart in the
+ * reference; the scores are identical whichever variant runs):
+ *   0  profile plane loaded in the column that uses it (fewest registers);
+ *   1  profile planes ping-ponged, stripe boundary prefetched;
+ *   2  (default) every memory access of a column pair issued at its start,
+ *      stores deferred by one pair, no data-dependent branch around a
+ *      memory access. */
 int esch_n_variants(void);
 int esch_get_variant(const esch_scorer *s);
 esch_status esch_set_variant(esch_scorer *s, int variant);

@@ -331,6 +331,29 @@ def test_every_kernel_variant_equals_oracle_and_autotune_picks_one():
             assert gpu.variant == chosen and all(t > 0 for t in ms)
 
 
+def test_every_kernel_variant_on_short_references():
+    """References of every length 1..40 (column counts that leave 0-3
+    columns after the paired / prefetching loops, the reference's padding
+    word read by the profile prefetch) against reads 1..300, every variant,
+    both modes, against the oracle."""
+    rng = G.Rng(0x5EED_0A02)
+    panel_l = [G.random_seq(rng, n, 0.05) for n in range(1, 41)]
+    panel = ea.Panel(panel_l)
+    reads = [G.random_seq(rng, 1 + rng.below(300), 0.03) for _ in range(200)]
+    queries = codes_of(reads)
+    for scoring in ((2, -1, -10, -1), (1, -1, -2, -1)):
+        for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
+            gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+            gpu.diag_ratio = 0
+            want = oracle(queries, panel.codes, scoring, mode)
+            for v in range(ea.load_library().esch_n_variants()):
+                gpu.variant = v
+                s32, ok = gpu.score_batch_i32(queries)
+                assert ok.all()
+                bad = np.argwhere(s32 != want)
+                assert bad.size == 0, (scoring, mode, v, bad[:4])
+
+
 def test_anti_diagonal_kernel_equals_oracle():
     """The anti-diagonal wavefront kernel, forced for every read (threshold 1)
     and for long reads only: lengths 1..3000 (one to several 128-row passes),
