@@ -65,7 +65,7 @@ struct DevDesc {
     uint32_t nextD;      // its work-queue cursor
     int32_t diag_ratio;  // adaptive routing to the anti-diagonal kernel, x16 (0: off)
     int32_t blocks16;    // resident blocks of the inter-sequence kernel's grid
-    uint32_t pad_[1];
+    int32_t max16;       // longest query on the int16 path (bins above it are empty)
     uint64_t cells;      // sum of len x panel bases over scored queries
 };
 

@@ -741,6 +741,7 @@ __global__ void k_zero(KArgs a) {
         a.desc->next16 = 0;
         a.desc->next32a = 0;
         a.desc->next32b = 0;
+        a.desc->max16 = 0;
         a.desc->cells = 0;
     }
 }
@@ -777,6 +778,7 @@ __global__ void k_classify(KArgs a) {
     const int64_t L = a.panel.max_len;
     const int64_t M = max(abs(a.sc.ma), abs(a.sc.mi));
     uint64_t cells = 0;
+    int max16 = 0;
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
         const int n = a.qlen[q];
         uint8_t c = 0;
@@ -797,6 +799,7 @@ __global__ void k_classify(KArgs a) {
             a.listD[pos] = q;
         } else if (c == 1) {
             atomicAdd(&a.hist[n], 1);
+            max16 = max(max16, n);
         } else if (c == 2) {
             const int pos = atomicAdd(&a.desc->n32, 1);
             a.list32[pos] = q;
@@ -810,14 +813,19 @@ __global__ void k_classify(KArgs a) {
         }
         if (c) cells += (uint64_t)n;
     }
-    // one atomic per wave for the cell count
-    for (int off = 32; off > 0; off >>= 1) cells += __shfl_down(cells, off, 64);
+    // one atomic per wave for the cell count and the longest int16 query
+    for (int off = 32; off > 0; off >>= 1) {
+        cells += __shfl_down(cells, off, 64);
+        max16 = max(max16, __shfl_down(max16, off, 64));
+    }
     if ((threadIdx.x & 63) == 0 && cells) atomicAdd((unsigned long long *)&a.desc->cells,
                                                     (unsigned long long)(cells * (uint64_t)a.panel.total_len));
+    if ((threadIdx.x & 63) == 0 && max16) atomicMax(&a.desc->max16, max16);
 }
 
-// Exclusive prefix over the length bins in DESCENDING length, one block of
-// 1024 threads, with the adaptive routing to the anti-diagonal kernel.
+// Exclusive prefix over the length bins in DESCENDING length (up to the
+// longest int16 query), one block of 1024 threads, with the adaptive
+// routing to the anti-diagonal kernel.
 //
 // rank(L) = reads longer than L. A read of length L would share its chunk
 // with the reads down to rank(L) + 127, of length L'. When L > rho L' + 64
@@ -847,79 +855,63 @@ __device__ int32_t block_exclusive_scan(int32_t v, int32_t *part, int32_t *total
 
 __global__ __launch_bounds__(1024) void k_scan(KArgs a) {
     __shared__ int32_t part[1024];
-    constexpr int NB = kMaxLen16 + 1;
-    constexpr int PER = (NB + 1023) / 1024;
+    // bins 0 .. top (longer ones are empty); thread t owns PER consecutive
+    // bins top - t PER, top - t PER - 1, ... (one bin each up to 1024)
+    const int top = a.desc->max16;
+    const int PER = (top + 1 + 1023) / 1024;
     const int t = threadIdx.x;
-    // pass 1: rank(L) = reads longer than L, into bin_start
+    const int b0 = top - t * PER;
+    const int nb = max(0, min(PER, b0 + 1));  // this thread's bins
     int32_t sum = 0;
-    for (int k = 0; k < PER; ++k) {
-        const int rb = t * PER + k;
-        if (rb < NB) sum += a.hist[kMaxLen16 - rb];
-    }
+    for (int k = 0; k < nb; ++k) sum += a.hist[b0 - k];
     int32_t total;
     int32_t run = block_exclusive_scan(sum, part, &total);
-    for (int k = 0; k < PER; ++k) {
-        const int rb = t * PER + k;
-        if (rb < NB) {
-            const int b = kMaxLen16 - rb;
-            a.bin_start[b] = run;
-            run += a.hist[b];
+    // rank(L) = reads longer than L (the routing test reads other threads'
+    // bins)
+    {
+        int32_t r = run;
+        for (int k = 0; k < nb; ++k) {
+            a.bin_start[b0 - k] = r;
+            r += a.hist[b0 - k];
         }
     }
     __syncthreads();
-    __threadfence_block();
-    // pass 2: routing flags (into bin_fill, zeroed again below)
     const int ratio = a.desc->diag_ratio;
-    const bool adaptive = ratio > 0;
     const int64_t items16 = (int64_t)((total + kChunk - 1) / kChunk) * ((a.panel.n_refs + 3) / 4);
     const bool starved = items16 < (int64_t)a.desc->blocks16;
-    int32_t kept = 0;
-    for (int k = 0; k < PER; ++k) {
-        const int rb = t * PER + k;
-        if (rb >= NB) continue;
-        const int L = kMaxLen16 - rb;
-        const int cnt = a.hist[L];
-        int flag = 0;
-        if (adaptive && cnt > 0 && L >= 128) {
+    // routing flags into bin_fill (k_zero cleared it; cleared again below)
+    int32_t kept = sum;
+    if (ratio > 0) {
+        for (int k = 0; k < nb; ++k) {
+            const int L = b0 - k;
+            const int cnt = a.hist[L];
+            if (cnt == 0 || L < 128) continue;
+            // the read at rank x (the chunk's last) has length L' and
+            // 16 L > ratio L' + 1024  <=>  L' <= Lt = (16 L - 1025) / ratio
+            //                         <=>  reads longer than Lt <= x
             const int r = a.bin_start[L];
-            int32_t L2;
-            if (r + kChunk - 1 >= total) {
-                // the shortest read: rank(L2) <= total - 1, smallest such L2
-                int lo = 0, hi = L;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (a.bin_start[mid] <= total - 1) hi = mid; else lo = mid + 1;
-                }
-                L2 = lo;
-            } else {
-                const int x = r + kChunk - 1;
-                int lo = 0, hi = L;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (a.bin_start[mid] <= x) hi = mid; else lo = mid + 1;
-                }
-                L2 = lo;
+            const int x = min(r + kChunk - 1, total - 1);
+            const int num = 16 * L - 1025;
+            const bool thin = num >= 0 && a.bin_start[num / ratio] <= x;  // num / ratio < L
+            if (thin || (starved && L >= 512)) {
+                a.bin_fill[L] = 1;
+                kept -= cnt;
             }
-            flag = (16 * L > ratio * L2 + 1024) || (starved && L >= 512);
         }
-        a.bin_fill[L] = flag;
-        if (!flag) kept += cnt;
     }
     __syncthreads();
-    // pass 3: compacted prefix over the kept bins; flagged bins get -1
+    // compacted prefix over the kept bins; routed bins get -1
     int32_t n16;
     run = block_exclusive_scan(kept, part, &n16);
-    for (int k = 0; k < PER; ++k) {
-        const int rb = t * PER + k;
-        if (rb >= NB) continue;
-        const int b = kMaxLen16 - rb;
-        if (a.bin_fill[b]) {
+    for (int k = 0; k < nb; ++k) {
+        const int b = b0 - k;
+        if (ratio > 0 && a.bin_fill[b]) {
             a.bin_start[b] = -1;
+            a.bin_fill[b] = 0;
         } else {
             a.bin_start[b] = run;
             run += a.hist[b];
         }
-        a.bin_fill[b] = 0;
     }
     if (t == 0) a.desc->n16 = n16;
 }
