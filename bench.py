@@ -80,6 +80,42 @@ def cpu_baseline(panel_codes, codes, off, lens, scoring, local, gpu_check):
     }
 
 
+def parse_lengths(text):
+    """'lognormal:MEDIAN:SIGMA:LO:HI' or 'uniform:LO:HI' -> synth length spec."""
+    kind, *vals = text.split(":")
+    if kind == "lognormal" and len(vals) == 4:
+        return ("lognormal", float(vals[0]), float(vals[1]), int(vals[2]), int(vals[3]))
+    if kind == "uniform" and len(vals) == 2:
+        return ("uniform", int(vals[0]), int(vals[1]))
+    raise SystemExit(f"bad --lengths {text!r}")
+
+
+def describe_lengths(length):
+    if length[0] == "lognormal":
+        return f"lognormal(median {length[1]:g}, sigma {length[2]:g}) clipped {length[3]}-{length[4]}"
+    return f"uniform {length[1]}-{length[2]}"
+
+
+def recorded_traffic(mode, reads, variant, lengths):
+    """HBM bytes per k_dp16 launch recorded from rocprofv3 counter runs
+    (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter reads for a
+    device copy of known size) in the committed profile summaries, for the
+    same workload (mode, read count, kernel variant, length distribution);
+    (None, None) when no summary matches. Counters cannot be read from
+    inside this process."""
+    import glob
+    import re
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as f:
+            text = f.read()
+        for m in re.finditer(r"<!-- bench-traffic (\{.*?\}) -->", text):
+            d = json.loads(m.group(1))
+            key = (d.get("mode"), d.get("reads"), d.get("variant"), d.get("lengths", "lognormal:90:0.6:20:1000"))
+            if key == (mode, reads, variant, lengths):
+                return int(d["fetch_bytes"]) + int(d["write_bytes"]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def calibrate_copy(nbytes):
     """One hipMemcpy device-to-device of nbytes (the runtime's copy kernel),
     the known-size reference for rocprof's HBM byte counters."""
@@ -164,6 +200,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-kernel comparison")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving run")
+    ap.add_argument("--lengths", default="lognormal:90:0.6:20:1000",
+                    help="read lengths: lognormal:MEDIAN:SIGMA:LO:HI or uniform:LO:HI")
+    ap.add_argument("--diag-ratio", type=float, default=None,
+                    help="per-length-bucket routing ratio to the anti-diagonal kernel "
+                         "(0 = off; default: the library's, or the autotuner's)")
     ap.add_argument("--variant", type=int, default=-1,
                     help="batched-kernel variant to run (default: autotune on a sample)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="serving arrivals per second")
@@ -188,14 +229,16 @@ def main():
     local = mode == ea.Mode.LOCAL
     t0 = time.perf_counter()
     panel_codes = synth.panel(seed=0x0123456789ABCDEF, n_refs=164, lo=140, hi=160)
-    codes, off, lens = synth.read_arrays(panel_codes, seed=0x5EED_2002, n=args.reads,
-                                         length=("lognormal", 90, 0.6, 20, 1000))
+    length = parse_lengths(args.lengths)
+    codes, off, lens = synth.read_arrays(panel_codes, seed=0x5EED_2002, n=args.reads, length=length)
     log(f"[bench] workload: {args.reads} reads, mean {lens.mean():.1f} nt, panel "
         f"{len(panel_codes)} refs / {sum(len(p) for p in panel_codes)} bases "
         f"({time.perf_counter() - t0:.1f} s to generate)")
 
     panel = ea.Panel(panel_codes)
     gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+    if args.diag_ratio is not None:
+        gpu.diag_ratio = args.diag_ratio
     if args.variant >= 0:
         gpu.variant = args.variant
         chosen, tune_ms = args.variant, []
@@ -227,6 +270,7 @@ def main():
     batch.wait()
     wall = time.perf_counter() - t
     cells, n16, n32 = batch.stats()
+    n_diag = batch.n_diag()
     ms_step = wall / args.steps * 1e3
     value = args.reads / (wall / args.steps)
     dp = float(np.mean(dp_ms)) / 1e3
@@ -234,7 +278,7 @@ def main():
     ops = OPS_PER_CELL[args.mode]
     achieved = cells * ops / dp
     log(f"[bench] {value:,.0f} reads/s, {ms_step:.1f} ms/step, DP {dp * 1e3:.1f} ms "
-        f"({gcups:,.0f} GCUPS), {n16} int16 / {n32} int32 queries")
+        f"({gcups:,.0f} GCUPS), {n16} int16 / {n_diag} anti-diagonal / {n32} int32 queries")
 
     plain = None
     if not args.no_plain:
@@ -264,6 +308,7 @@ def main():
         serve = serving(args.serve_rate, args.serve_seconds)
         log(f"[bench] serving: {serve}")
 
+    traffic, traffic_src = recorded_traffic(args.mode, args.reads, chosen, args.lengths)
     line = {
         "metric": "reads/s + GCUPS, bit-exact scores vs CPU; % VALU roofline; p50/p99 latency",
         "value": round(value, 1),
@@ -278,18 +323,23 @@ def main():
         "dtype": "i16x2 (int32 rescue)",
         "data": "synthetic",
         "config": {"workload": "batched uneven lengths: 164-ref panel (140-160 nt), "
-                               f"{args.reads} reads lognormal(median 90, sigma 0.6) clipped 20-1000, "
+                               f"{args.reads} reads {describe_lengths(length)}, "
                                f"both strands, {args.mode}, scoring 1,-1,-2,-1",
                    "reads": args.reads, "queries": 2 * args.reads, "n_refs": len(panel),
+                   "queries_int16": int(n16), "queries_anti_diagonal": int(n_diag),
+                   "queries_int32": int(n32),
                    "parallelism": "single GPU"},
         "gcups": round(gcups, 1),
         "dp_ms": round(dp * 1e3, 3),
         "cells": int(cells),
         "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3),
                      "peak": round(PEAK_I16 / 1e12, 3), "unit": "Tlane-op/s (int16)",
-                     "frac": round(achieved / PEAK_I16, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_I16, 4), "traffic": traffic,
+                     "traffic_unit": "B per k_dp16 launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE)",
+                     "traffic_source": traffic_src,
                      "ops_per_cell": ops},
         "kernel_variant": chosen,
+        "diag_ratio": gpu.diag_ratio,
         "autotune_ms": [round(x, 3) for x in tune_ms],
         "plain_kernel": plain,
         "cpu_baseline": cpu,

@@ -94,6 +94,8 @@ struct KArgs {
 };
 // Rows per lane of the anti-diagonal kernel (a pass covers 16 * kDiagRows).
 constexpr int kDiagRows = 8;
+constexpr int kDiag64MaxLen = 4096;  // longest reference k_diag64 keeps in LDS
+constexpr int kDiag64Pad = 192;      // LDS columns past max_len (block reads ahead)
 constexpr int kScrPad = 8;  // scratch columns past max_len per wave (prefetch overrun)
 constexpr int kDiagRatioDefault = 32;  // 2.0 in sixteenths
 
@@ -104,7 +106,8 @@ struct Launch {
     int block32;  // threads per k_dp32 block
     int grid_aux; // blocks of the bucketing / selection kernels
     int variant16;  // k_dp16 variant: 0 profile per column, 1 ping-pong, 2 deep prefetch
-    int grid_diag;  // persistent waves of k_diag16
+    int grid_diag;  // persistent waves of the anti-diagonal kernel
+    size_t diag_lds;  // k_diag64's LDS boundary bytes; 0: k_diag16 (global scratch)
 };
 constexpr int kDp16Variants = 3;
 constexpr int kDp16Default = 2;
@@ -113,7 +116,7 @@ hipError_t launch_dp16(const KArgs &a, const Launch &g, bool local, hipStream_t 
 hipError_t launch_dp32(const KArgs &a, const Launch &g, bool local, int which, hipStream_t st);
 hipError_t launch_select(const KArgs &a, const Launch &g, hipStream_t st);
 hipError_t launch_diag(const KArgs &a, const Launch &g, bool local, hipStream_t st);
-hipError_t diag_occupancy(bool local, int *blocks_per_cu);
+hipError_t diag_occupancy(bool local, size_t lds, int *blocks_per_cu);
 hipError_t launch_pairs(const KArgs &a, const Launch &g, hipStream_t st);
 hipError_t dp16_attributes(bool local, int variant, hipFuncAttributes *attr);
 hipError_t dp32_attributes(bool local, hipFuncAttributes *attr);

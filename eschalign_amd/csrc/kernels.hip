@@ -460,10 +460,13 @@ __global__ __launch_bounds__(64) void k_diag16(KArgs a) {
     const int nval_real = a.sc.ma - a.sc.o, nval_pre = -a.sc.o;
     constexpr int RP = 16 * R;
 
+    __shared__ uint32_t s_item;
     for (;;) {
-        uint32_t t0 = 0;
-        if (lane == 0) t0 = atomicAdd(&a.desc->nextD, 1u);
-        t0 = (uint32_t)uni((int)t0);
+        // the item index through LDS between barriers (see k_diag64)
+        __syncthreads();
+        if (lane == 0) s_item = atomicAdd(&a.desc->nextD, 1u);
+        __syncthreads();
+        const uint32_t t0 = (uint32_t)uni((int)s_item);
         if (t0 >= n_items) break;
         const int qi = (int)(t0 / (uint32_t)n_rg);
         const int g = (int)(t0 - (uint32_t)qi * (uint32_t)n_rg);
@@ -600,6 +603,204 @@ __global__ __launch_bounds__(64) void k_diag16(KArgs a) {
                 if (a.out16) a.out16[o] = (int16_t)(by - a.sc.o);
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_diag64 — the anti-diagonal DP with the whole wave as one row.
+//
+// One wave scores one read against TWO references (int16 halves). The 64
+// lanes form a single anti-diagonal: lane l holds R consecutive rows of the
+// pass (64 R rows: 128, 256 or 512 by the read's length) and processes
+// column j at step j + l, taking the bottom row (Ho, F) of lane l - 1 and
+// the column's reference selector with a `wave_shr:1` DPP move. Compared
+// with k_diag16 (four 16-lane rows per wave, one task each) a task needs a
+// quarter of the passes, which is what sets the latency of a few long
+// reads. The pass boundary (the pass's bottom row per column) stays in LDS:
+// lane 63 writes column j at step j + 63, and lane 0 reads it in the next
+// pass from a 64-column block that the whole wave loads one block ahead
+// (lane c holds column 64 b + c; v_readlane picks the step's column and
+// v_writelane puts it into lane 0). Columns are written and read in
+// program order by the one wave, so the single buffer is updated in place.
+// ---------------------------------------------------------------------------
+
+template <int R, bool LOCAL>
+__device__ __forceinline__ void diag64_task(const KArgs &a, const int q, const int n, const int refA,
+                                            const int refB, uint2 *__restrict__ lb, const int lane) {
+    const DevPanel &P = a.panel;
+    const v2s vo = splat(a.sc.o), ve = splat(a.sc.e), NEG = splat(-32768);
+    const int nval_real = a.sc.ma - a.sc.o, nval_pre = -a.sc.o;
+    constexpr int RP = 64 * R;
+    const int mA = refA >= 0 ? uni(P.rlen[refA]) : 0;
+    const int mB = refB >= 0 ? uni(P.rlen[refB]) : 0;
+    const int M = max(mA, mB);
+    const uint8_t *qc = a.qcodes + a.qoff[q];
+    const cwords w2A = (cwords)(P.ref2 + (refA >= 0 ? uni((int)P.w2off[refA]) : 0));
+    const cwords wnA = (cwords)(P.refN + (refA >= 0 ? uni((int)P.nmoff[refA]) : 0));
+    const cwords w2B = (cwords)(P.ref2 + (refB >= 0 ? uni((int)P.w2off[refB]) : 0));
+    const cwords wnB = (cwords)(P.refN + (refB >= 0 ? uni((int)P.nmoff[refB]) : 0));
+    const int Pn = (n + RP - 1) / RP;
+    const int pre = Pn * RP - n;
+    v2s best = LOCAL ? vo : NEG;  // Ho space
+    const uint32_t bh0 = as_u32(vo), bf0 = as_u32(NEG);
+
+    for (int p = 0; p < Pn; ++p) {
+        uint32_t tab0[R], tab1[R];
+        uint32_t prebits = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = p * RP + lane * R + r;
+            const int x = i < pre ? (int)kPreCode : (int)qc[i - pre];
+            prebits |= (i < pre ? 1u : 0u) << r;
+            tab0[r] = pack2(prof_val(x, 0, a.sc), prof_val(x, 1, a.sc));
+            tab1[r] = pack2(prof_val(x, 2, a.sc), prof_val(x, 3, a.sc));
+        }
+        v2s Ho[R], E[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            Ho[r] = vo;
+            E[r] = NEG;
+        }
+        const bool first = p == 0, lastpass = p + 1 == Pn;
+        uint32_t bot_h = bh0, bot_f = bf0, sel_prev = 0, nb_prev = 0;
+        uint32_t in_prev = bh0;  // Ho above-left (column -1: border)
+        uint32_t wA = 0, wB = 0, nA = 0, nB = 0;
+        // the previous pass's boundary, columns 64 b + lane (b = current block)
+        uint2 cur = make_uint2(bh0, bf0), nxt = make_uint2(bh0, bf0);
+        if (!first) {
+            cur = lb[lane];
+            nxt = lb[64 + lane];
+        }
+        for (int t = 0; t < M + 63; ++t) {
+            if ((t & 63) == 0 && t > 0 && !first) {
+                cur = nxt;
+                nxt = lb[t + 64 + lane];  // written by the previous pass only (this pass is at t - 63)
+            }
+            if ((t & 15) == 0) {
+                if (t < mA) wA = w2A[t >> 4];
+                if (t < mB) wB = w2B[t >> 4];
+            }
+            if ((t & 31) == 0) {
+                if (t < mA) nA = wnA[t >> 5];
+                if (t < mB) nB = wnB[t >> 5];
+            }
+            uint32_t sel0 = 0x0C0C0C0Cu, nb0 = 0;
+            if (t < M) {
+                const int cA = t < mA ? (int)((wA >> (2 * (t & 15))) & 3u) : 4;
+                const int cB = t < mB ? (int)((wB >> (2 * (t & 15))) & 3u) : 4;
+                const bool NA = t < mA && ((nA >> (t & 31)) & 1u);
+                const bool NB = t < mB && ((nB >> (t & 31)) & 1u);
+                sel0 = half_sel(NA ? 4 : cA) | (half_sel(NB ? 4 : cB) << 16);
+                nb0 = (NA ? 1u : 0u) | (NB ? 2u : 0u);
+            }
+            // lane 0's boundary for column t: the block's lane t & 63
+            uint32_t h0 = bh0, f0 = bf0;
+            if (!first && t < M) {
+                h0 = (uint32_t)__builtin_amdgcn_readlane((int)cur.x, t & 63);
+                f0 = (uint32_t)__builtin_amdgcn_readlane((int)cur.y, t & 63);
+            }
+            // shift the wavefront one lane down the wave; lane 0 keeps `old`
+            const uint32_t hin = (uint32_t)__builtin_amdgcn_update_dpp((int)h0, (int)bot_h, 0x138, 0xF, 0xF, false);
+            const uint32_t fin = (uint32_t)__builtin_amdgcn_update_dpp((int)f0, (int)bot_f, 0x138, 0xF, 0xF, false);
+            const uint32_t sel = (uint32_t)__builtin_amdgcn_update_dpp((int)sel0, (int)sel_prev, 0x138, 0xF, 0xF, false);
+            const uint32_t nb = (uint32_t)__builtin_amdgcn_update_dpp((int)nb0, (int)nb_prev, 0x138, 0xF, 0xF, false);
+            const uint32_t hdiag = in_prev;
+            in_prev = hin;
+            sel_prev = sel;
+            nb_prev = nb;
+            const int j = t - lane;
+            if (j >= 0 && j < M) {
+                v2s sc[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) sc[r] = as_v2s(__builtin_amdgcn_perm(tab1[r], tab0[r], sel));
+                if (nb) {
+                    // wildcard reference base: a match for every real row
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const short nv = (short)(((prebits >> r) & 1u) ? nval_pre : nval_real);
+                        if (nb & 1u) sc[r].x = nv;
+                        if (nb & 2u) sc[r].y = nv;
+                    }
+                }
+                v2s hou = as_v2s(hin), f = as_v2s(fin);
+                column16<R, LOCAL>(sc, Ho, E, hou, f, as_v2s(hdiag), vo, ve, best);
+                bot_h = as_u32(hou);
+                bot_f = as_u32(f);
+                if (lane == 63 && !lastpass) lb[j] = make_uint2(bot_h, bot_f);
+                if (!LOCAL) {
+                    if (lastpass && lane == 63) {
+                        // the read's last row (register R - 1 of lane 63, last pass)
+                        v2s v = hou;
+                        if (j >= mA) v.x = (short)-32768;
+                        if (j >= mB) v.y = (short)-32768;
+                        best = vmax(best, v);
+                    }
+                    if (j == mA - 1 || j == mB - 1) {
+                        // a reference's last column: every real row
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            v2s v = Ho[r];
+                            const bool real = !((prebits >> r) & 1u);
+                            if (!(real && j == mA - 1)) v.x = (short)-32768;
+                            if (!(real && j == mB - 1)) v.y = (short)-32768;
+                            best = vmax(best, v);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    // the task's best over the wave
+    int bx = best.x, by = best.y;
+    for (int off = 32; off > 0; off >>= 1) {
+        bx = max(bx, __shfl_xor(bx, off, 64));
+        by = max(by, __shfl_xor(by, off, 64));
+    }
+    if (lane == 0) {
+        const int n_refs = P.n_refs;
+        if (refA >= 0) {
+            const size_t o = (size_t)q * n_refs + refA;
+            if (a.out32) a.out32[o] = bx - a.sc.o;
+            if (a.out16) a.out16[o] = (int16_t)(bx - a.sc.o);
+        }
+        if (refB >= 0) {
+            const size_t o = (size_t)q * n_refs + refB;
+            if (a.out32) a.out32[o] = by - a.sc.o;
+            if (a.out16) a.out16[o] = (int16_t)(by - a.sc.o);
+        }
+    }
+}
+
+// Work item t = (read qi of listD, pair g of length-adjacent references);
+// dynamic LDS: (max_len + kDiag64Pad) columns of 8 B.
+template <bool LOCAL>
+__global__ __launch_bounds__(64) void k_diag64(KArgs a) {
+    extern __shared__ uint2 lb[];
+    __shared__ uint32_t s_item;
+    const int lane = threadIdx.x;
+    const DevPanel &P = a.panel;
+    const int n_refs = P.n_refs;
+    const int n_pairs = (n_refs + 1) / 2;
+    const int nd = uni(a.desc->nd);
+    const uint32_t n_items = (uint32_t)nd * (uint32_t)n_pairs;
+    for (;;) {
+        // the item index goes through LDS between two barriers (as in
+        // k_dp16): every lane reads it after lane 0 fetched it, whatever
+        // the compiler does with the lane-0 branches around the loop edge
+        __syncthreads();
+        if (lane == 0) s_item = atomicAdd(&a.desc->nextD, 1u);
+        __syncthreads();
+        const uint32_t t0 = (uint32_t)uni((int)s_item);
+        if (t0 >= n_items) break;
+        const int qi = (int)(t0 / (uint32_t)n_pairs);
+        const int g = (int)(t0 - (uint32_t)qi * (uint32_t)n_pairs);
+        const int q = uni(a.listD[qi]);
+        const int n = uni(a.qlen[q]);
+        const int refA = uni(P.rorder[2 * g]);
+        const int refB = 2 * g + 1 < n_refs ? uni(P.rorder[2 * g + 1]) : -1;
+        if (n <= 128) diag64_task<2, LOCAL>(a, q, n, refA, refB, lb, lane);
+        else if (n <= 256) diag64_task<4, LOCAL>(a, q, n, refA, refB, lb, lane);
+        else diag64_task<8, LOCAL>(a, q, n, refA, refB, lb, lane);
     }
 }
 
@@ -1136,12 +1337,20 @@ hipError_t launch_dp32(const KArgs &a, const Launch &g, bool local, int which, h
 }
 
 hipError_t launch_diag(const KArgs &a, const Launch &g, bool local, hipStream_t st) {
-    if (local) hipLaunchKernelGGL((k_diag16<kDiagRows, true>), dim3(g.grid_diag), dim3(64), 0, st, a);
-    else hipLaunchKernelGGL((k_diag16<kDiagRows, false>), dim3(g.grid_diag), dim3(64), 0, st, a);
+    if (g.diag_lds) {
+        if (local) hipLaunchKernelGGL((k_diag64<true>), dim3(g.grid_diag), dim3(64), g.diag_lds, st, a);
+        else hipLaunchKernelGGL((k_diag64<false>), dim3(g.grid_diag), dim3(64), g.diag_lds, st, a);
+    } else {
+        if (local) hipLaunchKernelGGL((k_diag16<kDiagRows, true>), dim3(g.grid_diag), dim3(64), 0, st, a);
+        else hipLaunchKernelGGL((k_diag16<kDiagRows, false>), dim3(g.grid_diag), dim3(64), 0, st, a);
+    }
     return hipGetLastError();
 }
 
-hipError_t diag_occupancy(bool local, int *blocks) {
+hipError_t diag_occupancy(bool local, size_t lds, int *blocks) {
+    if (lds)
+        return local ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_diag64<true>, 64, lds)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_diag64<false>, 64, lds);
     return local ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_diag16<kDiagRows, true>, 64, 0)
                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_diag16<kDiagRows, false>, 64, 0);
 }

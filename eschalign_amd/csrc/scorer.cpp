@@ -270,7 +270,10 @@ esch_status esch_scorer_new(int ordinal, const uint8_t *const *refs, const size_
     s->geo.grid32 = std::max(1, b32) * s->n_cu;
     s->geo.grid_aux = 4 * s->n_cu;
     int bd = 0;
-    HIPC(diag_occupancy(local, &bd));
+    // the wave-wide anti-diagonal kernel keeps its pass boundary in LDS;
+    // panels with longer references use the 16-lane one with global scratch
+    s->geo.diag_lds = max_len <= (size_t)kDiag64MaxLen ? (max_len + kDiag64Pad) * sizeof(uint2) : 0;
+    HIPC(diag_occupancy(local, s->geo.diag_lds, &bd));
     s->geo.grid_diag = std::max(1, bd) * s->n_cu;
     *out = s.release();
     return ESCH_OK;
@@ -318,6 +321,13 @@ int esch_local_bytes(const esch_scorer *s) { return s ? s->local16 : 0; }
 
 // ------------------------------------------------------------------ batches
 
+// k_diag16's per-row-task pass boundary in global memory (k_diag64 keeps it
+// in LDS and needs none)
+static size_t diag_scratch_bytes(const esch_scorer *s) {
+    if (s->geo.diag_lds) return sizeof(uint2);
+    return (size_t)s->geo.grid_diag * 4 * s->max_len * sizeof(uint2);
+}
+
 size_t esch_batch_footprint(const esch_scorer *s, size_t max_reads, size_t max_bases) {
     if (!s) return 0;
     const size_t rows = 2 * std::max<size_t>(max_reads, 1);
@@ -327,7 +337,7 @@ size_t esch_batch_footprint(const esch_scorer *s, size_t max_reads, size_t max_b
     f += (size_t)s->max_grid16 * 4 * (s->max_len + kScrPad) * 64 * sizeof(uint2);  // scratch16
     f += (size_t)s->geo.grid32 * s->geo.block32 * s->max_len * sizeof(int2);  // scratch32
     f += rows * 5 * 4;                                                   // records
-    f += rows * 4 + (size_t)s->geo.grid_diag * 4 * s->max_len * sizeof(uint2);  // anti-diagonal
+    f += rows * 4 + diag_scratch_bytes(s);                               // anti-diagonal
     return f;
 }
 
@@ -360,7 +370,7 @@ esch_status esch_batch_create(esch_scorer *s, size_t max_reads, size_t max_bases
     HIPC(hipMalloc(&b->d_scr32, (size_t)s->geo.grid32 * s->geo.block32 * s->max_len * sizeof(int2)));
     HIPC(hipMalloc(&b->d_recs, rows * 5 * 4));
     HIPC(hipMalloc(&b->d_listD, rows * 4));
-    HIPC(hipMalloc(&b->d_scrD, (size_t)s->geo.grid_diag * 4 * s->max_len * sizeof(uint2)));
+    HIPC(hipMalloc(&b->d_scrD, diag_scratch_bytes(s)));
     HIPC(hipMalloc(&b->d_desc, sizeof(DevDesc)));
     HIPC(hipHostMalloc(&b->h_codes, bases, hipHostMallocDefault));
     HIPC(hipHostMalloc(&b->h_qoff, rows * 8, hipHostMallocDefault));

@@ -383,6 +383,34 @@ def test_anti_diagonal_kernel_equals_oracle():
                 b.close()
 
 
+def test_anti_diagonal_fallback_for_long_references():
+    """A panel with a reference over 4096 nt keeps the anti-diagonal pass
+    boundary in global memory (the 16-lane kernel) instead of LDS: forced
+    for every read and for long ones, both modes, against the oracle."""
+    rng = G.Rng(0x5EED_0B05)
+    panel_l = [G.random_seq(rng, 4500, 0.01)] + G.random_panel(rng, 6, 50, 400)
+    panel = ea.Panel(panel_l)
+    reads = [G.chimeric_read(rng, panel_l[1:], 100 + rng.below(1500)) for _ in range(12)]
+    reads += [G.random_seq(rng, 1 + rng.below(60), 0.02) for _ in range(8)]
+    queries = codes_of(reads)
+    scoring = (2, -1, -10, -1)
+    for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
+        gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+        gpu.diag_ratio = 0
+        want = oracle(queries, panel.codes, scoring, mode)
+        for thr in (1, 500):
+            gpu.diag_min_len = thr
+            b = gpu.batch(len(queries), sum(len(q) for q in queries))
+            b.load(queries)
+            b.run(ea.RUN_SCORES32)
+            got, ok = b.scores32()
+            assert ok.all()
+            bad = np.argwhere(got != want)
+            assert bad.size == 0, (mode, thr, bad[:4])
+            assert b.n_diag() == sum(len(q) >= thr for q in queries)
+            b.close()
+
+
 def _routed(gpu, queries, ratio):
     gpu.diag_ratio = ratio
     b = gpu.batch(len(queries), sum(len(q) for q in queries))
