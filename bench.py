@@ -271,6 +271,12 @@ def main():
     wall = time.perf_counter() - t
     cells, n16, n32 = batch.stats()
     n_diag = batch.n_diag()
+    # the boundary hands over host buffers: time loading the same batch again
+    # (gather into pinned staging + host-to-device copy); not part of `value`
+    t = time.perf_counter()
+    batch.load_concat(codes, off, lens, both_strands=True)
+    batch.wait()
+    load_s = time.perf_counter() - t
     ms_step = wall / args.steps * 1e3
     value = args.reads / (wall / args.steps)
     dp = float(np.mean(dp_ms)) / 1e3
@@ -338,6 +344,8 @@ def main():
                      "traffic_unit": "B per k_dp16 launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE)",
                      "traffic_source": traffic_src,
                      "ops_per_cell": ops},
+        "host_load": {"ms": round(load_s * 1e3, 3), "bytes": int(lens.sum()),
+                      "pcie_inclusive_reads_per_s": round(args.reads / (load_s + wall / args.steps), 1)},
         "kernel_variant": chosen,
         "diag_ratio": gpu.diag_ratio,
         "autotune_ms": [round(x, 3) for x in tune_ms],

@@ -18,6 +18,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 using namespace esch;
@@ -435,17 +436,54 @@ esch_status batch_stage(esch_batch *b, size_t n, int both, int pairs,
         return fail(ESCH_CAPACITY, "batch holds %zu reads / %zu bases; got %zu / %zu", b->max_reads,
                     b->max_bases, n, total);
     HIPC(hipStreamSynchronize(b->stream));  // staging buffers may still feed a copy
+    // offsets first, then the gather into pinned staging on up to 8 host
+    // threads (1M short reads are a million small copies), then one
+    // vectorisable pass for the code check; the offending read is looked
+    // up only when the check fails
     size_t at = 0;
     for (size_t k = 0; k < n; ++k) {
         const uint8_t *p;
         size_t len;
         get(k, p, len);
-        for (size_t j = 0; j < len; ++j)
-            if (p[j] > 4) return fail(ESCH_INVALID_ARG, "read %zu carries code %d (codes are 0..=4)", k, p[j]);
-        if (len) memcpy(b->h_codes + at, p, len);
         b->h_qoff[k] = at;
         b->h_qlen[k] = (int32_t)len;
         at += len;
+    }
+    const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nt = total < ((size_t)8 << 20) ? 1 : std::min<size_t>(8, hw);
+    std::vector<uint8_t> bad(nt, 0);
+    auto gather = [&](size_t t) {
+        const size_t k0 = n * t / nt, k1 = n * (t + 1) / nt;
+        uint8_t mx = 0;
+        for (size_t k = k0; k < k1; ++k) {
+            const uint8_t *p;
+            size_t len;
+            get(k, p, len);
+            uint8_t *dst = b->h_codes + b->h_qoff[k];
+            if (len) memcpy(dst, p, len);
+        }
+        if (k1 > k0) {
+            const uint8_t *q = b->h_codes + b->h_qoff[k0];
+            const size_t end = (k1 < n ? b->h_qoff[k1] : total) - b->h_qoff[k0];
+            for (size_t j = 0; j < end; ++j) mx = q[j] > mx ? q[j] : mx;
+        }
+        bad[t] = mx > 4;
+    };
+    if (nt == 1) {
+        gather(0);
+    } else {
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < nt; ++t) pool.emplace_back(gather, t);
+        gather(0);
+        for (auto &th : pool) th.join();
+    }
+    for (size_t t = 0; t < nt; ++t) {
+        if (!bad[t]) continue;
+        for (size_t k = 0; k < n; ++k) {
+            const uint8_t *q = b->h_codes + b->h_qoff[k];
+            for (int32_t j = 0; j < b->h_qlen[k]; ++j)
+                if (q[j] > 4) return fail(ESCH_INVALID_ARG, "read %zu carries code %d (codes are 0..=4)", k, q[j]);
+        }
     }
     const size_t rows = both ? 2 * n : n;
     if (both) {

@@ -111,6 +111,33 @@ def test_gpu_refuses_a_reference_it_cannot_hold():
     assert e.value.kind == "ReferenceTooLong"
 
 
+def test_batch_load_refuses_codes_outside_the_alphabet():
+    """A read code above 4 is refused with INVALID_ARG naming the read, in a
+    small batch (one staging thread) and in the middle of a 12 MB batch (the
+    multi-threaded gather); a clean reload of the same batch then scores."""
+    rng = np.random.default_rng(7)
+    panel = ea.Panel([bytes(rng.choice(list(b"ACGT"), 150)) for _ in range(8)])
+    gpu = ea.GpuScorer(panel, ea.Scoring(), ea.Mode.LOCAL)
+    for n, length in ((20, 50), (80_000, 150)):
+        reads = [rng.integers(0, 5, length).astype(np.uint8) for _ in range(n)]
+        bad_at = n // 2 + 3
+        reads[bad_at] = reads[bad_at].copy()
+        reads[bad_at][length // 2] = 9
+        b = gpu.batch(n, n * length)
+        with pytest.raises(ea.GpuError) as e:
+            b.load(reads)
+        assert e.value.kind == "InvalidArg" and f"read {bad_at} " in str(e.value)
+        reads[bad_at][length // 2] = 2
+        b.load(reads)
+        b.run(ea.RUN_SCORES32)
+        got, ok = b.scores32()
+        assert ok.all()
+        k = [0, bad_at, n - 1]
+        want = oracle([reads[i] for i in k], panel.codes, (2, -1, -10, -1), ea.Mode.LOCAL)
+        assert np.array_equal(got[k], want)
+        b.close()
+
+
 def test_prepare_submit_finish_overlap():
     """Two batches in flight at once (prepare k+1 while k runs,
     crates/eschalign-cli/src/align.rs:2879-2887) give the same rows."""
