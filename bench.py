@@ -33,6 +33,9 @@ N_CU, CLOCK_HZ = 256, 2.4e9
 PEAK_I16 = N_CU * CLOCK_HZ * 128           # lane-ops/s
 PEAK_I32 = N_CU * CLOCK_HZ * 64
 OPS_PER_CELL = {"local": 10, "semiglobal": 8}
+# SQ_INSTS_VALU x 128 / cells of one k_dp16 launch of the default workload,
+# from the counter summary in profiles/ (mode, variant) -> lane-ops per cell
+VALU_OPS_PER_CELL_ISSUED = {("semiglobal", 2): round(354_222_295_060 * 128 / 5_303_272_453_076, 3)}
 
 
 def log(*a):
@@ -67,6 +70,15 @@ def cpu_baseline(panel_codes, codes, off, lens, scoring, local, gpu_check):
     dt, qs, out = run(n)
     cells = sum(len(q) for q in qs) * sum(len(p) for p in panel_codes)
     gpu_check(qs, out)
+    avx2 = None
+    if lanes == 32 and sup & 1:
+        # the AVX2 16-lane kernel on a quarter of the sample, reported apart
+        prof = O.Profile(panel_codes, scoring, 16)
+        n2 = max(2000, n // 4)
+        dt2, qs2, out2 = run(n2)
+        gpu_check(qs2, out2)
+        avx2 = {"reads_per_s": round(n2 / dt2, 1), "reads": n2,
+                "cells_per_s": round(sum(len(q) for q in qs2) * sum(len(p) for p in panel_codes) / dt2, 1)}
     return {
         "value": round(n / dt, 1),
         "unit": "reads/s",
@@ -77,7 +89,29 @@ def cpu_baseline(panel_codes, codes, off, lens, scoring, local, gpu_check):
                   f"inter-sequence i16 kernel restated from the reference, {threads} threads",
         "cells_per_s": round(cells / dt, 1),
         "seconds": round(dt, 2),
+        "host": host_cpu(),
+        "avx2": avx2,
     }
+
+
+def host_cpu():
+    """Model, logical CPUs and SMT of the host, from /proc/cpuinfo."""
+    model, siblings, cores = None, None, None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "siblings" and siblings is None:
+                    siblings = int(v)
+                elif k == "cpu cores" and cores is None:
+                    cores = int(v)
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(),
+            "smt": None if not (siblings and cores) else siblings > cores}
 
 
 def parse_lengths(text):
@@ -315,6 +349,9 @@ def main():
         log(f"[bench] serving: {serve}")
 
     traffic, traffic_src = recorded_traffic(args.mode, args.reads, chosen, args.lengths)
+    peak = ea.valu_peak16(0)
+    log(f"[bench] measured int16 VALU peak {peak['lane_ops_per_s'] / 1e12:.2f} T lane-op/s "
+        f"({peak['n_cu']} CUs, reported clock {peak['clock_khz'] / 1e3:.0f} MHz)")
     line = {
         "metric": "reads/s + GCUPS, bit-exact scores vs CPU; % VALU roofline; p50/p99 latency",
         "value": round(value, 1),
@@ -341,6 +378,9 @@ def main():
         "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3),
                      "peak": round(PEAK_I16 / 1e12, 3), "unit": "Tlane-op/s (int16)",
                      "frac": round(achieved / PEAK_I16, 4), "traffic": traffic,
+                     "peak_measured": round(peak["lane_ops_per_s"] / 1e12, 3),
+                     "frac_of_measured": round(achieved / peak["lane_ops_per_s"], 4),
+                     "valu_lane_ops_per_cell_issued": VALU_OPS_PER_CELL_ISSUED.get((args.mode, chosen)),
                      "traffic_unit": "B per k_dp16 launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE)",
                      "traffic_source": traffic_src,
                      "ops_per_cell": ops},

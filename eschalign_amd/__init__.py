@@ -63,7 +63,7 @@ EXPORTS = [
     "esch_batch_ready", "esch_server_new", "esch_server_free", "esch_server_add_panel",
     "esch_server_scorer", "esch_server_submit", "esch_server_wait", "esch_server_result",
     "esch_server_stats", "esch_set_diag_min_len", "esch_get_diag_min_len",
-    "esch_batch_stats_diag", "esch_set_diag_ratio", "esch_get_diag_ratio",
+    "esch_batch_stats_diag", "esch_set_diag_ratio", "esch_get_diag_ratio", "esch_valu_peak16",
 ]
 
 
@@ -174,6 +174,7 @@ def load_library():
     L.esch_get_diag_min_len.restype = i32
     L.esch_batch_stats_diag.argtypes = [vp, C.POINTER(C.c_uint64)]
     L.esch_set_diag_ratio.argtypes = [vp, C.c_int32]
+    L.esch_valu_peak16.argtypes = [C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.esch_get_diag_ratio.argtypes = [vp]
     _lib = L
     return L
@@ -182,6 +183,14 @@ def load_library():
 def _check(st: int):
     if st != OK:
         raise GpuError(st, load_library().esch_last_error().decode(errors="replace"))
+
+
+def valu_peak16(ordinal: int = 0) -> dict:
+    """The measured packed-int16 VALU peak (lane-ops/s) with the CU count and
+    the reported peak clock of device `ordinal`."""
+    v, cu, clk = C.c_double(), C.c_int(), C.c_int()
+    _check(load_library().esch_valu_peak16(ordinal, C.byref(v), C.byref(cu), C.byref(clk)))
+    return {"lane_ops_per_s": v.value, "n_cu": cu.value, "clock_khz": clk.value}
 
 
 def device_visible() -> bool:

@@ -117,6 +117,7 @@ hipError_t launch_dp32(const KArgs &a, const Launch &g, bool local, int which, h
 hipError_t launch_select(const KArgs &a, const Launch &g, hipStream_t st);
 hipError_t launch_diag(const KArgs &a, const Launch &g, bool local, hipStream_t st);
 hipError_t diag_occupancy(bool local, size_t lds, int *blocks_per_cu);
+hipError_t launch_peak16(uint32_t *out, int blocks, int iters, hipStream_t st);
 hipError_t launch_pairs(const KArgs &a, const Launch &g, hipStream_t st);
 hipError_t dp16_attributes(bool local, int variant, hipFuncAttributes *attr);
 hipError_t dp32_attributes(bool local, hipFuncAttributes *attr);

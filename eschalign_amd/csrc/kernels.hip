@@ -1298,6 +1298,38 @@ __global__ __launch_bounds__(256) void k_pairs(KArgs a) {
 // Host-side launchers.
 // ---------------------------------------------------------------------------
 
+// ---------------------------------------------------------------------------
+// k_peak16 — the packed-int16 VALU peak, measured: every lane runs 8
+// independent chains of the k_dp16 column's two operations (saturating
+// v_pk_add_i16 and v_pk_max_i16), `iters` x 16 of them (unrolled to 128 per
+// loop branch), over a grid that fills every SIMD. The result is stored so
+// nothing is dead.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_peak16(uint32_t *out, int iters, uint32_t seed) {
+    v2s x[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = as_v2s(seed * (threadIdx.x + 1 + k) ^ blockIdx.x);
+    const v2s a = as_v2s(seed | 1u), b = as_v2s(seed >> 3);
+#pragma unroll 8
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            x[k] = adds(x[k], a);
+            x[k] = vmax(x[k], b);
+        }
+    }
+    v2s r = x[0];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) r = vmax(r, x[k]);
+    out[blockIdx.x * blockDim.x + threadIdx.x] = as_u32(r);
+}
+
+hipError_t launch_peak16(uint32_t *out, int blocks, int iters, hipStream_t st) {
+    hipLaunchKernelGGL(k_peak16, dim3(blocks), dim3(256), 0, st, out, iters, 0x9E3779B9u);
+    return hipGetLastError();
+}
+
 hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st) {
     hipLaunchKernelGGL(k_zero, dim3((kMaxLen16 + 1 + 255) / 256), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_revcomp, dim3(g.grid_aux), dim3(256), 0, st, a);

@@ -310,6 +310,40 @@ esch_status esch_device_name(const esch_scorer *s, char *buf, size_t cap) {
     return ESCH_OK;
 }
 
+esch_status esch_valu_peak16(int32_t ordinal, double *lane_ops_per_s, int *n_cu, int *clock_khz) {
+    if (!lane_ops_per_s) return fail(ESCH_INVALID_ARG, "null argument");
+    if (!esch_device_visible()) return fail(ESCH_UNAVAILABLE, "no HIP device visible");
+    HIPC(hipSetDevice(ordinal));
+    hipDeviceProp_t prop;
+    HIPC(hipGetDeviceProperties(&prop, ordinal));
+    if (n_cu) *n_cu = prop.multiProcessorCount;
+    if (clock_khz) *clock_khz = prop.clockRate;
+    // 8 workgroups of 4 waves per CU: 8 waves per SIMD, every issue slot covered
+    const int blocks = 8 * prop.multiProcessorCount, iters = 1 << 14;
+    uint32_t *out = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    esch_status rc = ESCH_OK;
+    float ms = 0.0f;
+    if (hipMalloc(&out, (size_t)blocks * 256 * 4) != hipSuccess || hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(ESCH_DRIVER, "allocation for the peak probe failed");
+    } else if (launch_peak16(out, blocks, 64, st) != hipSuccess || hipEventRecord(e0, st) != hipSuccess ||
+               launch_peak16(out, blocks, iters, st) != hipSuccess || hipEventRecord(e1, st) != hipSuccess ||
+               hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = fail(ESCH_DRIVER, "peak probe launch failed: %s", hipGetErrorString(hipGetLastError()));
+    } else {
+        // 16 packed ops x 64 lanes x 2 halves per iteration per wave
+        const double waves = (double)blocks * 4.0;
+        *lane_ops_per_s = waves * iters * 16.0 * 128.0 / (ms * 1e-3);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (out) (void)hipFree(out);
+    return rc;
+}
+
 void esch_geometry(const esch_scorer *s, size_t *lanes, size_t *groups, size_t *blocks) {
     if (!s) return;
     if (lanes) *lanes = 256;

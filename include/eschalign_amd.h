@@ -122,6 +122,11 @@ esch_status esch_score_batch(esch_scorer *s, const uint8_t *const *queries, cons
  * the batched kernel, references per work item, persistent blocks). */
 esch_status esch_device_name(const esch_scorer *s, char *buf, size_t cap);
 void esch_geometry(const esch_scorer *s, size_t *lanes, size_t *groups, size_t *blocks);
+/* Measured packed-int16 VALU peak of device `ordinal` (lane-ops/s: a
+ * saturating v_pk_add_i16 / v_pk_max_i16 microbenchmark over every SIMD),
+ * with the CU count and the reported peak clock; no counterpart in the
+ * reference (the roofline of the batched kernel is quoted against it). */
+esch_status esch_valu_peak16(int32_t ordinal, double *lane_ops_per_s, int *n_cu, int *clock_khz);
 int esch_registers(const esch_scorer *s);   /* VGPRs of the batched kernel */
 int esch_local_bytes(const esch_scorer *s); /* private (spill) bytes of the batched kernel */
 
