@@ -88,6 +88,125 @@ int32_t eo_score(const uint8_t *q, size_t n, const uint8_t *r, size_t m,
     return best;
 }
 
+/* lib/scalar.rs:164-174: the traceback byte of a cell. */
+enum { EO_SRC_STOP = 0, EO_SRC_DIAG = 1, EO_SRC_INS = 2, EO_SRC_DEL = 3 };
+#define EO_E_EXTENDED 4u
+#define EO_F_EXTENDED 8u
+
+/* lib/scalar.rs:176-364: fill (column-major trace, trace[j * rows + i]),
+ * the end cell as the first maximum of a column-major scan over the mode's
+ * candidates, then the walk from it. */
+int eo_align(const uint8_t *q, size_t n, const uint8_t *r, size_t m, int32_t ma,
+             int32_t mi, int32_t o, int32_t e, int local, eo_alignment *out,
+             uint32_t *ops, size_t ops_cap) {
+    memset(out, 0, sizeof(*out));
+    if (n == 0 || m == 0) return 0;
+    const size_t rows = n + 1;
+    uint8_t *trace = (uint8_t *)calloc(rows * (m + 1), 1);
+    int32_t *hcol = (int32_t *)calloc(rows, sizeof(int32_t));
+    int32_t *ecol = (int32_t *)malloc(rows * sizeof(int32_t));
+    for (size_t i = 0; i < rows; i++) ecol[i] = EO_NEG;
+    int32_t best = local ? 0 : EO_NEG;
+    size_t bi = 0, bj = 0;
+    for (size_t j = 1; j <= m; j++) {
+        const uint8_t rj = r[j - 1];
+        uint8_t *col = trace + j * rows;
+        int32_t hdiag = 0, hup = 0, f = EO_NEG;
+        const int last_col = j == m;
+        for (size_t i = 1; i <= n; i++) {
+            const int32_t hleft = hcol[i];
+            const int32_t e_ext = ecol[i] + e;
+            const int32_t ev = max32(hleft + o, e_ext);
+            const int32_t f_ext = f + e;
+            f = max32(hup + o, f_ext);
+            const int32_t hd = hdiag + eo_substitution(ma, mi, q[i - 1], rj);
+            int32_t h = max32(max32(hd, ev), f);
+            if (local) h = max32(h, 0);
+            uint8_t t = h == f ? EO_SRC_INS : EO_SRC_DEL;
+            if (h == hd) t = EO_SRC_DIAG;
+            if (local && h == 0) t = EO_SRC_STOP;
+            if (ev == e_ext) t |= EO_E_EXTENDED;
+            if (f == f_ext) t |= EO_F_EXTENDED;
+            col[i] = t;
+            ecol[i] = ev;
+            hdiag = hleft;
+            hcol[i] = h;
+            hup = h;
+            if ((local || last_col) && h > best) {
+                best = h;
+                bi = i;
+                bj = j;
+            }
+        }
+        if (!local && !last_col && hup > best) {
+            best = hup;
+            bi = n;
+            bj = j;
+        }
+    }
+    free(hcol);
+    free(ecol);
+    int nops = 0;
+    if (bi != 0 && bj != 0) {
+        /* lib/scalar.rs:292-364: runs are collected last-first, then reversed */
+        size_t i = bi, j = bj;
+        int state = 0; /* 0 H, 1 F (insertion), 2 E (deletion) */
+        uint32_t cur = 3u, len = 0;
+        while (i > 0 && j > 0) {
+            const uint8_t t = trace[j * rows + i];
+            uint32_t op = 3u;
+            if (state == 0) {
+                const int src = t & 3;
+                if (src == EO_SRC_STOP) break;
+                if (src == EO_SRC_DIAG) {
+                    op = 0;
+                    i--;
+                    j--;
+                } else if (src == EO_SRC_INS) {
+                    state = 1;
+                } else {
+                    state = 2;
+                }
+            } else if (state == 1) {
+                op = 1;
+                if (!(t & EO_F_EXTENDED)) state = 0;
+                i--;
+            } else {
+                op = 2;
+                if (!(t & EO_E_EXTENDED)) state = 0;
+                j--;
+            }
+            if (op == 3u) continue;
+            if (op == cur) {
+                len++;
+            } else {
+                if (len) {
+                    if ((size_t)nops >= ops_cap) { free(trace); return -1; }
+                    ops[nops++] = cur << 30 | len;
+                }
+                cur = op;
+                len = 1;
+            }
+        }
+        if (len) {
+            if ((size_t)nops >= ops_cap) { free(trace); return -1; }
+            ops[nops++] = cur << 30 | len;
+        }
+        for (int a = 0, b = nops - 1; a < b; a++, b--) {
+            const uint32_t x = ops[a];
+            ops[a] = ops[b];
+            ops[b] = x;
+        }
+        out->score = best;
+        out->query_start = (int32_t)i;
+        out->query_end = (int32_t)bi;
+        out->ref_start = (int32_t)j;
+        out->ref_end = (int32_t)bj;
+    }
+    free(trace);
+    return nops;
+}
+
 /* lib/simd/mod.rs:222-227 */
 int eo_fits_i16(size_t n, size_t max_ref_len, int32_t ma, int32_t mi,
                 int32_t o, int32_t e) {

@@ -13,6 +13,12 @@
  *   eo_score          192 parasail 1.3.4 results (tests/golden/align_golden.json),
  *                     closed forms from lib/scalar.rs:378-430, brute-force
  *                     enumeration of every alignment for n, m <= 5.
+ *   eo_align          the same 192 results' end and start coordinates,
+ *                     the CIGAR span / first-last-op / rescore checks of
+ *                     tests/parasail_golden.rs:82-110, the known answers of
+ *                     lib/scalar.rs:378-430, and a full-matrix restatement of
+ *                     the documented tie rules (lib/scalar.rs:28-37) on every
+ *                     pair of tiny sequences.
  *   eo_fits_i16       the reference's own bounds test (lib/simd/mod.rs:1136-1140)
  *                     and the closed-form bound.
  *   eo_simd_*         equality with eo_score on the reference's own random
@@ -50,6 +56,17 @@ int32_t eo_substitution(int32_t ma, int32_t mi, uint8_t a, uint8_t b);
 /* lib/scalar.rs:120-163 — the score-only DP (local=1: Smith-Waterman). */
 int32_t eo_score(const uint8_t *q, size_t n, const uint8_t *r, size_t m,
                  int32_t ma, int32_t mi, int32_t o, int32_t e, int local);
+
+/* lib/scalar.rs:164-364 — the traceback DP and walk. ops[] receives the
+ * run-length operations (op << 30 | len; op 0 = M, 1 = I, 2 = D). Returns
+ * the number of runs, or -1 if ops_cap is too small. */
+typedef struct {
+    int32_t score;
+    int32_t query_start, query_end, ref_start, ref_end;
+} eo_alignment;
+int eo_align(const uint8_t *q, size_t n, const uint8_t *r, size_t m, int32_t ma,
+             int32_t mi, int32_t o, int32_t e, int local, eo_alignment *out,
+             uint32_t *ops, size_t ops_cap);
 
 /* lib/simd/mod.rs:222-227 */
 int eo_fits_i16(size_t n, size_t max_ref_len, int32_t ma, int32_t mi,

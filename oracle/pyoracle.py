@@ -39,6 +39,9 @@ def lib():
                                C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int]
         L.eo_score.restype = C.c_int32
         L.eo_fits_i16.argtypes = [C.c_size_t, C.c_size_t] + [C.c_int32] * 4
+        L.eo_align.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t] + [C.c_int32] * 4 + \
+            [C.c_int, C.c_void_p, C.c_void_p, C.c_size_t]
+        L.eo_align.restype = C.c_int
         L.eo_fits_i16.restype = C.c_int
         L.eo_score_matrix.argtypes = [U8P, U64P, I32P, C.c_size_t, U8P, U64P, I32P,
                                       C.c_size_t, C.c_int32, C.c_int32, C.c_int32,
@@ -99,6 +102,24 @@ def score(q, r, scoring, local: bool) -> int:
     r = _codes(r)
     ma, mi, o, e = scoring
     return lib().eo_score(q.ctypes.data, len(q), r.ctypes.data, len(r), ma, mi, o, e, int(local))
+
+
+OPS = "MID"
+
+
+def align(q, r, scoring, local: bool):
+    """eo_align (lib/scalar.rs:164-364): (score, query_start, query_end,
+    ref_start, ref_end, [(op char, len), ...])."""
+    q = _codes(q)
+    r = _codes(r)
+    rec = (C.c_int32 * 5)()
+    cap = len(q) + len(r) + 1
+    ops = np.zeros(cap, np.uint32)
+    k = lib().eo_align(q.ctypes.data, len(q), r.ctypes.data, len(r), *scoring, int(local),
+                       C.byref(rec), ops.ctypes.data, cap)
+    assert k >= 0
+    return (rec[0], rec[1], rec[2], rec[3], rec[4],
+            [(OPS[int(x) >> 30], int(x) & 0x3FFFFFFF) for x in ops[:k]])
 
 
 def fits_i16(n, max_ref_len, scoring) -> bool:
