@@ -64,6 +64,7 @@ EXPORTS = [
     "esch_server_scorer", "esch_server_submit", "esch_server_wait", "esch_server_result",
     "esch_server_stats", "esch_set_diag_min_len", "esch_get_diag_min_len",
     "esch_batch_stats_diag", "esch_set_diag_ratio", "esch_get_diag_ratio", "esch_valu_peak16",
+    "esch_align_pairs",
 ]
 
 
@@ -174,6 +175,7 @@ def load_library():
     L.esch_get_diag_min_len.restype = i32
     L.esch_batch_stats_diag.argtypes = [vp, C.POINTER(C.c_uint64)]
     L.esch_set_diag_ratio.argtypes = [vp, C.c_int32]
+    L.esch_align_pairs.argtypes = [vp, vp, vp, vp, sz, vp, vp, sz, C.POINTER(sz)]
     L.esch_valu_peak16.argtypes = [C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.esch_get_diag_ratio.argtypes = [vp]
     _lib = L
@@ -183,6 +185,31 @@ def load_library():
 def _check(st: int):
     if st != OK:
         raise GpuError(st, load_library().esch_last_error().decode(errors="replace"))
+
+
+@dataclass(frozen=True)
+class Alignment:
+    """lib/scalar.rs:63-79: a scored alignment with its traceback; spans are
+    0-based half-open, ops run-length (op, length) with op in "MID"."""
+    score: int
+    query_start: int
+    query_end: int
+    ref_start: int
+    ref_end: int
+    ops: tuple
+
+    def is_empty(self) -> bool:
+        """lib/scalar.rs:84-88: no read base aligned to a reference base."""
+        return not any(op == "M" for op, _ in self.ops)
+
+    def cigar(self) -> str:
+        return "".join(f"{n}{op}" for op, n in self.ops)
+
+
+class _AlignRec(C.Structure):
+    _fields_ = [("score", C.c_int32), ("query_start", C.c_int32), ("query_end", C.c_int32),
+                ("ref_start", C.c_int32), ("ref_end", C.c_int32), ("n_ops", C.c_int32),
+                ("ops_offset", C.c_int64)]
 
 
 def valu_peak16(ordinal: int = 0) -> dict:
@@ -449,6 +476,7 @@ class GpuScorer:
                                  C.byref(out)))
         self._h = out
         self.n_refs = len(panel)
+        self._rlen = [len(c) for c in panel.codes]
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -557,6 +585,30 @@ class GpuScorer:
     @diag_min_len.setter
     def diag_min_len(self, v: int):
         _check(_lib.esch_set_diag_min_len(self._h, int(v)))
+
+    def align_pairs(self, queries: Sequence[np.ndarray], refs: Sequence[int]) -> list:
+        """simd::align_pairs (lib/simd/mod.rs:405-499) on the device: the
+        alignment of queries[k] (codes) against panel reference refs[k], in
+        this scorer's scoring and mode."""
+        if len(queries) != len(refs):
+            raise ValueError("one reference per query")
+        keep, ptrs, lens = _ptr_array(queries)
+        n = len(keep)
+        refs_a = np.ascontiguousarray(refs, np.int32)
+        recs = (_AlignRec * max(n, 1))()
+        nr = len(self._rlen)
+        cap = int(sum(len(q) for q in keep)) + int(sum(self._rlen[r] for r in refs_a if 0 <= r < nr)) + 1
+        ops = np.zeros(cap, np.uint32)
+        used = C.c_size_t()
+        _check(_lib.esch_align_pairs(self._h, ptrs, lens, refs_a.ctypes.data, n, C.byref(recs),
+                                     ops.ctypes.data, cap, C.byref(used)))
+        out = []
+        for k in range(n):
+            r = recs[k]
+            seg = ops[r.ops_offset:r.ops_offset + r.n_ops]
+            out.append(Alignment(r.score, r.query_start, r.query_end, r.ref_start, r.ref_end,
+                                 tuple(("MID"[int(x) >> 30], int(x) & 0x3FFFFFFF) for x in seg)))
+        return out
 
     @property
     def diag_ratio(self) -> float:

@@ -118,6 +118,31 @@ hipError_t launch_select(const KArgs &a, const Launch &g, hipStream_t st);
 hipError_t launch_diag(const KArgs &a, const Launch &g, bool local, hipStream_t st);
 hipError_t diag_occupancy(bool local, size_t lds, int *blocks_per_cu);
 hipError_t launch_peak16(uint32_t *out, int blocks, int iters, hipStream_t st);
+
+// ---- winner traceback (trace.hip) ----
+constexpr int kTraceRec = 7;  // score, qs, qe, rs, re, runs, first run's slot in the worst-case region
+
+struct TraceArgs {
+    DevPanel panel;
+    Scoring sc;
+    const uint8_t *qcodes;   // read codes, concatenated
+    const uint64_t *qoff;    // per sorted pair: offset of its read
+    const int32_t *qlen;     // per sorted pair: read length
+    const int32_t *ref;      // per sorted pair: panel index
+    const int32_t *slot;     // per sorted pair: the caller's pair index
+    const uint64_t *gbase;   // per group: byte offset of its arena region
+    const int32_t *gS, *gM;  // per group: stripes of its longest read, its longest reference
+    uint8_t *arena;          // trace nibbles + stripe boundary, per group region
+    int32_t *recs;           // [caller index * kTraceRec]
+    uint32_t *ops_wc;        // worst-case run regions (read + reference length each)
+    const uint64_t *ops_off; // per caller index: its region's offset
+    int32_t n_pairs;
+    int32_t g0;              // first group of this launch
+};
+
+hipError_t launch_trace(const TraceArgs &a, int groups, bool local, hipStream_t st);
+hipError_t launch_trace_compact(const uint32_t *ops_wc, const uint64_t *ops_off, const int32_t *recs,
+                                const uint64_t *out_off, uint32_t *out, int32_t n_pairs, hipStream_t st);
 hipError_t launch_pairs(const KArgs &a, const Launch &g, hipStream_t st);
 hipError_t dp16_attributes(bool local, int variant, hipFuncAttributes *attr);
 hipError_t dp32_attributes(bool local, hipFuncAttributes *attr);

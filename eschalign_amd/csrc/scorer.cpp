@@ -90,6 +90,7 @@ struct esch_scorer {
     bool diag_ratio_tunable = true;     // false once the caller set the ratio
     std::mutex pool_mu;
     std::vector<esch_batch *> pool;  // idle batches for the drop-in API
+    std::vector<int32_t> h_rlen;     // reference lengths (traceback sizing)
 };
 
 struct esch_batch {
@@ -233,6 +234,7 @@ esch_status esch_scorer_new(int ordinal, const uint8_t *const *refs, const size_
     HIPC(hipMemcpy(s->d_w2off, w2off.data(), n_refs * 4, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(s->d_nmoff, nmoff.data(), n_refs * 4, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(s->d_rlen, rlen.data(), n_refs * 4, hipMemcpyHostToDevice));
+    s->h_rlen = rlen;
     // Longest first, stable (as lib/simd/mod.rs:299-300 orders its lane
     // groups): the four references of one workgroup then have nearly equal
     // lengths and finish their stripes together.
@@ -1040,3 +1042,176 @@ esch_status esch_select_pairs(esch_scorer *s, const uint8_t *const *mates1, cons
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ winner traceback
+
+namespace {
+
+// Device buffers of one esch_align_pairs call, released on every path.
+struct TraceBufs {
+    std::vector<void *> ptrs;
+    hipStream_t st = nullptr;
+    ~TraceBufs() {
+        if (st) (void)hipStreamSynchronize(st);
+        for (void *p : ptrs) (void)hipFree(p);
+        if (st) (void)hipStreamDestroy(st);
+    }
+    template <class T>
+    hipError_t alloc(T **p, size_t count) {
+        *p = nullptr;
+        const hipError_t e = hipMalloc(reinterpret_cast<void **>(p), std::max<size_t>(count, 1) * sizeof(T));
+        if (e == hipSuccess) ptrs.push_back(*p);
+        return e;
+    }
+};
+
+constexpr size_t kTraceArenaMax = (size_t)4 << 30;  // trace + boundary bytes per launch
+constexpr uint64_t kTraceMaxCells = (uint64_t)1 << 27;  // read x reference: a 4 GiB group region
+
+}  // namespace
+
+esch_status esch_align_pairs(esch_scorer *s, const uint8_t *const *queries, const size_t *lens,
+                             const int32_t *refs, size_t n, esch_alignment *out, uint32_t *ops,
+                             size_t ops_cap, size_t *ops_used) {
+    if (!s || (n && (!queries || !lens || !refs || !out))) return fail(ESCH_INVALID_ARG, "null argument");
+    if (ops_used) *ops_used = 0;
+    std::vector<int32_t> order;
+    order.reserve(n);
+    size_t bases = 0;
+    for (size_t k = 0; k < n; ++k) {
+        if (refs[k] < 0 || (size_t)refs[k] >= s->n_refs)
+            return fail(ESCH_INVALID_ARG, "pair %zu names reference %d of %zu", k, refs[k], s->n_refs);
+        // a group region is 64 pairs x (read / 16 stripes) x reference x 8 B
+        if ((uint64_t)lens[k] * (uint64_t)s->h_rlen[refs[k]] > kTraceMaxCells)
+            return fail(ESCH_INVALID_ARG, "pair %zu: %zu x %d cells is over the device traceback bound %llu", k,
+                        lens[k], s->h_rlen[refs[k]], (unsigned long long)kTraceMaxCells);
+        if (lens[k] && !queries[k]) return fail(ESCH_INVALID_ARG, "pair %zu: null read", k);
+        for (size_t j = 0; j < lens[k]; ++j)
+            if (queries[k][j] > 4)
+                return fail(ESCH_INVALID_ARG, "read %zu carries code %d (codes are 0..=4)", k, queries[k][j]);
+        out[k] = esch_alignment{};  // lib/scalar.rs:185-195: empty in, empty out
+        if (lens[k]) {
+            order.push_back((int32_t)k);
+            bases += lens[k];
+        }
+    }
+    const size_t np = order.size();
+    if (np == 0) return ESCH_OK;
+    // lib/simd/mod.rs:433: groups of similar read and reference lengths
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+        if (lens[x] != lens[y]) return lens[x] < lens[y];
+        return s->h_rlen[refs[x]] < s->h_rlen[refs[y]];
+    });
+    std::vector<uint8_t> hcodes(bases);
+    std::vector<uint64_t> hqoff(np), ops_off(n, 0);
+    std::vector<int32_t> hqlen(np), href(np), hslot(np);
+    size_t at = 0;
+    for (size_t i = 0; i < np; ++i) {
+        const int32_t k = order[i];
+        memcpy(hcodes.data() + at, queries[k], lens[k]);
+        hqoff[i] = at;
+        hqlen[i] = (int32_t)lens[k];
+        href[i] = refs[k];
+        hslot[i] = k;
+        at += lens[k];
+    }
+    size_t wc = 0;  // worst-case runs: read + reference length per pair
+    for (size_t k = 0; k < n; ++k) {
+        ops_off[k] = wc;
+        if (lens[k]) wc += lens[k] + (size_t)s->h_rlen[refs[k]];
+    }
+    const int ng = (int)((np + 63) / 64);
+    std::vector<int32_t> gS(ng), gM(ng);
+    std::vector<uint64_t> gbytes(ng), gbase(ng);
+    for (int g = 0; g < ng; ++g) {
+        int32_t mx_n = 0, mx_m = 0;
+        for (size_t i = (size_t)g * 64; i < std::min(np, (size_t)g * 64 + 64); ++i) {
+            mx_n = std::max(mx_n, hqlen[i]);
+            mx_m = std::max(mx_m, s->h_rlen[href[i]]);
+        }
+        gS[g] = (mx_n + 15) / 16;
+        gM[g] = mx_m;
+        gbytes[g] = ((uint64_t)gS[g] * gM[g] + gM[g]) * 512;
+    }
+    // launches of consecutive groups whose regions fit the arena together
+    std::vector<std::pair<int, int>> chunks;
+    size_t arena = 0;
+    for (int g0 = 0; g0 < ng;) {
+        uint64_t sum = 0;
+        int g1 = g0;
+        while (g1 < ng && (g1 == g0 || sum + gbytes[g1] <= kTraceArenaMax)) {
+            gbase[g1] = sum;
+            sum += gbytes[g1];
+            ++g1;
+        }
+        chunks.emplace_back(g0, g1);
+        arena = std::max<size_t>(arena, sum);
+        g0 = g1;
+    }
+
+    HIPC(hipSetDevice(s->ordinal));
+    TraceBufs B;
+    HIPC(hipStreamCreateWithFlags(&B.st, hipStreamNonBlocking));
+    uint8_t *d_codes = nullptr, *d_arena = nullptr;
+    uint64_t *d_qoff = nullptr, *d_gbase = nullptr, *d_ops_off = nullptr, *d_out_off = nullptr;
+    int32_t *d_qlen = nullptr, *d_ref = nullptr, *d_slot = nullptr, *d_gS = nullptr, *d_gM = nullptr,
+            *d_recs = nullptr;
+    uint32_t *d_ops_wc = nullptr, *d_out = nullptr;
+    HIPC(B.alloc(&d_codes, bases));
+    HIPC(B.alloc(&d_qoff, np));
+    HIPC(B.alloc(&d_qlen, np));
+    HIPC(B.alloc(&d_ref, np));
+    HIPC(B.alloc(&d_slot, np));
+    HIPC(B.alloc(&d_gbase, ng));
+    HIPC(B.alloc(&d_gS, ng));
+    HIPC(B.alloc(&d_gM, ng));
+    HIPC(B.alloc(&d_arena, arena));
+    HIPC(B.alloc(&d_recs, n * kTraceRec));
+    HIPC(B.alloc(&d_ops_wc, wc));
+    HIPC(B.alloc(&d_ops_off, n));
+    HIPC(hipMemcpyAsync(d_codes, hcodes.data(), bases, hipMemcpyHostToDevice, B.st));
+    HIPC(hipMemcpyAsync(d_qoff, hqoff.data(), np * 8, hipMemcpyHostToDevice, B.st));
+    HIPC(hipMemcpyAsync(d_qlen, hqlen.data(), np * 4, hipMemcpyHostToDevice, B.st));
+    HIPC(hipMemcpyAsync(d_ref, href.data(), np * 4, hipMemcpyHostToDevice, B.st));
+    HIPC(hipMemcpyAsync(d_slot, hslot.data(), np * 4, hipMemcpyHostToDevice, B.st));
+    HIPC(hipMemcpyAsync(d_gbase, gbase.data(), (size_t)ng * 8, hipMemcpyHostToDevice, B.st));
+    HIPC(hipMemcpyAsync(d_gS, gS.data(), (size_t)ng * 4, hipMemcpyHostToDevice, B.st));
+    HIPC(hipMemcpyAsync(d_gM, gM.data(), (size_t)ng * 4, hipMemcpyHostToDevice, B.st));
+    HIPC(hipMemcpyAsync(d_ops_off, ops_off.data(), n * 8, hipMemcpyHostToDevice, B.st));
+    // empty pairs are never traced: their records must read as zero runs
+    HIPC(hipMemsetAsync(d_recs, 0, n * kTraceRec * 4, B.st));
+    TraceArgs ta{s->panel, s->sc, d_codes, d_qoff, d_qlen, d_ref, d_slot, d_gbase, d_gS, d_gM,
+                 d_arena, d_recs, d_ops_wc, d_ops_off, (int32_t)np, 0};
+    const bool local = s->sc.local != 0;
+    for (const auto &c : chunks) {
+        ta.g0 = c.first;
+        HIPC(launch_trace(ta, c.second - c.first, local, B.st));
+    }
+    std::vector<int32_t> recs(n * kTraceRec, 0);
+    HIPC(hipMemcpyAsync(recs.data(), d_recs, n * kTraceRec * 4, hipMemcpyDeviceToHost, B.st));
+    HIPC(hipStreamSynchronize(B.st));
+    std::vector<uint64_t> out_off(n, 0);
+    size_t runs = 0;
+    for (size_t k = 0; k < n; ++k) {
+        out_off[k] = runs;
+        if (lens[k]) runs += (size_t)recs[k * kTraceRec + 5];
+    }
+    if (ops_used) *ops_used = runs;
+    if (runs > ops_cap || (runs && !ops))
+        return fail(ESCH_CAPACITY, "the alignments have %zu operation runs; ops holds %zu", runs, ops_cap);
+    if (runs) {
+        HIPC(B.alloc(&d_out_off, n));
+        HIPC(B.alloc(&d_out, runs));
+        // untraced (empty) pairs have zero runs and copy nothing
+        HIPC(hipMemcpyAsync(d_out_off, out_off.data(), n * 8, hipMemcpyHostToDevice, B.st));
+        HIPC(launch_trace_compact(d_ops_wc, d_ops_off, d_recs, d_out_off, d_out, (int32_t)n, B.st));
+        HIPC(hipMemcpyAsync(ops, d_out, runs * 4, hipMemcpyDeviceToHost, B.st));
+        HIPC(hipStreamSynchronize(B.st));
+    }
+    for (size_t k = 0; k < n; ++k) {
+        if (!lens[k]) continue;
+        const int32_t *r = recs.data() + k * kTraceRec;
+        out[k] = esch_alignment{r[0], r[1], r[2], r[3], r[4], r[5], (int64_t)out_off[k]};
+    }
+    return ESCH_OK;
+}

@@ -206,6 +206,28 @@ void esch_adapter_detect_pair(const uint8_t *r1, size_t l1, const uint8_t *r2, s
                               const uint8_t *a1, size_t la1, const uint8_t *a2, size_t la2,
                               size_t min_overlap, size_t out[2]);
 
+/* ---- winner tracebacks (lib/simd/mod.rs:405-499, lib/scalar.rs:164-364) ----
+ * The alignment of read queries[k] (codes, lens[k] bases) against panel
+ * reference refs[k], for every k: the score, the aligned spans (0-based,
+ * half-open: query[query_start..query_end] against
+ * reference[ref_start..ref_end]) and the run-length operations, written to
+ * ops[ops_offset .. ops_offset + n_ops) as (op << 30 | length) with op 0 = M,
+ * 1 = I, 2 = D. The optimum chosen among several is the reference's (first
+ * maximum of a column-major scan; diagonal, then insertion, then deletion;
+ * gaps extended before opened; local stops at the first zero). An empty read
+ * gives an empty alignment (all zero); so does a local alignment that cannot
+ * score above zero. *ops_used receives the number of runs; CAPACITY when
+ * ops_cap is smaller (read + reference length per pair always suffices). */
+typedef struct {
+    int32_t score;
+    int32_t query_start, query_end, ref_start, ref_end;
+    int32_t n_ops;
+    int64_t ops_offset;
+} esch_alignment;
+esch_status esch_align_pairs(esch_scorer *s, const uint8_t *const *queries, const size_t *lens,
+                             const int32_t *refs, size_t n, esch_alignment *out, uint32_t *ops,
+                             size_t ops_cap, size_t *ops_used);
+
 /* Kernel variants of the batched DP, and the autotuner that picks one by
  * timing each on a sample of the caller's reads (no counterpart in the
  * reference; the scores are identical whichever variant runs):

@@ -1,0 +1,122 @@
+"""Winner tracebacks on an MI355X (esch_align_pairs, the device counterpart
+of simd::align_pairs, lib/simd/mod.rs:405-499) against the oracle's
+traceback (eo_align, lib/scalar.rs:164-364), which is pinned to parasail's
+coordinates and to a restatement of the tie rules in test_oracle_align.py.
+Equality of score, coordinates and every run."""
+import numpy as np
+import pytest
+
+import eschalign_amd as ea
+from oracle import pyoracle as O
+import refgen as G
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_aligns(queries, panel_codes, refs, scoring, local):
+    out = []
+    for q, r in zip(queries, refs):
+        a = O.align(q, panel_codes[r], scoring, local)
+        out.append(ea.Alignment(a[0], a[1], a[2], a[3], a[4], tuple(a[5])))
+    return out
+
+
+def check(panel_l, queries, refs, scoring, mode):
+    panel = ea.Panel(panel_l)
+    gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+    got = gpu.align_pairs(queries, refs)
+    want = oracle_aligns(queries, panel.codes, refs, scoring, mode == ea.Mode.LOCAL)
+    for k, (g, w) in enumerate(zip(got, want)):
+        assert g == w, (k, len(queries[k]), refs[k], scoring, mode, g, w)
+    return got
+
+
+def test_trace_matches_parasail_golden_through_the_device():
+    """The 192 golden results (score and coordinates, parasail 1.3.4), each
+    pair traced on the device against a one-reference panel per pair."""
+    g = G.golden()
+    schemes = g["schemes"]
+    by = {}
+    for p in g["pairs"]:
+        for res in p["results"]:
+            by.setdefault((res["mode"], res["scheme"]), []).append((p, res))
+    n = 0
+    for (mode_s, scheme), items in by.items():
+        scoring = tuple(schemes[scheme])
+        mode = ea.Mode.LOCAL if mode_s == "local" else ea.Mode.SEMIGLOBAL
+        panel_l = [p["reference"].encode() for p, _ in items]
+        queries = [G.encode(p["query"].encode()) for p, _ in items]
+        got = check(panel_l, queries, list(range(len(items))), scoring, mode)
+        for a, (_, res) in zip(got, items):
+            assert a.score == res["score"]
+            assert (a.query_start, a.query_end, a.ref_start, a.ref_end) == (
+                res["query_start"], res["query_end"], res["ref_start"], res["ref_end"])
+            n += 1
+    assert n == 192
+
+
+def test_trace_random_pairs_every_scheme():
+    """Reads of lengths 1-700 (one to 44 stripes, ragged within a group)
+    against references of 1-500 nt with wildcards, chimeric and random
+    reads, five schemes, both modes."""
+    rng = G.Rng(0x5EED_0C01)
+    panel_l = [G.random_seq(rng, 1 + rng.below(500), 0.02) for _ in range(60)]
+    reads = [G.random_seq(rng, 1 + rng.below(60), 0.03) for _ in range(100)]
+    reads += [G.chimeric_read(rng, panel_l, 20 + rng.below(680)) for _ in range(200)]
+    queries = [G.encode(r) for r in reads]
+    refs = [rng.below(len(panel_l)) for _ in queries]
+    for scoring in G.SCHEMES_GPU:
+        for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
+            check(panel_l, queries, refs, scoring, mode)
+
+
+def test_trace_edges():
+    """Empty reads (empty alignment), one-base reads and references, a local
+    alignment that cannot score (empty), exact and reverse-complement
+    matches, the same pair many times in one group, and many groups."""
+    rng = G.Rng(0x5EED_0C02)
+    panel_l = [b"A", b"ACGT", b"CCCC", G.random_seq(rng, 300, 0.0), G.random_seq(rng, 17, 0.0)]
+    panel = ea.Panel(panel_l)
+    queries = [np.zeros(0, np.uint8), G.encode(b"A"), G.encode(b"T"), G.encode(b"AAAA"),
+               G.encode(panel_l[3]), ea.reverse_complement_codes(G.encode(panel_l[3])),
+               G.encode(panel_l[4][3:16])]
+    refs = [3, 0, 0, 2, 3, 3, 4]
+    queries += [G.encode(panel_l[4])] * 130
+    refs += [4] * 130
+    for scoring in ((2, -1, -10, -1), (1, -1, -2, -1)):
+        for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
+            got = check(panel_l, queries, refs, scoring, mode)
+            assert got[0] == ea.Alignment(0, 0, 0, 0, 0, ())
+            if mode == ea.Mode.LOCAL:
+                assert got[3].is_empty() and got[3].score == 0
+            assert got[4].cigar() == "300M" and got[4].score == 300 * scoring[0]
+    gpu = ea.GpuScorer(panel, ea.Scoring(), ea.Mode.LOCAL)
+    with pytest.raises(ea.GpuError):
+        gpu.align_pairs([G.encode(b"ACGT")], [len(panel_l)])
+    with pytest.raises(ea.GpuError):
+        gpu.align_pairs([np.array([0, 9], np.uint8)], [0])
+
+
+def test_trace_of_the_selected_winners():
+    """The path's use: score reads against the panel on the device, select
+    the winners (ties included), trace every winner on the device; equal to
+    the oracle's traceback of each (read or reverse complement, reference)."""
+    panel_l = G.fixture_panel("trna_reference.fa")
+    panel = ea.Panel(panel_l)
+    reads = [G.encode(r) for r in G.fixture_reads()]
+    rng = G.Rng(0x5EED_0C03)
+    reads += [G.encode(G.chimeric_read(rng, panel_l, 60 + rng.below(120))) for _ in range(200)]
+    scoring = (2, -1, -10, -1)
+    gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), ea.Mode.LOCAL)
+    sel = gpu.select_batch(reads, both_strands=True, tie_cap=8)
+    qs, rs = [], []
+    for k, s in enumerate(sel):
+        for ref, rev in s.take:
+            qs.append(ea.reverse_complement_codes(reads[k]) if rev else reads[k])
+            rs.append(ref)
+    assert len(qs) >= len(reads) // 2
+    got = gpu.align_pairs(qs, rs)
+    want = oracle_aligns(qs, panel.codes, rs, scoring, True)
+    assert got == want
+    for a, q, r in zip(got, qs, rs):
+        assert a.score == O.score(q, panel.codes[r], scoring, True)
