@@ -114,6 +114,59 @@ def host_cpu():
             "smt": None if not (siblings and cores) else siblings > cores}
 
 
+def trace_leg(gpu, batch, panel_codes, codes, off, lens, scoring, local, n_reads, cpu):
+    """The winner tracebacks of the first n_reads reads: every selected
+    (read or reverse complement, reference) of the step's fused selection,
+    traced on the device (esch_align_pairs); on a bounded sample, the
+    oracle's scalar restatement of the reference's traceback on the host
+    cores, checked equal (score, coordinates, runs)."""
+    import eschalign_amd as ea
+    recs, ties = batch.select_raw()
+    qs, rs = [], []
+    for k in range(min(n_reads, len(recs))):
+        if not recs[k, 3] & 1:
+            continue
+        q = codes[int(off[k]):int(off[k]) + int(lens[k])]
+        rc = None
+        for t in ties[k, :min(int(recs[k, 4]), ties.shape[1])]:
+            if int(t) & 1:
+                rc = ea.reverse_complement_codes(q) if rc is None else rc
+                qs.append(rc)
+            else:
+                qs.append(q)
+            rs.append(int(t) >> 1)
+    rlen = np.array([len(p) for p in panel_codes], np.int64)
+    cells = int(sum(len(q) * int(rlen[r]) for q, r in zip(qs, rs)))
+    gpu.align_pairs_raw(qs[:2000], rs[:2000])  # warm: code objects, allocator
+    t = time.perf_counter()
+    al, _ = gpu.align_pairs_raw(qs, rs)
+    wall = time.perf_counter() - t
+    kms = gpu.align_last_ms()
+    out = {"reads": min(n_reads, len(recs)), "pairs": len(qs), "cells": cells,
+           "pairs_per_s": round(len(qs) / wall, 1), "wall_ms": round(wall * 1e3, 2),
+           "kernel_ms": round(kms, 3), "kernel_gcups": round(cells / (kms / 1e3) / 1e9, 1),
+           "wall_gcups": round(cells / wall / 1e9, 1)}
+    if cpu:
+        from oracle import pyoracle as O
+        threads = min(16, os.cpu_count() or 1)
+        m = min(len(qs), 2000)
+        t = time.perf_counter()
+        O.align_batch(qs[:m], panel_codes, rs[:m], scoring, local, threads)
+        dt = time.perf_counter() - t
+        m = int(min(len(qs), max(m, m * 8.0 / max(dt, 1e-3))))  # about 8 s of CPU work
+        t = time.perf_counter()
+        ref = O.align_batch(qs[:m], panel_codes, rs[:m], scoring, local, threads)
+        dt = time.perf_counter() - t
+        assert np.array_equal(ref, al[:m, :6].astype(np.int32)), "device traceback != CPU traceback"
+        cpu_cells = sum(len(q) * int(rlen[r]) for q, r in zip(qs[:m], rs[:m]))
+        out["cpu_baseline"] = {"pairs_per_s": round(m / dt, 1), "cells_per_s": round(cpu_cells / dt, 1),
+                               "threads": threads, "kind": "port",
+                               "sample": f"first {m} traced pairs, scalar traceback restated from "
+                                         f"lib/scalar.rs:164-364 (the reference's AVX-512 traceback "
+                                         f"kernel is not restated), equal on every pair"}
+    return out
+
+
 def parse_lengths(text):
     """'lognormal:MEDIAN:SIGMA:LO:HI' or 'uniform:LO:HI' -> synth length spec."""
     kind, *vals = text.split(":")
@@ -234,6 +287,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-kernel comparison")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving run")
+    ap.add_argument("--no-trace", action="store_true", help="skip the winner-traceback leg")
+    ap.add_argument("--trace-reads", type=int, default=200_000,
+                    help="reads whose selected winners the traceback leg traces")
     ap.add_argument("--lengths", default="lognormal:90:0.6:20:1000",
                     help="read lengths: lognormal:MEDIAN:SIGMA:LO:HI or uniform:LO:HI")
     ap.add_argument("--diag-ratio", type=float, default=None,
@@ -343,6 +399,14 @@ def main():
         cpu = cpu_baseline(panel_codes, codes, off, lens, scoring, local, gpu_check)
         log(f"[bench] CPU baseline: {cpu}")
 
+    trace = None
+    if not args.no_trace:
+        batch.run(what, tie_cap=tie_cap)  # the selection the winners come from
+        batch.wait()
+        trace = trace_leg(gpu, batch, panel_codes, codes, off, lens, scoring, local, args.trace_reads,
+                          not args.no_cpu)
+        log(f"[bench] winner tracebacks: {trace}")
+
     serve = None
     if not args.no_serving:
         serve = serving(args.serve_rate, args.serve_seconds)
@@ -391,6 +455,7 @@ def main():
         "autotune_ms": [round(x, 3) for x in tune_ms],
         "plain_kernel": plain,
         "cpu_baseline": cpu,
+        "trace": trace,
         "serving": serve,
     }
     print(json.dumps(line), flush=True)

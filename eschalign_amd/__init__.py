@@ -64,7 +64,7 @@ EXPORTS = [
     "esch_server_scorer", "esch_server_submit", "esch_server_wait", "esch_server_result",
     "esch_server_stats", "esch_set_diag_min_len", "esch_get_diag_min_len",
     "esch_batch_stats_diag", "esch_set_diag_ratio", "esch_get_diag_ratio", "esch_valu_peak16",
-    "esch_align_pairs",
+    "esch_align_pairs", "esch_align_last_ms",
 ]
 
 
@@ -176,6 +176,8 @@ def load_library():
     L.esch_batch_stats_diag.argtypes = [vp, C.POINTER(C.c_uint64)]
     L.esch_set_diag_ratio.argtypes = [vp, C.c_int32]
     L.esch_align_pairs.argtypes = [vp, vp, vp, vp, sz, vp, vp, sz, C.POINTER(sz)]
+    L.esch_align_last_ms.argtypes = [vp]
+    L.esch_align_last_ms.restype = C.c_float
     L.esch_valu_peak16.argtypes = [C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.esch_get_diag_ratio.argtypes = [vp]
     _lib = L
@@ -204,12 +206,6 @@ class Alignment:
 
     def cigar(self) -> str:
         return "".join(f"{n}{op}" for op, n in self.ops)
-
-
-class _AlignRec(C.Structure):
-    _fields_ = [("score", C.c_int32), ("query_start", C.c_int32), ("query_end", C.c_int32),
-                ("ref_start", C.c_int32), ("ref_end", C.c_int32), ("n_ops", C.c_int32),
-                ("ops_offset", C.c_int64)]
 
 
 def valu_peak16(ordinal: int = 0) -> dict:
@@ -586,29 +582,45 @@ class GpuScorer:
     def diag_min_len(self, v: int):
         _check(_lib.esch_set_diag_min_len(self._h, int(v)))
 
-    def align_pairs(self, queries: Sequence[np.ndarray], refs: Sequence[int]) -> list:
-        """simd::align_pairs (lib/simd/mod.rs:405-499) on the device: the
-        alignment of queries[k] (codes) against panel reference refs[k], in
-        this scorer's scoring and mode."""
+    def align_pairs_raw(self, queries: Sequence[np.ndarray], refs: Sequence[int]):
+        """esch_align_pairs: (records int64 [n, 7] = score, query_start,
+        query_end, ref_start, ref_end, n_ops, ops_offset; ops uint32, each
+        op << 30 | length with op 0 = M, 1 = I, 2 = D)."""
         if len(queries) != len(refs):
             raise ValueError("one reference per query")
         keep, ptrs, lens = _ptr_array(queries)
         n = len(keep)
         refs_a = np.ascontiguousarray(refs, np.int32)
-        recs = (_AlignRec * max(n, 1))()
+        recs = np.zeros((max(n, 1), 4), np.int64)  # esch_alignment: 6 x int32, then int64 at byte 24
         nr = len(self._rlen)
-        cap = int(sum(len(q) for q in keep)) + int(sum(self._rlen[r] for r in refs_a if 0 <= r < nr)) + 1
+        rl = np.asarray(self._rlen, np.int64)
+        ok = (refs_a >= 0) & (refs_a < nr)
+        cap = int(sum(len(q) for q in keep)) + int(rl[refs_a[ok]].sum()) + 1
         ops = np.zeros(cap, np.uint32)
         used = C.c_size_t()
-        _check(_lib.esch_align_pairs(self._h, ptrs, lens, refs_a.ctypes.data, n, C.byref(recs),
+        _check(_lib.esch_align_pairs(self._h, ptrs, lens, refs_a.ctypes.data, n, recs.ctypes.data,
                                      ops.ctypes.data, cap, C.byref(used)))
+        r32 = recs.view(np.int32).reshape(-1, 8)[:n]
+        out = np.zeros((n, 7), np.int64)
+        out[:, :6] = r32[:, :6]
+        out[:, 6] = recs[:n, 3]
+        return out, ops[:used.value]
+
+    def align_pairs(self, queries: Sequence[np.ndarray], refs: Sequence[int]) -> list:
+        """simd::align_pairs (lib/simd/mod.rs:405-499) on the device: the
+        alignment of queries[k] (codes) against panel reference refs[k], in
+        this scorer's scoring and mode."""
+        recs, ops = self.align_pairs_raw(queries, refs)
         out = []
-        for k in range(n):
-            r = recs[k]
-            seg = ops[r.ops_offset:r.ops_offset + r.n_ops]
-            out.append(Alignment(r.score, r.query_start, r.query_end, r.ref_start, r.ref_end,
+        for r in recs:
+            seg = ops[int(r[6]):int(r[6]) + int(r[5])]
+            out.append(Alignment(int(r[0]), int(r[1]), int(r[2]), int(r[3]), int(r[4]),
                                  tuple(("MID"[int(x) >> 30], int(x) & 0x3FFFFFFF) for x in seg)))
         return out
+
+    def align_last_ms(self) -> float:
+        """Device time of the last align_pairs call's traceback kernels."""
+        return float(_lib.esch_align_last_ms(self._h))
 
     @property
     def diag_ratio(self) -> float:

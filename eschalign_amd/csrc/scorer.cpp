@@ -91,6 +91,7 @@ struct esch_scorer {
     std::mutex pool_mu;
     std::vector<esch_batch *> pool;  // idle batches for the drop-in API
     std::vector<int32_t> h_rlen;     // reference lengths (traceback sizing)
+    float trace_ms = 0.0f;           // device time of the last esch_align_pairs kernels
 };
 
 struct esch_batch {
@@ -320,8 +321,9 @@ esch_status esch_valu_peak16(int32_t ordinal, double *lane_ops_per_s, int *n_cu,
     HIPC(hipGetDeviceProperties(&prop, ordinal));
     if (n_cu) *n_cu = prop.multiProcessorCount;
     if (clock_khz) *clock_khz = prop.clockRate;
-    // 8 workgroups of 4 waves per CU: 8 waves per SIMD, every issue slot covered
-    const int blocks = 8 * prop.multiProcessorCount, iters = 1 << 14;
+    // 8 workgroups of 4 waves per CU: 8 waves per SIMD, every issue slot
+    // covered; a ~15 ms warm-up, then ~110 ms timed, so the clock has ramped
+    const int blocks = 8 * prop.multiProcessorCount, iters = 1 << 19;
     uint32_t *out = nullptr;
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -330,7 +332,7 @@ esch_status esch_valu_peak16(int32_t ordinal, double *lane_ops_per_s, int *n_cu,
     if (hipMalloc(&out, (size_t)blocks * 256 * 4) != hipSuccess || hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
         rc = fail(ESCH_DRIVER, "allocation for the peak probe failed");
-    } else if (launch_peak16(out, blocks, 64, st) != hipSuccess || hipEventRecord(e0, st) != hipSuccess ||
+    } else if (launch_peak16(out, blocks, 1 << 16, st) != hipSuccess || hipEventRecord(e0, st) != hipSuccess ||
                launch_peak16(out, blocks, iters, st) != hipSuccess || hipEventRecord(e1, st) != hipSuccess ||
                hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
         rc = fail(ESCH_DRIVER, "peak probe launch failed: %s", hipGetErrorString(hipGetLastError()));
@@ -1051,9 +1053,12 @@ namespace {
 struct TraceBufs {
     std::vector<void *> ptrs;
     hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     ~TraceBufs() {
         if (st) (void)hipStreamSynchronize(st);
         for (void *p : ptrs) (void)hipFree(p);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
         if (st) (void)hipStreamDestroy(st);
     }
     template <class T>
@@ -1183,13 +1188,18 @@ esch_status esch_align_pairs(esch_scorer *s, const uint8_t *const *queries, cons
     TraceArgs ta{s->panel, s->sc, d_codes, d_qoff, d_qlen, d_ref, d_slot, d_gbase, d_gS, d_gM,
                  d_arena, d_recs, d_ops_wc, d_ops_off, (int32_t)np, 0};
     const bool local = s->sc.local != 0;
+    HIPC(hipEventCreate(&B.e0));
+    HIPC(hipEventCreate(&B.e1));
+    HIPC(hipEventRecord(B.e0, B.st));
     for (const auto &c : chunks) {
         ta.g0 = c.first;
         HIPC(launch_trace(ta, c.second - c.first, local, B.st));
     }
+    HIPC(hipEventRecord(B.e1, B.st));
     std::vector<int32_t> recs(n * kTraceRec, 0);
     HIPC(hipMemcpyAsync(recs.data(), d_recs, n * kTraceRec * 4, hipMemcpyDeviceToHost, B.st));
     HIPC(hipStreamSynchronize(B.st));
+    HIPC(hipEventElapsedTime(&s->trace_ms, B.e0, B.e1));
     std::vector<uint64_t> out_off(n, 0);
     size_t runs = 0;
     for (size_t k = 0; k < n; ++k) {
@@ -1215,3 +1225,5 @@ esch_status esch_align_pairs(esch_scorer *s, const uint8_t *const *queries, cons
     }
     return ESCH_OK;
 }
+
+float esch_align_last_ms(const esch_scorer *s) { return s ? s->trace_ms : 0.0f; }

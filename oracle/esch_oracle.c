@@ -207,6 +207,64 @@ int eo_align(const uint8_t *q, size_t n, const uint8_t *r, size_t m, int32_t ma,
     return nops;
 }
 
+/* eo_align over pairs (query k against reference refs[k]) on `threads`
+ * workers, pairs dealt in interleaved order: recs[k * 6] = score,
+ * query_start, query_end, ref_start, ref_end, runs (the CPU side of the
+ * traceback measurement and its equality check). */
+typedef struct {
+    const uint8_t *qcodes, *rcodes;
+    const uint64_t *qoff, *roff;
+    const int32_t *qlen, *rlen, *refs;
+    size_t n;
+    int32_t ma, mi, o, e;
+    int local, t, threads;
+    int32_t *recs;
+} align_job;
+
+static void *align_worker(void *arg) {
+    const align_job *j = (const align_job *)arg;
+    size_t cap = 0;
+    uint32_t *ops = NULL;
+    for (size_t k = (size_t)j->t; k < j->n; k += (size_t)j->threads) {
+        const int32_t r = j->refs[k];
+        const size_t need = (size_t)j->qlen[k] + (size_t)j->rlen[r] + 1;
+        if (need > cap) {
+            free(ops);
+            cap = need;
+            ops = (uint32_t *)malloc(cap * sizeof(uint32_t));
+        }
+        eo_alignment a;
+        const int runs = eo_align(j->qcodes + j->qoff[k], (size_t)j->qlen[k], j->rcodes + j->roff[r],
+                                  (size_t)j->rlen[r], j->ma, j->mi, j->o, j->e, j->local, &a, ops, cap);
+        int32_t *o = j->recs + k * 6;
+        o[0] = a.score;
+        o[1] = a.query_start;
+        o[2] = a.query_end;
+        o[3] = a.ref_start;
+        o[4] = a.ref_end;
+        o[5] = runs;
+    }
+    free(ops);
+    return NULL;
+}
+
+void eo_align_batch(const uint8_t *qcodes, const uint64_t *qoff, const int32_t *qlen,
+                    const uint8_t *rcodes, const uint64_t *roff, const int32_t *rlen,
+                    const int32_t *refs, size_t n, int32_t ma, int32_t mi, int32_t o,
+                    int32_t e, int local, int threads, int32_t *recs) {
+    if (threads < 1) threads = 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)threads);
+    align_job *jobs = (align_job *)malloc(sizeof(align_job) * (size_t)threads);
+    for (int t = 0; t < threads; t++) {
+        align_job j = {qcodes, rcodes, qoff, roff, qlen, rlen, refs, n, ma, mi, o, e, local, t, threads, recs};
+        jobs[t] = j;
+        pthread_create(&th[t], NULL, align_worker, &jobs[t]);
+    }
+    for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+    free(th);
+    free(jobs);
+}
+
 /* lib/simd/mod.rs:222-227 */
 int eo_fits_i16(size_t n, size_t max_ref_len, int32_t ma, int32_t mi,
                 int32_t o, int32_t e) {

@@ -68,6 +68,13 @@ int eo_align(const uint8_t *q, size_t n, const uint8_t *r, size_t m, int32_t ma,
              int32_t mi, int32_t o, int32_t e, int local, eo_alignment *out,
              uint32_t *ops, size_t ops_cap);
 
+/* eo_align over pairs on `threads` workers; recs[k * 6] = score,
+ * query_start, query_end, ref_start, ref_end, number of runs. */
+void eo_align_batch(const uint8_t *qcodes, const uint64_t *qoff, const int32_t *qlen,
+                    const uint8_t *rcodes, const uint64_t *roff, const int32_t *rlen,
+                    const int32_t *refs, size_t n, int32_t ma, int32_t mi, int32_t o,
+                    int32_t e, int local, int threads, int32_t *recs);
+
 /* lib/simd/mod.rs:222-227 */
 int eo_fits_i16(size_t n, size_t max_ref_len, int32_t ma, int32_t mi,
                 int32_t o, int32_t e);

@@ -42,6 +42,8 @@ def lib():
         L.eo_align.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t] + [C.c_int32] * 4 + \
             [C.c_int, C.c_void_p, C.c_void_p, C.c_size_t]
         L.eo_align.restype = C.c_int
+        L.eo_align_batch.argtypes = [U8P, U64P, I32P, U8P, U64P, I32P, I32P, C.c_size_t] + \
+            [C.c_int32] * 4 + [C.c_int, C.c_int, I32P]
         L.eo_fits_i16.restype = C.c_int
         L.eo_score_matrix.argtypes = [U8P, U64P, I32P, C.c_size_t, U8P, U64P, I32P,
                                       C.c_size_t, C.c_int32, C.c_int32, C.c_int32,
@@ -120,6 +122,20 @@ def align(q, r, scoring, local: bool):
     assert k >= 0
     return (rec[0], rec[1], rec[2], rec[3], rec[4],
             [(OPS[int(x) >> 30], int(x) & 0x3FFFFFFF) for x in ops[:k]])
+
+
+def align_batch(queries, refs_codes, ref_idx, scoring, local: bool, threads: int | None = None):
+    """eo_align for query k against refs_codes[ref_idx[k]] on threads:
+    int32 [n, 6] = score, query_start, query_end, ref_start, ref_end, runs."""
+    threads = threads or min(8, os.cpu_count() or 1)
+    qc, qo, ql = concat(queries)
+    rc, ro, rl = concat(refs_codes)
+    idx = np.ascontiguousarray(ref_idx, np.int32)
+    out = np.zeros((max(len(queries), 1), 6), np.int32)
+    if len(queries):
+        lib().eo_align_batch(qc, qo, ql, rc, ro, rl, idx, len(queries), *scoring, int(local),
+                             threads, out.reshape(-1))
+    return out[:len(queries)]
 
 
 def fits_i16(n, max_ref_len, scoring) -> bool:

@@ -153,3 +153,17 @@ def test_adapter_equals_oracle_random():
             read = read[: n - k] + bytes(tail)
         mo = int(rng.integers(1, 8))
         assert ea.adapter_find(read, R1, mo) == O.adapter_find(read, R1, mo)
+
+
+def test_alignment_record_layout_matches_the_header(tmp_path):
+    """The Python mirror reads esch_alignment as 32-byte records with the
+    int64 ops offset at byte 24; the header's struct, compiled, agrees."""
+    import subprocess
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "eschalign_amd.h"\n'
+                   'int main(void) { printf("%zu %zu", sizeof(esch_alignment), '
+                   'offsetof(esch_alignment, ops_offset)); return 0; }\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    assert out.split() == ["32", "24"]
