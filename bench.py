@@ -117,48 +117,46 @@ def host_cpu():
 def trace_leg(gpu, batch, panel_codes, codes, off, lens, scoring, local, n_reads, cpu):
     """The winner tracebacks of the first n_reads reads: every selected
     (read or reverse complement, reference) of the step's fused selection,
-    traced on the device (esch_align_pairs); on a bounded sample, the
-    oracle's scalar restatement of the reference's traceback on the host
+    traced on the device (esch_align_pairs_concat); on a bounded sample,
+    the oracle's scalar restatement of the reference's traceback on the host
     cores, checked equal (score, coordinates, runs)."""
-    import eschalign_amd as ea
     recs, ties = batch.select_raw()
-    qs, rs = [], []
-    for k in range(min(n_reads, len(recs))):
-        if not recs[k, 3] & 1:
-            continue
-        q = codes[int(off[k]):int(off[k]) + int(lens[k])]
-        rc = None
-        for t in ties[k, :min(int(recs[k, 4]), ties.shape[1])]:
-            if int(t) & 1:
-                rc = ea.reverse_complement_codes(q) if rc is None else rc
-                qs.append(rc)
-            else:
-                qs.append(q)
-            rs.append(int(t) >> 1)
+    nr = min(n_reads, len(recs))
+    take = np.where((recs[:nr, 3] & 1) != 0, np.minimum(recs[:nr, 4], ties.shape[1]), 0)
+    mask = np.arange(ties.shape[1])[None, :] < take[:, None]
+    k_idx = np.nonzero(mask)[0]          # read order, then tie order
+    ent = ties[:nr][mask].astype(np.int64)
+    refs, rev = (ent >> 1).astype(np.int32), (ent & 1) != 0
+    # reverse complements as slices of the whole code array reverse-complemented
+    total = len(codes)
+    big = np.concatenate([codes, np.where(codes < 4, 3 - codes, 4).astype(np.uint8)[::-1]])
+    o, ln = off[k_idx].astype(np.int64), lens[k_idx].astype(np.int64)
+    qoff = np.where(rev, total + (total - o - ln), o).astype(np.uint64)
+    qlen = ln.astype(np.int32)
     rlen = np.array([len(p) for p in panel_codes], np.int64)
-    cells = int(sum(len(q) * int(rlen[r]) for q, r in zip(qs, rs)))
-    gpu.align_pairs_raw(qs[:2000], rs[:2000])  # warm: code objects, allocator
+    cells = int((ln * rlen[refs]).sum())
+    gpu.align_pairs_concat(big, qoff[:2000], qlen[:2000], refs[:2000])  # warm: code objects, buffers
     t = time.perf_counter()
-    al, _ = gpu.align_pairs_raw(qs, rs)
+    al, _ = gpu.align_pairs_concat(big, qoff, qlen, refs)
     wall = time.perf_counter() - t
     kms = gpu.align_last_ms()
-    out = {"reads": min(n_reads, len(recs)), "pairs": len(qs), "cells": cells,
-           "pairs_per_s": round(len(qs) / wall, 1), "wall_ms": round(wall * 1e3, 2),
+    out = {"reads": int(nr), "pairs": int(len(refs)), "cells": cells,
+           "pairs_per_s": round(len(refs) / wall, 1), "wall_ms": round(wall * 1e3, 2),
            "kernel_ms": round(kms, 3), "kernel_gcups": round(cells / (kms / 1e3) / 1e9, 1),
            "wall_gcups": round(cells / wall / 1e9, 1)}
     if cpu:
         from oracle import pyoracle as O
         threads = min(16, os.cpu_count() or 1)
-        m = min(len(qs), 2000)
+        m = min(len(refs), 2000)
         t = time.perf_counter()
-        O.align_batch(qs[:m], panel_codes, rs[:m], scoring, local, threads)
+        O.align_batch_concat(big, qoff[:m], qlen[:m], panel_codes, refs[:m], scoring, local, threads)
         dt = time.perf_counter() - t
-        m = int(min(len(qs), max(m, m * 8.0 / max(dt, 1e-3))))  # about 8 s of CPU work
+        m = int(min(len(refs), max(m, m * 8.0 / max(dt, 1e-3))))  # about 8 s of CPU work
         t = time.perf_counter()
-        ref = O.align_batch(qs[:m], panel_codes, rs[:m], scoring, local, threads)
+        ref = O.align_batch_concat(big, qoff[:m], qlen[:m], panel_codes, refs[:m], scoring, local, threads)
         dt = time.perf_counter() - t
         assert np.array_equal(ref, al[:m, :6].astype(np.int32)), "device traceback != CPU traceback"
-        cpu_cells = sum(len(q) * int(rlen[r]) for q, r in zip(qs[:m], rs[:m]))
+        cpu_cells = int((ln[:m] * rlen[refs[:m]]).sum())
         out["cpu_baseline"] = {"pairs_per_s": round(m / dt, 1), "cells_per_s": round(cpu_cells / dt, 1),
                                "threads": threads, "kind": "port",
                                "sample": f"first {m} traced pairs, scalar traceback restated from "

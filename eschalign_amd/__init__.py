@@ -64,7 +64,7 @@ EXPORTS = [
     "esch_server_scorer", "esch_server_submit", "esch_server_wait", "esch_server_result",
     "esch_server_stats", "esch_set_diag_min_len", "esch_get_diag_min_len",
     "esch_batch_stats_diag", "esch_set_diag_ratio", "esch_get_diag_ratio", "esch_valu_peak16",
-    "esch_align_pairs", "esch_align_last_ms",
+    "esch_align_pairs", "esch_align_last_ms", "esch_align_pairs_concat",
 ]
 
 
@@ -177,6 +177,7 @@ def load_library():
     L.esch_set_diag_ratio.argtypes = [vp, C.c_int32]
     L.esch_align_pairs.argtypes = [vp, vp, vp, vp, sz, vp, vp, sz, C.POINTER(sz)]
     L.esch_align_last_ms.argtypes = [vp]
+    L.esch_align_pairs_concat.argtypes = [vp, vp, vp, vp, vp, sz, vp, vp, sz, C.POINTER(sz)]
     L.esch_align_last_ms.restype = C.c_float
     L.esch_valu_peak16.argtypes = [C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.esch_get_diag_ratio.argtypes = [vp]
@@ -600,6 +601,31 @@ class GpuScorer:
         used = C.c_size_t()
         _check(_lib.esch_align_pairs(self._h, ptrs, lens, refs_a.ctypes.data, n, recs.ctypes.data,
                                      ops.ctypes.data, cap, C.byref(used)))
+        r32 = recs.view(np.int32).reshape(-1, 8)[:n]
+        out = np.zeros((n, 7), np.int64)
+        out[:, :6] = r32[:, :6]
+        out[:, 6] = recs[:n, 3]
+        return out, ops[:used.value]
+
+    def align_pairs_concat(self, codes: np.ndarray, offs: np.ndarray, lens: np.ndarray, refs):
+        """esch_align_pairs_concat: read k is codes[offs[k]:offs[k] + lens[k]];
+        returns what align_pairs_raw does."""
+        codes = np.ascontiguousarray(codes, np.uint8)
+        offs = np.ascontiguousarray(offs, np.uint64)
+        lens = np.ascontiguousarray(lens, np.int32)
+        refs_a = np.ascontiguousarray(refs, np.int32)
+        n = len(lens)
+        if len(refs_a) != n:
+            raise ValueError("one reference per query")
+        rl = np.asarray(self._rlen, np.int64)
+        ok = (refs_a >= 0) & (refs_a < len(rl))
+        cap = int(lens.astype(np.int64).sum()) + int(rl[refs_a[ok]].sum()) + 1
+        recs = np.zeros((max(n, 1), 4), np.int64)
+        ops = np.zeros(cap, np.uint32)
+        used = C.c_size_t()
+        _check(_lib.esch_align_pairs_concat(self._h, codes.ctypes.data, offs.ctypes.data, lens.ctypes.data,
+                                            refs_a.ctypes.data, n, recs.ctypes.data, ops.ctypes.data, cap,
+                                            C.byref(used)))
         r32 = recs.view(np.int32).reshape(-1, 8)[:n]
         out = np.zeros((n, 7), np.int64)
         out[:, :6] = r32[:, :6]

@@ -69,6 +69,9 @@ bool fits_i16(size_t n, size_t L, int32_t ma, int32_t mi, int32_t o, int32_t e) 
 
 }  // namespace
 
+struct TraceCache;
+void trace_cache_free(esch_scorer *s);
+
 struct esch_scorer {
     int ordinal = 0;
     std::string device_name;
@@ -92,6 +95,7 @@ struct esch_scorer {
     std::vector<esch_batch *> pool;  // idle batches for the drop-in API
     std::vector<int32_t> h_rlen;     // reference lengths (traceback sizing)
     float trace_ms = 0.0f;           // device time of the last esch_align_pairs kernels
+    TraceCache *trace = nullptr;     // traceback buffers kept between calls
 };
 
 struct esch_batch {
@@ -288,6 +292,7 @@ void esch_batch_free(esch_batch *b);
 void esch_scorer_free(esch_scorer *s) {
     if (!s) return;
     for (esch_batch *b : s->pool) esch_batch_free(b);
+    trace_cache_free(s);
     (void)hipFree(s->d_ref2);
     (void)hipFree(s->d_refN);
     (void)hipFree(s->d_w2off);
@@ -1049,55 +1054,93 @@ esch_status esch_select_pairs(esch_scorer *s, const uint8_t *const *mates1, cons
 
 namespace {
 
-// Device buffers of one esch_align_pairs call, released on every path.
-struct TraceBufs {
-    std::vector<void *> ptrs;
-    hipStream_t st = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    ~TraceBufs() {
-        if (st) (void)hipStreamSynchronize(st);
-        for (void *p : ptrs) (void)hipFree(p);
-        if (e0) (void)hipEventDestroy(e0);
-        if (e1) (void)hipEventDestroy(e1);
-        if (st) (void)hipStreamDestroy(st);
-    }
-    template <class T>
-    hipError_t alloc(T **p, size_t count) {
-        *p = nullptr;
-        const hipError_t e = hipMalloc(reinterpret_cast<void **>(p), std::max<size_t>(count, 1) * sizeof(T));
-        if (e == hipSuccess) ptrs.push_back(*p);
-        return e;
-    }
-};
-
-constexpr size_t kTraceArenaMax = (size_t)4 << 30;  // trace + boundary bytes per launch
+constexpr size_t kTraceArenaMax = (size_t)4 << 30;      // trace + boundary bytes per launch
+constexpr size_t kTraceArenaKeep = (size_t)1 << 30;     // cached across calls up to this
 constexpr uint64_t kTraceMaxCells = (uint64_t)1 << 27;  // read x reference: a 4 GiB group region
 
 }  // namespace
 
-esch_status esch_align_pairs(esch_scorer *s, const uint8_t *const *queries, const size_t *lens,
-                             const int32_t *refs, size_t n, esch_alignment *out, uint32_t *ops,
-                             size_t ops_cap, size_t *ops_used) {
-    if (!s || (n && (!queries || !lens || !refs || !out))) return fail(ESCH_INVALID_ARG, "null argument");
+// Grow-only device buffer of the traceback cache.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t need(size_t bytes) {
+        bytes = std::max<size_t>(bytes, 256);
+        if (bytes <= cap) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        const hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipSuccess) cap = bytes;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+// Device buffers, stream and events of esch_align_pairs, kept by the scorer
+// between calls (one call at a time per scorer).
+struct TraceCache {
+    std::mutex mu;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    DevBuf codes, qoff, qlen, ref, slot, gbase, gS, gM, arena, recs, ops_wc, ops_off, out_off, out;
+    ~TraceCache() {
+        if (st) (void)hipStreamSynchronize(st);
+        for (DevBuf *b : {&codes, &qoff, &qlen, &ref, &slot, &gbase, &gS, &gM, &arena, &recs, &ops_wc, &ops_off,
+                          &out_off, &out})
+            b->release();
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+void trace_cache_free(esch_scorer *s) {
+    if (s->trace) {
+        (void)hipSetDevice(s->ordinal);
+        delete s->trace;
+        s->trace = nullptr;
+    }
+}
+
+// The traceback of n pairs whose reads come from get(k) -> (codes, length).
+static esch_status align_core(esch_scorer *s, size_t n,
+                              const std::function<void(size_t, const uint8_t *&, size_t &)> &get,
+                              const int32_t *refs, esch_alignment *out, uint32_t *ops, size_t ops_cap,
+                              size_t *ops_used) {
     if (ops_used) *ops_used = 0;
     std::vector<int32_t> order;
     order.reserve(n);
+    std::vector<size_t> lens(n);
     size_t bases = 0;
     for (size_t k = 0; k < n; ++k) {
+        const uint8_t *q;
+        size_t len;
+        get(k, q, len);
+        lens[k] = len;
         if (refs[k] < 0 || (size_t)refs[k] >= s->n_refs)
             return fail(ESCH_INVALID_ARG, "pair %zu names reference %d of %zu", k, refs[k], s->n_refs);
         // a group region is 64 pairs x (read / 16 stripes) x reference x 8 B
-        if ((uint64_t)lens[k] * (uint64_t)s->h_rlen[refs[k]] > kTraceMaxCells)
-            return fail(ESCH_INVALID_ARG, "pair %zu: %zu x %d cells is over the device traceback bound %llu", k,
-                        lens[k], s->h_rlen[refs[k]], (unsigned long long)kTraceMaxCells);
-        if (lens[k] && !queries[k]) return fail(ESCH_INVALID_ARG, "pair %zu: null read", k);
-        for (size_t j = 0; j < lens[k]; ++j)
-            if (queries[k][j] > 4)
-                return fail(ESCH_INVALID_ARG, "read %zu carries code %d (codes are 0..=4)", k, queries[k][j]);
+        if ((uint64_t)len * (uint64_t)s->h_rlen[refs[k]] > kTraceMaxCells)
+            return fail(ESCH_INVALID_ARG, "pair %zu: %zu x %d cells is over the device traceback bound %llu", k, len,
+                        s->h_rlen[refs[k]], (unsigned long long)kTraceMaxCells);
+        if (len && !q) return fail(ESCH_INVALID_ARG, "pair %zu: null read", k);
+        uint8_t mx = 0;
+        for (size_t j = 0; j < len; ++j) mx = q[j] > mx ? q[j] : mx;
+        if (mx > 4) {
+            for (size_t j = 0; j < len; ++j)
+                if (q[j] > 4) return fail(ESCH_INVALID_ARG, "read %zu carries code %d (codes are 0..=4)", k, q[j]);
+        }
         out[k] = esch_alignment{};  // lib/scalar.rs:185-195: empty in, empty out
-        if (lens[k]) {
+        if (len) {
             order.push_back((int32_t)k);
-            bases += lens[k];
+            bases += len;
         }
     }
     const size_t np = order.size();
@@ -1113,12 +1156,15 @@ esch_status esch_align_pairs(esch_scorer *s, const uint8_t *const *queries, cons
     size_t at = 0;
     for (size_t i = 0; i < np; ++i) {
         const int32_t k = order[i];
-        memcpy(hcodes.data() + at, queries[k], lens[k]);
+        const uint8_t *q;
+        size_t len;
+        get(k, q, len);
+        memcpy(hcodes.data() + at, q, len);
         hqoff[i] = at;
-        hqlen[i] = (int32_t)lens[k];
+        hqlen[i] = (int32_t)len;
         href[i] = refs[k];
         hslot[i] = k;
-        at += lens[k];
+        at += len;
     }
     size_t wc = 0;  // worst-case runs: read + reference length per pair
     for (size_t k = 0; k < n; ++k) {
@@ -1155,51 +1201,52 @@ esch_status esch_align_pairs(esch_scorer *s, const uint8_t *const *queries, cons
     }
 
     HIPC(hipSetDevice(s->ordinal));
-    TraceBufs B;
-    HIPC(hipStreamCreateWithFlags(&B.st, hipStreamNonBlocking));
-    uint8_t *d_codes = nullptr, *d_arena = nullptr;
-    uint64_t *d_qoff = nullptr, *d_gbase = nullptr, *d_ops_off = nullptr, *d_out_off = nullptr;
-    int32_t *d_qlen = nullptr, *d_ref = nullptr, *d_slot = nullptr, *d_gS = nullptr, *d_gM = nullptr,
-            *d_recs = nullptr;
-    uint32_t *d_ops_wc = nullptr, *d_out = nullptr;
-    HIPC(B.alloc(&d_codes, bases));
-    HIPC(B.alloc(&d_qoff, np));
-    HIPC(B.alloc(&d_qlen, np));
-    HIPC(B.alloc(&d_ref, np));
-    HIPC(B.alloc(&d_slot, np));
-    HIPC(B.alloc(&d_gbase, ng));
-    HIPC(B.alloc(&d_gS, ng));
-    HIPC(B.alloc(&d_gM, ng));
-    HIPC(B.alloc(&d_arena, arena));
-    HIPC(B.alloc(&d_recs, n * kTraceRec));
-    HIPC(B.alloc(&d_ops_wc, wc));
-    HIPC(B.alloc(&d_ops_off, n));
-    HIPC(hipMemcpyAsync(d_codes, hcodes.data(), bases, hipMemcpyHostToDevice, B.st));
-    HIPC(hipMemcpyAsync(d_qoff, hqoff.data(), np * 8, hipMemcpyHostToDevice, B.st));
-    HIPC(hipMemcpyAsync(d_qlen, hqlen.data(), np * 4, hipMemcpyHostToDevice, B.st));
-    HIPC(hipMemcpyAsync(d_ref, href.data(), np * 4, hipMemcpyHostToDevice, B.st));
-    HIPC(hipMemcpyAsync(d_slot, hslot.data(), np * 4, hipMemcpyHostToDevice, B.st));
-    HIPC(hipMemcpyAsync(d_gbase, gbase.data(), (size_t)ng * 8, hipMemcpyHostToDevice, B.st));
-    HIPC(hipMemcpyAsync(d_gS, gS.data(), (size_t)ng * 4, hipMemcpyHostToDevice, B.st));
-    HIPC(hipMemcpyAsync(d_gM, gM.data(), (size_t)ng * 4, hipMemcpyHostToDevice, B.st));
-    HIPC(hipMemcpyAsync(d_ops_off, ops_off.data(), n * 8, hipMemcpyHostToDevice, B.st));
+    if (!s->trace) s->trace = new TraceCache;
+    TraceCache &T = *s->trace;
+    std::lock_guard<std::mutex> lock(T.mu);
+    if (!T.st) HIPC(hipStreamCreateWithFlags(&T.st, hipStreamNonBlocking));
+    if (!T.e0) HIPC(hipEventCreate(&T.e0));
+    if (!T.e1) HIPC(hipEventCreate(&T.e1));
+    HIPC(T.codes.need(bases));
+    HIPC(T.qoff.need(np * 8));
+    HIPC(T.qlen.need(np * 4));
+    HIPC(T.ref.need(np * 4));
+    HIPC(T.slot.need(np * 4));
+    HIPC(T.gbase.need((size_t)ng * 8));
+    HIPC(T.gS.need((size_t)ng * 4));
+    HIPC(T.gM.need((size_t)ng * 4));
+    HIPC(T.arena.need(arena));
+    HIPC(T.recs.need(n * kTraceRec * 4));
+    HIPC(T.ops_wc.need(wc * 4));
+    HIPC(T.ops_off.need(n * 8));
+    hipStream_t st = T.st;
+    HIPC(hipMemcpyAsync(T.codes.p, hcodes.data(), bases, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(T.qoff.p, hqoff.data(), np * 8, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(T.qlen.p, hqlen.data(), np * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(T.ref.p, href.data(), np * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(T.slot.p, hslot.data(), np * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(T.gbase.p, gbase.data(), (size_t)ng * 8, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(T.gS.p, gS.data(), (size_t)ng * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(T.gM.p, gM.data(), (size_t)ng * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(T.ops_off.p, ops_off.data(), n * 8, hipMemcpyHostToDevice, st));
     // empty pairs are never traced: their records must read as zero runs
-    HIPC(hipMemsetAsync(d_recs, 0, n * kTraceRec * 4, B.st));
-    TraceArgs ta{s->panel, s->sc, d_codes, d_qoff, d_qlen, d_ref, d_slot, d_gbase, d_gS, d_gM,
-                 d_arena, d_recs, d_ops_wc, d_ops_off, (int32_t)np, 0};
+    HIPC(hipMemsetAsync(T.recs.p, 0, n * kTraceRec * 4, st));
+    TraceArgs ta{s->panel, s->sc, T.codes.as<uint8_t>(), T.qoff.as<uint64_t>(), T.qlen.as<int32_t>(),
+                 T.ref.as<int32_t>(), T.slot.as<int32_t>(), T.gbase.as<uint64_t>(), T.gS.as<int32_t>(),
+                 T.gM.as<int32_t>(), T.arena.as<uint8_t>(), T.recs.as<int32_t>(), T.ops_wc.as<uint32_t>(),
+                 T.ops_off.as<uint64_t>(), (int32_t)np, 0};
     const bool local = s->sc.local != 0;
-    HIPC(hipEventCreate(&B.e0));
-    HIPC(hipEventCreate(&B.e1));
-    HIPC(hipEventRecord(B.e0, B.st));
+    HIPC(hipEventRecord(T.e0, st));
     for (const auto &c : chunks) {
         ta.g0 = c.first;
-        HIPC(launch_trace(ta, c.second - c.first, local, B.st));
+        HIPC(launch_trace(ta, c.second - c.first, local, st));
     }
-    HIPC(hipEventRecord(B.e1, B.st));
+    HIPC(hipEventRecord(T.e1, st));
     std::vector<int32_t> recs(n * kTraceRec, 0);
-    HIPC(hipMemcpyAsync(recs.data(), d_recs, n * kTraceRec * 4, hipMemcpyDeviceToHost, B.st));
-    HIPC(hipStreamSynchronize(B.st));
-    HIPC(hipEventElapsedTime(&s->trace_ms, B.e0, B.e1));
+    HIPC(hipMemcpyAsync(recs.data(), T.recs.p, n * kTraceRec * 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    HIPC(hipEventElapsedTime(&s->trace_ms, T.e0, T.e1));
+    if (T.arena.cap > kTraceArenaKeep) T.arena.release();
     std::vector<uint64_t> out_off(n, 0);
     size_t runs = 0;
     for (size_t k = 0; k < n; ++k) {
@@ -1210,13 +1257,14 @@ esch_status esch_align_pairs(esch_scorer *s, const uint8_t *const *queries, cons
     if (runs > ops_cap || (runs && !ops))
         return fail(ESCH_CAPACITY, "the alignments have %zu operation runs; ops holds %zu", runs, ops_cap);
     if (runs) {
-        HIPC(B.alloc(&d_out_off, n));
-        HIPC(B.alloc(&d_out, runs));
+        HIPC(T.out_off.need(n * 8));
+        HIPC(T.out.need(runs * 4));
         // untraced (empty) pairs have zero runs and copy nothing
-        HIPC(hipMemcpyAsync(d_out_off, out_off.data(), n * 8, hipMemcpyHostToDevice, B.st));
-        HIPC(launch_trace_compact(d_ops_wc, d_ops_off, d_recs, d_out_off, d_out, (int32_t)n, B.st));
-        HIPC(hipMemcpyAsync(ops, d_out, runs * 4, hipMemcpyDeviceToHost, B.st));
-        HIPC(hipStreamSynchronize(B.st));
+        HIPC(hipMemcpyAsync(T.out_off.p, out_off.data(), n * 8, hipMemcpyHostToDevice, st));
+        HIPC(launch_trace_compact(T.ops_wc.as<uint32_t>(), T.ops_off.as<uint64_t>(), T.recs.as<int32_t>(),
+                                  T.out_off.as<uint64_t>(), T.out.as<uint32_t>(), (int32_t)n, st));
+        HIPC(hipMemcpyAsync(ops, T.out.p, runs * 4, hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
     }
     for (size_t k = 0; k < n; ++k) {
         if (!lens[k]) continue;
@@ -1224,6 +1272,32 @@ esch_status esch_align_pairs(esch_scorer *s, const uint8_t *const *queries, cons
         out[k] = esch_alignment{r[0], r[1], r[2], r[3], r[4], r[5], (int64_t)out_off[k]};
     }
     return ESCH_OK;
+}
+
+esch_status esch_align_pairs(esch_scorer *s, const uint8_t *const *queries, const size_t *lens,
+                             const int32_t *refs, size_t n, esch_alignment *out, uint32_t *ops,
+                             size_t ops_cap, size_t *ops_used) {
+    if (!s || (n && (!queries || !lens || !refs || !out))) return fail(ESCH_INVALID_ARG, "null argument");
+    return align_core(
+        s, n,
+        [&](size_t k, const uint8_t *&p, size_t &len) {
+            p = queries[k];
+            len = lens[k];
+        },
+        refs, out, ops, ops_cap, ops_used);
+}
+
+esch_status esch_align_pairs_concat(esch_scorer *s, const uint8_t *codes, const uint64_t *offs,
+                                    const int32_t *lens, const int32_t *refs, size_t n, esch_alignment *out,
+                                    uint32_t *ops, size_t ops_cap, size_t *ops_used) {
+    if (!s || (n && (!codes || !offs || !lens || !refs || !out))) return fail(ESCH_INVALID_ARG, "null argument");
+    return align_core(
+        s, n,
+        [&](size_t k, const uint8_t *&p, size_t &len) {
+            p = codes + offs[k];
+            len = (size_t)(lens[k] < 0 ? 0 : lens[k]);
+        },
+        refs, out, ops, ops_cap, ops_used);
 }
 
 float esch_align_last_ms(const esch_scorer *s) { return s ? s->trace_ms : 0.0f; }

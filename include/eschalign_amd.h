@@ -227,6 +227,11 @@ typedef struct {
 esch_status esch_align_pairs(esch_scorer *s, const uint8_t *const *queries, const size_t *lens,
                              const int32_t *refs, size_t n, esch_alignment *out, uint32_t *ops,
                              size_t ops_cap, size_t *ops_used);
+/* The same for reads in one concatenated buffer: read k is
+ * codes[offs[k] .. offs[k] + lens[k]). */
+esch_status esch_align_pairs_concat(esch_scorer *s, const uint8_t *codes, const uint64_t *offs,
+                                    const int32_t *lens, const int32_t *refs, size_t n, esch_alignment *out,
+                                    uint32_t *ops, size_t ops_cap, size_t *ops_used);
 /* Device time (ms, HIP events) of the traceback kernels of the scorer's
  * last esch_align_pairs call (diagnostics; no counterpart in the reference). */
 float esch_align_last_ms(const esch_scorer *s);

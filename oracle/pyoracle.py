@@ -127,15 +127,23 @@ def align(q, r, scoring, local: bool):
 def align_batch(queries, refs_codes, ref_idx, scoring, local: bool, threads: int | None = None):
     """eo_align for query k against refs_codes[ref_idx[k]] on threads:
     int32 [n, 6] = score, query_start, query_end, ref_start, ref_end, runs."""
+    return align_batch_concat(*concat(queries), refs_codes, ref_idx, scoring, local, threads)
+
+
+def align_batch_concat(qc, qo, ql, refs_codes, ref_idx, scoring, local: bool, threads: int | None = None):
+    """align_batch for queries qc[qo[k]:qo[k] + ql[k]]."""
     threads = threads or min(8, os.cpu_count() or 1)
-    qc, qo, ql = concat(queries)
+    qc = np.ascontiguousarray(qc, np.uint8)
+    qo = np.ascontiguousarray(qo, np.uint64)
+    ql = np.ascontiguousarray(ql, np.int32)
+    n = len(ql)
     rc, ro, rl = concat(refs_codes)
     idx = np.ascontiguousarray(ref_idx, np.int32)
-    out = np.zeros((max(len(queries), 1), 6), np.int32)
-    if len(queries):
-        lib().eo_align_batch(qc, qo, ql, rc, ro, rl, idx, len(queries), *scoring, int(local),
-                             threads, out.reshape(-1))
-    return out[:len(queries)]
+    out = np.zeros((max(n, 1), 6), np.int32)
+    if n:
+        lib().eo_align_batch(qc, qo, ql, rc, ro, rl, idx, n, *scoring, int(local), threads,
+                             out.reshape(-1))
+    return out[:n]
 
 
 def fits_i16(n, max_ref_len, scoring) -> bool:

@@ -120,3 +120,27 @@ def test_trace_of_the_selected_winners():
     assert got == want
     for a, q, r in zip(got, qs, rs):
         assert a.score == O.score(q, panel.codes[r], scoring, True)
+
+
+def test_trace_concat_entry_and_repeated_calls():
+    """esch_align_pairs_concat equals esch_align_pairs; the cached buffers
+    give the same answers on a second, smaller call and on a larger one."""
+    rng = G.Rng(0x5EED_0C04)
+    panel_l = [G.random_seq(rng, 50 + rng.below(300), 0.01) for _ in range(20)]
+    panel = ea.Panel(panel_l)
+    gpu = ea.GpuScorer(panel, ea.Scoring(1, -1, -2, -1), ea.Mode.SEMIGLOBAL)
+    for n in (500, 40, 3000):
+        qs = [G.encode(G.chimeric_read(rng, panel_l, 1 + rng.below(400))) for _ in range(n)]
+        rs = [rng.below(len(panel_l)) for _ in qs]
+        a = gpu.align_pairs(qs, rs)
+        codes = np.concatenate(qs)
+        lens = np.array([len(q) for q in qs], np.int32)
+        offs = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.uint64)
+        recs, ops = gpu.align_pairs_concat(codes, offs, lens, rs)
+        for k, al in enumerate(a):
+            r = recs[k]
+            assert (al.score, al.query_start, al.query_end, al.ref_start, al.ref_end) == tuple(int(x) for x in r[:5])
+            seg = ops[int(r[6]):int(r[6]) + int(r[5])]
+            assert al.ops == tuple(("MID"[int(x) >> 30], int(x) & 0x3FFFFFFF) for x in seg)
+        want = oracle_aligns(qs[:200], panel.codes, rs[:200], (1, -1, -2, -1), False)
+        assert a[:200] == want
