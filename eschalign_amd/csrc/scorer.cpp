@@ -1083,15 +1083,41 @@ struct DevBuf {
     T *as() const { return static_cast<T *>(p); }
 };
 
-// Device buffers, stream and events of esch_align_pairs, kept by the scorer
-// between calls (one call at a time per scorer).
+// Grow-only pinned host buffer (staging of the traceback's largest copies).
+struct HostBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t need(size_t bytes) {
+        bytes = std::max<size_t>(bytes, 256);
+        if (bytes <= cap) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        const hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+        if (e == hipSuccess) cap = bytes;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+// Device buffers, pinned staging, stream and events of esch_align_pairs,
+// kept by the scorer between calls (one call at a time per scorer).
 struct TraceCache {
     std::mutex mu;
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     DevBuf codes, qoff, qlen, ref, slot, gbase, gS, gM, arena, recs, ops_wc, ops_off, out_off, out;
+    HostBuf h_codes, h_recs;
     ~TraceCache() {
         if (st) (void)hipStreamSynchronize(st);
+        h_codes.release();
+        h_recs.release();
         for (DevBuf *b : {&codes, &qoff, &qlen, &ref, &slot, &gbase, &gS, &gM, &arena, &recs, &ops_wc, &ops_off,
                           &out_off, &out})
             b->release();
@@ -1115,56 +1141,96 @@ static esch_status align_core(esch_scorer *s, size_t n,
                               const int32_t *refs, esch_alignment *out, uint32_t *ops, size_t ops_cap,
                               size_t *ops_used) {
     if (ops_used) *ops_used = 0;
-    std::vector<int32_t> order;
-    order.reserve(n);
     std::vector<size_t> lens(n);
-    size_t bases = 0;
+    std::vector<const uint8_t *> ptr(n);
+    size_t bases = 0, np = 0, max_len = 0;
     for (size_t k = 0; k < n; ++k) {
-        const uint8_t *q;
-        size_t len;
-        get(k, q, len);
-        lens[k] = len;
+        get(k, ptr[k], lens[k]);
+        const size_t len = lens[k];
         if (refs[k] < 0 || (size_t)refs[k] >= s->n_refs)
             return fail(ESCH_INVALID_ARG, "pair %zu names reference %d of %zu", k, refs[k], s->n_refs);
         // a group region is 64 pairs x (read / 16 stripes) x reference x 8 B
         if ((uint64_t)len * (uint64_t)s->h_rlen[refs[k]] > kTraceMaxCells)
             return fail(ESCH_INVALID_ARG, "pair %zu: %zu x %d cells is over the device traceback bound %llu", k, len,
                         s->h_rlen[refs[k]], (unsigned long long)kTraceMaxCells);
-        if (len && !q) return fail(ESCH_INVALID_ARG, "pair %zu: null read", k);
-        uint8_t mx = 0;
-        for (size_t j = 0; j < len; ++j) mx = q[j] > mx ? q[j] : mx;
-        if (mx > 4) {
-            for (size_t j = 0; j < len; ++j)
-                if (q[j] > 4) return fail(ESCH_INVALID_ARG, "read %zu carries code %d (codes are 0..=4)", k, q[j]);
-        }
+        if (len && !ptr[k]) return fail(ESCH_INVALID_ARG, "pair %zu: null read", k);
         out[k] = esch_alignment{};  // lib/scalar.rs:185-195: empty in, empty out
         if (len) {
-            order.push_back((int32_t)k);
+            ++np;
             bases += len;
+            max_len = std::max(max_len, len);
         }
     }
-    const size_t np = order.size();
     if (np == 0) return ESCH_OK;
-    // lib/simd/mod.rs:433: groups of similar read and reference lengths
-    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-        if (lens[x] != lens[y]) return lens[x] < lens[y];
-        return s->h_rlen[refs[x]] < s->h_rlen[refs[y]];
-    });
-    std::vector<uint8_t> hcodes(bases);
+    // Groups of similar read and reference lengths (lib/simd/mod.rs:433
+    // sorts ascending), longest first: a wave's time is its group's longest
+    // read x longest reference, so the longest groups are dispatched first.
+    // Two stable counting sorts: by reference length, then by read length.
+    std::vector<int32_t> order(np), tmp(np);
+    {
+        std::vector<size_t> cnt((size_t)s->max_len + 2, 0);
+        for (size_t k = 0; k < n; ++k)
+            if (lens[k]) ++cnt[s->max_len - (size_t)s->h_rlen[refs[k]]];
+        size_t run = 0;
+        for (size_t& c : cnt) { const size_t x = c; c = run; run += x; }
+        for (size_t k = 0; k < n; ++k)
+            if (lens[k]) tmp[cnt[s->max_len - (size_t)s->h_rlen[refs[k]]]++] = (int32_t)k;
+        std::vector<size_t> cl(max_len + 2, 0);
+        for (size_t i = 0; i < np; ++i) ++cl[max_len - lens[tmp[i]]];
+        run = 0;
+        for (size_t& c : cl) { const size_t x = c; c = run; run += x; }
+        for (size_t i = 0; i < np; ++i) order[cl[max_len - lens[tmp[i]]]++] = tmp[i];
+    }
+    HIPC(hipSetDevice(s->ordinal));
+    if (!s->trace) s->trace = new TraceCache;
+    TraceCache &T = *s->trace;
+    std::lock_guard<std::mutex> lock(T.mu);
+    if (T.st) HIPC(hipStreamSynchronize(T.st));  // the pinned staging may still feed a copy
+    HIPC(T.h_codes.need(bases));
+    HIPC(T.h_recs.need(n * kTraceRec * 4));
+    uint8_t *const hcodes = T.h_codes.as<uint8_t>();
     std::vector<uint64_t> hqoff(np), ops_off(n, 0);
     std::vector<int32_t> hqlen(np), href(np), hslot(np);
     size_t at = 0;
     for (size_t i = 0; i < np; ++i) {
         const int32_t k = order[i];
-        const uint8_t *q;
-        size_t len;
-        get(k, q, len);
-        memcpy(hcodes.data() + at, q, len);
         hqoff[i] = at;
-        hqlen[i] = (int32_t)len;
+        hqlen[i] = (int32_t)lens[k];
         href[i] = refs[k];
         hslot[i] = k;
-        at += len;
+        at += lens[k];
+    }
+    // the gather into staging and the code check on up to 8 host threads
+    {
+        const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+        const size_t nt = bases < ((size_t)8 << 20) ? 1 : std::min<size_t>(8, hw);
+        std::vector<uint8_t> bad(nt, 0);
+        auto gather = [&](size_t t) {
+            const size_t i0 = np * t / nt, i1 = np * (t + 1) / nt;
+            uint8_t mx = 0;
+            for (size_t i = i0; i < i1; ++i) {
+                const uint8_t *q = ptr[order[i]];
+                uint8_t *d = hcodes + hqoff[i];
+                memcpy(d, q, (size_t)hqlen[i]);
+                for (int32_t j = 0; j < hqlen[i]; ++j) mx = d[j] > mx ? d[j] : mx;
+            }
+            bad[t] = mx > 4;
+        };
+        if (nt == 1) {
+            gather(0);
+        } else {
+            std::vector<std::thread> pool;
+            for (size_t t = 1; t < nt; ++t) pool.emplace_back(gather, t);
+            gather(0);
+            for (auto &th : pool) th.join();
+        }
+        for (size_t t = 0; t < nt; ++t) {
+            if (!bad[t]) continue;
+            for (size_t k = 0; k < n; ++k)
+                for (size_t j = 0; j < lens[k]; ++j)
+                    if (ptr[k][j] > 4)
+                        return fail(ESCH_INVALID_ARG, "read %zu carries code %d (codes are 0..=4)", k, ptr[k][j]);
+        }
     }
     size_t wc = 0;  // worst-case runs: read + reference length per pair
     for (size_t k = 0; k < n; ++k) {
@@ -1200,10 +1266,6 @@ static esch_status align_core(esch_scorer *s, size_t n,
         g0 = g1;
     }
 
-    HIPC(hipSetDevice(s->ordinal));
-    if (!s->trace) s->trace = new TraceCache;
-    TraceCache &T = *s->trace;
-    std::lock_guard<std::mutex> lock(T.mu);
     if (!T.st) HIPC(hipStreamCreateWithFlags(&T.st, hipStreamNonBlocking));
     if (!T.e0) HIPC(hipEventCreate(&T.e0));
     if (!T.e1) HIPC(hipEventCreate(&T.e1));
@@ -1220,7 +1282,7 @@ static esch_status align_core(esch_scorer *s, size_t n,
     HIPC(T.ops_wc.need(wc * 4));
     HIPC(T.ops_off.need(n * 8));
     hipStream_t st = T.st;
-    HIPC(hipMemcpyAsync(T.codes.p, hcodes.data(), bases, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(T.codes.p, hcodes, bases, hipMemcpyHostToDevice, st));
     HIPC(hipMemcpyAsync(T.qoff.p, hqoff.data(), np * 8, hipMemcpyHostToDevice, st));
     HIPC(hipMemcpyAsync(T.qlen.p, hqlen.data(), np * 4, hipMemcpyHostToDevice, st));
     HIPC(hipMemcpyAsync(T.ref.p, href.data(), np * 4, hipMemcpyHostToDevice, st));
@@ -1242,8 +1304,8 @@ static esch_status align_core(esch_scorer *s, size_t n,
         HIPC(launch_trace(ta, c.second - c.first, local, st));
     }
     HIPC(hipEventRecord(T.e1, st));
-    std::vector<int32_t> recs(n * kTraceRec, 0);
-    HIPC(hipMemcpyAsync(recs.data(), T.recs.p, n * kTraceRec * 4, hipMemcpyDeviceToHost, st));
+    int32_t *const recs = T.h_recs.as<int32_t>();
+    HIPC(hipMemcpyAsync(recs, T.recs.p, n * kTraceRec * 4, hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
     HIPC(hipEventElapsedTime(&s->trace_ms, T.e0, T.e1));
     if (T.arena.cap > kTraceArenaKeep) T.arena.release();
@@ -1268,7 +1330,7 @@ static esch_status align_core(esch_scorer *s, size_t n,
     }
     for (size_t k = 0; k < n; ++k) {
         if (!lens[k]) continue;
-        const int32_t *r = recs.data() + k * kTraceRec;
+        const int32_t *r = recs + k * kTraceRec;
         out[k] = esch_alignment{r[0], r[1], r[2], r[3], r[4], r[5], (int64_t)out_off[k]};
     }
     return ESCH_OK;

@@ -6,9 +6,9 @@
 //   k_trace          the traceback DP and the walk, one pair per lane
 //   k_trace_compact  the walked operations packed into the caller's order
 //
-// Layout. Pairs are sorted by (read length, reference length) on the host
-// and cut into groups of 64; a 64-lane wave takes one group, one pair per
-// lane. Each lane runs its own exact int32 Gotoh DP (the reference's
+// Layout. Pairs are sorted by (read length, reference length), longest
+// first, on the host and cut into groups of 64; a 64-lane wave takes one
+// group, one pair per lane. Each lane runs its own exact int32 Gotoh DP (the reference's
 // recurrence and sentinel, lib/scalar.rs:120-163) over its read in stripes
 // of 16 rows held in registers, walking its reference's columns (2-bit
 // codes and wildcard bitmap, per-lane loads); the stripe's bottom row (H,
