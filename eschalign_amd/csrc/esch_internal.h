@@ -105,12 +105,12 @@ struct Launch {
     int grid32;   // persistent blocks of k_dp32
     int block32;  // threads per k_dp32 block
     int grid_aux; // blocks of the bucketing / selection kernels
-    int variant16;  // k_dp16 variant: 0 profile per column, 1 ping-pong, 2 deep prefetch
+    int variant16;  // k_dp16 variant: 0 profile per column, 1 ping-pong, 2 pair schedule, 3 quad schedule
     int grid_diag;  // persistent waves of the anti-diagonal kernel
     size_t diag_lds;  // k_diag64's LDS boundary bytes; 0: k_diag16 (global scratch)
 };
-constexpr int kDp16Variants = 3;
-constexpr int kDp16Default = 2;
+constexpr int kDp16Variants = 4;
+constexpr int kDp16Default = 3;
 hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st);
 hipError_t launch_dp16(const KArgs &a, const Launch &g, bool local, hipStream_t st);
 hipError_t launch_dp32(const KArgs &a, const Launch &g, bool local, int which, hipStream_t st);

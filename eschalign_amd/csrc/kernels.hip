@@ -202,7 +202,7 @@ __device__ __forceinline__ void stripe16(const uint4 *__restrict__ prof, uint2 *
 // to m + 1), the profile prefetch reads at most the reference's padding
 // word (column m), and whether the stripe is the first (border instead of
 // scratch) or the last (no stores) is a template argument.
-template <int RR, bool LOCAL, bool FIRST, bool LAST>
+template <int RR, bool LOCAL, bool FIRST, bool LAST, bool QUAD>
 __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 *__restrict__ scr,
                                           const int lane, const cwords w2, const cwords wn,
                                           const int m, const int row0, const int pA, const int pB,
@@ -246,6 +246,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
         return make_uint2(as_u32(hou), as_u32(f));
     };
 
+    if constexpr (!QUAD) {
     v2s sa[RR], sb[RR];
     plane(code(0), sa);
     uint2 r0 = bnd(0), r1 = bnd(1);
@@ -275,6 +276,66 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
         if constexpr (!LAST) s0[(size_t)j * 64] = o0;
     }
 
+    } else {
+    v2s sa[RR], sb[RR];
+    plane(code(0), sa);
+    uint2 r0 = bnd(0), r1 = bnd(1);
+    uint2 o0 = border, o1 = border;
+    int j = 0;
+    // Four columns per iteration as two pairs whose boundary registers swap
+    // roles (r for the first pair, q for the second), so no loaded register
+    // is copied: a copy would need its load complete where it is made.
+    for (; j + 3 < m; j += 4) {
+        const uint2 q0 = bnd(j + 2), q1 = bnd(j + 3);
+        if constexpr (!LAST) {
+            s0[(ptrdiff_t)(j - 2) * 64] = o0;
+            s0[(ptrdiff_t)(j - 1) * 64] = o1;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        plane(code(j + 1), sb);
+        o0 = col(sa, r0);
+        plane(code(j + 2), sa);
+        o1 = col(sb, r1);
+        __builtin_amdgcn_sched_barrier(0);
+        r0 = bnd(j + 4);
+        r1 = bnd(j + 5);
+        if constexpr (!LAST) {
+            s0[(ptrdiff_t)j * 64] = o0;
+            s0[(ptrdiff_t)(j + 1) * 64] = o1;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        plane(code(j + 3), sb);
+        o0 = col(sa, q0);
+        plane(code(j + 4), sa);  // j + 4 <= m: at most the padding word
+        o1 = col(sb, q1);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    // 0-3 columns left: one more pair, then a single column
+    if (j + 1 < m) {
+        const uint2 q0 = bnd(j + 2), q1 = bnd(j + 3);
+        if constexpr (!LAST) {
+            s0[(ptrdiff_t)(j - 2) * 64] = o0;
+            s0[(ptrdiff_t)(j - 1) * 64] = o1;
+        }
+        plane(code(j + 1), sb);
+        o0 = col(sa, r0);
+        plane(code(j + 2), sa);
+        o1 = col(sb, r1);
+        r0 = q0;
+        r1 = q1;
+        j += 2;
+    }
+    if constexpr (!LAST) {
+        s0[(ptrdiff_t)(j - 2) * 64] = o0;
+        s0[(ptrdiff_t)(j - 1) * 64] = o1;
+    }
+    if (j < m) {
+        o0 = col(sa, r0);
+        if constexpr (!LAST) s0[(size_t)j * 64] = o0;
+    }
+
+    }
+
     if constexpr (!LOCAL) {
 #pragma unroll
         for (int r = 0; r < RR; ++r) {
@@ -286,13 +347,13 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
     }
 }
 
-template <int RR, bool LOCAL, bool FIRST>
+template <int RR, bool LOCAL, bool FIRST, bool QUAD>
 __device__ __forceinline__ void stripe16d_any(const uint4 *prof, uint2 *scr, const int lane,
                                               const cwords w2, const cwords wn, const int m,
                                               const bool last, const int row0, const int pA,
                                               const int pB, const v2s vo, const v2s ve, v2s &best) {
-    if (last) stripe16d<RR, LOCAL, FIRST, true>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best);
-    else stripe16d<RR, LOCAL, FIRST, false>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best);
+    if (last) stripe16d<RR, LOCAL, FIRST, true, QUAD>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best);
+    else stripe16d<RR, LOCAL, FIRST, false, QUAD>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best);
 }
 
 template <bool LOCAL, int PF>
@@ -373,17 +434,18 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
             __syncthreads();
             if (!active) continue;
             const bool first = s == 0, last = s + 1 == S;
-            if constexpr (PF == 2) {
+            if constexpr (PF >= 2) {
+                constexpr bool Q = PF == 3;
                 if (s > 0) {
-                    stripe16d_any<16, LOCAL, false>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
+                    stripe16d_any<16, LOCAL, false, Q>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
                 } else if (RR == 16) {
-                    stripe16d_any<16, LOCAL, true>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
+                    stripe16d_any<16, LOCAL, true, Q>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
                 } else if (RR == 12) {
-                    stripe16d_any<12, LOCAL, true>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
+                    stripe16d_any<12, LOCAL, true, Q>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
                 } else if (RR == 8) {
-                    stripe16d_any<8, LOCAL, true>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
+                    stripe16d_any<8, LOCAL, true, Q>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
                 } else {
-                    stripe16d_any<4, LOCAL, true>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
+                    stripe16d_any<4, LOCAL, true, Q>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
                 }
             } else {
                 constexpr bool PP = PF == 1;
@@ -1342,6 +1404,7 @@ hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st) {
 // Kernel variants of k_dp16, chosen per scorer (set by the autotuner).
 template <bool LOCAL>
 static const void *dp16_fn(int variant) {
+    if (variant == 3) return reinterpret_cast<const void *>(&k_dp16<LOCAL, 3>);
     if (variant == 2) return reinterpret_cast<const void *>(&k_dp16<LOCAL, 2>);
     if (variant == 1) return reinterpret_cast<const void *>(&k_dp16<LOCAL, 1>);
     return reinterpret_cast<const void *>(&k_dp16<LOCAL, 0>);
@@ -1351,11 +1414,13 @@ hipError_t launch_dp16(const KArgs &a, const Launch &g, bool local, hipStream_t 
     const int v = g.variant16;
     const dim3 grid(g.grid16), block(64 * kWaves16);
     if (local) {
-        if (v == 2) hipLaunchKernelGGL((k_dp16<true, 2>), grid, block, 0, st, a);
+        if (v == 3) hipLaunchKernelGGL((k_dp16<true, 3>), grid, block, 0, st, a);
+        else if (v == 2) hipLaunchKernelGGL((k_dp16<true, 2>), grid, block, 0, st, a);
         else if (v == 1) hipLaunchKernelGGL((k_dp16<true, 1>), grid, block, 0, st, a);
         else hipLaunchKernelGGL((k_dp16<true, 0>), grid, block, 0, st, a);
     } else {
-        if (v == 2) hipLaunchKernelGGL((k_dp16<false, 2>), grid, block, 0, st, a);
+        if (v == 3) hipLaunchKernelGGL((k_dp16<false, 3>), grid, block, 0, st, a);
+        else if (v == 2) hipLaunchKernelGGL((k_dp16<false, 2>), grid, block, 0, st, a);
         else if (v == 1) hipLaunchKernelGGL((k_dp16<false, 1>), grid, block, 0, st, a);
         else hipLaunchKernelGGL((k_dp16<false, 0>), grid, block, 0, st, a);
     }
