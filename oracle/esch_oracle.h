@@ -27,6 +27,14 @@
  *                     sort-based restatement in the tests.
  *   eo_pair_cands     lib/mapper/pair.rs:450-556 cases (score 120 paralog
  *                     choice, min-score, strands) and a brute-force sort.
+ *   eo_map_reads      lib/mapper.rs:571-589 (ties in reference order, capped
+ *                     hits), :709-739 (min-score unmaps, reverse strand).
+ *   eo_map_pairs      every case of lib/mapper/pair.rs:449-573 (paralog
+ *                     choice, insert and orientation limits, template strands,
+ *                     min-score per mate, batched = one at a time) and the
+ *                     outcomes crates/eschalign-cli/tests/align_paired.rs
+ *                     states for the nine pairs of the paired fixture
+ *                     (positions, inserts, proper or not, ties).
  *   eo_adapter_*      lib/adapter.rs:116-197 known-answer tests.
  *   eo_encode etc.    lib/alphabet.rs:86-105 known answers.
  */
@@ -105,6 +113,56 @@ int32_t eo_select(const int32_t *fwd, const int32_t *rev, size_t n_refs,
 int32_t eo_pair_cands(const int32_t *const rows[4], size_t n_refs,
                       int both_strands, int32_t min_score, int32_t *cand,
                       int32_t *sum, int32_t *s0, int32_t *s1);
+
+/* ---- read and pair mapping (esch_map.c) ----
+ * Aligner::map_reads_scored (lib/mapper.rs:400-550) and map_pairs_scored
+ * (lib/mapper/pair.rs:157-387) on caller-computed panel scores, with every
+ * traceback by eo_align. The records have the layout of the library's
+ * esch_hit / esch_read_mapping / esch_pair_mapping. */
+typedef struct {
+    int32_t reference;
+    int32_t reverse;          /* the read's / mate's own strand */
+    int32_t template_reverse; /* proper-pair hits: PairHit::reverse */
+    int32_t insert;           /* proper-pair hits: PairHit::insert */
+    int32_t score, query_start, query_end, ref_start, ref_end, n_ops;
+    int64_t ops_offset;
+} eo_hit;
+typedef struct {
+    int32_t best, n_tied, subopt;
+    int32_t flags; /* bit 0 best present (non-empty read), bit 1 suboptimal present */
+    int32_t n_hits, reserved;
+    int64_t hit_offset;
+} eo_read_mapping;
+typedef struct {
+    int32_t proper; /* 1 Proper, 0 Separate */
+    int32_t score, n_tied;
+    int32_t flags; /* proper: bit m = suboptimal[m] present */
+    int32_t subopt[2];
+    int32_t n_hits; /* proper: pair hits, 2 eo_hit each (mate 1, mate 2) */
+    int32_t reserved;
+    int64_t hit_offset;
+    eo_read_mapping separate[2];
+} eo_pair_mapping;
+typedef struct eo_mapped eo_mapped;
+/* fwd_rows[k * n_refs + r] (and rev_rows, NULL = forward strand only): read
+ * k's scores as sequenced / reverse-complemented. max_ties < 0 = all. */
+eo_mapped *eo_map_reads(const uint8_t *codes, const uint64_t *off, const int32_t *len, size_t n,
+                        const int32_t *fwd_rows, const int32_t *rev_rows, const uint8_t *rcodes,
+                        const uint64_t *roff, const int32_t *rlen, size_t n_refs, int32_t ma, int32_t mi,
+                        int32_t o, int32_t e, int local, int32_t min_score, int64_t max_ties, int threads);
+/* Mates interleaved: read 2p is pair p's mate 1, 2p + 1 its mate 2; both
+ * row sets are required (the fallback maps each mate on both strands). */
+eo_mapped *eo_map_pairs(const uint8_t *codes, const uint64_t *off, const int32_t *len, size_t n_pairs,
+                        const int32_t *fwd_rows, const int32_t *rev_rows, const uint8_t *rcodes,
+                        const uint64_t *roff, const int32_t *rlen, size_t n_refs, int32_t ma, int32_t mi,
+                        int32_t o, int32_t e, int local, int32_t min_score, int64_t max_ties, int64_t max_insert,
+                        int both_strands, int threads);
+size_t eo_mapped_len(const eo_mapped *m);
+const eo_read_mapping *eo_mapped_reads(const eo_mapped *m);
+const eo_pair_mapping *eo_mapped_pairs(const eo_mapped *m);
+const eo_hit *eo_mapped_hits(const eo_mapped *m, size_t *n);
+const uint32_t *eo_mapped_ops(const eo_mapped *m, size_t *n);
+void eo_mapped_free(eo_mapped *m);
 
 /* lib/adapter.rs:68-91 — adapter bases at the 3' end of a read (letters). */
 size_t eo_adapter_find(const uint8_t *read, size_t n, const uint8_t *adapter,

@@ -56,6 +56,22 @@ def lib():
         L.eo_pair_cands.argtypes = [C.c_void_p * 4, C.c_size_t, C.c_int, C.c_int32,
                                     I32P, I32P, I32P, I32P]
         L.eo_pair_cands.restype = C.c_int32
+        L.eo_map_reads.argtypes = [U8P, U64P, I32P, C.c_size_t, I32P, C.c_void_p, U8P, U64P, I32P,
+                                   C.c_size_t] + [C.c_int32] * 4 + [C.c_int, C.c_int32, C.c_int64, C.c_int]
+        L.eo_map_reads.restype = C.c_void_p
+        L.eo_map_pairs.argtypes = [U8P, U64P, I32P, C.c_size_t, I32P, I32P, U8P, U64P, I32P,
+                                   C.c_size_t] + [C.c_int32] * 4 + [C.c_int, C.c_int32, C.c_int64,
+                                                                    C.c_int64, C.c_int, C.c_int]
+        L.eo_map_pairs.restype = C.c_void_p
+        L.eo_mapped_len.argtypes = [C.c_void_p]
+        L.eo_mapped_len.restype = C.c_size_t
+        for f in ("eo_mapped_reads", "eo_mapped_pairs"):
+            getattr(L, f).argtypes = [C.c_void_p]
+            getattr(L, f).restype = C.c_void_p
+        for f in ("eo_mapped_hits", "eo_mapped_ops"):
+            getattr(L, f).argtypes = [C.c_void_p, C.POINTER(C.c_size_t)]
+            getattr(L, f).restype = C.c_void_p
+        L.eo_mapped_free.argtypes = [C.c_void_p]
         L.eo_adapter_find.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t,
                                       C.c_size_t]
         L.eo_adapter_find.restype = C.c_size_t
@@ -219,6 +235,128 @@ def pair_cands(rows, min_score=0, both_strands=False):
     k = lib().eo_pair_cands(ptrs, n_refs, int(both_strands), min_score, cand, sm, s0, s1)
     return [(int(sm[i]), int(cand[i]) >> 1, bool(cand[i] & 1), int(s0[i]), int(s1[i]))
             for i in range(k)]
+
+
+# Records of eo_map_reads / eo_map_pairs (the layout of the library's
+# esch_hit, esch_read_mapping and esch_pair_mapping).
+HIT_DTYPE = np.dtype([("reference", "<i4"), ("reverse", "<i4"), ("template_reverse", "<i4"),
+                      ("insert", "<i4"), ("score", "<i4"), ("query_start", "<i4"),
+                      ("query_end", "<i4"), ("ref_start", "<i4"), ("ref_end", "<i4"),
+                      ("n_ops", "<i4"), ("ops_offset", "<i8")])
+READ_DTYPE = np.dtype([("best", "<i4"), ("n_tied", "<i4"), ("subopt", "<i4"), ("flags", "<i4"),
+                       ("n_hits", "<i4"), ("reserved", "<i4"), ("hit_offset", "<i8")])
+PAIR_DTYPE = np.dtype([("proper", "<i4"), ("score", "<i4"), ("n_tied", "<i4"), ("flags", "<i4"),
+                       ("subopt", "<i4", (2,)), ("n_hits", "<i4"), ("reserved", "<i4"),
+                       ("hit_offset", "<i8"), ("separate", READ_DTYPE, (2,))])
+
+
+def _mapped_arrays(L, m, pairs: bool):
+    n = int(L.eo_mapped_len(m))
+    nh, no = C.c_size_t(), C.c_size_t()
+    hp = L.eo_mapped_hits(m, C.byref(nh))
+    op = L.eo_mapped_ops(m, C.byref(no))
+    dt = PAIR_DTYPE if pairs else READ_DTYPE
+    rp = L.eo_mapped_pairs(m) if pairs else L.eo_mapped_reads(m)
+    recs = np.frombuffer(C.string_at(rp, n * dt.itemsize), dtype=dt).copy() if n else np.zeros(0, dt)
+    hits = (np.frombuffer(C.string_at(hp, nh.value * HIT_DTYPE.itemsize), dtype=HIT_DTYPE).copy()
+            if nh.value else np.zeros(0, HIT_DTYPE))
+    ops = (np.frombuffer(C.string_at(op, no.value * 4), dtype=np.uint32).copy()
+           if no.value else np.zeros(0, np.uint32))
+    return recs, hits, ops
+
+
+def hit_form(h, ops):
+    """A hit as (reference, reverse, score, query_start, query_end,
+    ref_start, ref_end, ((op, len), ...))."""
+    o0, k = int(h["ops_offset"]), int(h["n_ops"])
+    runs = tuple((OPS[int(x) >> 30], int(x) & 0x3FFFFFFF) for x in ops[o0:o0 + k])
+    return (int(h["reference"]), bool(h["reverse"]), int(h["score"]), int(h["query_start"]),
+            int(h["query_end"]), int(h["ref_start"]), int(h["ref_end"]), runs)
+
+
+def read_form(r, hits, ops):
+    """ReadMapping (lib/mapper.rs:58-72) as (best_score or None, n_tied,
+    suboptimal or None, (hit, ...))."""
+    flags = int(r["flags"])
+    h0, nh = int(r["hit_offset"]), int(r["n_hits"])
+    return (int(r["best"]) if flags & 1 else None, int(r["n_tied"]),
+            int(r["subopt"]) if flags & 2 else None,
+            tuple(hit_form(hits[h0 + i], ops) for i in range(nh)))
+
+
+def pair_form(p, hits, ops):
+    """PairMapping (lib/mapper/pair.rs:90-107): ("proper", score, n_tied,
+    (suboptimal per mate), ((reference, template reverse, insert, mate 1 hit,
+    mate 2 hit), ...)) or ("separate", mate 1 ReadMapping, mate 2 ReadMapping)."""
+    if int(p["proper"]):
+        flags = int(p["flags"])
+        sub = tuple(int(p["subopt"][m]) if flags & (1 << m) else None for m in range(2))
+        h0 = int(p["hit_offset"])
+        phs = []
+        for i in range(int(p["n_hits"])):
+            a, b = hits[h0 + 2 * i], hits[h0 + 2 * i + 1]
+            phs.append((int(a["reference"]), bool(a["template_reverse"]), int(a["insert"]),
+                        hit_form(a, ops), hit_form(b, ops)))
+        return ("proper", int(p["score"]), int(p["n_tied"]), sub, tuple(phs))
+    return ("separate", read_form(p["separate"][0], hits, ops), read_form(p["separate"][1], hits, ops))
+
+
+def map_reads(reads, refs, scoring, local: bool, min_score=0, max_ties=None, both_strands=False,
+              threads: int | None = None, rows=None):
+    """Aligner::map_reads_scored (lib/mapper.rs:400-550) on codes: one
+    read_form per read. rows = (fwd, rev) panel scores if already computed."""
+    threads = threads or min(8, os.cpu_count() or 1)
+    L = lib()
+    reads = [_codes(r) for r in reads]
+    if rows is None:
+        fwd = score_matrix(reads, refs, scoring, local, threads)
+        rev = score_matrix([revcomp(r) for r in reads], refs, scoring, local, threads) if both_strands else None
+    else:
+        fwd, rev = rows
+    fwd = np.ascontiguousarray(fwd, np.int32).reshape(-1)
+    rev = None if rev is None else np.ascontiguousarray(rev, np.int32).reshape(-1)
+    qc, qo, ql = concat(reads)
+    rc, ro, rl = concat(refs)
+    mt = -1 if max_ties is None else int(max_ties)
+    m = L.eo_map_reads(qc, qo, ql, len(reads), fwd if len(fwd) else np.zeros(1, np.int32),
+                       None if rev is None else rev.ctypes.data, rc, ro, rl, len(refs), *scoring,
+                       int(local), min_score, mt, threads)
+    try:
+        recs, hits, ops = _mapped_arrays(L, m, False)
+    finally:
+        L.eo_mapped_free(m)
+    return [read_form(r, hits, ops) for r in recs]
+
+
+def map_pairs(pairs, refs, scoring, local: bool, min_score=0, max_ties=None, max_insert=500,
+              both_strands=False, threads: int | None = None, rows=None, raw=False):
+    """Aligner::map_pairs_scored (lib/mapper/pair.rs:157-387) on codes:
+    pairs = [(mate 1, mate 2), ...]; one pair_form per pair (raw: the
+    record arrays). rows = (fwd, rev) scores of the interleaved mates."""
+    threads = threads or min(8, os.cpu_count() or 1)
+    L = lib()
+    mates = [_codes(x) for p in pairs for x in p]
+    if rows is None:
+        fwd = score_matrix(mates, refs, scoring, local, threads)
+        rev = score_matrix([revcomp(x) for x in mates], refs, scoring, local, threads)
+    else:
+        fwd, rev = rows
+    fwd = np.ascontiguousarray(fwd, np.int32).reshape(-1)
+    rev = np.ascontiguousarray(rev, np.int32).reshape(-1)
+    if not len(fwd):
+        fwd = rev = np.zeros(1, np.int32)
+    qc, qo, ql = concat(mates)
+    rc, ro, rl = concat(refs)
+    mt = -1 if max_ties is None else int(max_ties)
+    m = L.eo_map_pairs(qc, qo, ql, len(pairs), fwd, rev, rc, ro, rl, len(refs), *scoring, int(local),
+                       min_score, mt, int(max_insert), int(both_strands), threads)
+    try:
+        recs, hits, ops = _mapped_arrays(L, m, True)
+    finally:
+        L.eo_mapped_free(m)
+    if raw:
+        return recs, hits, ops
+    return [pair_form(p, hits, ops) for p in recs]
 
 
 def adapter_find(read: bytes, adapter: bytes, min_overlap: int) -> int:
