@@ -64,7 +64,10 @@ EXPORTS = [
     "esch_server_scorer", "esch_server_submit", "esch_server_wait", "esch_server_result",
     "esch_server_stats", "esch_set_diag_min_len", "esch_get_diag_min_len",
     "esch_batch_stats_diag", "esch_set_diag_ratio", "esch_get_diag_ratio", "esch_valu_peak16",
-    "esch_align_pairs", "esch_align_last_ms", "esch_align_pairs_concat",
+    "esch_align_pairs", "esch_align_last_ms", "esch_align_pairs_concat", "esch_map_reads",
+    "esch_map_pairs", "esch_map_pairs_concat", "esch_mapped_len", "esch_mapped_reads",
+    "esch_mapped_pairs", "esch_mapped_hits", "esch_mapped_ops", "esch_mapped_stats", "esch_mapped_free",
+    "esch_set_map_chunk",
 ]
 
 
@@ -78,6 +81,26 @@ class ServerStats(C.Structure):
     _fields_ = [("completed", C.c_uint64), ("batches", C.c_uint64),
                 ("queued_units", C.c_size_t), ("hbm_in_use", C.c_size_t),
                 ("hbm_peak", C.c_size_t), ("slots", C.c_size_t), ("busy_slots", C.c_size_t)]
+
+
+class MapStats(C.Structure):
+    """esch_map_stats."""
+    _fields_ = [("score_ms", C.c_double), ("select_ms", C.c_double), ("trace_ms", C.c_double),
+                ("total_ms", C.c_double), ("rounds", C.c_int32), ("reserved", C.c_int32),
+                ("traced", C.c_int64)]
+
+
+# Records of esch_map_reads / esch_map_pairs (esch_hit, esch_read_mapping,
+# esch_pair_mapping in include/eschalign_amd.h).
+HIT_DTYPE = np.dtype([("reference", "<i4"), ("reverse", "<i4"), ("template_reverse", "<i4"),
+                      ("insert", "<i4"), ("score", "<i4"), ("query_start", "<i4"),
+                      ("query_end", "<i4"), ("ref_start", "<i4"), ("ref_end", "<i4"),
+                      ("n_ops", "<i4"), ("ops_offset", "<i8")])
+READ_MAPPING_DTYPE = np.dtype([("best", "<i4"), ("n_tied", "<i4"), ("subopt", "<i4"), ("flags", "<i4"),
+                               ("n_hits", "<i4"), ("reserved", "<i4"), ("hit_offset", "<i8")])
+PAIR_MAPPING_DTYPE = np.dtype([("proper", "<i4"), ("score", "<i4"), ("n_tied", "<i4"), ("flags", "<i4"),
+                               ("subopt", "<i4", (2,)), ("n_hits", "<i4"), ("reserved", "<i4"),
+                               ("hit_offset", "<i8"), ("separate", READ_MAPPING_DTYPE, (2,))])
 
 
 class GpuError(Exception):
@@ -181,6 +204,22 @@ def load_library():
     L.esch_align_last_ms.restype = C.c_float
     L.esch_valu_peak16.argtypes = [C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.esch_get_diag_ratio.argtypes = [vp]
+    L.esch_map_reads.argtypes = [vp, vp, vp, sz, C.c_int, i32, i64, C.POINTER(vp)]
+    L.esch_map_pairs.argtypes = [vp, vp, vp, vp, vp, sz, C.c_int, i32, i64, i64, C.POINTER(vp)]
+    L.esch_map_pairs_concat.argtypes = [vp, vp, vp, vp, sz, C.c_int, i32, i64, i64, C.POINTER(vp)]
+    L.esch_mapped_len.argtypes = [vp]
+    L.esch_mapped_len.restype = sz
+    for f in ("esch_mapped_reads", "esch_mapped_pairs"):
+        getattr(L, f).argtypes = [vp]
+        getattr(L, f).restype = vp
+    for f in ("esch_mapped_hits", "esch_mapped_ops"):
+        getattr(L, f).argtypes = [vp, C.POINTER(sz)]
+        getattr(L, f).restype = vp
+    L.esch_mapped_stats.argtypes = [vp, C.POINTER(MapStats)]
+    L.esch_mapped_stats.restype = None
+    L.esch_mapped_free.argtypes = [vp]
+    L.esch_mapped_free.restype = None
+    L.esch_set_map_chunk.argtypes = [vp, sz]
     _lib = L
     return L
 
@@ -207,6 +246,110 @@ class Alignment:
 
     def cigar(self) -> str:
         return "".join(f"{n}{op}" for op, n in self.ops)
+
+
+@dataclass(frozen=True)
+class Hit:
+    """lib/mapper.rs:43-56: one placement of a read (md / nm, a SAM concern
+    computed from the letters, are not part of it here)."""
+    reference: int
+    reverse: bool
+    alignment: Alignment
+
+
+@dataclass(frozen=True)
+class ReadMapping:
+    """lib/mapper.rs:58-72."""
+    best_score: Optional[int]
+    n_tied: int
+    suboptimal: Optional[int]
+    hits: tuple
+
+    def is_mapped(self) -> bool:
+        return bool(self.hits)
+
+
+@dataclass(frozen=True)
+class PairHit:
+    """lib/mapper/pair.rs:69-88: a placement of both mates on one reference."""
+    reference: int
+    reverse: bool
+    mates: tuple
+    insert: int
+
+    @property
+    def score(self) -> int:
+        return self.mates[0].alignment.score + self.mates[1].alignment.score
+
+
+@dataclass(frozen=True)
+class ProperPair:
+    """PairMapping::Proper (lib/mapper/pair.rs:94-103)."""
+    score: int
+    n_tied: int
+    suboptimal: tuple
+    hits: tuple
+
+
+@dataclass(frozen=True)
+class SeparatePair:
+    """PairMapping::Separate (lib/mapper/pair.rs:104-106): each mate on its
+    own, as an unpaired read on both strands."""
+    mates: tuple
+
+
+def _hit(h, ops) -> Hit:
+    o0, k = int(h["ops_offset"]), int(h["n_ops"])
+    runs = tuple(("MID"[int(x) >> 30], int(x) & 0x3FFFFFFF) for x in ops[o0:o0 + k])
+    return Hit(int(h["reference"]), bool(h["reverse"]),
+               Alignment(int(h["score"]), int(h["query_start"]), int(h["query_end"]),
+                         int(h["ref_start"]), int(h["ref_end"]), runs))
+
+
+def _read_mapping(r, hits, ops) -> ReadMapping:
+    flags, h0 = int(r["flags"]), int(r["hit_offset"])
+    return ReadMapping(int(r["best"]) if flags & 1 else None, int(r["n_tied"]),
+                       int(r["subopt"]) if flags & 2 else None,
+                       tuple(_hit(hits[h0 + i], ops) for i in range(int(r["n_hits"]))))
+
+
+def _pair_mapping(p, hits, ops):
+    if not int(p["proper"]):
+        return SeparatePair((_read_mapping(p["separate"][0], hits, ops),
+                             _read_mapping(p["separate"][1], hits, ops)))
+    flags, h0 = int(p["flags"]), int(p["hit_offset"])
+    phs = []
+    for i in range(int(p["n_hits"])):
+        a, b = hits[h0 + 2 * i], hits[h0 + 2 * i + 1]
+        phs.append(PairHit(int(a["reference"]), bool(a["template_reverse"]),
+                           (_hit(a, ops), _hit(b, ops)), int(a["insert"])))
+    return ProperPair(int(p["score"]), int(p["n_tied"]),
+                      tuple(int(p["subopt"][m]) if flags & (1 << m) else None for m in range(2)),
+                      tuple(phs))
+
+
+def _take_mapped(m, pairs: bool):
+    """Copy an esch_mapped's records out and free it: (records, hits, ops,
+    stats dict)."""
+    L = _lib
+    try:
+        n = int(L.esch_mapped_len(m))
+        dt = PAIR_MAPPING_DTYPE if pairs else READ_MAPPING_DTYPE
+        rp = L.esch_mapped_pairs(m) if pairs else L.esch_mapped_reads(m)
+        nh, no = C.c_size_t(), C.c_size_t()
+        hp = L.esch_mapped_hits(m, C.byref(nh))
+        op = L.esch_mapped_ops(m, C.byref(no))
+        recs = np.frombuffer(C.string_at(rp, n * dt.itemsize), dt).copy() if n else np.zeros(0, dt)
+        hits = (np.frombuffer(C.string_at(hp, nh.value * HIT_DTYPE.itemsize), HIT_DTYPE).copy()
+                if nh.value else np.zeros(0, HIT_DTYPE))
+        ops = (np.frombuffer(C.string_at(op, no.value * 4), np.uint32).copy()
+               if no.value else np.zeros(0, np.uint32))
+        st = MapStats()
+        L.esch_mapped_stats(m, C.byref(st))
+        stats = {f: getattr(st, f) for f, _ in MapStats._fields_ if f != "reserved"}
+    finally:
+        L.esch_mapped_free(m)
+    return recs, hits, ops, stats
 
 
 def valu_peak16(ordinal: int = 0) -> dict:
@@ -643,6 +786,64 @@ class GpuScorer:
             out.append(Alignment(int(r[0]), int(r[1]), int(r[2]), int(r[3]), int(r[4]),
                                  tuple(("MID"[int(x) >> 30], int(x) & 0x3FFFFFFF) for x in seg)))
         return out
+
+    def set_map_chunk(self, reads: int):
+        """Reads (mates) per device chunk of the mapping calls (0: by memory)."""
+        _check(_lib.esch_set_map_chunk(self._h, int(reads)))
+
+    def map_reads_raw(self, reads: Sequence[np.ndarray], both_strands=False, min_score=0, max_ties=None):
+        """esch_map_reads: (records READ_MAPPING_DTYPE, hits HIT_DTYPE, ops,
+        stats)."""
+        keep, ptrs, lens = _ptr_array(reads)
+        m = C.c_void_p()
+        mt = -1 if max_ties is None else int(max_ties)
+        _check(_lib.esch_map_reads(self._h, ptrs, lens, len(keep), int(both_strands), int(min_score), mt,
+                                   C.byref(m)))
+        return _take_mapped(m, False)
+
+    def map_reads(self, reads: Sequence[np.ndarray], both_strands=False, min_score=0, max_ties=None) -> list:
+        """Aligner::map_reads_scored (lib/mapper.rs:400-550) on the device:
+        one ReadMapping per read (codes)."""
+        recs, hits, ops, _ = self.map_reads_raw(reads, both_strands, min_score, max_ties)
+        return [_read_mapping(r, hits, ops) for r in recs]
+
+    def map_pairs_raw(self, mates1: Sequence[np.ndarray], mates2: Sequence[np.ndarray], both_strands=False,
+                      min_score=0, max_ties=None, max_insert=500):
+        """esch_map_pairs: (records PAIR_MAPPING_DTYPE, hits HIT_DTYPE, ops,
+        stats)."""
+        if len(mates1) != len(mates2):
+            raise ValueError("mates1 and mates2 differ in length")
+        k1, p1, l1 = _ptr_array(mates1)
+        k2, p2, l2 = _ptr_array(mates2)
+        m = C.c_void_p()
+        mt = -1 if max_ties is None else int(max_ties)
+        _check(_lib.esch_map_pairs(self._h, p1, l1, p2, l2, len(k1), int(both_strands), int(min_score), mt,
+                                   int(max_insert), C.byref(m)))
+        return _take_mapped(m, True)
+
+    def map_pairs_concat(self, codes: np.ndarray, offs: np.ndarray, lens: np.ndarray, both_strands=False,
+                         min_score=0, max_ties=None, max_insert=500):
+        """esch_map_pairs_concat: mates interleaved (2p mate 1, 2p + 1 mate 2)
+        in one buffer; returns what map_pairs_raw does."""
+        codes = np.ascontiguousarray(codes, np.uint8)
+        offs = np.ascontiguousarray(offs, np.uint64)
+        lens = np.ascontiguousarray(lens, np.int32)
+        if len(lens) % 2:
+            raise ValueError("mates come in pairs")
+        m = C.c_void_p()
+        mt = -1 if max_ties is None else int(max_ties)
+        _check(_lib.esch_map_pairs_concat(self._h, codes.ctypes.data, offs.ctypes.data, lens.ctypes.data,
+                                          len(lens) // 2, int(both_strands), int(min_score), mt, int(max_insert),
+                                          C.byref(m)))
+        return _take_mapped(m, True)
+
+    def map_pairs(self, pairs, both_strands=False, min_score=0, max_ties=None, max_insert=500) -> list:
+        """Aligner::map_pairs_scored (lib/mapper/pair.rs:157-387) on the
+        device: pairs = [(mate 1, mate 2), ...] as codes (adapters trimmed);
+        one ProperPair or SeparatePair each."""
+        recs, hits, ops, _ = self.map_pairs_raw([p[0] for p in pairs], [p[1] for p in pairs], both_strands,
+                                                min_score, max_ties, max_insert)
+        return [_pair_mapping(r, hits, ops) for r in recs]
 
     def align_last_ms(self) -> float:
         """Device time of the last align_pairs call's traceback kernels."""

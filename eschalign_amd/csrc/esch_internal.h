@@ -144,6 +144,57 @@ hipError_t launch_trace(const TraceArgs &a, int groups, bool local, hipStream_t 
 hipError_t launch_trace_compact(const uint32_t *ops_wc, const uint64_t *ops_off, const int32_t *recs,
                                 const uint64_t *out_off, uint32_t *out, int32_t n_pairs, hipStream_t st);
 hipError_t launch_pairs(const KArgs &a, const Launch &g, hipStream_t st);
+// Selection over a batch's resident int32 rows for the mapping tier loop
+// (resolve.hip). Rows: read x as sequenced is row x, its reverse complement
+// row n_reads + x; pair p's mates are reads 2p and 2p + 1. One wave per
+// listed entry. A list entry's candidates go to cands + cand_off[e] when
+// cand_off is set (room for all of them), else to cands + e * cap (the first
+// cap of them; the record still counts them all).
+struct TierArgs {
+    const int32_t *out32;
+    int32_t n_reads, n_refs;
+    int32_t strands;        // template strands: 1 forward, 2 both
+    int32_t min_score;
+    const int32_t *list;    // pair indices
+    const int32_t *below;   // per entry: only sums below this (INT32_MAX: all)
+    int32_t n_list;
+    int32_t cap;
+    const int64_t *cand_off;
+    int32_t *recs;          // per entry: n_tier, sum, found (kTierRec)
+    int32_t *cands;         // (ref << 1) | template strand, (ref, strand) order
+};
+constexpr int kTierRec = 3;
+
+struct SuboptArgs {
+    const int32_t *out32;
+    int32_t n_reads, n_refs, strands;
+    const int32_t *list;     // pair indices
+    const int64_t *set_off;  // per entry: its tie set in `set`
+    const int32_t *set_n;
+    const int32_t *set;      // (ref << 1) | template strand, ascending
+    int32_t n_list;
+    int32_t *out;            // per entry: suboptimal of mate 1, of mate 2, flags (kSuboptRec)
+};
+constexpr int kSuboptRec = 3;
+
+struct MateSelArgs {
+    const int32_t *out32;
+    int32_t n_reads, n_refs;
+    int32_t both;            // reverse-complement rows present and candidates
+    int32_t min_score;
+    int64_t max_ties;        // < 0: all
+    const int32_t *list;     // read (row) indices
+    int32_t n_list;
+    int32_t cap;
+    const int64_t *cand_off;
+    int32_t *recs;           // per entry: best, n_tied, subopt, flags, n_take (kSelRec)
+    int32_t *ties;           // (ref << 1) | reverse, tie order
+};
+constexpr int kSelRec = 5;
+
+hipError_t launch_pair_tier(const TierArgs &a, int max_blocks, hipStream_t st);
+hipError_t launch_pair_subopt(const SuboptArgs &a, int max_blocks, hipStream_t st);
+hipError_t launch_mate_select(const MateSelArgs &a, int max_blocks, hipStream_t st);
 hipError_t dp16_attributes(bool local, int variant, hipFuncAttributes *attr);
 hipError_t dp32_attributes(bool local, hipFuncAttributes *attr);
 hipError_t dp16_occupancy(bool local, int variant, int *blocks_per_cu);

@@ -1,0 +1,622 @@
+// Read and pair mapping on the device: Aligner::map_reads_scored
+// (lib/mapper.rs:400-550) and Aligner::map_pairs_scored with its tier loop
+// and Separate fallback (lib/mapper/pair.rs:157-387).
+//
+// A chunk of reads (or pairs, mates interleaved) is loaded into a device
+// batch with both orientations and scored into resident int32 rows; every
+// selection then runs on those rows (resolve.hip) and every traceback reads
+// its read straight from the batch's codes (k_trace), so the host moves only
+// lists of indices, tie lists and the traced records.
+//
+// Pairs. Round r lists every pair still open with the sum of the tier that
+// failed in round r - 1 as its bound (none in round 1); k_pair_tier returns
+// each listed pair's next tier, whose candidates are traced, both mates
+// each; a pair whose tier holds a candidate passing pair_ok
+// (lib/mapper/pair.rs:392-410) is closed with that tier's passing
+// candidates as its tie set, a pair whose tier fails stays open, and a pair
+// with no tier left is Separate. Every round is batched over all open pairs,
+// as the reference's loop is (lib/mapper/pair.rs:236-286). Then
+// k_pair_subopt gives each proper pair's per-mate suboptimal, and the
+// Separate pairs' mates are mapped like unpaired reads on both strands.
+#include "host_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+using namespace esch;
+
+namespace {
+
+constexpr int kTierCap = 16;    // candidates per pair returned inline per round
+constexpr int kSelCap = 16;     // ties per read returned inline
+constexpr int kResolveBlocks = 4096;
+constexpr size_t kChunkRowBytes = (size_t)4 << 30;  // resident int32 rows per chunk
+
+using Clock = std::chrono::steady_clock;
+
+double ms_since(Clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+}
+
+bool has_match(const uint32_t *ops, int32_t n) {
+    for (int32_t k = 0; k < n; ++k)
+        if ((ops[k] >> 30) == 0u) return true;
+    return false;
+}
+
+// Traced (read, reference) jobs: records and their runs.
+struct Traced {
+    std::vector<esch_alignment> al;
+    std::vector<uint32_t> ops;
+    bool aligned(size_t k) const { return has_match(ops.data() + al[k].ops_offset, al[k].n_ops); }
+};
+
+// Hits of one read or pair, collected out of order and placed in input
+// order at the end: hits[k].alignment.ops_offset indexes ops.
+struct Pool {
+    std::vector<esch_hit> hits;
+    std::vector<uint32_t> ops;
+    void add(const Traced &t, size_t k, int32_t ref, int32_t rev, int32_t trev, int32_t insert) {
+        esch_hit h{ref, rev, trev, insert, t.al[k]};
+        h.alignment.ops_offset = (int64_t)ops.size();
+        const uint32_t *src = t.ops.data() + t.al[k].ops_offset;
+        ops.insert(ops.end(), src, src + t.al[k].n_ops);
+        hits.push_back(h);
+    }
+};
+
+}  // namespace
+
+struct MapCache {
+    DevBuf list, below, recs, cands, cand_off, set, set_off, set_n, out;
+};
+
+void map_cache_free(esch_scorer *s) {
+    if (s->map) {
+        (void)hipSetDevice(s->ordinal);
+        delete s->map;
+        s->map = nullptr;
+    }
+}
+
+struct esch_mapped {
+    std::vector<esch_read_mapping> reads;
+    std::vector<esch_pair_mapping> pairs;
+    std::vector<esch_hit> hits;
+    std::vector<uint32_t> ops;
+    bool is_pairs = false;
+    esch_map_stats stats{};
+};
+
+namespace {
+
+template <class T>
+esch_status upload(DevBuf &d, const std::vector<T> &h, hipStream_t st) {
+    HIPC(d.need(h.size() * sizeof(T)));
+    if (!h.empty()) HIPC(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    return ESCH_OK;
+}
+
+template <class T>
+esch_status download(std::vector<T> &h, const DevBuf &d, size_t n, hipStream_t st) {
+    h.resize(n);
+    if (n) HIPC(hipMemcpyAsync(h.data(), d.p, n * sizeof(T), hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    return ESCH_OK;
+}
+
+// One mapping call's context: the scorer, the batch holding the chunk and
+// the cache of small device buffers.
+struct Ctx {
+    esch_scorer *s;
+    esch_batch *b;
+    MapCache &C;
+    esch_map_stats &stats;
+    // row x of the batch: its codes on the device and its length
+    const uint8_t *row_ptr(int32_t x) const { return b->d_codes + b->h_qoff[x]; }
+    size_t row_len(int32_t x) const { return (size_t)b->h_qlen[x]; }
+
+    // Trace (row, ref) jobs from the batch's resident codes.
+    esch_status trace(const std::vector<int32_t> &rows, const std::vector<int32_t> &refs, Traced &t) {
+        t.al.assign(rows.size(), esch_alignment{});
+        t.ops.clear();
+        if (rows.empty()) return ESCH_OK;
+        esch_status st = align_core(
+            s, rows.size(),
+            [&](size_t k, const uint8_t *&p, size_t &len) {
+                p = row_ptr(rows[k]);
+                len = row_len(rows[k]);
+            },
+            refs.data(), t.al.data(), nullptr, 0, nullptr, b->d_codes, &t.ops);
+        stats.trace_ms += s->trace_ms;
+        stats.traced += (int64_t)rows.size();
+        return st;
+    }
+
+    // The unpaired selection and tracebacks of listed rows (lib/mapper.rs:
+    // 418-550): out[e] for rows[e]; hits[e] into pool with hit_offset = the
+    // pool index of the read's first hit.
+    esch_status map_rows(const std::vector<int32_t> &rows, int both, int32_t min_score, int64_t max_ties,
+                         std::vector<esch_read_mapping> &out, Pool &pool) {
+        const size_t n = rows.size();
+        out.assign(n, esch_read_mapping{});
+        if (!n) return ESCH_OK;
+        hipStream_t st = b->stream;
+        const auto t0 = Clock::now();
+        if (esch_status e = upload(C.list, rows, st)) return e;
+        HIPC(C.recs.need(n * kSelRec * 4));
+        HIPC(C.cands.need(n * kSelCap * 4));
+        MateSelArgs a{b->d_out32, (int32_t)b->n_reads, (int32_t)s->n_refs, both, min_score, max_ties,
+                      C.list.as<int32_t>(), (int32_t)n, kSelCap, nullptr, C.recs.as<int32_t>(), C.cands.as<int32_t>()};
+        HIPC(launch_mate_select(a, kResolveBlocks, st));
+        std::vector<int32_t> recs, ties;
+        if (esch_status e = download(recs, C.recs, n * kSelRec, st)) return e;
+        if (esch_status e = download(ties, C.cands, n * kSelCap, st)) return e;
+        // ties beyond the inline capacity: those reads again, with room for all
+        std::vector<int32_t> over, over_rows;
+        std::vector<int64_t> over_off;
+        int64_t total = 0;
+        for (size_t e = 0; e < n; ++e)
+            if (recs[e * kSelRec + 4] > kSelCap) {
+                over.push_back((int32_t)e);
+                over_rows.push_back(rows[e]);
+                over_off.push_back(total);
+                total += recs[e * kSelRec + 4];
+            }
+        std::vector<int32_t> ties_over;
+        if (!over.empty()) {
+            if (esch_status e = upload(C.list, over_rows, st)) return e;
+            if (esch_status e = upload(C.cand_off, over_off, st)) return e;
+            HIPC(C.out.need((size_t)total * 4));
+            HIPC(C.set.need(over.size() * kSelRec * 4));
+            MateSelArgs o = a;
+            o.list = C.list.as<int32_t>();
+            o.n_list = (int32_t)over.size();
+            o.cand_off = C.cand_off.as<int64_t>();
+            o.recs = C.set.as<int32_t>();
+            o.ties = C.out.as<int32_t>();
+            HIPC(launch_mate_select(o, kResolveBlocks, st));
+            if (esch_status e = download(ties_over, C.out, (size_t)total, st)) return e;
+        }
+        stats.select_ms += ms_since(t0);
+        std::vector<int64_t> tie_at(n, -1);  // into ties_over for the overflowed
+        for (size_t i = 0; i < over.size(); ++i) tie_at[over[i]] = over_off[i];
+        auto tie = [&](size_t e, int32_t j) -> int32_t {
+            return tie_at[e] >= 0 ? ties_over[(size_t)tie_at[e] + (size_t)j] : ties[e * kSelCap + (size_t)j];
+        };
+        // every taken tie traced (lib/mapper.rs:486-496)
+        std::vector<int32_t> jrows, jrefs;
+        std::vector<size_t> first(n, 0);
+        for (size_t e = 0; e < n; ++e) {
+            first[e] = jrows.size();
+            for (int32_t j = 0; j < recs[e * kSelRec + 4]; ++j) {
+                const int32_t c = tie(e, j);
+                jrows.push_back((c & 1) ? (int32_t)b->n_reads + rows[e] : rows[e]);
+                jrefs.push_back(c >> 1);
+            }
+        }
+        Traced t;
+        if (esch_status e = trace(jrows, jrefs, t)) return e;
+        for (size_t e = 0; e < n; ++e) {
+            const int32_t *r = &recs[e * kSelRec];
+            esch_read_mapping &m = out[e];
+            m.best = r[0];
+            m.n_tied = r[1];
+            m.subopt = r[2];
+            m.flags = r[3];
+            m.hit_offset = (int64_t)pool.hits.size();
+            const int32_t take = r[4];
+            // unmapped: nothing taken, or the primary aligns no base
+            // (lib/mapper.rs:508-512)
+            if (take == 0 || !t.aligned(first[e])) continue;
+            for (int32_t j = 0; j < take; ++j) {
+                const int32_t c = tie(e, j);
+                pool.add(t, first[e] + (size_t)j, c >> 1, c & 1, 0, 0);
+            }
+            m.n_hits = take;
+        }
+        return ESCH_OK;
+    }
+};
+
+// Copy a read mapping's hits from the pool to the result, in order.
+void place_read(esch_mapped *M, const Pool &pool, esch_read_mapping &m) {
+    const int64_t h0 = m.hit_offset;
+    m.hit_offset = (int64_t)M->hits.size();
+    for (int32_t k = 0; k < m.n_hits; ++k) {
+        esch_hit h = pool.hits[(size_t)(h0 + k)];
+        const uint32_t *src = pool.ops.data() + h.alignment.ops_offset;
+        h.alignment.ops_offset = (int64_t)M->ops.size();
+        M->ops.insert(M->ops.end(), src, src + h.alignment.n_ops);
+        M->hits.push_back(h);
+    }
+}
+
+using Getter = std::function<void(size_t, const uint8_t *&, size_t &)>;
+
+size_t chunk_rows(const esch_scorer *s) {
+    // resident int32 rows of a chunk: each read (mate) and its reverse complement
+    if (s->map_chunk) return 2 * s->map_chunk;
+    return std::max<size_t>(2048, kChunkRowBytes / (4 * std::max<size_t>(s->n_refs, 1)));
+}
+
+esch_status scorer_ready(esch_scorer *s) {
+    HIPC(hipSetDevice(s->ordinal));
+    if (!s->map) s->map = new MapCache;
+    return ESCH_OK;
+}
+
+// Load reads [k0, k1) of get into a batch (both orientations when both) and
+// score them into resident int32 rows.
+esch_status load_and_score(esch_scorer *s, const Getter &get, size_t k0, size_t k1, int both, int pairs,
+                           esch_batch **out, esch_map_stats &stats) {
+    const size_t n = k1 - k0;
+    std::vector<const uint8_t *> ptrs(n);
+    std::vector<size_t> lens(n);
+    size_t bases = 0;
+    for (size_t k = 0; k < n; ++k) {
+        get(k0 + k, ptrs[k], lens[k]);
+        bases += lens[k];
+    }
+    esch_batch *b = nullptr;
+    if (esch_status st = take_batch(s, n, bases, &b)) return st;
+    esch_status st = esch_batch_load(b, ptrs.data(), lens.data(), n, both, pairs);
+    if (!st) st = esch_batch_run(b, ESCH_RUN_SCORES32, 0, -1, 1);
+    if (!st) st = esch_batch_wait(b);
+    float total = 0.0f;
+    if (!st) st = esch_batch_timing(b, nullptr, &total);
+    if (st) {
+        give_batch(s, b);
+        return st;
+    }
+    stats.score_ms += total;
+    *out = b;
+    return ESCH_OK;
+}
+
+esch_status map_reads_core(esch_scorer *s, size_t n, const Getter &get, int both, int32_t min_score,
+                           int64_t max_ties, esch_mapped **out) {
+    const auto t0 = Clock::now();
+    std::unique_ptr<esch_mapped> M(new esch_mapped);
+    M->reads.resize(n);
+    if (esch_status st = scorer_ready(s)) return st;
+    const size_t chunk = chunk_rows(s) / 2;
+    for (size_t k0 = 0; k0 < n; k0 += chunk) {
+        const size_t k1 = std::min(n, k0 + chunk);
+        esch_batch *b = nullptr;
+        if (esch_status st = load_and_score(s, get, k0, k1, both, 0, &b, M->stats)) return st;
+        Ctx cx{s, b, *s->map, M->stats};
+        std::vector<int32_t> rows;  // non-empty reads; empty ones have best None
+        for (size_t k = k0; k < k1; ++k)
+            if (b->h_qlen[k - k0]) rows.push_back((int32_t)(k - k0));
+        std::vector<esch_read_mapping> maps;
+        Pool pool;
+        esch_status st = cx.map_rows(rows, both, min_score, max_ties, maps, pool);
+        give_batch(s, b);
+        if (st) return st;
+        for (size_t k = k0; k < k1; ++k) M->reads[k] = esch_read_mapping{};
+        for (size_t e = 0; e < rows.size(); ++e) M->reads[k0 + (size_t)rows[e]] = maps[e];
+        for (size_t k = k0; k < k1; ++k) {
+            esch_read_mapping &m = M->reads[k];
+            if (m.n_hits) place_read(M.get(), pool, m);
+            else m.hit_offset = (int64_t)M->hits.size();
+        }
+    }
+    M->stats.total_ms = ms_since(t0);
+    *out = M.release();
+    return ESCH_OK;
+}
+
+// pair_ok (lib/mapper/pair.rs:392-410) on the traced mates of a candidate.
+bool pair_ok(const esch_alignment &a0, bool al0, const esch_alignment &a1, bool al1, int t, int64_t max_insert) {
+    if (!al0 || !al1) return false;
+    // the mate read as sequenced on the template strand is the forward one
+    const esch_alignment &fw = t ? a1 : a0, &rv = t ? a0 : a1;
+    if (fw.ref_start >= rv.ref_end) return false;
+    const int32_t lo = std::min(a0.ref_start, a1.ref_start), hi = std::max(a0.ref_end, a1.ref_end);
+    return (int64_t)(hi - lo) <= max_insert;
+}
+
+esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, int both_strands, int32_t min_score,
+                           int64_t max_ties, int64_t max_insert, esch_mapped **out) {
+    const auto t0 = Clock::now();
+    std::unique_ptr<esch_mapped> M(new esch_mapped);
+    M->is_pairs = true;
+    M->pairs.resize(n_pairs);
+    if (esch_status st = scorer_ready(s)) return st;
+    const int strands = both_strands ? 2 : 1;
+    const size_t chunk = std::max<size_t>(1, chunk_rows(s) / 4);
+    for (size_t p0 = 0; p0 < n_pairs; p0 += chunk) {
+        const size_t p1 = std::min(n_pairs, p0 + chunk), c = p1 - p0;
+        esch_batch *b = nullptr;
+        if (esch_status st = load_and_score(s, mate, 2 * p0, 2 * p1, 1, 1, &b, M->stats)) return st;
+        struct Guard {
+            esch_scorer *s;
+            esch_batch *b;
+            ~Guard() { give_batch(s, b); }
+        } guard{s, b};
+        Ctx cx{s, b, *s->map, M->stats};
+        MapCache &C = *s->map;
+        hipStream_t st = b->stream;
+        const int32_t nrd = (int32_t)b->n_reads;  // 2c
+        std::vector<esch_pair_mapping> res(c);
+        Pool pool;
+        std::vector<int32_t> set_codes;           // tie sets of the proper pairs
+        std::vector<int64_t> set_off(c, -1);
+        std::vector<int32_t> set_n(c, 0);
+        std::vector<int32_t> list, below, sep;
+        for (size_t i = 0; i < c; ++i) {
+            if (b->h_qlen[2 * i] && b->h_qlen[2 * i + 1]) {
+                list.push_back((int32_t)i);
+                below.push_back(INT32_MAX);
+            } else {
+                sep.push_back((int32_t)i);  // an empty mate has no candidates
+            }
+        }
+        while (!list.empty()) {
+            ++M->stats.rounds;
+            const size_t n = list.size();
+            const auto ts = Clock::now();
+            if (esch_status e = upload(C.list, list, st)) return e;
+            if (esch_status e = upload(C.below, below, st)) return e;
+            HIPC(C.recs.need(n * kTierRec * 4));
+            HIPC(C.cands.need(n * kTierCap * 4));
+            TierArgs a{b->d_out32, nrd, (int32_t)s->n_refs, strands, min_score, C.list.as<int32_t>(),
+                       C.below.as<int32_t>(), (int32_t)n, kTierCap, nullptr, C.recs.as<int32_t>(),
+                       C.cands.as<int32_t>()};
+            HIPC(launch_pair_tier(a, kResolveBlocks, st));
+            std::vector<int32_t> recs, cands;
+            if (esch_status e = download(recs, C.recs, n * kTierRec, st)) return e;
+            if (esch_status e = download(cands, C.cands, n * kTierCap, st)) return e;
+            // tiers wider than the inline capacity: those pairs again
+            std::vector<int32_t> ol, ob;
+            std::vector<int64_t> oo, at(n, -1);
+            int64_t total = 0;
+            for (size_t e = 0; e < n; ++e)
+                if (recs[e * kTierRec] > kTierCap) {
+                    at[e] = total;
+                    ol.push_back(list[e]);
+                    ob.push_back(below[e]);
+                    oo.push_back(total);
+                    total += recs[e * kTierRec];
+                }
+            std::vector<int32_t> wide;
+            if (!ol.empty()) {
+                if (esch_status e = upload(C.list, ol, st)) return e;
+                if (esch_status e = upload(C.below, ob, st)) return e;
+                if (esch_status e = upload(C.cand_off, oo, st)) return e;
+                HIPC(C.out.need((size_t)total * 4));
+                HIPC(C.set.need(ol.size() * kTierRec * 4));
+                TierArgs w = a;
+                w.list = C.list.as<int32_t>();
+                w.below = C.below.as<int32_t>();
+                w.n_list = (int32_t)ol.size();
+                w.cand_off = C.cand_off.as<int64_t>();
+                w.recs = C.set.as<int32_t>();
+                w.cands = C.out.as<int32_t>();
+                HIPC(launch_pair_tier(w, kResolveBlocks, st));
+                if (esch_status e = download(wide, C.out, (size_t)total, st)) return e;
+            }
+            M->stats.select_ms += ms_since(ts);
+            auto cand = [&](size_t e, int32_t j) -> int32_t {
+                return at[e] >= 0 ? wide[(size_t)at[e] + (size_t)j] : cands[e * kTierCap + (size_t)j];
+            };
+            // both mates of every candidate of every listed tier
+            std::vector<int32_t> jrows, jrefs;
+            std::vector<size_t> first(n, 0);
+            for (size_t e = 0; e < n; ++e) {
+                first[e] = jrows.size();
+                const int32_t i = list[e];
+                for (int32_t j = 0; j < recs[e * kTierRec]; ++j) {
+                    const int32_t cd = cand(e, j), t = cd & 1;
+                    jrows.push_back(t ? nrd + 2 * i : 2 * i);          // mate 1
+                    jrows.push_back(t ? 2 * i + 1 : nrd + 2 * i + 1);  // mate 2
+                    jrefs.push_back(cd >> 1);
+                    jrefs.push_back(cd >> 1);
+                }
+            }
+            Traced tr;
+            if (esch_status e = cx.trace(jrows, jrefs, tr)) return e;
+            std::vector<int32_t> nl, nb;
+            for (size_t e = 0; e < n; ++e) {
+                const int32_t i = list[e];
+                const int32_t nt = recs[e * kTierRec];
+                if (!recs[e * kTierRec + 2]) {
+                    sep.push_back(i);  // no tier left
+                    continue;
+                }
+                std::vector<int32_t> ok;
+                for (int32_t j = 0; j < nt; ++j) {
+                    const size_t k = first[e] + 2 * (size_t)j;
+                    if (pair_ok(tr.al[k], tr.aligned(k), tr.al[k + 1], tr.aligned(k + 1), cand(e, j) & 1, max_insert))
+                        ok.push_back(j);
+                }
+                if (ok.empty()) {
+                    nl.push_back(i);
+                    nb.push_back(recs[e * kTierRec + 1]);
+                    continue;
+                }
+                // the tie set (lib/mapper/pair.rs:292-358): primary first, up
+                // to max_ties + 1 kept
+                esch_pair_mapping &m = res[(size_t)i];
+                m.proper = 1;
+                m.score = recs[e * kTierRec + 1];
+                m.n_tied = (int32_t)ok.size();
+                m.hit_offset = (int64_t)pool.hits.size();
+                set_off[(size_t)i] = (int64_t)set_codes.size();
+                set_n[(size_t)i] = (int32_t)ok.size();
+                int64_t keep = (int64_t)ok.size();
+                if (max_ties >= 0 && max_ties + 1 < keep) keep = max_ties + 1;
+                for (size_t q = 0; q < ok.size(); ++q) {
+                    const int32_t cd = cand(e, ok[q]);
+                    set_codes.push_back(cd);
+                    if ((int64_t)q >= keep) continue;
+                    const size_t k = first[e] + 2 * (size_t)ok[q];
+                    const esch_alignment &x = tr.al[k], &y = tr.al[k + 1];
+                    const int32_t insert = std::max(x.ref_end, y.ref_end) - std::min(x.ref_start, y.ref_start);
+                    const int32_t t = cd & 1;
+                    pool.add(tr, k, cd >> 1, t, t, insert);          // mate 1: reversed iff t
+                    pool.add(tr, k + 1, cd >> 1, !t, t, insert);     // mate 2: reversed iff !t
+                }
+                m.n_hits = (int32_t)keep;
+            }
+            list.swap(nl);
+            below.swap(nb);
+        }
+        // per-mate suboptimal of the proper pairs
+        std::vector<int32_t> pl, pn;
+        std::vector<int64_t> po;
+        for (size_t i = 0; i < c; ++i)
+            if (res[i].proper) {
+                pl.push_back((int32_t)i);
+                po.push_back(set_off[i]);
+                pn.push_back(set_n[i]);
+            }
+        if (!pl.empty()) {
+            const auto ts = Clock::now();
+            if (esch_status e = upload(C.list, pl, st)) return e;
+            if (esch_status e = upload(C.cand_off, po, st)) return e;
+            if (esch_status e = upload(C.set_n, pn, st)) return e;
+            if (esch_status e = upload(C.set, set_codes, st)) return e;
+            HIPC(C.out.need(pl.size() * kSuboptRec * 4));
+            SuboptArgs a{b->d_out32, nrd, (int32_t)s->n_refs, strands, C.list.as<int32_t>(),
+                         C.cand_off.as<int64_t>(), C.set_n.as<int32_t>(), C.set.as<int32_t>(), (int32_t)pl.size(),
+                         C.out.as<int32_t>()};
+            HIPC(launch_pair_subopt(a, kResolveBlocks, st));
+            std::vector<int32_t> so;
+            if (esch_status e = download(so, C.out, pl.size() * kSuboptRec, st)) return e;
+            M->stats.select_ms += ms_since(ts);
+            for (size_t q = 0; q < pl.size(); ++q) {
+                esch_pair_mapping &m = res[(size_t)pl[q]];
+                m.subopt[0] = so[q * kSuboptRec];
+                m.subopt[1] = so[q * kSuboptRec + 1];
+                m.flags = so[q * kSuboptRec + 2];
+            }
+        }
+        // the Separate pairs: each non-empty mate as an unpaired read on both
+        // strands (lib/mapper/pair.rs:360-387)
+        std::sort(sep.begin(), sep.end());
+        std::vector<int32_t> mrows;
+        for (int32_t i : sep)
+            for (int m = 0; m < 2; ++m)
+                if (b->h_qlen[2 * i + m]) mrows.push_back(2 * i + m);
+        std::vector<esch_read_mapping> mm;
+        if (esch_status e = cx.map_rows(mrows, 1, min_score, max_ties, mm, pool)) return e;
+        for (size_t q = 0; q < mrows.size(); ++q) {
+            const int32_t x = mrows[q];
+            res[(size_t)(x / 2)].separate[x % 2] = mm[q];
+        }
+        // into the result, in pair order
+        for (size_t i = 0; i < c; ++i) {
+            esch_pair_mapping &m = res[i];
+            if (m.proper) {
+                const int64_t h0 = m.hit_offset;
+                m.hit_offset = (int64_t)M->hits.size();
+                for (int32_t k = 0; k < 2 * m.n_hits; ++k) {
+                    esch_hit h = pool.hits[(size_t)(h0 + k)];
+                    const uint32_t *src = pool.ops.data() + h.alignment.ops_offset;
+                    h.alignment.ops_offset = (int64_t)M->ops.size();
+                    M->ops.insert(M->ops.end(), src, src + h.alignment.n_ops);
+                    M->hits.push_back(h);
+                }
+            } else {
+                m.hit_offset = (int64_t)M->hits.size();
+                for (int k = 0; k < 2; ++k) {
+                    if (m.separate[k].n_hits) place_read(M.get(), pool, m.separate[k]);
+                    else m.separate[k].hit_offset = (int64_t)M->hits.size();
+                }
+            }
+            M->pairs[p0 + i] = m;
+        }
+    }
+    M->stats.total_ms = ms_since(t0);
+    *out = M.release();
+    return ESCH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+esch_status esch_map_reads(esch_scorer *s, const uint8_t *const *reads, const size_t *lens, size_t n,
+                           int both_strands, int32_t min_score, int64_t max_ties, esch_mapped **out) {
+    if (!out) return fail(ESCH_INVALID_ARG, "out is null");
+    *out = nullptr;
+    if (!s || (n && (!reads || !lens))) return fail(ESCH_INVALID_ARG, "null argument");
+    return map_reads_core(
+        s, n,
+        [&](size_t k, const uint8_t *&p, size_t &len) {
+            p = reads[k];
+            len = lens[k];
+        },
+        both_strands ? 1 : 0, min_score, max_ties, out);
+}
+
+esch_status esch_map_pairs(esch_scorer *s, const uint8_t *const *mates1, const size_t *lens1,
+                           const uint8_t *const *mates2, const size_t *lens2, size_t n, int both_strands,
+                           int32_t min_score, int64_t max_ties, int64_t max_insert, esch_mapped **out) {
+    if (!out) return fail(ESCH_INVALID_ARG, "out is null");
+    *out = nullptr;
+    if (!s || (n && (!mates1 || !lens1 || !mates2 || !lens2))) return fail(ESCH_INVALID_ARG, "null argument");
+    if (max_insert < 0) return fail(ESCH_INVALID_ARG, "max_insert is negative (%lld)", (long long)max_insert);
+    return map_pairs_core(
+        s, n,
+        [&](size_t k, const uint8_t *&p, size_t &len) {
+            p = (k & 1) ? mates2[k >> 1] : mates1[k >> 1];
+            len = (k & 1) ? lens2[k >> 1] : lens1[k >> 1];
+        },
+        both_strands, min_score, max_ties, max_insert, out);
+}
+
+esch_status esch_map_pairs_concat(esch_scorer *s, const uint8_t *codes, const uint64_t *offs, const int32_t *lens,
+                                  size_t n_pairs, int both_strands, int32_t min_score, int64_t max_ties,
+                                  int64_t max_insert, esch_mapped **out) {
+    if (!out) return fail(ESCH_INVALID_ARG, "out is null");
+    *out = nullptr;
+    if (!s || (n_pairs && (!codes || !offs || !lens))) return fail(ESCH_INVALID_ARG, "null argument");
+    if (max_insert < 0) return fail(ESCH_INVALID_ARG, "max_insert is negative (%lld)", (long long)max_insert);
+    return map_pairs_core(
+        s, n_pairs,
+        [&](size_t k, const uint8_t *&p, size_t &len) {
+            p = codes + offs[k];
+            len = (size_t)(lens[k] < 0 ? 0 : lens[k]);
+        },
+        both_strands, min_score, max_ties, max_insert, out);
+}
+
+esch_status esch_set_map_chunk(esch_scorer *s, size_t reads) {
+    if (!s) return fail(ESCH_INVALID_ARG, "null scorer");
+    if (reads == 1) return fail(ESCH_INVALID_ARG, "a chunk holds at least one pair (2 reads)");
+    s->map_chunk = reads;
+    return ESCH_OK;
+}
+
+size_t esch_mapped_len(const esch_mapped *m) {
+    if (!m) return 0;
+    return m->is_pairs ? m->pairs.size() : m->reads.size();
+}
+const esch_read_mapping *esch_mapped_reads(const esch_mapped *m) {
+    return (m && !m->is_pairs) ? m->reads.data() : nullptr;
+}
+const esch_pair_mapping *esch_mapped_pairs(const esch_mapped *m) {
+    return (m && m->is_pairs) ? m->pairs.data() : nullptr;
+}
+const esch_hit *esch_mapped_hits(const esch_mapped *m, size_t *n) {
+    if (n) *n = m ? m->hits.size() : 0;
+    return m ? m->hits.data() : nullptr;
+}
+const uint32_t *esch_mapped_ops(const esch_mapped *m, size_t *n) {
+    if (n) *n = m ? m->ops.size() : 0;
+    return m ? m->ops.data() : nullptr;
+}
+void esch_mapped_stats(const esch_mapped *m, esch_map_stats *out) {
+    if (!out) return;
+    *out = m ? m->stats : esch_map_stats{};
+}
+void esch_mapped_free(esch_mapped *m) { delete m; }
+
+}  // extern "C"

@@ -4,6 +4,7 @@
 // (lib/cuda/mod.rs:328-749).
 #include "../../include/eschalign_amd.h"
 #include "esch_internal.h"
+#include "host_internal.h"
 
 #include <hip/hip_runtime.h>
 
@@ -24,8 +25,10 @@
 using namespace esch;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
+
+namespace esch {
 
 esch_status fail(esch_status st, const char *fmt, ...) {
     char buf[512];
@@ -37,12 +40,9 @@ esch_status fail(esch_status st, const char *fmt, ...) {
     return st;
 }
 
-#define HIPC(call)                                                                     \
-    do {                                                                               \
-        hipError_t e_ = (call);                                                        \
-        if (e_ != hipSuccess)                                                          \
-            return fail(ESCH_DRIVER, "%s failed: %s", #call, hipGetErrorString(e_));   \
-    } while (0)
+}  // namespace esch
+
+namespace {
 
 int32_t iabs(int32_t v) { return v < 0 ? -v : v; }
 
@@ -69,68 +69,6 @@ bool fits_i16(size_t n, size_t L, int32_t ma, int32_t mi, int32_t o, int32_t e) 
 
 }  // namespace
 
-struct TraceCache;
-void trace_cache_free(esch_scorer *s);
-
-struct esch_scorer {
-    int ordinal = 0;
-    std::string device_name;
-    int n_cu = 0;
-    Scoring sc{};
-    esch_mode mode = ESCH_LOCAL;
-    size_t n_refs = 0, max_len = 0;
-    int64_t total_len = 0;
-    uint32_t *d_ref2 = nullptr, *d_refN = nullptr, *d_w2off = nullptr, *d_nmoff = nullptr;
-    int32_t *d_rlen = nullptr, *d_rorder = nullptr;
-    DevPanel panel{};
-    Launch geo{};
-    int grid16_v[kDp16Variants] = {};   // persistent grid of each k_dp16 variant
-    int regs16_v[kDp16Variants] = {}, local16_v[kDp16Variants] = {};
-    int max_grid16 = 0;                 // scratch is sized for the largest
-    int regs16 = 0, local16 = 0;
-    int diag_min_len = 0;               // reads this long take k_diag16 (0: never)
-    int diag_ratio = kDiagRatioDefault; // padding-heavy length bins take k_diag16 (0: never)
-    bool diag_ratio_tunable = true;     // false once the caller set the ratio
-    std::mutex pool_mu;
-    std::vector<esch_batch *> pool;  // idle batches for the drop-in API
-    std::vector<int32_t> h_rlen;     // reference lengths (traceback sizing)
-    float trace_ms = 0.0f;           // device time of the last esch_align_pairs kernels
-    TraceCache *trace = nullptr;     // traceback buffers kept between calls
-};
-
-struct esch_batch {
-    esch_scorer *s = nullptr;
-    size_t max_reads = 0, max_bases = 0;  // capacity before strand expansion
-    size_t tie_cap_alloc = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
-    // device
-    uint8_t *d_codes = nullptr;
-    uint64_t *d_qoff = nullptr;
-    int32_t *d_qlen = nullptr;
-    uint8_t *d_cls = nullptr;
-    int32_t *d_hist = nullptr, *d_bin_start = nullptr, *d_bin_fill = nullptr;
-    int32_t *d_sorted16 = nullptr, *d_list32 = nullptr;
-    int32_t *d_out32 = nullptr;
-    int16_t *d_out16 = nullptr;
-    uint2 *d_scr16 = nullptr;
-    int2 *d_scr32 = nullptr;
-    int32_t *d_recs = nullptr, *d_ties = nullptr;
-    int32_t *d_listD = nullptr;
-    uint2 *d_scrD = nullptr;
-    DevDesc *d_desc = nullptr;
-    // pinned staging
-    uint8_t *h_codes = nullptr;
-    uint64_t *h_qoff = nullptr;
-    int32_t *h_qlen = nullptr;
-    DevDesc *h_desc = nullptr;
-    // last load / run
-    size_t n_reads = 0, n_rows = 0;
-    int both = 0, pairs = 0;
-    int last_what = 0;
-    bool timed_dp = false;
-    std::map<int, hipGraphExec_t> graphs;
-};
 
 struct esch_prepared {
     size_t n = 0;
@@ -293,6 +231,7 @@ void esch_scorer_free(esch_scorer *s) {
     if (!s) return;
     for (esch_batch *b : s->pool) esch_batch_free(b);
     trace_cache_free(s);
+    map_cache_free(s);
     (void)hipFree(s->d_ref2);
     (void)hipFree(s->d_refN);
     (void)hipFree(s->d_w2off);
@@ -817,7 +756,9 @@ size_t esch_prepared_n_scored(const esch_prepared *p) {
     return c;
 }
 
-namespace {
+}  // extern "C"
+
+namespace esch {
 
 esch_status take_batch(esch_scorer *s, size_t n, size_t bases, esch_batch **out) {
     {
@@ -848,7 +789,9 @@ void give_batch(esch_scorer *s, esch_batch *b) {
     }
 }
 
-}  // namespace
+}  // namespace esch
+
+extern "C" {
 
 esch_status esch_submit(esch_scorer *s, esch_prepared *p, esch_pending **out) {
     std::unique_ptr<esch_prepared> prep(p);
@@ -1060,52 +1003,6 @@ constexpr uint64_t kTraceMaxCells = (uint64_t)1 << 27;  // read x reference: a 4
 
 }  // namespace
 
-// Grow-only device buffer of the traceback cache.
-struct DevBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-    hipError_t need(size_t bytes) {
-        bytes = std::max<size_t>(bytes, 256);
-        if (bytes <= cap) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-        const hipError_t e = hipMalloc(&p, bytes);
-        if (e == hipSuccess) cap = bytes;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-    template <class T>
-    T *as() const { return static_cast<T *>(p); }
-};
-
-// Grow-only pinned host buffer (staging of the traceback's largest copies).
-struct HostBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-    hipError_t need(size_t bytes) {
-        bytes = std::max<size_t>(bytes, 256);
-        if (bytes <= cap) return hipSuccess;
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        cap = 0;
-        const hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
-        if (e == hipSuccess) cap = bytes;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-    template <class T>
-    T *as() const { return static_cast<T *>(p); }
-};
-
 // Device buffers, pinned staging, stream and events of esch_align_pairs,
 // kept by the scorer between calls (one call at a time per scorer).
 struct TraceCache {
@@ -1136,10 +1033,10 @@ void trace_cache_free(esch_scorer *s) {
 }
 
 // The traceback of n pairs whose reads come from get(k) -> (codes, length).
-static esch_status align_core(esch_scorer *s, size_t n,
-                              const std::function<void(size_t, const uint8_t *&, size_t &)> &get,
-                              const int32_t *refs, esch_alignment *out, uint32_t *ops, size_t ops_cap,
-                              size_t *ops_used) {
+esch_status esch::align_core(esch_scorer *s, size_t n,
+                             const std::function<void(size_t, const uint8_t *&, size_t &)> &get,
+                             const int32_t *refs, esch_alignment *out, uint32_t *ops, size_t ops_cap,
+                             size_t *ops_used, const uint8_t *dev_codes, std::vector<uint32_t> *ops_vec) {
     if (ops_used) *ops_used = 0;
     std::vector<size_t> lens(n);
     std::vector<const uint8_t *> ptr(n);
@@ -1186,7 +1083,7 @@ static esch_status align_core(esch_scorer *s, size_t n,
     TraceCache &T = *s->trace;
     std::lock_guard<std::mutex> lock(T.mu);
     if (T.st) HIPC(hipStreamSynchronize(T.st));  // the pinned staging may still feed a copy
-    HIPC(T.h_codes.need(bases));
+    if (!dev_codes) HIPC(T.h_codes.need(bases));
     HIPC(T.h_recs.need(n * kTraceRec * 4));
     uint8_t *const hcodes = T.h_codes.as<uint8_t>();
     std::vector<uint64_t> hqoff(np), ops_off(n, 0);
@@ -1194,14 +1091,16 @@ static esch_status align_core(esch_scorer *s, size_t n,
     size_t at = 0;
     for (size_t i = 0; i < np; ++i) {
         const int32_t k = order[i];
-        hqoff[i] = at;
+        // resident reads are addressed where they lie; staged ones packed
+        hqoff[i] = dev_codes ? (uint64_t)(ptr[k] - dev_codes) : at;
         hqlen[i] = (int32_t)lens[k];
         href[i] = refs[k];
         hslot[i] = k;
         at += lens[k];
     }
     // the gather into staging and the code check on up to 8 host threads
-    {
+    // (resident reads were checked when they were loaded)
+    if (!dev_codes) {
         const size_t hw = std::max(1u, std::thread::hardware_concurrency());
         const size_t nt = bases < ((size_t)8 << 20) ? 1 : std::min<size_t>(8, hw);
         std::vector<uint8_t> bad(nt, 0);
@@ -1269,7 +1168,7 @@ static esch_status align_core(esch_scorer *s, size_t n,
     if (!T.st) HIPC(hipStreamCreateWithFlags(&T.st, hipStreamNonBlocking));
     if (!T.e0) HIPC(hipEventCreate(&T.e0));
     if (!T.e1) HIPC(hipEventCreate(&T.e1));
-    HIPC(T.codes.need(bases));
+    if (!dev_codes) HIPC(T.codes.need(bases));
     HIPC(T.qoff.need(np * 8));
     HIPC(T.qlen.need(np * 4));
     HIPC(T.ref.need(np * 4));
@@ -1282,7 +1181,7 @@ static esch_status align_core(esch_scorer *s, size_t n,
     HIPC(T.ops_wc.need(wc * 4));
     HIPC(T.ops_off.need(n * 8));
     hipStream_t st = T.st;
-    HIPC(hipMemcpyAsync(T.codes.p, hcodes, bases, hipMemcpyHostToDevice, st));
+    if (!dev_codes) HIPC(hipMemcpyAsync(T.codes.p, hcodes, bases, hipMemcpyHostToDevice, st));
     HIPC(hipMemcpyAsync(T.qoff.p, hqoff.data(), np * 8, hipMemcpyHostToDevice, st));
     HIPC(hipMemcpyAsync(T.qlen.p, hqlen.data(), np * 4, hipMemcpyHostToDevice, st));
     HIPC(hipMemcpyAsync(T.ref.p, href.data(), np * 4, hipMemcpyHostToDevice, st));
@@ -1293,7 +1192,7 @@ static esch_status align_core(esch_scorer *s, size_t n,
     HIPC(hipMemcpyAsync(T.ops_off.p, ops_off.data(), n * 8, hipMemcpyHostToDevice, st));
     // empty pairs are never traced: their records must read as zero runs
     HIPC(hipMemsetAsync(T.recs.p, 0, n * kTraceRec * 4, st));
-    TraceArgs ta{s->panel, s->sc, T.codes.as<uint8_t>(), T.qoff.as<uint64_t>(), T.qlen.as<int32_t>(),
+    TraceArgs ta{s->panel, s->sc, dev_codes ? dev_codes : T.codes.as<uint8_t>(), T.qoff.as<uint64_t>(), T.qlen.as<int32_t>(),
                  T.ref.as<int32_t>(), T.slot.as<int32_t>(), T.gbase.as<uint64_t>(), T.gS.as<int32_t>(),
                  T.gM.as<int32_t>(), T.arena.as<uint8_t>(), T.recs.as<int32_t>(), T.ops_wc.as<uint32_t>(),
                  T.ops_off.as<uint64_t>(), (int32_t)np, 0};
@@ -1316,6 +1215,11 @@ static esch_status align_core(esch_scorer *s, size_t n,
         if (lens[k]) runs += (size_t)recs[k * kTraceRec + 5];
     }
     if (ops_used) *ops_used = runs;
+    if (ops_vec) {
+        ops_vec->resize(runs);
+        ops = ops_vec->data();
+        ops_cap = runs;
+    }
     if (runs > ops_cap || (runs && !ops))
         return fail(ESCH_CAPACITY, "the alignments have %zu operation runs; ops holds %zu", runs, ops_cap);
     if (runs) {

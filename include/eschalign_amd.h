@@ -236,6 +236,90 @@ esch_status esch_align_pairs_concat(esch_scorer *s, const uint8_t *codes, const 
  * last esch_align_pairs call (diagnostics; no counterpart in the reference). */
 float esch_align_last_ms(const esch_scorer *s);
 
+/* ---- read and pair mapping: scoring, selection, tracebacks and the pair
+ * tier loop on the device (lib/mapper.rs:400-550, lib/mapper/pair.rs:157-387).
+ * Reads and mates are codes (adapters already trimmed, as the CLI does,
+ * crates/eschalign-cli/src/align.rs:1642-1662); MD / NM are left to the
+ * caller (lib/sam.rs:76-140 needs only the letters and these records).
+ *
+ * esch_hit — Hit (lib/mapper.rs:43-56) without md / nm; for the hits of a
+ * proper pair also PairHit::reverse and PairHit::insert
+ * (lib/mapper/pair.rs:69-81). */
+typedef struct {
+    int32_t reference;
+    int32_t reverse;          /* the read's (mate's) own strand */
+    int32_t template_reverse; /* proper-pair hits: the template strand; else 0 */
+    int32_t insert;           /* proper-pair hits: the span of both mates; else 0 */
+    esch_alignment alignment; /* ops in esch_mapped_ops */
+} esch_hit;
+
+/* ReadMapping (lib/mapper.rs:58-72). */
+typedef struct {
+    int32_t best;     /* best_score (valid when flags bit 0) */
+    int32_t n_tied;
+    int32_t subopt;   /* suboptimal (valid when flags bit 1) */
+    int32_t flags;    /* bit 0: best present (read non-empty); bit 1: suboptimal present */
+    int32_t n_hits;   /* 0: unmapped */
+    int32_t reserved;
+    int64_t hit_offset; /* hits[hit_offset .. hit_offset + n_hits) */
+} esch_read_mapping;
+
+/* PairMapping (lib/mapper/pair.rs:90-107). */
+typedef struct {
+    int32_t proper;    /* 1: Proper; 0: Separate */
+    int32_t score;     /* Proper: the summed score of the tie set */
+    int32_t n_tied;    /* Proper: size of the tie set */
+    int32_t flags;     /* Proper: bit m = suboptimal[m] present */
+    int32_t subopt[2]; /* Proper: per mate, the best score outside the tie set */
+    int32_t n_hits;    /* Proper: pair hits kept (primary first, up to max_ties + 1);
+                          hit h is hits[hit_offset + 2h] (mate 1) and [.. + 2h + 1] (mate 2) */
+    int32_t reserved;
+    int64_t hit_offset;
+    esch_read_mapping separate[2]; /* Separate: mate 1 and mate 2 as unpaired reads */
+} esch_pair_mapping;
+
+typedef struct esch_mapped esch_mapped;
+
+/* Aligner::map_reads_scored (lib/mapper.rs:400-550) with the scores computed
+ * on the device: selection over (ref r forward, ref r reverse, r + 1, ...),
+ * the taken ties (max_ties + 1, all when max_ties < 0; none below
+ * min_score) traced, a read unmapped when its primary traces to nothing. */
+esch_status esch_map_reads(esch_scorer *s, const uint8_t *const *reads, const size_t *lens, size_t n,
+                           int both_strands, int32_t min_score, int64_t max_ties, esch_mapped **out);
+/* Aligner::map_pairs_scored (lib/mapper/pair.rs:157-387): candidates
+ * (reference, template strand) with both mates >= min_score, taken tier by
+ * tier in descending summed score; the first tier with a candidate passing
+ * pair_ok (both mates aligned, no outie, insert <= max_insert) is the tie
+ * set; pairs with none are mapped mate by mate on both strands. */
+esch_status esch_map_pairs(esch_scorer *s, const uint8_t *const *mates1, const size_t *lens1,
+                           const uint8_t *const *mates2, const size_t *lens2, size_t n, int both_strands,
+                           int32_t min_score, int64_t max_ties, int64_t max_insert, esch_mapped **out);
+/* The same with mates in one buffer, interleaved: mate m of pair p is
+ * codes[offs[2p + m] .. + lens[2p + m]). */
+esch_status esch_map_pairs_concat(esch_scorer *s, const uint8_t *codes, const uint64_t *offs, const int32_t *lens,
+                                  size_t n_pairs, int both_strands, int32_t min_score, int64_t max_ties,
+                                  int64_t max_insert, esch_mapped **out);
+size_t esch_mapped_len(const esch_mapped *m);
+const esch_read_mapping *esch_mapped_reads(const esch_mapped *m); /* NULL for pairs */
+const esch_pair_mapping *esch_mapped_pairs(const esch_mapped *m); /* NULL for reads */
+const esch_hit *esch_mapped_hits(const esch_mapped *m, size_t *n);
+const uint32_t *esch_mapped_ops(const esch_mapped *m, size_t *n);
+/* Where the time of the call went (diagnostics; no counterpart in the
+ * reference): ms of scoring (device events), selection kernels, traceback
+ * kernels, and the whole call; tier rounds; pairs traced. */
+typedef struct {
+    double score_ms, select_ms, trace_ms, total_ms;
+    int32_t rounds;
+    int32_t reserved;
+    int64_t traced;
+} esch_map_stats;
+void esch_mapped_stats(const esch_mapped *m, esch_map_stats *out);
+void esch_mapped_free(esch_mapped *m);
+/* Reads (mates) per device chunk of the mapping calls; 0 (the default)
+ * sizes chunks so their resident int32 rows take about 4 GiB. Results do
+ * not depend on it (no counterpart in the reference). */
+esch_status esch_set_map_chunk(esch_scorer *s, size_t reads);
+
 /* Kernel variants of the batched DP, and the autotuner that picks one by
  * timing each on a sample of the caller's reads (no counterpart in the
  * reference; the scores are identical whichever variant runs):

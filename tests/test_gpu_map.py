@@ -1,0 +1,252 @@
+"""Read and pair mapping on an MI355X (esch_map_reads / esch_map_pairs: the
+device counterparts of Aligner::map_reads_scored, lib/mapper.rs:400-550, and
+map_pairs_scored with its tier loop and Separate fallback,
+lib/mapper/pair.rs:157-387) against the oracle's eo_map_reads /
+eo_map_pairs, which test_oracle_map.py pins to the reference's own cases and
+to the paired fixture. Equality of every record: kind, scores, tie counts,
+suboptimal scores, every hit's reference, strands, insert, coordinates and
+runs."""
+import os
+
+import numpy as np
+import pytest
+
+import eschalign_amd as ea
+from eschalign_amd import synth
+from oracle import pyoracle as O
+import refgen as G
+from test_oracle_map import TRUSEQ_R1, TRUSEQ_R2, rc, xorshift_panel
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT = (2, -1, -10, -1)
+
+
+def dev_reads(gpu, reads, **kw):
+    recs, hits, ops, _ = gpu.map_reads_raw(reads, **kw)
+    return [O.read_form(r, hits, ops) for r in recs]
+
+
+def dev_pairs(gpu, pairs, **kw):
+    recs, hits, ops, stats = gpu.map_pairs_raw([p[0] for p in pairs], [p[1] for p in pairs], **kw)
+    return [O.pair_form(r, hits, ops) for r in recs], stats
+
+
+def compare_pairs(gpu, panel_codes, pairs, scoring, local, **kw):
+    got, stats = dev_pairs(gpu, pairs, **kw)
+    want = O.map_pairs(pairs, panel_codes, scoring, local, **kw)
+    for k, (g, w) in enumerate(zip(got, want)):
+        assert g == w, (k, kw, g, w)
+    assert len(got) == len(pairs)
+    return got, stats
+
+
+def test_map_reads_equals_oracle():
+    """Random panels and the gpu_parity read mix (every length 1-40, reads
+    from the panel, random reads), an empty and an all-wildcard read; both
+    modes, one and both strands, min-score and max-ties."""
+    rng = G.Rng(0x5EED0201)
+    panel_l = G.random_panel(rng, 24, 40, 200)
+    reads = [G.encode(r) for r in G.reads_for(rng, panel_l, 160)]
+    reads += [np.zeros(0, np.uint8), G.encode(b"NNNNNN")]
+    panel = ea.Panel(panel_l)
+    for scoring, mode in ((DEFAULT, ea.Mode.LOCAL), ((1, -1, -2, -1), ea.Mode.SEMIGLOBAL)):
+        gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+        local = mode == ea.Mode.LOCAL
+        for both, min_score, max_ties in ((False, 0, None), (True, 0, None), (True, 25, 0), (True, 0, 2)):
+            kw = dict(both_strands=both, min_score=min_score, max_ties=max_ties)
+            got = dev_reads(gpu, reads, **kw)
+            want = O.map_reads(reads, panel.codes, scoring, local, **kw)
+            for k, (g, w) in enumerate(zip(got, want)):
+                assert g == w, (k, kw, g, w)
+        # the object API reads the same records
+        objs = gpu.map_reads(reads[:50], both_strands=True)
+        forms = dev_reads(gpu, reads[:50], both_strands=True)
+        for o, f in zip(objs, forms):
+            assert (o.best_score, o.n_tied, o.suboptimal) == f[:3]
+            assert tuple((h.reference, h.reverse, h.alignment.score, h.alignment.query_start,
+                          h.alignment.query_end, h.alignment.ref_start, h.alignment.ref_end,
+                          tuple(h.alignment.ops)) for h in o.hits) == f[3]
+
+
+def test_map_pairs_reference_cases():
+    """lib/mapper/pair.rs:449-573 on the device: the same outcomes, and
+    records equal to the oracle's under every option."""
+    a, b, c = xorshift_panel()
+    panel = ea.Panel([a, b, c])
+    gpu = ea.GpuScorer(panel, ea.Scoring(*DEFAULT), ea.Mode.SEMIGLOBAL)
+    e = G.encode
+    m = gpu.map_pairs([(e(a[:30]), e(rc(a[30:])))])[0]
+    assert isinstance(m, ea.ProperPair) and m.n_tied == 1 and m.score == 120
+    assert m.hits[0].reference == 0 and m.hits[0].insert == 60
+    assert not m.hits[0].mates[0].reverse and m.hits[0].mates[1].reverse
+    opts = dict(min_score=30)
+    assert isinstance(gpu.map_pairs([(e(a[:20]), e(rc(a[40:])))], max_insert=50, **opts)[0], ea.SeparatePair)
+    assert isinstance(gpu.map_pairs([(e(a[:20]), e(rc(a[40:])))], **opts)[0], ea.ProperPair)
+    assert isinstance(gpu.map_pairs([(e(a[:20]), e(a[40:]))], **opts)[0], ea.SeparatePair)
+    assert isinstance(gpu.map_pairs([(e(a[40:]), e(rc(a[:20])))], **opts)[0], ea.SeparatePair)
+    assert isinstance(gpu.map_pairs([(e(rc(a[40:])), e(a[:20]))], **opts)[0], ea.SeparatePair)
+    m = gpu.map_pairs([(e(rc(a[40:])), e(a[:20]))], both_strands=True, **opts)[0]
+    assert isinstance(m, ea.ProperPair) and m.hits[0].reverse
+    assert m.hits[0].mates[0].reverse and not m.hits[0].mates[1].reverse
+    assert isinstance(gpu.map_pairs([(e(a[:30]), e(rc(a[30:])))], min_score=61)[0], ea.SeparatePair)
+    pairs = [(e(a[:30]), e(rc(a[30:]))), (e(b[:30]), e(rc(b[30:]))), (e(a[:20]), e(a[40:])),
+             (np.zeros(0, np.uint8), e(a[:20])), (e(rc(a[40:])), e(a[:20])), (e(a[40:]), e(rc(a[:20])))]
+    for mode in (ea.Mode.SEMIGLOBAL, ea.Mode.LOCAL):
+        g2 = ea.GpuScorer(panel, ea.Scoring(*DEFAULT), mode)
+        for kw in (dict(), dict(both_strands=True), dict(min_score=30, max_insert=50),
+                   dict(both_strands=True, min_score=30, max_ties=0)):
+            compare_pairs(g2, panel.codes, pairs, DEFAULT, mode == ea.Mode.LOCAL, **kw)
+
+
+def fixture_pairs():
+    panel = G.read_fasta(os.path.join(HERE, "golden", "pairs", "pairs_panel.fa"))
+    r1 = G.read_fastq(os.path.join(HERE, "golden", "pairs", "pairs_R1.fq"))
+    r2 = G.read_fastq(os.path.join(HERE, "golden", "pairs", "pairs_R2.fq"))
+    names, pairs = [], []
+    for (n1, s1), (_, s2) in zip(r1, r2):
+        ad = ea.adapter_detect_pair(s1, s2)
+        names.append(n1)
+        pairs.append((ea.encode(s1[:len(s1) - ad[0]]), ea.encode(s2[:len(s2) - ad[1]])))
+    return [n for n, _ in panel], [s for _, s in panel], names, pairs
+
+
+def test_map_pairs_paired_fixture():
+    """crates/eschalign-cli/tests/align_paired.rs on the fixture (mates less
+    their adapters, semi-global, --min-score 40): equal to the oracle under
+    the default and --max-insert 120, and the outcomes the CLI tests state."""
+    ref_names, refs, names, pairs = fixture_pairs()
+    panel = ea.Panel(list(zip(ref_names, refs)))
+    for mode in (ea.Mode.SEMIGLOBAL, ea.Mode.LOCAL):
+        gpu = ea.GpuScorer(panel, ea.Scoring(*DEFAULT), mode)
+        for kw in (dict(min_score=40), dict(min_score=40, max_insert=120), dict(min_score=40, both_strands=True)):
+            compare_pairs(gpu, panel.codes, pairs, DEFAULT, mode == ea.Mode.LOCAL, **kw)
+    gpu = ea.GpuScorer(panel, ea.Scoring(*DEFAULT), ea.Mode.SEMIGLOBAL)
+    got = dict(zip(names, gpu.map_pairs(pairs, min_score=40)))
+    ref = {n: i for i, n in enumerate(ref_names)}
+    m = got["paralog"]
+    assert isinstance(m, ea.ProperPair) and m.n_tied == 1 and m.hits[0].reference == ref["P1"]
+    assert m.hits[0].insert == 130
+    assert [x.alignment.ref_start for x in m.hits[0].mates] == [20, 90]
+    assert m.suboptimal[0] == m.hits[0].mates[0].alignment.score
+    m = got["pair_tie"]
+    assert m.n_tied == 2 and [h.reference for h in m.hits] == [ref["P1"], ref["P2"]]
+    m = got["short_insert"]
+    assert m.hits[0].insert == 40 and m.hits[0].reference == ref["U1"]
+    assert [x.alignment.cigar() for x in m.hits[0].mates] == ["40M", "40M"]
+    m = got["one_mate_reads_through"]
+    assert [x.alignment.cigar() for x in m.hits[0].mates] == ["40M", "35M"]
+    assert isinstance(got["wrong_orientation"], ea.SeparatePair)
+    m = got["one_unmapped"]
+    assert isinstance(m, ea.SeparatePair) and m.mates[0].is_mapped() and not m.mates[1].is_mapped()
+    sep = gpu.map_pairs([pairs[names.index("over_max_insert")]], min_score=40, max_insert=120)[0]
+    assert isinstance(sep, ea.SeparatePair)
+    assert [x.hits[0].reference for x in sep.mates] == [ref["U1"]] * 2
+
+
+def rand_seq(rng, n):
+    return bytes(np.frombuffer(b"ACGT", np.uint8)[rng.integers(0, 4, n)])
+
+
+def random_pairs(seed, n_pairs):
+    """Panels of paralog pairs (one substitution apart) and pairs of every
+    kind: FR, RF, FF, outies, junk mates, empty mates, long gaps apart."""
+    rng = np.random.default_rng(seed)
+    base = [rand_seq(rng, int(rng.integers(60, 160))) for _ in range(10)]
+    refs = []
+    for s in base:
+        refs.append(s)
+        t = bytearray(s)
+        t[int(rng.integers(len(t)))] = ord("ACGT"[int(rng.integers(4))])
+        refs.append(bytes(t))
+    pairs = []
+    for _ in range(n_pairs):
+        s = refs[int(rng.integers(len(refs)))]
+        i = int(rng.integers(0, len(s) - 30))
+        j = int(rng.integers(i + 15, len(s) + 1))
+        x = s[i:i + int(rng.integers(12, 45))]
+        y = s[max(0, j - int(rng.integers(12, 45))):j]
+        kind = int(rng.integers(7))
+        junk = rand_seq(rng, len(y))
+        p = [(x, rc(y)), (rc(y), x), (x, y), (rc(x), y), (x, junk), (b"", x), (x, rc(y))][kind]
+        pairs.append((ea.encode(p[0]), ea.encode(p[1])))
+    return refs, pairs
+
+
+def test_map_pairs_random_equal_oracle_and_chunking():
+    """Random pairs of every kind on paralog panels under many options
+    (fall-through to lower tiers, ties, both template strands, max-ties);
+    results identical when the call is cut into small device chunks."""
+    refs, pairs = random_pairs(0x9A1F, 300)
+    panel = ea.Panel(refs)
+    for scoring, mode in ((DEFAULT, ea.Mode.SEMIGLOBAL), ((1, -1, -2, -1), ea.Mode.LOCAL)):
+        gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+        local = mode == ea.Mode.LOCAL
+        for kw in (dict(), dict(both_strands=True, min_score=10), dict(min_score=15, max_insert=60, max_ties=1),
+                   dict(both_strands=True, max_ties=0, max_insert=80)):
+            whole, stats = compare_pairs(gpu, panel.codes, pairs, scoring, local, **kw)
+            assert stats["rounds"] >= 2  # some pairs fall through their top tier
+            gpu.set_map_chunk(38)  # 19 pairs per chunk
+            cut, _ = dev_pairs(gpu, pairs, **kw)
+            gpu.set_map_chunk(0)
+            assert cut == whole
+        reads = [p[k] for p in pairs for k in range(2)]
+        full = dev_reads(gpu, reads, both_strands=True)
+        gpu.set_map_chunk(25)
+        assert dev_reads(gpu, reads, both_strands=True) == full
+        gpu.set_map_chunk(0)
+
+
+def test_map_pairs_wide_tiers_and_tie_lists():
+    """40 identical references: a pair's tier (and a Separate mate's tie
+    list) is wider than what one selection round returns inline, so the
+    device takes its second pass for them; equal to the oracle with all ties
+    and with max_ties."""
+    rng = np.random.default_rng(7)
+    s = rand_seq(rng, 120)
+    other = rand_seq(rng, 120)
+    refs = [s] * 40 + [other]
+    panel = ea.Panel(refs)
+    e = ea.encode
+    pairs = [(e(s[:40]), e(rc(s[70:110]))),   # proper on all 40: a 40-wide tie set
+             (e(s[:40]), e(s[70:110])),       # FF: Separate under a floor, 40 ties per mate
+             (e(rc(s[70:110])), e(s[:40])),   # RF: proper only on the reverse template
+             (e(other[:50]), e(rc(other[60:])))]
+    for mode in (ea.Mode.SEMIGLOBAL, ea.Mode.LOCAL):
+        gpu = ea.GpuScorer(panel, ea.Scoring(*DEFAULT), mode)
+        for kw in (dict(), dict(both_strands=True), dict(min_score=30), dict(min_score=30, max_ties=3),
+                   dict(both_strands=True, max_ties=20)):
+            compare_pairs(gpu, panel.codes, pairs, DEFAULT, mode == ea.Mode.LOCAL, **kw)
+        # (a floor makes the FF pair's reversed mate "not aligned": Separate)
+        got, _ = dev_pairs(gpu, pairs, min_score=30)
+        assert got[0][0] == "proper" and got[0][2] == 40 and len(got[0][4]) == 40
+        assert got[1][0] == "separate" and got[1][1][1] == 40 and len(got[1][1][3]) == 40
+
+
+def test_map_pairs_paired_workload_sample():
+    """500 references of 70-95 nt, 1,500 pairs of 150-nt mates from inserts
+    N(120, 30) that read into the TruSeq adapters, mates trimmed, semi-global
+    2,-1,-10,-1, both template strands and forward only: equal to the oracle
+    (its rows from the SIMD restatement, pinned to the scalar DP)."""
+    panel_l = synth.panel(seed=0xC0F4, n_refs=500, lo=70, hi=95)
+    codes, off, lens = synth.pair_arrays(panel_l, seed=0xC0F5, n_pairs=1500)
+    letters = [bytes(b"ACGTN"[c] for c in codes[int(o):int(o) + int(n)]) for o, n in zip(off, lens)]
+    pairs = []
+    for k in range(0, len(letters), 2):
+        a, b = letters[k], letters[k + 1]
+        ad = ea.adapter_detect_pair(a, b)
+        pairs.append((ea.encode(a[:len(a) - ad[0]]), ea.encode(b[:len(b) - ad[1]])))
+    sup = O.simd_supported()
+    prof = O.Profile(panel_l, DEFAULT, 32 if sup & 2 else 16)
+    mates = [x for p in pairs for x in p]
+    fwd = prof.score_batch(*O.concat(mates), False, 8)
+    rev = prof.score_batch(*O.concat([O.revcomp(x) for x in mates]), False, 8)
+    gpu = ea.GpuScorer(ea.Panel(panel_l), ea.Scoring(*DEFAULT), ea.Mode.SEMIGLOBAL)
+    for kw in (dict(), dict(both_strands=True)):
+        got, stats = dev_pairs(gpu, pairs, **kw)
+        want = O.map_pairs(pairs, panel_l, DEFAULT, False, rows=(fwd, rev), **kw)
+        for k, (g, w) in enumerate(zip(got, want)):
+            assert g == w, (k, kw)
+        kinds = [g[0] for g in got]
+        assert kinds.count("proper") > len(pairs) // 2

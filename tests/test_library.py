@@ -167,3 +167,35 @@ def test_alignment_record_layout_matches_the_header(tmp_path):
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
     assert out.split() == ["32", "24"]
+
+
+def test_mapping_record_layouts_match_the_header(tmp_path):
+    """esch_hit / esch_read_mapping / esch_pair_mapping, compiled from the
+    header, have the size and field offsets of the Python dtypes (and of the
+    oracle's records, which tests compare with them)."""
+    import subprocess
+    from oracle import pyoracle as O
+    fields = {
+        "esch_hit": ["reference", "reverse", "template_reverse", "insert", "alignment"],
+        "esch_read_mapping": ["best", "n_tied", "subopt", "flags", "n_hits", "hit_offset"],
+        "esch_pair_mapping": ["proper", "score", "n_tied", "flags", "subopt", "n_hits", "hit_offset",
+                              "separate"],
+    }
+    body = "".join(
+        f'printf("{t} %zu", sizeof({t}));' + "".join(f'printf(" %zu", offsetof({t}, {f}));' for f in fs)
+        + 'printf("\\n");' for t, fs in fields.items())
+    src = tmp_path / "maplayout.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "eschalign_amd.h"\n'
+                   f'int main(void) {{ {body} return 0; }}\n')
+    exe = tmp_path / "maplayout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = {ln.split()[0]: [int(x) for x in ln.split()[1:]]
+           for ln in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines()}
+    hit = ea.HIT_DTYPE
+    assert got["esch_hit"] == [hit.itemsize] + [hit.fields[f][1] for f in
+                                                ["reference", "reverse", "template_reverse", "insert", "score"]]
+    rd = ea.READ_MAPPING_DTYPE
+    assert got["esch_read_mapping"] == [rd.itemsize] + [rd.fields[f][1] for f in fields["esch_read_mapping"]]
+    pd = ea.PAIR_MAPPING_DTYPE
+    assert got["esch_pair_mapping"] == [pd.itemsize] + [pd.fields[f][1] for f in fields["esch_pair_mapping"]]
+    assert (O.HIT_DTYPE, O.READ_DTYPE, O.PAIR_DTYPE) == (hit, rd, pd)
