@@ -13,8 +13,11 @@ needed) and the fused best / tie selection. value = reads per second.
 
 Also reported: the DP kernel's time from HIP events on its stream, GCUPS and
 the fraction of the int16 VALU lane-op roofline; the plain one-thread-per-
-pair int32 kernel on the same batch; and the reference's CPU kernel (the
-oracle's AVX-512/AVX2 restatement) on a bounded sample on the host cores.
+pair int32 kernel on the same batch; the reference's CPU kernel (the
+oracle's AVX-512/AVX2 restatement) on a bounded sample on the host cores;
+the winner tracebacks of the step's selection; the paired workload (2M
+pairs of 150-nt mates on a 500-reference panel) mapped end to end with the
+pair tier loop on the device, against the oracle on a sample; and serving.
 """
 import argparse
 import json
@@ -166,6 +169,97 @@ def trace_leg(gpu, batch, panel_codes, codes, off, lens, scoring, local, n_reads
     return out
 
 
+def pairs_leg(n_pairs, cpu, both_strands=False):
+    """The paired workload: 500 references of 70-95 nt; n_pairs pairs of
+    150-nt mates from inserts N(120, 30) that read into the TruSeq adapters
+    (0.5% substitutions, 0.05% indels); each mate less its adapter
+    (esch_adapter_trim_pairs); semi-global 2,-1,-10,-1, --max-insert 500.
+    One esch_map_pairs_concat call end to end from host buffers: staging and
+    upload, both orientations of every mate scored, the tier loop, the
+    tracebacks, the per-mate suboptimal scores, the Separate fallback and the
+    records back. On a bounded sample, the oracle on the host cores (panel
+    rows by the SIMD restatement of the reference's CPU kernel, then the
+    restated map_pairs_scored with its scalar tracebacks), equal on every
+    pair."""
+    import eschalign_amd as ea
+    from eschalign_amd import synth
+    from oracle import pyoracle as O
+
+    scoring = (2, -1, -10, -1)
+    t0 = time.perf_counter()
+    panel = synth.panel(seed=0xC0F4, n_refs=500, lo=70, hi=95)
+    codes, off, lens = synth.pair_arrays(panel, seed=0xC0F5, n_pairs=n_pairs)
+    tl = ea.adapter_trim_pairs(codes, off, lens)
+    log(f"[bench] paired workload: {n_pairs} pairs, mates {lens.mean():.1f} nt, "
+        f"{tl.mean():.1f} nt after adapter trimming ({time.perf_counter() - t0:.1f} s to generate)")
+    gpu = ea.GpuScorer(ea.Panel(panel), ea.Scoring(*scoring), ea.Mode.SEMIGLOBAL)
+    opts = dict(both_strands=both_strands, min_score=0, max_ties=None, max_insert=500)
+    gpu.map_pairs_concat(codes, off[:4000], tl[:4000], **opts)  # warm: code objects, buffers
+    t = time.perf_counter()
+    recs, hits, ops, st = gpu.map_pairs_concat(codes, off, tl, **opts)
+    wall = time.perf_counter() - t
+    cells = int(st["cells"])  # what was scored: two orientations per pair forward-only, four with both
+    proper = int(recs["proper"].sum())
+    out = {"pairs": n_pairs, "pairs_per_s": round(n_pairs / wall, 1), "wall_ms": round(wall * 1e3, 1),
+           "lib_ms": round(st["total_ms"], 1), "load_ms": round(st["load_ms"], 1),
+           "score_ms": round(st["score_ms"], 1), "select_ms": round(st["select_ms"], 1),
+           "trace_ms": round(st["trace_ms"], 1), "trace_wall_ms": round(st["trace_wall_ms"], 1),
+           "assemble_ms": round(st["assemble_ms"], 1), "rounds": int(st["rounds"]),
+           "traced": int(st["traced"]), "proper": proper, "separate": n_pairs - proper, "cells": cells,
+           "score_gcups": round(cells / (st["score_ms"] / 1e3) / 1e9, 1),
+           "workload": "500 refs 70-95 nt, 150-nt mates, inserts N(120, 30) into TruSeq adapters, "
+                       f"trimmed, semiglobal 2,-1,-10,-1, max_insert 500, "
+                       f"{'both template strands' if both_strands else 'forward template strand'}"}
+    if cpu:
+        sup = O.simd_supported()
+        lanes = 32 if sup & 2 else 16
+        threads = min(16, os.cpu_count() or 1)
+        prof = O.Profile(panel, scoring, lanes)
+
+        def run(m):
+            # rows on demand, as the reference's CPU path scores them
+            # (lib/mapper/pair.rs:195-207): forward template strand only, mate
+            # 1 as sequenced and mate 2 reverse-complemented; both orientations
+            # of a mate only when its pair falls back to Separate
+            mates = [codes[int(off[k]):int(off[k]) + int(tl[k])] for k in range(2 * m)]
+            pairs = [(mates[2 * k], mates[2 * k + 1]) for k in range(m)]
+            t = time.perf_counter()
+            nr = len(panel)
+            fwd = np.zeros((2 * m, nr), np.int32)
+            rev = np.zeros((2 * m, nr), np.int32)
+            if both_strands:
+                fwd[:] = prof.score_batch(*O.concat(mates), False, threads)
+                rev[:] = prof.score_batch(*O.concat([O.revcomp(x) for x in mates]), False, threads)
+            else:
+                fwd[0::2] = prof.score_batch(*O.concat(mates[0::2]), False, threads)
+                rev[1::2] = prof.score_batch(*O.concat([O.revcomp(x) for x in mates[1::2]]), False, threads)
+            res = O.map_pairs(pairs, panel, scoring, False, rows=(fwd, rev), threads=threads, **opts)
+            sep = [k for k in range(m) if res[k][0] == "separate"]
+            if sep and not both_strands:
+                sf = prof.score_batch(*O.concat([mates[2 * k + 1] for k in sep]), False, threads)
+                sr = prof.score_batch(*O.concat([O.revcomp(mates[2 * k]) for k in sep]), False, threads)
+                f2 = np.stack([r for q, k in enumerate(sep) for r in (fwd[2 * k], sf[q])])
+                r2 = np.stack([r for q, k in enumerate(sep) for r in (sr[q], rev[2 * k + 1])])
+                again = O.map_pairs([pairs[k] for k in sep], panel, scoring, False, rows=(f2, r2),
+                                    threads=threads, **opts)
+                for k, r in zip(sep, again):
+                    res[k] = r
+            return time.perf_counter() - t, res
+
+        dt, _ = run(1000)
+        m = int(min(n_pairs, max(1000, 1000 * 10.0 / max(dt, 1e-3))))
+        dt, res = run(m)
+        for k in range(m):
+            assert O.pair_form(recs[k], hits, ops) == res[k], f"device pair mapping != CPU on pair {k}"
+        out["cpu_baseline"] = {"pairs_per_s": round(m / dt, 1), "threads": threads, "kind": "port",
+                               "sample": f"first {m} pairs: the orientations the tiers need scored by the "
+                                         f"{'AVX-512BW' if lanes == 32 else 'AVX2'} restatement of the "
+                                         f"reference CPU kernel (the others only for Separate pairs), then "
+                                         f"map_pairs_scored restated with scalar tracebacks "
+                                         f"(lib/mapper/pair.rs:157-387), equal on every pair"}
+    return out
+
+
 def parse_lengths(text):
     """'lognormal:MEDIAN:SIGMA:LO:HI' or 'uniform:LO:HI' -> synth length spec."""
     kind, *vals = text.split(":")
@@ -287,6 +381,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-kernel comparison")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving run")
     ap.add_argument("--no-trace", action="store_true", help="skip the winner-traceback leg")
+    ap.add_argument("--no-pairs", action="store_true", help="skip the paired-mapping leg")
+    ap.add_argument("--pairs", type=int, default=2_000_000, help="pairs of the paired-mapping leg")
     ap.add_argument("--trace-reads", type=int, default=200_000,
                     help="reads whose selected winners the traceback leg traces")
     ap.add_argument("--lengths", default="lognormal:90:0.6:20:1000",
@@ -406,6 +502,11 @@ def main():
                           not args.no_cpu)
         log(f"[bench] winner tracebacks: {trace}")
 
+    pairs = None
+    if not args.no_pairs:
+        pairs = pairs_leg(args.pairs, not args.no_cpu)
+        log(f"[bench] paired mapping: {pairs}")
+
     serve = None
     if not args.no_serving:
         serve = serving(args.serve_rate, args.serve_seconds)
@@ -455,6 +556,7 @@ def main():
         "plain_kernel": plain,
         "cpu_baseline": cpu,
         "trace": trace,
+        "pairs": pairs,
         "serving": serve,
     }
     print(json.dumps(line), flush=True)

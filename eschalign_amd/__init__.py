@@ -67,7 +67,7 @@ EXPORTS = [
     "esch_align_pairs", "esch_align_last_ms", "esch_align_pairs_concat", "esch_map_reads",
     "esch_map_pairs", "esch_map_pairs_concat", "esch_mapped_len", "esch_mapped_reads",
     "esch_mapped_pairs", "esch_mapped_hits", "esch_mapped_ops", "esch_mapped_stats", "esch_mapped_free",
-    "esch_set_map_chunk",
+    "esch_set_map_chunk", "esch_adapter_trim_pairs",
 ]
 
 
@@ -86,8 +86,9 @@ class ServerStats(C.Structure):
 class MapStats(C.Structure):
     """esch_map_stats."""
     _fields_ = [("score_ms", C.c_double), ("select_ms", C.c_double), ("trace_ms", C.c_double),
+                ("trace_wall_ms", C.c_double), ("load_ms", C.c_double), ("assemble_ms", C.c_double),
                 ("total_ms", C.c_double), ("rounds", C.c_int32), ("reserved", C.c_int32),
-                ("traced", C.c_int64)]
+                ("traced", C.c_int64), ("cells", C.c_int64)]
 
 
 # Records of esch_map_reads / esch_map_pairs (esch_hit, esch_read_mapping,
@@ -220,6 +221,8 @@ def load_library():
     L.esch_mapped_free.argtypes = [vp]
     L.esch_mapped_free.restype = None
     L.esch_set_map_chunk.argtypes = [vp, sz]
+    L.esch_adapter_trim_pairs.argtypes = [vp, vp, vp, sz, C.c_char_p, sz, C.c_char_p, sz, sz, vp]
+    L.esch_adapter_trim_pairs.restype = None
     _lib = L
     return L
 
@@ -328,6 +331,14 @@ def _pair_mapping(p, hits, ops):
                       tuple(phs))
 
 
+def _copy_out(ptr, n: int, dt: np.dtype) -> np.ndarray:
+    """n records of dtype dt at a library pointer, copied once."""
+    out = np.empty(n, dt)
+    if n:
+        C.memmove(out.ctypes.data, ptr, n * dt.itemsize)
+    return out
+
+
 def _take_mapped(m, pairs: bool):
     """Copy an esch_mapped's records out and free it: (records, hits, ops,
     stats dict)."""
@@ -339,11 +350,9 @@ def _take_mapped(m, pairs: bool):
         nh, no = C.c_size_t(), C.c_size_t()
         hp = L.esch_mapped_hits(m, C.byref(nh))
         op = L.esch_mapped_ops(m, C.byref(no))
-        recs = np.frombuffer(C.string_at(rp, n * dt.itemsize), dt).copy() if n else np.zeros(0, dt)
-        hits = (np.frombuffer(C.string_at(hp, nh.value * HIT_DTYPE.itemsize), HIT_DTYPE).copy()
-                if nh.value else np.zeros(0, HIT_DTYPE))
-        ops = (np.frombuffer(C.string_at(op, no.value * 4), np.uint32).copy()
-               if no.value else np.zeros(0, np.uint32))
+        recs = _copy_out(rp, n, dt)
+        hits = _copy_out(hp, nh.value, HIT_DTYPE)
+        ops = _copy_out(op, no.value, np.dtype(np.uint32))
         st = MapStats()
         L.esch_mapped_stats(m, C.byref(st))
         stats = {f: getattr(st, f) for f, _ in MapStats._fields_ if f != "reserved"}
@@ -505,6 +514,23 @@ def adapter_find(read: bytes, adapter: bytes = TRUSEQ_R1, min_overlap: int = DEF
     """adapter::find (lib/adapter.rs:68-91), letters in."""
     return int(load_library().esch_adapter_find(bytes(read), len(read), bytes(adapter),
                                                  len(adapter), min_overlap))
+
+
+def adapter_trim_pairs(codes: np.ndarray, offs: np.ndarray, lens: np.ndarray, a1: bytes = TRUSEQ_R1,
+                       a2: bytes = TRUSEQ_R2, min_overlap: int = DEFAULT_MIN_OVERLAP) -> np.ndarray:
+    """esch_adapter_trim_pairs: mates interleaved as codes; each mate's
+    length less its detected 3' adapter (int32, one per mate)."""
+    L = load_library()
+    codes = np.ascontiguousarray(codes, np.uint8)
+    offs = np.ascontiguousarray(offs, np.uint64)
+    lens = np.ascontiguousarray(lens, np.int32)
+    if len(lens) % 2:
+        raise ValueError("mates come in pairs")
+    out = np.zeros(len(lens), np.int32)
+    if len(lens):
+        L.esch_adapter_trim_pairs(codes.ctypes.data, offs.ctypes.data, lens.ctypes.data, len(lens) // 2,
+                                  a1, len(a1), a2, len(a2), min_overlap, out.ctypes.data)
+    return out
 
 
 def adapter_detect_pair(r1: bytes, r2: bytes, a1: bytes = TRUSEQ_R1, a2: bytes = TRUSEQ_R2,

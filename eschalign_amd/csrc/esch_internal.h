@@ -150,10 +150,13 @@ hipError_t launch_pairs(const KArgs &a, const Launch &g, hipStream_t st);
 // listed entry. A list entry's candidates go to cands + cand_off[e] when
 // cand_off is set (room for all of them), else to cands + e * cap (the first
 // cap of them; the record still counts them all).
+// With half set (forward template strand only), pair p holds two rows:
+// 2p = mate 1 as sequenced, 2p + 1 = mate 2 reverse-complemented.
 struct TierArgs {
     const int32_t *out32;
     int32_t n_reads, n_refs;
-    int32_t strands;        // template strands: 1 forward, 2 both
+    int32_t half;
+    int32_t strands;        // template strands: 1 forward, 2 both (not with half)
     int32_t min_score;
     const int32_t *list;    // pair indices
     const int32_t *below;   // per entry: only sums below this (INT32_MAX: all)
@@ -167,7 +170,7 @@ constexpr int kTierRec = 3;
 
 struct SuboptArgs {
     const int32_t *out32;
-    int32_t n_reads, n_refs, strands;
+    int32_t n_reads, n_refs, half, strands;
     const int32_t *list;     // pair indices
     const int64_t *set_off;  // per entry: its tie set in `set`
     const int32_t *set_n;

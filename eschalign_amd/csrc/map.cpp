@@ -25,6 +25,7 @@
 #include <climits>
 #include <cstring>
 #include <memory>
+#include <thread>
 #include <vector>
 
 using namespace esch;
@@ -125,6 +126,7 @@ struct Ctx {
         t.al.assign(rows.size(), esch_alignment{});
         t.ops.clear();
         if (rows.empty()) return ESCH_OK;
+        const auto tw = Clock::now();
         esch_status st = align_core(
             s, rows.size(),
             [&](size_t k, const uint8_t *&p, size_t &len) {
@@ -133,6 +135,7 @@ struct Ctx {
             },
             refs.data(), t.al.data(), nullptr, 0, nullptr, b->d_codes, &t.ops);
         stats.trace_ms += s->trace_ms;
+        stats.trace_wall_ms += ms_since(tw);
         stats.traced += (int64_t)rows.size();
         return st;
     }
@@ -254,6 +257,7 @@ esch_status scorer_ready(esch_scorer *s) {
 // score them into resident int32 rows.
 esch_status load_and_score(esch_scorer *s, const Getter &get, size_t k0, size_t k1, int both, int pairs,
                            esch_batch **out, esch_map_stats &stats) {
+    const auto t0 = Clock::now();
     const size_t n = k1 - k0;
     std::vector<const uint8_t *> ptrs(n);
     std::vector<size_t> lens(n);
@@ -265,15 +269,19 @@ esch_status load_and_score(esch_scorer *s, const Getter &get, size_t k0, size_t 
     esch_batch *b = nullptr;
     if (esch_status st = take_batch(s, n, bases, &b)) return st;
     esch_status st = esch_batch_load(b, ptrs.data(), lens.data(), n, both, pairs);
+    stats.load_ms += ms_since(t0);
     if (!st) st = esch_batch_run(b, ESCH_RUN_SCORES32, 0, -1, 1);
     if (!st) st = esch_batch_wait(b);
     float total = 0.0f;
+    uint64_t cells = 0;
     if (!st) st = esch_batch_timing(b, nullptr, &total);
+    if (!st) st = esch_batch_stats(b, &cells, nullptr, nullptr);
     if (st) {
         give_batch(s, b);
         return st;
     }
     stats.score_ms += total;
+    stats.cells += (int64_t)cells;
     *out = b;
     return ESCH_OK;
 }
@@ -298,6 +306,7 @@ esch_status map_reads_core(esch_scorer *s, size_t n, const Getter &get, int both
         esch_status st = cx.map_rows(rows, both, min_score, max_ties, maps, pool);
         give_batch(s, b);
         if (st) return st;
+        const auto ta = Clock::now();
         for (size_t k = k0; k < k1; ++k) M->reads[k] = esch_read_mapping{};
         for (size_t e = 0; e < rows.size(); ++e) M->reads[k0 + (size_t)rows[e]] = maps[e];
         for (size_t k = k0; k < k1; ++k) {
@@ -305,6 +314,7 @@ esch_status map_reads_core(esch_scorer *s, size_t n, const Getter &get, int both
             if (m.n_hits) place_read(M.get(), pool, m);
             else m.hit_offset = (int64_t)M->hits.size();
         }
+        M->stats.assemble_ms += ms_since(ta);
     }
     M->stats.total_ms = ms_since(t0);
     *out = M.release();
@@ -321,6 +331,50 @@ bool pair_ok(const esch_alignment &a0, bool al0, const esch_alignment &a1, bool 
     return (int64_t)(hi - lo) <= max_insert;
 }
 
+// Reverse complements of mate 2 of pairs [p0, p1) into one host buffer
+// (lib/alphabet.rs:58-64), over up to 8 threads; false when a mate carries a
+// code outside 0..=4 (*bad = its pair).
+bool rc_mates2(const Getter &mate, size_t p0, size_t p1, std::vector<uint8_t> &buf, std::vector<size_t> &at,
+               size_t *bad) {
+    const size_t c = p1 - p0;
+    at.assign(c + 1, 0);
+    for (size_t i = 0; i < c; ++i) {
+        const uint8_t *p;
+        size_t len;
+        mate(2 * (p0 + i) + 1, p, len);
+        at[i + 1] = at[i] + len;
+    }
+    buf.resize(std::max<size_t>(at[c], 1));
+    const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nt = at[c] < ((size_t)8 << 20) ? 1 : std::min<size_t>(8, hw);
+    std::vector<size_t> first_bad(nt, SIZE_MAX);
+    auto work = [&](size_t t) {
+        for (size_t i = c * t / nt; i < c * (t + 1) / nt; ++i) {
+            const uint8_t *p;
+            size_t len;
+            mate(2 * (p0 + i) + 1, p, len);
+            uint8_t *d = buf.data() + at[i];
+            uint8_t mx = 0;
+            for (size_t j = 0; j < len; ++j) {
+                const uint8_t x = p[len - 1 - j];
+                mx = x > mx ? x : mx;
+                d[j] = x < 4 ? (uint8_t)(3 - x) : (uint8_t)4;
+            }
+            if (mx > 4 && first_bad[t] == SIZE_MAX) first_bad[t] = p0 + i;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < nt; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (auto &th : pool) th.join();
+    for (size_t t = 0; t < nt; ++t)
+        if (first_bad[t] != SIZE_MAX) {
+            *bad = first_bad[t];
+            return false;
+        }
+    return true;
+}
+
 esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, int both_strands, int32_t min_score,
                            int64_t max_ties, int64_t max_insert, esch_mapped **out) {
     const auto t0 = Clock::now();
@@ -329,20 +383,48 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
     M->pairs.resize(n_pairs);
     if (esch_status st = scorer_ready(s)) return st;
     const int strands = both_strands ? 2 : 1;
-    const size_t chunk = std::max<size_t>(1, chunk_rows(s) / 4);
+    // Forward template strand only: the tiers need mate 1 as sequenced and
+    // mate 2 reverse-complemented, nothing else, so only those two rows per
+    // pair are scored (half); the other orientations are scored later for
+    // the mates of Separate pairs alone (the reference scores rows on
+    // demand too, lib/mapper/pair.rs:195-207). Both strands: all four.
+    const bool half = !both_strands;
+    const size_t chunk = std::max<size_t>(1, chunk_rows(s) / (half ? 2 : 4));
+    std::vector<uint8_t> rc2;
+    std::vector<size_t> rc2_at;
     for (size_t p0 = 0; p0 < n_pairs; p0 += chunk) {
         const size_t p1 = std::min(n_pairs, p0 + chunk), c = p1 - p0;
         esch_batch *b = nullptr;
-        if (esch_status st = load_and_score(s, mate, 2 * p0, 2 * p1, 1, 1, &b, M->stats)) return st;
+        if (half) {
+            size_t bad = 0;
+            if (!rc_mates2(mate, p0, p1, rc2, rc2_at, &bad))
+                return fail(ESCH_INVALID_ARG, "pair %zu: mate 2 carries a code outside 0..=4", bad);
+            const Getter rows = [&](size_t k, const uint8_t *&p, size_t &len) {
+                if (k & 1) {
+                    p = rc2.data() + rc2_at[k >> 1];
+                    len = rc2_at[(k >> 1) + 1] - rc2_at[k >> 1];
+                } else {
+                    mate(2 * p0 + k, p, len);
+                }
+            };
+            if (esch_status st = load_and_score(s, rows, 0, 2 * c, 0, 0, &b, M->stats)) return st;
+        } else {
+            if (esch_status st = load_and_score(s, mate, 2 * p0, 2 * p1, 1, 1, &b, M->stats)) return st;
+        }
         struct Guard {
             esch_scorer *s;
             esch_batch *b;
-            ~Guard() { give_batch(s, b); }
-        } guard{s, b};
+            ~Guard() {
+                if (b) give_batch(s, b);
+            }
+        } guard{s, b}, guard_sep{s, nullptr};
         Ctx cx{s, b, *s->map, M->stats};
         MapCache &C = *s->map;
         hipStream_t st = b->stream;
         const int32_t nrd = (int32_t)b->n_reads;  // 2c
+        // the batch rows of pair i's mates on template strand t
+        auto row1 = [&](int32_t i, int32_t t) { return half ? 2 * i : (t ? nrd + 2 * i : 2 * i); };
+        auto row2 = [&](int32_t i, int32_t t) { return half ? 2 * i + 1 : (t ? 2 * i + 1 : nrd + 2 * i + 1); };
         std::vector<esch_pair_mapping> res(c);
         Pool pool;
         std::vector<int32_t> set_codes;           // tie sets of the proper pairs
@@ -365,7 +447,7 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
             if (esch_status e = upload(C.below, below, st)) return e;
             HIPC(C.recs.need(n * kTierRec * 4));
             HIPC(C.cands.need(n * kTierCap * 4));
-            TierArgs a{b->d_out32, nrd, (int32_t)s->n_refs, strands, min_score, C.list.as<int32_t>(),
+            TierArgs a{b->d_out32, nrd, (int32_t)s->n_refs, half ? 1 : 0, strands, min_score, C.list.as<int32_t>(),
                        C.below.as<int32_t>(), (int32_t)n, kTierCap, nullptr, C.recs.as<int32_t>(),
                        C.cands.as<int32_t>()};
             HIPC(launch_pair_tier(a, kResolveBlocks, st));
@@ -413,8 +495,8 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
                 const int32_t i = list[e];
                 for (int32_t j = 0; j < recs[e * kTierRec]; ++j) {
                     const int32_t cd = cand(e, j), t = cd & 1;
-                    jrows.push_back(t ? nrd + 2 * i : 2 * i);          // mate 1
-                    jrows.push_back(t ? 2 * i + 1 : nrd + 2 * i + 1);  // mate 2
+                    jrows.push_back(row1(i, t));
+                    jrows.push_back(row2(i, t));
                     jrefs.push_back(cd >> 1);
                     jrefs.push_back(cd >> 1);
                 }
@@ -459,8 +541,8 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
                     const esch_alignment &x = tr.al[k], &y = tr.al[k + 1];
                     const int32_t insert = std::max(x.ref_end, y.ref_end) - std::min(x.ref_start, y.ref_start);
                     const int32_t t = cd & 1;
-                    pool.add(tr, k, cd >> 1, t, t, insert);          // mate 1: reversed iff t
-                    pool.add(tr, k + 1, cd >> 1, !t, t, insert);     // mate 2: reversed iff !t
+                    pool.add(tr, k, cd >> 1, t, t, insert);       // mate 1: reversed iff t
+                    pool.add(tr, k + 1, cd >> 1, !t, t, insert);  // mate 2: reversed iff !t
                 }
                 m.n_hits = (int32_t)keep;
             }
@@ -483,7 +565,7 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
             if (esch_status e = upload(C.set_n, pn, st)) return e;
             if (esch_status e = upload(C.set, set_codes, st)) return e;
             HIPC(C.out.need(pl.size() * kSuboptRec * 4));
-            SuboptArgs a{b->d_out32, nrd, (int32_t)s->n_refs, strands, C.list.as<int32_t>(),
+            SuboptArgs a{b->d_out32, nrd, (int32_t)s->n_refs, half ? 1 : 0, strands, C.list.as<int32_t>(),
                          C.cand_off.as<int64_t>(), C.set_n.as<int32_t>(), C.set.as<int32_t>(), (int32_t)pl.size(),
                          C.out.as<int32_t>()};
             HIPC(launch_pair_subopt(a, kResolveBlocks, st));
@@ -498,19 +580,32 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
             }
         }
         // the Separate pairs: each non-empty mate as an unpaired read on both
-        // strands (lib/mapper/pair.rs:360-387)
+        // strands (lib/mapper/pair.rs:360-387); with half rows, those mates
+        // are scored again in both orientations in a batch of their own
         std::sort(sep.begin(), sep.end());
-        std::vector<int32_t> mrows;
+        std::vector<int32_t> mates_of;  // chunk-relative mate index 2i + m
         for (int32_t i : sep)
             for (int m = 0; m < 2; ++m)
-                if (b->h_qlen[2 * i + m]) mrows.push_back(2 * i + m);
+                if (b->h_qlen[2 * i + m]) mates_of.push_back(2 * i + m);
         std::vector<esch_read_mapping> mm;
-        if (esch_status e = cx.map_rows(mrows, 1, min_score, max_ties, mm, pool)) return e;
-        for (size_t q = 0; q < mrows.size(); ++q) {
-            const int32_t x = mrows[q];
+        if (half && !mates_of.empty()) {
+            const Getter sm = [&](size_t k, const uint8_t *&p, size_t &len) { mate(2 * p0 + (size_t)mates_of[k], p, len); };
+            esch_batch *bs = nullptr;
+            if (esch_status e = load_and_score(s, sm, 0, mates_of.size(), 1, 0, &bs, M->stats)) return e;
+            guard_sep.b = bs;
+            Ctx cs{s, bs, *s->map, M->stats};
+            std::vector<int32_t> rows(mates_of.size());
+            for (size_t q = 0; q < rows.size(); ++q) rows[q] = (int32_t)q;
+            if (esch_status e = cs.map_rows(rows, 1, min_score, max_ties, mm, pool)) return e;
+        } else if (esch_status e = cx.map_rows(mates_of, 1, min_score, max_ties, mm, pool)) {
+            return e;
+        }
+        for (size_t q = 0; q < mates_of.size(); ++q) {
+            const int32_t x = mates_of[q];
             res[(size_t)(x / 2)].separate[x % 2] = mm[q];
         }
         // into the result, in pair order
+        const auto ta = Clock::now();
         for (size_t i = 0; i < c; ++i) {
             esch_pair_mapping &m = res[i];
             if (m.proper) {
@@ -532,6 +627,7 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
             }
             M->pairs[p0 + i] = m;
         }
+        M->stats.assemble_ms += ms_since(ta);
     }
     M->stats.total_ms = ms_since(t0);
     *out = M.release();

@@ -28,7 +28,13 @@ __device__ __forceinline__ int wmax(int v) {
 
 struct PairRows {
     const int32_t *m1f, *m2f, *m1r, *m2r;
-    __device__ PairRows(const int32_t *out32, int n_reads, int nr, int p) {
+    __device__ PairRows(const int32_t *out32, int n_reads, int nr, int p, int half) {
+        if (half) {  // the forward template strand's two rows only
+            m1f = out32 + (size_t)(2 * p) * nr;
+            m2r = out32 + (size_t)(2 * p + 1) * nr;
+            m1r = m2f = nullptr;
+            return;
+        }
         m1f = out32 + (size_t)(2 * p) * nr;
         m2f = out32 + (size_t)(2 * p + 1) * nr;
         m1r = out32 + (size_t)(n_reads + 2 * p) * nr;
@@ -46,7 +52,7 @@ __global__ __launch_bounds__(256) void k_pair_tier(TierArgs a) {
     const int waves = (gridDim.x * blockDim.x) >> 6;
     const int nr = a.n_refs, strands = a.strands, nc = nr * strands;
     for (int e = wave; e < a.n_list; e += waves) {
-        const PairRows R(a.out32, a.n_reads, nr, a.list[e]);
+        const PairRows R(a.out32, a.n_reads, nr, a.list[e], a.half);
         const int32_t lim = a.below[e];
         // candidate k: reference k / strands on template strand k % strands
         auto sum_of = [&](int k, bool &ok) -> int32_t {
@@ -107,7 +113,7 @@ __global__ __launch_bounds__(256) void k_pair_subopt(SuboptArgs a) {
     const int waves = (gridDim.x * blockDim.x) >> 6;
     const int nr = a.n_refs, strands = a.strands, nc = nr * strands;
     for (int e = wave; e < a.n_list; e += waves) {
-        const PairRows R(a.out32, a.n_reads, nr, a.list[e]);
+        const PairRows R(a.out32, a.n_reads, nr, a.list[e], a.half);
         const int32_t *set = a.set + a.set_off[e];
         const int ns = a.set_n[e];
         int32_t s1 = INT32_MIN, s2 = INT32_MIN;

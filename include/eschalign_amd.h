@@ -206,6 +206,14 @@ void esch_adapter_detect_pair(const uint8_t *r1, size_t l1, const uint8_t *r2, s
                               const uint8_t *a1, size_t la1, const uint8_t *a2, size_t la2,
                               size_t min_overlap, size_t out[2]);
 
+/* The same pre-step over a batch of pairs held as codes, mates interleaved
+ * (mate m of pair p at codes[offs[2p + m]], lens[2p + m] bases); adapters are
+ * letters. out[k] = lens[k] less mate k's detected adapter: the queries the
+ * CLI aligns (crates/eschalign-cli/src/align.rs:1642-1662). */
+void esch_adapter_trim_pairs(const uint8_t *codes, const uint64_t *offs, const int32_t *lens, size_t n_pairs,
+                             const uint8_t *a1, size_t la1, const uint8_t *a2, size_t la2, size_t min_overlap,
+                             int32_t *out);
+
 /* ---- winner tracebacks (lib/simd/mod.rs:405-499, lib/scalar.rs:164-364) ----
  * The alignment of read queries[k] (codes, lens[k] bases) against panel
  * reference refs[k], for every k: the score, the aligned spans (0-based,
@@ -305,13 +313,19 @@ const esch_pair_mapping *esch_mapped_pairs(const esch_mapped *m); /* NULL for re
 const esch_hit *esch_mapped_hits(const esch_mapped *m, size_t *n);
 const uint32_t *esch_mapped_ops(const esch_mapped *m, size_t *n);
 /* Where the time of the call went (diagnostics; no counterpart in the
- * reference): ms of scoring (device events), selection kernels, traceback
- * kernels, and the whole call; tier rounds; pairs traced. */
+ * reference), in ms: */
 typedef struct {
-    double score_ms, select_ms, trace_ms, total_ms;
-    int32_t rounds;
+    double score_ms;      /* device scoring pipelines (HIP events) */
+    double select_ms;     /* tier / suboptimal / selection kernels with their copies */
+    double trace_ms;      /* device traceback kernels (HIP events) */
+    double trace_wall_ms; /* the traceback calls, host staging and copies included */
+    double load_ms;       /* staging reads into device batches */
+    double assemble_ms;   /* records placed in input order */
+    double total_ms;      /* the whole call */
+    int32_t rounds;       /* tier rounds */
     int32_t reserved;
-    int64_t traced;
+    int64_t traced;       /* (read, reference) tracebacks */
+    int64_t cells;        /* DP cells scored (read length x panel bases) */
 } esch_map_stats;
 void esch_mapped_stats(const esch_mapped *m, esch_map_stats *out);
 void esch_mapped_free(esch_mapped *m);

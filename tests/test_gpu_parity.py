@@ -116,7 +116,7 @@ def test_batch_load_refuses_codes_outside_the_alphabet():
     small batch (one staging thread) and in the middle of a 12 MB batch (the
     multi-threaded gather); a clean reload of the same batch then scores."""
     rng = np.random.default_rng(7)
-    panel = ea.Panel([bytes(rng.choice(list(b"ACGT"), 150)) for _ in range(8)])
+    panel = ea.Panel([bytes(rng.choice(list(b"ACGT"), 150).astype(np.uint8)) for _ in range(8)])
     gpu = ea.GpuScorer(panel, ea.Scoring(), ea.Mode.LOCAL)
     for n, length in ((20, 50), (80_000, 150)):
         reads = [rng.integers(0, 5, length).astype(np.uint8) for _ in range(n)]

@@ -143,7 +143,7 @@ def test_adapter_equals_oracle_random():
     rng = np.random.default_rng(5)
     for _ in range(2000):
         n = int(rng.integers(0, 60))
-        read = bytes(rng.choice(list(b"ACGTN"), n, p=[0.24, 0.24, 0.24, 0.24, 0.04]))
+        read = bytes(rng.choice(list(b"ACGTN"), n, p=[0.24, 0.24, 0.24, 0.24, 0.04]).astype(np.uint8))
         if n and rng.random() < 0.7:
             k = int(rng.integers(0, min(n, len(R1)) + 1))
             tail = bytearray(R1[:k])
@@ -199,3 +199,22 @@ def test_mapping_record_layouts_match_the_header(tmp_path):
     pd = ea.PAIR_MAPPING_DTYPE
     assert got["esch_pair_mapping"] == [pd.itemsize] + [pd.fields[f][1] for f in fields["esch_pair_mapping"]]
     assert (O.HIT_DTYPE, O.READ_DTYPE, O.PAIR_DTYPE) == (hit, rd, pd)
+
+
+def test_adapter_trim_pairs_equals_detect_pair():
+    """The batched pre-step on codes gives, for every pair of the paired
+    workload's generator (inserts N(120, 30) reading into the adapters) and of
+    the paired fixture, the oracle's detect_pair (lib/adapter.rs:95-110)."""
+    from eschalign_amd import synth
+    panel = synth.panel(seed=0xC0F4, n_refs=50, lo=70, hi=95)
+    codes, off, lens = synth.pair_arrays(panel, seed=0xAD, n_pairs=6000)
+    got = ea.adapter_trim_pairs(codes, off, lens)
+    letters = np.frombuffer(b"ACGTN", np.uint8)[codes].tobytes()
+    trimmed = 0
+    for p in range(len(lens) // 2):
+        a = letters[int(off[2 * p]):int(off[2 * p]) + int(lens[2 * p])]
+        b = letters[int(off[2 * p + 1]):int(off[2 * p + 1]) + int(lens[2 * p + 1])]
+        d = O.adapter_detect_pair(a, b, R1, R2, 5)
+        assert [int(got[2 * p]), int(got[2 * p + 1])] == [len(a) - d[0], len(b) - d[1]], p
+        trimmed += d[0] > 0
+    assert trimmed > 1000
