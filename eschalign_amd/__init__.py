@@ -67,7 +67,7 @@ EXPORTS = [
     "esch_align_pairs", "esch_align_last_ms", "esch_align_pairs_concat", "esch_map_reads",
     "esch_map_pairs", "esch_map_pairs_concat", "esch_mapped_len", "esch_mapped_reads",
     "esch_mapped_pairs", "esch_mapped_hits", "esch_mapped_ops", "esch_mapped_stats", "esch_mapped_free",
-    "esch_set_map_chunk", "esch_adapter_trim_pairs",
+    "esch_set_map_chunk", "esch_adapter_trim_pairs", "esch_md_nm", "esch_cigar_string",
 ]
 
 
@@ -223,6 +223,10 @@ def load_library():
     L.esch_set_map_chunk.argtypes = [vp, sz]
     L.esch_adapter_trim_pairs.argtypes = [vp, vp, vp, sz, C.c_char_p, sz, C.c_char_p, sz, sz, vp]
     L.esch_adapter_trim_pairs.restype = None
+    L.esch_md_nm.argtypes = [C.c_char_p, sz, C.c_char_p, sz, sz, sz, vp, sz, C.c_char_p, sz, C.POINTER(C.c_uint32)]
+    L.esch_md_nm.restype = sz
+    L.esch_cigar_string.argtypes = [vp, sz, sz, sz, sz, C.c_char_p, sz]
+    L.esch_cigar_string.restype = sz
     _lib = L
     return L
 
@@ -251,13 +255,27 @@ class Alignment:
         return "".join(f"{n}{op}" for op, n in self.ops)
 
 
+def reverse_complement(letters: bytes) -> bytes:
+    """alphabet::reverse_complement on letters (lib/alphabet.rs:69-80):
+    reversed, A/C/G/T(U) complemented upper-case, anything else N."""
+    return bytes(reversed(letters.upper())).translate(_RC_LETTERS)
+
+
 @dataclass(frozen=True)
 class Hit:
-    """lib/mapper.rs:43-56: one placement of a read (md / nm, a SAM concern
-    computed from the letters, are not part of it here)."""
+    """lib/mapper.rs:43-56: one placement of a read. MD / NM come from the
+    letters (md_nm below), as the reference computes them when it builds
+    the record (lib/mapper.rs:520-532)."""
     reference: int
     reverse: bool
     alignment: Alignment
+
+    def md_nm(self, read: bytes, reference: bytes):
+        """(MD, NM) of this hit for the read's letters as sequenced and the
+        reference's letters as in the FASTA."""
+        letters = reverse_complement(read) if self.reverse else read
+        a = self.alignment
+        return md_nm(letters, reference, a.query_start, a.ref_start, a.ops)
 
 
 @dataclass(frozen=True)
@@ -361,6 +379,39 @@ def _take_mapped(m, pairs: bool):
     return recs, hits, ops, stats
 
 
+def _runs(ops) -> np.ndarray:
+    return np.array([("MID".index(o) << 30) | int(n) for o, n in ops] or [0], np.uint32)
+
+
+def md_nm(read: bytes, reference: bytes, query_start: int, ref_start: int, ops):
+    """sam::md_nm (lib/sam.rs:76-124): (MD, NM) as samtools calmd writes them
+    for `ops` ([(op, len)], op in "MID") aligning read letters (as aligned)
+    against reference letters from query_start / ref_start."""
+    L = load_library()
+    r = _runs(ops)
+    nm = C.c_uint32()
+    need = L.esch_md_nm(read, len(read), reference, len(reference), query_start, ref_start, r.ctypes.data,
+                        len(ops), None, 0, C.byref(nm))
+    if need == C.c_size_t(-1).value:
+        raise ValueError("the operations run past the read or the reference")
+    buf = C.create_string_buffer(need + 1)
+    L.esch_md_nm(read, len(read), reference, len(reference), query_start, ref_start, r.ctypes.data, len(ops),
+                 buf, need + 1, C.byref(nm))
+    return buf.value.decode(), nm.value
+
+
+def cigar_string(ops, query_len: int, query_start: int, query_end: int) -> str:
+    """sam::cigar_string (lib/sam.rs:32-52): soft clips around the ops."""
+    L = load_library()
+    r = _runs(ops)
+    need = L.esch_cigar_string(r.ctypes.data, len(ops), query_len, query_start, query_end, None, 0)
+    if need == C.c_size_t(-1).value:
+        raise ValueError("inconsistent span")
+    buf = C.create_string_buffer(need + 1)
+    L.esch_cigar_string(r.ctypes.data, len(ops), query_len, query_start, query_end, buf, need + 1)
+    return buf.value.decode()
+
+
 def valu_peak16(ordinal: int = 0) -> dict:
     """The measured packed-int16 VALU peak (lane-ops/s) with the CU count and
     the reported peak clock of device `ordinal`."""
@@ -376,6 +427,9 @@ def device_visible() -> bool:
 
 # ---------------------------------------------------------------- alphabet
 
+_RC_LETTERS = bytes(
+    {ord("A"): ord("T"), ord("C"): ord("G"), ord("G"): ord("C"), ord("T"): ord("A"), ord("U"): ord("A")}.get(b, ord("N"))
+    for b in range(256))
 _TABLE = np.full(256, 4, dtype=np.uint8)
 for _ch, _c in ((b"A", 0), (b"C", 1), (b"G", 2), (b"T", 3), (b"U", 3)):
     _TABLE[_ch[0]] = _c

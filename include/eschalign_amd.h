@@ -334,6 +334,20 @@ void esch_mapped_free(esch_mapped *m);
  * not depend on it (no counterpart in the reference). */
 esch_status esch_set_map_chunk(esch_scorer *s, size_t reads);
 
+/* ---- SAM fields (lib/sam.rs:32-124; host code) ----
+ * sam::md_nm: MD and NM as samtools calmd writes them for an alignment of
+ * read letters (in the orientation aligned) against reference letters (as
+ * in the FASTA), ops beginning at query_start / ref_start. MD is written
+ * NUL-terminated into md (truncated to md_cap - 1 characters); returns its
+ * full length, or (size_t)-1 when the ops run past either sequence. */
+size_t esch_md_nm(const uint8_t *read, size_t n, const uint8_t *reference, size_t m, size_t query_start,
+                  size_t ref_start, const uint32_t *ops, size_t n_ops, char *md, size_t md_cap, uint32_t *nm);
+/* sam::cigar_string: the CIGAR of an aligned span inside a read of
+ * query_len bases, soft clips before query_start and after query_end;
+ * returns the full length ((size_t)-1 on inconsistent spans). */
+size_t esch_cigar_string(const uint32_t *ops, size_t n_ops, size_t query_len, size_t query_start, size_t query_end,
+                         char *out, size_t cap);
+
 /* Kernel variants of the batched DP, and the autotuner that picks one by
  * timing each on a sample of the caller's reads (no counterpart in the
  * reference; the scores are identical whichever variant runs):

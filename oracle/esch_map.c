@@ -6,7 +6,8 @@
  *   pair_ok / span              lib/mapper/pair.rs:392-420
  * on panel scores the caller computed (the `prescored` rows of both), with
  * eo_select / eo_pair_cands for the selection and eo_align for every
- * traceback. MD / NM strings (lib/sam.rs) are not part of the records.
+ * traceback. MD / NM strings are not part of the records; eo_md_nm and
+ * eo_cigar_string (lib/sam.rs:32-124) at the end of this file give them.
  *
  * Pairs are independent of one another (lib/mapper/pair.rs:558-573 checks
  * that batching does not change a result), so each is mapped on its own,
@@ -15,6 +16,7 @@
 #include "esch_oracle.h"
 
 #include <pthread.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -409,4 +411,74 @@ void eo_mapped_free(eo_mapped *m) {
     free(m->hits);
     free(m->ops);
     free(m);
+}
+
+/* ---- SAM fields: sam::md_nm and sam::cigar_string (lib/sam.rs:32-124) ---- */
+
+static unsigned char eo_upper(unsigned char b) { return (b >= 'a' && b <= 'z') ? (unsigned char)(b - 32) : b; }
+
+/* calmd's match (lib/sam.rs:57-69): both letters A/C/G/T, U as T, case
+ * ignored, and equal. */
+static int eo_calmd_match(unsigned char q, unsigned char r) {
+    q = eo_upper(q);
+    r = eo_upper(r);
+    if (q == 'U') q = 'T';
+    if (r == 'U') r = 'T';
+    return q == r && (r == 'A' || r == 'C' || r == 'G' || r == 'T');
+}
+
+static void eo_put_num(char *md, size_t *k, unsigned v) { *k += (size_t)sprintf(md + *k, "%u", v); }
+
+int eo_md_nm(const uint8_t *read, size_t n, const uint8_t *ref, size_t m, size_t qs, size_t rs,
+             const uint32_t *ops, size_t n_ops, char *md, size_t md_cap, uint32_t *nm) {
+    /* worst case: a number and a letter per aligned base, plus the deletions */
+    size_t need = 16;
+    for (size_t k = 0; k < n_ops; k++) need += 12 * (size_t)(ops[k] & 0x3FFFFFFFu) + 12;
+    if (md_cap < need) return -1;
+    size_t i = qs, j = rs, w = 0;
+    unsigned run = 0, edits = 0;
+    for (size_t k = 0; k < n_ops; k++) {
+        const unsigned op = ops[k] >> 30, len = ops[k] & 0x3FFFFFFFu;
+        if (op == 0) {
+            for (unsigned t = 0; t < len; t++, i++, j++) {
+                if (i >= n || j >= m) return -1;
+                if (eo_calmd_match(read[i], ref[j])) {
+                    run++;
+                } else {
+                    eo_put_num(md, &w, run);
+                    md[w++] = (char)eo_upper(ref[j]);
+                    run = 0;
+                    edits++;
+                }
+            }
+        } else if (op == 1) {
+            i += len;
+            edits += len;
+        } else {
+            eo_put_num(md, &w, run);
+            md[w++] = '^';
+            for (unsigned t = 0; t < len; t++) {
+                if (j + t >= m) return -1;
+                md[w++] = (char)eo_upper(ref[j + t]);
+            }
+            run = 0;
+            j += len;
+            edits += len;
+        }
+    }
+    eo_put_num(md, &w, run);
+    md[w] = 0;
+    *nm = edits;
+    return (int)w;
+}
+
+int eo_cigar_string(const uint32_t *ops, size_t n_ops, size_t query_len, size_t qs, size_t qe, char *out,
+                    size_t cap) {
+    if (cap < 24 * (n_ops + 2)) return -1;
+    size_t w = 0;
+    if (qs > 0) w += (size_t)sprintf(out + w, "%zuS", qs);
+    for (size_t k = 0; k < n_ops; k++) w += (size_t)sprintf(out + w, "%u%c", ops[k] & 0x3FFFFFFFu, "MID"[ops[k] >> 30]);
+    if (query_len > qe) w += (size_t)sprintf(out + w, "%zuS", query_len - qe);
+    out[w] = 0;
+    return (int)w;
 }

@@ -35,6 +35,10 @@
  *                     outcomes crates/eschalign-cli/tests/align_paired.rs
  *                     states for the nine pairs of the paired fixture
  *                     (positions, inserts, proper or not, ties).
+ *   eo_md_nm          the 21 samtools 1.23.1 calmd results of
+ *                     tests/golden/calmd_golden.json (crates/eschalign/tests/
+ *                     sam_fields.rs) and lib/sam.rs:150-156.
+ *   eo_cigar_string   the same golden CIGARs round-tripped, lib/sam.rs:143-148.
  *   eo_adapter_*      lib/adapter.rs:116-197 known-answer tests.
  *   eo_encode etc.    lib/alphabet.rs:86-105 known answers.
  */
@@ -163,6 +167,15 @@ const eo_pair_mapping *eo_mapped_pairs(const eo_mapped *m);
 const eo_hit *eo_mapped_hits(const eo_mapped *m, size_t *n);
 const uint32_t *eo_mapped_ops(const eo_mapped *m, size_t *n);
 void eo_mapped_free(eo_mapped *m);
+
+/* sam::md_nm (lib/sam.rs:76-124) on letters: MD into md (NUL-terminated),
+ * NM into *nm; returns MD's length, -1 if md_cap is short or the ops run
+ * past a sequence. sam::cigar_string (lib/sam.rs:32-52): soft clips around
+ * the ops; returns the length, -1 if cap is short. */
+int eo_md_nm(const uint8_t *read, size_t n, const uint8_t *ref, size_t m, size_t qs, size_t rs,
+             const uint32_t *ops, size_t n_ops, char *md, size_t md_cap, uint32_t *nm);
+int eo_cigar_string(const uint32_t *ops, size_t n_ops, size_t query_len, size_t qs, size_t qe, char *out,
+                    size_t cap);
 
 /* lib/adapter.rs:68-91 — adapter bases at the 3' end of a read (letters). */
 size_t eo_adapter_find(const uint8_t *read, size_t n, const uint8_t *adapter,

@@ -72,6 +72,12 @@ def lib():
             getattr(L, f).argtypes = [C.c_void_p, C.POINTER(C.c_size_t)]
             getattr(L, f).restype = C.c_void_p
         L.eo_mapped_free.argtypes = [C.c_void_p]
+        L.eo_md_nm.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t, C.c_size_t, C.c_size_t,
+                               C.c_void_p, C.c_size_t, C.c_char_p, C.c_size_t, C.POINTER(C.c_uint32)]
+        L.eo_md_nm.restype = C.c_int
+        L.eo_cigar_string.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.c_char_p,
+                                      C.c_size_t]
+        L.eo_cigar_string.restype = C.c_int
         L.eo_adapter_find.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t,
                                       C.c_size_t]
         L.eo_adapter_find.restype = C.c_size_t
@@ -357,6 +363,32 @@ def map_pairs(pairs, refs, scoring, local: bool, min_score=0, max_ties=None, max
     if raw:
         return recs, hits, ops
     return [pair_form(p, hits, ops) for p in recs]
+
+
+def _runs(ops):
+    """[(op char, len), ...] -> uint32 runs (op << 30 | len)."""
+    return np.array([(OPS.index(o) << 30) | n for o, n in ops] or [0], np.uint32), len(ops)
+
+
+def md_nm(read: bytes, ref: bytes, query_start: int, ref_start: int, ops):
+    """sam::md_nm (lib/sam.rs:76-124) on letters: (MD, NM)."""
+    r, k = _runs(ops)
+    cap = 16 + sum(12 * n + 12 for _, n in ops)
+    buf = C.create_string_buffer(cap)
+    nm = C.c_uint32()
+    w = lib().eo_md_nm(read, len(read), ref, len(ref), query_start, ref_start, r.ctypes.data, k, buf, cap,
+                       C.byref(nm))
+    assert w >= 0
+    return buf.value.decode(), nm.value
+
+
+def cigar_string(ops, query_len: int, query_start: int, query_end: int) -> str:
+    """sam::cigar_string (lib/sam.rs:32-52)."""
+    r, k = _runs(ops)
+    cap = 24 * (k + 2)
+    buf = C.create_string_buffer(cap)
+    assert lib().eo_cigar_string(r.ctypes.data, k, query_len, query_start, query_end, buf, cap) >= 0
+    return buf.value.decode()
 
 
 def adapter_find(read: bytes, adapter: bytes, min_overlap: int) -> int:
