@@ -383,6 +383,10 @@ def main():
     ap.add_argument("--no-trace", action="store_true", help="skip the winner-traceback leg")
     ap.add_argument("--no-pairs", action="store_true", help="skip the paired-mapping leg")
     ap.add_argument("--pairs", type=int, default=2_000_000, help="pairs of the paired-mapping leg")
+    ap.add_argument("--pairs-only", action="store_true",
+                    help="only run the paired-mapping leg and print its record (profiling)")
+    ap.add_argument("--pairs-both", action="store_true",
+                    help="paired-mapping leg: consider both template strands")
     ap.add_argument("--trace-reads", type=int, default=200_000,
                     help="reads whose selected winners the traceback leg traces")
     ap.add_argument("--lengths", default="lognormal:90:0.6:20:1000",
@@ -405,6 +409,10 @@ def main():
         print(f"bench.py measures one GPU only (the scoring path shards by reads with no "
               f"collective); --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+
+    if args.pairs_only:
+        print(json.dumps(pairs_leg(args.pairs, not args.no_cpu, args.pairs_both)), flush=True)
+        return
 
     import eschalign_amd as ea
     from eschalign_amd import synth
@@ -504,7 +512,7 @@ def main():
 
     pairs = None
     if not args.no_pairs:
-        pairs = pairs_leg(args.pairs, not args.no_cpu)
+        pairs = pairs_leg(args.pairs, not args.no_cpu, args.pairs_both)
         log(f"[bench] paired mapping: {pairs}")
 
     serve = None

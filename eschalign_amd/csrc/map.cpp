@@ -32,8 +32,12 @@ using namespace esch;
 
 namespace {
 
-constexpr int kTierCap = 16;    // candidates per pair returned inline per round
-constexpr int kSelCap = 16;     // ties per read returned inline
+// Candidates per pair (ties per read) returned inline per selection round;
+// a wider tier or tie list takes a second pass with room for all of it.
+// Most tiers hold one candidate, so a small inline width keeps the copy
+// back to the host small.
+constexpr int kTierCap = 4;
+constexpr int kSelCap = 4;
 constexpr int kResolveBlocks = 4096;
 constexpr size_t kChunkRowBytes = (size_t)4 << 30;  // resident int32 rows per chunk
 
@@ -307,6 +311,8 @@ esch_status map_reads_core(esch_scorer *s, size_t n, const Getter &get, int both
         give_batch(s, b);
         if (st) return st;
         const auto ta = Clock::now();
+        M->hits.reserve(M->hits.size() + pool.hits.size());
+        M->ops.reserve(M->ops.size() + pool.ops.size());
         for (size_t k = k0; k < k1; ++k) M->reads[k] = esch_read_mapping{};
         for (size_t e = 0; e < rows.size(); ++e) M->reads[k0 + (size_t)rows[e]] = maps[e];
         for (size_t k = k0; k < k1; ++k) {
@@ -490,6 +496,8 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
             // both mates of every candidate of every listed tier
             std::vector<int32_t> jrows, jrefs;
             std::vector<size_t> first(n, 0);
+            jrows.reserve(2 * n);
+            jrefs.reserve(2 * n);
             for (size_t e = 0; e < n; ++e) {
                 first[e] = jrows.size();
                 const int32_t i = list[e];
@@ -503,7 +511,9 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
             }
             Traced tr;
             if (esch_status e = cx.trace(jrows, jrefs, tr)) return e;
-            std::vector<int32_t> nl, nb;
+            std::vector<int32_t> nl, nb, ok;
+            pool.hits.reserve(pool.hits.size() + jrows.size());
+            pool.ops.reserve(pool.ops.size() + tr.ops.size());
             for (size_t e = 0; e < n; ++e) {
                 const int32_t i = list[e];
                 const int32_t nt = recs[e * kTierRec];
@@ -511,7 +521,7 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
                     sep.push_back(i);  // no tier left
                     continue;
                 }
-                std::vector<int32_t> ok;
+                ok.clear();
                 for (int32_t j = 0; j < nt; ++j) {
                     const size_t k = first[e] + 2 * (size_t)j;
                     if (pair_ok(tr.al[k], tr.aligned(k), tr.al[k + 1], tr.aligned(k + 1), cand(e, j) & 1, max_insert))
@@ -606,6 +616,8 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
         }
         // into the result, in pair order
         const auto ta = Clock::now();
+        M->hits.reserve(M->hits.size() + pool.hits.size());
+        M->ops.reserve(M->ops.size() + pool.ops.size());
         for (size_t i = 0; i < c; ++i) {
             esch_pair_mapping &m = res[i];
             if (m.proper) {

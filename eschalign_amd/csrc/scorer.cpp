@@ -1088,15 +1088,31 @@ esch_status esch::align_core(esch_scorer *s, size_t n,
     uint8_t *const hcodes = T.h_codes.as<uint8_t>();
     std::vector<uint64_t> hqoff(np), ops_off(n, 0);
     std::vector<int32_t> hqlen(np), href(np), hslot(np);
-    size_t at = 0;
-    for (size_t i = 0; i < np; ++i) {
-        const int32_t k = order[i];
-        // resident reads are addressed where they lie; staged ones packed
-        hqoff[i] = dev_codes ? (uint64_t)(ptr[k] - dev_codes) : at;
-        hqlen[i] = (int32_t)lens[k];
-        href[i] = refs[k];
-        hslot[i] = k;
-        at += lens[k];
+    {
+        // per sorted pair: its read, length, reference and caller index
+        // (random reads of the caller's arrays, so spread over threads)
+        auto fill = [&](size_t i0, size_t i1) {
+            for (size_t i = i0; i < i1; ++i) {
+                const int32_t k = order[i];
+                hqoff[i] = dev_codes ? (uint64_t)(ptr[k] - dev_codes) : 0;
+                hqlen[i] = (int32_t)lens[k];
+                href[i] = refs[k];
+                hslot[i] = k;
+            }
+        };
+        const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+        const size_t nt = np < ((size_t)1 << 18) ? 1 : std::min<size_t>(8, hw);
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < nt; ++t) pool.emplace_back(fill, np * t / nt, np * (t + 1) / nt);
+        fill(0, np / nt);
+        for (auto &th : pool) th.join();
+        if (!dev_codes) {  // staged reads are packed in sorted order
+            size_t at = 0;
+            for (size_t i = 0; i < np; ++i) {
+                hqoff[i] = at;
+                at += (size_t)hqlen[i];
+            }
+        }
     }
     // the gather into staging and the code check on up to 8 host threads
     // (resident reads were checked when they were loaded)
