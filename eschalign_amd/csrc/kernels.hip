@@ -1035,14 +1035,47 @@ __device__ __forceinline__ bool fits_i16_dev(int64_t n, int64_t L, const Scoring
     return k * M + abs(sc.o) + abs(sc.e) < 15000 && L < 15000;
 }
 
-__global__ void k_classify(KArgs a) {
+// A list append with one atomic per wave: each lane that wants a slot gets
+// the list's old length plus its rank among those lanes (slots come in no
+// particular order; nothing downstream depends on it). Every active lane of
+// the wave must call it.
+__device__ __forceinline__ int wave_append(int32_t *counter, bool want) {
+    const uint64_t m = __ballot(want);
+    if (!m) return 0;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader, 64);
+    return base + __popcll(m & ((1ull << lane) - 1ull));
+}
+
+// Length bins shorter than this are counted per block in LDS and added to
+// the global bins once per block: reads of one length are common (mates
+// trimmed to similar lengths), and same-address global atomics serialise.
+constexpr int kLdsBins = 4096;
+
+// A block's contiguous share of n items (coalesced, and the same for every
+// pass of a kernel over them).
+__device__ __forceinline__ void block_range(int n, int &q0, int &q1) {
+    const int per = (n + (int)gridDim.x - 1) / (int)gridDim.x;
+    q0 = min(n, (int)blockIdx.x * per);
+    q1 = min(n, q0 + per);
+}
+
+__global__ __launch_bounds__(256) void k_classify(KArgs a) {
+    __shared__ int32_t hist[kLdsBins];
+    for (int t = threadIdx.x; t < kLdsBins; t += blockDim.x) hist[t] = 0;
+    __syncthreads();
     const int nq = a.desc->n_queries;
     const int dropin = a.desc->dropin, force32 = a.desc->force32;
     const int64_t L = a.panel.max_len;
     const int64_t M = max(abs(a.sc.ma), abs(a.sc.mi));
     uint64_t cells = 0;
     int max16 = 0;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+    int q0, q1;
+    block_range(nq, q0, q1);
+    for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
         const int n = a.qlen[q];
         uint8_t c = 0;
         if (n > 0) {
@@ -1057,15 +1090,16 @@ __global__ void k_classify(KArgs a) {
         }
         if (c == 1 && a.desc->diag_min_len > 0 && n >= a.desc->diag_min_len) c = 3;
         a.cls[q] = c;
+        const int posD = wave_append(&a.desc->nd, c == 3);
+        const int pos32 = wave_append(&a.desc->n32, c == 2);
         if (c == 3) {
-            const int pos = atomicAdd(&a.desc->nd, 1);
-            a.listD[pos] = q;
+            a.listD[posD] = q;
         } else if (c == 1) {
-            atomicAdd(&a.hist[n], 1);
+            if (n < kLdsBins) atomicAdd(&hist[n], 1);
+            else atomicAdd(&a.hist[n], 1);
             max16 = max(max16, n);
         } else if (c == 2) {
-            const int pos = atomicAdd(&a.desc->n32, 1);
-            a.list32[pos] = q;
+            a.list32[pos32] = q;
         } else if (a.out16 || a.out32) {
             // Unscored rows read back as zeros (ScoreMatrix::from_parts).
             const int nr = a.panel.n_refs;
@@ -1076,6 +1110,9 @@ __global__ void k_classify(KArgs a) {
         }
         if (c) cells += (uint64_t)n;
     }
+    __syncthreads();
+    for (int t = threadIdx.x; t < kLdsBins; t += blockDim.x)
+        if (hist[t]) atomicAdd(&a.hist[t], hist[t]);
     // one atomic per wave for the cell count and the longest int16 query
     for (int off = 32; off > 0; off >>= 1) {
         cells += __shfl_down(cells, off, 64);
@@ -1179,20 +1216,42 @@ __global__ __launch_bounds__(1024) void k_scan(KArgs a) {
     if (t == 0) a.desc->n16 = n16;
 }
 
-__global__ void k_scatter(KArgs a) {
+// Each int16 query's slot in sorted16: its bin's start plus its rank in the
+// bin. A block first counts its queries per short bin in LDS, reserves a
+// range of each bin with one global atomic, then hands out the slots from
+// LDS; longer bins take global atomics directly.
+__global__ __launch_bounds__(256) void k_scatter(KArgs a) {
+    __shared__ int32_t cnt[kLdsBins], base[kLdsBins];
+    for (int t = threadIdx.x; t < kLdsBins; t += blockDim.x) cnt[t] = 0;
+    __syncthreads();
     const int nq = a.desc->n_queries;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+    int q0, q1;
+    block_range(nq, q0, q1);
+    for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
         if (a.cls[q] != 1) continue;
         const int b = a.qlen[q];
-        const int start = a.bin_start[b];
-        if (start < 0) {
-            // routed to the anti-diagonal kernel
-            a.cls[q] = 3;
-            a.listD[atomicAdd(&a.desc->nd, 1)] = q;
-            continue;
+        if (b < kLdsBins && a.bin_start[b] >= 0) atomicAdd(&cnt[b], 1);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < kLdsBins; t += blockDim.x)
+        if (cnt[t]) {
+            base[t] = atomicAdd(&a.bin_fill[t], cnt[t]);
+            cnt[t] = 0;
         }
-        const int pos = start + atomicAdd(&a.bin_fill[b], 1);
-        a.sorted16[pos] = q;
+    __syncthreads();
+    for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+        const bool is16 = a.cls[q] == 1;
+        const int b = is16 ? a.qlen[q] : 0;
+        const int start = is16 ? a.bin_start[b] : 0;
+        const bool diag = is16 && start < 0;  // the bin was routed to the anti-diagonal kernel
+        const int posD = wave_append(&a.desc->nd, diag);
+        if (diag) {
+            a.cls[q] = 3;
+            a.listD[posD] = q;
+        } else if (is16) {
+            const int pos = b < kLdsBins ? base[b] + atomicAdd(&cnt[b], 1) : atomicAdd(&a.bin_fill[b], 1);
+            a.sorted16[start + pos] = q;
+        }
     }
 }
 
