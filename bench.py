@@ -15,7 +15,8 @@ Also reported: the DP kernel's time from HIP events on its stream, GCUPS and
 the fraction of the int16 VALU lane-op roofline; the plain one-thread-per-
 pair int32 kernel on the same batch; the reference's CPU kernel (the
 oracle's AVX-512/AVX2 restatement) on a bounded sample on the host cores;
-the winner tracebacks of the step's selection; the paired workload (2M
+the winner tracebacks of the step's selection; map_reads_scored whole on
+the same reads (scores to records); the paired workload (2M
 pairs of 150-nt mates on a 500-reference panel) mapped end to end with the
 pair tier loop on the device, against the oracle on a sample; and serving.
 """
@@ -260,6 +261,52 @@ def pairs_leg(n_pairs, cpu, both_strands=False):
     return out
 
 
+def map_reads_leg(gpu, panel_codes, codes, off, lens, scoring, local, n_reads, cpu):
+    """Aligner::map_reads_scored whole on the device (esch_map_reads): the
+    first n_reads reads of the workload from host buffers, both strands,
+    scored, selected, every taken tie traced, records out. On a bounded
+    sample, the oracle on the host cores (rows by the SIMD restatement of
+    the reference CPU kernel, then the restated map_reads_scored with scalar
+    tracebacks), equal on every read."""
+    from oracle import pyoracle as O
+
+    gpu.map_reads_concat(codes, off[:4000], lens[:4000], both_strands=True)  # warm: buffers
+    t = time.perf_counter()
+    recs, hits, ops, st = gpu.map_reads_concat(codes, off[:n_reads], lens[:n_reads], both_strands=True)
+    wall = time.perf_counter() - t
+    out = {"reads": n_reads, "reads_per_s": round(n_reads / wall, 1), "wall_ms": round(wall * 1e3, 1),
+           "lib_ms": round(st["total_ms"], 1), "load_ms": round(st["load_ms"], 1),
+           "score_ms": round(st["score_ms"], 1), "select_ms": round(st["select_ms"], 1),
+           "trace_ms": round(st["trace_ms"], 1), "trace_wall_ms": round(st["trace_wall_ms"], 1),
+           "assemble_ms": round(st["assemble_ms"], 1), "traced": int(st["traced"]),
+           "mapped": int((recs["n_hits"] > 0).sum())}
+    if cpu:
+        sup = O.simd_supported()
+        lanes = 32 if sup & 2 else 16
+        threads = min(16, os.cpu_count() or 1)
+        prof = O.Profile(panel_codes, scoring, lanes)
+
+        def run(m):
+            qs = [codes[int(off[k]):int(off[k]) + int(lens[k])] for k in range(m)]
+            t = time.perf_counter()
+            fwd = prof.score_batch(*O.concat(qs), local, threads)
+            rev = prof.score_batch(*O.concat([O.revcomp(q) for q in qs]), local, threads)
+            res = O.map_reads(qs, panel_codes, scoring, local, both_strands=True, threads=threads, rows=(fwd, rev))
+            return time.perf_counter() - t, res
+
+        dt, _ = run(2000)
+        m = int(min(n_reads, max(2000, 2000 * 10.0 / max(dt, 1e-3))))
+        dt, res = run(m)
+        for k in range(m):
+            assert O.read_form(recs[k], hits, ops) == res[k], f"device read mapping != CPU on read {k}"
+        out["cpu_baseline"] = {"reads_per_s": round(m / dt, 1), "threads": threads, "kind": "port",
+                               "sample": f"first {m} reads: both strands scored by the "
+                                         f"{'AVX-512BW' if lanes == 32 else 'AVX2'} restatement of the "
+                                         f"reference CPU kernel, then map_reads_scored restated with scalar "
+                                         f"tracebacks (lib/mapper.rs:400-550), equal on every read"}
+    return out
+
+
 def parse_lengths(text):
     """'lognormal:MEDIAN:SIGMA:LO:HI' or 'uniform:LO:HI' -> synth length spec."""
     kind, *vals = text.split(":")
@@ -382,6 +429,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving run")
     ap.add_argument("--no-trace", action="store_true", help="skip the winner-traceback leg")
     ap.add_argument("--no-pairs", action="store_true", help="skip the paired-mapping leg")
+    ap.add_argument("--no-map", action="store_true", help="skip the read-mapping leg")
+    ap.add_argument("--map-reads", type=int, default=1_000_000, help="reads of the read-mapping leg")
     ap.add_argument("--pairs", type=int, default=2_000_000, help="pairs of the paired-mapping leg")
     ap.add_argument("--pairs-only", action="store_true",
                     help="only run the paired-mapping leg and print its record (profiling)")
@@ -510,6 +559,12 @@ def main():
                           not args.no_cpu)
         log(f"[bench] winner tracebacks: {trace}")
 
+    mapped = None
+    if not args.no_map:
+        mapped = map_reads_leg(gpu, panel_codes, codes, off, lens, scoring, local,
+                               min(args.map_reads, args.reads), not args.no_cpu)
+        log(f"[bench] read mapping: {mapped}")
+
     pairs = None
     if not args.no_pairs:
         pairs = pairs_leg(args.pairs, not args.no_cpu, args.pairs_both)
@@ -564,6 +619,7 @@ def main():
         "plain_kernel": plain,
         "cpu_baseline": cpu,
         "trace": trace,
+        "map_reads": mapped,
         "pairs": pairs,
         "serving": serve,
     }

@@ -68,6 +68,7 @@ EXPORTS = [
     "esch_map_pairs", "esch_map_pairs_concat", "esch_mapped_len", "esch_mapped_reads",
     "esch_mapped_pairs", "esch_mapped_hits", "esch_mapped_ops", "esch_mapped_stats", "esch_mapped_free",
     "esch_set_map_chunk", "esch_adapter_trim_pairs", "esch_md_nm", "esch_cigar_string",
+    "esch_map_reads_concat", "esch_align_last_wavefront",
 ]
 
 
@@ -203,10 +204,13 @@ def load_library():
     L.esch_align_last_ms.argtypes = [vp]
     L.esch_align_pairs_concat.argtypes = [vp, vp, vp, vp, vp, sz, vp, vp, sz, C.POINTER(sz)]
     L.esch_align_last_ms.restype = C.c_float
+    L.esch_align_last_wavefront.argtypes = [vp]
+    L.esch_align_last_wavefront.restype = sz
     L.esch_valu_peak16.argtypes = [C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.esch_get_diag_ratio.argtypes = [vp]
     L.esch_map_reads.argtypes = [vp, vp, vp, sz, C.c_int, i32, i64, C.POINTER(vp)]
     L.esch_map_pairs.argtypes = [vp, vp, vp, vp, vp, sz, C.c_int, i32, i64, i64, C.POINTER(vp)]
+    L.esch_map_reads_concat.argtypes = [vp, vp, vp, vp, sz, C.c_int, i32, i64, C.POINTER(vp)]
     L.esch_map_pairs_concat.argtypes = [vp, vp, vp, vp, sz, C.c_int, i32, i64, i64, C.POINTER(vp)]
     L.esch_mapped_len.argtypes = [vp]
     L.esch_mapped_len.restype = sz
@@ -881,6 +885,19 @@ class GpuScorer:
                                    C.byref(m)))
         return _take_mapped(m, False)
 
+    def map_reads_concat(self, codes: np.ndarray, offs: np.ndarray, lens: np.ndarray, both_strands=False,
+                         min_score=0, max_ties=None):
+        """esch_map_reads_concat: read k is codes[offs[k]:offs[k] + lens[k]];
+        returns what map_reads_raw does."""
+        codes = np.ascontiguousarray(codes, np.uint8)
+        offs = np.ascontiguousarray(offs, np.uint64)
+        lens = np.ascontiguousarray(lens, np.int32)
+        m = C.c_void_p()
+        mt = -1 if max_ties is None else int(max_ties)
+        _check(_lib.esch_map_reads_concat(self._h, codes.ctypes.data, offs.ctypes.data, lens.ctypes.data,
+                                          len(lens), int(both_strands), int(min_score), mt, C.byref(m)))
+        return _take_mapped(m, False)
+
     def map_reads(self, reads: Sequence[np.ndarray], both_strands=False, min_score=0, max_ties=None) -> list:
         """Aligner::map_reads_scored (lib/mapper.rs:400-550) on the device:
         one ReadMapping per read (codes)."""
@@ -928,6 +945,11 @@ class GpuScorer:
     def align_last_ms(self) -> float:
         """Device time of the last align_pairs call's traceback kernels."""
         return float(_lib.esch_align_last_ms(self._h))
+
+    def align_last_wavefront(self) -> int:
+        """Pairs of the last align_pairs call traced by the anti-diagonal
+        (one wave per pair) kernel."""
+        return int(_lib.esch_align_last_wavefront(self._h))
 
     @property
     def diag_ratio(self) -> float:

@@ -138,9 +138,32 @@ struct TraceArgs {
     const uint64_t *ops_off; // per caller index: its region's offset
     int32_t n_pairs;
     int32_t g0;              // first group of this launch
+    int32_t pair0;           // sorted index of the first pair of the groups (the anti-diagonal ones come first)
+    // anti-diagonal traceback (k_trace_diag): sorted pairs 0 .. n_diag - 1,
+    // one wave each, pair k's trace region at arena + dbase[k]
+    const uint64_t *dbase;
+    int32_t n_diag;
+    int32_t d0;              // first of them in this launch
 };
+// k_trace_diag keeps a pass's bottom row in LDS: references up to this long
+constexpr int kTraceDiagMaxRef = 4096;
+constexpr int kTraceDiagPad = 192;  // LDS columns past the reference (block reads ahead)
+// Rows per lane of k_trace_diag for a read of n bases: R in {2, 4, 8}
+// minimising passes x R (the steps of the task), ties to the larger R.
+__host__ __device__ inline int trace_diag_rows(int n) {
+    int best = 8, cost = ((n + 511) / 512) * 8;
+    for (int r = 4; r >= 2; r /= 2) {
+        const int c = ((n + 64 * r - 1) / (64 * r)) * r;
+        if (c < cost) {
+            cost = c;
+            best = r;
+        }
+    }
+    return best;
+}
 
 hipError_t launch_trace(const TraceArgs &a, int groups, bool local, hipStream_t st);
+hipError_t launch_trace_diag(const TraceArgs &a, int pairs, int max_ref, bool local, hipStream_t st);
 hipError_t launch_trace_compact(const uint32_t *ops_wc, const uint64_t *ops_off, const int32_t *recs,
                                 const uint64_t *out_off, uint32_t *out, int32_t n_pairs, hipStream_t st);
 hipError_t launch_pairs(const KArgs &a, const Launch &g, hipStream_t st);

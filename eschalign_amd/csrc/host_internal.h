@@ -64,6 +64,7 @@ struct esch_scorer {
     std::vector<esch_batch *> pool;  // idle batches for the drop-in API
     std::vector<int32_t> h_rlen;     // reference lengths (traceback sizing)
     float trace_ms = 0.0f;           // device time of the last esch_align_pairs kernels
+    size_t trace_diag = 0;           // pairs of the last call traced by the wavefront kernel
     TraceCache *trace = nullptr;     // traceback buffers kept between calls
     MapCache *map = nullptr;         // mapping buffers kept between calls
     size_t map_chunk = 0;            // reads per mapping chunk (0: by the rows' memory)

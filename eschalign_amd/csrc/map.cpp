@@ -664,6 +664,21 @@ esch_status esch_map_reads(esch_scorer *s, const uint8_t *const *reads, const si
         both_strands ? 1 : 0, min_score, max_ties, out);
 }
 
+esch_status esch_map_reads_concat(esch_scorer *s, const uint8_t *codes, const uint64_t *offs, const int32_t *lens,
+                                  size_t n, int both_strands, int32_t min_score, int64_t max_ties,
+                                  esch_mapped **out) {
+    if (!out) return fail(ESCH_INVALID_ARG, "out is null");
+    *out = nullptr;
+    if (!s || (n && (!codes || !offs || !lens))) return fail(ESCH_INVALID_ARG, "null argument");
+    return map_reads_core(
+        s, n,
+        [&](size_t k, const uint8_t *&p, size_t &len) {
+            p = codes + offs[k];
+            len = (size_t)(lens[k] < 0 ? 0 : lens[k]);
+        },
+        both_strands ? 1 : 0, min_score, max_ties, out);
+}
+
 esch_status esch_map_pairs(esch_scorer *s, const uint8_t *const *mates1, const size_t *lens1,
                            const uint8_t *const *mates2, const size_t *lens2, size_t n, int both_strands,
                            int32_t min_score, int64_t max_ties, int64_t max_insert, esch_mapped **out) {

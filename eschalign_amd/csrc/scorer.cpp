@@ -999,7 +999,15 @@ namespace {
 
 constexpr size_t kTraceArenaMax = (size_t)4 << 30;      // trace + boundary bytes per launch
 constexpr size_t kTraceArenaKeep = (size_t)1 << 30;     // cached across calls up to this
-constexpr uint64_t kTraceMaxCells = (uint64_t)1 << 27;  // read x reference: a 4 GiB group region
+constexpr uint64_t kTraceMaxCells = (uint64_t)1 << 27;  // read x reference: a 4 GiB group region (one lane a pair)
+constexpr uint64_t kTraceDiagMinCells = (uint64_t)1 << 15;  // below this a pair never takes the wavefront kernel
+
+// Trace bytes of one pair of the anti-diagonal traceback (k_trace_diag).
+uint64_t trace_diag_bytes(size_t n, int m) {
+    const int R = trace_diag_rows((int)n);
+    const uint64_t passes = (n + 64 * (size_t)R - 1) / (64 * (size_t)R);
+    return passes * ((uint64_t)m + 63) * 256;
+}
 
 }  // namespace
 
@@ -1007,20 +1015,23 @@ constexpr uint64_t kTraceMaxCells = (uint64_t)1 << 27;  // read x reference: a 4
 // kept by the scorer between calls (one call at a time per scorer).
 struct TraceCache {
     std::mutex mu;
-    hipStream_t st = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    DevBuf codes, qoff, qlen, ref, slot, gbase, gS, gM, arena, recs, ops_wc, ops_off, out_off, out;
+    hipStream_t st = nullptr, st2 = nullptr;  // one-lane kernel; wavefront kernel beside it
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    DevBuf codes, qoff, qlen, ref, slot, gbase, gS, gM, dbase, arena, arena2, recs, ops_wc, ops_off, out_off, out;
     HostBuf h_codes, h_recs;
     ~TraceCache() {
         if (st) (void)hipStreamSynchronize(st);
         h_codes.release();
         h_recs.release();
-        for (DevBuf *b : {&codes, &qoff, &qlen, &ref, &slot, &gbase, &gS, &gM, &arena, &recs, &ops_wc, &ops_off,
-                          &out_off, &out})
+        if (st2) (void)hipStreamSynchronize(st2);
+        for (DevBuf *b : {&codes, &qoff, &qlen, &ref, &slot, &gbase, &gS, &gM, &dbase, &arena, &arena2, &recs,
+                          &ops_wc, &ops_off, &out_off, &out})
             b->release();
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
+        if (e2) (void)hipEventDestroy(e2);
         if (st) (void)hipStreamDestroy(st);
+        if (st2) (void)hipStreamDestroy(st2);
     }
 };
 
@@ -1046,10 +1057,13 @@ esch_status esch::align_core(esch_scorer *s, size_t n,
         const size_t len = lens[k];
         if (refs[k] < 0 || (size_t)refs[k] >= s->n_refs)
             return fail(ESCH_INVALID_ARG, "pair %zu names reference %d of %zu", k, refs[k], s->n_refs);
-        // a group region is 64 pairs x (read / 16 stripes) x reference x 8 B
-        if ((uint64_t)len * (uint64_t)s->h_rlen[refs[k]] > kTraceMaxCells)
-            return fail(ESCH_INVALID_ARG, "pair %zu: %zu x %d cells is over the device traceback bound %llu", k, len,
-                        s->h_rlen[refs[k]], (unsigned long long)kTraceMaxCells);
+        // one lane a pair: a group region is 64 pairs x (read / 16 stripes)
+        // x reference x 8 B; a larger pair takes the wavefront kernel, whose
+        // region (read x (reference + 63) / 2 B) must fit the arena
+        const int rl = s->h_rlen[refs[k]];
+        if ((uint64_t)len * (uint64_t)rl > kTraceMaxCells &&
+            (rl > kTraceDiagMaxRef || trace_diag_bytes(len, rl) > kTraceArenaMax))
+            return fail(ESCH_INVALID_ARG, "pair %zu: %zu x %d cells is over the device traceback bound", k, len, rl);
         if (len && !ptr[k]) return fail(ESCH_INVALID_ARG, "pair %zu: null read", k);
         out[k] = esch_alignment{};  // lib/scalar.rs:185-195: empty in, empty out
         if (len) {
@@ -1077,6 +1091,33 @@ esch_status esch::align_core(esch_scorer *s, size_t n,
         run = 0;
         for (size_t& c : cl) { const size_t x = c; c = run; run += x; }
         for (size_t i = 0; i < np; ++i) order[cl[max_len - lens[tmp[i]]]++] = tmp[i];
+    }
+    // Pairs for the wavefront kernel first: those a lane could not hold, and
+    // those long enough that one lane would set the call's time: more cells
+    // than 2^-18 of the call's (a device of 1,024 SIMDs traces about 2^16
+    // lanes' worth at once, and a long pair's lane shares its SIMD with a
+    // few other waves), and at least 2^15 (below that the wave per pair of
+    // the wavefront kernel costs more than it saves). Measured on the 1M-
+    // and 200k-read winner sets: 17.5 and 5.1 ms against 22.2 and 12.1 ms
+    // with every pair on the one-lane kernel.
+    size_t n_diag = 0;
+    {
+        uint64_t W = 0;
+        for (size_t i = 0; i < np; ++i) W += (uint64_t)lens[order[i]] * (uint64_t)s->h_rlen[refs[order[i]]];
+        const uint64_t thr = std::max<uint64_t>(kTraceDiagMinCells, W >> 18);
+        std::vector<int32_t> dl, rest;
+        for (size_t i = 0; i < np; ++i) {
+            const int32_t k = order[i];
+            const int rl = s->h_rlen[refs[k]];
+            const uint64_t cells = (uint64_t)lens[k] * (uint64_t)rl;
+            const bool diag = rl <= kTraceDiagMaxRef && (cells > kTraceMaxCells || (lens[k] >= 128 && cells >= thr));
+            (diag ? dl : rest).push_back(k);
+        }
+        n_diag = dl.size();
+        if (n_diag) {
+            std::copy(dl.begin(), dl.end(), order.begin());
+            std::copy(rest.begin(), rest.end(), order.begin() + (long)n_diag);
+        }
     }
     HIPC(hipSetDevice(s->ordinal));
     if (!s->trace) s->trace = new TraceCache;
@@ -1152,12 +1193,14 @@ esch_status esch::align_core(esch_scorer *s, size_t n,
         ops_off[k] = wc;
         if (lens[k]) wc += lens[k] + (size_t)s->h_rlen[refs[k]];
     }
-    const int ng = (int)((np + 63) / 64);
+    s->trace_diag = n_diag;
+    const size_t n_lane = np - n_diag;  // pairs of the one-lane kernel, after the wavefront ones
+    const int ng = (int)((n_lane + 63) / 64);
     std::vector<int32_t> gS(ng), gM(ng);
     std::vector<uint64_t> gbytes(ng), gbase(ng);
     for (int g = 0; g < ng; ++g) {
         int32_t mx_n = 0, mx_m = 0;
-        for (size_t i = (size_t)g * 64; i < std::min(np, (size_t)g * 64 + 64); ++i) {
+        for (size_t i = n_diag + (size_t)g * 64; i < std::min(np, n_diag + (size_t)g * 64 + 64); ++i) {
             mx_n = std::max(mx_n, hqlen[i]);
             mx_m = std::max(mx_m, s->h_rlen[href[i]]);
         }
@@ -1165,9 +1208,27 @@ esch_status esch::align_core(esch_scorer *s, size_t n,
         gM[g] = mx_m;
         gbytes[g] = ((uint64_t)gS[g] * gM[g] + gM[g]) * 512;
     }
-    // launches of consecutive groups whose regions fit the arena together
-    std::vector<std::pair<int, int>> chunks;
-    size_t arena = 0;
+    // launches of consecutive wavefront pairs, then of consecutive groups,
+    // whose regions fit the arena together
+    std::vector<uint64_t> dbase(std::max<size_t>(n_diag, 1));
+    std::vector<std::pair<int, int>> dchunks, chunks;
+    size_t arena = 0, arena2 = 0;  // the wavefront kernel runs beside the other: its own arena
+    int diag_max_ref = 0;
+    for (size_t d0 = 0; d0 < n_diag;) {
+        uint64_t sum = 0;
+        size_t d1 = d0;
+        while (d1 < n_diag) {
+            const uint64_t b = trace_diag_bytes((size_t)hqlen[d1], s->h_rlen[href[d1]]);
+            if (d1 > d0 && sum + b > kTraceArenaMax) break;
+            dbase[d1] = sum;
+            sum += b;
+            diag_max_ref = std::max(diag_max_ref, s->h_rlen[href[d1]]);
+            ++d1;
+        }
+        dchunks.emplace_back((int)d0, (int)d1);
+        arena2 = std::max<size_t>(arena2, sum);
+        d0 = d1;
+    }
     for (int g0 = 0; g0 < ng;) {
         uint64_t sum = 0;
         int g1 = g0;
@@ -1184,6 +1245,8 @@ esch_status esch::align_core(esch_scorer *s, size_t n,
     if (!T.st) HIPC(hipStreamCreateWithFlags(&T.st, hipStreamNonBlocking));
     if (!T.e0) HIPC(hipEventCreate(&T.e0));
     if (!T.e1) HIPC(hipEventCreate(&T.e1));
+    if (n_diag && !T.st2) HIPC(hipStreamCreateWithFlags(&T.st2, hipStreamNonBlocking));
+    if (n_diag && !T.e2) HIPC(hipEventCreateWithFlags(&T.e2, hipEventDisableTiming));
     if (!dev_codes) HIPC(T.codes.need(bases));
     HIPC(T.qoff.need(np * 8));
     HIPC(T.qlen.need(np * 4));
@@ -1192,7 +1255,9 @@ esch_status esch::align_core(esch_scorer *s, size_t n,
     HIPC(T.gbase.need((size_t)ng * 8));
     HIPC(T.gS.need((size_t)ng * 4));
     HIPC(T.gM.need((size_t)ng * 4));
+    HIPC(T.dbase.need(dbase.size() * 8));
     HIPC(T.arena.need(arena));
+    if (n_diag) HIPC(T.arena2.need(arena2));
     HIPC(T.recs.need(n * kTraceRec * 4));
     HIPC(T.ops_wc.need(wc * 4));
     HIPC(T.ops_off.need(n * 8));
@@ -1205,25 +1270,41 @@ esch_status esch::align_core(esch_scorer *s, size_t n,
     HIPC(hipMemcpyAsync(T.gbase.p, gbase.data(), (size_t)ng * 8, hipMemcpyHostToDevice, st));
     HIPC(hipMemcpyAsync(T.gS.p, gS.data(), (size_t)ng * 4, hipMemcpyHostToDevice, st));
     HIPC(hipMemcpyAsync(T.gM.p, gM.data(), (size_t)ng * 4, hipMemcpyHostToDevice, st));
+    if (n_diag) HIPC(hipMemcpyAsync(T.dbase.p, dbase.data(), n_diag * 8, hipMemcpyHostToDevice, st));
     HIPC(hipMemcpyAsync(T.ops_off.p, ops_off.data(), n * 8, hipMemcpyHostToDevice, st));
     // empty pairs are never traced: their records must read as zero runs
     HIPC(hipMemsetAsync(T.recs.p, 0, n * kTraceRec * 4, st));
     TraceArgs ta{s->panel, s->sc, dev_codes ? dev_codes : T.codes.as<uint8_t>(), T.qoff.as<uint64_t>(), T.qlen.as<int32_t>(),
                  T.ref.as<int32_t>(), T.slot.as<int32_t>(), T.gbase.as<uint64_t>(), T.gS.as<int32_t>(),
                  T.gM.as<int32_t>(), T.arena.as<uint8_t>(), T.recs.as<int32_t>(), T.ops_wc.as<uint32_t>(),
-                 T.ops_off.as<uint64_t>(), (int32_t)np, 0};
+                 T.ops_off.as<uint64_t>(), (int32_t)np, 0, (int32_t)n_diag, T.dbase.as<uint64_t>(),
+                 (int32_t)n_diag, 0};
     const bool local = s->sc.local != 0;
     HIPC(hipEventRecord(T.e0, st));
+    if (n_diag) {
+        // the long pairs' waves run beside the one-lane groups (after the
+        // copies above, which e0 follows on st)
+        HIPC(hipStreamWaitEvent(T.st2, T.e0, 0));
+        TraceArgs td = ta;
+        td.arena = T.arena2.as<uint8_t>();
+        for (const auto &c : dchunks) {
+            td.d0 = c.first;
+            HIPC(launch_trace_diag(td, c.second - c.first, diag_max_ref, local, T.st2));
+        }
+        HIPC(hipEventRecord(T.e2, T.st2));
+    }
     for (const auto &c : chunks) {
         ta.g0 = c.first;
         HIPC(launch_trace(ta, c.second - c.first, local, st));
     }
+    if (n_diag) HIPC(hipStreamWaitEvent(st, T.e2, 0));
     HIPC(hipEventRecord(T.e1, st));
     int32_t *const recs = T.h_recs.as<int32_t>();
     HIPC(hipMemcpyAsync(recs, T.recs.p, n * kTraceRec * 4, hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
     HIPC(hipEventElapsedTime(&s->trace_ms, T.e0, T.e1));
     if (T.arena.cap > kTraceArenaKeep) T.arena.release();
+    if (T.arena2.cap > kTraceArenaKeep) T.arena2.release();
     std::vector<uint64_t> out_off(n, 0);
     size_t runs = 0;
     for (size_t k = 0; k < n; ++k) {
@@ -1283,3 +1364,4 @@ esch_status esch_align_pairs_concat(esch_scorer *s, const uint8_t *codes, const 
 }
 
 float esch_align_last_ms(const esch_scorer *s) { return s ? s->trace_ms : 0.0f; }
+size_t esch_align_last_wavefront(const esch_scorer *s) { return s ? s->trace_diag : 0; }

@@ -243,6 +243,11 @@ esch_status esch_align_pairs_concat(esch_scorer *s, const uint8_t *codes, const 
 /* Device time (ms, HIP events) of the traceback kernels of the scorer's
  * last esch_align_pairs call (diagnostics; no counterpart in the reference). */
 float esch_align_last_ms(const esch_scorer *s);
+/* How many pairs of that call took the anti-diagonal (one wave per pair)
+ * traceback kernel rather than one lane each: pairs over 2^27 cells, and
+ * pairs of at least 128 read bases whose cells reach 2^-18 of the call's
+ * (and 2^15), on references up to 4,096 nt. Diagnostics. */
+size_t esch_align_last_wavefront(const esch_scorer *s);
 
 /* ---- read and pair mapping: scoring, selection, tracebacks and the pair
  * tier loop on the device (lib/mapper.rs:400-550, lib/mapper/pair.rs:157-387).
@@ -294,6 +299,10 @@ typedef struct esch_mapped esch_mapped;
  * min_score) traced, a read unmapped when its primary traces to nothing. */
 esch_status esch_map_reads(esch_scorer *s, const uint8_t *const *reads, const size_t *lens, size_t n,
                            int both_strands, int32_t min_score, int64_t max_ties, esch_mapped **out);
+/* The same with reads in one buffer: read k is codes[offs[k] .. + lens[k]). */
+esch_status esch_map_reads_concat(esch_scorer *s, const uint8_t *codes, const uint64_t *offs, const int32_t *lens,
+                                  size_t n, int both_strands, int32_t min_score, int64_t max_ties,
+                                  esch_mapped **out);
 /* Aligner::map_pairs_scored (lib/mapper/pair.rs:157-387): candidates
  * (reference, template strand) with both mates >= min_score, taken tier by
  * tier in descending summed score; the first tier with a candidate passing

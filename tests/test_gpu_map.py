@@ -60,6 +60,10 @@ def test_map_reads_equals_oracle():
             want = O.map_reads(reads, panel.codes, scoring, local, **kw)
             for k, (g, w) in enumerate(zip(got, want)):
                 assert g == w, (k, kw, g, w)
+        # the concatenated entry point gives the same records
+        qc, qo, ql = O.concat(reads)
+        recs, hits, ops, _ = gpu.map_reads_concat(qc, qo, ql, both_strands=True, min_score=0, max_ties=2)
+        assert [O.read_form(r, hits, ops) for r in recs] == dev_reads(gpu, reads, both_strands=True, max_ties=2)
         # the object API reads the same records
         objs = gpu.map_reads(reads[:50], both_strands=True)
         forms = dev_reads(gpu, reads[:50], both_strands=True)

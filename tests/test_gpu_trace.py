@@ -144,3 +144,65 @@ def test_trace_concat_entry_and_repeated_calls():
             assert al.ops == tuple(("MID"[int(x) >> 30], int(x) & 0x3FFFFFFF) for x in seg)
         want = oracle_aligns(qs[:200], panel.codes, rs[:200], (1, -1, -2, -1), False)
         assert a[:200] == want
+
+
+def test_trace_long_reads_take_the_wavefront_kernel():
+    """Reads of 300-5,000 nt against references of 100-1,200 nt (wildcards,
+    chimeric reads), both modes, three schemes: the pairs that would set the
+    one-lane kernel's time take the anti-diagonal kernel (R = 2, 4 and 8
+    rows per lane, one to ten passes); equal to the oracle."""
+    rng = G.Rng(0x5EED_0C05)
+    panel_l = [G.random_seq(rng, 100 + rng.below(1100), 0.02) for _ in range(12)]
+    lens = [300, 330, 400, 511, 512, 513, 700, 1024, 1500, 2600, 5000] + [300 + rng.below(2000) for _ in range(9)]
+    queries = [G.encode(G.chimeric_read(rng, panel_l, n)) for n in lens]
+    queries += [G.encode(panel_l[3][:200] + panel_l[5] + panel_l[7][50:])]
+    refs = [rng.below(len(panel_l)) for _ in queries]
+    for scoring in ((2, -1, -10, -1), (1, -1, -2, -1), (2, -1, 0, 0)):
+        for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
+            panel = ea.Panel(panel_l)
+            gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+            got = gpu.align_pairs(queries, refs)
+            # the routing rule: at least 128 bases and max(2^15, cells / 2^18) cells
+            cells = [len(q) * len(panel_l[r]) for q, r in zip(queries, refs)]
+            thr = max(1 << 15, sum(cells) >> 18)
+            assert gpu.align_last_wavefront() == sum(1 for q, c in zip(queries, cells) if len(q) >= 128 and c >= thr)
+            assert gpu.align_last_wavefront() >= len(queries) - 3
+            want = oracle_aligns(queries, panel.codes, refs, scoring, mode == ea.Mode.LOCAL)
+            for k, (g, w) in enumerate(zip(got, want)):
+                assert g == w, (k, len(queries[k]), refs[k], scoring, mode)
+
+
+def test_trace_mixed_call_both_kernels():
+    """One call with 3,000 short pairs and a few long ones: the long ones on
+    the wavefront kernel, the rest on the one-lane kernel, every pair equal
+    to the oracle and to the same pairs traced alone."""
+    rng = G.Rng(0x5EED_0C06)
+    panel_l = [G.random_seq(rng, 80 + rng.below(400), 0.01) for _ in range(30)]
+    panel = ea.Panel(panel_l)
+    queries = [G.encode(G.chimeric_read(rng, panel_l, 20 + rng.below(150))) for _ in range(3000)]
+    queries += [G.encode(G.chimeric_read(rng, panel_l, 900 + rng.below(2000))) for _ in range(6)]
+    refs = [rng.below(len(panel_l)) for _ in queries]
+    for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
+        gpu = ea.GpuScorer(panel, ea.Scoring(2, -1, -10, -1), mode)
+        got = gpu.align_pairs(queries, refs)
+        assert 6 <= gpu.align_last_wavefront() < len(queries) // 2
+        want = oracle_aligns(queries, panel.codes, refs, (2, -1, -10, -1), mode == ea.Mode.LOCAL)
+        assert got == want
+        short = gpu.align_pairs(queries[:3000], refs[:3000])
+        assert short == got[:3000]
+
+
+def test_trace_pair_over_the_one_lane_bound():
+    """A 36,000-nt read against a 4,000-nt reference (1.44e8 cells, over the
+    one-lane kernel's 2^27): traced by the wavefront kernel instead of being
+    refused; equal to the oracle."""
+    rng = G.Rng(0x5EED_0C07)
+    ref = G.random_seq(rng, 4000, 0.0)
+    read = G.random_seq(rng, 16000, 0.0) + G.mutate(rng, ref) + G.random_seq(rng, 16000, 0.0)
+    read = read[:36000]
+    panel = ea.Panel([ref])
+    gpu = ea.GpuScorer(panel, ea.Scoring(2, -1, -10, -1), ea.Mode.LOCAL)
+    got = gpu.align_pairs([G.encode(read)], [0])
+    assert gpu.align_last_wavefront() == 1
+    assert got == oracle_aligns([G.encode(read)], panel.codes, [0], (2, -1, -10, -1), True)
+    assert got[0].score > 4000
