@@ -997,8 +997,11 @@ esch_status esch_select_pairs(esch_scorer *s, const uint8_t *const *mates1, cons
 
 namespace {
 
-constexpr size_t kTraceArenaMax = (size_t)4 << 30;      // trace + boundary bytes per launch
-constexpr size_t kTraceArenaKeep = (size_t)1 << 30;     // cached across calls up to this
+// Trace + boundary bytes per launch: a call whose regions do not fit is cut
+// into launches run one after the other, each with its own tail, so the
+// arena is sized for HBM (16 GiB holds the regions of a million winners).
+constexpr size_t kTraceArenaMax = (size_t)16 << 30;
+constexpr size_t kTraceArenaKeep = (size_t)4 << 30;     // cached across calls up to this
 constexpr uint64_t kTraceMaxCells = (uint64_t)1 << 27;  // read x reference: a 4 GiB group region (one lane a pair)
 constexpr uint64_t kTraceDiagMinCells = (uint64_t)1 << 15;  // below this a pair never takes the wavefront kernel
 
