@@ -140,7 +140,7 @@ def trace_leg(gpu, batch, panel_codes, codes, off, lens, scoring, local, n_reads
     qlen = ln.astype(np.int32)
     rlen = np.array([len(p) for p in panel_codes], np.int64)
     cells = int((ln * rlen[refs]).sum())
-    gpu.align_pairs_concat(big, qoff[:2000], qlen[:2000], refs[:2000])  # warm: code objects, buffers
+    gpu.align_pairs_concat(big, qoff, qlen, refs)  # warm: code objects, the call's buffers (steady state)
     t = time.perf_counter()
     al, _ = gpu.align_pairs_concat(big, qoff, qlen, refs)
     wall = time.perf_counter() - t
@@ -195,7 +195,7 @@ def pairs_leg(n_pairs, cpu, both_strands=False):
         f"{tl.mean():.1f} nt after adapter trimming ({time.perf_counter() - t0:.1f} s to generate)")
     gpu = ea.GpuScorer(ea.Panel(panel), ea.Scoring(*scoring), ea.Mode.SEMIGLOBAL)
     opts = dict(both_strands=both_strands, min_score=0, max_ties=None, max_insert=500)
-    gpu.map_pairs_concat(codes, off[:4000], tl[:4000], **opts)  # warm: code objects, buffers
+    gpu.map_pairs_concat(codes, off, tl, **opts)  # warm: code objects, the call's buffers (steady state)
     t = time.perf_counter()
     recs, hits, ops, st = gpu.map_pairs_concat(codes, off, tl, **opts)
     wall = time.perf_counter() - t
@@ -270,7 +270,7 @@ def map_reads_leg(gpu, panel_codes, codes, off, lens, scoring, local, n_reads, c
     tracebacks), equal on every read."""
     from oracle import pyoracle as O
 
-    gpu.map_reads_concat(codes, off[:4000], lens[:4000], both_strands=True)  # warm: buffers
+    gpu.map_reads_concat(codes, off[:n_reads], lens[:n_reads], both_strands=True)  # warm: the call's buffers
     t = time.perf_counter()
     recs, hits, ops, st = gpu.map_reads_concat(codes, off[:n_reads], lens[:n_reads], both_strands=True)
     wall = time.perf_counter() - t

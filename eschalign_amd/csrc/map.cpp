@@ -25,6 +25,7 @@
 #include <climits>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -76,8 +77,13 @@ struct Pool {
 
 }  // namespace
 
+// Buffers kept by the scorer between mapping calls; one call at a time per
+// scorer (the mutex), as the traceback cache.
 struct MapCache {
+    std::mutex mu;
     DevBuf list, below, recs, cands, cand_off, set, set_off, set_n, out;
+    std::vector<uint8_t> rc2;   // mate 2's reverse complements of a chunk (host)
+    std::vector<size_t> rc2_at;
 };
 
 void map_cache_free(esch_scorer *s) {
@@ -253,6 +259,8 @@ size_t chunk_rows(const esch_scorer *s) {
 
 esch_status scorer_ready(esch_scorer *s) {
     HIPC(hipSetDevice(s->ordinal));
+    static std::mutex create_mu;
+    std::lock_guard<std::mutex> g(create_mu);
     if (!s->map) s->map = new MapCache;
     return ESCH_OK;
 }
@@ -296,6 +304,7 @@ esch_status map_reads_core(esch_scorer *s, size_t n, const Getter &get, int both
     std::unique_ptr<esch_mapped> M(new esch_mapped);
     M->reads.resize(n);
     if (esch_status st = scorer_ready(s)) return st;
+    std::lock_guard<std::mutex> call(s->map->mu);
     const size_t chunk = chunk_rows(s) / 2;
     for (size_t k0 = 0; k0 < n; k0 += chunk) {
         const size_t k1 = std::min(n, k0 + chunk);
@@ -350,7 +359,7 @@ bool rc_mates2(const Getter &mate, size_t p0, size_t p1, std::vector<uint8_t> &b
         mate(2 * (p0 + i) + 1, p, len);
         at[i + 1] = at[i] + len;
     }
-    buf.resize(std::max<size_t>(at[c], 1));
+    if (buf.size() < at[c] + 1) buf.resize(at[c] + 1);
     const size_t hw = std::max(1u, std::thread::hardware_concurrency());
     const size_t nt = at[c] < ((size_t)8 << 20) ? 1 : std::min<size_t>(8, hw);
     std::vector<size_t> first_bad(nt, SIZE_MAX);
@@ -388,6 +397,7 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
     M->is_pairs = true;
     M->pairs.resize(n_pairs);
     if (esch_status st = scorer_ready(s)) return st;
+    std::lock_guard<std::mutex> call(s->map->mu);
     const int strands = both_strands ? 2 : 1;
     // Forward template strand only: the tiers need mate 1 as sequenced and
     // mate 2 reverse-complemented, nothing else, so only those two rows per
@@ -396,8 +406,8 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
     // demand too, lib/mapper/pair.rs:195-207). Both strands: all four.
     const bool half = !both_strands;
     const size_t chunk = std::max<size_t>(1, chunk_rows(s) / (half ? 2 : 4));
-    std::vector<uint8_t> rc2;
-    std::vector<size_t> rc2_at;
+    std::vector<uint8_t> &rc2 = s->map->rc2;  // grow-only across calls
+    std::vector<size_t> &rc2_at = s->map->rc2_at;
     for (size_t p0 = 0; p0 < n_pairs; p0 += chunk) {
         const size_t p1 = std::min(n_pairs, p0 + chunk), c = p1 - p0;
         esch_batch *b = nullptr;

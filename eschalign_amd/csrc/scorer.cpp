@@ -1021,11 +1021,13 @@ struct TraceCache {
     hipStream_t st = nullptr, st2 = nullptr;  // one-lane kernel; wavefront kernel beside it
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
     DevBuf codes, qoff, qlen, ref, slot, gbase, gS, gM, dbase, arena, arena2, recs, ops_wc, ops_off, out_off, out;
-    HostBuf h_codes, h_recs;
+    HostBuf h_codes, h_recs, h_off, h_ops;
     ~TraceCache() {
         if (st) (void)hipStreamSynchronize(st);
         h_codes.release();
         h_recs.release();
+        h_off.release();
+        h_ops.release();
         if (st2) (void)hipStreamSynchronize(st2);
         for (DevBuf *b : {&codes, &qoff, &qlen, &ref, &slot, &gbase, &gS, &gM, &dbase, &arena, &arena2, &recs,
                           &ops_wc, &ops_off, &out_off, &out})
@@ -1308,29 +1310,31 @@ esch_status esch::align_core(esch_scorer *s, size_t n,
     HIPC(hipEventElapsedTime(&s->trace_ms, T.e0, T.e1));
     if (T.arena.cap > kTraceArenaKeep) T.arena.release();
     if (T.arena2.cap > kTraceArenaKeep) T.arena2.release();
-    std::vector<uint64_t> out_off(n, 0);
+    HIPC(T.h_off.need(n * 8));
+    uint64_t *const out_off = T.h_off.as<uint64_t>();  // pinned: copied to the device
     size_t runs = 0;
     for (size_t k = 0; k < n; ++k) {
         out_off[k] = runs;
         if (lens[k]) runs += (size_t)recs[k * kTraceRec + 5];
     }
     if (ops_used) *ops_used = runs;
-    if (ops_vec) {
-        ops_vec->resize(runs);
-        ops = ops_vec->data();
-        ops_cap = runs;
-    }
-    if (runs > ops_cap || (runs && !ops))
+    if (ops_vec) ops_vec->clear();  // filled from the pinned staging below
+    if (!ops_vec && (runs > ops_cap || (runs && !ops)))
         return fail(ESCH_CAPACITY, "the alignments have %zu operation runs; ops holds %zu", runs, ops_cap);
     if (runs) {
         HIPC(T.out_off.need(n * 8));
         HIPC(T.out.need(runs * 4));
+        HIPC(T.h_ops.need(runs * 4));
         // untraced (empty) pairs have zero runs and copy nothing
-        HIPC(hipMemcpyAsync(T.out_off.p, out_off.data(), n * 8, hipMemcpyHostToDevice, st));
+        HIPC(hipMemcpyAsync(T.out_off.p, out_off, n * 8, hipMemcpyHostToDevice, st));
         HIPC(launch_trace_compact(T.ops_wc.as<uint32_t>(), T.ops_off.as<uint64_t>(), T.recs.as<int32_t>(),
                                   T.out_off.as<uint64_t>(), T.out.as<uint32_t>(), (int32_t)n, st));
-        HIPC(hipMemcpyAsync(ops, T.out.p, runs * 4, hipMemcpyDeviceToHost, st));
+        HIPC(hipMemcpyAsync(T.h_ops.p, T.out.p, runs * 4, hipMemcpyDeviceToHost, st));
         HIPC(hipStreamSynchronize(st));
+        // pinned staging to the caller's memory
+        const uint32_t *h = T.h_ops.as<uint32_t>();
+        if (ops_vec) ops_vec->assign(h, h + runs);
+        else memcpy(ops, h, runs * 4);
     }
     for (size_t k = 0; k < n; ++k) {
         if (!lens[k]) continue;

@@ -254,3 +254,23 @@ def test_map_pairs_paired_workload_sample():
             assert g == w, (k, kw)
         kinds = [g[0] for g in got]
         assert kinds.count("proper") > len(pairs) // 2
+
+
+def test_map_reads_longer_than_the_drop_in_bound():
+    """Reads of 4,500-6,000 nt (over the reference's 4,096-nt GPU bound,
+    cli/align.rs:1751-1774 keeps them on the CPU) mapped whole on the
+    device: scored by the int16 path at that length, their winners traced
+    by the anti-diagonal kernel; both modes, both strands, equal to the
+    oracle."""
+    rng = G.Rng(0x5EED0207)
+    panel_l = [G.random_seq(rng, 200 + rng.below(800), 0.01) for _ in range(8)]
+    reads = [G.encode(G.chimeric_read(rng, panel_l, 4500 + rng.below(1500))) for _ in range(16)]
+    reads += [G.encode(G.mutate(rng, panel_l[k % 8])) for k in range(40)]
+    panel = ea.Panel(panel_l)
+    for scoring, mode in ((DEFAULT, ea.Mode.LOCAL), ((1, -1, -2, -1), ea.Mode.SEMIGLOBAL)):
+        gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+        got = dev_reads(gpu, reads, both_strands=True)
+        want = O.map_reads(reads, panel.codes, scoring, mode == ea.Mode.LOCAL, both_strands=True)
+        for k, (g, w) in enumerate(zip(got, want)):
+            assert g == w, (k, len(reads[k]))
+        assert all(g[3] for g in got[:16])  # every long read maps
