@@ -207,7 +207,9 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
                                           const int lane, const cwords w2, const cwords wn,
                                           const int m, const int row0, const int pA, const int pB,
                                           const v2s vo, const v2s ve, v2s &best) {
-    constexpr int NQ = RR / 4;
+    static_assert(RR % 2 == 0 && RR >= 2 && RR <= 16, "an even number of rows");
+    constexpr int NQ = RR / 4;          // whole quads of rows
+    constexpr bool HALF = RR % 4 == 2;  // and the first two rows of one more
     const v2s NEG = splat(-32768);
     v2s Ho[RR], E[RR];
 #pragma unroll
@@ -229,6 +231,11 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
             sc[4 * q + 1] = as_v2s(v.y);
             sc[4 * q + 2] = as_v2s(v.z);
             sc[4 * q + 3] = as_v2s(v.w);
+        }
+        if constexpr (HALF) {
+            const uint2 v = *reinterpret_cast<const uint2 *>(pc + NQ * 64);
+            sc[4 * NQ + 0] = as_v2s(v.x);
+            sc[4 * NQ + 1] = as_v2s(v.y);
         }
     };
     uint2 *const s0 = scr + 2 * 64 + lane;  // column 0 (two padding columns in front)
@@ -393,7 +400,11 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
         // of which the first has RR rows (a multiple of 4) and the rest 16.
         const int nmax = uni(a.qlen[a.sorted16[k * kChunk]]);
         const int S = (nmax + kStripe - 1) / kStripe;
-        const int RR = ((nmax - kStripe * (S - 1) + 3) / 4) * 4;
+        // variants 2 and 3 take first stripes of any even height, the
+        // others multiples of 4 (less top padding: 0.5 rows a read on
+        // average against 1.5)
+        constexpr int G = PF >= 2 ? 2 : 4;
+        const int RR = ((nmax - kStripe * (S - 1) + G - 1) / G) * G;
         const int T = RR + kStripe * (S - 1);
         const int pA = T - nA, pB = T - nB;
         const uint8_t *qA = a.qcodes + (qa >= 0 ? a.qoff[qa] : 0);
@@ -415,8 +426,10 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
 #pragma unroll
                 for (int rr = 0; rr < 4; ++rr) {
                     const int i = row0 + 4 * wv + rr;
-                    const int xa = (i < pA) ? kPreCode : qA[i - pA];
-                    const int xb = (i < pB) ? kPreCode : qB[i - pB];
+                    // rows past the chunk's plan (a first stripe of 2, 6, 10 or
+                    // 14 rows builds whole quads) are never read: padding codes
+                    const int xa = (i < pA || i >= T) ? kPreCode : qA[i - pA];
+                    const int xb = (i < pB || i >= T) ? kPreCode : qB[i - pB];
                     int bestA = -32768, bestB = -32768;
 #pragma unroll
                     for (int c = 0; c < 4; ++c) {
@@ -438,14 +451,15 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
                 constexpr bool Q = PF == 3;
                 if (s > 0) {
                     stripe16d_any<16, LOCAL, false, Q>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
-                } else if (RR == 16) {
-                    stripe16d_any<16, LOCAL, true, Q>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
-                } else if (RR == 12) {
-                    stripe16d_any<12, LOCAL, true, Q>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
-                } else if (RR == 8) {
-                    stripe16d_any<8, LOCAL, true, Q>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
                 } else {
-                    stripe16d_any<4, LOCAL, true, Q>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
+                    switch (RR) {
+#define ESCH_FIRST(H) \
+    case H: stripe16d_any<H, LOCAL, true, Q>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best); break;
+                        ESCH_FIRST(16) ESCH_FIRST(14) ESCH_FIRST(12) ESCH_FIRST(10)
+                        ESCH_FIRST(8) ESCH_FIRST(6) ESCH_FIRST(4)
+                        default: stripe16d_any<2, LOCAL, true, Q>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
+#undef ESCH_FIRST
+                    }
                 }
             } else {
                 constexpr bool PP = PF == 1;
