@@ -39,8 +39,7 @@ PEAK_I32 = N_CU * CLOCK_HZ * 64
 OPS_PER_CELL = {"local": 10, "semiglobal": 8}
 # SQ_INSTS_VALU x 128 / cells of one k_dp16 launch of the default workload,
 # from the counter summary in profiles/ (mode, variant) -> lane-ops per cell
-VALU_OPS_PER_CELL_ISSUED = {("semiglobal", 2): round(354_222_295_060 * 128 / 5_303_272_453_076, 3),
-                            ("semiglobal", 3): round(348_166_699_424 * 128 / 5_303_272_453_076, 3)}
+VALU_OPS_PER_CELL_ISSUED = {("semiglobal", 3): round(345_033_311_192 * 128 / 5_303_272_453_076, 3)}
 
 
 def log(*a):
