@@ -218,3 +218,29 @@ def test_adapter_trim_pairs_equals_detect_pair():
         assert [int(got[2 * p]), int(got[2 * p + 1])] == [len(a) - d[0], len(b) - d[1]], p
         trimmed += d[0] > 0
     assert trimmed > 1000
+
+
+def test_mapping_and_sam_entry_points_refuse_bad_arguments():
+    """The mapping calls and the SAM field writers refuse null or
+    inconsistent arguments with a status, never a crash, before any device
+    work."""
+    L = ea.load_library()
+    out = C.c_void_p()
+    assert L.esch_map_reads(None, None, None, 0, 0, 0, -1, C.byref(out)) == ea.INVALID_ARG
+    assert L.esch_map_reads(None, None, None, 0, 0, 0, -1, None) == ea.INVALID_ARG
+    assert L.esch_map_pairs(None, None, None, None, None, 0, 0, 0, -1, 500, C.byref(out)) == ea.INVALID_ARG
+    assert L.esch_map_pairs_concat(None, None, None, None, 0, 0, 0, -1, 500, C.byref(out)) == ea.INVALID_ARG
+    assert L.esch_map_reads_concat(None, None, None, None, 0, 0, 0, -1, C.byref(out)) == ea.INVALID_ARG
+    assert L.esch_set_map_chunk(None, 0) == ea.INVALID_ARG
+    assert L.esch_mapped_len(None) == 0
+    assert L.esch_mapped_pairs(None) is None and L.esch_mapped_reads(None) is None
+    n = C.c_size_t(7)
+    assert L.esch_mapped_hits(None, C.byref(n)) is None and n.value == 0
+    L.esch_mapped_free(None)
+    # ops running past the read, or spans out of order
+    with pytest.raises(ValueError):
+        ea.md_nm(b"ACGT", b"ACGT", 0, 0, [("M", 5)])
+    with pytest.raises(ValueError):
+        ea.md_nm(b"ACGT", b"ACGT", 0, 2, [("D", 3)])
+    with pytest.raises(ValueError):
+        ea.cigar_string([("M", 2)], 4, 3, 2)
