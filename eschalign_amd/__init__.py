@@ -75,7 +75,7 @@ EXPORTS = [
 class ServerConfig(C.Structure):
     _fields_ = [("ordinal", C.c_int), ("hbm_budget", C.c_size_t),
                 ("max_batch_reads", C.c_size_t), ("max_batch_bases", C.c_size_t),
-                ("max_slots", C.c_int)]
+                ("max_slots", C.c_int), ("max_batch_cells", C.c_size_t)]
 
 
 class ServerStats(C.Structure):
@@ -1102,9 +1102,10 @@ class Server:
     (include/eschalign_amd.h, esch_server_*)."""
 
     def __init__(self, hbm_budget=32 << 30, max_batch_reads=50_000, max_batch_bases=8_000_000,
-                 max_slots=4, ordinal=0):
+                 max_slots=2, ordinal=0, max_batch_cells=1 << 38):
         L = load_library()
-        cfg = ServerConfig(ordinal, hbm_budget, max_batch_reads, max_batch_bases, max_slots)
+        cfg = ServerConfig(ordinal, hbm_budget, max_batch_reads, max_batch_bases, max_slots,
+                           max_batch_cells)
         out = C.c_void_p()
         _check(L.esch_server_new(C.byref(cfg), C.byref(out)))
         self._h = out

@@ -70,6 +70,10 @@ struct esch_scorer {
     size_t map_chunk = 0;            // reads per mapping chunk (0: by the rows' memory)
 };
 
+// Tie capacity (tie indices per read) a batch allocates when it is created;
+// a run asking for more reallocates, which waits for the device.
+constexpr size_t kTieCapAlloc = 8;
+
 struct esch_batch {
     esch_scorer *s = nullptr;
     size_t max_reads = 0, max_bases = 0;  // capacity before strand expansion

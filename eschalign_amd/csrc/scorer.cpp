@@ -321,6 +321,7 @@ size_t esch_batch_footprint(const esch_scorer *s, size_t max_reads, size_t max_b
     f += (size_t)s->geo.grid32 * s->geo.block32 * s->max_len * sizeof(int2);  // scratch32
     f += rows * 5 * 4;                                                   // records
     f += rows * 4 + diag_scratch_bytes(s);                               // anti-diagonal
+    f += std::max<size_t>(max_reads, 1) * kTieCapAlloc * 4;              // ties
     return f;
 }
 
@@ -355,6 +356,8 @@ esch_status esch_batch_create(esch_scorer *s, size_t max_reads, size_t max_bases
     HIPC(hipMalloc(&b->d_listD, rows * 4));
     HIPC(hipMalloc(&b->d_scrD, diag_scratch_bytes(s)));
     HIPC(hipMalloc(&b->d_desc, sizeof(DevDesc)));
+    HIPC(hipMalloc(&b->d_ties, b->max_reads * kTieCapAlloc * 4));
+    b->tie_cap_alloc = kTieCapAlloc;
     HIPC(hipHostMalloc(&b->h_codes, bases, hipHostMallocDefault));
     HIPC(hipHostMalloc(&b->h_qoff, rows * 8, hipHostMallocDefault));
     HIPC(hipHostMalloc(&b->h_qlen, rows * 4, hipHostMallocDefault));
@@ -567,7 +570,7 @@ esch_status esch_batch_run(esch_batch *b, int what, int32_t min_score, int64_t m
         if (tie_cap == 0) tie_cap = 1;
         if (tie_cap > b->tie_cap_alloc) {
             HIPC(hipStreamSynchronize(b->stream));
-            (void)hipFree(b->d_ties);
+            if (b->d_ties) (void)hipFree(b->d_ties);
             b->d_ties = nullptr;
             HIPC(hipMalloc(&b->d_ties, std::max<size_t>(b->max_reads, 1) * tie_cap * 4));
             b->tie_cap_alloc = tie_cap;
@@ -695,9 +698,14 @@ esch_status esch_batch_read_select(esch_batch *b, esch_read_select *out, int32_t
     if (!(b->last_what & ESCH_RUN_SELECT)) return fail(ESCH_INVALID_ARG, "last run did not select");
     HIPC(hipSetDevice(b->s->ordinal));
     HIPC(hipStreamSynchronize(b->stream));
+    // copies ordered on the batch's own stream: they wait for nothing that
+    // other streams have in flight
     const size_t n = b->n_reads;
-    if (out && n) HIPC(hipMemcpy(out, b->d_recs, n * sizeof(esch_read_select), hipMemcpyDeviceToHost));
-    if (ties && n) HIPC(hipMemcpy(ties, b->d_ties, n * b->h_desc->tie_cap * 4, hipMemcpyDeviceToHost));
+    if (out && n)
+        HIPC(hipMemcpyAsync(out, b->d_recs, n * sizeof(esch_read_select), hipMemcpyDeviceToHost, b->stream));
+    if (ties && n)
+        HIPC(hipMemcpyAsync(ties, b->d_ties, n * b->h_desc->tie_cap * 4, hipMemcpyDeviceToHost, b->stream));
+    HIPC(hipStreamSynchronize(b->stream));
     return ESCH_OK;
 }
 
@@ -707,8 +715,11 @@ esch_status esch_batch_read_pairs(esch_batch *b, esch_pair_select *out, int32_t 
     HIPC(hipSetDevice(b->s->ordinal));
     HIPC(hipStreamSynchronize(b->stream));
     const size_t n = b->n_reads / 2;
-    if (out && n) HIPC(hipMemcpy(out, b->d_recs, n * sizeof(esch_pair_select), hipMemcpyDeviceToHost));
-    if (top && n) HIPC(hipMemcpy(top, b->d_ties, n * b->h_desc->tie_cap * 4, hipMemcpyDeviceToHost));
+    if (out && n)
+        HIPC(hipMemcpyAsync(out, b->d_recs, n * sizeof(esch_pair_select), hipMemcpyDeviceToHost, b->stream));
+    if (top && n)
+        HIPC(hipMemcpyAsync(top, b->d_ties, n * b->h_desc->tie_cap * 4, hipMemcpyDeviceToHost, b->stream));
+    HIPC(hipStreamSynchronize(b->stream));
     return ESCH_OK;
 }
 

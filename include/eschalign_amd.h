@@ -427,11 +427,14 @@ esch_status esch_batch_read_pairs(esch_batch *b, esch_pair_select *out, int32_t 
 
 /* ---------------------------------------------------------------------
  * Serving: device-resident panels and a continuous-batching request loop
- * (no counterpart in the reference). A scheduler thread packs queued
- * requests that share a panel and options into device batches as slots
- * free up, keeps up to max_slots batches in flight on their own streams,
- * and creates a slot only while the summed footprint of all slots stays
- * within hbm_budget. Every result equals the request run alone.
+ * (no counterpart in the reference). Requests are cut into units of at
+ * most one batch (max_batch_reads, max_batch_bases, max_batch_cells). A
+ * scheduler thread packs queued units that share a panel and options into
+ * device batches, the units of the request with the least queued work
+ * first, and keeps up to max_slots batches in flight on their own streams.
+ * Idle slots are kept for reuse and released, least recently used first,
+ * only when a new slot would not fit hbm_budget. Every result equals the
+ * request run alone.
  * --------------------------------------------------------------------- */
 typedef struct esch_server esch_server;
 typedef struct {
@@ -440,6 +443,10 @@ typedef struct {
     size_t max_batch_reads;  /* reads per device batch (before strands) */
     size_t max_batch_bases;  /* bases per device batch */
     int max_slots;           /* batches in flight at once */
+    size_t max_batch_cells;  /* DP cells per device batch (read bases x strands x panel
+                                bases; 0: no limit): a larger request is cut into batches
+                                of at most this much work, so smaller requests run between
+                                them (a single read over the limit is a batch of its own) */
 } esch_server_config;
 typedef struct {
     uint64_t completed;      /* requests finished */

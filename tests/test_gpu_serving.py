@@ -16,7 +16,10 @@ from oracle import pyoracle as O
 pytestmark = pytest.mark.gpu
 
 
-def test_served_results_equal_alone_results():
+@pytest.mark.parametrize("cells", [None, 300_000_000])
+def test_served_results_equal_alone_results(cells):
+    # cells: a DP-cell limit per device batch small enough to cut most
+    # requests into many units of length-sorted reads
     scoring = ea.Scoring(1, -1, -2, -1)
     mode = ea.Mode.SEMIGLOBAL
     panels = [synth.panel(seed=11 + k, n_refs=n, lo=140, hi=400) for k, n in enumerate((40, 90, 160))]
@@ -25,7 +28,8 @@ def test_served_results_equal_alone_results():
     slot = probe.batch_footprint(4000, 600_000)
     del probe
     budget = int(2.5 * slot)
-    server = ea.Server(hbm_budget=budget, max_batch_reads=4000, max_batch_bases=600_000, max_slots=3)
+    kw = {} if cells is None else {"max_batch_cells": cells}
+    server = ea.Server(hbm_budget=budget, max_batch_reads=4000, max_batch_bases=600_000, max_slots=3, **kw)
     ids = [server.add_panel(ea.Panel(p), scoring, mode) for p in panels]
     alone = [ea.GpuScorer(ea.Panel(p), scoring, mode) for p in panels]
 
@@ -57,6 +61,8 @@ def test_served_results_equal_alone_results():
     assert st["completed"] == len(requests)
     assert st["hbm_peak"] <= budget
     assert st["batches"] >= 1
+    if cells is not None:
+        assert st["batches"] > 2 * len(requests)
     print(f"[serving] {len(requests)} requests in {wall:.2f} s, {st['batches']} batches, "
           f"p50 {np.percentile(lat, 50) * 1e3:.1f} ms, p99 {np.percentile(lat, 99) * 1e3:.1f} ms, "
           f"peak HBM {st['hbm_peak'] / 2**30:.2f} GiB of {budget / 2**30:.2f}")
@@ -73,4 +79,33 @@ def test_served_results_equal_alone_results():
         b, nt, so, take = O.select(sm[k], sm[n + k], opts["min_score"], opts["max_ties"])
         r = results[0][k]
         assert (r.best, r.n_tied, r.suboptimal, r.take) == (b, nt, so, take[:8])
+    server.close()
+
+
+def test_small_request_overtakes_large_one():
+    """A request with little work, submitted after a large one, finishes
+    first: units of the request with the least queued work go first."""
+    scoring = ea.Scoring(1, -1, -2, -1)
+    mode = ea.Mode.SEMIGLOBAL
+    panel = synth.panel(seed=23, n_refs=300, lo=140, hi=400)
+    server = ea.Server(hbm_budget=16 << 30, max_batch_reads=4000, max_batch_bases=600_000, max_slots=2,
+                       max_batch_cells=500_000_000)
+    pid = server.add_panel(ea.Panel(panel), scoring, mode)
+    big = synth.reads(panel, seed=5, n=6000, length=("lognormal", 90, 0.6, 20, 500))
+    small = synth.reads(panel, seed=6, n=5, length=("lognormal", 90, 0.6, 20, 500))
+    t_big = time.perf_counter()
+    r_big = server.submit(pid, big, both_strands=True, tie_cap=4)
+    t_small = time.perf_counter()
+    r_small = server.submit(pid, small, both_strands=True, tie_cap=4)
+    assert server.wait(r_small, timeout_ms=120_000)
+    assert server.wait(r_big, timeout_ms=120_000)
+    res_small, l_small = server.result(r_small)
+    res_big, l_big = server.result(r_big)
+    st = server.stats()
+    print(f"[serving] large {l_big * 1e3:.1f} ms, small {l_small * 1e3:.1f} ms, {st['batches']} batches")
+    assert st["batches"] > 50
+    assert t_small + l_small < t_big + l_big
+    alone = ea.GpuScorer(ea.Panel(panel), scoring, mode)
+    assert res_small == alone.select_batch(small, both_strands=True, tie_cap=4)
+    assert res_big == alone.select_batch(big, both_strands=True, tie_cap=4)
     server.close()
