@@ -8,11 +8,14 @@ through 5% substitution / 3% insertion / 3% deletion errors, 10% random;
 both strands, semiglobal mode, scoring 1,-1,-2,-1. Synthetic data.
 
 A step is one pass of the device pipeline over the batch, already resident
-in HBM: length bucketing, the batched packed-int16 DP (int32 rescue where
-needed) and the fused best / tie selection. value = reads per second.
+in HBM: length bucketing, the batched packed 16-bit DP (in fp16 on panels
+whose value bound fits its exact integers, as this one does; int16 and
+int32 rescue where needed) and the fused best / tie selection. value =
+reads per second.
 
 Also reported: the DP kernel's time from HIP events on its stream, GCUPS and
-the fraction of the int16 VALU lane-op roofline; the plain one-thread-per-
+the fraction of the packed 16-bit VALU lane-op roofline at the kernel's own
+minimum op count per cell; the plain one-thread-per-
 pair int32 kernel on the same batch; the reference's CPU kernel (the
 oracle's AVX-512/AVX2 restatement) on a bounded sample on the host cores;
 the winner tracebacks of the step's selection; map_reads_scored whole on
@@ -36,10 +39,17 @@ import numpy as np  # noqa: E402
 N_CU, CLOCK_HZ = 256, 2.4e9
 PEAK_I16 = N_CU * CLOCK_HZ * 128           # lane-ops/s
 PEAK_I32 = N_CU * CLOCK_HZ * 64
-OPS_PER_CELL = {"local": 10, "semiglobal": 8}
+# Minimum packed 16-bit lane-ops per DP cell of the recurrence: 4 adds and
+# 4 two-input maxima in semiglobal mode, plus the zero floor and the running
+# best in local mode; with the fp16 variant's three-input maximum (variant 4)
+# H takes one max instead of two, and the column best a three-way tree.
+OPS_PER_CELL = {("semiglobal", False): 8, ("local", False): 10,
+                ("semiglobal", True): 7, ("local", True): 8.5}
+F16_VARIANT = 4
 # SQ_INSTS_VALU x 128 / cells of one k_dp16 launch of the default workload,
 # from the counter summary in profiles/ (mode, variant) -> lane-ops per cell
-VALU_OPS_PER_CELL_ISSUED = {("semiglobal", 3): round(345_033_311_192 * 128 / 5_303_272_453_076, 3)}
+VALU_OPS_PER_CELL_ISSUED = {("semiglobal", 3): round(345_033_311_192 * 128 / 5_303_272_453_076, 3),
+                            ("semiglobal", 4): round(304_396_230_832 * 128 / 5_303_272_453_076, 3)}
 
 
 def log(*a):
@@ -529,7 +539,8 @@ def main():
     value = args.reads / (wall / args.steps)
     dp = float(np.mean(dp_ms)) / 1e3
     gcups = cells / dp / 1e9
-    ops = OPS_PER_CELL[args.mode]
+    f16 = chosen == F16_VARIANT
+    ops = OPS_PER_CELL[(args.mode, f16)]
     achieved = cells * ops / dp
     log(f"[bench] {value:,.0f} reads/s, {ms_step:.1f} ms/step, DP {dp * 1e3:.1f} ms "
         f"({gcups:,.0f} GCUPS), {n16} int16 / {n_diag} anti-diagonal / {n32} int32 queries")
@@ -596,7 +607,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "i16x2 (int32 rescue)",
+        "dtype": "f16x2 (integer-exact; int16 / int32 rescue)" if f16 else "i16x2 (int32 rescue)",
         "data": "synthetic",
         "config": {"workload": "batched uneven lengths: 164-ref panel (140-160 nt), "
                                f"{args.reads} reads {describe_lengths(length)}, "
@@ -609,14 +620,16 @@ def main():
         "dp_ms": round(dp * 1e3, 3),
         "cells": int(cells),
         "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3),
-                     "peak": round(PEAK_I16 / 1e12, 3), "unit": "Tlane-op/s (int16)",
+                     "peak": round(PEAK_I16 / 1e12, 3), "unit": "Tlane-op/s (packed 16-bit)",
                      "frac": round(achieved / PEAK_I16, 4), "traffic": traffic,
                      "peak_measured": round(peak["lane_ops_per_s"] / 1e12, 3),
                      "frac_of_measured": round(achieved / peak["lane_ops_per_s"], 4),
                      "valu_lane_ops_per_cell_issued": VALU_OPS_PER_CELL_ISSUED.get((args.mode, chosen)),
                      "traffic_unit": "B per k_dp16 launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE)",
                      "traffic_source": traffic_src,
-                     "ops_per_cell": ops},
+                     "ops_per_cell": ops,
+                     # cells/s against the int16 kernel's bound (8 / 10 ops per cell)
+                     "frac_of_int16_roofline": round(cells * OPS_PER_CELL[(args.mode, False)] / dp / PEAK_I16, 4)},
         "host_load": {"ms": round(load_s * 1e3, 3), "bytes": int(lens.sum()),
                       "pcie_inclusive_reads_per_s": round(args.reads / (load_s + wall / args.steps), 1)},
         "kernel_variant": chosen,

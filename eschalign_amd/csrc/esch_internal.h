@@ -22,6 +22,16 @@ constexpr int kMaxLen16 = 65536;
 // and -32768 (saturating) is the only -inf, so a bound below 32767 keeps
 // every real value away from both clamps.
 constexpr int64_t kBound16 = 32000;
+// The same bound for k_dp16's fp16 variant: fp16 holds every integer of
+// magnitude up to 2048 exactly (11 significant bits), and -inf exists, so a
+// panel whose value bound, at k = its longest reference, is at most 2048
+// scores every read exactly in fp16 arithmetic.
+constexpr int64_t kBoundF16 = 2048;
+__host__ __device__ inline bool fits_f16(int64_t max_ref_len, int32_t ma, int32_t mi, int32_t o, int32_t e) {
+    const int64_t M = ma < 0 ? -(int64_t)ma : ma, Mi = mi < 0 ? -(int64_t)mi : mi;
+    const int64_t O = o < 0 ? -(int64_t)o : o, Ee = e < 0 ? -(int64_t)e : e;
+    return max_ref_len * (M > Mi ? M : Mi) + O + Ee <= kBoundF16;
+}
 // -inf of the int32 kernel, the oracle's i32::MIN / 4 (lib/scalar.rs:42).
 constexpr int32_t kNeg32 = INT32_MIN / 4;
 
@@ -105,12 +115,14 @@ struct Launch {
     int grid32;   // persistent blocks of k_dp32
     int block32;  // threads per k_dp32 block
     int grid_aux; // blocks of the bucketing / selection kernels
-    int variant16;  // k_dp16 variant: 0 profile per column, 1 ping-pong, 2 pair schedule, 3 quad schedule
+    int variant16;  // k_dp16 variant: 0 profile per column, 1 ping-pong, 2 pair schedule, 3 quad schedule,
+                    // 4 quad schedule in fp16 (panels within kBoundF16 only)
     int grid_diag;  // persistent waves of the anti-diagonal kernel
     size_t diag_lds;  // k_diag64's LDS boundary bytes; 0: k_diag16 (global scratch)
 };
-constexpr int kDp16Variants = 4;
-constexpr int kDp16Default = 3;
+constexpr int kDp16Variants = 5;
+constexpr int kDp16Default = 3;  // the int16 default; panels within kBoundF16 take kDp16F16
+constexpr int kDp16F16 = 4;
 hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st);
 hipError_t launch_dp16(const KArgs &a, const Launch &g, bool local, hipStream_t st);
 hipError_t launch_dp32(const KArgs &a, const Launch &g, bool local, int which, hipStream_t st);

@@ -35,6 +35,8 @@ using esch::Launch;
 using esch::Scoring;
 using esch::kDiagRatioDefault;
 using esch::kDp16Variants;
+using esch::kDp16F16;
+using esch::fits_f16;
 
 struct TraceCache;
 struct MapCache;
@@ -57,6 +59,7 @@ struct esch_scorer {
     int regs16_v[kDp16Variants] = {}, local16_v[kDp16Variants] = {};
     int max_grid16 = 0;                 // scratch is sized for the largest
     int regs16 = 0, local16 = 0;
+    bool f16_ok = false;             // the panel fits k_dp16's fp16 variant (fits_f16)
     int diag_min_len = 0;               // reads this long take k_diag16 (0: never)
     int diag_ratio = kDiagRatioDefault; // padding-heavy length bins take k_diag16 (0: never)
     bool diag_ratio_tunable = true;     // false once the caller set the ratio

@@ -27,6 +27,51 @@ __device__ __forceinline__ uint32_t as_u32(v2s v) { return __builtin_bit_cast(ui
 __device__ __forceinline__ v2s as_v2s(uint32_t v) { return __builtin_bit_cast(v2s, v); }
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+typedef _Float16 v2h __attribute__((ext_vector_type(2)));
+
+// Packed arithmetic of k_dp16: two int16 lanes with saturating adds (-32768
+// is -inf), or, for a panel whose value bound fits the integers fp16 holds
+// exactly (kBoundF16), two fp16 lanes (-inf is -inf, and the max of three
+// is one v_pk_maximum3_f16 where int16 needs two v_pk_max_i16).
+template <bool F16>
+struct Pk {
+    using T = v2s;
+    static __device__ __forceinline__ T add(T a, T b) { return __builtin_elementwise_add_sat(a, b); }
+    static __device__ __forceinline__ T max(T a, T b) { return __builtin_elementwise_max(a, b); }
+    static __device__ __forceinline__ T max3(T a, T b, T c) { return max(max(a, b), c); }
+    static __device__ __forceinline__ T splat(int v) { return T{(short)v, (short)v}; }
+    static __device__ __forceinline__ T neg() { return splat(-32768); }
+    static __device__ __forceinline__ T from(uint32_t v) { return __builtin_bit_cast(T, v); }
+    static __device__ __forceinline__ uint32_t bits(T v) { return __builtin_bit_cast(uint32_t, v); }
+    static __device__ __forceinline__ uint32_t enc(int v) { return (uint32_t)(uint16_t)(int16_t)v; }
+    static __device__ __forceinline__ void neg_lo(T &v) { v.x = (short)-32768; }
+    static __device__ __forceinline__ void neg_hi(T &v) { v.y = (short)-32768; }
+    static __device__ __forceinline__ int lo(T v) { return v.x; }
+    static __device__ __forceinline__ int hi(T v) { return v.y; }
+};
+
+template <>
+struct Pk<true> {
+    using T = v2h;
+    static __device__ __forceinline__ T add(T a, T b) { return a + b; }
+    // IEEE maximum: no NaN can arise, so no quieting ops are needed
+    static __device__ __forceinline__ T max(T a, T b) { return __builtin_elementwise_maximum(a, b); }
+    static __device__ __forceinline__ T max3(T a, T b, T c) {
+        return __builtin_elementwise_maximum(__builtin_elementwise_maximum(a, b), c);
+    }
+    static __device__ __forceinline__ T splat(int v) { return T{(_Float16)v, (_Float16)v}; }
+    static __device__ __forceinline__ T neg() { return from(0xFC00FC00u); }
+    static __device__ __forceinline__ T from(uint32_t v) { return __builtin_bit_cast(T, v); }
+    static __device__ __forceinline__ uint32_t bits(T v) { return __builtin_bit_cast(uint32_t, v); }
+    static __device__ __forceinline__ uint32_t enc(int v) {
+        return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v);
+    }
+    static __device__ __forceinline__ void neg_lo(T &v) { v = from((bits(v) & 0xFFFF0000u) | 0xFC00u); }
+    static __device__ __forceinline__ void neg_hi(T &v) { v = from((bits(v) & 0x0000FFFFu) | 0xFC000000u); }
+    static __device__ __forceinline__ int lo(T v) { return (int)(float)v.x; }
+    static __device__ __forceinline__ int hi(T v) { return (int)(float)v.y; }
+};
+
 // ---------------------------------------------------------------------------
 // k_dp16 — batched packed-int16 DP.
 //
@@ -71,20 +116,53 @@ __device__ __forceinline__ uint32_t pack2(int lo, int hi) {
     return (uint32_t)(uint16_t)(int16_t)lo | ((uint32_t)(uint16_t)(int16_t)hi << 16);
 }
 
-template <int RR, bool LOCAL>
-__device__ __forceinline__ void column16(const v2s (&sc)[RR], v2s (&Ho)[RR], v2s (&E)[RR],
-                                         v2s &hou, v2s &f, v2s hod, v2s vo, v2s ve, v2s &best) {
+// The maximum of W packed values by three-way maxima, a level at a time;
+// a level's leftovers (W mod 3) carry up: W / 2 ops.
+template <bool F16, int W>
+__device__ __forceinline__ typename Pk<F16>::T max3_all(const typename Pk<F16>::T (&t)[W]) {
+    using P = Pk<F16>;
+    if constexpr (W == 1) {
+        return t[0];
+    } else if constexpr (W == 2) {
+        return P::max(t[0], t[1]);
+    } else {
+        constexpr int NW = W / 3 + W % 3;
+        typename P::T u[NW];
+#pragma unroll
+        for (int i = 0; i < W / 3; ++i) u[i] = P::max3(t[3 * i], t[3 * i + 1], t[3 * i + 2]);
+#pragma unroll
+        for (int i = 0; i < W % 3; ++i) u[W / 3 + i] = t[3 * (W / 3) + i];
+        return max3_all<F16, NW>(u);
+    }
+}
+
+template <int RR, bool LOCAL, bool F16 = false>
+__device__ __forceinline__ void column16(const typename Pk<F16>::T (&sc)[RR], typename Pk<F16>::T (&Ho)[RR],
+                                         typename Pk<F16>::T (&E)[RR], typename Pk<F16>::T &hou,
+                                         typename Pk<F16>::T &f, typename Pk<F16>::T hod,
+                                         typename Pk<F16>::T vo, typename Pk<F16>::T ve,
+                                         typename Pk<F16>::T &best) {
+    using P = Pk<F16>;
+    using T = typename P::T;
 #pragma unroll
     for (int r = 0; r < RR; ++r) {
-        E[r] = vmax(Ho[r], adds(E[r], ve));
-        f = vmax(hou, adds(f, ve));
-        v2s h = vmax(vmax(adds(hod, sc[r]), E[r]), f);
-        if constexpr (LOCAL) h = vmax(h, splat(0));
+        E[r] = P::max(Ho[r], P::add(E[r], ve));
+        f = P::max(hou, P::add(f, ve));
+        T h = P::max3(P::add(hod, sc[r]), E[r], f);
+        if constexpr (LOCAL) h = P::max(h, P::splat(0));
         hod = Ho[r];
-        Ho[r] = adds(h, vo);
+        Ho[r] = P::add(h, vo);
         hou = Ho[r];
     }
-    if constexpr (LOCAL) {
+    if constexpr (LOCAL && F16) {
+        // The column's best cell folded into `best` by a tree of three-way
+        // maxima (RR / 2 ops for RR rows).
+        T t[RR + 1];
+#pragma unroll
+        for (int r = 0; r < RR; ++r) t[r] = Ho[r];
+        t[RR] = best;
+        best = max3_all<F16, RR + 1>(t);
+    } else if constexpr (LOCAL) {
         // The column's best cell, in Ho = H + o space, as a depth-4 tree
         // (same 16 max ops as a running max, without a serial chain).
         v2s t[RR / 2];
@@ -202,15 +280,18 @@ __device__ __forceinline__ void stripe16(const uint4 *__restrict__ prof, uint2 *
 // to m + 1), the profile prefetch reads at most the reference's padding
 // word (column m), and whether the stripe is the first (border instead of
 // scratch) or the last (no stores) is a template argument.
-template <int RR, bool LOCAL, bool FIRST, bool LAST, bool QUAD>
+template <int RR, bool LOCAL, bool FIRST, bool LAST, bool QUAD, bool F16>
 __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 *__restrict__ scr,
                                           const int lane, const cwords w2, const cwords wn,
                                           const int m, const int row0, const int pA, const int pB,
-                                          const v2s vo, const v2s ve, v2s &best) {
+                                          const typename Pk<F16>::T vo, const typename Pk<F16>::T ve,
+                                          typename Pk<F16>::T &best) {
     static_assert(RR % 2 == 0 && RR >= 2 && RR <= 16, "an even number of rows");
+    using P = Pk<F16>;
+    using v2s = typename P::T;  // the packed value type of this instance
     constexpr int NQ = RR / 4;          // whole quads of rows
     constexpr bool HALF = RR % 4 == 2;  // and the first two rows of one more
-    const v2s NEG = splat(-32768);
+    const v2s NEG = P::neg();
     v2s Ho[RR], E[RR];
 #pragma unroll
     for (int r = 0; r < RR; ++r) {
@@ -227,30 +308,30 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const uint4 v = pc[q * 64];
-            sc[4 * q + 0] = as_v2s(v.x);
-            sc[4 * q + 1] = as_v2s(v.y);
-            sc[4 * q + 2] = as_v2s(v.z);
-            sc[4 * q + 3] = as_v2s(v.w);
+            sc[4 * q + 0] = P::from(v.x);
+            sc[4 * q + 1] = P::from(v.y);
+            sc[4 * q + 2] = P::from(v.z);
+            sc[4 * q + 3] = P::from(v.w);
         }
         if constexpr (HALF) {
             const uint2 v = *reinterpret_cast<const uint2 *>(pc + NQ * 64);
-            sc[4 * NQ + 0] = as_v2s(v.x);
-            sc[4 * NQ + 1] = as_v2s(v.y);
+            sc[4 * NQ + 0] = P::from(v.x);
+            sc[4 * NQ + 1] = P::from(v.y);
         }
     };
     uint2 *const s0 = scr + 2 * 64 + lane;  // column 0 (two padding columns in front)
-    const uint2 border = make_uint2(as_u32(vo), as_u32(NEG));
+    const uint2 border = make_uint2(P::bits(vo), P::bits(NEG));
     auto bnd = [&](int j) -> uint2 {
         if constexpr (FIRST) return border;
         else return s0[(size_t)j * 64];
     };
     auto col = [&](const v2s (&sc)[RR], uint2 b) -> uint2 {
-        v2s hou = as_v2s(b.x), f = as_v2s(b.y);
+        v2s hou = P::from(b.x), f = P::from(b.y);
         const v2s hod = hdt;
         hdt = hou;
-        column16<RR, LOCAL>(sc, Ho, E, hou, f, hod, vo, ve, best);
-        if constexpr (!LOCAL && LAST) best = vmax(best, hou);
-        return make_uint2(as_u32(hou), as_u32(f));
+        column16<RR, LOCAL, F16>(sc, Ho, E, hou, f, hod, vo, ve, best);
+        if constexpr (!LOCAL && LAST) best = P::max(best, hou);
+        return make_uint2(P::bits(hou), P::bits(f));
     };
 
     if constexpr (!QUAD) {
@@ -347,20 +428,21 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
 #pragma unroll
         for (int r = 0; r < RR; ++r) {
             v2s v = Ho[r];
-            if (row0 + r < pA) v.x = (short)-32768;
-            if (row0 + r < pB) v.y = (short)-32768;
-            best = vmax(best, v);
+            if (row0 + r < pA) P::neg_lo(v);
+            if (row0 + r < pB) P::neg_hi(v);
+            best = P::max(best, v);
         }
     }
 }
 
-template <int RR, bool LOCAL, bool FIRST, bool QUAD>
+template <int RR, bool LOCAL, bool FIRST, bool QUAD, bool F16>
 __device__ __forceinline__ void stripe16d_any(const uint4 *prof, uint2 *scr, const int lane,
                                               const cwords w2, const cwords wn, const int m,
                                               const bool last, const int row0, const int pA,
-                                              const int pB, const v2s vo, const v2s ve, v2s &best) {
-    if (last) stripe16d<RR, LOCAL, FIRST, true, QUAD>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best);
-    else stripe16d<RR, LOCAL, FIRST, false, QUAD>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best);
+                                              const int pB, const typename Pk<F16>::T vo,
+                                              const typename Pk<F16>::T ve, typename Pk<F16>::T &best) {
+    if (last) stripe16d<RR, LOCAL, FIRST, true, QUAD, F16>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best);
+    else stripe16d<RR, LOCAL, FIRST, false, QUAD, F16>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best);
 }
 
 template <bool LOCAL, int PF>
@@ -377,7 +459,11 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
     const int n_chunks = (n16 + kChunk - 1) / kChunk;
     const uint32_t n_items = (uint32_t)n_chunks * (uint32_t)n_groups;
     uint2 *scr = a.scratch16 + ((size_t)blockIdx.x * kWaves16 + wv) * (size_t)(P.max_len + kScrPad) * 64;
-    const v2s vo = splat(a.sc.o), ve = splat(a.sc.e), NEG = splat(-32768);
+    // variant 4: variant 3's schedule in fp16 arithmetic
+    constexpr bool F16 = PF == 4;
+    using PK = Pk<F16>;
+    using V = typename PK::T;
+    const V vo = PK::splat(a.sc.o), ve = PK::splat(a.sc.e), NEG = PK::neg();
 
     for (;;) {
         __syncthreads();
@@ -414,7 +500,7 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
         const cwords w2 = (cwords)(P.ref2 + uni((int)P.w2off[ref]));
         const cwords wn = (cwords)(P.refN + uni((int)P.nmoff[ref]));
 
-        v2s best = LOCAL ? vo : NEG;  // Ho = H + o space (local starts at H = 0)
+        V best = LOCAL ? vo : NEG;  // Ho = H + o space (local starts at H = 0)
 
         for (int s = 0; s < S; ++s) {
             const int row0 = s == 0 ? 0 : RR + kStripe * (s - 1);
@@ -436,9 +522,9 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
                         const int va = prof_val(xa, c, a.sc), vb = prof_val(xb, c, a.sc);
                         bestA = max(bestA, va);
                         bestB = max(bestB, vb);
-                        pl[c][rr] = pack2(va, vb);
+                        pl[c][rr] = PK::enc(va) | (PK::enc(vb) << 16);
                     }
-                    pl[4][rr] = pack2(bestA, bestB);
+                    pl[4][rr] = PK::enc(bestA) | (PK::enc(bestB) << 16);
                 }
 #pragma unroll
                 for (int c = 0; c < 5; ++c)
@@ -448,16 +534,16 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
             if (!active) continue;
             const bool first = s == 0, last = s + 1 == S;
             if constexpr (PF >= 2) {
-                constexpr bool Q = PF == 3;
+                constexpr bool Q = PF >= 3;
                 if (s > 0) {
-                    stripe16d_any<16, LOCAL, false, Q>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
+                    stripe16d_any<16, LOCAL, false, Q, F16>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
                 } else {
                     switch (RR) {
 #define ESCH_FIRST(H) \
-    case H: stripe16d_any<H, LOCAL, true, Q>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best); break;
+    case H: stripe16d_any<H, LOCAL, true, Q, F16>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best); break;
                         ESCH_FIRST(16) ESCH_FIRST(14) ESCH_FIRST(12) ESCH_FIRST(10)
                         ESCH_FIRST(8) ESCH_FIRST(6) ESCH_FIRST(4)
-                        default: stripe16d_any<2, LOCAL, true, Q>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
+                        default: stripe16d_any<2, LOCAL, true, Q, F16>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
 #undef ESCH_FIRST
                     }
                 }
@@ -478,7 +564,7 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
             (void)first;
         }
         if (!active) continue;
-        const int bA = best.x - a.sc.o, bB = best.y - a.sc.o;
+        const int bA = PK::lo(best) - a.sc.o, bB = PK::hi(best) - a.sc.o;
         if (qa >= 0) {
             const size_t o = (size_t)qa * n_refs + ref;
             if (a.out32) a.out32[o] = bA;
@@ -1477,6 +1563,7 @@ hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st) {
 // Kernel variants of k_dp16, chosen per scorer (set by the autotuner).
 template <bool LOCAL>
 static const void *dp16_fn(int variant) {
+    if (variant == 4) return reinterpret_cast<const void *>(&k_dp16<LOCAL, 4>);
     if (variant == 3) return reinterpret_cast<const void *>(&k_dp16<LOCAL, 3>);
     if (variant == 2) return reinterpret_cast<const void *>(&k_dp16<LOCAL, 2>);
     if (variant == 1) return reinterpret_cast<const void *>(&k_dp16<LOCAL, 1>);
@@ -1487,12 +1574,14 @@ hipError_t launch_dp16(const KArgs &a, const Launch &g, bool local, hipStream_t 
     const int v = g.variant16;
     const dim3 grid(g.grid16), block(64 * kWaves16);
     if (local) {
-        if (v == 3) hipLaunchKernelGGL((k_dp16<true, 3>), grid, block, 0, st, a);
+        if (v == 4) hipLaunchKernelGGL((k_dp16<true, 4>), grid, block, 0, st, a);
+        else if (v == 3) hipLaunchKernelGGL((k_dp16<true, 3>), grid, block, 0, st, a);
         else if (v == 2) hipLaunchKernelGGL((k_dp16<true, 2>), grid, block, 0, st, a);
         else if (v == 1) hipLaunchKernelGGL((k_dp16<true, 1>), grid, block, 0, st, a);
         else hipLaunchKernelGGL((k_dp16<true, 0>), grid, block, 0, st, a);
     } else {
-        if (v == 3) hipLaunchKernelGGL((k_dp16<false, 3>), grid, block, 0, st, a);
+        if (v == 4) hipLaunchKernelGGL((k_dp16<false, 4>), grid, block, 0, st, a);
+        else if (v == 3) hipLaunchKernelGGL((k_dp16<false, 3>), grid, block, 0, st, a);
         else if (v == 2) hipLaunchKernelGGL((k_dp16<false, 2>), grid, block, 0, st, a);
         else if (v == 1) hipLaunchKernelGGL((k_dp16<false, 1>), grid, block, 0, st, a);
         else hipLaunchKernelGGL((k_dp16<false, 0>), grid, block, 0, st, a);

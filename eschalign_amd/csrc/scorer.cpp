@@ -208,10 +208,12 @@ esch_status esch_scorer_new(int ordinal, const uint8_t *const *refs, const size_
     HIPC(dp32_occupancy(local, block32, &b32));
     // default: the deep-prefetch variant (the fastest on the measured
     // workloads); esch_autotune re-decides on the caller's reads
-    s->geo.variant16 = kDp16Default;
-    s->geo.grid16 = s->grid16_v[kDp16Default];
-    s->regs16 = s->regs16_v[kDp16Default];
-    s->local16 = s->local16_v[kDp16Default];
+    s->f16_ok = fits_f16((int64_t)max_len, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e);
+    const int v0 = s->f16_ok ? kDp16F16 : kDp16Default;
+    s->geo.variant16 = v0;
+    s->geo.grid16 = s->grid16_v[v0];
+    s->regs16 = s->regs16_v[v0];
+    s->local16 = s->local16_v[v0];
     s->geo.block32 = block32;
     s->geo.grid32 = std::max(1, b32) * s->n_cu;
     s->geo.grid_aux = 4 * s->n_cu;
@@ -855,6 +857,11 @@ esch_status esch_score_batch(esch_scorer *s, const uint8_t *const *queries, cons
 esch_status esch_set_variant(esch_scorer *s, int variant) {
     if (!s || variant < 0 || variant >= kDp16Variants)
         return fail(ESCH_INVALID_ARG, "variant %d outside 0..%d", variant, kDp16Variants - 1);
+    if (variant == kDp16F16 && !s->f16_ok)
+        return fail(ESCH_INVALID_ARG,
+                    "variant %d (fp16) needs longest reference x max(|match|, |mismatch|) + |open| + "
+                    "|extend| <= %lld",
+                    variant, (long long)esch::kBoundF16);
     s->geo.variant16 = variant;
     s->geo.grid16 = s->grid16_v[variant];
     s->regs16 = s->regs16_v[variant];
@@ -904,6 +911,10 @@ esch_status esch_autotune(esch_scorer *s, const uint8_t *const *reads, const siz
     int best_v = s->geo.variant16;
     float best_ms = 0.0f;
     for (int v = 0; !st && v < kDp16Variants; ++v) {
+        if (v == kDp16F16 && !s->f16_ok) {
+            if (ms) ms[v] = -1.0f;  // not applicable to this panel
+            continue;
+        }
         st = esch_set_variant(s, v);
         float dp = 0.0f;
         if (!st) st = timed(&dp);

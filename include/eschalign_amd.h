@@ -362,14 +362,23 @@ size_t esch_cigar_string(const uint32_t *ops, size_t n_ops, size_t query_len, si
  * reference; the scores are identical whichever variant runs):
  *   0  profile plane loaded in the column that uses it (fewest registers);
  *   1  profile planes ping-ponged, stripe boundary prefetched;
- *   2  (default) every memory access of a column pair issued at its start,
- *      stores deferred by one pair, no data-dependent branch around a
- *      memory access. */
+ *   2  every memory access of a column pair issued at its start, stores
+ *      deferred by one pair, no data-dependent branch around a memory
+ *      access;
+ *   3  (default) columns in quads: two pairs whose boundary registers swap
+ *      roles;
+ *   4  (default where it applies) variant 3 in packed fp16 arithmetic, whose
+ *      three-way maximum is one instruction: 7 packed ops per cell pair in
+ *      semiglobal mode instead of 8. Exact only while every value is an
+ *      integer fp16 holds exactly, so only for panels with longest reference
+ *      x max(|match|, |mismatch|) + |open| + |extend| <= 2048;
+ *      esch_set_variant refuses it (ESCH_INVALID_ARG) elsewhere. */
 int esch_n_variants(void);
 int esch_get_variant(const esch_scorer *s);
 esch_status esch_set_variant(esch_scorer *s, int variant);
 /* ms[v] (may be NULL, esch_n_variants() entries) receives each variant's DP
- * time on the sample; the fastest is selected and returned in *chosen.
+ * time on the sample (-1 for a variant the panel does not admit); the
+ * fastest is selected and returned in *chosen.
  * Then, unless esch_set_diag_ratio fixed it, the per-length-bucket routing
  * ratio is swept over {off, 1.5, 2, 3, 4} on the same sample; another one
  * replaces the default 2.0 only when more than 2% faster

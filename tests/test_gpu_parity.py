@@ -358,6 +358,44 @@ def test_every_kernel_variant_equals_oracle_and_autotune_picks_one():
             assert gpu.variant == chosen and all(t > 0 for t in ms)
 
 
+def test_fp16_variant_at_its_bound():
+    """Variant 4 computes in fp16, exact for integers of magnitude up to
+    2048. Panels at that bound (longest reference x max(|match|, |mismatch|)
+    + |open| + |extend| = 2047 and 2048), with reads that reach the extreme
+    values (a reference read back whole: the highest score; runs of one
+    base against a reference of another: cells down at the low end of the
+    bound), equal the
+    oracle and variant 3 in both modes. One base longer, the panel refuses
+    the variant and defaults to the int16 one."""
+    rng = G.Rng(0x5EED_0A03)
+    for scoring, L in (((2, -1, -10, -1), 1018), ((1, -1, -2, -1), 2045)):
+        refs = [G.random_seq(rng, L, 0.0), b"C" * L, G.random_seq(rng, L // 2, 0.02),
+                G.random_seq(rng, 60, 0.0)]
+        reads = [refs[0], b"A" * L, b"A" * (L // 3), b"G" * 5, refs[0][100:900],
+                 G.random_seq(rng, 700, 0.03), G.random_seq(rng, L + 40, 0.0)]
+        reads += [G.random_seq(rng, 1 + rng.below(400), 0.02) for _ in range(60)]
+        queries = codes_of(reads)
+        queries += [ea.reverse_complement_codes(q) for q in queries]
+        panel = ea.Panel(refs)
+        for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
+            gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+            assert gpu.variant == 4, (scoring, mode)
+            want = oracle(queries, panel.codes, scoring, mode)
+            s32, ok = gpu.score_batch_i32(queries)
+            assert ok.all() and np.array_equal(s32, want), (scoring, mode)
+            assert want.max() >= L - 20, want.max()
+            gpu.variant = 3
+            s3, ok3 = gpu.score_batch_i32(queries)
+            assert ok3.all() and np.array_equal(s3, want), (scoring, mode, 3)
+        longer = ea.Panel(refs + [G.random_seq(rng, L + 1, 0.0)])
+        gpu = ea.GpuScorer(longer, ea.Scoring(*scoring), ea.Mode.LOCAL)
+        assert gpu.variant == 3
+        with pytest.raises(ea.GpuError):
+            gpu.variant = 4
+        chosen, ms = gpu.autotune(queries[:40])
+        assert chosen != 4 and ms[4] == -1.0
+
+
 def test_every_kernel_variant_on_short_references():
     """References of every length 1..40 (column counts that leave 0-3
     columns after the paired / prefetching loops, the reference's padding
