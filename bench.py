@@ -545,6 +545,29 @@ def main():
     log(f"[bench] {value:,.0f} reads/s, {ms_step:.1f} ms/step, DP {dp * 1e3:.1f} ms "
         f"({gcups:,.0f} GCUPS), {n16} int16 / {n_diag} anti-diagonal / {n32} int32 queries")
 
+    # With the fp16 variant chosen, the same steps again with the int16
+    # default (variant 3), so both arithmetics' rates are on record.
+    int16 = None
+    if f16:
+        gpu.variant = 3
+        batch.run(what, tie_cap=tie_cap)
+        batch.wait()
+        d16 = []
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            batch.run(what, tie_cap=tie_cap)
+            batch.wait()
+            d16.append(batch.timing()[0])
+        w16 = time.perf_counter() - t
+        gpu.variant = chosen
+        dp16 = float(np.mean(d16)) / 1e3
+        ops16 = OPS_PER_CELL[(args.mode, False)]
+        int16 = {"variant": 3, "reads_per_s": round(args.reads / (w16 / args.steps), 1),
+                 "ms_per_step": round(w16 / args.steps * 1e3, 3), "dp_ms": round(dp16 * 1e3, 3),
+                 "gcups": round(cells / dp16 / 1e9, 1), "ops_per_cell": ops16,
+                 "roofline_frac": round(cells * ops16 / dp16 / PEAK_I16, 4)}
+        log(f"[bench] int16 variant 3 on the same batch: {int16}")
+
     plain = None
     if not args.no_plain:
         batch.run(ea.RUN_SCORES32 | ea.RUN_SELECT | ea.RUN_PLAIN | ea.RUN_NO_GRAPH, tie_cap=tie_cap)
@@ -633,6 +656,7 @@ def main():
         "host_load": {"ms": round(load_s * 1e3, 3), "bytes": int(lens.sum()),
                       "pcie_inclusive_reads_per_s": round(args.reads / (load_s + wall / args.steps), 1)},
         "kernel_variant": chosen,
+        "int16_variant": int16,
         "diag_ratio": gpu.diag_ratio,
         "autotune_ms": [round(x, 3) for x in tune_ms],
         "plain_kernel": plain,
