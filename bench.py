@@ -41,11 +41,11 @@ PEAK_I16 = N_CU * CLOCK_HZ * 128           # lane-ops/s
 PEAK_I32 = N_CU * CLOCK_HZ * 64
 # Minimum packed 16-bit lane-ops per DP cell of the recurrence: 4 adds and
 # 4 two-input maxima in semiglobal mode, plus the zero floor and the running
-# best in local mode; with the fp16 variant's three-input maximum (variant 4)
+# best in local mode; with the fp16 variants' three-input maximum (4 and 5)
 # H takes one max instead of two, and the column best a three-way tree.
 OPS_PER_CELL = {("semiglobal", False): 8, ("local", False): 10,
                 ("semiglobal", True): 7, ("local", True): 8.5}
-F16_VARIANT = 4
+F16_VARIANTS = (4, 5)  # k_dp16 variants that compute in fp16
 # SQ_INSTS_VALU x 128 / cells of one k_dp16 launch of the default workload,
 # from the counter summary in profiles/ (mode, variant) -> lane-ops per cell
 VALU_OPS_PER_CELL_ISSUED = {("semiglobal", 3): round(345_033_311_192 * 128 / 5_303_272_453_076, 3),
@@ -539,13 +539,13 @@ def main():
     value = args.reads / (wall / args.steps)
     dp = float(np.mean(dp_ms)) / 1e3
     gcups = cells / dp / 1e9
-    f16 = chosen == F16_VARIANT
+    f16 = chosen in F16_VARIANTS
     ops = OPS_PER_CELL[(args.mode, f16)]
     achieved = cells * ops / dp
     log(f"[bench] {value:,.0f} reads/s, {ms_step:.1f} ms/step, DP {dp * 1e3:.1f} ms "
         f"({gcups:,.0f} GCUPS), {n16} int16 / {n_diag} anti-diagonal / {n32} int32 queries")
 
-    # With the fp16 variant chosen, the same steps again with the int16
+    # With an fp16 variant chosen, the same steps again with the int16
     # default (variant 3), so both arithmetics' rates are on record.
     int16 = None
     if f16:

@@ -116,13 +116,14 @@ struct Launch {
     int block32;  // threads per k_dp32 block
     int grid_aux; // blocks of the bucketing / selection kernels
     int variant16;  // k_dp16 variant: 0 profile per column, 1 ping-pong, 2 pair schedule, 3 quad schedule,
-                    // 4 quad schedule in fp16 (panels within kBoundF16 only)
+                    // 4 quad schedule in fp16, 5 eight-column schedule in fp16 (the
+                    // fp16 variants: panels within kBoundF16 only)
     int grid_diag;  // persistent waves of the anti-diagonal kernel
     size_t diag_lds;  // k_diag64's LDS boundary bytes; 0: k_diag16 (global scratch)
 };
-constexpr int kDp16Variants = 5;
+constexpr int kDp16Variants = 6;
 constexpr int kDp16Default = 3;  // the int16 default; panels within kBoundF16 take kDp16F16
-constexpr int kDp16F16 = 4;
+constexpr int kDp16F16 = 4;      // variants kDp16F16 and up compute in fp16
 hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st);
 hipError_t launch_dp16(const KArgs &a, const Launch &g, bool local, hipStream_t st);
 hipError_t launch_dp32(const KArgs &a, const Launch &g, bool local, int which, hipStream_t st);

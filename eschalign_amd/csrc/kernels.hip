@@ -280,7 +280,7 @@ __device__ __forceinline__ void stripe16(const uint4 *__restrict__ prof, uint2 *
 // to m + 1), the profile prefetch reads at most the reference's padding
 // word (column m), and whether the stripe is the first (border instead of
 // scratch) or the last (no stores) is a template argument.
-template <int RR, bool LOCAL, bool FIRST, bool LAST, bool QUAD, bool F16>
+template <int RR, bool LOCAL, bool FIRST, bool LAST, bool QUAD, bool F16, bool OCT = false>
 __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 *__restrict__ scr,
                                           const int lane, const cwords w2, const cwords wn,
                                           const int m, const int row0, const int pA, const int pB,
@@ -334,7 +334,92 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
         return make_uint2(P::bits(hou), P::bits(f));
     };
 
-    if constexpr (!QUAD) {
+    if constexpr (OCT) {
+    v2s sa[RR], sb[RR];
+    plane(code(0), sa);
+    uint2 a0 = bnd(0), a1 = bnd(1), b0 = bnd(2), b1 = bnd(3);
+    uint2 o0 = border, o1 = border;
+    int j = 0;
+    // Eight columns per iteration as four pairs over four sets of boundary
+    // registers (a, b, c, d), each loaded two pairs before its use: the
+    // boundary loads run four columns ahead (the farthest, j + 11 <= m + 3,
+    // reads the scratch padding) and no loaded register is copied.
+    for (; j + 7 < m; j += 8) {
+        const uint2 c0 = bnd(j + 4), c1 = bnd(j + 5);
+        if constexpr (!LAST) {
+            s0[(ptrdiff_t)(j - 2) * 64] = o0;
+            s0[(ptrdiff_t)(j - 1) * 64] = o1;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        plane(code(j + 1), sb);
+        o0 = col(sa, a0);
+        plane(code(j + 2), sa);
+        o1 = col(sb, a1);
+        __builtin_amdgcn_sched_barrier(0);
+        const uint2 d0 = bnd(j + 6), d1 = bnd(j + 7);
+        if constexpr (!LAST) {
+            s0[(ptrdiff_t)j * 64] = o0;
+            s0[(ptrdiff_t)(j + 1) * 64] = o1;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        plane(code(j + 3), sb);
+        o0 = col(sa, b0);
+        plane(code(j + 4), sa);
+        o1 = col(sb, b1);
+        __builtin_amdgcn_sched_barrier(0);
+        a0 = bnd(j + 8);
+        a1 = bnd(j + 9);
+        if constexpr (!LAST) {
+            s0[(ptrdiff_t)(j + 2) * 64] = o0;
+            s0[(ptrdiff_t)(j + 3) * 64] = o1;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        plane(code(j + 5), sb);
+        o0 = col(sa, c0);
+        plane(code(j + 6), sa);
+        o1 = col(sb, c1);
+        __builtin_amdgcn_sched_barrier(0);
+        b0 = bnd(j + 10);
+        b1 = bnd(j + 11);
+        if constexpr (!LAST) {
+            s0[(ptrdiff_t)(j + 4) * 64] = o0;
+            s0[(ptrdiff_t)(j + 5) * 64] = o1;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        plane(code(j + 7), sb);
+        o0 = col(sa, d0);
+        plane(code(j + 8), sa);  // j + 8 <= m: at most the padding word
+        o1 = col(sb, d1);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    // 0-7 columns left: pairs with the sets moved down (copies, once per
+    // stripe), then a single column
+    while (j + 1 < m) {
+        const uint2 c0 = bnd(j + 4), c1 = bnd(j + 5);  // j + 5 <= m + 3
+        if constexpr (!LAST) {
+            s0[(ptrdiff_t)(j - 2) * 64] = o0;
+            s0[(ptrdiff_t)(j - 1) * 64] = o1;
+        }
+        plane(code(j + 1), sb);
+        o0 = col(sa, a0);
+        plane(code(j + 2), sa);
+        o1 = col(sb, a1);
+        a0 = b0;
+        a1 = b1;
+        b0 = c0;
+        b1 = c1;
+        j += 2;
+    }
+    if constexpr (!LAST) {
+        s0[(ptrdiff_t)(j - 2) * 64] = o0;
+        s0[(ptrdiff_t)(j - 1) * 64] = o1;
+    }
+    if (j < m) {
+        o0 = col(sa, a0);
+        if constexpr (!LAST) s0[(size_t)j * 64] = o0;
+    }
+
+    } else if constexpr (!QUAD) {
     v2s sa[RR], sb[RR];
     plane(code(0), sa);
     uint2 r0 = bnd(0), r1 = bnd(1);
@@ -435,14 +520,14 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
     }
 }
 
-template <int RR, bool LOCAL, bool FIRST, bool QUAD, bool F16>
+template <int RR, bool LOCAL, bool FIRST, bool QUAD, bool F16, bool OCT>
 __device__ __forceinline__ void stripe16d_any(const uint4 *prof, uint2 *scr, const int lane,
                                               const cwords w2, const cwords wn, const int m,
                                               const bool last, const int row0, const int pA,
                                               const int pB, const typename Pk<F16>::T vo,
                                               const typename Pk<F16>::T ve, typename Pk<F16>::T &best) {
-    if (last) stripe16d<RR, LOCAL, FIRST, true, QUAD, F16>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best);
-    else stripe16d<RR, LOCAL, FIRST, false, QUAD, F16>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best);
+    if (last) stripe16d<RR, LOCAL, FIRST, true, QUAD, F16, OCT>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best);
+    else stripe16d<RR, LOCAL, FIRST, false, QUAD, F16, OCT>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best);
 }
 
 template <bool LOCAL, int PF>
@@ -459,11 +544,17 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
     const int n_chunks = (n16 + kChunk - 1) / kChunk;
     const uint32_t n_items = (uint32_t)n_chunks * (uint32_t)n_groups;
     uint2 *scr = a.scratch16 + ((size_t)blockIdx.x * kWaves16 + wv) * (size_t)(P.max_len + kScrPad) * 64;
-    // variant 4: variant 3's schedule in fp16 arithmetic
-    constexpr bool F16 = PF == 4;
+    // variants 4 and 5: variant 3's schedule, and eight columns per
+    // iteration with the boundary four columns ahead, in fp16 arithmetic
+    constexpr bool F16 = PF >= 4;
+    constexpr bool OCT = PF == 5;
     using PK = Pk<F16>;
     using V = typename PK::T;
     const V vo = PK::splat(a.sc.o), ve = PK::splat(a.sc.e), NEG = PK::neg();
+    // The profile's values, encoded once: s - o for a match and for a
+    // mismatch, and -o for the top padding rows (the values prof_val gives).
+    const uint32_t eMa = PK::enc(a.sc.ma - a.sc.o), eMi = PK::enc(a.sc.mi - a.sc.o);
+    const uint32_t ePre = PK::enc(-a.sc.o);
 
     for (;;) {
         __syncthreads();
@@ -516,15 +607,15 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
                     // 14 rows builds whole quads) are never read: padding codes
                     const int xa = (i < pA || i >= T) ? kPreCode : qA[i - pA];
                     const int xb = (i < pB || i >= T) ? kPreCode : qB[i - pB];
-                    int bestA = -32768, bestB = -32768;
 #pragma unroll
                     for (int c = 0; c < 4; ++c) {
-                        const int va = prof_val(xa, c, a.sc), vb = prof_val(xb, c, a.sc);
-                        bestA = max(bestA, va);
-                        bestB = max(bestB, vb);
-                        pl[c][rr] = PK::enc(va) | (PK::enc(vb) << 16);
+                        const uint32_t ea = xa == kPreCode ? ePre : (xa == c || xa == 4) ? eMa : eMi;
+                        const uint32_t eb = xb == kPreCode ? ePre : (xb == c || xb == 4) ? eMa : eMi;
+                        pl[c][rr] = ea | (eb << 16);
                     }
-                    pl[4][rr] = PK::enc(bestA) | (PK::enc(bestB) << 16);
+                    // plane 4, a wildcard reference base: a match for every
+                    // read base (also the best of the four, mismatch < match)
+                    pl[4][rr] = (xa == kPreCode ? ePre : eMa) | ((xb == kPreCode ? ePre : eMa) << 16);
                 }
 #pragma unroll
                 for (int c = 0; c < 5; ++c)
@@ -534,16 +625,16 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
             if (!active) continue;
             const bool first = s == 0, last = s + 1 == S;
             if constexpr (PF >= 2) {
-                constexpr bool Q = PF >= 3;
+                constexpr bool Q = PF == 3 || PF == 4;
                 if (s > 0) {
-                    stripe16d_any<16, LOCAL, false, Q, F16>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
+                    stripe16d_any<16, LOCAL, false, Q, F16, OCT>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
                 } else {
                     switch (RR) {
 #define ESCH_FIRST(H) \
-    case H: stripe16d_any<H, LOCAL, true, Q, F16>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best); break;
+    case H: stripe16d_any<H, LOCAL, true, Q, F16, OCT>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best); break;
                         ESCH_FIRST(16) ESCH_FIRST(14) ESCH_FIRST(12) ESCH_FIRST(10)
                         ESCH_FIRST(8) ESCH_FIRST(6) ESCH_FIRST(4)
-                        default: stripe16d_any<2, LOCAL, true, Q, F16>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
+                        default: stripe16d_any<2, LOCAL, true, Q, F16, OCT>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
 #undef ESCH_FIRST
                     }
                 }
@@ -1563,6 +1654,7 @@ hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st) {
 // Kernel variants of k_dp16, chosen per scorer (set by the autotuner).
 template <bool LOCAL>
 static const void *dp16_fn(int variant) {
+    if (variant == 5) return reinterpret_cast<const void *>(&k_dp16<LOCAL, 5>);
     if (variant == 4) return reinterpret_cast<const void *>(&k_dp16<LOCAL, 4>);
     if (variant == 3) return reinterpret_cast<const void *>(&k_dp16<LOCAL, 3>);
     if (variant == 2) return reinterpret_cast<const void *>(&k_dp16<LOCAL, 2>);
@@ -1574,13 +1666,15 @@ hipError_t launch_dp16(const KArgs &a, const Launch &g, bool local, hipStream_t 
     const int v = g.variant16;
     const dim3 grid(g.grid16), block(64 * kWaves16);
     if (local) {
-        if (v == 4) hipLaunchKernelGGL((k_dp16<true, 4>), grid, block, 0, st, a);
+        if (v == 5) hipLaunchKernelGGL((k_dp16<true, 5>), grid, block, 0, st, a);
+        else if (v == 4) hipLaunchKernelGGL((k_dp16<true, 4>), grid, block, 0, st, a);
         else if (v == 3) hipLaunchKernelGGL((k_dp16<true, 3>), grid, block, 0, st, a);
         else if (v == 2) hipLaunchKernelGGL((k_dp16<true, 2>), grid, block, 0, st, a);
         else if (v == 1) hipLaunchKernelGGL((k_dp16<true, 1>), grid, block, 0, st, a);
         else hipLaunchKernelGGL((k_dp16<true, 0>), grid, block, 0, st, a);
     } else {
-        if (v == 4) hipLaunchKernelGGL((k_dp16<false, 4>), grid, block, 0, st, a);
+        if (v == 5) hipLaunchKernelGGL((k_dp16<false, 5>), grid, block, 0, st, a);
+        else if (v == 4) hipLaunchKernelGGL((k_dp16<false, 4>), grid, block, 0, st, a);
         else if (v == 3) hipLaunchKernelGGL((k_dp16<false, 3>), grid, block, 0, st, a);
         else if (v == 2) hipLaunchKernelGGL((k_dp16<false, 2>), grid, block, 0, st, a);
         else if (v == 1) hipLaunchKernelGGL((k_dp16<false, 1>), grid, block, 0, st, a);

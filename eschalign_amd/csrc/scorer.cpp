@@ -857,7 +857,7 @@ esch_status esch_score_batch(esch_scorer *s, const uint8_t *const *queries, cons
 esch_status esch_set_variant(esch_scorer *s, int variant) {
     if (!s || variant < 0 || variant >= kDp16Variants)
         return fail(ESCH_INVALID_ARG, "variant %d outside 0..%d", variant, kDp16Variants - 1);
-    if (variant == kDp16F16 && !s->f16_ok)
+    if (variant >= kDp16F16 && !s->f16_ok)
         return fail(ESCH_INVALID_ARG,
                     "variant %d (fp16) needs longest reference x max(|match|, |mismatch|) + |open| + "
                     "|extend| <= %lld",
@@ -911,7 +911,7 @@ esch_status esch_autotune(esch_scorer *s, const uint8_t *const *reads, const siz
     int best_v = s->geo.variant16;
     float best_ms = 0.0f;
     for (int v = 0; !st && v < kDp16Variants; ++v) {
-        if (v == kDp16F16 && !s->f16_ok) {
+        if (v >= kDp16F16 && !s->f16_ok) {
             if (ms) ms[v] = -1.0f;  // not applicable to this panel
             continue;
         }

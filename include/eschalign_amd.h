@@ -372,7 +372,9 @@ size_t esch_cigar_string(const uint32_t *ops, size_t n_ops, size_t query_len, si
  *      semiglobal mode instead of 8. Exact only while every value is an
  *      integer fp16 holds exactly, so only for panels with longest reference
  *      x max(|match|, |mismatch|) + |open| + |extend| <= 2048;
- *      esch_set_variant refuses it (ESCH_INVALID_ARG) elsewhere. */
+ *      esch_set_variant refuses it (ESCH_INVALID_ARG) elsewhere;
+ *   5  the fp16 arithmetic of variant 4 over eight columns per iteration,
+ *      the stripe boundary loaded four columns ahead; the same condition. */
 int esch_n_variants(void);
 int esch_get_variant(const esch_scorer *s);
 esch_status esch_set_variant(esch_scorer *s, int variant);

@@ -384,16 +384,18 @@ def test_fp16_variant_at_its_bound():
             s32, ok = gpu.score_batch_i32(queries)
             assert ok.all() and np.array_equal(s32, want), (scoring, mode)
             assert want.max() >= L - 20, want.max()
-            gpu.variant = 3
-            s3, ok3 = gpu.score_batch_i32(queries)
-            assert ok3.all() and np.array_equal(s3, want), (scoring, mode, 3)
+            for v in (5, 3):
+                gpu.variant = v
+                sv, okv = gpu.score_batch_i32(queries)
+                assert okv.all() and np.array_equal(sv, want), (scoring, mode, v)
         longer = ea.Panel(refs + [G.random_seq(rng, L + 1, 0.0)])
         gpu = ea.GpuScorer(longer, ea.Scoring(*scoring), ea.Mode.LOCAL)
         assert gpu.variant == 3
-        with pytest.raises(ea.GpuError):
-            gpu.variant = 4
+        for v in (4, 5):
+            with pytest.raises(ea.GpuError):
+                gpu.variant = v
         chosen, ms = gpu.autotune(queries[:40])
-        assert chosen != 4 and ms[4] == -1.0
+        assert chosen < 4 and ms[4] == ms[5] == -1.0
 
 
 def test_every_kernel_variant_on_short_references():
