@@ -48,8 +48,9 @@ OPS_PER_CELL = {("semiglobal", False): 8, ("local", False): 10,
 F16_VARIANTS = (4, 5)  # k_dp16 variants that compute in fp16
 # SQ_INSTS_VALU x 128 / cells of one k_dp16 launch of the default workload,
 # from the counter summary in profiles/ (mode, variant) -> lane-ops per cell
-VALU_OPS_PER_CELL_ISSUED = {("semiglobal", 3): round(345_033_311_192 * 128 / 5_303_272_453_076, 3),
-                            ("semiglobal", 4): round(304_396_230_832 * 128 / 5_303_272_453_076, 3)}
+VALU_OPS_PER_CELL_ISSUED = {("semiglobal", 3): round(344_021_737_036 * 128 / 5_303_272_453_076, 3),
+                            ("semiglobal", 4): round(304_396_230_832 * 128 / 5_303_272_453_076, 3),
+                            ("semiglobal", 5): round(301_256_340_206 * 128 / 5_303_272_453_076, 3)}
 
 
 def log(*a):
