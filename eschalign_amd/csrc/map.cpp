@@ -265,10 +265,11 @@ esch_status scorer_ready(esch_scorer *s) {
     return ESCH_OK;
 }
 
-// Load reads [k0, k1) of get into a batch (both orientations when both) and
-// score them into resident int32 rows.
-esch_status load_and_score(esch_scorer *s, const Getter &get, size_t k0, size_t k1, int both, int pairs,
-                           esch_batch **out, esch_map_stats &stats) {
+// Load reads [k0, k1) of get into a batch (both orientations when both):
+// the gather into pinned staging and the upload, queued on the batch's
+// stream. The scoring is launched by score_chunk.
+esch_status load_chunk(esch_scorer *s, const Getter &get, size_t k0, size_t k1, int both, int pairs,
+                       esch_batch **out, esch_map_stats &stats) {
     const auto t0 = Clock::now();
     const size_t n = k1 - k0;
     std::vector<const uint8_t *> ptrs(n);
@@ -280,20 +281,41 @@ esch_status load_and_score(esch_scorer *s, const Getter &get, size_t k0, size_t 
     }
     esch_batch *b = nullptr;
     if (esch_status st = take_batch(s, n, bases, &b)) return st;
-    esch_status st = esch_batch_load(b, ptrs.data(), lens.data(), n, both, pairs);
+    if (esch_status st = esch_batch_load(b, ptrs.data(), lens.data(), n, both, pairs)) {
+        give_batch(s, b);
+        return st;
+    }
     stats.load_ms += ms_since(t0);
-    if (!st) st = esch_batch_run(b, ESCH_RUN_SCORES32, 0, -1, 1);
-    if (!st) st = esch_batch_wait(b);
+    *out = b;
+    return ESCH_OK;
+}
+
+// Launch the scoring of a loaded batch into resident int32 rows.
+esch_status score_chunk(esch_batch *b) { return esch_batch_run(b, ESCH_RUN_SCORES32, 0, -1, 1); }
+
+// Wait for a batch's scoring; count its device time and cells.
+esch_status finish_chunk(esch_batch *b, esch_map_stats &stats) {
+    if (esch_status st = esch_batch_wait(b)) return st;
     float total = 0.0f;
     uint64_t cells = 0;
-    if (!st) st = esch_batch_timing(b, nullptr, &total);
-    if (!st) st = esch_batch_stats(b, &cells, nullptr, nullptr);
+    if (esch_status st = esch_batch_timing(b, nullptr, &total)) return st;
+    if (esch_status st = esch_batch_stats(b, &cells, nullptr, nullptr)) return st;
+    stats.score_ms += total;
+    stats.cells += (int64_t)cells;
+    return ESCH_OK;
+}
+
+// The three in a row: reads [k0, k1) of get scored into a batch's rows.
+esch_status load_and_score(esch_scorer *s, const Getter &get, size_t k0, size_t k1, int both, int pairs,
+                           esch_batch **out, esch_map_stats &stats) {
+    esch_batch *b = nullptr;
+    if (esch_status st = load_chunk(s, get, k0, k1, both, pairs, &b, stats)) return st;
+    esch_status st = score_chunk(b);
+    if (!st) st = finish_chunk(b, stats);
     if (st) {
         give_batch(s, b);
         return st;
     }
-    stats.score_ms += total;
-    stats.cells += (int64_t)cells;
     *out = b;
     return ESCH_OK;
 }
@@ -406,34 +428,59 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
     // demand too, lib/mapper/pair.rs:195-207). Both strands: all four.
     const bool half = !both_strands;
     const size_t chunk = std::max<size_t>(1, chunk_rows(s) / (half ? 2 : 4));
+    const size_t n_chunks = (n_pairs + chunk - 1) / chunk;
     std::vector<uint8_t> &rc2 = s->map->rc2;  // grow-only across calls
     std::vector<size_t> &rc2_at = s->map->rc2_at;
-    for (size_t p0 = 0; p0 < n_pairs; p0 += chunk) {
-        const size_t p1 = std::min(n_pairs, p0 + chunk), c = p1 - p0;
-        esch_batch *b = nullptr;
-        if (half) {
-            size_t bad = 0;
-            if (!rc_mates2(mate, p0, p1, rc2, rc2_at, &bad))
-                return fail(ESCH_INVALID_ARG, "pair %zu: mate 2 carries a code outside 0..=4", bad);
-            const Getter rows = [&](size_t k, const uint8_t *&p, size_t &len) {
-                if (k & 1) {
-                    p = rc2.data() + rc2_at[k >> 1];
-                    len = rc2_at[(k >> 1) + 1] - rc2_at[k >> 1];
-                } else {
-                    mate(2 * p0 + k, p, len);
-                }
-            };
-            if (esch_status st = load_and_score(s, rows, 0, 2 * c, 0, 0, &b, M->stats)) return st;
-        } else {
-            if (esch_status st = load_and_score(s, mate, 2 * p0, 2 * p1, 1, 1, &b, M->stats)) return st;
+    // Stage chunk q into a batch: mate 2's reverse complements (half rows),
+    // the gather into pinned staging and the upload. Its scoring is launched
+    // separately, so the next chunk is staged while this one scores, and a
+    // chunk's records are assembled while the next one scores. (The tier
+    // loop's own kernels run between the two scorings: queued beside the
+    // persistent scoring kernel they would wait for it.)
+    auto stage = [&](size_t q, esch_batch **out) -> esch_status {
+        const size_t p0 = q * chunk, p1 = std::min(n_pairs, p0 + chunk), c = p1 - p0;
+        if (!half) return load_chunk(s, mate, 2 * p0, 2 * p1, 1, 1, out, M->stats);
+        const auto t = Clock::now();
+        size_t bad = 0;
+        if (!rc_mates2(mate, p0, p1, rc2, rc2_at, &bad))
+            return fail(ESCH_INVALID_ARG, "pair %zu: mate 2 carries a code outside 0..=4", bad);
+        M->stats.load_ms += ms_since(t);
+        const Getter rows = [&, p0](size_t k, const uint8_t *&p, size_t &len) {
+            if (k & 1) {
+                p = rc2.data() + rc2_at[k >> 1];
+                len = rc2_at[(k >> 1) + 1] - rc2_at[k >> 1];
+            } else {
+                mate(2 * p0 + k, p, len);
+            }
+        };
+        return load_chunk(s, rows, 0, 2 * c, 0, 0, out, M->stats);
+    };
+    // cur: the chunk being resolved (scored or scoring); next: staged
+    struct Held {
+        esch_scorer *s;
+        esch_batch *cur = nullptr, *next = nullptr;
+        ~Held() {
+            if (cur) give_batch(s, cur);
+            if (next) give_batch(s, next);
         }
+    } held{s};
+    if (n_chunks) {
+        if (esch_status st = stage(0, &held.cur)) return st;
+        if (esch_status st = score_chunk(held.cur)) return st;
+    }
+    for (size_t ci = 0; ci < n_chunks; ++ci) {
+        const size_t p0 = ci * chunk, p1 = std::min(n_pairs, p0 + chunk), c = p1 - p0;
+        if (ci + 1 < n_chunks)
+            if (esch_status st = stage(ci + 1, &held.next)) return st;
+        esch_batch *b = held.cur;
+        if (esch_status st = finish_chunk(b, M->stats)) return st;
         struct Guard {
             esch_scorer *s;
             esch_batch *b;
             ~Guard() {
                 if (b) give_batch(s, b);
             }
-        } guard{s, b}, guard_sep{s, nullptr};
+        } guard_sep{s, nullptr};
         Ctx cx{s, b, *s->map, M->stats};
         MapCache &C = *s->map;
         hipStream_t st = b->stream;
@@ -620,10 +667,13 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
         } else if (esch_status e = cx.map_rows(mates_of, 1, min_score, max_ties, mm, pool)) {
             return e;
         }
-        for (size_t q = 0; q < mates_of.size(); ++q) {
-            const int32_t x = mates_of[q];
-            res[(size_t)(x / 2)].separate[x % 2] = mm[q];
+        for (size_t e = 0; e < mates_of.size(); ++e) {
+            const int32_t x = mates_of[e];
+            res[(size_t)(x / 2)].separate[x % 2] = mm[e];
         }
+        // this chunk's device work is done: the next chunk scores meanwhile
+        if (held.next)
+            if (esch_status e = score_chunk(held.next)) return e;
         // into the result, in pair order
         const auto ta = Clock::now();
         M->hits.reserve(M->hits.size() + pool.hits.size());
@@ -650,6 +700,9 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
             M->pairs[p0 + i] = m;
         }
         M->stats.assemble_ms += ms_since(ta);
+        give_batch(s, held.cur);
+        held.cur = held.next;
+        held.next = nullptr;
     }
     M->stats.total_ms = ms_since(t0);
     *out = M.release();
