@@ -1326,8 +1326,9 @@ __global__ __launch_bounds__(256) void k_classify(KArgs a) {
 // (bin_start = -1). And when the whole batch makes fewer inter-sequence
 // work items (chunks x groups of 4 references) than that kernel keeps
 // resident blocks, it cannot fill the device, so every bin of length >= 512
-// goes to k_diag16 too, whose items are (read, 8 references). The remaining
-// bins are compacted into sorted16 order.
+// goes to k_diag16 too, whose items are (read, 8 references). A batch whose
+// kept rows do not fill one chunk goes there whole when that is cheaper.
+// The remaining bins are compacted into sorted16 order.
 __device__ int32_t block_exclusive_scan(int32_t v, int32_t *part, int32_t *total) {
     const int t = threadIdx.x;
     part[t] = v;
@@ -1394,9 +1395,33 @@ __global__ __launch_bounds__(1024) void k_scan(KArgs a) {
     // compacted prefix over the kept bins; routed bins get -1
     int32_t n16;
     run = block_exclusive_scan(kept, part, &n16);
+    // Kept rows that do not fill one chunk still cost k_dp16 a whole chunk:
+    // 128 rows at the longest kept length, however few they are (a small
+    // request against a large panel). The anti-diagonal kernel costs rho
+    // times each row padded to its passes of 64 R rows (R = 2, 4, 8 by
+    // length, as diag64_task); when that is less, every kept bin goes.
+    __shared__ int32_t s_top;
+    bool all = false;
+    if (ratio > 0 && n16 > 0 && n16 < kChunk) {
+        if (t == 0) s_top = 0;
+        __syncthreads();
+        int32_t pad = 0, top = 0;
+        for (int k = 0; k < nb; ++k) {
+            const int L = b0 - k;
+            const int cnt = a.hist[L];
+            if (cnt == 0 || a.bin_fill[L]) continue;
+            const int rp = 64 * (L <= 128 ? 2 : L <= 256 ? 4 : 8);
+            pad += cnt * ((L + rp - 1) / rp) * rp;
+            top = max(top, L);
+        }
+        if (top) atomicMax(&s_top, top);
+        int32_t pad_total;
+        (void)block_exclusive_scan(pad, part, &pad_total);
+        all = (int64_t)ratio * pad_total < (int64_t)16 * kChunk * s_top;
+    }
     for (int k = 0; k < nb; ++k) {
         const int b = b0 - k;
-        if (ratio > 0 && a.bin_fill[b]) {
+        if (ratio > 0 && (a.bin_fill[b] || all)) {
             a.bin_start[b] = -1;
             a.bin_fill[b] = 0;
         } else {
@@ -1404,7 +1429,7 @@ __global__ __launch_bounds__(1024) void k_scan(KArgs a) {
             run += a.hist[b];
         }
     }
-    if (t == 0) a.desc->n16 = n16;
+    if (t == 0) a.desc->n16 = all ? 0 : n16;
 }
 
 // Each int16 query's slot in sorted16: its bin's start plus its rank in the

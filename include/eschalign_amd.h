@@ -189,7 +189,10 @@ int32_t esch_get_diag_min_len(const esch_scorer *s);
  * reads of length L take the anti-diagonal kernel when L > rho * L' + 64,
  * L' the length of the read 127 ranks later (the chunk they would lead),
  * or when L >= 512 and the batch is too small to fill the device with
- * inter-sequence work items (128-read chunks x 4 references).
+ * inter-sequence work items (128-read chunks x 4 references). Reads left
+ * that do not fill one 128-read chunk all go when rho times their rows,
+ * padded to the anti-diagonal kernel's passes (64 R rows; R = 2, 4, 8 for
+ * lengths up to 128, 256, beyond), is less than 128 x the longest of them.
  * rho = ratio_x16 / 16, default 32 (2.0); 0 turns the routing off;
  * otherwise 16..1024. esch_autotune sweeps it. Identical scores either
  * way. */

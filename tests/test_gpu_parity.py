@@ -489,24 +489,37 @@ def _routed(gpu, queries, ratio):
     return got, ok, nd
 
 
+def diag_padded(L):
+    """Rows the anti-diagonal kernel computes for a read of length L: whole
+    passes of 64 R rows, R = 2, 4 or 8 by length."""
+    rp = 64 * (2 if L <= 128 else 4 if L <= 256 else 8)
+    return -(-L // rp) * rp
+
+
 def routed_count(lens, ratio, n_refs, blocks16):
     """How many reads the per-length-bucket rule routes: a length bin L
     (>= 128) goes when L > ratio * L' + 64, L' the length of the read 127
     ranks later (or the shortest read), or when L >= 512 and the batch's
     inter-sequence work items (128-read chunks x groups of 4 references)
-    are fewer than that kernel's resident blocks."""
+    are fewer than that kernel's resident blocks. Then, when the kept reads
+    do not fill one 128-read chunk and ratio x their padded rows is less
+    than 128 x the longest kept length, the kept reads go too."""
     srt = sorted(lens, reverse=True)
     total = len(srt)
     starved = -(-total // 128) * -(-n_refs // 4) < blocks16
-    count = 0
+    r16 = round(ratio * 16)
+    routed = set()
     for L in set(lens):
         if L < 128:
             continue
         r = sum(1 for x in srt if x > L)
         L2 = srt[min(r + 127, total - 1)]
-        if 16 * L > round(ratio * 16) * L2 + 1024 or (starved and L >= 512):
-            count += lens.count(L)
-    return count
+        if 16 * L > r16 * L2 + 1024 or (starved and L >= 512):
+            routed.add(L)
+    kept = [L for L in lens if L not in routed]
+    if r16 > 0 and 0 < len(kept) < 128 and r16 * sum(diag_padded(L) for L in kept) < 16 * 128 * max(kept):
+        return len(lens)
+    return len(lens) - len(kept)
 
 
 def test_adaptive_routing_per_length_bucket():
@@ -563,6 +576,38 @@ def test_adaptive_routing_dense_batch_keeps_the_chunks():
         assert ok.all() and ok0.all() and np.array_equal(got, off), ratio
         assert nd == routed_count(lens, ratio, 164, blocks16), (ratio, nd)
     assert routed_count(lens, 2.0, 164, blocks16) == 10
+
+
+def test_small_batch_against_a_wide_panel_takes_the_anti_diagonal_kernel():
+    """A batch too small to fill one 128-row chunk of the inter-sequence
+    kernel (a few reads against 900 references) goes to the anti-diagonal
+    kernel whole when its padded rows cost less there; reads so short that
+    their passes would be mostly padding stay. Scores equal the oracle's
+    either way, and the routed count is the rule's."""
+    rng = G.Rng(0x5EED_0B04)
+    panel_l = G.random_panel(rng, 900, 60, 300)
+    panel = ea.Panel(panel_l)
+    scoring = (1, -1, -2, -1)
+    cases = {
+        "six reads": [G.chimeric_read(rng, panel_l, 80 + rng.below(100)) for _ in range(6)],
+        "fifty reads": [G.chimeric_read(rng, panel_l, 90 + rng.below(300)) for _ in range(50)],
+        "short reads": [G.random_seq(rng, 20 + rng.below(20), 0.0) for _ in range(70)],
+    }
+    for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
+        gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+        blocks16 = gpu.geometry()[2]
+        for name, reads in cases.items():
+            queries = codes_of(reads)
+            lens = [len(q) for q in queries]
+            want = oracle(queries, panel.codes, scoring, mode)
+            got, ok, nd = _routed(gpu, queries, 2.0)
+            assert ok.all() and np.array_equal(got, want), (name, mode)
+            expect = routed_count(lens, 2.0, len(panel), blocks16)
+            assert nd == expect, (name, mode, nd, expect)
+            if name == "short reads":
+                assert nd == 0
+            elif name == "six reads":
+                assert nd == len(queries)
 
 
 def test_autotune_sweeps_the_routing_ratio():
