@@ -63,6 +63,7 @@ EXPORTS = [
     "esch_batch_ready", "esch_server_new", "esch_server_free", "esch_server_add_panel",
     "esch_server_scorer", "esch_server_submit", "esch_server_wait", "esch_server_result",
     "esch_server_stats", "esch_set_diag_min_len", "esch_get_diag_min_len",
+    "esch_set_swap_rows", "esch_get_swap_rows", "esch_batch_swapped",
     "esch_batch_stats_diag", "esch_set_diag_ratio", "esch_get_diag_ratio", "esch_valu_peak16",
     "esch_align_pairs", "esch_align_last_ms", "esch_align_pairs_concat", "esch_map_reads",
     "esch_map_pairs", "esch_map_pairs_concat", "esch_mapped_len", "esch_mapped_reads",
@@ -198,6 +199,11 @@ def load_library():
     L.esch_set_diag_min_len.argtypes = [vp, i32]
     L.esch_get_diag_min_len.argtypes = [vp]
     L.esch_get_diag_min_len.restype = i32
+    L.esch_set_swap_rows.argtypes = [vp, i32]
+    L.esch_get_swap_rows.argtypes = [vp]
+    L.esch_get_swap_rows.restype = i32
+    L.esch_batch_swapped.argtypes = [vp]
+    L.esch_batch_swapped.restype = C.c_int
     L.esch_batch_stats_diag.argtypes = [vp, C.POINTER(C.c_uint64)]
     L.esch_set_diag_ratio.argtypes = [vp, C.c_int32]
     L.esch_align_pairs.argtypes = [vp, vp, vp, vp, sz, vp, vp, sz, C.POINTER(sz)]
@@ -810,6 +816,17 @@ class GpuScorer:
     def diag_min_len(self, v: int):
         _check(_lib.esch_set_diag_min_len(self._h, int(v)))
 
+    @property
+    def swap_rows(self) -> int:
+        """Batches of up to this many rows are scored with the roles of
+        reads and references swapped (identical scores); set -1 for the
+        default rule, 0 to turn it off."""
+        return _lib.esch_get_swap_rows(self._h)
+
+    @swap_rows.setter
+    def swap_rows(self, v: int):
+        _check(_lib.esch_set_swap_rows(self._h, int(v)))
+
     def align_pairs_raw(self, queries: Sequence[np.ndarray], refs: Sequence[int]):
         """esch_align_pairs: (records int64 [n, 7] = score, query_start,
         query_end, ref_start, ref_end, n_ops, ops_offset; ops uint32, each
@@ -1060,6 +1077,10 @@ class DeviceBatch:
         v = C.c_uint64()
         _check(_lib.esch_batch_stats_diag(self._h, C.byref(v)))
         return v.value
+
+    def swapped(self) -> bool:
+        """The last run scored with the roles of reads and references swapped."""
+        return bool(_lib.esch_batch_swapped(self._h))
 
     def scores32(self):
         nr = self.scorer.n_refs

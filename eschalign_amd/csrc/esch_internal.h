@@ -101,7 +101,17 @@ struct KArgs {
     uint8_t *rc_codes;   // where reverse complements are written
     int32_t *listD;      // anti-diagonal-kernel queries
     uint2 *scratchD;     // [grid_diag * 4 rows * max_len]
+    // 0: out[q * n_refs + r]. 1: the roles are swapped (the queries are a
+    // panel's references and `panel` holds the reads), and the score of
+    // query q against reference r is the read r's against panel reference
+    // q: out[r * n_queries + q], the layout of an ordinary run.
+    int32_t swap;
 };
+
+// Where a kernel writes the score of query q against reference r.
+__device__ __forceinline__ size_t out_index(const KArgs &a, int q, int r) {
+    return a.swap ? (size_t)r * (size_t)a.desc->n_queries + (size_t)q : (size_t)q * (size_t)a.panel.n_refs + (size_t)r;
+}
 // Rows per lane of the anti-diagonal kernel (a pass covers 16 * kDiagRows).
 constexpr int kDiagRows = 8;
 constexpr int kDiag64MaxLen = 4096;  // longest reference k_diag64 keeps in LDS

@@ -71,11 +71,18 @@ struct esch_scorer {
     TraceCache *trace = nullptr;     // traceback buffers kept between calls
     MapCache *map = nullptr;         // mapping buffers kept between calls
     size_t map_chunk = 0;            // reads per mapping chunk (0: by the rows' memory)
+    // the references as queries, for runs with the roles swapped
+    uint8_t *d_pq_codes = nullptr;
+    uint64_t *d_pq_off = nullptr;
+    int32_t *d_pq_len = nullptr;
+    int32_t swap_rows = -1;          // swap roles up to this many rows (-1: by the rule)
 };
 
 // Tie capacity (tie indices per read) a batch allocates when it is created;
 // a run asking for more reallocates, which waits for the device.
 constexpr size_t kTieCapAlloc = 8;
+
+struct SwapBufs;
 
 struct esch_batch {
     esch_scorer *s = nullptr;
@@ -109,6 +116,8 @@ struct esch_batch {
     int last_what = 0;
     bool timed_dp = false;
     std::map<int, hipGraphExec_t> graphs;
+    SwapBufs *swap = nullptr;   // the swapped path's buffers (allocated on first use)
+    bool last_swapped = false;  // the last run scored with the roles swapped
 };
 
 namespace esch {

@@ -657,12 +657,12 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
         if (!active) continue;
         const int bA = PK::lo(best) - a.sc.o, bB = PK::hi(best) - a.sc.o;
         if (qa >= 0) {
-            const size_t o = (size_t)qa * n_refs + ref;
+            const size_t o = out_index(a, qa, ref);
             if (a.out32) a.out32[o] = bA;
             if (a.out16) a.out16[o] = (int16_t)bA;
         }
         if (qb >= 0) {
-            const size_t o = (size_t)qb * n_refs + ref;
+            const size_t o = out_index(a, qb, ref);
             if (a.out32) a.out32[o] = bB;
             if (a.out16) a.out16[o] = (int16_t)bB;
         }
@@ -846,12 +846,12 @@ __global__ __launch_bounds__(64) void k_diag16(KArgs a) {
         }
         if (l == 0) {
             if (refA >= 0) {
-                const size_t o = (size_t)q * n_refs + refA;
+                const size_t o = out_index(a, q, refA);
                 if (a.out32) a.out32[o] = bx - a.sc.o;
                 if (a.out16) a.out16[o] = (int16_t)(bx - a.sc.o);
             }
             if (refB >= 0) {
-                const size_t o = (size_t)q * n_refs + refB;
+                const size_t o = out_index(a, q, refB);
                 if (a.out32) a.out32[o] = by - a.sc.o;
                 if (a.out16) a.out16[o] = (int16_t)(by - a.sc.o);
             }
@@ -1010,14 +1010,13 @@ __device__ __forceinline__ void diag64_task(const KArgs &a, const int q, const i
         by = max(by, __shfl_xor(by, off, 64));
     }
     if (lane == 0) {
-        const int n_refs = P.n_refs;
         if (refA >= 0) {
-            const size_t o = (size_t)q * n_refs + refA;
+            const size_t o = out_index(a, q, refA);
             if (a.out32) a.out32[o] = bx - a.sc.o;
             if (a.out16) a.out16[o] = (int16_t)(bx - a.sc.o);
         }
         if (refB >= 0) {
-            const size_t o = (size_t)q * n_refs + refB;
+            const size_t o = out_index(a, q, refB);
             if (a.out32) a.out32[o] = by - a.sc.o;
             if (a.out16) a.out16[o] = (int16_t)(by - a.sc.o);
         }
@@ -1169,7 +1168,7 @@ __global__ __launch_bounds__(256) void k_dp32(KArgs a, int which) {
                     if (kStripe * s + r >= p) best = max(best, H[r]);
             }
         }
-        const size_t oidx = (size_t)q * n_refs + ref;
+        const size_t oidx = out_index(a, q, ref);
         if (a.out32) a.out32[oidx] = best;
         if (a.out16) a.out16[oidx] = (int16_t)best;
     }
@@ -1295,8 +1294,8 @@ __global__ __launch_bounds__(256) void k_classify(KArgs a) {
             // Unscored rows read back as zeros (ScoreMatrix::from_parts).
             const int nr = a.panel.n_refs;
             for (int r = 0; r < nr; ++r) {
-                if (a.out16) a.out16[(size_t)q * nr + r] = 0;
-                if (a.out32) a.out32[(size_t)q * nr + r] = 0;
+                if (a.out16) a.out16[out_index(a, q, r)] = 0;
+                if (a.out32) a.out32[out_index(a, q, r)] = 0;
             }
         }
         if (c) cells += (uint64_t)n;

@@ -188,6 +188,24 @@ esch_status esch_scorer_new(int ordinal, const uint8_t *const *refs, const size_
     HIPC(hipMemcpy(s->d_rorder, order.data(), n_refs * 4, hipMemcpyHostToDevice));
     s->panel = DevPanel{s->d_ref2, s->d_refN, s->d_w2off, s->d_nmoff, s->d_rlen, s->d_rorder,
                         (int32_t)n_refs, (int32_t)max_len, total};
+    // The references as queries (byte codes, offsets, lengths), for batches
+    // scored with the roles of reads and references swapped.
+    {
+        std::vector<uint8_t> codes((size_t)total);
+        std::vector<uint64_t> off(n_refs);
+        size_t at = 0;
+        for (size_t i = 0; i < n_refs; ++i) {
+            off[i] = at;
+            memcpy(codes.data() + at, refs[i], ref_lens[i]);
+            at += ref_lens[i];
+        }
+        HIPC(hipMalloc(&s->d_pq_codes, std::max<size_t>(at, 1)));
+        HIPC(hipMalloc(&s->d_pq_off, n_refs * 8));
+        HIPC(hipMalloc(&s->d_pq_len, n_refs * 4));
+        HIPC(hipMemcpy(s->d_pq_codes, codes.data(), at, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(s->d_pq_off, off.data(), n_refs * 8, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(s->d_pq_len, rlen.data(), n_refs * 4, hipMemcpyHostToDevice));
+    }
 
     // Launch geometry: persistent grids sized by occupancy.
     const bool local = mode == ESCH_LOCAL;
@@ -240,6 +258,9 @@ void esch_scorer_free(esch_scorer *s) {
     (void)hipFree(s->d_nmoff);
     (void)hipFree(s->d_rlen);
     (void)hipFree(s->d_rorder);
+    (void)hipFree(s->d_pq_codes);
+    (void)hipFree(s->d_pq_off);
+    (void)hipFree(s->d_pq_len);
     delete s;
 }
 
@@ -313,6 +334,42 @@ static size_t diag_scratch_bytes(const esch_scorer *s) {
     return (size_t)s->geo.grid_diag * 4 * s->max_len * sizeof(uint2);
 }
 
+// The swapped path's buffers: the batch's rows packed as a panel (2-bit
+// words, wildcard bitmap, offsets, lengths, longest-first order) and the
+// scorer's references bucketed as queries, with a descriptor of its own.
+// One device block and one pinned staging block share the layout of the
+// rows' panel; both are sized for the batch's capacity and allocated on the
+// first swapped run.
+struct SwapBufs {
+    size_t cap_rows = 0, cap_w2 = 0, cap_wn = 0;
+    size_t off_nm = 0, off_rlen = 0, off_rord = 0, off_w2 = 0, off_wn = 0, off_desc = 0, bytes = 0;
+    uint8_t *d_block = nullptr, *h_stage = nullptr;
+    uint8_t *d_cls = nullptr;
+    int32_t *d_hist = nullptr, *d_bin_start = nullptr, *d_bin_fill = nullptr;
+    int32_t *d_sorted16 = nullptr, *d_list32 = nullptr, *d_listD = nullptr;
+};
+
+static void swap_layout(size_t max_reads, size_t max_bases, SwapBufs &w) {
+    w.cap_rows = 2 * std::max<size_t>(max_reads, 1);
+    const size_t bases = 2 * std::max<size_t>(max_bases, 1);
+    // a row starts a new word and carries a padding word
+    w.cap_w2 = bases / 16 + 2 * w.cap_rows + 1;
+    w.cap_wn = bases / 32 + 2 * w.cap_rows + 1;
+    w.off_nm = w.cap_rows * 4;
+    w.off_rlen = w.off_nm + w.cap_rows * 4;
+    w.off_rord = w.off_rlen + w.cap_rows * 4;
+    w.off_w2 = w.off_rord + w.cap_rows * 4;
+    w.off_wn = w.off_w2 + w.cap_w2 * 4;
+    w.off_desc = (w.off_wn + w.cap_wn * 4 + 15) / 16 * 16;
+    w.bytes = w.off_desc + sizeof(DevDesc);
+}
+
+static size_t swap_bytes(const esch_scorer *s, size_t max_reads, size_t max_bases) {
+    SwapBufs w;
+    swap_layout(max_reads, max_bases, w);
+    return w.bytes + s->n_refs * (1 + 3 * 4) + 3 * 4 * (kMaxLen16 + 1);
+}
+
 size_t esch_batch_footprint(const esch_scorer *s, size_t max_reads, size_t max_bases) {
     if (!s) return 0;
     const size_t rows = 2 * std::max<size_t>(max_reads, 1);
@@ -324,6 +381,7 @@ size_t esch_batch_footprint(const esch_scorer *s, size_t max_reads, size_t max_b
     f += rows * 5 * 4;                                                   // records
     f += rows * 4 + diag_scratch_bytes(s);                               // anti-diagonal
     f += std::max<size_t>(max_reads, 1) * kTieCapAlloc * 4;              // ties
+    f += swap_bytes(s, max_reads, max_bases);                            // swapped path
     return f;
 }
 
@@ -401,6 +459,18 @@ void esch_batch_free(esch_batch *b) {
     if (b->ev2) (void)hipEventDestroy(b->ev2);
     if (b->ev3) (void)hipEventDestroy(b->ev3);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (SwapBufs *w = b->swap) {
+        (void)hipFree(w->d_block);
+        (void)hipHostFree(w->h_stage);
+        (void)hipFree(w->d_cls);
+        (void)hipFree(w->d_hist);
+        (void)hipFree(w->d_bin_start);
+        (void)hipFree(w->d_bin_fill);
+        (void)hipFree(w->d_sorted16);
+        (void)hipFree(w->d_list32);
+        (void)hipFree(w->d_listD);
+        delete w;
+    }
     delete b;
 }
 
@@ -539,6 +609,149 @@ esch_status enqueue(esch_batch *b, int what, bool timed) {
     return ESCH_OK;
 }
 
+// Rows up to which a batch is scored with the roles swapped. With the
+// roles swapped the references are the queries, bucketed into well-filled
+// 128-row chunks, and every row is a reference of its exact length, so no
+// row pads another; the work items are (chunk of references, 4 rows). On
+// one MI355X that wins at every batch size measured (6 to 6,400 rows)
+// against 10,000 references, and loses against 1,000 or 164, whose few
+// reference chunks leave the device underfilled: the default swaps for
+// panels of at least 32 chunks and batches of at most one row per
+// reference.
+size_t swap_limit(const esch_scorer *s) {
+    if (s->swap_rows >= 0) return (size_t)s->swap_rows;
+    return s->n_refs >= 32 * 128 ? s->n_refs : 0;
+}
+
+bool swap_eligible(const esch_batch *b, int what) {
+    const esch_scorer *s = b->s;
+    if ((what & (ESCH_RUN_SCORES16 | ESCH_RUN_PLAIN | ESCH_RUN_PAIRS)) || !(what & ESCH_RUN_SCORES32)) return false;
+    if (s->diag_min_len > 0) return false;  // the caller routes by length explicitly
+    if (b->n_rows == 0 || b->n_rows > swap_limit(s)) return false;
+    // every row a non-empty reference no longer than the panel's (the
+    // scratch is sized by the panel's longest reference)
+    for (size_t r = 0; r < b->n_rows; ++r)
+        if (b->h_qlen[r] <= 0 || (size_t)b->h_qlen[r] > s->max_len) return false;
+    return true;
+}
+
+esch_status swap_ensure(esch_batch *b) {
+    if (b->swap) return ESCH_OK;
+    std::unique_ptr<SwapBufs> w(new SwapBufs);
+    swap_layout(b->max_reads, b->max_bases, *w);
+    const size_t nq = b->s->n_refs;
+    esch_status st = ESCH_OK;
+    auto dev = [&](auto **p, size_t n) {
+        if (!st && hipMalloc(p, std::max<size_t>(n, 1)) != hipSuccess) st = fail(ESCH_DRIVER, "hipMalloc failed");
+    };
+    dev(&w->d_block, w->bytes);
+    dev(&w->d_cls, nq);
+    dev(&w->d_hist, 4 * (kMaxLen16 + 1));
+    dev(&w->d_bin_start, 4 * (kMaxLen16 + 1));
+    dev(&w->d_bin_fill, 4 * (kMaxLen16 + 1));
+    dev(&w->d_sorted16, 4 * nq);
+    dev(&w->d_list32, 4 * nq);
+    dev(&w->d_listD, 4 * nq);
+    if (!st && hipHostMalloc(&w->h_stage, w->bytes, hipHostMallocDefault) != hipSuccess)
+        st = fail(ESCH_DRIVER, "hipHostMalloc failed");
+    b->swap = w.release();  // freed with the batch, also when incomplete
+    return st;
+}
+
+// Score the batch's rows against the panel with the roles swapped: the
+// rows packed as a panel (reverse complements made here), the scorer's
+// references as the queries, k_dp16 / k_dp32 writing every score where an
+// ordinary run would (KArgs::swap), then the ordinary selection. The score
+// of a read against a reference equals the reference's against the read:
+// the substitution, the gaps, the zero borders and the best cell (anywhere
+// in local mode; on the last row or column in semiglobal mode) are the same
+// with the two sequences exchanged.
+esch_status run_swapped(esch_batch *b, int what) {
+    esch_scorer *s = b->s;
+    if (esch_status st = swap_ensure(b)) return st;
+    SwapBufs &w = *b->swap;
+    const size_t rows = b->n_rows, n = b->n_reads;
+    uint8_t *h = w.h_stage;
+    uint32_t *w2off = reinterpret_cast<uint32_t *>(h);
+    uint32_t *nmoff = reinterpret_cast<uint32_t *>(h + w.off_nm);
+    int32_t *rlen = reinterpret_cast<int32_t *>(h + w.off_rlen);
+    int32_t *rord = reinterpret_cast<int32_t *>(h + w.off_rord);
+    uint32_t *ref2 = reinterpret_cast<uint32_t *>(h + w.off_w2);
+    uint32_t *refN = reinterpret_cast<uint32_t *>(h + w.off_wn);
+    DevDesc *desc = reinterpret_cast<DevDesc *>(h + w.off_desc);
+    size_t n2 = 0, nn = 0;
+    int32_t max_len = 0;
+    int64_t total = 0;
+    for (size_t r = 0; r < rows; ++r) {
+        const int32_t L = b->h_qlen[r];
+        const uint8_t *src = b->h_codes + b->h_qoff[r < n ? r : r - n];
+        const bool rc = r >= n;
+        w2off[r] = (uint32_t)n2;
+        nmoff[r] = (uint32_t)nn;
+        rlen[r] = L;
+        const size_t k2 = (size_t)(L + 15) / 16 + 1, kn = (size_t)(L + 31) / 32 + 1;
+        memset(ref2 + n2, 0, k2 * 4);
+        memset(refN + nn, 0, kn * 4);
+        for (int32_t j = 0; j < L; ++j) {
+            uint8_t c = rc ? src[L - 1 - j] : src[j];
+            if (rc && c < 4) c = (uint8_t)(3 - c);
+            if (c == 4) refN[nn + (size_t)j / 32] |= 1u << (j % 32);
+            else ref2[n2 + (size_t)j / 16] |= (uint32_t)c << (2 * (j % 16));
+        }
+        n2 += k2;
+        nn += kn;
+        max_len = std::max(max_len, L);
+        total += L;
+    }
+    for (size_t r = 0; r < rows; ++r) rord[r] = (int32_t)r;
+    std::stable_sort(rord, rord + rows, [&](int32_t x, int32_t y) { return rlen[x] > rlen[y]; });
+    memset(desc, 0, sizeof(DevDesc));
+    desc->n_queries = (int32_t)s->n_refs;
+    desc->n_reads = (int32_t)s->n_refs;
+    desc->tie_cap = 1;
+    desc->max_ties = -1;
+    desc->tstrands = 1;
+    desc->blocks16 = (int32_t)s->geo.grid16;  // routing off: no anti-diagonal rows
+    hipStream_t st = b->stream;
+    uint8_t *d = w.d_block;
+    HIPC(hipMemcpyAsync(d, h, rows * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(d + w.off_nm, h + w.off_nm, rows * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(d + w.off_rlen, h + w.off_rlen, rows * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(d + w.off_rord, h + w.off_rord, rows * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(d + w.off_w2, h + w.off_w2, n2 * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(d + w.off_wn, h + w.off_wn, nn * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(d + w.off_desc, h + w.off_desc, sizeof(DevDesc), hipMemcpyHostToDevice, st));
+
+    const KArgs a0 = make_args(b, what);
+    KArgs a = a0;
+    a.panel = DevPanel{reinterpret_cast<uint32_t *>(d + w.off_w2), reinterpret_cast<uint32_t *>(d + w.off_wn),
+                       reinterpret_cast<uint32_t *>(d), reinterpret_cast<uint32_t *>(d + w.off_nm),
+                       reinterpret_cast<int32_t *>(d + w.off_rlen), reinterpret_cast<int32_t *>(d + w.off_rord),
+                       (int32_t)rows, max_len, total};
+    a.desc = reinterpret_cast<DevDesc *>(d + w.off_desc);
+    a.qcodes = s->d_pq_codes;
+    a.qoff = s->d_pq_off;
+    a.qlen = s->d_pq_len;
+    a.cls = w.d_cls;
+    a.hist = w.d_hist;
+    a.bin_start = w.d_bin_start;
+    a.bin_fill = w.d_bin_fill;
+    a.sorted16 = w.d_sorted16;
+    a.list32 = w.d_list32;
+    a.listD = w.d_listD;
+    a.out16 = nullptr;
+    a.swap = 1;
+    const bool local = s->sc.local != 0;
+    HIPC(launch_prep(a, s->geo, st));
+    HIPC(hipEventRecord(b->ev1, st));
+    HIPC(launch_dp16(a, s->geo, local, st));
+    HIPC(launch_dp32(a, s->geo, local, 1, st));
+    HIPC(hipEventRecord(b->ev2, st));
+    HIPC(hipMemsetAsync(b->d_cls, 1, rows, st));  // every row scored (a run outside any graph)
+    if (what & ESCH_RUN_SELECT) HIPC(launch_select(a0, s->geo, st));
+    return ESCH_OK;
+}
+
 }  // namespace
 
 esch_status esch_batch_load(esch_batch *b, const uint8_t *const *reads, const size_t *lens, size_t n,
@@ -602,7 +815,11 @@ esch_status esch_batch_run(esch_batch *b, int what, int32_t min_score, int64_t m
     // the kernel variant is baked into the captured launches
     const int key = (what & ~(ESCH_RUN_NO_GRAPH | ESCH_RUN_PAIR_BOTH)) | (s->geo.variant16 << 16);
     HIPC(hipEventRecord(b->ev0, b->stream));
-    if (!graph) {
+    b->last_swapped = swap_eligible(b, what);
+    if (b->last_swapped) {
+        b->timed_dp = true;
+        if (esch_status st = run_swapped(b, what)) return st;
+    } else if (!graph) {
         b->timed_dp = true;
         if (esch_status st = enqueue(b, key, true)) return st;
     } else {
@@ -649,10 +866,15 @@ esch_status esch_batch_timing(const esch_batch *b, float *dp_ms, float *total_ms
     return ESCH_OK;
 }
 
+// The descriptor of the last run's scoring (the swapped path keeps its own).
+static const DevDesc *scoring_desc(const esch_batch *b) {
+    return b->last_swapped ? reinterpret_cast<const DevDesc *>(b->swap->d_block + b->swap->off_desc) : b->d_desc;
+}
+
 esch_status esch_batch_stats(const esch_batch *b, uint64_t *cells, uint64_t *n16, uint64_t *n32) {
     if (!b) return fail(ESCH_INVALID_ARG, "null batch");
     DevDesc d;
-    HIPC(hipMemcpy(&d, b->d_desc, sizeof d, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(&d, scoring_desc(b), sizeof d, hipMemcpyDeviceToHost));
     if (cells) *cells = d.cells;
     if (n16) *n16 = (uint64_t)d.n16 + (uint64_t)d.nd;
     if (n32) *n32 = (uint64_t)d.n32;
@@ -662,7 +884,7 @@ esch_status esch_batch_stats(const esch_batch *b, uint64_t *cells, uint64_t *n16
 esch_status esch_batch_stats_diag(const esch_batch *b, uint64_t *n_diag) {
     if (!b || !n_diag) return fail(ESCH_INVALID_ARG, "null argument");
     DevDesc d;
-    HIPC(hipMemcpy(&d, b->d_desc, sizeof d, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(&d, scoring_desc(b), sizeof d, hipMemcpyDeviceToHost));
     *n_diag = (uint64_t)d.nd;
     return ESCH_OK;
 }
@@ -870,6 +1092,16 @@ esch_status esch_set_variant(esch_scorer *s, int variant) {
 }
 
 int esch_get_variant(const esch_scorer *s) { return s ? s->geo.variant16 : -1; }
+
+esch_status esch_set_swap_rows(esch_scorer *s, int32_t rows) {
+    if (!s || rows < -1) return fail(ESCH_INVALID_ARG, "swap rows must be -1 (the rule), 0 (never) or a row count");
+    s->swap_rows = rows;
+    return ESCH_OK;
+}
+
+int32_t esch_get_swap_rows(const esch_scorer *s) { return s ? (int32_t)swap_limit(s) : -1; }
+
+int esch_batch_swapped(const esch_batch *b) { return b && b->last_swapped ? 1 : 0; }
 
 esch_status esch_set_diag_min_len(esch_scorer *s, int32_t min_len) {
     if (!s || min_len < 0) return fail(ESCH_INVALID_ARG, "bad anti-diagonal threshold");

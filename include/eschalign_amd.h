@@ -198,6 +198,21 @@ int32_t esch_get_diag_min_len(const esch_scorer *s);
  * way. */
 esch_status esch_set_diag_ratio(esch_scorer *s, int32_t ratio_x16);
 int32_t esch_get_diag_ratio(const esch_scorer *s);
+/* A batch of up to `rows` rows (reads x strands; by default as many rows
+ * as the panel has references, when it has at least 4,096, and none
+ * otherwise) is scored with the roles swapped: the panel's references as
+ * the queries, bucketed into well-filled 128-row chunks, and the batch's
+ * rows as the references. The score of a read against a reference is the
+ * reference's against the read (the same substitutions, gaps, zero borders
+ * and best cell), so the results are identical; a few reads against a
+ * large panel no longer pay a whole chunk per group of references, and no
+ * read pads another. Only for non-empty rows no longer
+ * than the panel's longest reference, int32 scores (the fused selection
+ * included), and no length threshold set (esch_set_diag_min_len).
+ * rows: -1 restores the rule, 0 turns it off. esch_get_swap_rows returns
+ * the limit in force. */
+esch_status esch_set_swap_rows(esch_scorer *s, int32_t rows);
+int32_t esch_get_swap_rows(const esch_scorer *s);
 
 /* adapter::find — lib/adapter.rs:68-91. Letters (not codes): how many bases
  * at the 3' end of `read` are the adapter's prefix (0 = none). The pre-step
@@ -437,6 +452,8 @@ esch_status esch_batch_stats_diag(const esch_batch *b, uint64_t *n_diag);
 esch_status esch_batch_read_scores32(esch_batch *b, int32_t *scores, uint8_t *scored);
 esch_status esch_batch_read_scores16(esch_batch *b, int16_t *scores, uint8_t *scored);
 esch_status esch_batch_read_select(esch_batch *b, esch_read_select *out, int32_t *ties);
+/* 1 when the batch's last run scored with the roles swapped (esch_set_swap_rows). */
+int esch_batch_swapped(const esch_batch *b);
 esch_status esch_batch_read_pairs(esch_batch *b, esch_pair_select *out, int32_t *top);
 
 /* ---------------------------------------------------------------------

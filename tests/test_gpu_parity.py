@@ -595,6 +595,7 @@ def test_small_batch_against_a_wide_panel_takes_the_anti_diagonal_kernel():
     }
     for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
         gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+        gpu.swap_rows = 0  # the swapped path would take these batches first
         blocks16 = gpu.geometry()[2]
         for name, reads in cases.items():
             queries = codes_of(reads)
@@ -608,6 +609,96 @@ def test_small_batch_against_a_wide_panel_takes_the_anti_diagonal_kernel():
                 assert nd == 0
             elif name == "six reads":
                 assert nd == len(queries)
+
+
+def _scores(gpu, queries):
+    b = gpu.batch(len(queries), sum(len(q) for q in queries))
+    b.load(queries)
+    b.run(ea.RUN_SCORES32)
+    got, ok = b.scores32()
+    swapped = b.swapped()
+    cells = b.stats()[0]
+    b.close()
+    return got, ok, swapped, cells
+
+
+def test_swapped_roles_equal_the_oracle():
+    """A few reads against many references are scored with the roles
+    swapped (the references as the queries, the reads as a panel): every
+    scheme of the parity suite, both modes, wildcards in reads and panel,
+    reads shorter and longer than most references, the int32 matrix and the
+    fused selection on both strands, equal to the oracle and to the ordinary
+    path; a high-scoring scheme whose long references exceed the int16
+    bound takes the int32 rescue with the roles swapped too."""
+    rng = G.Rng(0x5EED_0B06)
+    panel_l = G.random_panel(rng, 300, 40, 460)
+    reads = [G.random_seq(rng, 1 + rng.below(30), 0.05) for _ in range(6)]
+    reads += [G.chimeric_read(rng, panel_l, 60 + rng.below(300)) for _ in range(14)]
+    reads.append(panel_l[7])
+    assert max(map(len, reads)) <= max(map(len, panel_l))
+    queries = codes_of(reads)
+    panel = ea.Panel(panel_l)
+    for scoring in G.SCHEMES_GPU:
+        for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
+            gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+            assert gpu.swap_rows == 0  # by the rule only from 4,096 references
+            gpu.swap_rows = 127
+            want = oracle(queries, panel.codes, scoring, mode)
+            got, ok, swapped, cells = _scores(gpu, queries)
+            assert swapped and ok.all(), (scoring, mode)
+            bad = np.argwhere(got != want)
+            assert bad.size == 0, (scoring, mode, bad[:4])
+            gpu.swap_rows = 0
+            got0, ok0, swapped0, cells0 = _scores(gpu, queries)
+            assert not swapped0 and ok0.all() and np.array_equal(got0, want) and cells0 == cells
+            sel0 = gpu.select_batch(queries, both_strands=True, min_score=3, max_ties=2)
+            gpu.swap_rows = 127
+            sel = gpu.select_batch(queries, both_strands=True, min_score=3, max_ties=2)
+            assert sel == sel0, (scoring, mode)
+            n = len(queries)
+            rows = queries + [ea.reverse_complement_codes(q) for q in queries]
+            sm = oracle(rows, panel.codes, scoring, mode)
+            for k in range(n):
+                b, nt, so, take = O.select(sm[k], sm[n + k], 3, 2)
+                assert (sel[k].best, sel[k].n_tied, sel[k].suboptimal, sel[k].take) == (b, nt, so, take), k
+    # the int32 rescue with the roles swapped: long references at a high
+    # match score exceed the int16 bound
+    big = [G.random_seq(rng, 1900 + rng.below(100), 0.01) for _ in range(3)] + G.random_panel(rng, 260, 40, 200)
+    long_reads = [big[0][100:1800], big[2][:1500]] + [G.chimeric_read(rng, big[3:], 300) for _ in range(3)]
+    lq = codes_of(long_reads)
+    panel = ea.Panel(big)
+    for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
+        gpu = ea.GpuScorer(panel, ea.Scoring(20, -1, -10, -1), mode)
+        gpu.swap_rows = 64
+        want = oracle(lq, panel.codes, (20, -1, -10, -1), mode)
+        got, ok, swapped, _ = _scores(gpu, lq)
+        assert swapped and ok.all() and np.array_equal(got, want), mode
+        assert want.max() > 32000
+
+
+
+
+def test_swapped_roles_by_the_rule_on_a_wide_panel():
+    """By the default rule a panel of at least 4,096 references swaps roles
+    for batches of up to one row per reference: 40 reads on both strands
+    against 4,100 references, fused selection, equal to the ordinary path
+    and, on a sample, to the oracle."""
+    rng = G.Rng(0x5EED_0B07)
+    panel_l = G.random_panel(rng, 4100, 60, 120)
+    reads = [G.chimeric_read(rng, panel_l, 40 + rng.below(120)) for _ in range(40)]
+    queries = codes_of(reads)
+    panel = ea.Panel(panel_l)
+    gpu = ea.GpuScorer(panel, ea.Scoring(2, -1, -10, -1), ea.Mode.LOCAL)
+    assert gpu.swap_rows == 4100
+    got, ok, swapped, _ = _scores(gpu, queries)
+    assert swapped and ok.all()
+    sel = gpu.select_batch(queries, both_strands=True, max_ties=3)
+    gpu.swap_rows = 0
+    got0, ok0, swapped0, _ = _scores(gpu, queries)
+    assert not swapped0 and np.array_equal(got, got0)
+    assert gpu.select_batch(queries, both_strands=True, max_ties=3) == sel
+    want = oracle(queries[:6], panel.codes, (2, -1, -10, -1), ea.Mode.LOCAL)
+    assert np.array_equal(got[:6], want)
 
 
 def test_autotune_sweeps_the_routing_ratio():
