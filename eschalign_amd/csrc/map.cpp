@@ -341,15 +341,31 @@ esch_status map_reads_core(esch_scorer *s, size_t n, const Getter &get, int both
         esch_status st = cx.map_rows(rows, both, min_score, max_ties, maps, pool);
         give_batch(s, b);
         if (st) return st;
+        // The pool holds this chunk's hits in read order already: it
+        // becomes the result's (moved for the first chunk, appended with
+        // its offsets moved on after that), and a read without hits points
+        // where the next read's hits start.
         const auto ta = Clock::now();
-        M->hits.reserve(M->hits.size() + pool.hits.size());
-        M->ops.reserve(M->ops.size() + pool.ops.size());
+        const int64_t hb = (int64_t)M->hits.size(), ob = (int64_t)M->ops.size();
+        if (hb == 0 && ob == 0) {
+            M->hits.swap(pool.hits);
+            M->ops.swap(pool.ops);
+        } else {
+            for (esch_hit &h : pool.hits) h.alignment.ops_offset += ob;
+            M->hits.insert(M->hits.end(), pool.hits.begin(), pool.hits.end());
+            M->ops.insert(M->ops.end(), pool.ops.begin(), pool.ops.end());
+        }
         for (size_t k = k0; k < k1; ++k) M->reads[k] = esch_read_mapping{};
-        for (size_t e = 0; e < rows.size(); ++e) M->reads[k0 + (size_t)rows[e]] = maps[e];
-        for (size_t k = k0; k < k1; ++k) {
+        for (size_t e = 0; e < rows.size(); ++e) {
+            esch_read_mapping &m = M->reads[k0 + (size_t)rows[e]];
+            m = maps[e];
+            m.hit_offset += hb;
+        }
+        int64_t next = (int64_t)M->hits.size();  // reads without hits, from the back
+        for (size_t k = k1; k-- > k0;) {
             esch_read_mapping &m = M->reads[k];
-            if (m.n_hits) place_read(M.get(), pool, m);
-            else m.hit_offset = (int64_t)M->hits.size();
+            if (m.n_hits) next = m.hit_offset;
+            else m.hit_offset = next;
         }
         M->stats.assemble_ms += ms_since(ta);
     }

@@ -349,6 +349,18 @@ struct SwapBufs {
     int32_t *d_sorted16 = nullptr, *d_list32 = nullptr, *d_listD = nullptr;
 };
 
+static void swap_release(SwapBufs *w) {
+    (void)hipFree(w->d_block);
+    (void)hipHostFree(w->h_stage);
+    (void)hipFree(w->d_cls);
+    (void)hipFree(w->d_hist);
+    (void)hipFree(w->d_bin_start);
+    (void)hipFree(w->d_bin_fill);
+    (void)hipFree(w->d_sorted16);
+    (void)hipFree(w->d_list32);
+    (void)hipFree(w->d_listD);
+}
+
 static void swap_layout(size_t max_reads, size_t max_bases, SwapBufs &w) {
     w.cap_rows = 2 * std::max<size_t>(max_reads, 1);
     const size_t bases = 2 * std::max<size_t>(max_bases, 1);
@@ -459,17 +471,9 @@ void esch_batch_free(esch_batch *b) {
     if (b->ev2) (void)hipEventDestroy(b->ev2);
     if (b->ev3) (void)hipEventDestroy(b->ev3);
     if (b->stream) (void)hipStreamDestroy(b->stream);
-    if (SwapBufs *w = b->swap) {
-        (void)hipFree(w->d_block);
-        (void)hipHostFree(w->h_stage);
-        (void)hipFree(w->d_cls);
-        (void)hipFree(w->d_hist);
-        (void)hipFree(w->d_bin_start);
-        (void)hipFree(w->d_bin_fill);
-        (void)hipFree(w->d_sorted16);
-        (void)hipFree(w->d_list32);
-        (void)hipFree(w->d_listD);
-        delete w;
+    if (b->swap) {
+        swap_release(b->swap);
+        delete b->swap;
     }
     delete b;
 }
@@ -654,8 +658,12 @@ esch_status swap_ensure(esch_batch *b) {
     dev(&w->d_listD, 4 * nq);
     if (!st && hipHostMalloc(&w->h_stage, w->bytes, hipHostMallocDefault) != hipSuccess)
         st = fail(ESCH_DRIVER, "hipHostMalloc failed");
-    b->swap = w.release();  // freed with the batch, also when incomplete
-    return st;
+    if (st) {  // nothing half-built stays behind: the next run tries again
+        swap_release(w.get());
+        return st;
+    }
+    b->swap = w.release();
+    return ESCH_OK;
 }
 
 // Score the batch's rows against the panel with the roles swapped: the
