@@ -1283,13 +1283,11 @@ struct TraceCache {
     hipStream_t st = nullptr, st2 = nullptr;  // one-lane kernel; wavefront kernel beside it
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
     DevBuf codes, qoff, qlen, ref, slot, gbase, gS, gM, dbase, arena, arena2, recs, ops_wc, ops_off, out_off, out;
-    HostBuf h_codes, h_recs, h_off, h_ops;
+    HostBuf h_codes, h_recs, h_off, h_ops, h_qoff, h_qlen, h_ref, h_slot, h_opsoff;
     ~TraceCache() {
         if (st) (void)hipStreamSynchronize(st);
-        h_codes.release();
-        h_recs.release();
-        h_off.release();
-        h_ops.release();
+        for (HostBuf *h : {&h_codes, &h_recs, &h_off, &h_ops, &h_qoff, &h_qlen, &h_ref, &h_slot, &h_opsoff})
+            h->release();
         if (st2) (void)hipStreamSynchronize(st2);
         for (DevBuf *b : {&codes, &qoff, &qlen, &ref, &slot, &gbase, &gS, &gM, &dbase, &arena, &arena2, &recs,
                           &ops_wc, &ops_off, &out_off, &out})
@@ -1316,48 +1314,124 @@ esch_status esch::align_core(esch_scorer *s, size_t n,
                              const int32_t *refs, esch_alignment *out, uint32_t *ops, size_t ops_cap,
                              size_t *ops_used, const uint8_t *dev_codes, std::vector<uint32_t> *ops_vec) {
     if (ops_used) *ops_used = 0;
-    std::vector<size_t> lens(n);
-    std::vector<const uint8_t *> ptr(n);
-    size_t bases = 0, np = 0, max_len = 0;
-    for (size_t k = 0; k < n; ++k) {
-        get(k, ptr[k], lens[k]);
-        const size_t len = lens[k];
-        if (refs[k] < 0 || (size_t)refs[k] >= s->n_refs)
-            return fail(ESCH_INVALID_ARG, "pair %zu names reference %d of %zu", k, refs[k], s->n_refs);
-        // one lane a pair: a group region is 64 pairs x (read / 16 stripes)
-        // x reference x 8 B; a larger pair takes the wavefront kernel, whose
-        // region (read x (reference + 63) / 2 B) must fit the arena
-        const int rl = s->h_rlen[refs[k]];
-        if ((uint64_t)len * (uint64_t)rl > kTraceMaxCells &&
-            (rl > kTraceDiagMaxRef || trace_diag_bytes(len, rl) > kTraceArenaMax))
-            return fail(ESCH_INVALID_ARG, "pair %zu: %zu x %d cells is over the device traceback bound", k, len, rl);
-        if (len && !ptr[k]) return fail(ESCH_INVALID_ARG, "pair %zu: null read", k);
-        out[k] = esch_alignment{};  // lib/scalar.rs:185-195: empty in, empty out
-        if (len) {
-            ++np;
-            bases += len;
-            max_len = std::max(max_len, len);
+    std::unique_ptr<size_t[]> lens(new size_t[std::max<size_t>(n, 1)]);
+    std::unique_ptr<const uint8_t *[]> ptr(new const uint8_t *[std::max<size_t>(n, 1)]);
+    // The caller's pairs, checked on up to 8 host threads (a mapping call
+    // traces millions); an error names the first bad pair, raised here on
+    // the calling thread (the message is per thread).
+    struct Part {
+        size_t np = 0, bases = 0, max_len = 0, bad = SIZE_MAX;
+        uint64_t W = 0, max_cells = 0;
+        int why = 0;
+    };
+    const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nt = n < ((size_t)1 << 18) ? 1 : std::min<size_t>(8, hw);
+    std::vector<Part> part(nt);
+    auto check = [&](size_t t) {
+        Part &P = part[t];
+        for (size_t k = n * t / nt; k < n * (t + 1) / nt; ++k) {
+            get(k, ptr[k], lens[k]);
+            const size_t len = lens[k];
+            if (refs[k] < 0 || (size_t)refs[k] >= s->n_refs) {
+                P.bad = k;
+                P.why = 1;
+                return;
+            }
+            // one lane a pair: a group region is 64 pairs x (read / 16
+            // stripes) x reference x 8 B; a larger pair takes the wavefront
+            // kernel, whose region (read x (reference + 63) / 2 B) must fit
+            // the arena
+            const int rl = s->h_rlen[refs[k]];
+            const uint64_t cells = (uint64_t)len * (uint64_t)rl;
+            if (cells > kTraceMaxCells && (rl > kTraceDiagMaxRef || trace_diag_bytes(len, rl) > kTraceArenaMax)) {
+                P.bad = k;
+                P.why = 2;
+                return;
+            }
+            if (len && !ptr[k]) {
+                P.bad = k;
+                P.why = 3;
+                return;
+            }
+            out[k] = esch_alignment{};  // lib/scalar.rs:185-195: empty in, empty out
+            if (len) {
+                ++P.np;
+                P.bases += len;
+                P.max_len = std::max(P.max_len, len);
+                P.W += cells;
+                P.max_cells = std::max(P.max_cells, cells);
+            }
         }
+    };
+    {
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < nt; ++t) pool.emplace_back(check, t);
+        check(0);
+        for (auto &th : pool) th.join();
+    }
+    size_t np = 0, bases = 0, max_len = 0;
+    uint64_t W = 0, max_cells = 0;
+    for (const Part &P : part) {
+        if (P.bad != SIZE_MAX) {
+            const size_t k = P.bad;
+            if (P.why == 1) return fail(ESCH_INVALID_ARG, "pair %zu names reference %d of %zu", k, refs[k], s->n_refs);
+            if (P.why == 2)
+                return fail(ESCH_INVALID_ARG, "pair %zu: %zu x %d cells is over the device traceback bound", k,
+                            lens[k], s->h_rlen[refs[k]]);
+            return fail(ESCH_INVALID_ARG, "pair %zu: null read", k);
+        }
+        np += P.np;
+        bases += P.bases;
+        max_len = std::max(max_len, P.max_len);
+        W += P.W;
+        max_cells = std::max(max_cells, P.max_cells);
     }
     if (np == 0) return ESCH_OK;
     // Groups of similar read and reference lengths (lib/simd/mod.rs:433
     // sorts ascending), longest first: a wave's time is its group's longest
     // read x longest reference, so the longest groups are dispatched first.
-    // Two stable counting sorts: by reference length, then by read length.
-    std::vector<int32_t> order(np), tmp(np);
+    // Order: read length descending, then reference length descending,
+    // stable; one counting sort on the pair of lengths when its bins are
+    // few, else two (by reference length, then by read length).
+    std::vector<int32_t> order(np);
     {
-        std::vector<size_t> cnt((size_t)s->max_len + 2, 0);
-        for (size_t k = 0; k < n; ++k)
-            if (lens[k]) ++cnt[s->max_len - (size_t)s->h_rlen[refs[k]]];
-        size_t run = 0;
-        for (size_t& c : cnt) { const size_t x = c; c = run; run += x; }
-        for (size_t k = 0; k < n; ++k)
-            if (lens[k]) tmp[cnt[s->max_len - (size_t)s->h_rlen[refs[k]]]++] = (int32_t)k;
-        std::vector<size_t> cl(max_len + 2, 0);
-        for (size_t i = 0; i < np; ++i) ++cl[max_len - lens[tmp[i]]];
-        run = 0;
-        for (size_t& c : cl) { const size_t x = c; c = run; run += x; }
-        for (size_t i = 0; i < np; ++i) order[cl[max_len - lens[tmp[i]]]++] = tmp[i];
+        const size_t RM = (size_t)s->max_len + 1, QM = max_len + 1;
+        if (RM * QM <= ((size_t)1 << 20)) {
+            std::vector<uint32_t> cnt(RM * QM + 1, 0);
+            auto key = [&](size_t k) { return (max_len - lens[k]) * RM + ((size_t)s->max_len - (size_t)s->h_rlen[refs[k]]); };
+            for (size_t k = 0; k < n; ++k)
+                if (lens[k]) ++cnt[key(k)];
+            uint32_t run = 0;
+            for (uint32_t &c : cnt) {
+                const uint32_t x = c;
+                c = run;
+                run += x;
+            }
+            for (size_t k = 0; k < n; ++k)
+                if (lens[k]) order[cnt[key(k)]++] = (int32_t)k;
+        } else {
+            std::vector<int32_t> tmp(np);
+            std::vector<size_t> cnt((size_t)s->max_len + 2, 0);
+            for (size_t k = 0; k < n; ++k)
+                if (lens[k]) ++cnt[s->max_len - (size_t)s->h_rlen[refs[k]]];
+            size_t run = 0;
+            for (size_t &c : cnt) {
+                const size_t x = c;
+                c = run;
+                run += x;
+            }
+            for (size_t k = 0; k < n; ++k)
+                if (lens[k]) tmp[cnt[s->max_len - (size_t)s->h_rlen[refs[k]]]++] = (int32_t)k;
+            std::vector<size_t> cl(max_len + 2, 0);
+            for (size_t i = 0; i < np; ++i) ++cl[max_len - lens[tmp[i]]];
+            run = 0;
+            for (size_t &c : cl) {
+                const size_t x = c;
+                c = run;
+                run += x;
+            }
+            for (size_t i = 0; i < np; ++i) order[cl[max_len - lens[tmp[i]]]++] = tmp[i];
+        }
     }
     // Pairs for the wavefront kernel first: those a lane could not hold, and
     // those long enough that one lane would set the call's time: more cells
@@ -1366,13 +1440,13 @@ esch_status esch::align_core(esch_scorer *s, size_t n,
     // few other waves), and at least 2^15 (below that the wave per pair of
     // the wavefront kernel costs more than it saves). Measured on the 1M-
     // and 200k-read winner sets: 17.5 and 5.1 ms against 22.2 and 12.1 ms
-    // with every pair on the one-lane kernel.
+    // with every pair on the one-lane kernel. No pair qualifies (the usual
+    // case for short reads) when the largest is under both bounds.
     size_t n_diag = 0;
-    {
-        uint64_t W = 0;
-        for (size_t i = 0; i < np; ++i) W += (uint64_t)lens[order[i]] * (uint64_t)s->h_rlen[refs[order[i]]];
-        const uint64_t thr = std::max<uint64_t>(kTraceDiagMinCells, W >> 18);
+    const uint64_t thr = std::max<uint64_t>(kTraceDiagMinCells, W >> 18);
+    if (max_cells >= thr || max_cells > kTraceMaxCells) {
         std::vector<int32_t> dl, rest;
+        rest.reserve(np);
         for (size_t i = 0; i < np; ++i) {
             const int32_t k = order[i];
             const int rl = s->h_rlen[refs[k]];
@@ -1393,12 +1467,20 @@ esch_status esch::align_core(esch_scorer *s, size_t n,
     if (T.st) HIPC(hipStreamSynchronize(T.st));  // the pinned staging may still feed a copy
     if (!dev_codes) HIPC(T.h_codes.need(bases));
     HIPC(T.h_recs.need(n * kTraceRec * 4));
+    // per sorted pair (pinned, kept between calls): read offset, length,
+    // reference, caller index; per caller pair: its worst-case runs' offset
+    HIPC(T.h_qoff.need(np * 8));
+    HIPC(T.h_qlen.need(np * 4));
+    HIPC(T.h_ref.need(np * 4));
+    HIPC(T.h_slot.need(np * 4));
+    HIPC(T.h_opsoff.need(n * 8));
     uint8_t *const hcodes = T.h_codes.as<uint8_t>();
-    std::vector<uint64_t> hqoff(np), ops_off(n, 0);
-    std::vector<int32_t> hqlen(np), href(np), hslot(np);
+    uint64_t *const hqoff = T.h_qoff.as<uint64_t>();
+    int32_t *const hqlen = T.h_qlen.as<int32_t>(), *const href = T.h_ref.as<int32_t>();
+    int32_t *const hslot = T.h_slot.as<int32_t>();
+    uint64_t *const ops_off = T.h_opsoff.as<uint64_t>();
     {
-        // per sorted pair: its read, length, reference and caller index
-        // (random reads of the caller's arrays, so spread over threads)
+        // random reads of the caller's arrays, so spread over threads
         auto fill = [&](size_t i0, size_t i1) {
             for (size_t i = i0; i < i1; ++i) {
                 const int32_t k = order[i];
@@ -1408,11 +1490,10 @@ esch_status esch::align_core(esch_scorer *s, size_t n,
                 hslot[i] = k;
             }
         };
-        const size_t hw = std::max(1u, std::thread::hardware_concurrency());
-        const size_t nt = np < ((size_t)1 << 18) ? 1 : std::min<size_t>(8, hw);
+        const size_t ft = np < ((size_t)1 << 18) ? 1 : std::min<size_t>(8, hw);
         std::vector<std::thread> pool;
-        for (size_t t = 1; t < nt; ++t) pool.emplace_back(fill, np * t / nt, np * (t + 1) / nt);
-        fill(0, np / nt);
+        for (size_t t = 1; t < ft; ++t) pool.emplace_back(fill, np * t / ft, np * (t + 1) / ft);
+        fill(0, np / ft);
         for (auto &th : pool) th.join();
         if (!dev_codes) {  // staged reads are packed in sorted order
             size_t at = 0;
@@ -1425,11 +1506,10 @@ esch_status esch::align_core(esch_scorer *s, size_t n,
     // the gather into staging and the code check on up to 8 host threads
     // (resident reads were checked when they were loaded)
     if (!dev_codes) {
-        const size_t hw = std::max(1u, std::thread::hardware_concurrency());
-        const size_t nt = bases < ((size_t)8 << 20) ? 1 : std::min<size_t>(8, hw);
-        std::vector<uint8_t> bad(nt, 0);
+        const size_t gt = bases < ((size_t)8 << 20) ? 1 : std::min<size_t>(8, hw);
+        std::vector<uint8_t> bad(gt, 0);
         auto gather = [&](size_t t) {
-            const size_t i0 = np * t / nt, i1 = np * (t + 1) / nt;
+            const size_t i0 = np * t / gt, i1 = np * (t + 1) / gt;
             uint8_t mx = 0;
             for (size_t i = i0; i < i1; ++i) {
                 const uint8_t *q = ptr[order[i]];
@@ -1439,15 +1519,15 @@ esch_status esch::align_core(esch_scorer *s, size_t n,
             }
             bad[t] = mx > 4;
         };
-        if (nt == 1) {
+        if (gt == 1) {
             gather(0);
         } else {
             std::vector<std::thread> pool;
-            for (size_t t = 1; t < nt; ++t) pool.emplace_back(gather, t);
+            for (size_t t = 1; t < gt; ++t) pool.emplace_back(gather, t);
             gather(0);
             for (auto &th : pool) th.join();
         }
-        for (size_t t = 0; t < nt; ++t) {
+        for (size_t t = 0; t < gt; ++t) {
             if (!bad[t]) continue;
             for (size_t k = 0; k < n; ++k)
                 for (size_t j = 0; j < lens[k]; ++j)
@@ -1530,15 +1610,15 @@ esch_status esch::align_core(esch_scorer *s, size_t n,
     HIPC(T.ops_off.need(n * 8));
     hipStream_t st = T.st;
     if (!dev_codes) HIPC(hipMemcpyAsync(T.codes.p, hcodes, bases, hipMemcpyHostToDevice, st));
-    HIPC(hipMemcpyAsync(T.qoff.p, hqoff.data(), np * 8, hipMemcpyHostToDevice, st));
-    HIPC(hipMemcpyAsync(T.qlen.p, hqlen.data(), np * 4, hipMemcpyHostToDevice, st));
-    HIPC(hipMemcpyAsync(T.ref.p, href.data(), np * 4, hipMemcpyHostToDevice, st));
-    HIPC(hipMemcpyAsync(T.slot.p, hslot.data(), np * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(T.qoff.p, hqoff, np * 8, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(T.qlen.p, hqlen, np * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(T.ref.p, href, np * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(T.slot.p, hslot, np * 4, hipMemcpyHostToDevice, st));
     HIPC(hipMemcpyAsync(T.gbase.p, gbase.data(), (size_t)ng * 8, hipMemcpyHostToDevice, st));
     HIPC(hipMemcpyAsync(T.gS.p, gS.data(), (size_t)ng * 4, hipMemcpyHostToDevice, st));
     HIPC(hipMemcpyAsync(T.gM.p, gM.data(), (size_t)ng * 4, hipMemcpyHostToDevice, st));
     if (n_diag) HIPC(hipMemcpyAsync(T.dbase.p, dbase.data(), n_diag * 8, hipMemcpyHostToDevice, st));
-    HIPC(hipMemcpyAsync(T.ops_off.p, ops_off.data(), n * 8, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(T.ops_off.p, ops_off, n * 8, hipMemcpyHostToDevice, st));
     // empty pairs are never traced: their records must read as zero runs
     HIPC(hipMemsetAsync(T.recs.p, 0, n * kTraceRec * 4, st));
     TraceArgs ta{s->panel, s->sc, dev_codes ? dev_codes : T.codes.as<uint8_t>(), T.qoff.as<uint64_t>(), T.qlen.as<int32_t>(),
