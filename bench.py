@@ -41,11 +41,16 @@ PEAK_I16 = N_CU * CLOCK_HZ * 128           # lane-ops/s
 PEAK_I32 = N_CU * CLOCK_HZ * 64
 # Minimum packed 16-bit lane-ops per DP cell of the recurrence: 4 adds and
 # 4 two-input maxima in semiglobal mode, plus the zero floor and the running
-# best in local mode; with the fp16 variants' three-input maximum (4 and 5)
-# H takes one max instead of two, and the column best a three-way tree.
-OPS_PER_CELL = {("semiglobal", False): 8, ("local", False): 10,
-                ("semiglobal", True): 7, ("local", True): 8.5}
-F16_VARIANTS = (4, 5)  # k_dp16 variants that compute in fp16
+# best in local mode; with the fp16 variants' three-input maximum (4, 5, 7)
+# H takes one max instead of two, and the column best a three-way tree. The
+# row-shifted semiglobal column (6, 7) drops the vertical gap's extension
+# add: one op less. Keyed (mode, fp16, row-shifted).
+OPS_PER_CELL = {("semiglobal", False, False): 8, ("local", False, False): 10,
+                ("semiglobal", True, False): 7, ("local", True, False): 8.5,
+                ("semiglobal", False, True): 7, ("local", False, True): 10,
+                ("semiglobal", True, True): 6, ("local", True, True): 8.5}
+F16_VARIANTS = (4, 5, 7)  # k_dp16 variants that compute in fp16
+SHIFTED_VARIANTS = (6, 7)  # and those with the row-shifted semiglobal column
 # SQ_INSTS_VALU x 128 / cells of one k_dp16 launch of the default workload,
 # from the counter summary in profiles/ (mode, variant) -> lane-ops per cell
 VALU_OPS_PER_CELL_ISSUED = {("semiglobal", 3): round(344_021_737_036 * 128 / 5_303_272_453_076, 3),
@@ -541,16 +546,19 @@ def main():
     dp = float(np.mean(dp_ms)) / 1e3
     gcups = cells / dp / 1e9
     f16 = chosen in F16_VARIANTS
-    ops = OPS_PER_CELL[(args.mode, f16)]
+    ops = OPS_PER_CELL[(args.mode, f16, chosen in SHIFTED_VARIANTS)]
     achieved = cells * ops / dp
     log(f"[bench] {value:,.0f} reads/s, {ms_step:.1f} ms/step, DP {dp * 1e3:.1f} ms "
         f"({gcups:,.0f} GCUPS), {n16} int16 / {n_diag} anti-diagonal / {n32} int32 queries")
 
-    # With an fp16 variant chosen, the same steps again with the int16
-    # default (variant 3), so both arithmetics' rates are on record.
+    # With an fp16 variant chosen, the same steps again with its int16
+    # counterpart (variant 6 for the row-shifted 7, else 3), so both
+    # arithmetics' rates are on record.
     int16 = None
+    shifted = chosen in SHIFTED_VARIANTS
+    v16 = 6 if shifted else 3
     if f16:
-        gpu.variant = 3
+        gpu.variant = v16
         batch.run(what, tie_cap=tie_cap)
         batch.wait()
         d16 = []
@@ -562,12 +570,12 @@ def main():
         w16 = time.perf_counter() - t
         gpu.variant = chosen
         dp16 = float(np.mean(d16)) / 1e3
-        ops16 = OPS_PER_CELL[(args.mode, False)]
-        int16 = {"variant": 3, "reads_per_s": round(args.reads / (w16 / args.steps), 1),
+        ops16 = OPS_PER_CELL[(args.mode, False, shifted)]
+        int16 = {"variant": v16, "reads_per_s": round(args.reads / (w16 / args.steps), 1),
                  "ms_per_step": round(w16 / args.steps * 1e3, 3), "dp_ms": round(dp16 * 1e3, 3),
                  "gcups": round(cells / dp16 / 1e9, 1), "ops_per_cell": ops16,
                  "roofline_frac": round(cells * ops16 / dp16 / PEAK_I16, 4)}
-        log(f"[bench] int16 variant 3 on the same batch: {int16}")
+        log(f"[bench] int16 variant {v16} on the same batch: {int16}")
 
     plain = None
     if not args.no_plain:
@@ -653,7 +661,7 @@ def main():
                      "traffic_source": traffic_src,
                      "ops_per_cell": ops,
                      # cells/s against the int16 kernel's bound (8 / 10 ops per cell)
-                     "frac_of_int16_roofline": round(cells * OPS_PER_CELL[(args.mode, False)] / dp / PEAK_I16, 4)},
+                     "frac_of_int16_roofline": round(cells * OPS_PER_CELL[(args.mode, False, shifted)] / dp / PEAK_I16, 4)},
         "host_load": {"ms": round(load_s * 1e3, 3), "bytes": int(lens.sum()),
                       "pcie_inclusive_reads_per_s": round(args.reads / (load_s + wall / args.steps), 1)},
         "kernel_variant": chosen,

@@ -32,6 +32,16 @@ __host__ __device__ inline bool fits_f16(int64_t max_ref_len, int32_t ma, int32_
     const int64_t O = o < 0 ? -(int64_t)o : o, Ee = e < 0 ? -(int64_t)e : e;
     return max_ref_len * (M > Mi ? M : Mi) + O + Ee <= kBoundF16;
 }
+// k_dp16's row-shifted semiglobal column (variants 6 and 7) holds a stripe
+// row r's values shifted by |e| r (r = -1 .. 15) and offset by o or e:
+// every value lies in [-(kM + |o| + 4|e|), kM + 16|e|], and the profile's
+// stripe-top entries s - o + e h (h <= 16) within M + |o| + 16|e|. With 17|e|
+// for the shift, the unshifted bounds carry over.
+__host__ __device__ inline int64_t shifted_bound(int64_t max_ref_len, int32_t ma, int32_t mi, int32_t o, int32_t e) {
+    const int64_t M = ma < 0 ? -(int64_t)ma : ma, Mi = mi < 0 ? -(int64_t)mi : mi;
+    const int64_t O = o < 0 ? -(int64_t)o : o, Ee = e < 0 ? -(int64_t)e : e;
+    return max_ref_len * (M > Mi ? M : Mi) + O + 17 * Ee;
+}
 // -inf of the int32 kernel, the oracle's i32::MIN / 4 (lib/scalar.rs:42).
 constexpr int32_t kNeg32 = INT32_MIN / 4;
 
@@ -127,13 +137,17 @@ struct Launch {
     int grid_aux; // blocks of the bucketing / selection kernels
     int variant16;  // k_dp16 variant: 0 profile per column, 1 ping-pong, 2 pair schedule, 3 quad schedule,
                     // 4 quad schedule in fp16, 5 eight-column schedule in fp16 (the
-                    // fp16 variants: panels within kBoundF16 only)
+                    // fp16 variants: panels within kBoundF16 only), 6 and 7: 3 and 5
+                    // with the row-shifted semiglobal column (shifted_bound)
     int grid_diag;  // persistent waves of the anti-diagonal kernel
     size_t diag_lds;  // k_diag64's LDS boundary bytes; 0: k_diag16 (global scratch)
 };
-constexpr int kDp16Variants = 6;
+constexpr int kDp16Variants = 8;
 constexpr int kDp16Default = 3;  // the int16 default; panels within kBoundF16 take kDp16F16
-constexpr int kDp16F16 = 4;      // variants kDp16F16 and up compute in fp16
+constexpr int kDp16F16 = 4;      // the first variant computing in fp16
+// variants computing in fp16, and those with the row-shifted semiglobal column
+constexpr bool dp16_is_f16(int v) { return v == 4 || v == 5 || v == 7; }
+constexpr bool dp16_is_shifted(int v) { return v == 6 || v == 7; }
 hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st);
 hipError_t launch_dp16(const KArgs &a, const Launch &g, bool local, hipStream_t st);
 hipError_t launch_dp32(const KArgs &a, const Launch &g, bool local, int which, hipStream_t st);

@@ -60,6 +60,7 @@ struct esch_scorer {
     int max_grid16 = 0;                 // scratch is sized for the largest
     int regs16 = 0, local16 = 0;
     bool f16_ok = false;             // the panel fits k_dp16's fp16 variant (fits_f16)
+    bool usable16[kDp16Variants] = {};  // variants exact on this panel (fp16 and shift bounds)
     int diag_min_len = 0;               // reads this long take k_diag16 (0: never)
     int diag_ratio = kDiagRatioDefault; // padding-heavy length bins take k_diag16 (0: never)
     bool diag_ratio_tunable = true;     // false once the caller set the ratio

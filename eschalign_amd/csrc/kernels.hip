@@ -180,6 +180,106 @@ __device__ __forceinline__ void column16(const typename Pk<F16>::T (&sc)[RR], ty
     }
 }
 
+// The semiglobal column with every value of stripe row r shifted by -e r
+// (r = 0 .. RR-1 within the stripe; -1 for the row above), which takes the
+// extension out of the vertical gap chain. In this space
+//   P = H - e r + o,  R = E - e r + e,  Q = F - e r + e
+// and the recurrence of the header becomes
+//   R[j]    = max(P[j-1], R[j-1]) + e
+//   Q[r]    = max(P[r-1], Q[r-1])
+//   P[r][j] = max(P[r-1][j-1] + (s - o), R, Q) + (o - e)
+// (expand each side: the shifts cancel), 6 packed ops per row where the
+// unshifted column takes 7 (F's extension), and the profile is unchanged
+// (s - o). The row above arrives from the previous stripe in that stripe's
+// space, whose bottom row r' = h - 1 (h its height) is this stripe's row -1:
+// values there are lower by c = e h here. ADJ adds c to Q at row 0, and
+// the profile's row 0 carries + c for the diagonal term.
+template <int RR, bool ADJ, bool F16>
+__device__ __forceinline__ void column16s(const typename Pk<F16>::T (&sc)[RR], typename Pk<F16>::T (&Pv)[RR],
+                                          typename Pk<F16>::T (&R)[RR], typename Pk<F16>::T &pu,
+                                          typename Pk<F16>::T &q, typename Pk<F16>::T pd,
+                                          typename Pk<F16>::T voe, typename Pk<F16>::T ve,
+                                          typename Pk<F16>::T vc) {
+    using P = Pk<F16>;
+#pragma unroll
+    for (int r = 0; r < RR; ++r) {
+        R[r] = P::add(P::max(Pv[r], R[r]), ve);
+        q = P::max(pu, q);
+        if constexpr (ADJ) {
+            if (r == 0) q = P::add(q, vc);
+        }
+        const typename P::T h = P::max3(P::add(pd, sc[r]), R[r], q);
+        pd = Pv[r];
+        Pv[r] = P::add(h, voe);
+        pu = Pv[r];
+    }
+}
+
+// Rows q*4 .. q*4+3 of a profile plane (q < RR / 4), or its last two rows
+// (q = RR / 4, RR % 4 == 2), from the plane at pc (this lane's column).
+template <int RR, bool F16>
+__device__ __forceinline__ void plane_part(const uint4 *pc, int q, typename Pk<F16>::T (&sc)[RR]) {
+    using P = Pk<F16>;
+    if (4 * q + 4 <= RR) {
+        const uint4 v = pc[q * 64];
+        sc[4 * q + 0] = P::from(v.x);
+        sc[4 * q + 1] = P::from(v.y);
+        sc[4 * q + 2] = P::from(v.z);
+        sc[4 * q + 3] = P::from(v.w);
+    } else if (4 * q + 2 <= RR) {
+        const uint2 v = *reinterpret_cast<const uint2 *>(pc + q * 64);
+        sc[4 * q + 0] = P::from(v.x);
+        sc[4 * q + 1] = P::from(v.y);
+    }
+}
+
+// Two consecutive columns of column16s, A then B, run one row apart: B's
+// row r - 1 needs only A's rows up to r - 1, so the two serial chains (Q,
+// then H, then P, row after row) interleave instead of running back to
+// back; each cell's arithmetic is column16s'. A plane's quad of rows is
+// reloaded, from nA / nB (the next pair's columns), once its column has
+// used it, so the next pair's profile is in flight during this one.
+template <int RR, bool ADJ, bool F16>
+__device__ __forceinline__ void column16s_pair(typename Pk<F16>::T (&scA)[RR], typename Pk<F16>::T (&scB)[RR],
+                                               typename Pk<F16>::T (&Pv)[RR], typename Pk<F16>::T (&R)[RR],
+                                               typename Pk<F16>::T &puA, typename Pk<F16>::T &qA,
+                                               typename Pk<F16>::T pdA, typename Pk<F16>::T &puB,
+                                               typename Pk<F16>::T &qB, typename Pk<F16>::T pdB,
+                                               typename Pk<F16>::T voe, typename Pk<F16>::T ve,
+                                               typename Pk<F16>::T vc, const uint4 *nA, const uint4 *nB) {
+    using P = Pk<F16>;
+    using T = typename P::T;
+    constexpr int NP = (RR + 3) / 4;  // quads, the last one possibly half
+    auto row = [&](int r, const T (&sc)[RR], T &pu, T &q, T &pd) {
+        R[r] = P::add(P::max(Pv[r], R[r]), ve);
+        q = P::max(pu, q);
+        if constexpr (ADJ) {
+            if (r == 0) q = P::add(q, vc);
+        }
+        const T h = P::max3(P::add(pd, sc[r]), R[r], q);
+        pd = Pv[r];
+        Pv[r] = P::add(h, voe);
+        pu = Pv[r];
+    };
+    row(0, scA, puA, qA, pdA);
+#pragma unroll
+    for (int r = 1; r < RR; ++r) {
+        row(r, scA, puA, qA, pdA);
+        row(r - 1, scB, puB, qB, pdB);
+        if ((r & 3) == 3) plane_part<RR, F16>(nA, r >> 2, scA);
+        if (((r - 1) & 3) == 3) plane_part<RR, F16>(nB, (r - 1) >> 2, scB);
+    }
+    row(RR - 1, scB, puB, qB, pdB);
+    // the quads not yet reloaded: A's from row (RR / 4) * 4 on, B's from
+    // ((RR - 1) / 4) * 4 on
+#pragma unroll
+    for (int q = RR / 4; q < NP; ++q) plane_part<RR, F16>(nA, q, scA);
+#pragma unroll
+    for (int q = (RR - 1) / 4; q < NP; ++q) {
+        if (4 * q + 3 > RR - 2 || 4 * q + 3 >= RR) plane_part<RR, F16>(nB, q, scB);
+    }
+}
+
 // One stripe of RR rows (RR = 16, or 4/8/12 for a chunk's first stripe)
 // against the wave's reference, columns 0 .. m-1. row0 is the stripe's first
 // row in the chunk's padded row numbering, pA / pB the padding rows of the
@@ -279,26 +379,46 @@ __device__ __forceinline__ void stripe16(const uint4 *__restrict__ prof, uint2 *
 // scratch, loads past the last column read padding behind it (columns up
 // to m + 1), the profile prefetch reads at most the reference's padding
 // word (column m), and whether the stripe is the first (border instead of
-// scratch) or the last (no stores) is a template argument.
-template <int RR, bool LOCAL, bool FIRST, bool LAST, bool QUAD, bool F16, bool OCT = false>
+// scratch) or the last (no stores) is a template argument. SH (semiglobal
+// only) runs the row-shifted column (column16s), c being the shift offset
+// of the row above (0 for a first stripe, whose border is given in this
+// stripe's space).
+template <int RR, bool LOCAL, bool FIRST, bool LAST, bool QUAD, bool F16, bool OCT = false, bool SH = false>
 __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 *__restrict__ scr,
                                           const int lane, const cwords w2, const cwords wn,
                                           const int m, const int row0, const int pA, const int pB,
                                           const typename Pk<F16>::T vo, const typename Pk<F16>::T ve,
-                                          typename Pk<F16>::T &best) {
+                                          typename Pk<F16>::T &best, const int c = 0) {
     static_assert(RR % 2 == 0 && RR >= 2 && RR <= 16, "an even number of rows");
+    static_assert(!(SH && LOCAL), "the row shift is a semiglobal form");
     using P = Pk<F16>;
     using v2s = typename P::T;  // the packed value type of this instance
     constexpr int NQ = RR / 4;          // whole quads of rows
     constexpr bool HALF = RR % 4 == 2;  // and the first two rows of one more
     const v2s NEG = P::neg();
+    // SH: o and e as ints again (vo, ve are their splats), the shifted
+    // space's constants, and the last stripe's bottom-row best kept apart
+    // (it is in row RR - 1's space)
+    const int so = SH ? P::lo(vo) : 0, se = SH ? P::lo(ve) : 0;
+    // wave-uniform constants through readfirstlane: scalar registers, each
+    // an operand of a packed op, where a vector copy would cost a VGPR
+    auto us = [](v2s v) { return P::from((uint32_t)uni((int)P::bits(v))); };
+    const v2s voe = us(P::splat(so - se)), vc = us(P::splat(c));
+    const v2s vne = us(P::splat(-se)), vse = SH ? us(ve) : ve;
+    v2s bestL = NEG;
     v2s Ho[RR], E[RR];
+    // SH: the left border's o - e r as a chain of adds from a base the
+    // compiler cannot see through, so the RR constants are not hoisted out
+    // of the stripe loop into registers held for the whole kernel
+    v2s t = vo;
+    if constexpr (SH) asm volatile("" : "+v"(t));
 #pragma unroll
     for (int r = 0; r < RR; ++r) {
-        Ho[r] = vo;
+        Ho[r] = t;  // H = 0 on the left border
+        if constexpr (SH) t = P::add(t, vne);
         E[r] = NEG;
     }
-    v2s hdt = vo;
+    v2s hdt = SH ? P::splat(so + se - c) : vo;
     auto code = [&](int j) -> int {
         const uint32_t c2 = (w2[j >> 4] >> (2 * (j & 15))) & 3u;
         return ((wn[j >> 5] >> (j & 31)) & 1u) ? 4 : (int)c2;
@@ -320,7 +440,8 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
         }
     };
     uint2 *const s0 = scr + 2 * 64 + lane;  // column 0 (two padding columns in front)
-    const uint2 border = make_uint2(P::bits(vo), P::bits(NEG));
+    // the row above a first stripe: H = 0, F = -inf (SH: P = o + e)
+    const uint2 border = make_uint2(P::bits(SH ? P::splat(so + se) : vo), P::bits(NEG));
     auto bnd = [&](int j) -> uint2 {
         if constexpr (FIRST) return border;
         else return s0[(size_t)j * 64];
@@ -329,17 +450,44 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
         v2s hou = P::from(b.x), f = P::from(b.y);
         const v2s hod = hdt;
         hdt = hou;
-        column16<RR, LOCAL, F16>(sc, Ho, E, hou, f, hod, vo, ve, best);
-        if constexpr (!LOCAL && LAST) best = P::max(best, hou);
+        if constexpr (SH) {
+            column16s<RR, !FIRST, F16>(sc, Ho, E, hou, f, hod, voe, vse, vc);
+            if constexpr (LAST) bestL = P::max(bestL, hou);
+        } else {
+            column16<RR, LOCAL, F16>(sc, Ho, E, hou, f, hod, vo, ve, best);
+            if constexpr (!LOCAL && LAST) best = P::max(best, hou);
+        }
         return make_uint2(P::bits(hou), P::bits(f));
     };
 
     if constexpr (OCT) {
     v2s sa[RR], sb[RR];
     plane(code(0), sa);
+    if constexpr (SH) plane(code(1 < m ? 1 : m), sb);
     uint2 a0 = bnd(0), a1 = bnd(1), b0 = bnd(2), b1 = bnd(3);
     uint2 o0 = border, o1 = border;
     int j = 0;
+    // Columns jj (plane sa) and jj + 1 into o0 and o1. SH: both planes are
+    // loaded on entry, and the pair loads those of columns jj + 2 and jj + 3
+    // (at most the padding column m) for the next one as it goes.
+    auto pair = [&](int jj, uint2 x0, uint2 x1) {
+        if constexpr (SH) {
+            v2s puA = P::from(x0.x), qA = P::from(x0.y), puB = P::from(x1.x), qB = P::from(x1.y);
+            const v2s pdA = hdt, pdB = puA;
+            hdt = puB;
+            const int ja = jj + 2 < m ? jj + 2 : m, jb = jj + 3 < m ? jj + 3 : m;
+            column16s_pair<RR, !FIRST, F16>(sa, sb, Ho, E, puA, qA, pdA, puB, qB, pdB, voe, vse, vc,
+                                            prof + code(ja) * 4 * 64 + lane, prof + code(jb) * 4 * 64 + lane);
+            if constexpr (LAST) bestL = P::max3(bestL, puA, puB);
+            o0 = make_uint2(P::bits(puA), P::bits(qA));
+            o1 = make_uint2(P::bits(puB), P::bits(qB));
+        } else {
+            plane(code(jj + 1), sb);
+            o0 = col(sa, x0);
+            plane(code(jj + 2), sa);  // jj + 2 <= m: at most the padding word
+            o1 = col(sb, x1);
+        }
+    };
     // Eight columns per iteration as four pairs over four sets of boundary
     // registers (a, b, c, d), each loaded two pairs before its use: the
     // boundary loads run four columns ahead (the farthest, j + 11 <= m + 3,
@@ -351,10 +499,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
             s0[(ptrdiff_t)(j - 1) * 64] = o1;
         }
         __builtin_amdgcn_sched_barrier(0);
-        plane(code(j + 1), sb);
-        o0 = col(sa, a0);
-        plane(code(j + 2), sa);
-        o1 = col(sb, a1);
+        pair(j, a0, a1);
         __builtin_amdgcn_sched_barrier(0);
         const uint2 d0 = bnd(j + 6), d1 = bnd(j + 7);
         if constexpr (!LAST) {
@@ -362,10 +507,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
             s0[(ptrdiff_t)(j + 1) * 64] = o1;
         }
         __builtin_amdgcn_sched_barrier(0);
-        plane(code(j + 3), sb);
-        o0 = col(sa, b0);
-        plane(code(j + 4), sa);
-        o1 = col(sb, b1);
+        pair(j + 2, b0, b1);
         __builtin_amdgcn_sched_barrier(0);
         a0 = bnd(j + 8);
         a1 = bnd(j + 9);
@@ -374,10 +516,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
             s0[(ptrdiff_t)(j + 3) * 64] = o1;
         }
         __builtin_amdgcn_sched_barrier(0);
-        plane(code(j + 5), sb);
-        o0 = col(sa, c0);
-        plane(code(j + 6), sa);
-        o1 = col(sb, c1);
+        pair(j + 4, c0, c1);
         __builtin_amdgcn_sched_barrier(0);
         b0 = bnd(j + 10);
         b1 = bnd(j + 11);
@@ -386,10 +525,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
             s0[(ptrdiff_t)(j + 5) * 64] = o1;
         }
         __builtin_amdgcn_sched_barrier(0);
-        plane(code(j + 7), sb);
-        o0 = col(sa, d0);
-        plane(code(j + 8), sa);  // j + 8 <= m: at most the padding word
-        o1 = col(sb, d1);
+        pair(j + 6, d0, d1);
         __builtin_amdgcn_sched_barrier(0);
     }
     // 0-7 columns left: pairs with the sets moved down (copies, once per
@@ -510,24 +646,33 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
     }
 
     if constexpr (!LOCAL) {
+        // the last column's real rows, in Ho = H + o space (SH: P + e r,
+        // the e r again a chain from an opaque zero)
+        v2s k = P::splat(0);
+        if constexpr (SH) asm volatile("" : "+v"(k));
 #pragma unroll
         for (int r = 0; r < RR; ++r) {
-            v2s v = Ho[r];
+            v2s v = SH ? P::add(Ho[r], k) : Ho[r];
+            if constexpr (SH) {
+                if (r + 1 < RR) k = P::add(k, vse);
+            }
             if (row0 + r < pA) P::neg_lo(v);
             if (row0 + r < pB) P::neg_hi(v);
             best = P::max(best, v);
         }
+        if constexpr (SH && LAST) best = P::max(best, P::add(bestL, k));  // k = e (RR - 1)
     }
 }
 
-template <int RR, bool LOCAL, bool FIRST, bool QUAD, bool F16, bool OCT>
+template <int RR, bool LOCAL, bool FIRST, bool QUAD, bool F16, bool OCT, bool SH = false>
 __device__ __forceinline__ void stripe16d_any(const uint4 *prof, uint2 *scr, const int lane,
                                               const cwords w2, const cwords wn, const int m,
                                               const bool last, const int row0, const int pA,
                                               const int pB, const typename Pk<F16>::T vo,
-                                              const typename Pk<F16>::T ve, typename Pk<F16>::T &best) {
-    if (last) stripe16d<RR, LOCAL, FIRST, true, QUAD, F16, OCT>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best);
-    else stripe16d<RR, LOCAL, FIRST, false, QUAD, F16, OCT>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best);
+                                              const typename Pk<F16>::T ve, typename Pk<F16>::T &best,
+                                              const int c = 0) {
+    if (last) stripe16d<RR, LOCAL, FIRST, true, QUAD, F16, OCT, SH>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best, c);
+    else stripe16d<RR, LOCAL, FIRST, false, QUAD, F16, OCT, SH>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best, c);
 }
 
 template <bool LOCAL, int PF>
@@ -545,9 +690,12 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
     const uint32_t n_items = (uint32_t)n_chunks * (uint32_t)n_groups;
     uint2 *scr = a.scratch16 + ((size_t)blockIdx.x * kWaves16 + wv) * (size_t)(P.max_len + kScrPad) * 64;
     // variants 4 and 5: variant 3's schedule, and eight columns per
-    // iteration with the boundary four columns ahead, in fp16 arithmetic
-    constexpr bool F16 = PF >= 4;
-    constexpr bool OCT = PF == 5;
+    // iteration with the boundary four columns ahead, in fp16 arithmetic;
+    // 6 and 7: variants 3 and 5 with the row-shifted semiglobal column
+    // (column16s; in local mode they are variants 3 and 5)
+    constexpr bool F16 = PF == 4 || PF == 5 || PF == 7;
+    constexpr bool OCT = PF == 5 || PF == 7;
+    constexpr bool SH = !LOCAL && PF >= 6;
     using PK = Pk<F16>;
     using V = typename PK::T;
     const V vo = PK::splat(a.sc.o), ve = PK::splat(a.sc.e), NEG = PK::neg();
@@ -596,6 +744,9 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
         for (int s = 0; s < S; ++s) {
             const int row0 = s == 0 ? 0 : RR + kStripe * (s - 1);
             const int height = s == 0 ? RR : kStripe;
+            // SH: the row above, the previous stripe's bottom row, is lower
+            // by e x (the previous stripe's height) in this stripe's space
+            const int cadj = (SH && s > 0) ? a.sc.e * (s == 1 ? RR : kStripe) : 0;
             // This stripe's profile: wave wv builds rows row0 + 4 wv .. + 3.
             __syncthreads();
             if (4 * wv < height) {
@@ -607,15 +758,20 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
                     // 14 rows builds whole quads) are never read: padding codes
                     const int xa = (i < pA || i >= T) ? kPreCode : qA[i - pA];
                     const int xb = (i < pB || i >= T) ? kPreCode : qB[i - pB];
+                    // the stripe's row 0 carries the shift offset (SH)
+                    const bool top = SH && cadj != 0 && i == row0;
+                    const uint32_t rMa = top ? PK::enc(a.sc.ma - a.sc.o + cadj) : eMa;
+                    const uint32_t rMi = top ? PK::enc(a.sc.mi - a.sc.o + cadj) : eMi;
+                    const uint32_t rPre = top ? PK::enc(-a.sc.o + cadj) : ePre;
 #pragma unroll
                     for (int c = 0; c < 4; ++c) {
-                        const uint32_t ea = xa == kPreCode ? ePre : (xa == c || xa == 4) ? eMa : eMi;
-                        const uint32_t eb = xb == kPreCode ? ePre : (xb == c || xb == 4) ? eMa : eMi;
+                        const uint32_t ea = xa == kPreCode ? rPre : (xa == c || xa == 4) ? rMa : rMi;
+                        const uint32_t eb = xb == kPreCode ? rPre : (xb == c || xb == 4) ? rMa : rMi;
                         pl[c][rr] = ea | (eb << 16);
                     }
                     // plane 4, a wildcard reference base: a match for every
                     // read base (also the best of the four, mismatch < match)
-                    pl[4][rr] = (xa == kPreCode ? ePre : eMa) | ((xb == kPreCode ? ePre : eMa) << 16);
+                    pl[4][rr] = (xa == kPreCode ? rPre : rMa) | ((xb == kPreCode ? rPre : rMa) << 16);
                 }
 #pragma unroll
                 for (int c = 0; c < 5; ++c)
@@ -625,16 +781,16 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
             if (!active) continue;
             const bool first = s == 0, last = s + 1 == S;
             if constexpr (PF >= 2) {
-                constexpr bool Q = PF == 3 || PF == 4;
+                constexpr bool Q = PF == 3 || PF == 4 || PF == 6;
                 if (s > 0) {
-                    stripe16d_any<16, LOCAL, false, Q, F16, OCT>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
+                    stripe16d_any<16, LOCAL, false, Q, F16, OCT, SH>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best, cadj);
                 } else {
                     switch (RR) {
 #define ESCH_FIRST(H) \
-    case H: stripe16d_any<H, LOCAL, true, Q, F16, OCT>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best); break;
+    case H: stripe16d_any<H, LOCAL, true, Q, F16, OCT, SH>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best); break;
                         ESCH_FIRST(16) ESCH_FIRST(14) ESCH_FIRST(12) ESCH_FIRST(10)
                         ESCH_FIRST(8) ESCH_FIRST(6) ESCH_FIRST(4)
-                        default: stripe16d_any<2, LOCAL, true, Q, F16, OCT>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
+                        default: stripe16d_any<2, LOCAL, true, Q, F16, OCT, SH>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
 #undef ESCH_FIRST
                     }
                 }
@@ -1678,6 +1834,8 @@ hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st) {
 // Kernel variants of k_dp16, chosen per scorer (set by the autotuner).
 template <bool LOCAL>
 static const void *dp16_fn(int variant) {
+    if (variant == 7) return reinterpret_cast<const void *>(&k_dp16<LOCAL, 7>);
+    if (variant == 6) return reinterpret_cast<const void *>(&k_dp16<LOCAL, 6>);
     if (variant == 5) return reinterpret_cast<const void *>(&k_dp16<LOCAL, 5>);
     if (variant == 4) return reinterpret_cast<const void *>(&k_dp16<LOCAL, 4>);
     if (variant == 3) return reinterpret_cast<const void *>(&k_dp16<LOCAL, 3>);
@@ -1690,14 +1848,18 @@ hipError_t launch_dp16(const KArgs &a, const Launch &g, bool local, hipStream_t 
     const int v = g.variant16;
     const dim3 grid(g.grid16), block(64 * kWaves16);
     if (local) {
-        if (v == 5) hipLaunchKernelGGL((k_dp16<true, 5>), grid, block, 0, st, a);
+        if (v == 7) hipLaunchKernelGGL((k_dp16<true, 7>), grid, block, 0, st, a);
+        else if (v == 6) hipLaunchKernelGGL((k_dp16<true, 6>), grid, block, 0, st, a);
+        else if (v == 5) hipLaunchKernelGGL((k_dp16<true, 5>), grid, block, 0, st, a);
         else if (v == 4) hipLaunchKernelGGL((k_dp16<true, 4>), grid, block, 0, st, a);
         else if (v == 3) hipLaunchKernelGGL((k_dp16<true, 3>), grid, block, 0, st, a);
         else if (v == 2) hipLaunchKernelGGL((k_dp16<true, 2>), grid, block, 0, st, a);
         else if (v == 1) hipLaunchKernelGGL((k_dp16<true, 1>), grid, block, 0, st, a);
         else hipLaunchKernelGGL((k_dp16<true, 0>), grid, block, 0, st, a);
     } else {
-        if (v == 5) hipLaunchKernelGGL((k_dp16<false, 5>), grid, block, 0, st, a);
+        if (v == 7) hipLaunchKernelGGL((k_dp16<false, 7>), grid, block, 0, st, a);
+        else if (v == 6) hipLaunchKernelGGL((k_dp16<false, 6>), grid, block, 0, st, a);
+        else if (v == 5) hipLaunchKernelGGL((k_dp16<false, 5>), grid, block, 0, st, a);
         else if (v == 4) hipLaunchKernelGGL((k_dp16<false, 4>), grid, block, 0, st, a);
         else if (v == 3) hipLaunchKernelGGL((k_dp16<false, 3>), grid, block, 0, st, a);
         else if (v == 2) hipLaunchKernelGGL((k_dp16<false, 2>), grid, block, 0, st, a);

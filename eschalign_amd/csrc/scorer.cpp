@@ -227,7 +227,17 @@ esch_status esch_scorer_new(int ordinal, const uint8_t *const *refs, const size_
     // default: the deep-prefetch variant (the fastest on the measured
     // workloads); esch_autotune re-decides on the caller's reads
     s->f16_ok = fits_f16((int64_t)max_len, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e);
-    const int v0 = s->f16_ok ? kDp16F16 : kDp16Default;
+    {
+        const int64_t sb = esch::shifted_bound((int64_t)max_len, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e);
+        for (int v = 0; v < kDp16Variants; ++v) {
+            const bool f16 = esch::dp16_is_f16(v);
+            const bool shift_ok = local || !esch::dp16_is_shifted(v) || sb <= (f16 ? esch::kBoundF16 : esch::kBound16);
+            s->usable16[v] = (!f16 || s->f16_ok) && shift_ok;
+        }
+    }
+    // semiglobal: the row-shifted variants where their bound holds
+    int v0 = s->f16_ok ? kDp16F16 : kDp16Default;
+    if (!local) v0 = s->usable16[7] ? 7 : s->f16_ok ? kDp16F16 : s->usable16[6] ? 6 : kDp16Default;
     s->geo.variant16 = v0;
     s->geo.grid16 = s->grid16_v[v0];
     s->regs16 = s->regs16_v[v0];
@@ -1087,11 +1097,16 @@ esch_status esch_score_batch(esch_scorer *s, const uint8_t *const *queries, cons
 esch_status esch_set_variant(esch_scorer *s, int variant) {
     if (!s || variant < 0 || variant >= kDp16Variants)
         return fail(ESCH_INVALID_ARG, "variant %d outside 0..%d", variant, kDp16Variants - 1);
-    if (variant >= kDp16F16 && !s->f16_ok)
+    if (esch::dp16_is_f16(variant) && !s->f16_ok)
         return fail(ESCH_INVALID_ARG,
                     "variant %d (fp16) needs longest reference x max(|match|, |mismatch|) + |open| + "
                     "|extend| <= %lld",
                     variant, (long long)esch::kBoundF16);
+    if (!s->usable16[variant])
+        return fail(ESCH_INVALID_ARG,
+                    "variant %d (row-shifted) needs longest reference x max(|match|, |mismatch|) + |open| + "
+                    "17 |extend| <= %lld",
+                    variant, (long long)(esch::dp16_is_f16(variant) ? esch::kBoundF16 : esch::kBound16));
     s->geo.variant16 = variant;
     s->geo.grid16 = s->grid16_v[variant];
     s->regs16 = s->regs16_v[variant];
@@ -1151,7 +1166,7 @@ esch_status esch_autotune(esch_scorer *s, const uint8_t *const *reads, const siz
     int best_v = s->geo.variant16;
     float best_ms = 0.0f;
     for (int v = 0; !st && v < kDp16Variants; ++v) {
-        if (v >= kDp16F16 && !s->f16_ok) {
+        if (!s->usable16[v]) {
             if (ms) ms[v] = -1.0f;  // not applicable to this panel
             continue;
         }
