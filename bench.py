@@ -41,16 +41,16 @@ PEAK_I16 = N_CU * CLOCK_HZ * 128           # lane-ops/s
 PEAK_I32 = N_CU * CLOCK_HZ * 64
 # Minimum packed 16-bit lane-ops per DP cell of the recurrence: 4 adds and
 # 4 two-input maxima in semiglobal mode, plus the zero floor and the running
-# best in local mode; with the fp16 variants' three-input maximum (4, 5, 7)
+# best in local mode; with the fp16 variants' three-input maximum (4, 5, 7, 8)
 # H takes one max instead of two, and the column best a three-way tree. The
-# row-shifted semiglobal column (6, 7) drops the vertical gap's extension
+# row-shifted semiglobal column (6, 7, 8) drops the vertical gap's extension
 # add: one op less. Keyed (mode, fp16, row-shifted).
 OPS_PER_CELL = {("semiglobal", False, False): 8, ("local", False, False): 10,
                 ("semiglobal", True, False): 7, ("local", True, False): 8.5,
                 ("semiglobal", False, True): 7, ("local", False, True): 10,
                 ("semiglobal", True, True): 6, ("local", True, True): 8.5}
-F16_VARIANTS = (4, 5, 7)  # k_dp16 variants that compute in fp16
-SHIFTED_VARIANTS = (6, 7)  # and those with the row-shifted semiglobal column
+F16_VARIANTS = (4, 5, 7, 8)  # k_dp16 variants that compute in fp16
+SHIFTED_VARIANTS = (6, 7, 8)  # and those with the row-shifted semiglobal column
 # SQ_INSTS_VALU x 128 / cells of one k_dp16 launch of the default workload,
 # from the counter summary in profiles/ (mode, variant) -> lane-ops per cell
 VALU_OPS_PER_CELL_ISSUED = {("semiglobal", 3): round(344_021_737_036 * 128 / 5_303_272_453_076, 3),

@@ -137,17 +137,17 @@ struct Launch {
     int grid_aux; // blocks of the bucketing / selection kernels
     int variant16;  // k_dp16 variant: 0 profile per column, 1 ping-pong, 2 pair schedule, 3 quad schedule,
                     // 4 quad schedule in fp16, 5 eight-column schedule in fp16 (the
-                    // fp16 variants: panels within kBoundF16 only), 6 and 7: 3 and 5
-                    // with the row-shifted semiglobal column (shifted_bound)
+                    // fp16 variants: panels within kBoundF16 only), 6, 7 and 8: 3, 5
+                    // and 4 with the row-shifted semiglobal column (shifted_bound)
     int grid_diag;  // persistent waves of the anti-diagonal kernel
     size_t diag_lds;  // k_diag64's LDS boundary bytes; 0: k_diag16 (global scratch)
 };
-constexpr int kDp16Variants = 8;
+constexpr int kDp16Variants = 9;
 constexpr int kDp16Default = 3;  // the int16 default; panels within kBoundF16 take kDp16F16
 constexpr int kDp16F16 = 4;      // the first variant computing in fp16
 // variants computing in fp16, and those with the row-shifted semiglobal column
-constexpr bool dp16_is_f16(int v) { return v == 4 || v == 5 || v == 7; }
-constexpr bool dp16_is_shifted(int v) { return v == 6 || v == 7; }
+constexpr bool dp16_is_f16(int v) { return v == 4 || v == 5 || v == 7 || v == 8; }
+constexpr bool dp16_is_shifted(int v) { return v >= 6; }
 hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st);
 hipError_t launch_dp16(const KArgs &a, const Launch &g, bool local, hipStream_t st);
 hipError_t launch_dp32(const KArgs &a, const Launch &g, bool local, int which, hipStream_t st);

@@ -460,11 +460,9 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
         return make_uint2(P::bits(hou), P::bits(f));
     };
 
-    if constexpr (OCT) {
     v2s sa[RR], sb[RR];
     plane(code(0), sa);
     if constexpr (SH) plane(code(1 < m ? 1 : m), sb);
-    uint2 a0 = bnd(0), a1 = bnd(1), b0 = bnd(2), b1 = bnd(3);
     uint2 o0 = border, o1 = border;
     int j = 0;
     // Columns jj (plane sa) and jj + 1 into o0 and o1. SH: both planes are
@@ -488,6 +486,8 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
             o1 = col(sb, x1);
         }
     };
+    if constexpr (OCT) {
+    uint2 a0 = bnd(0), a1 = bnd(1), b0 = bnd(2), b1 = bnd(3);
     // Eight columns per iteration as four pairs over four sets of boundary
     // registers (a, b, c, d), each loaded two pairs before its use: the
     // boundary loads run four columns ahead (the farthest, j + 11 <= m + 3,
@@ -536,10 +536,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
             s0[(ptrdiff_t)(j - 2) * 64] = o0;
             s0[(ptrdiff_t)(j - 1) * 64] = o1;
         }
-        plane(code(j + 1), sb);
-        o0 = col(sa, a0);
-        plane(code(j + 2), sa);
-        o1 = col(sb, a1);
+        pair(j, a0, a1);
         a0 = b0;
         a1 = b1;
         b0 = c0;
@@ -556,11 +553,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
     }
 
     } else if constexpr (!QUAD) {
-    v2s sa[RR], sb[RR];
-    plane(code(0), sa);
     uint2 r0 = bnd(0), r1 = bnd(1);
-    uint2 o0 = border, o1 = border;
-    int j = 0;
     for (; j + 1 < m; j += 2) {
         const uint2 q0 = bnd(j + 2), q1 = bnd(j + 3);
         if constexpr (!LAST) {
@@ -568,10 +561,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
             s0[(ptrdiff_t)(j - 1) * 64] = o1;
         }
         __builtin_amdgcn_sched_barrier(0);
-        plane(code(j + 1), sb);
-        o0 = col(sa, r0);
-        plane(code(j + 2), sa);
-        o1 = col(sb, r1);
+        pair(j, r0, r1);
         __builtin_amdgcn_sched_barrier(0);  // the copies (and their wait) stay at the end
         r0 = q0;
         r1 = q1;
@@ -586,11 +576,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
     }
 
     } else {
-    v2s sa[RR], sb[RR];
-    plane(code(0), sa);
     uint2 r0 = bnd(0), r1 = bnd(1);
-    uint2 o0 = border, o1 = border;
-    int j = 0;
     // Four columns per iteration as two pairs whose boundary registers swap
     // roles (r for the first pair, q for the second), so no loaded register
     // is copied: a copy would need its load complete where it is made.
@@ -601,10 +587,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
             s0[(ptrdiff_t)(j - 1) * 64] = o1;
         }
         __builtin_amdgcn_sched_barrier(0);
-        plane(code(j + 1), sb);
-        o0 = col(sa, r0);
-        plane(code(j + 2), sa);
-        o1 = col(sb, r1);
+        pair(j, r0, r1);
         __builtin_amdgcn_sched_barrier(0);
         r0 = bnd(j + 4);
         r1 = bnd(j + 5);
@@ -613,10 +596,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
             s0[(ptrdiff_t)(j + 1) * 64] = o1;
         }
         __builtin_amdgcn_sched_barrier(0);
-        plane(code(j + 3), sb);
-        o0 = col(sa, q0);
-        plane(code(j + 4), sa);  // j + 4 <= m: at most the padding word
-        o1 = col(sb, q1);
+        pair(j + 2, q0, q1);
         __builtin_amdgcn_sched_barrier(0);
     }
     // 0-3 columns left: one more pair, then a single column
@@ -626,10 +606,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
             s0[(ptrdiff_t)(j - 2) * 64] = o0;
             s0[(ptrdiff_t)(j - 1) * 64] = o1;
         }
-        plane(code(j + 1), sb);
-        o0 = col(sa, r0);
-        plane(code(j + 2), sa);
-        o1 = col(sb, r1);
+        pair(j, r0, r1);
         r0 = q0;
         r1 = q1;
         j += 2;
@@ -691,9 +668,9 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
     uint2 *scr = a.scratch16 + ((size_t)blockIdx.x * kWaves16 + wv) * (size_t)(P.max_len + kScrPad) * 64;
     // variants 4 and 5: variant 3's schedule, and eight columns per
     // iteration with the boundary four columns ahead, in fp16 arithmetic;
-    // 6 and 7: variants 3 and 5 with the row-shifted semiglobal column
-    // (column16s; in local mode they are variants 3 and 5)
-    constexpr bool F16 = PF == 4 || PF == 5 || PF == 7;
+    // 6, 7 and 8: variants 3, 5 and 4 with the row-shifted semiglobal
+    // column (column16s; in local mode they are variants 3, 5 and 4)
+    constexpr bool F16 = PF == 4 || PF == 5 || PF == 7 || PF == 8;
     constexpr bool OCT = PF == 5 || PF == 7;
     constexpr bool SH = !LOCAL && PF >= 6;
     using PK = Pk<F16>;
@@ -781,7 +758,7 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
             if (!active) continue;
             const bool first = s == 0, last = s + 1 == S;
             if constexpr (PF >= 2) {
-                constexpr bool Q = PF == 3 || PF == 4 || PF == 6;
+                constexpr bool Q = PF == 3 || PF == 4 || PF == 6 || PF == 8;
                 if (s > 0) {
                     stripe16d_any<16, LOCAL, false, Q, F16, OCT, SH>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best, cadj);
                 } else {
@@ -1834,6 +1811,7 @@ hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st) {
 // Kernel variants of k_dp16, chosen per scorer (set by the autotuner).
 template <bool LOCAL>
 static const void *dp16_fn(int variant) {
+    if (variant == 8) return reinterpret_cast<const void *>(&k_dp16<LOCAL, 8>);
     if (variant == 7) return reinterpret_cast<const void *>(&k_dp16<LOCAL, 7>);
     if (variant == 6) return reinterpret_cast<const void *>(&k_dp16<LOCAL, 6>);
     if (variant == 5) return reinterpret_cast<const void *>(&k_dp16<LOCAL, 5>);
@@ -1848,7 +1826,8 @@ hipError_t launch_dp16(const KArgs &a, const Launch &g, bool local, hipStream_t 
     const int v = g.variant16;
     const dim3 grid(g.grid16), block(64 * kWaves16);
     if (local) {
-        if (v == 7) hipLaunchKernelGGL((k_dp16<true, 7>), grid, block, 0, st, a);
+        if (v == 8) hipLaunchKernelGGL((k_dp16<true, 8>), grid, block, 0, st, a);
+        else if (v == 7) hipLaunchKernelGGL((k_dp16<true, 7>), grid, block, 0, st, a);
         else if (v == 6) hipLaunchKernelGGL((k_dp16<true, 6>), grid, block, 0, st, a);
         else if (v == 5) hipLaunchKernelGGL((k_dp16<true, 5>), grid, block, 0, st, a);
         else if (v == 4) hipLaunchKernelGGL((k_dp16<true, 4>), grid, block, 0, st, a);
@@ -1857,7 +1836,8 @@ hipError_t launch_dp16(const KArgs &a, const Launch &g, bool local, hipStream_t 
         else if (v == 1) hipLaunchKernelGGL((k_dp16<true, 1>), grid, block, 0, st, a);
         else hipLaunchKernelGGL((k_dp16<true, 0>), grid, block, 0, st, a);
     } else {
-        if (v == 7) hipLaunchKernelGGL((k_dp16<false, 7>), grid, block, 0, st, a);
+        if (v == 8) hipLaunchKernelGGL((k_dp16<false, 8>), grid, block, 0, st, a);
+        else if (v == 7) hipLaunchKernelGGL((k_dp16<false, 7>), grid, block, 0, st, a);
         else if (v == 6) hipLaunchKernelGGL((k_dp16<false, 6>), grid, block, 0, st, a);
         else if (v == 5) hipLaunchKernelGGL((k_dp16<false, 5>), grid, block, 0, st, a);
         else if (v == 4) hipLaunchKernelGGL((k_dp16<false, 4>), grid, block, 0, st, a);

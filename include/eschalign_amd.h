@@ -400,7 +400,11 @@ size_t esch_cigar_string(const uint32_t *ops, size_t n_ops, size_t query_len, si
  *      + |open| + 17 |extend| <= 32000; in local mode it is variant 3;
  *   7  (semiglobal default where it applies) variant 5 with that shift: 6
  *      packed ops instead of 7; in semiglobal mode it needs the same sum at
- *      most 2048, in local mode it is variant 5. */
+ *      most 2048, in local mode it is variant 5;
+ *   8  variant 4 with that shift (fewer registers than 7, one more resident
+ *      wave, a shallower prefetch); the condition of 7. In the shifted
+ *      variants the two columns of a pair run one row apart, so their
+ *      serial chains interleave. */
 int esch_n_variants(void);
 int esch_get_variant(const esch_scorer *s);
 esch_status esch_set_variant(esch_scorer *s, int variant);

@@ -398,11 +398,11 @@ def test_fp16_variant_at_its_bound():
         assert chosen not in F16_VARIANTS and all(ms[v] == -1.0 for v in F16_VARIANTS)
 
 
-F16_VARIANTS = (4, 5, 7)
+F16_VARIANTS = (4, 5, 7, 8)
 
 
 def test_row_shifted_variants_at_their_bound():
-    """Variants 6 (int16) and 7 (fp16) hold every value of a stripe row r
+    """Variants 6 (int16), 7 and 8 (fp16) hold every value of a stripe row r
     shifted by |e| r: exact while longest reference x M + |o| + 17 |e| stays
     within the arithmetic's bound (2048 for fp16, 32000 for int16). Panels
     exactly at it, with reads that reach both ends of the value range (a
@@ -412,10 +412,11 @@ def test_row_shifted_variants_at_their_bound():
     in local mode the variants are the unshifted 3 and 5)."""
     rng = G.Rng(0x5EED_0A04)
     # (scoring, L, variant): L x M + |o| + 17 |e| equals the bound
-    for scoring, L, v in (((1, -1, -2, -1), 2029, 7), ((2, -1, -11, -3), 993, 7),
+    for scoring, L, v in (((1, -1, -2, -1), 2029, 7), ((2, -1, -11, -3), 993, 8),
                           ((100, -100, -200, -100), 301, 6)):
         M = max(abs(scoring[0]), abs(scoring[1]))
         assert L * M + abs(scoring[2]) + 17 * abs(scoring[3]) == (32000 if v == 6 else 2048)
+        others = (7, 8) if v != 6 else (6,)
         refs = [G.random_seq(rng, L, 0.0), b"C" * L, G.random_seq(rng, L // 2, 0.02),
                 G.random_seq(rng, 40, 0.0)]
         reads = [refs[0], b"A" * L, b"A" * (L // 3), b"G" * 5, refs[0][L // 9:L - L // 9],
@@ -427,18 +428,21 @@ def test_row_shifted_variants_at_their_bound():
         gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), ea.Mode.SEMIGLOBAL)
         gpu.diag_ratio = 0
         want = oracle(queries, panel.codes, scoring, ea.Mode.SEMIGLOBAL)
-        gpu.variant = v
-        s32, ok = gpu.score_batch_i32(queries)
-        assert ok.all()
-        bad = np.argwhere(s32 != want)
-        assert bad.size == 0, (scoring, v, bad[:4])
+        for w in others:
+            gpu.variant = w
+            s32, ok = gpu.score_batch_i32(queries)
+            assert ok.all()
+            bad = np.argwhere(s32 != want)
+            assert bad.size == 0, (scoring, w, bad[:4])
         assert want.max() >= L * scoring[0] - 20 * abs(scoring[0]), want.max()
         longer = ea.Panel(refs + [G.random_seq(rng, L + 1, 0.0)])
         gpu = ea.GpuScorer(longer, ea.Scoring(*scoring), ea.Mode.SEMIGLOBAL)
-        with pytest.raises(ea.GpuError):
-            gpu.variant = v
+        for w in others:
+            with pytest.raises(ea.GpuError):
+                gpu.variant = w
         gpu = ea.GpuScorer(longer, ea.Scoring(*scoring), ea.Mode.LOCAL)
-        gpu.variant = v
+        for w in others:
+            gpu.variant = w
 
 
 def test_every_kernel_variant_on_short_references():
