@@ -55,7 +55,9 @@ SHIFTED_VARIANTS = (6, 7, 8)  # and those with the row-shifted semiglobal column
 # from the counter summary in profiles/ (mode, variant) -> lane-ops per cell
 VALU_OPS_PER_CELL_ISSUED = {("semiglobal", 3): round(344_021_737_036 * 128 / 5_303_272_453_076, 3),
                             ("semiglobal", 4): round(304_396_230_832 * 128 / 5_303_272_453_076, 3),
-                            ("semiglobal", 5): round(301_256_340_206 * 128 / 5_303_272_453_076, 3)}
+                            ("semiglobal", 5): round(301_256_340_206 * 128 / 5_303_272_453_076, 3),
+                            ("semiglobal", 6): round(306_294_990_254 * 128 / 5_303_272_453_076, 3),
+                            ("semiglobal", 7): round(263_781_725_920 * 128 / 5_303_272_453_076, 3)}
 
 
 def log(*a):
