@@ -43,12 +43,13 @@ PEAK_I32 = N_CU * CLOCK_HZ * 64
 # 4 two-input maxima in semiglobal mode, plus the zero floor and the running
 # best in local mode; with the fp16 variants' three-input maximum (4, 5, 7, 8)
 # H takes one max instead of two, and the column best a three-way tree. The
-# row-shifted semiglobal column (6, 7, 8) drops the vertical gap's extension
-# add: one op less. Keyed (mode, fp16, row-shifted).
+# shifted variants (6, 7, 8) drop one gap chain's extension add: rows shifted
+# in semiglobal mode (one op less), columns in local mode (one less again).
+# Keyed (mode, fp16, shifted).
 OPS_PER_CELL = {("semiglobal", False, False): 8, ("local", False, False): 10,
                 ("semiglobal", True, False): 7, ("local", True, False): 8.5,
-                ("semiglobal", False, True): 7, ("local", False, True): 10,
-                ("semiglobal", True, True): 6, ("local", True, True): 8.5}
+                ("semiglobal", False, True): 7, ("local", False, True): 9,
+                ("semiglobal", True, True): 6, ("local", True, True): 7.5}
 F16_VARIANTS = (4, 5, 7, 8)  # k_dp16 variants that compute in fp16
 SHIFTED_VARIANTS = (6, 7, 8)  # and those with the row-shifted semiglobal column
 # SQ_INSTS_VALU x 128 / cells of one k_dp16 launch of the default workload,

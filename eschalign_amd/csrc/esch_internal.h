@@ -42,6 +42,16 @@ __host__ __device__ inline int64_t shifted_bound(int64_t max_ref_len, int32_t ma
     const int64_t O = o < 0 ? -(int64_t)o : o, Ee = e < 0 ? -(int64_t)e : e;
     return max_ref_len * (M > Mi ? M : Mi) + O + 17 * Ee;
 }
+// In local mode those variants hold reference column j's values shifted by
+// |e| j (j = -1 .. L - 1): every value lies within
+// [-(|o| + 3|e| + M), kM + |e| (L + 2) + M], the floor and the carried best
+// included, so the bound grows with the longest reference L.
+__host__ __device__ inline int64_t colshift_bound(int64_t max_ref_len, int32_t ma, int32_t mi, int32_t o, int32_t e) {
+    const int64_t M = ma < 0 ? -(int64_t)ma : ma, Mi = mi < 0 ? -(int64_t)mi : mi;
+    const int64_t O = o < 0 ? -(int64_t)o : o, Ee = e < 0 ? -(int64_t)e : e;
+    const int64_t MM = M > Mi ? M : Mi;
+    return max_ref_len * (MM + Ee) + O + 3 * Ee + MM;
+}
 // -inf of the int32 kernel, the oracle's i32::MIN / 4 (lib/scalar.rs:42).
 constexpr int32_t kNeg32 = INT32_MIN / 4;
 
@@ -138,7 +148,8 @@ struct Launch {
     int variant16;  // k_dp16 variant: 0 profile per column, 1 ping-pong, 2 pair schedule, 3 quad schedule,
                     // 4 quad schedule in fp16, 5 eight-column schedule in fp16 (the
                     // fp16 variants: panels within kBoundF16 only), 6, 7 and 8: 3, 5
-                    // and 4 with the row-shifted semiglobal column (shifted_bound)
+                    // and 4 with the row-shifted semiglobal column (shifted_bound),
+                    // in local mode the column-shifted one (colshift_bound)
     int grid_diag;  // persistent waves of the anti-diagonal kernel
     size_t diag_lds;  // k_diag64's LDS boundary bytes; 0: k_diag16 (global scratch)
 };

@@ -280,6 +280,81 @@ __device__ __forceinline__ void column16s_pair(typename Pk<F16>::T (&scA)[RR], t
     }
 }
 
+// The local column with every value of reference column j shifted by -e j,
+// which takes the extension out of the horizontal gap chain instead:
+//   P = H - e j + o,  R = E - e j + e,  Q = F - e j + e
+//   R[j]    = max(P[j-1], R[j-1])
+//   Q[r]    = max(P[r-1], Q[r-1]) + e
+//   P[r][j] = max(P[r-1][j-1] + (s - o), R, Q, z_j) + (o - e)
+// where z_j = e (1 - j) is the zero floor H >= 0 in this space: one value
+// per column, shared by every row (a row shift would need one per row, and a
+// best per row). 7 packed ops per row where the unshifted local column takes
+// 8, and a column's rows share one shift, so their best folds by a tree of
+// three-way maxima as before. Returns the column's best P (column j space).
+template <int RR, bool F16>
+__device__ __forceinline__ typename Pk<F16>::T column16c(const typename Pk<F16>::T (&sc)[RR],
+                                                         typename Pk<F16>::T (&Pv)[RR],
+                                                         typename Pk<F16>::T (&R)[RR], typename Pk<F16>::T &pu,
+                                                         typename Pk<F16>::T &q, typename Pk<F16>::T pd,
+                                                         typename Pk<F16>::T voe, typename Pk<F16>::T ve,
+                                                         typename Pk<F16>::T z) {
+    using P = Pk<F16>;
+#pragma unroll
+    for (int r = 0; r < RR; ++r) {
+        R[r] = P::max(Pv[r], R[r]);
+        q = P::add(P::max(pu, q), ve);
+        const typename P::T h = P::max(P::max3(P::add(pd, sc[r]), R[r], q), z);
+        pd = Pv[r];
+        Pv[r] = P::add(h, voe);
+        pu = Pv[r];
+    }
+    return max3_all<F16, RR>(Pv);
+}
+
+// Two consecutive columns of column16c run one row apart (as column16s_pair),
+// the next pair's planes reloaded quad by quad; tA / tB receive the two
+// columns' best P, each folded two rows at a time while its values are live.
+template <int RR, bool F16>
+__device__ __forceinline__ void column16c_pair(typename Pk<F16>::T (&scA)[RR], typename Pk<F16>::T (&scB)[RR],
+                                               typename Pk<F16>::T (&Pv)[RR], typename Pk<F16>::T (&R)[RR],
+                                               typename Pk<F16>::T &puA, typename Pk<F16>::T &qA,
+                                               typename Pk<F16>::T pdA, typename Pk<F16>::T &puB,
+                                               typename Pk<F16>::T &qB, typename Pk<F16>::T pdB,
+                                               typename Pk<F16>::T voe, typename Pk<F16>::T ve,
+                                               typename Pk<F16>::T zA, typename Pk<F16>::T zB,
+                                               typename Pk<F16>::T &tA, typename Pk<F16>::T &tB,
+                                               const uint4 *nA, const uint4 *nB) {
+    using P = Pk<F16>;
+    using T = typename P::T;
+    constexpr int NP = (RR + 3) / 4;
+    auto row = [&](int r, const T (&sc)[RR], T &pu, T &q, T &pd, T z) {
+        R[r] = P::max(Pv[r], R[r]);
+        q = P::add(P::max(pu, q), ve);
+        const T h = P::max(P::max3(P::add(pd, sc[r]), R[r], q), z);
+        pd = Pv[r];
+        Pv[r] = P::add(h, voe);
+        pu = Pv[r];
+    };
+    // rows r - 1 and r (r odd) into a column's running best
+    auto fold = [&](int r, T &t) { t = r == 1 ? P::max(Pv[0], Pv[1]) : P::max3(t, Pv[r - 1], Pv[r]); };
+    row(0, scA, puA, qA, pdA, zA);
+#pragma unroll
+    for (int r = 1; r < RR; ++r) {
+        row(r, scA, puA, qA, pdA, zA);
+        if (r & 1) fold(r, tA);  // before B overwrites row r - 1
+        row(r - 1, scB, puB, qB, pdB, zB);
+        if (!(r & 1) && r >= 2) fold(r - 1, tB);
+        if ((r & 3) == 3) plane_part<RR, F16>(nA, r >> 2, scA);
+        if (((r - 1) & 3) == 3) plane_part<RR, F16>(nB, (r - 1) >> 2, scB);
+    }
+    row(RR - 1, scB, puB, qB, pdB, zB);
+    fold(RR - 1, tB);
+#pragma unroll
+    for (int q = RR / 4; q < NP; ++q) plane_part<RR, F16>(nA, q, scA);
+#pragma unroll
+    for (int q = (RR - 1) / 4; q < NP; ++q) plane_part<RR, F16>(nB, q, scB);
+}
+
 // One stripe of RR rows (RR = 16, or 4/8/12 for a chunk's first stripe)
 // against the wave's reference, columns 0 .. m-1. row0 is the stripe's first
 // row in the chunk's padded row numbering, pA / pB the padding rows of the
@@ -382,8 +457,10 @@ __device__ __forceinline__ void stripe16(const uint4 *__restrict__ prof, uint2 *
 // scratch) or the last (no stores) is a template argument. SH (semiglobal
 // only) runs the row-shifted column (column16s), c being the shift offset
 // of the row above (0 for a first stripe, whose border is given in this
-// stripe's space).
-template <int RR, bool LOCAL, bool FIRST, bool LAST, bool QUAD, bool F16, bool OCT = false, bool SH = false>
+// stripe's space); CS (local only) the column-shifted one (column16c), with
+// `best` carried in column j's space through the stripe.
+template <int RR, bool LOCAL, bool FIRST, bool LAST, bool QUAD, bool F16, bool OCT = false, bool SH = false,
+          bool CS = false>
 __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 *__restrict__ scr,
                                           const int lane, const cwords w2, const cwords wn,
                                           const int m, const int row0, const int pA, const int pB,
@@ -391,6 +468,8 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
                                           typename Pk<F16>::T &best, const int c = 0) {
     static_assert(RR % 2 == 0 && RR >= 2 && RR <= 16, "an even number of rows");
     static_assert(!(SH && LOCAL), "the row shift is a semiglobal form");
+    static_assert(!(CS && !LOCAL), "the column shift is a local form");
+    constexpr bool XS = SH || CS;  // a shifted column: pairs run interleaved
     using P = Pk<F16>;
     using v2s = typename P::T;  // the packed value type of this instance
     constexpr int NQ = RR / 4;          // whole quads of rows
@@ -399,13 +478,19 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
     // SH: o and e as ints again (vo, ve are their splats), the shifted
     // space's constants, and the last stripe's bottom-row best kept apart
     // (it is in row RR - 1's space)
-    const int so = SH ? P::lo(vo) : 0, se = SH ? P::lo(ve) : 0;
+    const int so = XS ? P::lo(vo) : 0, se = XS ? P::lo(ve) : 0;
     // wave-uniform constants through readfirstlane: scalar registers, each
     // an operand of a packed op, where a vector copy would cost a VGPR
     auto us = [](v2s v) { return P::from((uint32_t)uni((int)P::bits(v))); };
     const v2s voe = us(P::splat(so - se)), vc = us(P::splat(c));
-    const v2s vne = us(P::splat(-se)), vse = SH ? us(ve) : ve;
+    const v2s vne = us(P::splat(-se)), vse = XS ? us(ve) : ve;
     v2s bestL = NEG;
+    // CS: the top border's P = o - e j (first stripe), the floor z_j =
+    // e (1 - j), both advanced by -e a column, and best moved from H + o
+    // (column 0's space) to column -1's
+    v2s vtop = P::splat(so), z = P::splat(se);
+    const v2s vn2e = us(P::splat(-2 * se));
+    if constexpr (CS) best = P::add(best, us(P::splat(se)));
     v2s Ho[RR], E[RR];
     // SH: the left border's o - e r as a chain of adds from a base the
     // compiler cannot see through, so the RR constants are not hoisted out
@@ -414,11 +499,11 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
     if constexpr (SH) asm volatile("" : "+v"(t));
 #pragma unroll
     for (int r = 0; r < RR; ++r) {
-        Ho[r] = t;  // H = 0 on the left border
+        Ho[r] = CS ? us(P::splat(so + se)) : t;  // H = 0 on the left border
         if constexpr (SH) t = P::add(t, vne);
         E[r] = NEG;
     }
-    v2s hdt = SH ? P::splat(so + se - c) : vo;
+    v2s hdt = SH ? P::splat(so + se - c) : CS ? us(P::splat(so + se)) : vo;
     auto code = [&](int j) -> int {
         const uint32_t c2 = (w2[j >> 4] >> (2 * (j & 15))) & 3u;
         return ((wn[j >> 5] >> (j & 31)) & 1u) ? 4 : (int)c2;
@@ -448,9 +533,17 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
     };
     auto col = [&](const v2s (&sc)[RR], uint2 b) -> uint2 {
         v2s hou = P::from(b.x), f = P::from(b.y);
+        if constexpr (CS && FIRST) {
+            hou = vtop;
+            vtop = P::add(vtop, vne);
+        }
         const v2s hod = hdt;
         hdt = hou;
-        if constexpr (SH) {
+        if constexpr (CS) {
+            const v2s tcol = column16c<RR, F16>(sc, Ho, E, hou, f, hod, voe, vse, z);
+            z = P::add(z, vne);
+            best = P::max(P::add(best, vne), tcol);
+        } else if constexpr (SH) {
             column16s<RR, !FIRST, F16>(sc, Ho, E, hou, f, hod, voe, vse, vc);
             if constexpr (LAST) bestL = P::max(bestL, hou);
         } else {
@@ -462,14 +555,33 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
 
     v2s sa[RR], sb[RR];
     plane(code(0), sa);
-    if constexpr (SH) plane(code(1 < m ? 1 : m), sb);
+    if constexpr (XS) plane(code(1 < m ? 1 : m), sb);
     uint2 o0 = border, o1 = border;
     int j = 0;
     // Columns jj (plane sa) and jj + 1 into o0 and o1. SH: both planes are
     // loaded on entry, and the pair loads those of columns jj + 2 and jj + 3
     // (at most the padding column m) for the next one as it goes.
     auto pair = [&](int jj, uint2 x0, uint2 x1) {
-        if constexpr (SH) {
+        if constexpr (CS) {
+            v2s puA = P::from(x0.x), qA = P::from(x0.y), puB = P::from(x1.x), qB = P::from(x1.y);
+            if constexpr (FIRST) {
+                puA = vtop;
+                puB = P::add(vtop, vne);
+                vtop = P::add(puB, vne);
+            }
+            const v2s pdA = hdt, pdB = puA;
+            hdt = puB;
+            const v2s zA = z, zB = P::add(z, vne);
+            z = P::add(zB, vne);
+            const int ja = jj + 2 < m ? jj + 2 : m, jb = jj + 3 < m ? jj + 3 : m;
+            v2s tA, tB;
+            column16c_pair<RR, F16>(sa, sb, Ho, E, puA, qA, pdA, puB, qB, pdB, voe, vse, zA, zB, tA, tB,
+                                    prof + code(ja) * 4 * 64 + lane, prof + code(jb) * 4 * 64 + lane);
+            // best from column jj - 1's space to jj + 1's
+            best = P::max3(P::add(best, vn2e), P::add(tA, vne), tB);
+            o0 = make_uint2(P::bits(puA), P::bits(qA));
+            o1 = make_uint2(P::bits(puB), P::bits(qB));
+        } else if constexpr (SH) {
             v2s puA = P::from(x0.x), qA = P::from(x0.y), puB = P::from(x1.x), qB = P::from(x1.y);
             const v2s pdA = hdt, pdB = puA;
             hdt = puB;
@@ -622,6 +734,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
 
     }
 
+    if constexpr (CS) best = P::add(best, us(P::splat(se * (m - 1))));  // column m - 1's space to H + o
     if constexpr (!LOCAL) {
         // the last column's real rows, in Ho = H + o space (SH: P + e r,
         // the e r again a chain from an opaque zero)
@@ -641,15 +754,15 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
     }
 }
 
-template <int RR, bool LOCAL, bool FIRST, bool QUAD, bool F16, bool OCT, bool SH = false>
+template <int RR, bool LOCAL, bool FIRST, bool QUAD, bool F16, bool OCT, bool SH = false, bool CS = false>
 __device__ __forceinline__ void stripe16d_any(const uint4 *prof, uint2 *scr, const int lane,
                                               const cwords w2, const cwords wn, const int m,
                                               const bool last, const int row0, const int pA,
                                               const int pB, const typename Pk<F16>::T vo,
                                               const typename Pk<F16>::T ve, typename Pk<F16>::T &best,
                                               const int c = 0) {
-    if (last) stripe16d<RR, LOCAL, FIRST, true, QUAD, F16, OCT, SH>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best, c);
-    else stripe16d<RR, LOCAL, FIRST, false, QUAD, F16, OCT, SH>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best, c);
+    if (last) stripe16d<RR, LOCAL, FIRST, true, QUAD, F16, OCT, SH, CS>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best, c);
+    else stripe16d<RR, LOCAL, FIRST, false, QUAD, F16, OCT, SH, CS>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best, c);
 }
 
 template <bool LOCAL, int PF>
@@ -669,10 +782,11 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
     // variants 4 and 5: variant 3's schedule, and eight columns per
     // iteration with the boundary four columns ahead, in fp16 arithmetic;
     // 6, 7 and 8: variants 3, 5 and 4 with the row-shifted semiglobal
-    // column (column16s; in local mode they are variants 3, 5 and 4)
+    // column (column16s), in local mode the column-shifted one (column16c)
     constexpr bool F16 = PF == 4 || PF == 5 || PF == 7 || PF == 8;
     constexpr bool OCT = PF == 5 || PF == 7;
     constexpr bool SH = !LOCAL && PF >= 6;
+    constexpr bool CS = LOCAL && PF >= 6;  // local mode: the column shift (column16c)
     using PK = Pk<F16>;
     using V = typename PK::T;
     const V vo = PK::splat(a.sc.o), ve = PK::splat(a.sc.e), NEG = PK::neg();
@@ -760,14 +874,14 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
             if constexpr (PF >= 2) {
                 constexpr bool Q = PF == 3 || PF == 4 || PF == 6 || PF == 8;
                 if (s > 0) {
-                    stripe16d_any<16, LOCAL, false, Q, F16, OCT, SH>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best, cadj);
+                    stripe16d_any<16, LOCAL, false, Q, F16, OCT, SH, CS>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best, cadj);
                 } else {
                     switch (RR) {
 #define ESCH_FIRST(H) \
-    case H: stripe16d_any<H, LOCAL, true, Q, F16, OCT, SH>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best); break;
+    case H: stripe16d_any<H, LOCAL, true, Q, F16, OCT, SH, CS>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best); break;
                         ESCH_FIRST(16) ESCH_FIRST(14) ESCH_FIRST(12) ESCH_FIRST(10)
                         ESCH_FIRST(8) ESCH_FIRST(6) ESCH_FIRST(4)
-                        default: stripe16d_any<2, LOCAL, true, Q, F16, OCT, SH>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
+                        default: stripe16d_any<2, LOCAL, true, Q, F16, OCT, SH, CS>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
 #undef ESCH_FIRST
                     }
                 }

@@ -228,16 +228,17 @@ esch_status esch_scorer_new(int ordinal, const uint8_t *const *refs, const size_
     // workloads); esch_autotune re-decides on the caller's reads
     s->f16_ok = fits_f16((int64_t)max_len, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e);
     {
-        const int64_t sb = esch::shifted_bound((int64_t)max_len, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e);
+        // the shifted variants: rows shifted in semiglobal mode, columns in local
+        const int64_t sb = local ? esch::colshift_bound((int64_t)max_len, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e)
+                                 : esch::shifted_bound((int64_t)max_len, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e);
         for (int v = 0; v < kDp16Variants; ++v) {
             const bool f16 = esch::dp16_is_f16(v);
-            const bool shift_ok = local || !esch::dp16_is_shifted(v) || sb <= (f16 ? esch::kBoundF16 : esch::kBound16);
+            const bool shift_ok = !esch::dp16_is_shifted(v) || sb <= (f16 ? esch::kBoundF16 : esch::kBound16);
             s->usable16[v] = (!f16 || s->f16_ok) && shift_ok;
         }
     }
-    // semiglobal: the row-shifted variants where their bound holds
-    int v0 = s->f16_ok ? kDp16F16 : kDp16Default;
-    if (!local) v0 = s->usable16[7] ? 7 : s->f16_ok ? kDp16F16 : s->usable16[6] ? 6 : kDp16Default;
+    // the shifted variants where their bound holds
+    const int v0 = s->usable16[7] ? 7 : s->f16_ok ? kDp16F16 : s->usable16[6] ? 6 : kDp16Default;
     s->geo.variant16 = v0;
     s->geo.grid16 = s->grid16_v[v0];
     s->regs16 = s->regs16_v[v0];
@@ -1104,8 +1105,11 @@ esch_status esch_set_variant(esch_scorer *s, int variant) {
                     variant, (long long)esch::kBoundF16);
     if (!s->usable16[variant])
         return fail(ESCH_INVALID_ARG,
-                    "variant %d (row-shifted) needs longest reference x max(|match|, |mismatch|) + |open| + "
-                    "17 |extend| <= %lld",
+                    s->sc.local ? "variant %d (column-shifted in local mode) needs longest reference x "
+                                  "(max(|match|, |mismatch|) + |extend|) + |open| + 3 |extend| + "
+                                  "max(|match|, |mismatch|) <= %lld"
+                                : "variant %d (row-shifted) needs longest reference x max(|match|, |mismatch|) + "
+                                  "|open| + 17 |extend| <= %lld",
                     variant, (long long)(esch::dp16_is_f16(variant) ? esch::kBoundF16 : esch::kBound16));
     s->geo.variant16 = variant;
     s->geo.grid16 = s->grid16_v[variant];

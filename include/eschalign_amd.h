@@ -383,28 +383,31 @@ size_t esch_cigar_string(const uint32_t *ops, size_t n_ops, size_t query_len, si
  *   2  every memory access of a column pair issued at its start, stores
  *      deferred by one pair, no data-dependent branch around a memory
  *      access;
- *   3  (local-mode default) columns in quads: two pairs whose boundary
- *      registers swap roles;
- *   4  (local-mode default where it applies) variant 3 in packed fp16
- *      arithmetic, whose three-way maximum is one instruction: 7 packed ops
- *      per cell pair in semiglobal mode instead of 8. Exact only while every
- *      value is an integer fp16 holds exactly, so only for panels with
- *      longest reference x max(|match|, |mismatch|) + |open| + |extend| <=
- *      2048; esch_set_variant refuses it (ESCH_INVALID_ARG) elsewhere;
+ *   3  (default where no other applies) columns in quads: two pairs whose
+ *      boundary registers swap roles;
+ *   4  (default where 7 does not apply and 2048 holds) variant 3 in packed
+ *      fp16 arithmetic, whose three-way maximum is one instruction: 7
+ *      packed ops per cell pair in semiglobal mode instead of 8. Exact only
+ *      while every value is an integer fp16 holds exactly, so only for
+ *      panels with longest reference x max(|match|, |mismatch|) + |open| +
+ *      |extend| <= 2048; esch_set_variant refuses it (ESCH_INVALID_ARG)
+ *      elsewhere;
  *   5  the fp16 arithmetic of variant 4 over eight columns per iteration,
  *      the stripe boundary loaded four columns ahead; the same condition;
- *   6  (semiglobal default where 7 does not apply) variant 3 with every
- *      value of a stripe row r shifted by |e| r, which takes the extension
- *      add out of the vertical gap chain: 7 packed ops instead of 8. In
- *      semiglobal mode it needs longest reference x max(|match|, |mismatch|)
- *      + |open| + 17 |extend| <= 32000; in local mode it is variant 3;
- *   7  (semiglobal default where it applies) variant 5 with that shift: 6
- *      packed ops instead of 7; in semiglobal mode it needs the same sum at
- *      most 2048, in local mode it is variant 5;
- *   8  variant 4 with that shift (fewer registers than 7, one more resident
- *      wave, a shallower prefetch); the condition of 7. In the shifted
- *      variants the two columns of a pair run one row apart, so their
- *      serial chains interleave. */
+ *   6  (default where 7 does not apply and 4 neither) variant 3 with
+ *      shifted values, which take one gap chain's extension add out:
+ *      semiglobal mode shifts every value of stripe row r by |e| r (7 packed
+ *      ops instead of 8; needs longest reference x M + |open| + 17 |extend|
+ *      <= 32000, M = max(|match|, |mismatch|)); local mode shifts reference
+ *      column j by |e| j (9 instead of 10; needs longest reference x (M +
+ *      |extend|) + |open| + 3 |extend| + M <= 32000);
+ *   7  (default where it applies) variant 5 with those shifts: 6 packed ops
+ *      instead of 7 in semiglobal mode, 7.5 instead of 8.5 in local mode;
+ *      the same sums at most 2048;
+ *   8  variant 4 with those shifts (fewer registers than 7, one more
+ *      resident wave, a shallower prefetch); the condition of 7.
+ *      In the shifted variants the two columns of a pair run one row apart,
+ *      so their serial chains interleave. */
 int esch_n_variants(void);
 int esch_get_variant(const esch_scorer *s);
 esch_status esch_set_variant(esch_scorer *s, int variant);

@@ -366,7 +366,7 @@ def test_fp16_variant_at_its_bound():
     base against a reference of another: cells down at the low end of the
     bound), equal the
     oracle and variant 3 in both modes. One base longer, the panel refuses
-    the variant and defaults to the int16 one."""
+    the fp16 variants and defaults to an int16 one."""
     rng = G.Rng(0x5EED_0A03)
     for scoring, L in (((2, -1, -10, -1), 1018), ((1, -1, -2, -1), 2045)):
         refs = [G.random_seq(rng, L, 0.0), b"C" * L, G.random_seq(rng, L // 2, 0.02),
@@ -390,7 +390,7 @@ def test_fp16_variant_at_its_bound():
                 assert okv.all() and np.array_equal(sv, want), (scoring, mode, v)
         longer = ea.Panel(refs + [G.random_seq(rng, L + 1, 0.0)])
         gpu = ea.GpuScorer(longer, ea.Scoring(*scoring), ea.Mode.LOCAL)
-        assert gpu.variant == 3
+        assert gpu.variant == 6  # the int16 column-shifted variant (its bound holds)
         for v in F16_VARIANTS:
             with pytest.raises(ea.GpuError):
                 gpu.variant = v
@@ -440,8 +440,48 @@ def test_row_shifted_variants_at_their_bound():
         for w in others:
             with pytest.raises(ea.GpuError):
                 gpu.variant = w
+
+
+def test_column_shifted_local_variants_at_their_bound():
+    """In local mode variants 6, 7 and 8 hold every value of reference
+    column j shifted by |e| j: exact while longest reference x (M + |e|) +
+    |o| + 3 |e| + M stays within 2048 (fp16) or 32000 (int16). Panels
+    exactly at it, with reads reaching the top score (a reference read back
+    whole) and the floor (runs of one base against a reference of another),
+    several stripes long, equal the oracle; one base longer, local mode
+    refuses the variant while semiglobal mode (its row-shift bound) keeps
+    it."""
+    rng = G.Rng(0x5EED_0A05)
+    for scoring, L, vs in (((1, -1, -2, -1), 1021, (7, 8)), ((2, -1, -12, -3), 405, (7, 8)),
+                           ((100, -100, -200, -100), 157, (6,))):
+        M = max(abs(scoring[0]), abs(scoring[1]))
+        e = abs(scoring[3])
+        assert L * (M + e) + abs(scoring[2]) + 3 * e + M == (32000 if vs == (6,) else 2048)
+        refs = [G.random_seq(rng, L, 0.0), b"C" * L, G.random_seq(rng, L // 2, 0.02),
+                G.random_seq(rng, 40, 0.0)]
+        reads = [refs[0], b"A" * L, b"A" * (L // 3), b"G" * 5, refs[0][L // 9:L - L // 9],
+                 G.random_seq(rng, L + 40, 0.03), refs[0] + refs[0][:100], b"C" * 3 + refs[0]]
+        reads += [G.random_seq(rng, 1 + rng.below(L), 0.02) for _ in range(40)]
+        queries = codes_of(reads)
+        queries += [ea.reverse_complement_codes(q) for q in queries]
+        panel = ea.Panel(refs)
+        gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), ea.Mode.LOCAL)
+        gpu.diag_ratio = 0
+        want = oracle(queries, panel.codes, scoring, ea.Mode.LOCAL)
+        for w in vs:
+            gpu.variant = w
+            s32, ok = gpu.score_batch_i32(queries)
+            assert ok.all()
+            bad = np.argwhere(s32 != want)
+            assert bad.size == 0, (scoring, w, bad[:4])
+        assert want.max() == L * scoring[0], want.max()
+        longer = ea.Panel(refs + [G.random_seq(rng, L + 1, 0.0)])
         gpu = ea.GpuScorer(longer, ea.Scoring(*scoring), ea.Mode.LOCAL)
-        for w in others:
+        for w in vs:
+            with pytest.raises(ea.GpuError):
+                gpu.variant = w
+        gpu = ea.GpuScorer(longer, ea.Scoring(*scoring), ea.Mode.SEMIGLOBAL)
+        for w in vs:
             gpu.variant = w
 
 
