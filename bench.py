@@ -58,7 +58,9 @@ VALU_OPS_PER_CELL_ISSUED = {("semiglobal", 3): round(344_021_737_036 * 128 / 5_3
                             ("semiglobal", 4): round(304_396_230_832 * 128 / 5_303_272_453_076, 3),
                             ("semiglobal", 5): round(301_256_340_206 * 128 / 5_303_272_453_076, 3),
                             ("semiglobal", 6): round(306_294_990_254 * 128 / 5_303_272_453_076, 3),
-                            ("semiglobal", 7): round(263_781_725_920 * 128 / 5_303_272_453_076, 3)}
+                            ("semiglobal", 7): round(263_781_725_920 * 128 / 5_303_272_453_076, 3),
+                            ("local", 6): round(389_701_369_198 * 128 / 5_303_272_453_076, 3),
+                            ("local", 7): round(327_569_070_244 * 128 / 5_303_272_453_076, 3)}
 
 
 def log(*a):
