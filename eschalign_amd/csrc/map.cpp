@@ -251,6 +251,14 @@ void place_read(esch_mapped *M, const Pool &pool, esch_read_mapping &m) {
 
 using Getter = std::function<void(size_t, const uint8_t *&, size_t &)>;
 
+// Room for `extra` more elements, growing geometrically: an exact reserve
+// per chunk would copy everything gathered so far at every chunk.
+template <class V>
+void reserve_more(V &v, size_t extra) {
+    const size_t need = v.size() + extra;
+    if (need > v.capacity()) v.reserve(std::max(need, 2 * v.capacity()));
+}
+
 size_t chunk_rows(const esch_scorer *s) {
     // resident int32 rows of a chunk: each read (mate) and its reverse complement
     if (s->map_chunk) return 2 * s->map_chunk;
@@ -443,7 +451,14 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
     // the mates of Separate pairs alone (the reference scores rows on
     // demand too, lib/mapper/pair.rs:195-207). Both strands: all four.
     const bool half = !both_strands;
-    const size_t chunk = std::max<size_t>(1, chunk_rows(s) / (half ? 2 : 4));
+    // pairs per chunk: what the rows' memory allows, and by default about a
+    // sixteenth of the call (at least 2^16 pairs), so that staging, the tier
+    // loop and the assembly of one chunk are a small part of the whole
+    // (those of the first and last chunk are not hidden behind scoring).
+    // Measured on 2M pairs of 500 references: 897 k pairs/s in two chunks,
+    // 944 k in eight, 959 k in sixteen.
+    const size_t cap = std::max<size_t>(1, chunk_rows(s) / (half ? 2 : 4));
+    const size_t chunk = s->map_chunk ? cap : std::min(cap, std::max<size_t>((size_t)1 << 16, (n_pairs + 15) / 16));
     const size_t n_chunks = (n_pairs + chunk - 1) / chunk;
     std::vector<uint8_t> &rc2 = s->map->rc2;  // grow-only across calls
     std::vector<size_t> &rc2_at = s->map->rc2_at;
@@ -585,8 +600,8 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
             Traced tr;
             if (esch_status e = cx.trace(jrows, jrefs, tr)) return e;
             std::vector<int32_t> nl, nb, ok;
-            pool.hits.reserve(pool.hits.size() + jrows.size());
-            pool.ops.reserve(pool.ops.size() + tr.ops.size());
+            reserve_more(pool.hits, jrows.size());
+            reserve_more(pool.ops, tr.ops.size());
             for (size_t e = 0; e < n; ++e) {
                 const int32_t i = list[e];
                 const int32_t nt = recs[e * kTierRec];
@@ -692,8 +707,8 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
             if (esch_status e = score_chunk(held.next)) return e;
         // into the result, in pair order
         const auto ta = Clock::now();
-        M->hits.reserve(M->hits.size() + pool.hits.size());
-        M->ops.reserve(M->ops.size() + pool.ops.size());
+        reserve_more(M->hits, pool.hits.size());
+        reserve_more(M->ops, pool.ops.size());
         for (size_t i = 0; i < c; ++i) {
             esch_pair_mapping &m = res[i];
             if (m.proper) {
