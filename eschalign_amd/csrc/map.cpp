@@ -150,13 +150,23 @@ struct Ctx {
         return st;
     }
 
-    // The unpaired selection and tracebacks of listed rows (lib/mapper.rs:
-    // 418-550): out[e] for rows[e]; hits[e] into pool with hit_offset = the
-    // pool index of the read's first hit.
-    esch_status map_rows(const std::vector<int32_t> &rows, int both, int32_t min_score, int64_t max_ties,
-                         std::vector<esch_read_mapping> &out, Pool &pool) {
+    // The device half of mapping listed rows: the fused selection (ties past
+    // the inline capacity again, with room for all) and the tracebacks of
+    // every taken tie (lib/mapper.rs:418-496). `build` then makes the
+    // records and hits on the host, so a caller can launch the next chunk's
+    // scoring between the two.
+    struct Selected {
+        std::vector<int32_t> recs, ties, ties_over;
+        std::vector<int64_t> tie_at;
+        std::vector<size_t> first;
+        Traced t;
+        int32_t tie(size_t e, int32_t j) const {
+            return tie_at[e] >= 0 ? ties_over[(size_t)tie_at[e] + (size_t)j] : ties[e * kSelCap + (size_t)j];
+        }
+    };
+    esch_status select_trace(const std::vector<int32_t> &rows, int both, int32_t min_score, int64_t max_ties,
+                             Selected &S) {
         const size_t n = rows.size();
-        out.assign(n, esch_read_mapping{});
         if (!n) return ESCH_OK;
         hipStream_t st = b->stream;
         const auto t0 = Clock::now();
@@ -166,10 +176,9 @@ struct Ctx {
         MateSelArgs a{b->d_out32, (int32_t)b->n_reads, (int32_t)s->n_refs, both, min_score, max_ties,
                       C.list.as<int32_t>(), (int32_t)n, kSelCap, nullptr, C.recs.as<int32_t>(), C.cands.as<int32_t>()};
         HIPC(launch_mate_select(a, kResolveBlocks, st));
-        std::vector<int32_t> recs, ties;
-        if (esch_status e = download(recs, C.recs, n * kSelRec, st)) return e;
-        if (esch_status e = download(ties, C.cands, n * kSelCap, st)) return e;
-        // ties beyond the inline capacity: those reads again, with room for all
+        if (esch_status e = download(S.recs, C.recs, n * kSelRec, st)) return e;
+        if (esch_status e = download(S.ties, C.cands, n * kSelCap, st)) return e;
+        const std::vector<int32_t> &recs = S.recs;
         std::vector<int32_t> over, over_rows;
         std::vector<int64_t> over_off;
         int64_t total = 0;
@@ -180,7 +189,6 @@ struct Ctx {
                 over_off.push_back(total);
                 total += recs[e * kSelRec + 4];
             }
-        std::vector<int32_t> ties_over;
         if (!over.empty()) {
             if (esch_status e = upload(C.list, over_rows, st)) return e;
             if (esch_status e = upload(C.cand_off, over_off, st)) return e;
@@ -193,29 +201,30 @@ struct Ctx {
             o.recs = C.set.as<int32_t>();
             o.ties = C.out.as<int32_t>();
             HIPC(launch_mate_select(o, kResolveBlocks, st));
-            if (esch_status e = download(ties_over, C.out, (size_t)total, st)) return e;
+            if (esch_status e = download(S.ties_over, C.out, (size_t)total, st)) return e;
         }
         stats.select_ms += ms_since(t0);
-        std::vector<int64_t> tie_at(n, -1);  // into ties_over for the overflowed
-        for (size_t i = 0; i < over.size(); ++i) tie_at[over[i]] = over_off[i];
-        auto tie = [&](size_t e, int32_t j) -> int32_t {
-            return tie_at[e] >= 0 ? ties_over[(size_t)tie_at[e] + (size_t)j] : ties[e * kSelCap + (size_t)j];
-        };
+        S.tie_at.assign(n, -1);  // into ties_over for the overflowed
+        for (size_t i = 0; i < over.size(); ++i) S.tie_at[over[i]] = over_off[i];
         // every taken tie traced (lib/mapper.rs:486-496)
         std::vector<int32_t> jrows, jrefs;
-        std::vector<size_t> first(n, 0);
+        S.first.assign(n, 0);
         for (size_t e = 0; e < n; ++e) {
-            first[e] = jrows.size();
+            S.first[e] = jrows.size();
             for (int32_t j = 0; j < recs[e * kSelRec + 4]; ++j) {
-                const int32_t c = tie(e, j);
+                const int32_t c = S.tie(e, j);
                 jrows.push_back((c & 1) ? (int32_t)b->n_reads + rows[e] : rows[e]);
                 jrefs.push_back(c >> 1);
             }
         }
-        Traced t;
-        if (esch_status e = trace(jrows, jrefs, t)) return e;
+        return trace(jrows, jrefs, S.t);
+    }
+
+    // The host half: records and hits (lib/mapper.rs:498-550); no device use.
+    static void build(size_t n, const Selected &S, std::vector<esch_read_mapping> &out, Pool &pool) {
+        out.assign(n, esch_read_mapping{});
         for (size_t e = 0; e < n; ++e) {
-            const int32_t *r = &recs[e * kSelRec];
+            const int32_t *r = &S.recs[e * kSelRec];
             esch_read_mapping &m = out[e];
             m.best = r[0];
             m.n_tied = r[1];
@@ -225,13 +234,23 @@ struct Ctx {
             const int32_t take = r[4];
             // unmapped: nothing taken, or the primary aligns no base
             // (lib/mapper.rs:508-512)
-            if (take == 0 || !t.aligned(first[e])) continue;
+            if (take == 0 || !S.t.aligned(S.first[e])) continue;
             for (int32_t j = 0; j < take; ++j) {
-                const int32_t c = tie(e, j);
-                pool.add(t, first[e] + (size_t)j, c >> 1, c & 1, 0, 0);
+                const int32_t c = S.tie(e, j);
+                pool.add(S.t, S.first[e] + (size_t)j, c >> 1, c & 1, 0, 0);
             }
             m.n_hits = take;
         }
+    }
+
+    // The unpaired selection and tracebacks of listed rows (lib/mapper.rs:
+    // 418-550): out[e] for rows[e]; hits[e] into pool with hit_offset = the
+    // pool index of the read's first hit.
+    esch_status map_rows(const std::vector<int32_t> &rows, int both, int32_t min_score, int64_t max_ties,
+                         std::vector<esch_read_mapping> &out, Pool &pool) {
+        Selected S;
+        if (esch_status e = select_trace(rows, both, min_score, max_ties, S)) return e;
+        build(rows.size(), S, out, pool);
         return ESCH_OK;
     }
 };
@@ -335,40 +354,58 @@ esch_status map_reads_core(esch_scorer *s, size_t n, const Getter &get, int both
     M->reads.resize(n);
     if (esch_status st = scorer_ready(s)) return st;
     std::lock_guard<std::mutex> call(s->map->mu);
-    const size_t chunk = chunk_rows(s) / 2;
-    for (size_t k0 = 0; k0 < n; k0 += chunk) {
-        const size_t k1 = std::min(n, k0 + chunk);
-        esch_batch *b = nullptr;
-        if (esch_status st = load_and_score(s, get, k0, k1, both, 0, &b, M->stats)) return st;
+    // reads per chunk: what the rows' memory allows, and by default about a
+    // quarter of the call (at least 2^17 reads). A chunk is staged while the
+    // one before scores, and its records are built while the next one
+    // scores; its selection and tracebacks run between the two scorings
+    // (beside the persistent scoring kernel they would wait for it).
+    // Measured on 1M reads of the 164-reference panel: 1.64 M reads/s in
+    // one chunk, 1.75 M in four, 1.67 M in sixteen (smaller traceback and
+    // scoring launches).
+    const size_t cap = std::max<size_t>(1, chunk_rows(s) / 2);
+    const size_t chunk = s->map_chunk ? cap : std::min(cap, std::max<size_t>((size_t)1 << 17, (n + 3) / 4));
+    const size_t n_chunks = (n + chunk - 1) / chunk;
+    struct Held {
+        esch_scorer *s;
+        esch_batch *cur = nullptr, *next = nullptr;
+        ~Held() {
+            if (cur) give_batch(s, cur);
+            if (next) give_batch(s, next);
+        }
+    } held{s};
+    if (n_chunks) {
+        if (esch_status st = load_chunk(s, get, 0, std::min(n, chunk), both, 0, &held.cur, M->stats)) return st;
+        if (esch_status st = score_chunk(held.cur)) return st;
+    }
+    for (size_t ci = 0; ci < n_chunks; ++ci) {
+        const size_t k0 = ci * chunk, k1 = std::min(n, k0 + chunk);
+        if (ci + 1 < n_chunks)
+            if (esch_status st = load_chunk(s, get, k1, std::min(n, k1 + chunk), both, 0, &held.next, M->stats))
+                return st;
+        esch_batch *b = held.cur;
+        if (esch_status st = finish_chunk(b, M->stats)) return st;
         Ctx cx{s, b, *s->map, M->stats};
         std::vector<int32_t> rows;  // non-empty reads; empty ones have best None
         for (size_t k = k0; k < k1; ++k)
             if (b->h_qlen[k - k0]) rows.push_back((int32_t)(k - k0));
+        Ctx::Selected S;
+        if (esch_status st = cx.select_trace(rows, both, min_score, max_ties, S)) return st;
+        // this chunk's device work is done: the next chunk scores meanwhile
+        if (held.next)
+            if (esch_status st = score_chunk(held.next)) return st;
+        // The chunk's hits go straight after the earlier chunks' (the
+        // result's arrays serve as the pool: offsets come out absolute), and
+        // a read without hits points where the next read's hits start.
+        const auto ta = Clock::now();
         std::vector<esch_read_mapping> maps;
         Pool pool;
-        esch_status st = cx.map_rows(rows, both, min_score, max_ties, maps, pool);
-        give_batch(s, b);
-        if (st) return st;
-        // The pool holds this chunk's hits in read order already: it
-        // becomes the result's (moved for the first chunk, appended with
-        // its offsets moved on after that), and a read without hits points
-        // where the next read's hits start.
-        const auto ta = Clock::now();
-        const int64_t hb = (int64_t)M->hits.size(), ob = (int64_t)M->ops.size();
-        if (hb == 0 && ob == 0) {
-            M->hits.swap(pool.hits);
-            M->ops.swap(pool.ops);
-        } else {
-            for (esch_hit &h : pool.hits) h.alignment.ops_offset += ob;
-            M->hits.insert(M->hits.end(), pool.hits.begin(), pool.hits.end());
-            M->ops.insert(M->ops.end(), pool.ops.begin(), pool.ops.end());
-        }
+        pool.hits.swap(M->hits);
+        pool.ops.swap(M->ops);
+        Ctx::build(rows.size(), S, maps, pool);
+        M->hits.swap(pool.hits);
+        M->ops.swap(pool.ops);
         for (size_t k = k0; k < k1; ++k) M->reads[k] = esch_read_mapping{};
-        for (size_t e = 0; e < rows.size(); ++e) {
-            esch_read_mapping &m = M->reads[k0 + (size_t)rows[e]];
-            m = maps[e];
-            m.hit_offset += hb;
-        }
+        for (size_t e = 0; e < rows.size(); ++e) M->reads[k0 + (size_t)rows[e]] = maps[e];
         int64_t next = (int64_t)M->hits.size();  // reads without hits, from the back
         for (size_t k = k1; k-- > k0;) {
             esch_read_mapping &m = M->reads[k];
@@ -376,6 +413,9 @@ esch_status map_reads_core(esch_scorer *s, size_t n, const Getter &get, int both
             else m.hit_offset = next;
         }
         M->stats.assemble_ms += ms_since(ta);
+        give_batch(s, held.cur);
+        held.cur = held.next;
+        held.next = nullptr;
     }
     M->stats.total_ms = ms_since(t0);
     *out = M.release();
