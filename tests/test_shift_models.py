@@ -1,0 +1,285 @@
+"""The shifted value spaces of k_dp16's variants 6-8 (eschalign_amd/csrc/
+kernels.hip: column16s / column16s_pair with the stripe rebase in semiglobal
+mode, column16c / column16c_pair in local mode), modelled step for step in
+Python on the CPU.
+
+Each model walks a read the way the kernel does. The read is padded at the
+top to its chunk's row plan (a first stripe of RR rows, then stripes of 16),
+columns go in pairs with a single tail column, each stripe's boundary is
+handed on in that stripe's own space, and the best is kept as the kernel
+keeps it. The models are compared with the oracle's score
+(lib/scalar.rs:120-163, restated in oracle/eschalign_oracle.c and pinned to
+the reference's vectors), so a wrong shift, rebase, floor or best
+conversion fails.
+
+They also record every value they form, against the ranges the exactness
+bounds in esch_internal.h claim (shifted_bound, colshift_bound). A value
+outside them would let the fp16 or int16 arithmetic round or clamp.
+"""
+import random
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+import refgen as G
+
+NEG = None  # -inf: an identity for max, absorbing for +
+
+
+def mx(*v):
+    w = [x for x in v if x is not None]
+    return max(w) if w else None
+
+
+def add(a, b):
+    return None if a is None or b is None else a + b
+
+
+class Range:
+    def __init__(self):
+        self.lo, self.hi = 0, 0
+
+    def see(self, *v):
+        for x in v:
+            if x is not None:
+                self.lo, self.hi = min(self.lo, x), max(self.hi, x)
+        return v[0] if len(v) == 1 else v
+
+
+def plan(nmax):
+    """k_dp16's row plan for a chunk whose longest read is nmax: S stripes,
+    the first of RR rows (an even count), T rows in all."""
+    S = (nmax + 15) // 16
+    RR = ((nmax - 16 * (S - 1) + 1) // 2) * 2
+    return S, RR, RR + 16 * (S - 1)
+
+
+def subst(x, c, ma, mi):
+    if x is None:
+        return 0  # a top padding row
+    return ma if (x == c or x == 4 or c == 4) else mi
+
+
+def walk_semiglobal_rows(q, r, scoring, nmax, rng_vals):
+    """Variant 7's semiglobal walk: row r of a stripe holds P = H - e r + o,
+    R = E - e r + e, Q = F - e r + e; the row above arrives in the previous
+    stripe's space, lower by c = e h, added to Q at row 0 and to the
+    profile's row 0; bests in H + o space (the last column's rows through
+    + e r, the last stripe's bottom row through + e (RR - 1))."""
+    ma, mi, o, e = scoring
+    S, RR, T = plan(nmax)
+    rows = [None] * (T - len(q)) + list(q)
+    pad = T - len(q)
+    m = len(r)
+    best = None
+    bottom = None
+    row0 = 0
+    for s in range(S):
+        h = RR if s == 0 else 16
+        c = 0 if s == 0 else e * (RR if s == 1 else 16)
+        Pv = [o - e * k for k in range(h)]
+        R = [NEG] * h
+        hdt = o + e - c
+        bestL = NEG
+        newb = []
+
+        def column(j, pu, qq, pd):
+            for k in range(h):
+                sc = subst(rows[row0 + k], r[j], ma, mi) - o + (c if k == 0 else 0)
+                R[k] = rng_vals.see(add(mx(Pv[k], R[k]), e))
+                qq = rng_vals.see(mx(pu, qq))
+                if k == 0 and s > 0:
+                    qq = rng_vals.see(add(qq, c))
+                hh = rng_vals.see(mx(rng_vals.see(pd + sc), R[k], qq))
+                pd = Pv[k]
+                Pv[k] = rng_vals.see(hh + (o - e))
+                pu = Pv[k]
+            return pu, qq
+
+        j = 0
+        while j < m:
+            for jj in ((j, j + 1) if j + 1 < m else (j,)):
+                pu, qq = (o + e, NEG) if s == 0 else bottom[jj]
+                pd = hdt
+                hdt = pu
+                pu, qq = column(jj, pu, qq, pd)
+                newb.append((pu, qq))
+                bestL = mx(bestL, pu)
+            j += 2
+        for k in range(h):
+            if row0 + k >= pad:
+                best = mx(best, rng_vals.see(Pv[k] + e * k))
+        if s == S - 1:
+            best = mx(best, rng_vals.see(add(bestL, e * (h - 1))))
+        bottom = newb
+        row0 += h
+    return best - o
+
+
+def walk_local_columns(q, r, scoring, nmax, rng_vals):
+    """Variant 7's local walk: column j holds P = H - e j + o,
+    R = E - e j + e, Q = F - e j + e, the zero floor is z_j = e (1 - j), and
+    the best is carried in the space of the last column done (two columns a
+    pair: best - 2e against the pair's tA - e and tB), back to H + o at the
+    stripe's end."""
+    ma, mi, o, e = scoring
+    S, RR, T = plan(nmax)
+    rows = [None] * (T - len(q)) + list(q)
+    m = len(r)
+    best = o  # H = 0
+    bottom = None
+    row0 = 0
+    for s in range(S):
+        h = RR if s == 0 else 16
+        Pv = [o + e] * h
+        R = [NEG] * h
+        hdt = o + e
+        vtop = o
+        z = e
+        b = best + e
+        newb = []
+
+        def column(j, pu, qq, pd, zf):
+            for k in range(h):
+                sc = subst(rows[row0 + k], r[j], ma, mi) - o
+                R[k] = rng_vals.see(mx(Pv[k], R[k]))
+                qq = rng_vals.see(add(mx(pu, qq), e))
+                hh = rng_vals.see(mx(mx(rng_vals.see(pd + sc), R[k], qq), zf))
+                pd = Pv[k]
+                Pv[k] = rng_vals.see(hh + (o - e))
+                pu = Pv[k]
+            return pu, qq, max(Pv)
+
+        j = 0
+        while j < m:
+            cols = (j, j + 1) if j + 1 < m else (j,)
+            ts = []
+            for jj in cols:
+                if s == 0:
+                    pu, qq = rng_vals.see(vtop), NEG
+                    vtop -= e
+                else:
+                    pu, qq = bottom[jj]
+                pd = hdt
+                hdt = pu
+                pu, qq, t = column(jj, pu, qq, pd, rng_vals.see(z))
+                z -= e
+                ts.append(t)
+                newb.append((pu, qq))
+            if len(cols) == 2:
+                b = rng_vals.see(max(rng_vals.see(b - 2 * e), rng_vals.see(ts[0] - e), ts[1]))
+            else:
+                b = rng_vals.see(max(rng_vals.see(b - e), ts[0]))
+            j += 2
+        best = rng_vals.see(b + e * (m - 1))
+        bottom = newb
+        row0 += h
+    return best - o
+
+
+def cases(seed, n_cases):
+    rng = random.Random(seed)
+    schemes = G.SCHEMES_GPU + [(1, -3, -1, -1), (2, -3, -5, -2)]
+    for _ in range(n_cases):
+        sc = schemes[rng.randrange(len(schemes))]
+        n, m = rng.randint(1, 60), rng.randint(1, 24)
+        q = [rng.randrange(5) if rng.random() < 0.1 else rng.randrange(4) for _ in range(n)]
+        ref = [rng.randrange(5) if rng.random() < 0.1 else rng.randrange(4) for _ in range(m)]
+        if rng.random() < 0.3:  # a read that matches the reference: the top of the range
+            q = (ref * (n // m + 1))[:n]
+        yield sc, q, ref, n + rng.randint(0, 20)
+
+
+def bounds(scoring):
+    """M = max(|match|, |mismatch|), |open|, |extend|."""
+    ma, mi, o, e = scoring
+    return max(abs(ma), abs(mi)), abs(o), abs(e)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_row_shifted_semiglobal_walk_equals_oracle(seed):
+    """The row shift (semiglobal): scores equal the oracle's, and every
+    value lies in [-(kM + |o| + 4|e|), kM + 16|e|] with k the read-reference
+    overlap bound min(n, m), the range shifted_bound rests on."""
+    for sc, q, ref, nmax in cases(seed, 150):
+        vals = Range()
+        got = walk_semiglobal_rows(q, ref, sc, nmax, vals)
+        want = O.score(np.array(q, np.uint8), np.array(ref, np.uint8), sc, False)
+        assert got == want, (sc, q, ref, nmax)
+        M, O_, E = bounds(sc)
+        k = min(len(q), len(ref))
+        assert -(k * M + O_ + 4 * E) <= vals.lo and vals.hi <= k * M + 16 * E, (sc, vals.lo, vals.hi, k)
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_column_shifted_local_walk_equals_oracle(seed):
+    """The column shift (local): scores equal the oracle's, and every value
+    lies in [-(|o| + 3|e| + M), kM + |e| (m + 2) + M], m the reference
+    length, the range colshift_bound rests on."""
+    for sc, q, ref, nmax in cases(seed, 150):
+        vals = Range()
+        got = walk_local_columns(q, ref, sc, nmax, vals)
+        want = O.score(np.array(q, np.uint8), np.array(ref, np.uint8), sc, True)
+        assert got == want, (sc, q, ref, nmax)
+        M, O_, E = bounds(sc)
+        k = min(len(q), len(ref))
+        lo, hi = -(O_ + 3 * E + M), k * M + E * (len(ref) + 2) + M
+        assert lo <= vals.lo and vals.hi <= hi, (sc, vals.lo, vals.hi, lo, hi)
+
+
+def test_a_wrong_rebase_is_caught():
+    """The semiglobal model with the stripe rebase left out (c = 0: the
+    shift restarted per stripe without moving the boundary into the new
+    stripe's space) disagrees with the oracle on reads of several stripes,
+    so the equality above sees the mistake it guards against."""
+    wrong = tried = 0
+    for sc, q, ref, nmax in cases(5, 80):
+        if nmax <= 16 or sc[3] == 0:
+            continue
+        tried += 1
+        got = walk_semiglobal_rows_without_rebase(q, ref, sc, nmax)
+        want = O.score(np.array(q, np.uint8), np.array(ref, np.uint8), sc, False)
+        wrong += got != want
+    assert tried and wrong > 0
+
+
+def walk_semiglobal_rows_without_rebase(q, r, scoring, nmax):
+    """walk_semiglobal_rows with c forced to 0 (the mistake)."""
+    ma, mi, o, e = scoring
+    S, RR, T = plan(nmax)
+    rows = [None] * (T - len(q)) + list(q)
+    pad = T - len(q)
+    m = len(r)
+    best = None
+    bottom = None
+    row0 = 0
+    for s in range(S):
+        h = RR if s == 0 else 16
+        Pv = [o - e * k for k in range(h)]
+        R = [NEG] * h
+        hdt = o + e
+        bestL = NEG
+        newb = []
+        for j in range(m):
+            pu, qq = (o + e, NEG) if s == 0 else bottom[j]
+            pd = hdt
+            hdt = pu
+            for k in range(h):
+                sc = subst(rows[row0 + k], r[j], ma, mi) - o
+                R[k] = add(mx(Pv[k], R[k]), e)
+                qq = mx(pu, qq)
+                hh = mx(pd + sc, R[k], qq)
+                pd = Pv[k]
+                Pv[k] = hh + (o - e)
+                pu = Pv[k]
+            newb.append((pu, qq))
+            bestL = mx(bestL, pu)
+        for k in range(h):
+            if row0 + k >= pad:
+                best = mx(best, Pv[k] + e * k)
+        if s == S - 1:
+            best = mx(best, add(bestL, e * (h - 1)))
+        bottom = newb
+        row0 += h
+    return best - o
