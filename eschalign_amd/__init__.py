@@ -63,7 +63,7 @@ EXPORTS = [
     "esch_batch_ready", "esch_server_new", "esch_server_free", "esch_server_add_panel",
     "esch_server_scorer", "esch_server_submit", "esch_server_wait", "esch_server_result",
     "esch_server_stats", "esch_set_diag_min_len", "esch_get_diag_min_len",
-    "esch_set_swap_rows", "esch_get_swap_rows", "esch_batch_swapped",
+    "esch_set_swap_rows", "esch_get_swap_rows", "esch_batch_swapped", "esch_batch_variant",
     "esch_batch_stats_diag", "esch_set_diag_ratio", "esch_get_diag_ratio", "esch_valu_peak16",
     "esch_align_pairs", "esch_align_last_ms", "esch_align_pairs_concat", "esch_map_reads",
     "esch_map_pairs", "esch_map_pairs_concat", "esch_mapped_len", "esch_mapped_reads",
@@ -204,6 +204,8 @@ def load_library():
     L.esch_get_swap_rows.restype = i32
     L.esch_batch_swapped.argtypes = [vp]
     L.esch_batch_swapped.restype = C.c_int
+    L.esch_batch_variant.argtypes = [vp]
+    L.esch_batch_variant.restype = C.c_int
     L.esch_batch_stats_diag.argtypes = [vp, C.POINTER(C.c_uint64)]
     L.esch_set_diag_ratio.argtypes = [vp, C.c_int32]
     L.esch_align_pairs.argtypes = [vp, vp, vp, vp, sz, vp, vp, sz, C.POINTER(sz)]
@@ -1081,6 +1083,10 @@ class DeviceBatch:
     def swapped(self) -> bool:
         """The last run scored with the roles of reads and references swapped."""
         return bool(_lib.esch_batch_swapped(self._h))
+
+    def variant(self) -> int:
+        """The batched kernel variant of the last run (-1: none, or swapped)."""
+        return int(_lib.esch_batch_variant(self._h))
 
     def scores32(self):
         nr = self.scorer.n_refs

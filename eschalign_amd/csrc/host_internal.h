@@ -61,6 +61,7 @@ struct esch_scorer {
     int regs16 = 0, local16 = 0;
     bool f16_ok = false;             // the panel fits k_dp16's fp16 variant (fits_f16)
     bool usable16[kDp16Variants] = {};  // variants exact on this panel (fp16 and shift bounds)
+    bool variant_fixed = false;         // set by esch_set_variant (the caller or the autotuner)
     int diag_min_len = 0;               // reads this long take k_diag16 (0: never)
     int diag_ratio = kDiagRatioDefault; // padding-heavy length bins take k_diag16 (0: never)
     bool diag_ratio_tunable = true;     // false once the caller set the ratio
@@ -88,6 +89,8 @@ struct SwapBufs;
 struct esch_batch {
     esch_scorer *s = nullptr;
     size_t max_reads = 0, max_bases = 0;  // capacity before strand expansion
+    size_t longest = 0;                   // the loaded reads' longest length
+    int last_variant = -1;                // k_dp16 variant of the last run (-1: none, or swapped)
     size_t tie_cap_alloc = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;

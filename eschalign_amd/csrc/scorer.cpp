@@ -512,7 +512,7 @@ esch_status batch_stage(esch_batch *b, size_t n, int both, int pairs,
     // threads (1M short reads are a million small copies), then one
     // vectorisable pass for the code check; the offending read is looked
     // up only when the check fails
-    size_t at = 0;
+    size_t at = 0, longest = 0;
     for (size_t k = 0; k < n; ++k) {
         const uint8_t *p;
         size_t len;
@@ -520,7 +520,9 @@ esch_status batch_stage(esch_batch *b, size_t n, int both, int pairs,
         b->h_qoff[k] = at;
         b->h_qlen[k] = (int32_t)len;
         at += len;
+        longest = std::max(longest, len);
     }
+    b->longest = longest;
     const size_t hw = std::max(1u, std::thread::hardware_concurrency());
     const size_t nt = total < ((size_t)8 << 20) ? 1 : std::min<size_t>(8, hw);
     std::vector<uint8_t> bad(nt, 0);
@@ -603,25 +605,43 @@ KArgs make_args(esch_batch *b, int what) {
     return a;
 }
 
-esch_status enqueue(esch_batch *b, int what, bool timed) {
+esch_status enqueue(esch_batch *b, int what, bool timed, const Launch &g) {
     esch_scorer *s = b->s;
     const KArgs a = make_args(b, what);
     const bool local = s->sc.local != 0;
     hipStream_t st = b->stream;
-    HIPC(launch_prep(a, s->geo, st));
+    HIPC(launch_prep(a, g, st));
     if (timed) HIPC(hipEventRecord(b->ev1, st));
     if (what & ESCH_RUN_PLAIN) {
-        HIPC(launch_dp32(a, s->geo, local, 0, st));
-        HIPC(launch_dp32(a, s->geo, local, 1, st));
+        HIPC(launch_dp32(a, g, local, 0, st));
+        HIPC(launch_dp32(a, g, local, 1, st));
     } else {
-        HIPC(launch_dp16(a, s->geo, local, st));
-        HIPC(launch_diag(a, s->geo, local, st));
-        HIPC(launch_dp32(a, s->geo, local, 1, st));
+        HIPC(launch_dp16(a, g, local, st));
+        HIPC(launch_diag(a, g, local, st));
+        HIPC(launch_dp32(a, g, local, 1, st));
     }
     if (timed) HIPC(hipEventRecord(b->ev2, st));
-    if (what & ESCH_RUN_SELECT) HIPC(launch_select(a, s->geo, st));
-    if (what & ESCH_RUN_PAIRS) HIPC(launch_pairs(a, s->geo, st));
+    if (what & ESCH_RUN_SELECT) HIPC(launch_select(a, g, st));
+    if (what & ESCH_RUN_PAIRS) HIPC(launch_pairs(a, g, st));
     return ESCH_OK;
+}
+
+// The k_dp16 variant a batch runs: the scorer's, unless the caller or the
+// autotuner fixed none and the panel admits no fp16 variant while this
+// batch's reads do. Every value of the fp16 and row-shifted recurrences is
+// bounded through k = min(read, reference), so the batch's longest read
+// caps k as well as the panel's longest reference does: short reads on a
+// panel of long references still get fp16 (semiglobal: the row-shifted 7,
+// 6 ops per cell, local: 4). The column shift's range grows with the
+// reference itself, so local mode does not take it from the reads.
+int batch_variant(const esch_batch *b) {
+    const esch_scorer *s = b->s;
+    const int v = s->geo.variant16;
+    if (s->variant_fixed || esch::dp16_is_f16(v)) return v;
+    const int64_t k = (int64_t)std::min<size_t>(s->max_len, b->longest);
+    if (!fits_f16(k, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e)) return v;
+    if (s->sc.local) return kDp16F16;
+    return esch::shifted_bound(k, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e) <= esch::kBoundF16 ? 7 : kDp16F16;
 }
 
 // Rows up to which a batch is scored with the roles swapped. With the
@@ -826,28 +846,33 @@ esch_status esch_batch_run(esch_batch *b, int what, int32_t min_score, int64_t m
     d.tstrands = (what & ESCH_RUN_PAIR_BOTH) ? 2 : 1;
     d.diag_min_len = (what & (ESCH_RUN_PLAIN | ESCH_RUN_SCORES16)) ? 0 : s->diag_min_len;
     d.diag_ratio = (what & ESCH_RUN_PLAIN) ? 0 : s->diag_ratio;
-    d.blocks16 = (int32_t)s->geo.grid16;
+    const int v16 = batch_variant(b);
+    Launch geo = s->geo;
+    geo.variant16 = v16;
+    geo.grid16 = s->grid16_v[v16];
+    d.blocks16 = (int32_t)geo.grid16;
     HIPC(hipMemcpyAsync(b->d_desc, b->h_desc, sizeof(DevDesc), hipMemcpyHostToDevice, b->stream));
     b->last_what = what;
     const bool graph = !(what & ESCH_RUN_NO_GRAPH);
     // the template-strand count is read from the descriptor, not baked in;
     // the kernel variant is baked into the captured launches
-    const int key = (what & ~(ESCH_RUN_NO_GRAPH | ESCH_RUN_PAIR_BOTH)) | (s->geo.variant16 << 16);
+    const int key = (what & ~(ESCH_RUN_NO_GRAPH | ESCH_RUN_PAIR_BOTH)) | (v16 << 16);
     HIPC(hipEventRecord(b->ev0, b->stream));
     b->last_swapped = swap_eligible(b, what);
+    b->last_variant = b->last_swapped ? -1 : v16;
     if (b->last_swapped) {
         b->timed_dp = true;
         if (esch_status st = run_swapped(b, what)) return st;
     } else if (!graph) {
         b->timed_dp = true;
-        if (esch_status st = enqueue(b, key, true)) return st;
+        if (esch_status st = enqueue(b, key, true, geo)) return st;
     } else {
         b->timed_dp = false;
         auto it = b->graphs.find(key);
         if (it == b->graphs.end()) {
             hipGraph_t g;
             HIPC(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            esch_status st = enqueue(b, key, false);
+            esch_status st = enqueue(b, key, false, geo);
             hipError_t ce = hipStreamEndCapture(b->stream, &g);
             if (st) return st;
             if (ce != hipSuccess) return fail(ESCH_DRIVER, "hipStreamEndCapture: %s", hipGetErrorString(ce));
@@ -1112,6 +1137,7 @@ esch_status esch_set_variant(esch_scorer *s, int variant) {
                                   "|open| + 17 |extend| <= %lld",
                     variant, (long long)(esch::dp16_is_f16(variant) ? esch::kBoundF16 : esch::kBound16));
     s->geo.variant16 = variant;
+    s->variant_fixed = true;
     s->geo.grid16 = s->grid16_v[variant];
     s->regs16 = s->regs16_v[variant];
     s->local16 = s->local16_v[variant];
@@ -1129,6 +1155,8 @@ esch_status esch_set_swap_rows(esch_scorer *s, int32_t rows) {
 int32_t esch_get_swap_rows(const esch_scorer *s) { return s ? (int32_t)swap_limit(s) : -1; }
 
 int esch_batch_swapped(const esch_batch *b) { return b && b->last_swapped ? 1 : 0; }
+
+int esch_batch_variant(const esch_batch *b) { return b ? b->last_variant : -1; }
 
 esch_status esch_set_diag_min_len(esch_scorer *s, int32_t min_len) {
     if (!s || min_len < 0) return fail(ESCH_INVALID_ARG, "bad anti-diagonal threshold");

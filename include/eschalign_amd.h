@@ -469,6 +469,15 @@ esch_status esch_batch_read_scores16(esch_batch *b, int16_t *scores, uint8_t *sc
 esch_status esch_batch_read_select(esch_batch *b, esch_read_select *out, int32_t *ties);
 /* 1 when the batch's last run scored with the roles swapped (esch_set_swap_rows). */
 int esch_batch_swapped(const esch_batch *b);
+/* The k_dp16 variant the batch's last run used (-1 before any run, or when
+ * it ran with the roles swapped). It is the scorer's variant, except when
+ * neither the caller nor the autotuner fixed one (esch_set_variant), the
+ * panel admits no fp16 variant, and the batch's reads do: every value is
+ * bounded through min(read, reference), so a batch whose longest read n
+ * gives min(n, longest reference) x max(|match|, |mismatch|) + |open| +
+ * 17 |extend| <= 2048 runs variant 7 in semiglobal mode (4 in local mode,
+ * from the same sum with |extend| once). No counterpart in the reference. */
+int esch_batch_variant(const esch_batch *b);
 esch_status esch_batch_read_pairs(esch_batch *b, esch_pair_select *out, int32_t *top);
 
 /* ---------------------------------------------------------------------

@@ -808,3 +808,45 @@ def test_autotune_sweeps_the_routing_ratio():
         gpu.diag_ratio = 0.5
     got, ok, _ = _routed(gpu, queries, 3.0)
     assert ok.all() and np.array_equal(got, oracle(queries, panel.codes, (1, -1, -2, -1), ea.Mode.LOCAL))
+
+
+@pytest.mark.gpu
+def test_short_reads_take_fp16_on_a_panel_of_long_references():
+    """Every fp16 and row-shifted value is bounded through min(read,
+    reference), so a batch's longest read caps it as the panel's longest
+    reference does. On a panel of 2,100-nt references (no fp16 variant
+    panel-wide: 2,100 + 2 + 1 > 2,048), a batch of reads up to 1,500 nt runs
+    variant 7 in semiglobal mode and 4 in local mode; a batch holding a
+    2,100-nt read (the top score, 2,100, is beyond fp16's exact integers)
+    stays on the scorer's int16 variant 6, and so does every batch once the
+    caller fixes a variant. Scores equal the oracle's in every case (the
+    captured graphs are keyed by the variant they launch)."""
+    rng = G.Rng(0x5EED_0A06)
+    scoring = (1, -1, -2, -1)
+    L = 2100
+    refs = [G.random_seq(rng, L, 0.0), G.random_seq(rng, L - 300, 0.01), b"C" * 900]
+    short = [refs[0][100:1600], refs[1][:700], b"A" * 400, G.random_seq(rng, 1500, 0.03)]
+    short += [G.random_seq(rng, 1 + rng.below(1500), 0.02) for _ in range(30)]
+    longer = short + [refs[0]]
+    panel = ea.Panel(refs)
+    for mode, fast in ((ea.Mode.SEMIGLOBAL, 7), (ea.Mode.LOCAL, 4)):
+        gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+        assert gpu.variant == 6
+        for reads, want_v in ((short, fast), (longer, 6), (short, fast)):
+            q = codes_of(reads)
+            b = gpu.batch(len(q), sum(len(x) for x in q))
+            b.load(q)
+            b.run(ea.RUN_SCORES32)
+            b.wait()
+            s32, ok = b.scores32()
+            assert b.variant() == want_v, (mode, len(reads), b.variant())
+            want = oracle(q, panel.codes, scoring, mode)
+            assert ok.all() and np.array_equal(s32, want), (mode, want_v)
+        gpu.variant = 6
+        q = codes_of(short)
+        b = gpu.batch(len(q), sum(len(x) for x in q))
+        b.load(q)
+        b.run(ea.RUN_SCORES32)
+        b.wait()
+        assert b.variant() == 6
+        assert np.array_equal(b.scores32()[0], oracle(q, panel.codes, scoring, mode))
