@@ -9,7 +9,8 @@ both strands, semiglobal mode, scoring 1,-1,-2,-1. Synthetic data.
 
 A step is one pass of the device pipeline over the batch, already resident
 in HBM: length bucketing, the batched packed 16-bit DP (in fp16 on panels
-whose value bound fits its exact integers, as this one does; int16 and
+whose value bound fits its exact integers, as this one does, with every
+stripe row's values shifted so that a cell takes 6 packed ops; int16 and
 int32 rescue where needed) and the fused best / tie selection. value =
 reads per second.
 
