@@ -1,0 +1,12 @@
+// The semiglobal-mode instances of k_dp16 (every variant) and their launch.
+#include "dp16.h"
+
+namespace esch {
+
+const void *dp16_fn_sg(int variant) { return dp16_fn_impl<false>(variant); }
+
+hipError_t launch_dp16_sg(const KArgs &a, const Launch &g, hipStream_t st) {
+    return launch_dp16_impl<false>(a, g, st);
+}
+
+}  // namespace esch

@@ -42,6 +42,55 @@ def test_config1_plain_cpu_reference_path():
     assert m.n_scored() == len(reads) and np.array_equal(m.scores.astype(np.int32), got)
 
 
+def test_config2_batched_uneven_lengths():
+    """The benchmark's own workload at 24,000 reads: the 164-ref panel of
+    140-160 nt (seed 0x0123456789ABCDEF), reads from synth.read_arrays with
+    seed 0x5EED_2002 and lognormal(90, 0.6) lengths clipped 20-1000, both
+    strands in one device batch, semiglobal 1,-1,-2,-1. The timed batch's
+    own int32 rows (as bench.py reads them) equal the CPU kernel for the
+    row-shifted fp16 variant 7, the int16 variant 6 and the plain
+    one-thread-per-pair kernel, eagerly and over three graph replays; the
+    fused selection equals the oracle's on those rows; a sample equals the
+    scalar oracle."""
+    panel = synth.panel(seed=0x0123456789ABCDEF, n_refs=164, lo=140, hi=160)
+    n = 24_000
+    codes, off, lens = synth.read_arrays(panel, seed=0x5EED_2002, n=n,
+                                         length=("lognormal", 90, 0.6, 20, 1000))
+    reads = [codes[int(off[k]):int(off[k]) + int(lens[k])] for k in range(n)]
+    rows = reads + [ea.reverse_complement_codes(q) for q in reads]
+    scoring = (1, -1, -2, -1)
+    want = simd_rows(panel, rows, scoring, False)
+    assert np.array_equal(want[:150], O.score_matrix(rows[:150], panel, scoring, False, THREADS))
+    assert np.array_equal(want[n:n + 150], O.score_matrix(rows[n:n + 150], panel, scoring, False, THREADS))
+    sel_want = [O.select(want[k], want[n + k], 0, None) for k in range(n)]
+    gpu = ea.GpuScorer(ea.Panel(panel), ea.Scoring(*scoring), ea.Mode.SEMIGLOBAL)
+    batch = gpu.batch(n, int(lens.sum()))
+    batch.load_concat(codes, off, lens, both_strands=True)
+    tie_cap = 8
+
+    def check(what, label):
+        batch.run(what, tie_cap=tie_cap)
+        batch.wait()
+        got, ok = batch.scores32()
+        assert ok.all(), label
+        bad = np.nonzero((got != want).any(axis=1))[0]
+        assert bad.size == 0, f"{label}: {bad.size} rows differ, first {bad[:5]}"
+        sel = batch.select()
+        for k in range(n):
+            b, nt, so, take = sel_want[k]
+            s = sel[k]
+            assert (s.best, s.n_tied, s.suboptimal, s.take) == (b, nt, so, take[:tie_cap]), f"{label}: read {k}"
+
+    base = ea.RUN_SCORES32 | ea.RUN_SELECT
+    for v in (7, 6):
+        gpu.variant = v
+        check(base | ea.RUN_NO_GRAPH, f"variant {v} eager")
+        assert batch.variant() == v
+        for rep in range(3):
+            check(base, f"variant {v} graph replay {rep}")
+    check(base | ea.RUN_PLAIN | ea.RUN_NO_GRAPH, "plain kernel")
+
+
 def test_config3_large_panel_near_ties():
     """A large panel of seeds plus 4 copies each at 2% substitutions (ties
     and near-ties common), refs 200-1000 nt, reads 100-1000 nt, local
