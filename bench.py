@@ -328,6 +328,152 @@ def map_reads_leg(gpu, panel_codes, codes, off, lens, scoring, local, n_reads, c
     return out
 
 
+def _timed_steps(batch, what, steps, tie_cap=1):
+    """Warm once, then `steps` eager runs: (wall s per step, mean DP ms from
+    the HIP events on the batch's stream)."""
+    batch.run(what, tie_cap=tie_cap)
+    batch.wait()
+    dps = []
+    t = time.perf_counter()
+    for _ in range(steps):
+        batch.run(what, tie_cap=tie_cap)
+        batch.wait()
+        dps.append(batch.timing()[0])
+    return (time.perf_counter() - t) / steps, float(np.mean(dps))
+
+
+def _roof(cells, dp_ms, mode, variant):
+    ops = OPS_PER_CELL[(mode, variant in F16_VARIANTS, variant in SHIFTED_VARIANTS)]
+    achieved = cells * ops / (dp_ms / 1e3)
+    return {"bound": "valu", "achieved": round(achieved / 1e12, 3), "peak": round(PEAK_I16 / 1e12, 3),
+            "unit": "Tlane-op/s (packed 16-bit)", "frac": round(achieved / PEAK_I16, 4), "ops_per_cell": ops}
+
+
+def config1_leg(steps, cpu):
+    """The plain CPU reference path's workload, on the device and on the
+    host: the 164-reference panel (140-160 nt), 10k reads (90% substrings
+    of 60-150 nt through the error model, 10% random), local 2,-1,-10,-1,
+    forward strand only, int32 scores. Device: the autotuned batched kernel
+    and the plain one-thread-per-pair kernel, from HBM; host: the oracle's
+    AVX-512 restatement of the reference's CPU kernel over all 10k reads on
+    the host cores, equal to the device's scores."""
+    import eschalign_amd as ea
+    from eschalign_amd import synth
+
+    panel = synth.panel(seed=0x0123456789ABCDEF, n_refs=164, lo=140, hi=160)
+    n = 10_000
+    codes, off, lens = synth.read_arrays(panel, seed=0xC0F1, n=n, length=("uniform", 60, 150))
+    reads = [codes[int(off[k]):int(off[k]) + int(lens[k])] for k in range(n)]
+    scoring = (2, -1, -10, -1)
+    gpu = ea.GpuScorer(ea.Panel(panel), ea.Scoring(*scoring), ea.Mode.LOCAL)
+    chosen, _ = gpu.autotune(reads, both_strands=False)
+    batch = gpu.batch(n, int(lens.sum()))
+    batch.load_concat(codes, off, lens, both_strands=False)
+    wall, dp = _timed_steps(batch, ea.RUN_SCORES32 | ea.RUN_NO_GRAPH, steps)
+    got, ok = batch.scores32()
+    cells, _, _ = batch.stats()
+    pwall, pdp = _timed_steps(batch, ea.RUN_SCORES32 | ea.RUN_PLAIN | ea.RUN_NO_GRAPH, 1)
+    gotp, _ = batch.scores32()
+    assert ok.all() and np.array_equal(gotp, got), "plain kernel != batched kernel"
+    out = {"workload": "164-ref panel (140-160 nt), 10k reads 60-150 nt (90% substrings with errors, "
+                       "10% random), local 2,-1,-10,-1, forward strand",
+           "reads": n, "cells": int(cells), "kernel_variant": chosen,
+           "reads_per_s": round(n / wall, 1), "ms_per_step": round(wall * 1e3, 3), "dp_ms": round(dp, 3),
+           "gcups": round(cells / (dp / 1e3) / 1e9, 1), "roofline": _roof(cells, dp, "local", chosen),
+           "plain_kernel": {"reads_per_s": round(n / pwall, 1), "dp_ms": round(pdp, 3),
+                            "gcups": round(cells / (pdp / 1e3) / 1e9, 1)}}
+    if cpu:
+        from oracle import pyoracle as O
+        sup = O.simd_supported()
+        lanes = 32 if sup & 2 else 16
+        threads = min(16, os.cpu_count() or 1)
+        prof = O.Profile(panel, scoring, lanes)
+        prof.score_batch(codes, off, lens, True, threads)  # warm
+        t = time.perf_counter()
+        want = prof.score_batch(codes, off, lens, True, threads)
+        dt = time.perf_counter() - t
+        assert np.array_equal(want, got), "device != CPU kernel on the plain CPU path's workload"
+        out["cpu_baseline"] = {"reads_per_s": round(n / dt, 1), "cells_per_s": round(cells / dt, 1),
+                               "cores": threads, "kind": "port",
+                               "sample": f"all {n} reads, {'AVX-512BW 32' if lanes == 32 else 'AVX2 16'}-lane "
+                                         f"restatement of the reference's CPU kernel, equal on every score"}
+    return out
+
+
+def config3_leg(n_reads, cpu, cpu_reads=2000):
+    """The large-panel workload: 10k references of 200-1000 nt (2k seeds
+    plus 4 copies of each at 2% substitutions, so ties and near-ties are
+    common), n_reads reads of 100-1000 nt, local 2,-1,-10,-1, forward
+    strand. The autotuner sweeps the batched kernel's variants and the
+    per-length-bucket routing to the anti-diagonal kernel on the first 1,000
+    reads; one timed pass of the whole batch with the fused selection. The
+    CPU baseline is the oracle's AVX-512 restatement of the reference's CPU
+    kernel on the first cpu_reads reads (a seeded subsample), equal to the
+    device's rows and tie sets."""
+    import eschalign_amd as ea
+    from eschalign_amd import synth
+
+    t0 = time.perf_counter()
+    panel = synth.panel_with_copies(seed=0xC0F3, n_seeds=2000, copies=4, lo=200, hi=1000)
+    codes, off, lens = synth.read_arrays(panel, seed=0xC0F3 + 1, n=n_reads, length=("uniform", 100, 1000))
+    gen_s = time.perf_counter() - t0
+    scoring = (2, -1, -10, -1)
+    gpu = ea.GpuScorer(ea.Panel(panel), ea.Scoring(*scoring), ea.Mode.LOCAL)
+    sample = [codes[int(off[k]):int(off[k]) + int(lens[k])] for k in range(min(n_reads, 1000))]
+    t0 = time.perf_counter()
+    chosen, tune_ms = gpu.autotune(sample, both_strands=False)
+    tune_s = time.perf_counter() - t0
+    batch = gpu.batch(n_reads, int(lens.sum()))
+    batch.load_concat(codes, off, lens, both_strands=False)
+    batch.wait()
+    tie_cap = 64
+    t = time.perf_counter()
+    batch.run(ea.RUN_SCORES32 | ea.RUN_SELECT | ea.RUN_NO_GRAPH, tie_cap=tie_cap)
+    batch.wait()
+    wall = time.perf_counter() - t
+    dp, _ = batch.timing()
+    cells, n16, n32 = batch.stats()
+    n_diag = batch.n_diag()
+    recs, ties = batch.select_raw()
+    tied = int((recs[:, 1] > 1).sum())
+    out = {"workload": f"10k refs 200-1000 nt (2k seeds + 4 copies at 2% substitutions), {n_reads} reads "
+                       f"100-1000 nt, local 2,-1,-10,-1, forward strand",
+           "reads": n_reads, "n_refs": len(panel), "panel_bases": int(sum(len(p) for p in panel)),
+           "cells": int(cells), "kernel_variant": chosen, "diag_ratio": gpu.diag_ratio,
+           "autotune_ms": [round(x, 3) for x in tune_ms], "autotune_s": round(tune_s, 2),
+           "reads_per_s": round(n_reads / wall, 1), "wall_ms": round(wall * 1e3, 1), "dp_ms": round(dp, 3),
+           "gcups": round(cells / (dp / 1e3) / 1e9, 1), "roofline": _roof(cells, dp, "local", chosen),
+           "queries_int16": int(n16), "queries_anti_diagonal": int(n_diag), "queries_int32": int(n32),
+           "reads_with_ties": tied, "generate_s": round(gen_s, 1)}
+    if cpu:
+        from oracle import pyoracle as O
+        sup = O.simd_supported()
+        lanes = 32 if sup & 2 else 16
+        threads = min(16, os.cpu_count() or 1)
+        prof = O.Profile(panel, scoring, lanes)
+        m = min(cpu_reads, n_reads)
+        t = time.perf_counter()
+        want = prof.score_batch(codes, off[:m], lens[:m], True, threads)
+        dt = time.perf_counter() - t
+        got, ok = batch.scores32()
+        assert ok[:m].all() and np.array_equal(got[:m], want), "device != CPU kernel on the large panel"
+        del got
+        for k in range(m):
+            b, nt, so, take = O.select(want[k], None, 0, None)
+            r = recs[k]
+            assert (r[0], r[1], r[4]) == (b, nt, len(take)), f"large panel: selection of read {k}"
+            assert list(ties[k, :min(len(take), tie_cap)]) == [(rr << 1) | int(rv) for rr, rv in take[:tie_cap]], k
+        cpu_cells = int(lens[:m].astype(np.int64).sum()) * out["panel_bases"]
+        out["cpu_baseline"] = {"reads_per_s": round(m / dt, 1), "cells_per_s": round(cpu_cells / dt, 1),
+                               "cores": threads, "kind": "port",
+                               "sample": f"the first {m} reads (seeded), "
+                                         f"{'AVX-512BW 32' if lanes == 32 else 'AVX2 16'}-lane restatement of the "
+                                         f"reference's CPU kernel (transposed at 256 nt and over), equal on "
+                                         f"every score and tie set"}
+    batch.close()
+    return out
+
+
 def parse_lengths(text):
     """'lognormal:MEDIAN:SIGMA:LO:HI' or 'uniform:LO:HI' -> synth length spec."""
     kind, *vals = text.split(":")
@@ -411,6 +557,7 @@ def serving(rate, duration, seed=0x5E4E):
         reqs.append((pid, synth.reads(panels[pid], seed=seed + 100 + r, n=n)))
     done = [None] * n_req
     lag = [0.0] * n_req
+    kept = {}
     t0 = time.perf_counter()
 
     def feeder():
@@ -428,19 +575,45 @@ def serving(rate, duration, seed=0x5E4E):
         while done[r] is None:
             time.sleep(0.0005)
         server.wait(done[r])
-        _, l = server.result(done[r])
+        res, l = server.result(done[r])
         lat.append(l + lag[r])
+        if len(reqs[r][1]) <= 2_000:
+            kept[r] = res
     th.join()
     wall = time.perf_counter() - t0
     st = server.stats()
     server.close()
+    # a served result equals the request run alone: the two smallest
+    # requests per panel, again through a scorer of their own
+    checked = 0
+    for k, p in enumerate(panels):
+        mine = sorted((len(reqs[r][1]), r) for r in kept if reqs[r][0] == k)[:2]
+        if not mine:
+            continue
+        alone = ea.GpuScorer(ea.Panel(p), scoring, ea.Mode.SEMIGLOBAL)
+        for _, r in mine:
+            want = alone.select_batch(reqs[r][1], both_strands=True, tie_cap=8)
+            assert want == kept[r], f"served request {r} != the same request alone"
+            checked += 1
+        del alone
     total = sum(len(x[1]) for x in reqs)
-    return {"requests": n_req, "rate_per_s": rate, "reads": total,
+    lat = np.array(lat)
+
+    def pct(v):
+        return {"requests": int(len(v)),
+                "p50_ms": round(float(np.percentile(v, 50)) * 1e3, 2),
+                "p90_ms": round(float(np.percentile(v, 90)) * 1e3, 2),
+                "p99_ms": round(float(np.percentile(v, 99)) * 1e3, 2),
+                "mean_ms": round(float(np.mean(v)) * 1e3, 2)} if len(v) else {"requests": 0}
+
+    pids = np.array([x[0] for x in reqs])
+    per_panel = {f"{len(p)}_refs": pct(lat[pids == k]) for k, p in enumerate(panels)}
+    for k, p in enumerate(panels):
+        per_panel[f"{len(p)}_refs"]["reads"] = int(sum(len(x[1]) for x in reqs if x[0] == k))
+    return {"requests": n_req, "rate_per_s": rate, "window_s": duration, "reads": total,
             "reads_per_s": round(total / wall, 1), "wall_s": round(wall, 2),
-            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
-            "p90_ms": round(float(np.percentile(lat, 90)) * 1e3, 2),
-            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
-            "mean_ms": round(float(np.mean(lat)) * 1e3, 2),
+            **{k: v for k, v in pct(lat).items() if k != "requests"},
+            "per_panel": per_panel, "checked_alone": checked,
             "batches": int(st["batches"]), "hbm_peak_gib": round(st["hbm_peak"] / 2**30, 2),
             "panels": "164 / 1,000 / 10,000 refs of 140-1000 nt"}
 
@@ -458,6 +631,11 @@ def main():
     ap.add_argument("--no-trace", action="store_true", help="skip the winner-traceback leg")
     ap.add_argument("--no-pairs", action="store_true", help="skip the paired-mapping leg")
     ap.add_argument("--no-map", action="store_true", help="skip the read-mapping leg")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the plain-CPU-path (10k reads) and large-panel (10k refs) legs")
+    ap.add_argument("--config3-reads", type=int, default=100_000, help="reads of the large-panel leg")
+    ap.add_argument("--only", choices=["config1", "config3", "serving"], default=None,
+                    help="only run that leg and print its record (profiling)")
     ap.add_argument("--map-reads", type=int, default=1_000_000, help="reads of the read-mapping leg")
     ap.add_argument("--pairs", type=int, default=2_000_000, help="pairs of the paired-mapping leg")
     ap.add_argument("--pairs-only", action="store_true",
@@ -473,8 +651,8 @@ def main():
                          "(0 = off; default: the library's, or the autotuner's)")
     ap.add_argument("--variant", type=int, default=-1,
                     help="batched-kernel variant to run (default: autotune on a sample)")
-    ap.add_argument("--serve-rate", type=float, default=2.0, help="serving arrivals per second")
-    ap.add_argument("--serve-seconds", type=float, default=12.0, help="serving arrival window")
+    ap.add_argument("--serve-rate", type=float, default=3.0, help="serving arrivals per second")
+    ap.add_argument("--serve-seconds", type=float, default=70.0, help="serving arrival window")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (scales rocprof's "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -487,6 +665,15 @@ def main():
               f"collective); --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
 
+    if args.only == "config1":
+        print(json.dumps(config1_leg(args.steps, not args.no_cpu)), flush=True)
+        return
+    if args.only == "config3":
+        print(json.dumps(config3_leg(args.config3_reads, not args.no_cpu)), flush=True)
+        return
+    if args.only == "serving":
+        print(json.dumps(serving(args.serve_rate, args.serve_seconds)), flush=True)
+        return
     if args.pairs_only:
         print(json.dumps(pairs_leg(args.pairs, not args.no_cpu, args.pairs_both)), flush=True)
         return
@@ -522,23 +709,36 @@ def main():
     batch = gpu.batch(args.reads, int(lens.sum()))
     batch.load_concat(codes, off, lens, both_strands=True)
     batch.wait()
-    what = ea.RUN_SCORES32 | ea.RUN_SELECT | ea.RUN_NO_GRAPH
+    eager = ea.RUN_SCORES32 | ea.RUN_SELECT | ea.RUN_NO_GRAPH
+    graph = ea.RUN_SCORES32 | ea.RUN_SELECT
     tie_cap = 8
 
     for _ in range(args.warmup):
-        batch.run(what, tie_cap=tie_cap)
-        batch.wait()
-    batch.wait()
-    dp_ms, total_ms = [], []
+        for what in (eager, graph):  # the first graph run captures the graph
+            batch.run(what, tie_cap=tie_cap)
+            batch.wait()
+    # 1. eager launches: HIP events on the batch's stream around the DP
+    # kernels give the kernel interval the roofline is priced on
+    dp_ms = []
     t = time.perf_counter()
     for _ in range(args.steps):
-        batch.run(what, tie_cap=tie_cap)
+        batch.run(eager, tie_cap=tie_cap)
         batch.wait()  # every step's events are read before the next is queued
-        d, tt = batch.timing()
-        dp_ms.append(d)
-        total_ms.append(tt)
-    batch.wait()
+        dp_ms.append(batch.timing()[0])
+    eager_wall = time.perf_counter() - t
+    eager_s32, _ = batch.scores32()
+    # 2. the captured graph replayed: the headline (the library's default
+    # run mode); its output is what the parity checks below read
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run(graph, tie_cap=tie_cap)
+        batch.wait()
     wall = time.perf_counter() - t
+    timed_s32, timed_ok = batch.scores32()
+    timed_recs, timed_ties = batch.select_raw()
+    assert timed_ok.all(), "a row of the timed batch was left unscored"
+    assert np.array_equal(timed_s32, eager_s32), "graph replay != eager launches"
+    del eager_s32
     cells, n16, n32 = batch.stats()
     n_diag = batch.n_diag()
     # the boundary hands over host buffers: time loading the same batch again
@@ -547,33 +747,47 @@ def main():
     batch.load_concat(codes, off, lens, both_strands=True)
     batch.wait()
     load_s = time.perf_counter() - t
+    # 3. whole steps from host buffers to host records: load (H2D), the
+    # graph, the selection records and tie lists back (D2H)
+    n_xfer = max(1, min(args.steps, 5))
+    t = time.perf_counter()
+    for _ in range(n_xfer):
+        batch.load_concat(codes, off, lens, both_strands=True)
+        batch.run(graph, tie_cap=tie_cap)
+        xr, xt = batch.select_raw()
+    xfer = (time.perf_counter() - t) / n_xfer
+    assert np.array_equal(xr, timed_recs) and np.array_equal(xt, timed_ties)
     ms_step = wall / args.steps * 1e3
     value = args.reads / (wall / args.steps)
     dp = float(np.mean(dp_ms)) / 1e3
     gcups = cells / dp / 1e9
     f16 = chosen in F16_VARIANTS
-    ops = OPS_PER_CELL[(args.mode, f16, chosen in SHIFTED_VARIANTS)]
+    shifted = chosen in SHIFTED_VARIANTS
+    ops = OPS_PER_CELL[(args.mode, f16, shifted)]
     achieved = cells * ops / dp
-    log(f"[bench] {value:,.0f} reads/s, {ms_step:.1f} ms/step, DP {dp * 1e3:.1f} ms "
-        f"({gcups:,.0f} GCUPS), {n16} int16 / {n_diag} anti-diagonal / {n32} int32 queries")
+    log(f"[bench] {value:,.0f} reads/s, {ms_step:.1f} ms/step (graph; eager "
+        f"{eager_wall / args.steps * 1e3:.1f}), DP {dp * 1e3:.1f} ms ({gcups:,.0f} GCUPS), {n16} int16 / "
+        f"{n_diag} anti-diagonal / {n32} int32 queries; from host buffers {xfer * 1e3:.1f} ms/step")
 
     # With an fp16 variant chosen, the same steps again with its int16
     # counterpart (variant 6 for the row-shifted 7, else 3), so both
     # arithmetics' rates are on record.
     int16 = None
-    shifted = chosen in SHIFTED_VARIANTS
     v16 = 6 if shifted else 3
     if f16:
         gpu.variant = v16
-        batch.run(what, tie_cap=tie_cap)
+        batch.run(eager, tie_cap=tie_cap)
         batch.wait()
         d16 = []
         t = time.perf_counter()
         for _ in range(args.steps):
-            batch.run(what, tie_cap=tie_cap)
+            batch.run(eager, tie_cap=tie_cap)
             batch.wait()
             d16.append(batch.timing()[0])
         w16 = time.perf_counter() - t
+        got16, _ = batch.scores32()
+        assert np.array_equal(got16, timed_s32), f"int16 variant {v16} != variant {chosen}"
+        del got16
         gpu.variant = chosen
         dp16 = float(np.mean(d16)) / 1e3
         ops16 = OPS_PER_CELL[(args.mode, False, shifted)]
@@ -585,40 +799,71 @@ def main():
 
     plain = None
     if not args.no_plain:
-        batch.run(ea.RUN_SCORES32 | ea.RUN_SELECT | ea.RUN_PLAIN | ea.RUN_NO_GRAPH, tie_cap=tie_cap)
+        pl = ea.RUN_SCORES32 | ea.RUN_SELECT | ea.RUN_PLAIN | ea.RUN_NO_GRAPH
+        batch.run(pl, tie_cap=tie_cap)
         batch.wait()
         t = time.perf_counter()
-        batch.run(ea.RUN_SCORES32 | ea.RUN_SELECT | ea.RUN_PLAIN | ea.RUN_NO_GRAPH, tie_cap=tie_cap)
+        batch.run(pl, tie_cap=tie_cap)
         batch.wait()
         pw = time.perf_counter() - t
         pdp, _ = batch.timing()
-        plain = {"reads_per_s": round(args.reads / pw, 1),
+        gotp, _ = batch.scores32()
+        assert np.array_equal(gotp, timed_s32), "plain kernel != batched kernel"
+        del gotp
+        plain_rate = args.reads / pw
+        plain = {"reads_per_s": round(plain_rate, 1),
                  "dp_ms": round(pdp, 3),
                  "gcups": round(cells / (pdp / 1e3) / 1e9, 1),
-                 "speedup": round(value / (args.reads / pw), 2)}
+                 "speedup": round(value / plain_rate, 2),
+                 "speedup_int16": round(int16["reads_per_s"] / plain_rate, 2) if int16 else None}
         log(f"[bench] plain int32 kernel: {plain}")
 
     cpu = None
     if not args.no_cpu:
+        n_all = args.reads
+
         def gpu_check(qs, cpu_scores):
-            got, ok = gpu.score_batch_i32(qs)
-            assert ok.all() and np.array_equal(got, cpu_scores), "GPU != CPU kernel on the sample"
+            # the timed batch's own rows: reads 0..n-1, then their reverse
+            # complements at n_all + 0..n-1 (the CPU sample's order)
+            n = len(qs) // 2
+            got = np.concatenate([timed_s32[:n], timed_s32[n_all:n_all + n]])
+            assert np.array_equal(got, cpu_scores), "timed GPU batch != CPU kernel on the sample"
+            # and the fused selection of the timed batch, on the first 20k
+            for k in range(min(n, 20_000)):
+                b, nt, so, take = O.select(cpu_scores[k], cpu_scores[n + k], 0, None)
+                r = timed_recs[k]
+                want = [(rr << 1) | int(rv) for rr, rv in take[:tie_cap]]
+                assert (r[0], r[1], r[4]) == (b, nt, len(take)), f"timed selection != CPU on read {k}"
+                assert (so is None) == (not (r[3] & 2)) and (so is None or r[2] == so), k
+                assert list(timed_ties[k, :min(len(take), tie_cap)]) == want, k
+        from oracle import pyoracle as O
         cpu = cpu_baseline(panel_codes, codes, off, lens, scoring, local, gpu_check)
         log(f"[bench] CPU baseline: {cpu}")
+    del timed_s32
 
     trace = None
     if not args.no_trace:
-        batch.run(what, tie_cap=tie_cap)  # the selection the winners come from
+        batch.run(graph, tie_cap=tie_cap)  # the selection the winners come from
         batch.wait()
         trace = trace_leg(gpu, batch, panel_codes, codes, off, lens, scoring, local, args.trace_reads,
                           not args.no_cpu)
         log(f"[bench] winner tracebacks: {trace}")
+    batch.close()
 
     mapped = None
     if not args.no_map:
         mapped = map_reads_leg(gpu, panel_codes, codes, off, lens, scoring, local,
                                min(args.map_reads, args.reads), not args.no_cpu)
         log(f"[bench] read mapping: {mapped}")
+    gpu_ratio = gpu.diag_ratio
+    del gpu
+
+    configs = {}
+    if not args.no_configs:
+        configs["plain_cpu_path_10k"] = config1_leg(args.steps, not args.no_cpu)
+        log(f"[bench] config plain CPU path: {configs['plain_cpu_path_10k']}")
+        configs["large_panel_10k_refs"] = config3_leg(args.config3_reads, not args.no_cpu)
+        log(f"[bench] config large panel: {configs['large_panel_10k_refs']}")
 
     pairs = None
     if not args.no_pairs:
@@ -634,6 +879,23 @@ def main():
     peak = ea.valu_peak16(0)
     log(f"[bench] measured int16 VALU peak {peak['lane_ops_per_s'] / 1e12:.2f} T lane-op/s "
         f"({peak['n_cu']} CUs, reported clock {peak['clock_khz'] / 1e3:.0f} MHz)")
+    roof = {"bound": "valu", "achieved": round(achieved / 1e12, 3),
+            "peak": round(PEAK_I16 / 1e12, 3), "unit": "Tlane-op/s (packed 16-bit)",
+            "frac": round(achieved / PEAK_I16, 4), "traffic": traffic,
+            "peak_measured": round(peak["lane_ops_per_s"] / 1e12, 3),
+            "frac_of_measured": round(achieved / peak["lane_ops_per_s"], 4),
+            "ops_per_cell": ops,
+            "ops_basis": "the minimum packed lane-ops per cell of the formulation the kernel runs "
+                         "(6: fp16 with the row shift; 7: int16 shifted or fp16 unshifted; 8: int16)",
+            "valu_lane_ops_per_cell_issued": VALU_OPS_PER_CELL_ISSUED.get((args.mode, chosen)),
+            "traffic_unit": "B per k_dp16 launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE)",
+            "traffic_source": traffic_src}
+    fracs = [roof["frac"], roof["frac_of_measured"]]
+    if int16:
+        fracs.append(int16["roofline_frac"])
+    for c in configs.values():
+        fracs.append(c["roofline"]["frac"])
+    assert all(0 < f <= 1 for f in fracs), f"a roofline fraction outside (0, 1]: {fracs}"
     line = {
         "metric": "reads/s + GCUPS, bit-exact scores vs CPU; % VALU roofline; p50/p99 latency",
         "value": round(value, 1),
@@ -654,28 +916,26 @@ def main():
                    "queries_int16": int(n16), "queries_anti_diagonal": int(n_diag),
                    "queries_int32": int(n32),
                    "parallelism": "single GPU"},
+        "value_scope": "device pipeline (bucketing, DP, fused selection) replayed from its captured "
+                       "hipGraph, reads resident in HBM; host transfers in transfer_inclusive",
+        "eager_ms_per_step": round(eager_wall / args.steps * 1e3, 3),
+        "graph_ms_per_step": round(ms_step, 3),
         "gcups": round(gcups, 1),
         "dp_ms": round(dp * 1e3, 3),
         "cells": int(cells),
-        "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3),
-                     "peak": round(PEAK_I16 / 1e12, 3), "unit": "Tlane-op/s (packed 16-bit)",
-                     "frac": round(achieved / PEAK_I16, 4), "traffic": traffic,
-                     "peak_measured": round(peak["lane_ops_per_s"] / 1e12, 3),
-                     "frac_of_measured": round(achieved / peak["lane_ops_per_s"], 4),
-                     "valu_lane_ops_per_cell_issued": VALU_OPS_PER_CELL_ISSUED.get((args.mode, chosen)),
-                     "traffic_unit": "B per k_dp16 launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE)",
-                     "traffic_source": traffic_src,
-                     "ops_per_cell": ops,
-                     # cells/s against the int16 kernel's bound (8 / 10 ops per cell)
-                     "frac_of_int16_roofline": round(cells * OPS_PER_CELL[(args.mode, False, shifted)] / dp / PEAK_I16, 4)},
-        "host_load": {"ms": round(load_s * 1e3, 3), "bytes": int(lens.sum()),
-                      "pcie_inclusive_reads_per_s": round(args.reads / (load_s + wall / args.steps), 1)},
+        "roofline": roof,
+        "transfer_inclusive": {"reads_per_s": round(args.reads / xfer, 1), "ms_per_step": round(xfer * 1e3, 3),
+                               "h2d_bytes": int(lens.sum()) + 12 * args.reads,
+                               "d2h_bytes": int(xr.nbytes + xt.nbytes), "load_ms": round(load_s * 1e3, 3),
+                               "scope": "reads from host buffers (gather into pinned staging, copy), the "
+                                        "graph, selection records and tie lists copied back"},
         "kernel_variant": chosen,
         "int16_variant": int16,
-        "diag_ratio": gpu.diag_ratio,
+        "diag_ratio": gpu_ratio,
         "autotune_ms": [round(x, 3) for x in tune_ms],
         "plain_kernel": plain,
         "cpu_baseline": cpu,
+        "configs": configs,
         "trace": trace,
         "map_reads": mapped,
         "pairs": pairs,
