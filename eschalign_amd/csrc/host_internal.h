@@ -61,7 +61,13 @@ struct esch_scorer {
     int regs16 = 0, local16 = 0;
     bool f16_ok = false;             // the panel fits k_dp16's fp16 variant (fits_f16)
     bool usable16[kDp16Variants] = {};  // variants exact on this panel (fp16 and shift bounds)
-    bool variant_fixed = false;         // set by esch_set_variant (the caller or the autotuner)
+    // geo (its variant and grid), regs16 / local16, variant_fixed and the
+    // routing ratio below are written by esch_set_variant, esch_autotune and
+    // esch_set_diag_ratio, possibly while another thread runs batches of this
+    // scorer (a server's scheduler): both sides take geo_mu, and a run works
+    // from a snapshot taken under it.
+    mutable std::mutex geo_mu;
+    bool variant_fixed = false;         // set by the caller (esch_set_variant): no per-batch upgrade
     int diag_min_len = 0;               // reads this long take k_diag16 (0: never)
     int diag_ratio = kDiagRatioDefault; // padding-heavy length bins take k_diag16 (0: never)
     bool diag_ratio_tunable = true;     // false once the caller set the ratio

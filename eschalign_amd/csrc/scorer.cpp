@@ -330,11 +330,22 @@ void esch_geometry(const esch_scorer *s, size_t *lanes, size_t *groups, size_t *
     if (!s) return;
     if (lanes) *lanes = 256;
     if (groups) *groups = 4;
-    if (blocks) *blocks = (size_t)s->geo.grid16;
+    if (blocks) {
+        std::lock_guard<std::mutex> g(s->geo_mu);
+        *blocks = (size_t)s->geo.grid16;
+    }
 }
 
-int esch_registers(const esch_scorer *s) { return s ? s->regs16 : 0; }
-int esch_local_bytes(const esch_scorer *s) { return s ? s->local16 : 0; }
+int esch_registers(const esch_scorer *s) {
+    if (!s) return 0;
+    std::lock_guard<std::mutex> g(s->geo_mu);
+    return s->regs16;
+}
+int esch_local_bytes(const esch_scorer *s) {
+    if (!s) return 0;
+    std::lock_guard<std::mutex> g(s->geo_mu);
+    return s->local16;
+}
 
 // ------------------------------------------------------------------ batches
 
@@ -626,18 +637,17 @@ esch_status enqueue(esch_batch *b, int what, bool timed, const Launch &g) {
     return ESCH_OK;
 }
 
-// The k_dp16 variant a batch runs: the scorer's, unless the caller or the
-// autotuner fixed none and the panel admits no fp16 variant while this
-// batch's reads do. Every value of the fp16 and row-shifted recurrences is
+// The k_dp16 variant a batch runs: the scorer's, unless the caller fixed
+// none (the default or the autotuner's pick) and the panel admits no fp16
+// variant while this batch's reads do. Every value of the fp16 and row-shifted recurrences is
 // bounded through k = min(read, reference), so the batch's longest read
 // caps k as well as the panel's longest reference does: short reads on a
 // panel of long references still get fp16 (semiglobal: the row-shifted 7,
 // 6 ops per cell, local: 4). The column shift's range grows with the
 // reference itself, so local mode does not take it from the reads.
-int batch_variant(const esch_batch *b) {
+int batch_variant(const esch_batch *b, int v, bool fixed) {
     const esch_scorer *s = b->s;
-    const int v = s->geo.variant16;
-    if (s->variant_fixed || esch::dp16_is_f16(v)) return v;
+    if (fixed || esch::dp16_is_f16(v)) return v;
     const int64_t k = (int64_t)std::min<size_t>(s->max_len, b->longest);
     if (!fits_f16(k, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e)) return v;
     if (s->sc.local) return kDp16F16;
@@ -705,7 +715,7 @@ esch_status swap_ensure(esch_batch *b) {
 // the substitution, the gaps, the zero borders and the best cell (anywhere
 // in local mode; on the last row or column in semiglobal mode) are the same
 // with the two sequences exchanged.
-esch_status run_swapped(esch_batch *b, int what) {
+esch_status run_swapped(esch_batch *b, int what, const Launch &geo) {
     esch_scorer *s = b->s;
     if (esch_status st = swap_ensure(b)) return st;
     SwapBufs &w = *b->swap;
@@ -750,7 +760,7 @@ esch_status run_swapped(esch_batch *b, int what) {
     desc->tie_cap = 1;
     desc->max_ties = -1;
     desc->tstrands = 1;
-    desc->blocks16 = (int32_t)s->geo.grid16;  // routing off: no anti-diagonal rows
+    desc->blocks16 = (int32_t)geo.grid16;  // routing off: no anti-diagonal rows
     hipStream_t st = b->stream;
     uint8_t *d = w.d_block;
     HIPC(hipMemcpyAsync(d, h, rows * 4, hipMemcpyHostToDevice, st));
@@ -781,13 +791,13 @@ esch_status run_swapped(esch_batch *b, int what) {
     a.out16 = nullptr;
     a.swap = 1;
     const bool local = s->sc.local != 0;
-    HIPC(launch_prep(a, s->geo, st));
+    HIPC(launch_prep(a, geo, st));
     HIPC(hipEventRecord(b->ev1, st));
-    HIPC(launch_dp16(a, s->geo, local, st));
-    HIPC(launch_dp32(a, s->geo, local, 1, st));
+    HIPC(launch_dp16(a, geo, local, st));
+    HIPC(launch_dp32(a, geo, local, 1, st));
     HIPC(hipEventRecord(b->ev2, st));
     HIPC(hipMemsetAsync(b->d_cls, 1, rows, st));  // every row scored (a run outside any graph)
-    if (what & ESCH_RUN_SELECT) HIPC(launch_select(a0, s->geo, st));
+    if (what & ESCH_RUN_SELECT) HIPC(launch_select(a0, geo, st));
     return ESCH_OK;
 }
 
@@ -845,9 +855,18 @@ esch_status esch_batch_run(esch_batch *b, int what, int32_t min_score, int64_t m
     d.max_ties = max_ties;
     d.tstrands = (what & ESCH_RUN_PAIR_BOTH) ? 2 : 1;
     d.diag_min_len = (what & (ESCH_RUN_PLAIN | ESCH_RUN_SCORES16)) ? 0 : s->diag_min_len;
-    d.diag_ratio = (what & ESCH_RUN_PLAIN) ? 0 : s->diag_ratio;
-    const int v16 = batch_variant(b);
-    Launch geo = s->geo;
+    Launch geo;
+    int ratio;
+    bool fixed;
+    {
+        std::lock_guard<std::mutex> g(s->geo_mu);
+        geo = s->geo;
+        ratio = s->diag_ratio;
+        fixed = s->variant_fixed;
+    }
+    d.diag_ratio = (what & ESCH_RUN_PLAIN) ? 0 : ratio;
+    const Launch geo_scorer = geo;  // the swapped path runs the scorer's own variant
+    const int v16 = batch_variant(b, geo.variant16, fixed);
     geo.variant16 = v16;
     geo.grid16 = s->grid16_v[v16];
     d.blocks16 = (int32_t)geo.grid16;
@@ -862,7 +881,7 @@ esch_status esch_batch_run(esch_batch *b, int what, int32_t min_score, int64_t m
     b->last_variant = b->last_swapped ? -1 : v16;
     if (b->last_swapped) {
         b->timed_dp = true;
-        if (esch_status st = run_swapped(b, what)) return st;
+        if (esch_status st = run_swapped(b, what, geo_scorer)) return st;
     } else if (!graph) {
         b->timed_dp = true;
         if (esch_status st = enqueue(b, key, true, geo)) return st;
@@ -1120,7 +1139,14 @@ esch_status esch_score_batch(esch_scorer *s, const uint8_t *const *queries, cons
     return esch_finish(s, pend, scores, scored);
 }
 
-esch_status esch_set_variant(esch_scorer *s, int variant) {
+}  // extern "C"
+
+namespace {
+
+// Select k_dp16 variant `variant` for the scorer's runs. fixed: the caller
+// chose it (no per-batch fp16 upgrade, batch_variant); otherwise it is the
+// autotuner's pick, which a batch of short enough reads still upgrades.
+esch_status set_variant(esch_scorer *s, int variant, bool fixed) {
     if (!s || variant < 0 || variant >= kDp16Variants)
         return fail(ESCH_INVALID_ARG, "variant %d outside 0..%d", variant, kDp16Variants - 1);
     if (esch::dp16_is_f16(variant) && !s->f16_ok)
@@ -1136,15 +1162,31 @@ esch_status esch_set_variant(esch_scorer *s, int variant) {
                                 : "variant %d (row-shifted) needs longest reference x max(|match|, |mismatch|) + "
                                   "|open| + 17 |extend| <= %lld",
                     variant, (long long)(esch::dp16_is_f16(variant) ? esch::kBoundF16 : esch::kBound16));
+    std::lock_guard<std::mutex> g(s->geo_mu);
     s->geo.variant16 = variant;
-    s->variant_fixed = true;
+    s->variant_fixed = fixed;
     s->geo.grid16 = s->grid16_v[variant];
     s->regs16 = s->regs16_v[variant];
     s->local16 = s->local16_v[variant];
     return ESCH_OK;
 }
 
-int esch_get_variant(const esch_scorer *s) { return s ? s->geo.variant16 : -1; }
+void set_ratio(esch_scorer *s, int32_t ratio_x16) {
+    std::lock_guard<std::mutex> g(s->geo_mu);
+    s->diag_ratio = ratio_x16;
+}
+
+}  // namespace
+
+extern "C" {
+
+esch_status esch_set_variant(esch_scorer *s, int variant) { return set_variant(s, variant, true); }
+
+int esch_get_variant(const esch_scorer *s) {
+    if (!s) return -1;
+    std::lock_guard<std::mutex> g(s->geo_mu);
+    return s->geo.variant16;
+}
 
 esch_status esch_set_swap_rows(esch_scorer *s, int32_t rows) {
     if (!s || rows < -1) return fail(ESCH_INVALID_ARG, "swap rows must be -1 (the rule), 0 (never) or a row count");
@@ -1170,12 +1212,17 @@ esch_status esch_set_diag_ratio(esch_scorer *s, int32_t ratio_x16) {
     if (!s) return fail(ESCH_INVALID_ARG, "null scorer");
     if (ratio_x16 != 0 && (ratio_x16 < 16 || ratio_x16 > 1024))
         return fail(ESCH_INVALID_ARG, "diag ratio must be 0 or 16..1024 (sixteenths)");
+    std::lock_guard<std::mutex> g(s->geo_mu);
     s->diag_ratio = ratio_x16;
     s->diag_ratio_tunable = false;
     return ESCH_OK;
 }
 
-int32_t esch_get_diag_ratio(const esch_scorer *s) { return s ? s->diag_ratio : -1; }
+int32_t esch_get_diag_ratio(const esch_scorer *s) {
+    if (!s) return -1;
+    std::lock_guard<std::mutex> g(s->geo_mu);
+    return s->diag_ratio;
+}
 int esch_n_variants(void) { return kDp16Variants; }
 
 esch_status esch_autotune(esch_scorer *s, const uint8_t *const *reads, const size_t *lens, size_t n,
@@ -1195,14 +1242,14 @@ esch_status esch_autotune(esch_scorer *s, const uint8_t *const *reads, const siz
         return esch_batch_timing(b, out, nullptr);
     };
     // 1. the inter-sequence kernel's variants, routing as configured
-    int best_v = s->geo.variant16;
+    int best_v = esch_get_variant(s);
     float best_ms = 0.0f;
     for (int v = 0; !st && v < kDp16Variants; ++v) {
         if (!s->usable16[v]) {
             if (ms) ms[v] = -1.0f;  // not applicable to this panel
             continue;
         }
-        st = esch_set_variant(s, v);
+        st = set_variant(s, v, true);  // exactly this variant while it is timed
         float dp = 0.0f;
         if (!st) st = timed(&dp);
         if (st) break;
@@ -1212,7 +1259,9 @@ esch_status esch_autotune(esch_scorer *s, const uint8_t *const *reads, const siz
             best_v = v;
         }
     }
-    if (!st) st = esch_set_variant(s, best_v);
+    // the pick is the autotuner's, not the caller's: a batch whose reads
+    // admit an fp16 variant the panel does not still takes it
+    if (!st) st = set_variant(s, best_v, false);
     // 2. the per-length-bucket routing between the inter-sequence and the
     // anti-diagonal kernel: the ratio rho is the anti-diagonal kernel's
     // per-cell cost relative to the inter-sequence one, a property of the
@@ -1220,17 +1269,22 @@ esch_status esch_autotune(esch_scorer *s, const uint8_t *const *reads, const siz
     // sample measures it. Another ratio replaces the default only when it
     // is faster by more than 2% (a sample without long tails times the same
     // under every ratio and keeps the default).
-    if (!st && s->diag_ratio_tunable) {
+    bool tunable;
+    {
+        std::lock_guard<std::mutex> g(s->geo_mu);
+        tunable = s->diag_ratio_tunable;
+    }
+    if (!st && tunable) {
         static const int32_t kRatios[] = {0, 24, 32, 48, 64};
         const int32_t base = kDiagRatioDefault;
         float base_ms = 0.0f;
-        s->diag_ratio = base;
+        set_ratio(s, base);
         st = timed(&base_ms);
         int32_t best_r = base;
         float best_r_ms = base_ms;
         for (int32_t r : kRatios) {
             if (st || r == base) continue;
-            s->diag_ratio = r;
+            set_ratio(s, r);
             float dp = 0.0f;
             st = timed(&dp);
             if (!st && dp < best_r_ms && dp < 0.98f * base_ms) {
@@ -1238,7 +1292,7 @@ esch_status esch_autotune(esch_scorer *s, const uint8_t *const *reads, const siz
                 best_r = r;
             }
         }
-        s->diag_ratio = best_r;
+        set_ratio(s, best_r);
     }
     give_batch(s, b);
     if (st) return st;

@@ -819,7 +819,7 @@ def test_short_reads_take_fp16_on_a_panel_of_long_references():
     variant 7 in semiglobal mode and 4 in local mode; a batch holding a
     2,100-nt read (the top score, 2,100, is beyond fp16's exact integers)
     stays on the scorer's int16 variant 6, and so does every batch once the
-    caller fixes a variant. Scores equal the oracle's in every case (the
+    caller fixes a variant (an autotuned scorer still upgrades). Scores equal the oracle's in every case (the
     captured graphs are keyed by the variant they launch)."""
     rng = G.Rng(0x5EED_0A06)
     scoring = (1, -1, -2, -1)
@@ -842,6 +842,17 @@ def test_short_reads_take_fp16_on_a_panel_of_long_references():
             assert b.variant() == want_v, (mode, len(reads), b.variant())
             want = oracle(q, panel.codes, scoring, mode)
             assert ok.all() and np.array_equal(s32, want), (mode, want_v)
+        # the autotuner's pick (among the int16 variants this panel admits)
+        # is not the caller's: a batch of short reads still takes fp16
+        chosen, _ = gpu.autotune(codes_of(short), both_strands=False)
+        assert chosen in (0, 1, 2, 3, 6)
+        q = codes_of(short)
+        b = gpu.batch(len(q), sum(len(x) for x in q))
+        b.load(q)
+        b.run(ea.RUN_SCORES32)
+        b.wait()
+        assert b.variant() == fast, (mode, chosen, b.variant())
+        assert np.array_equal(b.scores32()[0], oracle(q, panel.codes, scoring, mode))
         gpu.variant = 6
         q = codes_of(short)
         b = gpu.batch(len(q), sum(len(x) for x in q))
