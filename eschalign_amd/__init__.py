@@ -33,6 +33,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 lib_path = os.path.join(_HERE, "libeschalign_amd.so")
 
 MAX_READ_LEN = 4096
+NOALLOC_MAX_ROWS = 1 << 20
+NOALLOC_SCRATCH_BYTES = 256 << 20
 MAX_REF_LEN = 16384
 
 # esch_status
@@ -69,7 +71,7 @@ EXPORTS = [
     "esch_map_pairs", "esch_map_pairs_concat", "esch_mapped_len", "esch_mapped_reads",
     "esch_mapped_pairs", "esch_mapped_hits", "esch_mapped_ops", "esch_mapped_stats", "esch_mapped_free",
     "esch_set_map_chunk", "esch_adapter_trim_pairs", "esch_md_nm", "esch_cigar_string",
-    "esch_map_reads_concat", "esch_align_last_wavefront",
+    "esch_map_reads_concat", "esch_align_last_wavefront", "esch_score_device_noalloc",
 ]
 
 
@@ -156,6 +158,7 @@ def load_library():
     L.esch_registers.argtypes = [vp]
     L.esch_local_bytes.argtypes = [vp]
     L.esch_score_batch_i32.argtypes = [vp, vp, vp, sz, vp, vp]
+    L.esch_score_device_noalloc.argtypes = [vp, vp, vp, vp, sz, vp, vp, vp]
     L.esch_select_batch.argtypes = [vp, vp, vp, sz, C.c_int, i32, i64, sz, vp, vp]
     L.esch_select_pairs.argtypes = [vp, vp, vp, vp, vp, sz, C.c_int, i32, sz, vp, vp]
     L.esch_batch_create.argtypes = [vp, sz, sz, C.POINTER(vp)]
@@ -774,6 +777,14 @@ class GpuScorer:
         _check(_lib.esch_score_batch_i32(self._h, ptrs, lens, n, scores.ctypes.data,
                                          scored.ctypes.data))
         return scores[: n * self.n_refs].reshape(n, self.n_refs), scored[:n].astype(bool)
+
+    def score_device_noalloc(self, codes: int, offs: int, lens: int, n: int, scores: int, scored: int,
+                             stream: int = 0):
+        """esch_score_device_noalloc: the scoring of score_batch_i32 on
+        queries already in device memory (addresses as ints: codes u8,
+        offs u64, lens i32; out scores i32[n * n_refs], scored u8[n]),
+        queued on `stream` without any device allocation."""
+        _check(_lib.esch_score_device_noalloc(self._h, codes, offs, lens, n, scores, scored, stream or None))
 
     def select_batch(self, reads: Sequence[np.ndarray], both_strands=False, min_score=0,
                      max_ties=None, tie_cap=None) -> List[ReadSelect]:

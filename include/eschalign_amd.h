@@ -142,6 +142,31 @@ int esch_local_bytes(const esch_scorer *s); /* private (spill) bytes of the batc
 esch_status esch_score_batch_i32(esch_scorer *s, const uint8_t *const *queries,
                                  const size_t *lens, size_t n, int32_t *scores, uint8_t *scored);
 
+/* Allocation-free scoring. No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). The scoring of
+ * esch_score_batch_i32 (every non-empty query scored, the int32 rescue
+ * outside the int16 bound; an empty query gives a zero row, scored = 0)
+ * on queries already in DEVICE memory: query k is codes[offs[k] ..
+ * offs[k] + lens[k]); scores[n * n_refs] (int32) and scored[n] are device
+ * pointers too. The work is queued on `stream` (a hipStream_t; NULL = the
+ * null stream) and the call returns without waiting. It calls no hipMalloc
+ * or hipFree: every buffer of its kernels lives in one device array of
+ * ESCH_NOALLOC_SCRATCH_BYTES per GPU, held from the first call on that GPU
+ * until the process ends. So n is at most ESCH_NOALLOC_MAX_ROWS
+ * (ESCH_CAPACITY beyond), and a panel whose longest reference needs more
+ * DP boundary scratch than the array holds is refused with ESCH_CAPACITY
+ * (on a panel of long references the persistent grids are cut to fit the
+ * array). Calls that overlap on one GPU share its array and race: serialise
+ * them (one stream, or a wait between calls). The first call on a device
+ * resolves the array's address: make it, n = 0 does only that, before
+ * capturing a graph. */
+#define ESCH_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+#define ESCH_NOALLOC_MAX_ROWS ((size_t)1 << 20)
+esch_status esch_score_device_noalloc(esch_scorer *s, const uint8_t *codes, const uint64_t *offs,
+                                      const int32_t *lens, size_t n, int32_t *scores, uint8_t *scored,
+                                      void *stream);
+
 /* One read's score-stage selection, fused on device: the first half of
  * Aligner::map_reads_scored (lib/mapper.rs:444-472). */
 typedef struct {
