@@ -51,8 +51,10 @@ OPS_PER_CELL = {("semiglobal", False, False): 8, ("local", False, False): 10,
                 ("semiglobal", True, False): 7, ("local", True, False): 8.5,
                 ("semiglobal", False, True): 7, ("local", False, True): 9,
                 ("semiglobal", True, True): 6, ("local", True, True): 7.5}
-F16_VARIANTS = (4, 5, 7, 8)  # k_dp16 variants that compute in fp16
-SHIFTED_VARIANTS = (6, 7, 8)  # and those with the row-shifted semiglobal column
+F16_VARIANTS = (4, 5, 7, 8, 9, 10, 12, 13, 15, 16)  # k_dp16 variants that compute in fp16
+SHIFTED_VARIANTS = (6, 7, 8, 9, 10, 11, 12, 13, 14)  # and those with a shifted column
+# variant -> rows per DP stripe (9-11, 15, 16: 24 rows, 12-14: 32 rows, else 16)
+STRIPE_ROWS = {9: 24, 10: 24, 11: 24, 12: 32, 13: 32, 14: 32, 15: 24, 16: 24}
 # SQ_INSTS_VALU x 128 / cells of one k_dp16 launch of the default workload,
 # from the counter summary in profiles/ (mode, variant) -> lane-ops per cell
 VALU_OPS_PER_CELL_ISSUED = {("semiglobal", 3): round(344_021_737_036 * 128 / 5_303_272_453_076, 3),
@@ -404,9 +406,9 @@ def config3_leg(n_reads, cpu, cpu_reads=2000):
     """The large-panel workload: 10k references of 200-1000 nt (2k seeds
     plus 4 copies of each at 2% substitutions, so ties and near-ties are
     common), n_reads reads of 100-1000 nt, local 2,-1,-10,-1, forward
-    strand. The autotuner sweeps the batched kernel's variants and the
-    per-length-bucket routing to the anti-diagonal kernel on the first 1,000
-    reads; one timed pass of the whole batch with the fused selection. The
+    strand. The autotuner sweeps the batched kernel's variants (schedules
+    and stripe heights) and the per-length-bucket routing to the
+    anti-diagonal kernel on the first 500 reads; one timed pass of the whole batch with the fused selection. The
     CPU baseline is the oracle's AVX-512 restatement of the reference's CPU
     kernel on the first cpu_reads reads (a seeded subsample), equal to the
     device's rows and tie sets."""
@@ -419,7 +421,7 @@ def config3_leg(n_reads, cpu, cpu_reads=2000):
     gen_s = time.perf_counter() - t0
     scoring = (2, -1, -10, -1)
     gpu = ea.GpuScorer(ea.Panel(panel), ea.Scoring(*scoring), ea.Mode.LOCAL)
-    sample = [codes[int(off[k]):int(off[k]) + int(lens[k])] for k in range(min(n_reads, 1000))]
+    sample = [codes[int(off[k]):int(off[k]) + int(lens[k])] for k in range(min(n_reads, 500))]
     t0 = time.perf_counter()
     chosen, tune_ms = gpu.autotune(sample, both_strands=False)
     tune_s = time.perf_counter() - t0
@@ -439,7 +441,8 @@ def config3_leg(n_reads, cpu, cpu_reads=2000):
     out = {"workload": f"10k refs 200-1000 nt (2k seeds + 4 copies at 2% substitutions), {n_reads} reads "
                        f"100-1000 nt, local 2,-1,-10,-1, forward strand",
            "reads": n_reads, "n_refs": len(panel), "panel_bases": int(sum(len(p) for p in panel)),
-           "cells": int(cells), "kernel_variant": chosen, "diag_ratio": gpu.diag_ratio,
+           "cells": int(cells), "kernel_variant": chosen, "stripe_rows": STRIPE_ROWS.get(chosen, 16),
+           "diag_ratio": gpu.diag_ratio, "diag_plan_x16": gpu.diag_plan,
            "autotune_ms": [round(x, 3) for x in tune_ms], "autotune_s": round(tune_s, 2),
            "reads_per_s": round(n_reads / wall, 1), "wall_ms": round(wall * 1e3, 1), "dp_ms": round(dp, 3),
            "gcups": round(cells / (dp / 1e3) / 1e9, 1), "roofline": _roof(cells, dp, "local", chosen),
@@ -773,7 +776,9 @@ def main():
     # counterpart (variant 6 for the row-shifted 7, else 3), so both
     # arithmetics' rates are on record.
     int16 = None
-    v16 = 6 if shifted else 3
+    # the int16 schedule on the same stripes: 6 (or 11 / 14 on 24 / 32 rows)
+    v16 = {16: 6, 24: 11, 32: 14}[STRIPE_ROWS.get(chosen, 16)] if shifted else 3
+    # (the unshifted fp16 schedules' int16 counterpart is 3, on 16 rows)
     if f16:
         gpu.variant = v16
         batch.run(eager, tie_cap=tie_cap)
@@ -855,7 +860,7 @@ def main():
         mapped = map_reads_leg(gpu, panel_codes, codes, off, lens, scoring, local,
                                min(args.map_reads, args.reads), not args.no_cpu)
         log(f"[bench] read mapping: {mapped}")
-    gpu_ratio = gpu.diag_ratio
+    gpu_ratio, gpu_plan = gpu.diag_ratio, gpu.diag_plan
     del gpu
 
     configs = {}
@@ -930,8 +935,13 @@ def main():
                                "scope": "reads from host buffers (gather into pinned staging, copy), the "
                                         "graph, selection records and tie lists copied back"},
         "kernel_variant": chosen,
+        "stripe_rows": STRIPE_ROWS.get(chosen, 16),
         "int16_variant": int16,
         "diag_ratio": gpu_ratio,
+        "diag_plan": {"classes_nt": ["<128", "128-255", "256-511", "512-1023", "1024-2047", "2048-4095",
+                                     "4096-8191", ">=8192"],
+                      "per_class_x16": gpu_plan,
+                      "meaning": "0 inter-sequence, 1 anti-diagonal, 16..1024 ratio rule (rho x 16)"},
         "autotune_ms": [round(x, 3) for x in tune_ms],
         "plain_kernel": plain,
         "cpu_baseline": cpu,

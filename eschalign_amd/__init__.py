@@ -34,6 +34,10 @@ lib_path = os.path.join(_HERE, "libeschalign_amd.so")
 
 MAX_READ_LEN = 4096
 NOALLOC_MAX_ROWS = 1 << 20
+# length classes of the per-bucket routing plan: [lo, hi) nt
+LEN_CLASSES = 8
+LEN_CLASS_BOUNDS = [(0, 128), (128, 256), (256, 512), (512, 1024), (1024, 2048), (2048, 4096), (4096, 8192),
+                    (8192, None)]
 NOALLOC_SCRATCH_BYTES = 256 << 20
 MAX_REF_LEN = 16384
 
@@ -72,6 +76,7 @@ EXPORTS = [
     "esch_mapped_pairs", "esch_mapped_hits", "esch_mapped_ops", "esch_mapped_stats", "esch_mapped_free",
     "esch_set_map_chunk", "esch_adapter_trim_pairs", "esch_md_nm", "esch_cigar_string",
     "esch_map_reads_concat", "esch_align_last_wavefront", "esch_score_device_noalloc",
+    "esch_set_diag_plan", "esch_get_diag_plan",
 ]
 
 
@@ -211,6 +216,8 @@ def load_library():
     L.esch_batch_variant.restype = C.c_int
     L.esch_batch_stats_diag.argtypes = [vp, C.POINTER(C.c_uint64)]
     L.esch_set_diag_ratio.argtypes = [vp, C.c_int32]
+    L.esch_set_diag_plan.argtypes = [vp, vp, sz]
+    L.esch_get_diag_plan.argtypes = [vp, vp, sz]
     L.esch_align_pairs.argtypes = [vp, vp, vp, vp, sz, vp, vp, sz, C.POINTER(sz)]
     L.esch_align_last_ms.argtypes = [vp]
     L.esch_align_pairs_concat.argtypes = [vp, vp, vp, vp, vp, sz, vp, vp, sz, C.POINTER(sz)]
@@ -991,6 +998,20 @@ class GpuScorer:
     @diag_ratio.setter
     def diag_ratio(self, ratio: float):
         _check(_lib.esch_set_diag_ratio(self._h, int(round(ratio * 16))))
+
+    @property
+    def diag_plan(self) -> List[int]:
+        """Per length class (LEN_CLASS_BOUNDS), the routing to the
+        anti-diagonal kernel: 0 never, 1 always, 16..1024 the ratio rule
+        (rho x 16)."""
+        plan = np.zeros(LEN_CLASSES, np.int32)
+        _check(_lib.esch_get_diag_plan(self._h, plan.ctypes.data, LEN_CLASSES))
+        return [int(x) for x in plan]
+
+    @diag_plan.setter
+    def diag_plan(self, plan):
+        plan = np.ascontiguousarray(plan, np.int32)
+        _check(_lib.esch_set_diag_plan(self._h, plan.ctypes.data, len(plan)))
 
     def autotune(self, reads: Sequence[np.ndarray], both_strands=True):
         """Time every batched-kernel variant on `reads`, keep the fastest,

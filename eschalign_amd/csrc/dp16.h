@@ -452,13 +452,13 @@ __device__ __forceinline__ void stripe16(const uint4 *__restrict__ prof, uint2 *
 // stripe's space); CS (local only) the column-shifted one (column16c), with
 // `best` carried in column j's space through the stripe.
 template <int RR, bool LOCAL, bool FIRST, bool LAST, bool QUAD, bool F16, bool OCT = false, bool SH = false,
-          bool CS = false>
+          bool CS = false, int PQ = 4>
 __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 *__restrict__ scr,
                                           const int lane, const cwords w2, const cwords wn,
                                           const int m, const int row0, const int pA, const int pB,
                                           const typename Pk<F16>::T vo, const typename Pk<F16>::T ve,
                                           typename Pk<F16>::T &best, const int c = 0) {
-    static_assert(RR % 2 == 0 && RR >= 2 && RR <= 16, "an even number of rows");
+    static_assert(RR % 2 == 0 && RR >= 2 && RR <= 4 * PQ, "an even number of rows, at most the plane's");
     static_assert(!(SH && LOCAL), "the row shift is a semiglobal form");
     static_assert(!(CS && !LOCAL), "the column shift is a local form");
     constexpr bool XS = SH || CS;  // a shifted column: pairs run interleaved
@@ -501,7 +501,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
         return ((wn[j >> 5] >> (j & 31)) & 1u) ? 4 : (int)c2;
     };
     auto plane = [&](int c, v2s (&sc)[RR]) {
-        const uint4 *pc = prof + c * 4 * 64 + lane;
+        const uint4 *pc = prof + c * PQ * 64 + lane;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const uint4 v = pc[q * 64];
@@ -568,7 +568,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
             const int ja = jj + 2 < m ? jj + 2 : m, jb = jj + 3 < m ? jj + 3 : m;
             v2s tA, tB;
             column16c_pair<RR, F16>(sa, sb, Ho, E, puA, qA, pdA, puB, qB, pdB, voe, vse, zA, zB, tA, tB,
-                                    prof + code(ja) * 4 * 64 + lane, prof + code(jb) * 4 * 64 + lane);
+                                    prof + code(ja) * PQ * 64 + lane, prof + code(jb) * PQ * 64 + lane);
             // best from column jj - 1's space to jj + 1's
             best = P::max3(P::add(best, vn2e), P::add(tA, vne), tB);
             o0 = make_uint2(P::bits(puA), P::bits(qA));
@@ -579,7 +579,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
             hdt = puB;
             const int ja = jj + 2 < m ? jj + 2 : m, jb = jj + 3 < m ? jj + 3 : m;
             column16s_pair<RR, !FIRST, F16>(sa, sb, Ho, E, puA, qA, pdA, puB, qB, pdB, voe, vse, vc,
-                                            prof + code(ja) * 4 * 64 + lane, prof + code(jb) * 4 * 64 + lane);
+                                            prof + code(ja) * PQ * 64 + lane, prof + code(jb) * PQ * 64 + lane);
             if constexpr (LAST) bestL = P::max3(bestL, puA, puB);
             o0 = make_uint2(P::bits(puA), P::bits(qA));
             o1 = make_uint2(P::bits(puB), P::bits(qB));
@@ -746,21 +746,24 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
     }
 }
 
-template <int RR, bool LOCAL, bool FIRST, bool QUAD, bool F16, bool OCT, bool SH = false, bool CS = false>
+template <int RR, bool LOCAL, bool FIRST, bool QUAD, bool F16, bool OCT, bool SH = false, bool CS = false, int PQ = 4>
 __device__ __forceinline__ void stripe16d_any(const uint4 *prof, uint2 *scr, const int lane,
                                               const cwords w2, const cwords wn, const int m,
                                               const bool last, const int row0, const int pA,
                                               const int pB, const typename Pk<F16>::T vo,
                                               const typename Pk<F16>::T ve, typename Pk<F16>::T &best,
                                               const int c = 0) {
-    if (last) stripe16d<RR, LOCAL, FIRST, true, QUAD, F16, OCT, SH, CS>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best, c);
-    else stripe16d<RR, LOCAL, FIRST, false, QUAD, F16, OCT, SH, CS>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best, c);
+    if (last) stripe16d<RR, LOCAL, FIRST, true, QUAD, F16, OCT, SH, CS, PQ>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best, c);
+    else stripe16d<RR, LOCAL, FIRST, false, QUAD, F16, OCT, SH, CS, PQ>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best, c);
 }
 
-template <bool LOCAL, int PF>
+template <bool LOCAL, int PF, int ST = kStripe>
 __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
-    // prof[(c * 4 + quad) * 64 + lane] = rows 4 quad .. 4 quad + 3 of plane c
-    __shared__ uint4 prof[5 * 4 * 64];
+    static_assert(ST == 16 || ST == 24 || ST == 32, "16-, 24- or 32-row stripes");
+    static_assert(PF >= 2 || ST == 16, "variants 0 and 1 take 16-row stripes");
+    constexpr int PQ = ST / 4;  // quads of rows per profile plane
+    // prof[(c * PQ + quad) * 64 + lane] = rows 4 quad .. 4 quad + 3 of plane c
+    __shared__ uint4 prof[5 * PQ * 64];
     __shared__ uint32_t s_item;
     const int lane = threadIdx.x & 63;
     const int wv = uni(threadIdx.x >> 6);
@@ -807,13 +810,13 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
         // Longest first: the chunk's first read sets its row plan, S stripes
         // of which the first has RR rows (a multiple of 4) and the rest 16.
         const int nmax = uni(a.qlen[a.sorted16[k * kChunk]]);
-        const int S = (nmax + kStripe - 1) / kStripe;
+        const int S = (nmax + ST - 1) / ST;
         // variants 2 and 3 take first stripes of any even height, the
         // others multiples of 4 (less top padding: 0.5 rows a read on
         // average against 1.5)
         constexpr int G = PF >= 2 ? 2 : 4;
-        const int RR = ((nmax - kStripe * (S - 1) + G - 1) / G) * G;
-        const int T = RR + kStripe * (S - 1);
+        const int RR = ((nmax - ST * (S - 1) + G - 1) / G) * G;
+        const int T = RR + ST * (S - 1);
         const int pA = T - nA, pB = T - nB;
         const uint8_t *qA = a.qcodes + (qa >= 0 ? a.qoff[qa] : 0);
         const uint8_t *qB = a.qcodes + (qb >= 0 ? a.qoff[qb] : 0);
@@ -825,18 +828,22 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
         V best = LOCAL ? vo : NEG;  // Ho = H + o space (local starts at H = 0)
 
         for (int s = 0; s < S; ++s) {
-            const int row0 = s == 0 ? 0 : RR + kStripe * (s - 1);
-            const int height = s == 0 ? RR : kStripe;
+            const int row0 = s == 0 ? 0 : RR + ST * (s - 1);
+            const int height = s == 0 ? RR : ST;
             // SH: the row above, the previous stripe's bottom row, is lower
             // by e x (the previous stripe's height) in this stripe's space
-            const int cadj = (SH && s > 0) ? a.sc.e * (s == 1 ? RR : kStripe) : 0;
-            // This stripe's profile: wave wv builds rows row0 + 4 wv .. + 3.
+            const int cadj = (SH && s > 0) ? a.sc.e * (s == 1 ? RR : ST) : 0;
+            // This stripe's profile: wave wv builds the quads wv, wv + 4, ...
+            // (rows row0 + 4 quad .. + 3).
             __syncthreads();
-            if (4 * wv < height) {
+#pragma unroll
+            for (int qd0 = 0; qd0 < PQ; qd0 += kWaves16) {
+            const int qd = qd0 + wv;
+            if (4 * qd < height) {
                 uint32_t pl[5][4];
 #pragma unroll
                 for (int rr = 0; rr < 4; ++rr) {
-                    const int i = row0 + 4 * wv + rr;
+                    const int i = row0 + 4 * qd + rr;
                     // rows past the chunk's plan (a first stripe of 2, 6, 10 or
                     // 14 rows builds whole quads) are never read: padding codes
                     const int xa = (i < pA || i >= T) ? kPreCode : qA[i - pA];
@@ -858,7 +865,8 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
                 }
 #pragma unroll
                 for (int c = 0; c < 5; ++c)
-                    prof[(c * 4 + wv) * 64 + lane] = make_uint4(pl[c][0], pl[c][1], pl[c][2], pl[c][3]);
+                    prof[(c * PQ + qd) * 64 + lane] = make_uint4(pl[c][0], pl[c][1], pl[c][2], pl[c][3]);
+            }
             }
             __syncthreads();
             if (!active) continue;
@@ -866,15 +874,20 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
             if constexpr (PF >= 2) {
                 constexpr bool Q = PF == 3 || PF == 4 || PF == 6 || PF == 8;
                 if (s > 0) {
-                    stripe16d_any<16, LOCAL, false, Q, F16, OCT, SH, CS>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best, cadj);
+                    stripe16d_any<ST, LOCAL, false, Q, F16, OCT, SH, CS, PQ>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best, cadj);
                 } else {
                     switch (RR) {
 #define ESCH_FIRST(H) \
-    case H: stripe16d_any<H, LOCAL, true, Q, F16, OCT, SH, CS>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best); break;
+    case H: stripe16d_any<(H <= ST ? H : 2), LOCAL, true, Q, F16, OCT, SH, CS, PQ>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best); break;
                         ESCH_FIRST(16) ESCH_FIRST(14) ESCH_FIRST(12) ESCH_FIRST(10)
                         ESCH_FIRST(8) ESCH_FIRST(6) ESCH_FIRST(4)
-                        default: stripe16d_any<2, LOCAL, true, Q, F16, OCT, SH, CS>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
 #undef ESCH_FIRST
+#define ESCH_FIRST(H) \
+    case H: if constexpr (ST >= H) stripe16d_any<(H <= ST ? H : 2), LOCAL, true, Q, F16, OCT, SH, CS, PQ>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best); break;
+                        ESCH_FIRST(32) ESCH_FIRST(30) ESCH_FIRST(28) ESCH_FIRST(26) ESCH_FIRST(24)
+                        ESCH_FIRST(22) ESCH_FIRST(20) ESCH_FIRST(18)
+#undef ESCH_FIRST
+                        default: stripe16d_any<2, LOCAL, true, Q, F16, OCT, SH, CS, PQ>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
                     }
                 }
             } else {
@@ -908,45 +921,74 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
     }
 }
 
-// The variants of one mode: the kernel's host handle and its launch. Each
-// mode's instances live in a translation unit of their own (kernels_sg.hip,
-// kernels_lo.hip), so the two compile in parallel.
-template <bool LOCAL>
+// The variants of one mode and stripe height: the kernel's host handle and
+// its launch. Each (mode, stripe height) has a translation unit of its own
+// (kernels_{sg,lo}{,24,32}.hip), so they compile in parallel; the 24-row
+// units instantiate the schedules 4-8, the 32-row ones 6, 7 and 8.
+template <bool LOCAL, int ST>
 inline const void *dp16_fn_impl(int variant) {
-    switch (variant) {
-        case 8: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 8>);
-        case 7: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 7>);
-        case 6: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 6>);
-        case 5: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 5>);
-        case 4: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 4>);
-        case 3: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 3>);
-        case 2: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 2>);
-        case 1: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 1>);
-        default: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 0>);
+    switch (dp16_schedule(variant)) {
+        case 8: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 8, ST>);
+        case 7: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 7, ST>);
+        case 6: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 6, ST>);
+        default: break;
     }
+    if constexpr (ST <= 24) {
+        switch (dp16_schedule(variant)) {
+            case 5: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 5, ST>);
+            case 4: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 4, ST>);
+            default: break;
+        }
+    }
+    if constexpr (ST == 16) {
+        switch (variant) {
+            case 3: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 3>);
+            case 2: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 2>);
+            case 1: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 1>);
+            default: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 0>);
+        }
+    }
+    return nullptr;
 }
 
-template <bool LOCAL>
+template <bool LOCAL, int ST>
 inline hipError_t launch_dp16_impl(const KArgs &a, const Launch &g, hipStream_t st) {
     const dim3 grid(g.grid16), block(64 * kWaves16);
-    switch (g.variant16) {
-        case 8: hipLaunchKernelGGL((k_dp16<LOCAL, 8>), grid, block, 0, st, a); break;
-        case 7: hipLaunchKernelGGL((k_dp16<LOCAL, 7>), grid, block, 0, st, a); break;
-        case 6: hipLaunchKernelGGL((k_dp16<LOCAL, 6>), grid, block, 0, st, a); break;
-        case 5: hipLaunchKernelGGL((k_dp16<LOCAL, 5>), grid, block, 0, st, a); break;
-        case 4: hipLaunchKernelGGL((k_dp16<LOCAL, 4>), grid, block, 0, st, a); break;
-        case 3: hipLaunchKernelGGL((k_dp16<LOCAL, 3>), grid, block, 0, st, a); break;
-        case 2: hipLaunchKernelGGL((k_dp16<LOCAL, 2>), grid, block, 0, st, a); break;
-        case 1: hipLaunchKernelGGL((k_dp16<LOCAL, 1>), grid, block, 0, st, a); break;
-        default: hipLaunchKernelGGL((k_dp16<LOCAL, 0>), grid, block, 0, st, a); break;
+    switch (dp16_schedule(g.variant16)) {
+        case 8: hipLaunchKernelGGL((k_dp16<LOCAL, 8, ST>), grid, block, 0, st, a); return hipGetLastError();
+        case 7: hipLaunchKernelGGL((k_dp16<LOCAL, 7, ST>), grid, block, 0, st, a); return hipGetLastError();
+        case 6: hipLaunchKernelGGL((k_dp16<LOCAL, 6, ST>), grid, block, 0, st, a); return hipGetLastError();
+        default: break;
     }
-    return hipGetLastError();
+    if constexpr (ST <= 24) {
+        switch (dp16_schedule(g.variant16)) {
+            case 5: hipLaunchKernelGGL((k_dp16<LOCAL, 5, ST>), grid, block, 0, st, a); return hipGetLastError();
+            case 4: hipLaunchKernelGGL((k_dp16<LOCAL, 4, ST>), grid, block, 0, st, a); return hipGetLastError();
+            default: break;
+        }
+    }
+    if constexpr (ST == 16) {
+        switch (g.variant16) {
+            case 3: hipLaunchKernelGGL((k_dp16<LOCAL, 3>), grid, block, 0, st, a); break;
+            case 2: hipLaunchKernelGGL((k_dp16<LOCAL, 2>), grid, block, 0, st, a); break;
+            case 1: hipLaunchKernelGGL((k_dp16<LOCAL, 1>), grid, block, 0, st, a); break;
+            default: hipLaunchKernelGGL((k_dp16<LOCAL, 0>), grid, block, 0, st, a); break;
+        }
+        return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
 }
 
-// Defined in kernels_sg.hip (semiglobal) and kernels_lo.hip (local).
-const void *dp16_fn_sg(int variant);
-const void *dp16_fn_lo(int variant);
-hipError_t launch_dp16_sg(const KArgs &a, const Launch &g, hipStream_t st);
-hipError_t launch_dp16_lo(const KArgs &a, const Launch &g, hipStream_t st);
+// Defined in kernels_{sg,lo}{,24,32}.hip.
+#define ESCH_DP16_UNIT(M, ST)                            \
+    const void *dp16_fn_##M##ST(int variant);            \
+    hipError_t launch_dp16_##M##ST(const KArgs &a, const Launch &g, hipStream_t st);
+ESCH_DP16_UNIT(sg, 16)
+ESCH_DP16_UNIT(sg, 24)
+ESCH_DP16_UNIT(sg, 32)
+ESCH_DP16_UNIT(lo, 16)
+ESCH_DP16_UNIT(lo, 24)
+ESCH_DP16_UNIT(lo, 32)
+#undef ESCH_DP16_UNIT
 
 }  // namespace esch

@@ -33,14 +33,16 @@ __host__ __device__ inline bool fits_f16(int64_t max_ref_len, int32_t ma, int32_
     return max_ref_len * (M > Mi ? M : Mi) + O + Ee <= kBoundF16;
 }
 // k_dp16's row-shifted semiglobal column (variants 6 and 7) holds a stripe
-// row r's values shifted by |e| r (r = -1 .. 15) and offset by o or e:
-// every value lies in [-(kM + |o| + 4|e|), kM + 16|e|], and the profile's
-// stripe-top entries s - o + e h (h <= 16) within M + |o| + 16|e|. With 17|e|
-// for the shift, the unshifted bounds carry over.
-__host__ __device__ inline int64_t shifted_bound(int64_t max_ref_len, int32_t ma, int32_t mi, int32_t o, int32_t e) {
+// row r's values shifted by |e| r (r = -1 .. ST - 1 in a stripe of ST rows)
+// and offset by o or e: every value lies in [-(kM + |o| + 4|e|),
+// kM + ST |e|], and the profile's stripe-top entries s - o + e h (h <= ST)
+// within M + |o| + ST |e|. With (ST + 1) |e| for the shift, the unshifted
+// bounds carry over.
+__host__ __device__ inline int64_t shifted_bound(int64_t max_ref_len, int32_t ma, int32_t mi, int32_t o, int32_t e,
+                                                 int stripe = kStripe) {
     const int64_t M = ma < 0 ? -(int64_t)ma : ma, Mi = mi < 0 ? -(int64_t)mi : mi;
     const int64_t O = o < 0 ? -(int64_t)o : o, Ee = e < 0 ? -(int64_t)e : e;
-    return max_ref_len * (M > Mi ? M : Mi) + O + 17 * Ee;
+    return max_ref_len * (M > Mi ? M : Mi) + O + (stripe + 1) * Ee;
 }
 // In local mode those variants hold reference column j's values shifted by
 // |e| j (j = -1 .. L - 1): every value lies within
@@ -75,6 +77,17 @@ struct DevPanel {
     int64_t total_len;
 };
 
+// Length classes of the per-length-bucket routing plan: a read of length L
+// is in class len_class(L): 0 under 128 nt (never routed), then 128-255,
+// 256-511, 512-1023, 1024-2047, 2048-4095, 4096-8191, and 8192 and over.
+constexpr int kLenClasses = 8;
+__host__ __device__ inline int len_class(int L) {
+    if (L < 128) return 0;
+    int c = 1;
+    for (int b = 256; c < kLenClasses - 1 && L >= b; b *= 2) ++c;
+    return c;
+}
+
 // Device-resident batch descriptor: every kernel reads its sizes from here,
 // so one captured graph replays for any batch that fits the buffers.
 struct DevDesc {
@@ -97,6 +110,11 @@ struct DevDesc {
     int32_t blocks16;    // resident blocks of the inter-sequence kernel's grid
     int32_t max16;       // longest query on the int16 path (bins above it are empty)
     uint64_t cells;      // sum of len x panel bases over scored queries
+    // per length class (len_class), the routing of its length bins while
+    // diag_ratio > 0: 0 = they stay on the inter-sequence kernel, 1 = they
+    // take the anti-diagonal kernel, 16..1024 = the ratio rule with rho =
+    // plan / 16 (diag_ratio itself rules the small-batch cases)
+    int32_t plan[kLenClasses];
 };
 
 struct KArgs {
@@ -153,12 +171,30 @@ struct Launch {
     int grid_diag;  // persistent waves of the anti-diagonal kernel
     size_t diag_lds;  // k_diag64's LDS boundary bytes; 0: k_diag16 (global scratch)
 };
-constexpr int kDp16Variants = 9;
+// Variants 0-8 are k_dp16's schedules (above) on 16-row stripes; 9-16 run
+// some of them on taller stripes: 9, 10, 11 = schedules 7, 8, 6 on 24 rows;
+// 12, 13, 14 = 7, 8, 6 on 32 rows; 15, 16 = 4, 5 on 24 rows. A stripe
+// boundary per 24 or 32 rows instead of 16 means less HBM traffic and the
+// per-stripe work spread over more rows, held in more registers (fewer
+// resident waves where they pass 168).
+constexpr int kDp16Variants = 17;
 constexpr int kDp16Default = 3;  // the int16 default; panels within kBoundF16 take kDp16F16
 constexpr int kDp16F16 = 4;      // the first variant computing in fp16
+constexpr int kDp16Sched[kDp16Variants] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 7, 8, 6, 7, 8, 6, 4, 5};
+constexpr int kDp16Rows[kDp16Variants] = {16, 16, 16, 16, 16, 16, 16, 16, 16, 24, 24, 24, 32, 32, 32, 24, 24};
+constexpr int dp16_schedule(int v) { return kDp16Sched[v]; }
+constexpr int dp16_stripe(int v) { return kDp16Rows[v]; }
 // variants computing in fp16, and those with the row-shifted semiglobal column
-constexpr bool dp16_is_f16(int v) { return v == 4 || v == 5 || v == 7 || v == 8; }
-constexpr bool dp16_is_shifted(int v) { return v >= 6; }
+constexpr bool dp16_is_f16(int v) {
+    return dp16_schedule(v) == 4 || dp16_schedule(v) == 5 || dp16_schedule(v) == 7 || dp16_schedule(v) == 8;
+}
+constexpr bool dp16_is_shifted(int v) { return dp16_schedule(v) >= 6; }
+// The variant with schedule `schedule` on stripes of `stripe` rows (-1: none).
+constexpr int dp16_variant(int schedule, int stripe) {
+    for (int v = 0; v < kDp16Variants; ++v)
+        if (kDp16Sched[v] == schedule && kDp16Rows[v] == stripe) return v;
+    return -1;
+}
 hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st);
 hipError_t launch_dp16(const KArgs &a, const Launch &g, bool local, hipStream_t st);
 hipError_t launch_dp32(const KArgs &a, const Launch &g, bool local, int which, hipStream_t st);

@@ -70,7 +70,10 @@ struct esch_scorer {
     bool variant_fixed = false;         // set by the caller (esch_set_variant): no per-batch upgrade
     int diag_min_len = 0;               // reads this long take k_diag16 (0: never)
     int diag_ratio = kDiagRatioDefault; // padding-heavy length bins take k_diag16 (0: never)
-    bool diag_ratio_tunable = true;     // false once the caller set the ratio
+    bool diag_ratio_tunable = true;     // false once the caller set the ratio or the plan
+    int32_t diag_plan[esch::kLenClasses] = {kDiagRatioDefault, kDiagRatioDefault, kDiagRatioDefault,
+                                            kDiagRatioDefault, kDiagRatioDefault, kDiagRatioDefault,
+                                            kDiagRatioDefault, kDiagRatioDefault};  // per length class
     std::mutex pool_mu;
     std::vector<esch_batch *> pool;  // idle batches for the drop-in API
     std::vector<int32_t> h_rlen;     // reference lengths (traceback sizing)

@@ -714,12 +714,17 @@ __global__ __launch_bounds__(1024) void k_scan(KArgs a) {
         }
     }
     __syncthreads();
+    // the small-batch rules (starved, all) follow the ratio; each length
+    // class's bins follow its plan entry (0 for every class: no routing)
     const int ratio = a.desc->diag_ratio;
+    bool planned = false;
+#pragma unroll
+    for (int c = 0; c < kLenClasses; ++c) planned |= a.desc->plan[c] != 0;
     const int64_t items16 = (int64_t)((total + kChunk - 1) / kChunk) * ((a.panel.n_refs + 3) / 4);
-    const bool starved = items16 < (int64_t)a.desc->blocks16;
+    const bool starved = ratio > 0 && items16 < (int64_t)a.desc->blocks16;
     // routing flags into bin_fill (k_zero cleared it; cleared again below)
     int32_t kept = sum;
-    if (ratio > 0) {
+    if (ratio > 0 || planned) {
         for (int k = 0; k < nb; ++k) {
             const int L = b0 - k;
             const int cnt = a.hist[L];
@@ -730,7 +735,9 @@ __global__ __launch_bounds__(1024) void k_scan(KArgs a) {
             const int r = a.bin_start[L];
             const int x = min(r + kChunk - 1, total - 1);
             const int num = 16 * L - 1025;
-            const bool thin = num >= 0 && a.bin_start[num / ratio] <= x;  // num / ratio < L
+            // the length class's own rule (the autotuner's per-bucket plan)
+            const int rc = a.desc->plan[len_class(L)];
+            const bool thin = rc == 1 || (rc >= 16 && num >= 0 && a.bin_start[num / rc] <= x);  // num / rc < L
             if (thin || (starved && L >= 512)) {
                 a.bin_fill[L] = 1;
                 kept -= cnt;
@@ -767,7 +774,7 @@ __global__ __launch_bounds__(1024) void k_scan(KArgs a) {
     }
     for (int k = 0; k < nb; ++k) {
         const int b = b0 - k;
-        if (ratio > 0 && (a.bin_fill[b] || all)) {
+        if (a.bin_fill[b] || all) {
             a.bin_start[b] = -1;
             a.bin_fill[b] = 0;
         } else {
@@ -1023,7 +1030,19 @@ hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st) {
 }
 
 hipError_t launch_dp16(const KArgs &a, const Launch &g, bool local, hipStream_t st) {
-    return local ? launch_dp16_lo(a, g, st) : launch_dp16_sg(a, g, st);
+    switch (dp16_stripe(g.variant16)) {
+        case 24: return local ? launch_dp16_lo24(a, g, st) : launch_dp16_sg24(a, g, st);
+        case 32: return local ? launch_dp16_lo32(a, g, st) : launch_dp16_sg32(a, g, st);
+        default: return local ? launch_dp16_lo16(a, g, st) : launch_dp16_sg16(a, g, st);
+    }
+}
+
+static const void *dp16_fn(bool local, int variant) {
+    switch (dp16_stripe(variant)) {
+        case 24: return local ? dp16_fn_lo24(variant) : dp16_fn_sg24(variant);
+        case 32: return local ? dp16_fn_lo32(variant) : dp16_fn_sg32(variant);
+        default: return local ? dp16_fn_lo16(variant) : dp16_fn_sg16(variant);
+    }
 }
 
 hipError_t launch_dp32(const KArgs &a, const Launch &g, bool local, int which, hipStream_t st) {
@@ -1062,7 +1081,7 @@ hipError_t launch_pairs(const KArgs &a, const Launch &g, hipStream_t st) {
 }
 
 hipError_t dp16_attributes(bool local, int variant, hipFuncAttributes *attr) {
-    return hipFuncGetAttributes(attr, local ? dp16_fn_lo(variant) : dp16_fn_sg(variant));
+    return hipFuncGetAttributes(attr, dp16_fn(local, variant));
 }
 
 hipError_t dp32_attributes(bool local, hipFuncAttributes *attr) {
@@ -1072,7 +1091,7 @@ hipError_t dp32_attributes(bool local, hipFuncAttributes *attr) {
 
 hipError_t dp16_occupancy(bool local, int variant, int *blocks) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        blocks, local ? dp16_fn_lo(variant) : dp16_fn_sg(variant), 64 * kWaves16, 0);
+        blocks, dp16_fn(local, variant), 64 * kWaves16, 0);
 }
 
 hipError_t dp32_occupancy(bool local, int block, int *blocks) {

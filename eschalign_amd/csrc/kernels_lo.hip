@@ -1,12 +1,13 @@
-// The local-mode instances of k_dp16 (every variant) and their launch.
+// The local-mode instances of k_dp16 on 16-row stripes (every variant)
+// and their launch.
 #include "dp16.h"
 
 namespace esch {
 
-const void *dp16_fn_lo(int variant) { return dp16_fn_impl<true>(variant); }
+const void *dp16_fn_lo16(int variant) { return dp16_fn_impl<true, 16>(variant); }
 
-hipError_t launch_dp16_lo(const KArgs &a, const Launch &g, hipStream_t st) {
-    return launch_dp16_impl<true>(a, g, st);
+hipError_t launch_dp16_lo16(const KArgs &a, const Launch &g, hipStream_t st) {
+    return launch_dp16_impl<true, 16>(a, g, st);
 }
 
 }  // namespace esch

@@ -1,12 +1,13 @@
-// The semiglobal-mode instances of k_dp16 (every variant) and their launch.
+// The semiglobal-mode instances of k_dp16 on 16-row stripes (every variant)
+// and their launch.
 #include "dp16.h"
 
 namespace esch {
 
-const void *dp16_fn_sg(int variant) { return dp16_fn_impl<false>(variant); }
+const void *dp16_fn_sg16(int variant) { return dp16_fn_impl<false, 16>(variant); }
 
-hipError_t launch_dp16_sg(const KArgs &a, const Launch &g, hipStream_t st) {
-    return launch_dp16_impl<false>(a, g, st);
+hipError_t launch_dp16_sg16(const KArgs &a, const Launch &g, hipStream_t st) {
+    return launch_dp16_impl<false, 16>(a, g, st);
 }
 
 }  // namespace esch

@@ -60,7 +60,11 @@ esch_status scratch_on(int ordinal, uint8_t **out) {
 // The run's descriptor, written on the device (no copy into memory a graph
 // may replay): every field esch_batch_run sets for an int32 scoring run
 // without selection; k_zero clears the counters after it.
-__global__ void k_noalloc_desc(DevDesc *d, int32_t n, int32_t ratio, int32_t blocks16) {
+struct Plan {
+    int32_t c[kLenClasses];
+};
+
+__global__ void k_noalloc_desc(DevDesc *d, int32_t n, int32_t ratio, int32_t blocks16, Plan plan) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     d->n_queries = n;
     d->n_reads = n;
@@ -75,6 +79,7 @@ __global__ void k_noalloc_desc(DevDesc *d, int32_t n, int32_t ratio, int32_t blo
     d->diag_min_len = 0;
     d->diag_ratio = ratio;
     d->blocks16 = blocks16;
+    for (int c = 0; c < kLenClasses; ++c) d->plan[c] = plan.c[c];
 }
 
 // scored[q] = 1 when query q was scored (class 1, 2 or 3), else 0.
@@ -103,10 +108,12 @@ extern "C" esch_status esch_score_device_noalloc(esch_scorer *s, const uint8_t *
 
     Launch geo;
     int ratio;
+    Plan plan;
     {
         std::lock_guard<std::mutex> g(s->geo_mu);
         geo = s->geo;
         ratio = s->diag_ratio;
+        for (int c = 0; c < kLenClasses; ++c) plan.c[c] = s->diag_plan[c];
     }
     const size_t L = s->max_len;
     // layout: descriptor, length bins, per-row class and lists, then one
@@ -164,7 +171,7 @@ extern "C" esch_status esch_score_device_noalloc(esch_scorer *s, const uint8_t *
     hipStream_t st = static_cast<hipStream_t>(stream);
     const bool local = s->sc.local != 0;
     hipLaunchKernelGGL(k_noalloc_desc, dim3(1), dim3(64), 0, st, a.desc, (int32_t)n, (int32_t)ratio,
-                       (int32_t)grid16);
+                       (int32_t)grid16, plan);
     HIPC(hipGetLastError());
     HIPC(launch_prep(a, geo, st));
     HIPC(launch_dp16(a, geo, local, st));

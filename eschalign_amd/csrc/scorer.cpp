@@ -229,10 +229,11 @@ esch_status esch_scorer_new(int ordinal, const uint8_t *const *refs, const size_
     s->f16_ok = fits_f16((int64_t)max_len, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e);
     {
         // the shifted variants: rows shifted in semiglobal mode, columns in local
-        const int64_t sb = local ? esch::colshift_bound((int64_t)max_len, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e)
-                                 : esch::shifted_bound((int64_t)max_len, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e);
         for (int v = 0; v < kDp16Variants; ++v) {
             const bool f16 = esch::dp16_is_f16(v);
+            const int64_t sb = local ? esch::colshift_bound((int64_t)max_len, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e)
+                                     : esch::shifted_bound((int64_t)max_len, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e,
+                                                           esch::dp16_stripe(v));
             const bool shift_ok = !esch::dp16_is_shifted(v) || sb <= (f16 ? esch::kBoundF16 : esch::kBound16);
             s->usable16[v] = (!f16 || s->f16_ok) && shift_ok;
         }
@@ -651,7 +652,11 @@ int batch_variant(const esch_batch *b, int v, bool fixed) {
     const int64_t k = (int64_t)std::min<size_t>(s->max_len, b->longest);
     if (!fits_f16(k, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e)) return v;
     if (s->sc.local) return kDp16F16;
-    return esch::shifted_bound(k, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e) <= esch::kBoundF16 ? 7 : kDp16F16;
+    // the row-shifted fp16 schedule on the same stripe height
+    const int st = esch::dp16_stripe(v);
+    return esch::shifted_bound(k, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e, st) <= esch::kBoundF16
+               ? esch::dp16_variant(7, st)
+               : kDp16F16;
 }
 
 // Rows up to which a batch is scored with the roles swapped. With the
@@ -863,6 +868,7 @@ esch_status esch_batch_run(esch_batch *b, int what, int32_t min_score, int64_t m
         geo = s->geo;
         ratio = s->diag_ratio;
         fixed = s->variant_fixed;
+        for (int c = 0; c < esch::kLenClasses; ++c) d.plan[c] = s->diag_plan[c];
     }
     d.diag_ratio = (what & ESCH_RUN_PLAIN) ? 0 : ratio;
     const Launch geo_scorer = geo;  // the swapped path runs the scorer's own variant
@@ -1160,7 +1166,7 @@ esch_status set_variant(esch_scorer *s, int variant, bool fixed) {
                                   "(max(|match|, |mismatch|) + |extend|) + |open| + 3 |extend| + "
                                   "max(|match|, |mismatch|) <= %lld"
                                 : "variant %d (row-shifted) needs longest reference x max(|match|, |mismatch|) + "
-                                  "|open| + 17 |extend| <= %lld",
+                                  "|open| + (stripe rows + 1) |extend| <= %lld",
                     variant, (long long)(esch::dp16_is_f16(variant) ? esch::kBoundF16 : esch::kBound16));
     std::lock_guard<std::mutex> g(s->geo_mu);
     s->geo.variant16 = variant;
@@ -1171,10 +1177,19 @@ esch_status set_variant(esch_scorer *s, int variant, bool fixed) {
     return ESCH_OK;
 }
 
+// The routing ratio, and the plan of every length class with it.
 void set_ratio(esch_scorer *s, int32_t ratio_x16) {
     std::lock_guard<std::mutex> g(s->geo_mu);
     s->diag_ratio = ratio_x16;
+    for (int c = 0; c < esch::kLenClasses; ++c) s->diag_plan[c] = ratio_x16;
 }
+
+void set_plan_entry(esch_scorer *s, int c, int32_t v) {
+    std::lock_guard<std::mutex> g(s->geo_mu);
+    s->diag_plan[c] = v;
+}
+
+bool plan_value_ok(int32_t v) { return v == 0 || v == 1 || (v >= 16 && v <= 1024); }
 
 }  // namespace
 
@@ -1214,7 +1229,29 @@ esch_status esch_set_diag_ratio(esch_scorer *s, int32_t ratio_x16) {
         return fail(ESCH_INVALID_ARG, "diag ratio must be 0 or 16..1024 (sixteenths)");
     std::lock_guard<std::mutex> g(s->geo_mu);
     s->diag_ratio = ratio_x16;
+    for (int c = 0; c < esch::kLenClasses; ++c) s->diag_plan[c] = ratio_x16;
     s->diag_ratio_tunable = false;
+    return ESCH_OK;
+}
+
+esch_status esch_set_diag_plan(esch_scorer *s, const int32_t *plan, size_t n) {
+    if (!s || !plan || n != (size_t)esch::kLenClasses)
+        return fail(ESCH_INVALID_ARG, "the plan holds %d length classes", esch::kLenClasses);
+    for (size_t c = 0; c < n; ++c)
+        if (!plan_value_ok(plan[c]))
+            return fail(ESCH_INVALID_ARG, "plan entry %zu is %d: 0 (inter-sequence), 1 (anti-diagonal) or 16..1024",
+                        c, plan[c]);
+    std::lock_guard<std::mutex> g(s->geo_mu);
+    for (size_t c = 0; c < n; ++c) s->diag_plan[c] = plan[c];
+    s->diag_ratio_tunable = false;
+    return ESCH_OK;
+}
+
+esch_status esch_get_diag_plan(const esch_scorer *s, int32_t *plan, size_t n) {
+    if (!s || !plan || n != (size_t)esch::kLenClasses)
+        return fail(ESCH_INVALID_ARG, "the plan holds %d length classes", esch::kLenClasses);
+    std::lock_guard<std::mutex> g(s->geo_mu);
+    for (size_t c = 0; c < n; ++c) plan[c] = s->diag_plan[c];
     return ESCH_OK;
 }
 
@@ -1293,6 +1330,36 @@ esch_status esch_autotune(esch_scorer *s, const uint8_t *const *reads, const siz
             }
         }
         set_ratio(s, best_r);
+        // 3. per length class (the per-bucket plan): for each class the
+        // sample holds reads of (128 nt and over), keep the class on the
+        // inter-sequence kernel, send it to the anti-diagonal kernel, or
+        // route it by its own ratio, whichever is more than 2% faster than
+        // the class's current choice
+        int counts[esch::kLenClasses] = {};
+        for (size_t k = 0; k < n; ++k) ++counts[esch::len_class((int)std::min<size_t>(lens[k], INT32_MAX))];
+        if (!st) {
+            float cur_ms = 0.0f;
+            st = timed(&cur_ms);
+            static const int32_t kOptions[] = {0, 1, 24, 32, 48, 64};
+            for (int c = 1; !st && c < esch::kLenClasses; ++c) {
+                if (!counts[c]) continue;
+                const int32_t keep = best_r;
+                int32_t best_o = keep;
+                float best_o_ms = cur_ms;
+                for (int32_t opt : kOptions) {
+                    if (st || opt == keep) continue;
+                    set_plan_entry(s, c, opt);
+                    float dp = 0.0f;
+                    st = timed(&dp);
+                    if (!st && dp < best_o_ms && dp < 0.98f * cur_ms) {
+                        best_o_ms = dp;
+                        best_o = opt;
+                    }
+                }
+                set_plan_entry(s, c, best_o);
+                cur_ms = best_o_ms;
+            }
+        }
     }
     give_batch(s, b);
     if (st) return st;

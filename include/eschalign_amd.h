@@ -223,6 +223,19 @@ int32_t esch_get_diag_min_len(const esch_scorer *s);
  * way. */
 esch_status esch_set_diag_ratio(esch_scorer *s, int32_t ratio_x16);
 int32_t esch_get_diag_ratio(const esch_scorer *s);
+/* The per-length-bucket plan of that routing: one entry per length class,
+ * c = 0 under 128 nt (never routed), 1: 128-255, 2: 256-511, 3: 512-1023,
+ * 4: 1024-2047, 5: 2048-4095, 6: 4096-8191, 7: 8192 and over. An entry is
+ * 0 (the class's reads stay on the inter-sequence kernel), 1 (they take
+ * the anti-diagonal kernel) or 16..1024 (the ratio rule above with rho =
+ * entry / 16). esch_set_diag_ratio sets every entry to its ratio (so 0
+ * turns all routing off); esch_autotune tunes them per class. The ratio
+ * alone rules the small-batch cases (a batch that cannot fill the device,
+ * kept rows that do not fill a chunk). Identical scores whatever the plan.
+ * n must be ESCH_LEN_CLASSES. */
+#define ESCH_LEN_CLASSES 8
+esch_status esch_set_diag_plan(esch_scorer *s, const int32_t *plan, size_t n);
+esch_status esch_get_diag_plan(const esch_scorer *s, int32_t *plan, size_t n);
 /* A batch of up to `rows` rows (reads x strands; by default as many rows
  * as the panel has references, when it has at least 4,096, and none
  * otherwise) is scored with the roles swapped: the panel's references as
@@ -432,17 +445,28 @@ size_t esch_cigar_string(const uint32_t *ops, size_t n_ops, size_t query_len, si
  *   8  variant 4 with those shifts (fewer registers than 7, one more
  *      resident wave, a shallower prefetch); the condition of 7.
  *      In the shifted variants the two columns of a pair run one row apart,
- *      so their serial chains interleave. */
+ *      so their serial chains interleave.
+ *   9-16  the same schedules on taller stripes (the tile shape): 9, 10, 11
+ *      = 7, 8, 6 with 24 rows per stripe; 12, 13, 14 = 7, 8, 6 with 32 rows;
+ *      15, 16 = 4, 5 with 24 rows. A stripe hands its bottom row to the next
+ *      through device memory, so taller stripes move fewer boundary bytes
+ *      and spread each stripe's fixed work over more rows, in more
+ *      registers. The row-shifted ones need (rows + 1) |extend| in place of
+ *      17 |extend| in the condition of 6 / 7. */
 int esch_n_variants(void);
 int esch_get_variant(const esch_scorer *s);
 esch_status esch_set_variant(esch_scorer *s, int variant);
 /* ms[v] (may be NULL, esch_n_variants() entries) receives each variant's DP
  * time on the sample (-1 for a variant the panel does not admit); the
  * fastest is selected and returned in *chosen.
- * Then, unless esch_set_diag_ratio fixed it, the per-length-bucket routing
- * ratio is swept over {off, 1.5, 2, 3, 4} on the same sample; another one
- * replaces the default 2.0 only when more than 2% faster
- * (esch_get_diag_ratio reads the result). */
+ * Then, unless esch_set_diag_ratio or esch_set_diag_plan fixed it, the
+ * routing ratio is swept over {off, 1.5, 2, 3, 4} on the same sample
+ * (another one replaces the default 2.0 only when more than 2% faster), and
+ * then, for each length class the sample holds reads of, the class's plan
+ * entry over {inter-sequence, anti-diagonal, 1.5, 2, 3, 4}, kept when more
+ * than 2% faster (esch_get_diag_ratio / esch_get_diag_plan read the
+ * result). The variants include the stripe heights (16, 24, 32 rows), so
+ * the choice of variant is the tile shape's. */
 esch_status esch_autotune(esch_scorer *s, const uint8_t *const *reads, const size_t *lens, size_t n,
                           int both_strands, float *ms, int *chosen);
 

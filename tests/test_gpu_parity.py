@@ -404,19 +404,23 @@ F16_VARIANTS = (4, 5, 7, 8)
 def test_row_shifted_variants_at_their_bound():
     """Variants 6 (int16), 7 and 8 (fp16) hold every value of a stripe row r
     shifted by |e| r: exact while longest reference x M + |o| + 17 |e| stays
-    within the arithmetic's bound (2048 for fp16, 32000 for int16). Panels
+    within the arithmetic's bound (2048 for fp16, 32000 for int16); the same
+    schedules on 24- and 32-row stripes (9-14) need 25 |e| and 33 |e|. Panels
     exactly at it, with reads that reach both ends of the value range (a
     reference read back whole, runs of one base against a reference of
     another, reads several stripes long), equal the oracle in semiglobal
     mode; one base longer, the panel refuses the variant (semiglobal only:
     in local mode the variants are the unshifted 3 and 5)."""
     rng = G.Rng(0x5EED_0A04)
-    # (scoring, L, variant): L x M + |o| + 17 |e| equals the bound
-    for scoring, L, v in (((1, -1, -2, -1), 2029, 7), ((2, -1, -11, -3), 993, 8),
-                          ((100, -100, -200, -100), 301, 6)):
+    # (scoring, L, variants, stripe rows): L x M + |o| + (rows + 1) |e|
+    # equals the bound; the tall-stripe variants 9-14 on 24 and 32 rows
+    for scoring, L, others, st in (((1, -1, -2, -1), 2029, (7, 8), 16), ((2, -1, -11, -3), 993, (7, 8), 16),
+                                   ((100, -100, -200, -100), 301, (6,), 16),
+                                   ((1, -1, -2, -1), 2021, (9, 10), 24), ((100, -100, -200, -100), 293, (11,), 24),
+                                   ((2, -1, -11, -3), 969, (12, 13), 32), ((100, -100, -200, -100), 285, (14,), 32)):
         M = max(abs(scoring[0]), abs(scoring[1]))
-        assert L * M + abs(scoring[2]) + 17 * abs(scoring[3]) == (32000 if v == 6 else 2048)
-        others = (7, 8) if v != 6 else (6,)
+        f16 = others[0] not in (6, 11, 14)
+        assert L * M + abs(scoring[2]) + (st + 1) * abs(scoring[3]) == (2048 if f16 else 32000)
         refs = [G.random_seq(rng, L, 0.0), b"C" * L, G.random_seq(rng, L // 2, 0.02),
                 G.random_seq(rng, 40, 0.0)]
         reads = [refs[0], b"A" * L, b"A" * (L // 3), b"G" * 5, refs[0][L // 9:L - L // 9],
@@ -443,7 +447,7 @@ def test_row_shifted_variants_at_their_bound():
 
 
 def test_column_shifted_local_variants_at_their_bound():
-    """In local mode variants 6, 7 and 8 hold every value of reference
+    """In local mode variants 6, 7 and 8 (and 9-14) hold every value of reference
     column j shifted by |e| j: exact while longest reference x (M + |e|) +
     |o| + 3 |e| + M stays within 2048 (fp16) or 32000 (int16). Panels
     exactly at it, with reads reaching the top score (a reference read back
@@ -452,11 +456,13 @@ def test_column_shifted_local_variants_at_their_bound():
     refuses the variant while semiglobal mode (its row-shift bound) keeps
     it."""
     rng = G.Rng(0x5EED_0A05)
-    for scoring, L, vs in (((1, -1, -2, -1), 1021, (7, 8)), ((2, -1, -12, -3), 405, (7, 8)),
-                           ((100, -100, -200, -100), 157, (6,))):
+    # the schedules on 24- and 32-row stripes (9-14) shift columns the same way
+    f16, i16 = (7, 8, 9, 10, 12, 13), (6, 11, 14)
+    for scoring, L, vs in (((1, -1, -2, -1), 1021, f16), ((2, -1, -12, -3), 405, f16),
+                           ((100, -100, -200, -100), 157, i16)):
         M = max(abs(scoring[0]), abs(scoring[1]))
         e = abs(scoring[3])
-        assert L * (M + e) + abs(scoring[2]) + 3 * e + M == (32000 if vs == (6,) else 2048)
+        assert L * (M + e) + abs(scoring[2]) + 3 * e + M == (32000 if vs == i16 else 2048)
         refs = [G.random_seq(rng, L, 0.0), b"C" * L, G.random_seq(rng, L // 2, 0.02),
                 G.random_seq(rng, 40, 0.0)]
         reads = [refs[0], b"A" * L, b"A" * (L // 3), b"G" * 5, refs[0][L // 9:L - L // 9],
@@ -810,6 +816,51 @@ def test_autotune_sweeps_the_routing_ratio():
     assert ok.all() and np.array_equal(got, oracle(queries, panel.codes, (1, -1, -2, -1), ea.Mode.LOCAL))
 
 
+def test_autotune_per_length_bucket_plan():
+    """The routing plan per length class (0: stay on the inter-sequence
+    kernel, 1: take the anti-diagonal kernel, 16..1024: the ratio rule):
+    reads in six classes (50 nt to 4 kb); every plan tried gives the
+    oracle's scores, a plan of 1s sends exactly the reads of 128 nt and
+    over to the anti-diagonal kernel, the autotuner returns a valid plan
+    (scores again equal), and a plan the caller set is left alone."""
+    rng = G.Rng(0x5EED_0B05)
+    panel_l = G.random_panel(rng, 30, 150, 400)
+    spans = ((50, 127, 600), (128, 255, 300), (256, 511, 150), (512, 1023, 60), (1024, 2047, 20),
+             (2048, 4095, 6))
+    reads = [G.chimeric_read(rng, panel_l, lo + rng.below(hi - lo + 1)) for lo, hi, k in spans for _ in range(k)]
+    queries = codes_of(reads)
+    panel = ea.Panel(panel_l)
+    scoring = (2, -1, -10, -1)
+    want = oracle(queries, panel.codes, scoring, ea.Mode.SEMIGLOBAL)
+    gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), ea.Mode.SEMIGLOBAL)
+    long_reads = sum(len(q) >= 128 for q in queries)
+    for plan in ([0] * 8, [1] * 8, [32, 1, 0, 24, 1, 64, 32, 32], [0, 0, 0, 1, 1, 1, 1, 1], [16, 1024, 48, 0, 16, 1, 0, 1]):
+        gpu.diag_plan = plan
+        assert gpu.diag_plan == plan
+        b = gpu.batch(len(queries), sum(len(q) for q in queries))
+        b.load(queries)
+        b.run(ea.RUN_SCORES32)
+        got, ok = b.scores32()
+        nd = b.n_diag()
+        b.close()
+        assert ok.all() and np.array_equal(got, want), plan
+        if plan == [1] * 8:
+            assert nd == long_reads, (nd, long_reads)
+    for bad in ([2] + [0] * 7, [0] * 7, [2000] * 8):
+        with pytest.raises(ea.GpuError):
+            gpu.diag_plan = bad
+    tuned = ea.GpuScorer(panel, ea.Scoring(*scoring), ea.Mode.SEMIGLOBAL)
+    tuned.autotune(queries, both_strands=True)
+    plan = tuned.diag_plan
+    assert len(plan) == 8 and all(v in (0, 1) or 16 <= v <= 1024 for v in plan), plan
+    got, ok = tuned.score_batch_i32(queries)
+    assert ok.all() and np.array_equal(got, want), plan
+    fixed = [0, 1, 0, 1, 0, 1, 0, 1]
+    tuned.diag_plan = fixed
+    tuned.autotune(queries[:300], both_strands=True)
+    assert tuned.diag_plan == fixed
+
+
 @pytest.mark.gpu
 def test_short_reads_take_fp16_on_a_panel_of_long_references():
     """Every fp16 and row-shifted value is bounded through min(read,
@@ -845,13 +896,16 @@ def test_short_reads_take_fp16_on_a_panel_of_long_references():
         # the autotuner's pick (among the int16 variants this panel admits)
         # is not the caller's: a batch of short reads still takes fp16
         chosen, _ = gpu.autotune(codes_of(short), both_strands=False)
-        assert chosen in (0, 1, 2, 3, 6)
+        assert chosen in (0, 1, 2, 3, 6, 11, 14)  # the int16 variants
+        # semiglobal: the row-shifted fp16 schedule on the pick's stripes
+        rows = {11: 24, 14: 32}.get(chosen, 16)
+        up = {16: 7, 24: 9, 32: 12}[rows] if mode == ea.Mode.SEMIGLOBAL else 4
         q = codes_of(short)
         b = gpu.batch(len(q), sum(len(x) for x in q))
         b.load(q)
         b.run(ea.RUN_SCORES32)
         b.wait()
-        assert b.variant() == fast, (mode, chosen, b.variant())
+        assert b.variant() == up, (mode, chosen, b.variant())
         assert np.array_equal(b.scores32()[0], oracle(q, panel.codes, scoring, mode))
         gpu.variant = 6
         q = codes_of(short)

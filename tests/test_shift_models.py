@@ -1,4 +1,4 @@
-"""The shifted value spaces of k_dp16's variants 6-8 (eschalign_amd/csrc/
+"""The shifted value spaces of k_dp16's variants 6-14 (eschalign_amd/csrc/
 kernels.hip: column16s / column16s_pair with the stripe rebase in semiglobal
 mode, column16c / column16c_pair in local mode), modelled step for step in
 Python on the CPU.
@@ -47,12 +47,13 @@ class Range:
         return v[0] if len(v) == 1 else v
 
 
-def plan(nmax):
-    """k_dp16's row plan for a chunk whose longest read is nmax: S stripes,
-    the first of RR rows (an even count), T rows in all."""
-    S = (nmax + 15) // 16
-    RR = ((nmax - 16 * (S - 1) + 1) // 2) * 2
-    return S, RR, RR + 16 * (S - 1)
+def plan(nmax, st=16):
+    """k_dp16's row plan for a chunk whose longest read is nmax, on stripes
+    of st rows (16; 24 or 32 for variants 9-14): S stripes, the first of RR
+    rows (an even count), T rows in all."""
+    S = (nmax + st - 1) // st
+    RR = ((nmax - st * (S - 1) + 1) // 2) * 2
+    return S, RR, RR + st * (S - 1)
 
 
 def subst(x, c, ma, mi):
@@ -61,14 +62,14 @@ def subst(x, c, ma, mi):
     return ma if (x == c or x == 4 or c == 4) else mi
 
 
-def walk_semiglobal_rows(q, r, scoring, nmax, rng_vals):
+def walk_semiglobal_rows(q, r, scoring, nmax, rng_vals, st=16):
     """Variant 7's semiglobal walk: row r of a stripe holds P = H - e r + o,
     R = E - e r + e, Q = F - e r + e; the row above arrives in the previous
     stripe's space, lower by c = e h, added to Q at row 0 and to the
     profile's row 0; bests in H + o space (the last column's rows through
     + e r, the last stripe's bottom row through + e (RR - 1))."""
     ma, mi, o, e = scoring
-    S, RR, T = plan(nmax)
+    S, RR, T = plan(nmax, st)
     rows = [None] * (T - len(q)) + list(q)
     pad = T - len(q)
     m = len(r)
@@ -76,8 +77,8 @@ def walk_semiglobal_rows(q, r, scoring, nmax, rng_vals):
     bottom = None
     row0 = 0
     for s in range(S):
-        h = RR if s == 0 else 16
-        c = 0 if s == 0 else e * (RR if s == 1 else 16)
+        h = RR if s == 0 else st
+        c = 0 if s == 0 else e * (RR if s == 1 else st)
         Pv = [o - e * k for k in range(h)]
         R = [NEG] * h
         hdt = o + e - c
@@ -117,21 +118,21 @@ def walk_semiglobal_rows(q, r, scoring, nmax, rng_vals):
     return best - o
 
 
-def walk_local_columns(q, r, scoring, nmax, rng_vals):
+def walk_local_columns(q, r, scoring, nmax, rng_vals, st=16):
     """Variant 7's local walk: column j holds P = H - e j + o,
     R = E - e j + e, Q = F - e j + e, the zero floor is z_j = e (1 - j), and
     the best is carried in the space of the last column done (two columns a
     pair: best - 2e against the pair's tA - e and tB), back to H + o at the
     stripe's end."""
     ma, mi, o, e = scoring
-    S, RR, T = plan(nmax)
+    S, RR, T = plan(nmax, st)
     rows = [None] * (T - len(q)) + list(q)
     m = len(r)
     best = o  # H = 0
     bottom = None
     row0 = 0
     for s in range(S):
-        h = RR if s == 0 else 16
+        h = RR if s == 0 else st
         Pv = [o + e] * h
         R = [NEG] * h
         hdt = o + e
@@ -178,12 +179,12 @@ def walk_local_columns(q, r, scoring, nmax, rng_vals):
     return best - o
 
 
-def cases(seed, n_cases):
+def cases(seed, n_cases, n_max=60):
     rng = random.Random(seed)
     schemes = G.SCHEMES_GPU + [(1, -3, -1, -1), (2, -3, -5, -2)]
     for _ in range(n_cases):
         sc = schemes[rng.randrange(len(schemes))]
-        n, m = rng.randint(1, 60), rng.randint(1, 24)
+        n, m = rng.randint(1, n_max), rng.randint(1, 24)
         q = [rng.randrange(5) if rng.random() < 0.1 else rng.randrange(4) for _ in range(n)]
         ref = [rng.randrange(5) if rng.random() < 0.1 else rng.randrange(4) for _ in range(m)]
         if rng.random() < 0.3:  # a read that matches the reference: the top of the range
@@ -197,29 +198,31 @@ def bounds(scoring):
     return max(abs(ma), abs(mi)), abs(o), abs(e)
 
 
-@pytest.mark.parametrize("seed", [1, 2])
-def test_row_shifted_semiglobal_walk_equals_oracle(seed):
-    """The row shift (semiglobal): scores equal the oracle's, and every
-    value lies in [-(kM + |o| + 4|e|), kM + 16|e|] with k the read-reference
-    overlap bound min(n, m), the range shifted_bound rests on."""
-    for sc, q, ref, nmax in cases(seed, 150):
+@pytest.mark.parametrize("seed,st", [(1, 16), (2, 16), (6, 24), (7, 32)])
+def test_row_shifted_semiglobal_walk_equals_oracle(seed, st):
+    """The row shift (semiglobal), on stripes of 16 rows and of the tall
+    variants' 24 and 32: scores equal the oracle's, and every value lies in
+    [-(kM + |o| + 4|e|), kM + st |e|] with k the read-reference overlap
+    bound min(n, m), the range shifted_bound(..., st) rests on."""
+    for sc, q, ref, nmax in cases(seed, 150, n_max=4 * st):
         vals = Range()
-        got = walk_semiglobal_rows(q, ref, sc, nmax, vals)
+        got = walk_semiglobal_rows(q, ref, sc, nmax, vals, st)
         want = O.score(np.array(q, np.uint8), np.array(ref, np.uint8), sc, False)
-        assert got == want, (sc, q, ref, nmax)
+        assert got == want, (sc, q, ref, nmax, st)
         M, O_, E = bounds(sc)
         k = min(len(q), len(ref))
-        assert -(k * M + O_ + 4 * E) <= vals.lo and vals.hi <= k * M + 16 * E, (sc, vals.lo, vals.hi, k)
+        assert -(k * M + O_ + 4 * E) <= vals.lo and vals.hi <= k * M + st * E, (sc, vals.lo, vals.hi, k, st)
 
 
-@pytest.mark.parametrize("seed", [3, 4])
-def test_column_shifted_local_walk_equals_oracle(seed):
-    """The column shift (local): scores equal the oracle's, and every value
-    lies in [-(|o| + 3|e| + M), kM + |e| (m + 2) + M], m the reference
-    length, the range colshift_bound rests on."""
-    for sc, q, ref, nmax in cases(seed, 150):
+@pytest.mark.parametrize("seed,st", [(3, 16), (4, 16), (8, 24), (9, 32)])
+def test_column_shifted_local_walk_equals_oracle(seed, st):
+    """The column shift (local), on 16-, 24- and 32-row stripes: scores
+    equal the oracle's, and every value lies in [-(|o| + 3|e| + M),
+    kM + |e| (m + 2) + M], m the reference length, the range colshift_bound
+    rests on (whatever the stripe height)."""
+    for sc, q, ref, nmax in cases(seed, 150, n_max=4 * st):
         vals = Range()
-        got = walk_local_columns(q, ref, sc, nmax, vals)
+        got = walk_local_columns(q, ref, sc, nmax, vals, st)
         want = O.score(np.array(q, np.uint8), np.array(ref, np.uint8), sc, True)
         assert got == want, (sc, q, ref, nmax)
         M, O_, E = bounds(sc)
