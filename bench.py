@@ -62,6 +62,8 @@ VALU_OPS_PER_CELL_ISSUED = {("semiglobal", 3): round(344_021_737_036 * 128 / 5_3
                             ("semiglobal", 5): round(301_256_340_206 * 128 / 5_303_272_453_076, 3),
                             ("semiglobal", 6): round(306_294_990_254 * 128 / 5_303_272_453_076, 3),
                             ("semiglobal", 7): round(263_781_725_920 * 128 / 5_303_272_453_076, 3),
+                            ("semiglobal", 10): round(262_088_845_368 * 128 / 5_303_272_453_076, 3),
+                            ("semiglobal", 11): round(303_721_597_544 * 128 / 5_303_272_453_076, 3),
                             ("local", 6): round(389_701_369_198 * 128 / 5_303_272_453_076, 3),
                             ("local", 7): round(327_569_070_244 * 128 / 5_303_272_453_076, 3)}
 
@@ -402,7 +404,7 @@ def config1_leg(steps, cpu):
     return out
 
 
-def config3_leg(n_reads, cpu, cpu_reads=2000):
+def config3_leg(n_reads, cpu, cpu_reads=2000, variant=-1):
     """The large-panel workload: 10k references of 200-1000 nt (2k seeds
     plus 4 copies of each at 2% substitutions, so ties and near-ties are
     common), n_reads reads of 100-1000 nt, local 2,-1,-10,-1, forward
@@ -423,7 +425,11 @@ def config3_leg(n_reads, cpu, cpu_reads=2000):
     gpu = ea.GpuScorer(ea.Panel(panel), ea.Scoring(*scoring), ea.Mode.LOCAL)
     sample = [codes[int(off[k]):int(off[k]) + int(lens[k])] for k in range(min(n_reads, 500))]
     t0 = time.perf_counter()
-    chosen, tune_ms = gpu.autotune(sample, both_strands=False)
+    if variant >= 0:  # fixed (profiling): no autotuning dispatches
+        gpu.variant = variant
+        chosen, tune_ms = variant, []
+    else:
+        chosen, tune_ms = gpu.autotune(sample, both_strands=False)
     tune_s = time.perf_counter() - t0
     batch = gpu.batch(n_reads, int(lens.sum()))
     batch.load_concat(codes, off, lens, both_strands=False)
@@ -672,7 +678,7 @@ def main():
         print(json.dumps(config1_leg(args.steps, not args.no_cpu)), flush=True)
         return
     if args.only == "config3":
-        print(json.dumps(config3_leg(args.config3_reads, not args.no_cpu)), flush=True)
+        print(json.dumps(config3_leg(args.config3_reads, not args.no_cpu, variant=args.variant)), flush=True)
         return
     if args.only == "serving":
         print(json.dumps(serving(args.serve_rate, args.serve_seconds)), flush=True)
