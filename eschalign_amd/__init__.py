@@ -76,7 +76,7 @@ EXPORTS = [
     "esch_mapped_pairs", "esch_mapped_hits", "esch_mapped_ops", "esch_mapped_stats", "esch_mapped_free",
     "esch_set_map_chunk", "esch_adapter_trim_pairs", "esch_md_nm", "esch_cigar_string",
     "esch_map_reads_concat", "esch_align_last_wavefront", "esch_score_device_noalloc",
-    "esch_set_diag_plan", "esch_get_diag_plan",
+    "esch_set_diag_plan", "esch_get_diag_plan", "esch_bam_build_records",
 ]
 
 
@@ -248,6 +248,8 @@ def load_library():
     L.esch_md_nm.argtypes = [C.c_char_p, sz, C.c_char_p, sz, sz, sz, vp, sz, C.c_char_p, sz, C.POINTER(C.c_uint32)]
     L.esch_md_nm.restype = sz
     L.esch_cigar_string.argtypes = [vp, sz, sz, sz, sz, C.c_char_p, sz]
+    L.esch_bam_build_records.argtypes = [C.c_char_p, sz, sz, vp, vp, sz, vp, sz, vp, vp, vp, sz, C.c_int, vp, sz,
+                                         C.POINTER(sz), C.POINTER(sz)]
     L.esch_cigar_string.restype = sz
     _lib = L
     return L

@@ -413,6 +413,31 @@ size_t esch_md_nm(const uint8_t *read, size_t n, const uint8_t *reference, size_
 size_t esch_cigar_string(const uint32_t *ops, size_t n_ops, size_t query_len, size_t query_start, size_t query_end,
                          char *out, size_t cap);
 
+/* ---- output records (crates/eschalign-cli/src/align.rs:1084-1196; host code) ----
+ * build_records over the mappings of an unpaired run, each output record
+ * laid out as a BAM alignment record (block_size, then the fixed fields,
+ * name, CIGAR, sequence, qualities and data; SAM/BAM specification 4.2), as
+ * the CLI writes them. `in` holds the n input records as an uncompressed BAM
+ * record stream (each a u32 block_size and the record), in input order,
+ * primary records only; maps[k] is read k's mapping (hits / ops as
+ * esch_mapped holds them: esch_map_reads on the reads' codes, a read the
+ * caller keeps from alignment, such as one over --max-read-len, as an
+ * unmapped record with n_hits = 0). References are the panel's letters (as
+ * in the FASTA) and names. Each record keeps its input's name, sequence,
+ * qualities, QC-fail / duplicate flags and the data fields the run does not
+ * own; NM, MD, AS, XS, XA are rewritten (removed, the record's last field
+ * taking a removed one's place, then appended); a reverse-strand primary
+ * carries the reverse-complemented sequence and reversed qualities; MAPQ is
+ * 60 for a unique best and 0 for a tie; with secondary set, each other hit
+ * follows as a 0x100 record (MAPQ 0, no sequence, NM / MD / AS / RG). The
+ * records go to out (*out_len bytes, *n_records records); ESCH_CAPACITY
+ * when out_cap is smaller (*out_len says how much is needed). */
+esch_status esch_bam_build_records(const uint8_t *in, size_t in_bytes, size_t n, const esch_read_mapping *maps,
+                                   const esch_hit *hits, size_t n_hits, const uint32_t *ops, size_t n_ops,
+                                   const uint8_t *const *ref_letters, const size_t *ref_lens,
+                                   const char *const *ref_names, size_t n_refs, int secondary, uint8_t *out,
+                                   size_t out_cap, size_t *out_len, size_t *n_records);
+
 /* Kernel variants of the batched DP, and the autotuner that picks one by
  * timing each on a sample of the caller's reads (no counterpart in the
  * reference; the scores are identical whichever variant runs):

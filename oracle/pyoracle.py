@@ -308,9 +308,10 @@ def pair_form(p, hits, ops):
 
 
 def map_reads(reads, refs, scoring, local: bool, min_score=0, max_ties=None, both_strands=False,
-              threads: int | None = None, rows=None):
+              threads: int | None = None, rows=None, raw=False):
     """Aligner::map_reads_scored (lib/mapper.rs:400-550) on codes: one
-    read_form per read. rows = (fwd, rev) panel scores if already computed."""
+    read_form per read. rows = (fwd, rev) panel scores if already computed.
+    raw: the (records, hits, ops) arrays instead, in the library's layout."""
     threads = threads or min(8, os.cpu_count() or 1)
     L = lib()
     reads = [_codes(r) for r in reads]
@@ -331,6 +332,8 @@ def map_reads(reads, refs, scoring, local: bool, min_score=0, max_ties=None, bot
         recs, hits, ops = _mapped_arrays(L, m, False)
     finally:
         L.eo_mapped_free(m)
+    if raw:
+        return recs, hits, ops
     return [read_form(r, hits, ops) for r in recs]
 
 
