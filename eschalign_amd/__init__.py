@@ -89,7 +89,8 @@ class ServerConfig(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("completed", C.c_uint64), ("batches", C.c_uint64),
                 ("queued_units", C.c_size_t), ("hbm_in_use", C.c_size_t),
-                ("hbm_peak", C.c_size_t), ("slots", C.c_size_t), ("busy_slots", C.c_size_t)]
+                ("hbm_peak", C.c_size_t), ("slots", C.c_size_t), ("busy_slots", C.c_size_t),
+                ("swapped_batches", C.c_uint64)]
 
 
 class MapStats(C.Structure):

@@ -106,7 +106,7 @@ struct esch_server {
     std::vector<std::unique_ptr<Slot>> slots;
     uint64_t next_id = 1;
     size_t hbm_in_use = 0, hbm_peak = 0;  // slot footprints, counted from reservation to release
-    uint64_t completed = 0, batches = 0;
+    uint64_t completed = 0, batches = 0, swapped_batches = 0;
     bool stop = false;
     std::thread th;
 
@@ -315,6 +315,7 @@ struct esch_server {
                                         q.tie_cap);
                 if (st) err = esch_last_error();
             }
+            const bool swapped = !st && esch_batch_swapped(s->batch);
             lk.lock();
             if (st) {
                 for (auto &u : take) finish_unit(u, st, err);
@@ -331,6 +332,7 @@ struct esch_server {
             s->units = std::move(take);
             s->state = SlotState::Busy;
             ++batches;
+            swapped_batches += swapped ? 1 : 0;
             any = true;
         }
     }
@@ -519,6 +521,7 @@ esch_status esch_server_stats(esch_server *s, esch_server_stats_t *out) {
     out->hbm_peak = s->hbm_peak;
     out->slots = s->slots.size();
     out->busy_slots = (size_t)s->in_flight();
+    out->swapped_batches = s->swapped_batches;
     return ESCH_OK;
 }
 

@@ -585,6 +585,7 @@ typedef struct {
     size_t hbm_peak;         /* most bytes ever held by slots */
     size_t slots;            /* slots allocated */
     size_t busy_slots;       /* slots with a batch in flight */
+    uint64_t swapped_batches; /* batches scored with the roles swapped (esch_set_swap_rows) */
 } esch_server_stats_t;
 
 esch_status esch_server_new(const esch_server_config *cfg, esch_server **out);

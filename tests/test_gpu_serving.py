@@ -109,3 +109,60 @@ def test_small_request_overtakes_large_one():
     assert res_small == alone.select_batch(small, both_strands=True, tie_cap=4)
     assert res_big == alone.select_batch(big, both_strands=True, tie_cap=4)
     server.close()
+
+
+def test_config5_real_panels_and_swapped_roles():
+    """The serving configuration's own panels: 164, 1,000 and 10,000
+    references of 140-1000 nt, semiglobal 1,-1,-2,-1, reads lognormal(90,
+    0.6) clipped 20-1000, Poisson arrivals, request sizes log-uniform. On
+    the 10,000-reference panel a device batch of at most one row per
+    reference is scored with the roles swapped (the panel's references as
+    the queries); the server takes that path for the small requests, and
+    every served result equals the same request run alone on a scorer with
+    swapping off (the ordinary path), and a sample equals the CPU kernel."""
+    scoring = ea.Scoring(1, -1, -2, -1)
+    mode = ea.Mode.SEMIGLOBAL
+    panels = [synth.panel(seed=0x5E4E + k, n_refs=n, lo=140, hi=1000) for k, n in enumerate((164, 1000, 10000))]
+    server = ea.Server(hbm_budget=64 << 30, max_batch_reads=50_000, max_batch_bases=8_000_000, max_slots=2,
+                       max_batch_cells=1 << 38)
+    ids = [server.add_panel(ea.Panel(p), scoring, mode) for p in panels]
+    alone = [ea.GpuScorer(ea.Panel(p), scoring, mode) for p in panels]
+    for a in alone:
+        a.swap_rows = 0
+    rng = random.Random(0x5E4E)
+    requests = []
+    for r in range(36):
+        pid = r % 3 if r < 6 else rng.randrange(3)
+        n = int(round(np.exp(rng.uniform(0, np.log(4000)))))
+        reads = synth.reads(panels[pid], seed=0x5E4E + 100 + r, n=n)
+        requests.append((pid, reads))
+    submitted = []
+    for pid, reads in requests:
+        time.sleep(rng.expovariate(1 / 0.01))
+        submitted.append(server.submit(ids[pid], reads, both_strands=True, tie_cap=8))
+    results = []
+    for rid in submitted:
+        assert server.wait(rid, timeout_ms=180_000)
+        results.append(server.result(rid)[0])
+    st = server.stats()
+    assert st["completed"] == len(requests)
+    assert st["swapped_batches"] > 0, st
+    print(f"[serving] config-5 panels: {st['batches']} batches, {st['swapped_batches']} with the roles swapped")
+    for k, ((pid, reads), res) in enumerate(zip(requests, results)):
+        assert res == alone[pid].select_batch(reads, both_strands=True, tie_cap=8), (k, pid, len(reads))
+    # a sample of the 10k-panel requests against the CPU kernel
+    sup = O.simd_supported()
+    for k, (pid, reads) in enumerate(requests):
+        if pid != 2:
+            continue
+        q = reads[:12]
+        rows = q + [ea.reverse_complement_codes(x) for x in q]
+        prof = O.Profile(panels[2], scoring.tuple(), 32 if sup & 2 else 16)
+        sm = prof.score_batch(*O.concat(rows), False, min(16, os.cpu_count() or 1))
+        n = len(q)
+        for i in range(n):
+            b, nt, so, take = O.select(sm[i], sm[n + i], 0, None)
+            r = results[k][i]
+            assert (r.best, r.n_tied, r.suboptimal, r.take) == (b, nt, so, take[:8]), (k, i)
+        break
+    server.close()
