@@ -148,8 +148,9 @@ def trace_leg(gpu, batch, panel_codes, codes, off, lens, scoring, local, n_reads
     """The winner tracebacks of the first n_reads reads: every selected
     (read or reverse complement, reference) of the step's fused selection,
     traced on the device (esch_align_pairs_concat); on a bounded sample,
-    the oracle's scalar restatement of the reference's traceback on the host
-    cores, checked equal (score, coordinates, runs)."""
+    the oracle's restatement of the reference's AVX-512BW pair traceback
+    (simd::align_pairs) on the host cores, checked equal (score,
+    coordinates, runs)."""
     recs, ties = batch.select_raw()
     nr = min(n_reads, len(recs))
     take = np.where((recs[:nr, 3] & 1) != 0, np.minimum(recs[:nr, 4], ties.shape[1]), 0)
@@ -179,19 +180,21 @@ def trace_leg(gpu, batch, panel_codes, codes, off, lens, scoring, local, n_reads
         threads = min(16, os.cpu_count() or 1)
         m = min(len(refs), 2000)
         t = time.perf_counter()
-        O.align_batch_concat(big, qoff[:m], qlen[:m], panel_codes, refs[:m], scoring, local, threads)
+        O.simd_align_batch_concat(big, qoff[:m], qlen[:m], panel_codes, refs[:m], scoring, local, threads)
         dt = time.perf_counter() - t
         m = int(min(len(refs), max(m, m * 8.0 / max(dt, 1e-3))))  # about 8 s of CPU work
         t = time.perf_counter()
-        ref = O.align_batch_concat(big, qoff[:m], qlen[:m], panel_codes, refs[:m], scoring, local, threads)
+        ref = O.simd_align_batch_concat(big, qoff[:m], qlen[:m], panel_codes, refs[:m], scoring, local, threads)
         dt = time.perf_counter() - t
+        kernel = "AVX-512BW pair kernel" if O.simd_supported() & 2 else "scalar (no AVX-512BW on this host)"
         assert np.array_equal(ref, al[:m, :6].astype(np.int32)), "device traceback != CPU traceback"
         cpu_cells = int((ln[:m] * rlen[refs[:m]]).sum())
         out["cpu_baseline"] = {"pairs_per_s": round(m / dt, 1), "cells_per_s": round(cpu_cells / dt, 1),
                                "threads": threads, "kind": "port",
-                               "sample": f"first {m} traced pairs, scalar traceback restated from "
-                                         f"lib/scalar.rs:164-364 (the reference's AVX-512 traceback "
-                                         f"kernel is not restated), equal on every pair"}
+                               "sample": f"first {m} traced pairs, simd::align_pairs with the "
+                                         f"{kernel} restated from lib/simd/mod.rs:405-499 and "
+                                         f"lib/simd/avx512.rs:120-247 (chunks of 64 pairs, 32 lanes, "
+                                         f"the scalar walk), equal on every pair"}
     return out
 
 
@@ -327,8 +330,9 @@ def map_reads_leg(gpu, panel_codes, codes, off, lens, scoring, local, n_reads, c
         out["cpu_baseline"] = {"reads_per_s": round(m / dt, 1), "threads": threads, "kind": "port",
                                "sample": f"first {m} reads: both strands scored by the "
                                          f"{'AVX-512BW' if lanes == 32 else 'AVX2'} restatement of the "
-                                         f"reference CPU kernel, then map_reads_scored restated with scalar "
-                                         f"tracebacks (lib/mapper.rs:400-550), equal on every read"}
+                                         f"reference CPU kernel, then map_reads_scored restated with the "
+                                         f"winners of each 64 reads traced together by the AVX-512BW pair "
+                                         f"kernel (lib/mapper.rs:400-550), equal on every read"}
     return out
 
 

@@ -147,6 +147,83 @@ static void map_read(const ctx *c, const uint8_t *q, size_t n, const int32_t *fw
     free(ties);
 }
 
+/* map_reads_scored over a chunk of reads (lib/mapper.rs:400-553), as the
+ * CLI's workers run it on 64 reads at a time: every read's selection, then
+ * every taken tie of the chunk traced in one simd::align_pairs call (the
+ * AVX-512BW pair kernel, eo_simd_align_chunk), then the hits. Equal to
+ * map_read per read: the pair kernel equals eo_align pair for pair. */
+static void map_read_chunk(const ctx *c, eo_pair_scratch *ps, const uint8_t *const *qs, const size_t *ns,
+                           const int32_t *const *fwd, const int32_t *const *rev, size_t cnt, acc *a,
+                           eo_read_mapping *out) {
+    const size_t nr = c->n_refs;
+    int32_t *ties = (int32_t *)malloc(cnt * 2 * nr * sizeof(int32_t) + 1);
+    int32_t *take = (int32_t *)calloc(cnt + 1, sizeof(int32_t));
+    uint8_t **rcs = (uint8_t **)calloc(cnt + 1, sizeof(uint8_t *));
+    size_t n_pairs = 0;
+    for (size_t i = 0; i < cnt; i++) {
+        memset(&out[i], 0, sizeof out[i]);
+        if (ns[i] == 0) continue; /* best_score None (lib/mapper.rs:425-434) */
+        int32_t best, nt, hs, so;
+        take[i] = eo_select(fwd[i], rev[i], nr, c->min_score, c->max_ties, &best, &nt, &hs, &so, ties + i * 2 * nr,
+                            2 * nr);
+        out[i].best = best;
+        out[i].n_tied = nt;
+        out[i].subopt = hs ? so : 0;
+        out[i].flags = 1 | (hs ? 2 : 0);
+        if (take[i] > 0) {
+            rcs[i] = (uint8_t *)malloc(ns[i]);
+            eo_revcomp_codes(qs[i], ns[i], rcs[i]);
+            n_pairs += (size_t)take[i];
+        }
+    }
+    const uint8_t **pq = (const uint8_t **)malloc((n_pairs + 1) * sizeof(uint8_t *));
+    const uint8_t **pr = (const uint8_t **)malloc((n_pairs + 1) * sizeof(uint8_t *));
+    size_t *pql = (size_t *)malloc((n_pairs + 1) * sizeof(size_t)), *prl = (size_t *)malloc((n_pairs + 1) * sizeof(size_t));
+    size_t *poff = (size_t *)malloc((n_pairs + 1) * sizeof(size_t));
+    eo_alignment *al = (eo_alignment *)malloc((n_pairs + 1) * sizeof(eo_alignment));
+    int *runs = (int *)malloc((n_pairs + 1) * sizeof(int));
+    size_t k = 0, room = 0;
+    for (size_t i = 0; i < cnt; i++)
+        for (int32_t t = 0; t < take[i]; t++, k++) {
+            const int32_t tie = ties[i * 2 * nr + (size_t)t], ref = tie >> 1;
+            pq[k] = (tie & 1) ? rcs[i] : qs[i];
+            pql[k] = ns[i];
+            pr[k] = c->rcodes + c->roff[ref];
+            prl[k] = (size_t)c->rlen[ref];
+            poff[k] = room;
+            room += pql[k] + prl[k] + 1;
+        }
+    uint32_t *ops = (uint32_t *)malloc((room + 1) * sizeof(uint32_t));
+    eo_simd_align_chunk(ps, pq, pql, pr, prl, n_pairs, c->ma, c->mi, c->o, c->e, c->local, al, runs, ops, poff);
+    k = 0;
+    for (size_t i = 0; i < cnt; i++) {
+        out[i].hit_offset = (int64_t)a->n_hits;
+        if (take[i] <= 0) continue;
+        traced t0 = {al[k], runs[k], ops + poff[k]};
+        if (!is_empty(&t0)) {  /* the primary traces to something (lib/mapper.rs:508-512) */
+            for (int32_t t = 0; t < take[i]; t++) {
+                const int32_t tie = ties[i * 2 * nr + (size_t)t];
+                traced tt = {al[k + (size_t)t], runs[k + (size_t)t], ops + poff[k + (size_t)t]};
+                put_hit(a, &tt, tie >> 1, tie & 1, 0, 0);
+            }
+            out[i].n_hits = take[i];
+        }
+        k += (size_t)take[i];
+    }
+    for (size_t i = 0; i < cnt; i++) free(rcs[i]);
+    free(rcs);
+    free(take);
+    free(ties);
+    free(pq);
+    free(pr);
+    free(pql);
+    free(prl);
+    free(poff);
+    free(al);
+    free(runs);
+    free(ops);
+}
+
 static int mate_reverse(int m, int t) { return t ^ (m == 1); } /* lib/mapper/pair.rs:135-137 */
 
 /* pair_ok (lib/mapper/pair.rs:392-410) on two traced mates. */
@@ -282,6 +359,7 @@ typedef struct {
     acc a;
     eo_read_mapping *reads;
     eo_pair_mapping *prs;
+    eo_pair_scratch *ps;
 } job;
 
 static void *run_job(void *p) {
@@ -302,10 +380,24 @@ static void *run_job(void *p) {
             }
             map_pair(c, q, n, rows, j->both_strands, j->max_insert, &j->a, &j->prs[i]);
         } else {
-            map_read(c, j->codes + j->off[i], (size_t)j->len[i], j->fwd + i * nr, j->rev ? j->rev + i * nr : NULL,
-                     &j->a, &j->reads[i]);
+            /* chunks of 64 reads, their winners traced together */
+            const uint8_t *qs[64];
+            size_t ns[64];
+            const int32_t *fw[64], *rv[64];
+            const size_t cnt = j->i1 - i < 64 ? j->i1 - i : 64;
+            for (size_t t = 0; t < cnt; t++) {
+                qs[t] = j->codes + j->off[i + t];
+                ns[t] = (size_t)j->len[i + t];
+                fw[t] = j->fwd + (i + t) * nr;
+                rv[t] = j->rev ? j->rev + (i + t) * nr : NULL;
+            }
+            if (!j->ps) j->ps = eo_pair_scratch_new();
+            map_read_chunk(c, j->ps, qs, ns, fw, rv, cnt, &j->a, &j->reads[i]);
+            i += cnt - 1;
         }
     }
+    eo_pair_scratch_free(j->ps);
+    j->ps = NULL;
     return NULL;
 }
 

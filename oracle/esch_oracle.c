@@ -146,14 +146,26 @@ int eo_align(const uint8_t *q, size_t n, const uint8_t *r, size_t m, int32_t ma,
     }
     free(hcol);
     free(ecol);
+    const int nops = eo_walk_trace(trace, rows, 1, 0, best, bi, bj, out, ops, ops_cap);
+    free(trace);
+    return nops;
+}
+
+/* lib/scalar.rs:292-364 (walk): from end cell (bi, bj) of score `best`, the
+ * runs of the alignment through the traceback bytes, cell (i, j) at
+ * tb[(j * stride + i) * lanes + lane] (the scalar fill's one column per
+ * stride rows, lanes = 1; the SIMD pair kernel's 32 lanes per cell). Runs
+ * are collected last-first, then reversed; -1 when ops_cap is too small. */
+int eo_walk_trace(const uint8_t *tb, size_t stride, size_t lanes, size_t lane, int32_t best, size_t bi,
+                  size_t bj, eo_alignment *out, uint32_t *ops, size_t ops_cap) {
+    memset(out, 0, sizeof(*out));
     int nops = 0;
     if (bi != 0 && bj != 0) {
-        /* lib/scalar.rs:292-364: runs are collected last-first, then reversed */
         size_t i = bi, j = bj;
         int state = 0; /* 0 H, 1 F (insertion), 2 E (deletion) */
         uint32_t cur = 3u, len = 0;
         while (i > 0 && j > 0) {
-            const uint8_t t = trace[j * rows + i];
+            const uint8_t t = tb[(j * stride + i) * lanes + lane];
             uint32_t op = 3u;
             if (state == 0) {
                 const int src = t & 3;
@@ -181,7 +193,7 @@ int eo_align(const uint8_t *q, size_t n, const uint8_t *r, size_t m, int32_t ma,
                 len++;
             } else {
                 if (len) {
-                    if ((size_t)nops >= ops_cap) { free(trace); return -1; }
+                    if ((size_t)nops >= ops_cap) return -1;
                     ops[nops++] = cur << 30 | len;
                 }
                 cur = op;
@@ -189,7 +201,7 @@ int eo_align(const uint8_t *q, size_t n, const uint8_t *r, size_t m, int32_t ma,
             }
         }
         if (len) {
-            if ((size_t)nops >= ops_cap) { free(trace); return -1; }
+            if ((size_t)nops >= ops_cap) return -1;
             ops[nops++] = cur << 30 | len;
         }
         for (int a = 0, b = nops - 1; a < b; a++, b--) {
@@ -203,7 +215,6 @@ int eo_align(const uint8_t *q, size_t n, const uint8_t *r, size_t m, int32_t ma,
         out->ref_start = (int32_t)j;
         out->ref_end = (int32_t)bj;
     }
-    free(trace);
     return nops;
 }
 

@@ -80,12 +80,46 @@ int eo_align(const uint8_t *q, size_t n, const uint8_t *r, size_t m, int32_t ma,
              int32_t mi, int32_t o, int32_t e, int local, eo_alignment *out,
              uint32_t *ops, size_t ops_cap);
 
+/* lib/scalar.rs:292-364 — the walk from end cell (bi, bj) through
+ * traceback bytes tb[(j * stride + i) * lanes + lane]; shared by eo_align
+ * (lanes = 1) and the SIMD pair kernel (lanes = 32). */
+int eo_walk_trace(const uint8_t *tb, size_t stride, size_t lanes, size_t lane, int32_t best, size_t bi,
+                  size_t bj, eo_alignment *out, uint32_t *ops, size_t ops_cap);
+
 /* eo_align over pairs on `threads` workers; recs[k * 6] = score,
  * query_start, query_end, ref_start, ref_end, number of runs. */
 void eo_align_batch(const uint8_t *qcodes, const uint64_t *qoff, const int32_t *qlen,
                     const uint8_t *rcodes, const uint64_t *roff, const int32_t *rlen,
                     const int32_t *refs, size_t n, int32_t ma, int32_t mi, int32_t o,
                     int32_t e, int local, int threads, int32_t *recs);
+
+/* simd::align_pairs with the AVX-512BW pair kernel (lib/simd/mod.rs:384-499,
+ * lib/simd/avx512.rs:120-247): the pairs (read k against reference
+ * refs[k]) in chunks of 64 in input order, as the CLI's workers trace a
+ * chunk's winners; within a chunk the SIMD-eligible pairs (non-empty, both
+ * under 32,000, fits_i16) sorted by (read length, reference length) and cut
+ * into groups of 32 lanes, one pair per lane, each lane walked by the
+ * scalar walk; the others (and groups over 16 MiB of traceback) by
+ * eo_align. Chunks are dealt to `threads` workers. recs as eo_align_batch;
+ * ops (may be NULL) receives pair k's runs at ops + ops_off[k] (read +
+ * reference length + 1 slots each). Without AVX-512BW every pair takes
+ * eo_align. */
+void eo_simd_align_pairs(const uint8_t *qcodes, const uint64_t *qoff, const int32_t *qlen,
+                         const uint8_t *rcodes, const uint64_t *roff, const int32_t *rlen,
+                         const int32_t *refs, size_t n, int32_t ma, int32_t mi, int32_t o, int32_t e,
+                         int local, int threads, int32_t *recs, uint32_t *ops, const uint64_t *ops_off);
+
+/* The same on the calling thread for one call's pairs (a chunk of reads'
+ * taken ties, as map_reads_scored traces them, lib/mapper.rs:484-495): pair
+ * k is q[k] (ql[k] codes) against r[k] (rl[k]); al[k], runs[k], and its
+ * runs at ops + ops_off[k] (ql + rl + 1 slots). Buffers live in a scratch
+ * kept by the caller between calls. */
+typedef struct eo_pair_scratch eo_pair_scratch;
+eo_pair_scratch *eo_pair_scratch_new(void);
+void eo_pair_scratch_free(eo_pair_scratch *s);
+void eo_simd_align_chunk(eo_pair_scratch *s, const uint8_t *const *q, const size_t *ql, const uint8_t *const *r,
+                         const size_t *rl, size_t cnt, int32_t ma, int32_t mi, int32_t o, int32_t e, int local,
+                         eo_alignment *al, int *runs, uint32_t *ops, const size_t *ops_off);
 
 /* lib/simd/mod.rs:222-227 */
 int eo_fits_i16(size_t n, size_t max_ref_len, int32_t ma, int32_t mi,

@@ -12,6 +12,11 @@
  *   transposed kernel                    lib/simd/avx512.rs:278-354, avx2.rs:290-359
  *   dispatch (kernel_for, score_all)     lib/mapper.rs:216-222, 242-252
  *   score_scalar fallback                lib/simd/mod.rs:665-675
+ *   winner tracebacks (align_pairs,      lib/simd/mod.rs:384-499,
+ *     the AVX-512BW pair kernel)           lib/simd/avx512.rs:120-247
+ *
+ * The pair kernel is pinned to eo_align (itself pinned to the parasail
+ * goldens) pair for pair, runs included, in tests/test_oracle_align.py.
  */
 #include "esch_oracle.h"
 
@@ -422,6 +427,277 @@ void eo_simd_score_batch(const eo_profile *p, const uint8_t *qcodes,
         batch_job j = {p, qcodes, qoff, qlen, nq, local, out, &next, &mu};
         jobs[t] = j;
         pthread_create(&th[t], NULL, batch_worker, &jobs[t]);
+    }
+    for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+    free(th);
+    free(jobs);
+}
+
+/* ------------------------------------------------------------------------
+ * Winner tracebacks: simd::align_pairs (lib/simd/mod.rs:384-499) with the
+ * AVX-512BW pair kernel (lib/simd/avx512.rs:120-247). One (read, reference)
+ * pair per i16 lane, 32 lanes: the scalar fill's recurrence with wrapping
+ * i16 adds, its traceback byte per cell and lane, and each lane's end cell
+ * in the scalar path's scan order; every lane is then walked by
+ * eo_walk_trace, the scalar walk.
+ * ------------------------------------------------------------------------ */
+
+#define PAIR_LANES 32
+#define MAX_TRACE_BYTES ((size_t)16 << 20)
+
+__attribute__((target("avx512f,avx512bw"))) static void
+pairs512(int local, size_t n, size_t m, const int16_t *qlens, const int16_t *rlens, const int16_t *qt,
+         const int16_t *rt, int16_t ma, int16_t mi, int16_t go, int16_t ge, int16_t *hbuf, int16_t *ebuf,
+         uint8_t *tb, int16_t *best_out, int16_t *bi_out, int16_t *bj_out) {
+    const size_t stride = n + 1;
+    const __m512i zero = _mm512_setzero_si512(), neg = _mm512_set1_epi16(NEG16);
+    const __m512i wild = _mm512_set1_epi16(4), vm = _mm512_set1_epi16(ma), vx = _mm512_set1_epi16(mi);
+    const __m512i vo = _mm512_set1_epi16(go), ve = _mm512_set1_epi16(ge);
+    const __m512i t_diag = _mm512_set1_epi16(1), t_ins = _mm512_set1_epi16(2), t_del = _mm512_set1_epi16(3);
+    const __m512i t_eext = _mm512_set1_epi16(4), t_fext = _mm512_set1_epi16(8);
+    for (size_t i = 0; i < n; i++) {
+        _mm512_storeu_si512((void *)(hbuf + i * PAIR_LANES), zero);
+        _mm512_storeu_si512((void *)(ebuf + i * PAIR_LANES), neg);
+    }
+    const __m512i ql = _mm512_loadu_si512((const void *)qlens), rl = _mm512_loadu_si512((const void *)rlens);
+    __m512i best = local ? zero : neg, bi = zero, bj = zero;
+    for (size_t j = 0; j < m; j++) {
+        const __m512i rv = _mm512_loadu_si512((const void *)(rt + j * PAIR_LANES));
+        const __mmask32 rwild = _mm512_cmpeq_epi16_mask(rv, wild);
+        const __m512i jv = _mm512_set1_epi16((int16_t)(j + 1));
+        const __mmask32 jvalid = _mm512_cmpgt_epi16_mask(rl, _mm512_set1_epi16((int16_t)j));
+        const __mmask32 jlast = _mm512_cmpeq_epi16_mask(rl, jv);
+        uint8_t *tcol = tb + (j + 1) * stride * PAIR_LANES;
+        __m512i hdiag = zero, hup = zero, f = neg;
+        for (size_t i = 0; i < n; i++) {
+            const __m512i qv = _mm512_loadu_si512((const void *)(qt + i * PAIR_LANES));
+            const __m512i i0 = _mm512_set1_epi16((int16_t)i), iv = _mm512_set1_epi16((int16_t)(i + 1));
+            const __mmask32 same = _mm512_cmpeq_epi16_mask(qv, rv) | rwild | _mm512_cmpeq_epi16_mask(qv, wild);
+            const __m512i s = _mm512_mask_blend_epi16(same, vx, vm);
+            const __m512i hleft = _mm512_loadu_si512((const void *)(hbuf + i * PAIR_LANES));
+            const __m512i e_ext = _mm512_add_epi16(_mm512_loadu_si512((const void *)(ebuf + i * PAIR_LANES)), ve);
+            const __m512i ev = _mm512_max_epi16(_mm512_add_epi16(hleft, vo), e_ext);
+            _mm512_storeu_si512((void *)(ebuf + i * PAIR_LANES), ev);
+            const __m512i f_ext = _mm512_add_epi16(f, ve);
+            f = _mm512_max_epi16(_mm512_add_epi16(hup, vo), f_ext);
+            const __m512i hd = _mm512_add_epi16(hdiag, s);
+            hdiag = hleft;
+            __m512i h = _mm512_max_epi16(_mm512_max_epi16(hd, ev), f);
+            if (local) h = _mm512_max_epi16(h, zero);
+            /* the source, in the scalar path's order of preference */
+            __m512i t = t_del;
+            t = _mm512_mask_mov_epi16(t, _mm512_cmpeq_epi16_mask(h, f), t_ins);
+            t = _mm512_mask_mov_epi16(t, _mm512_cmpeq_epi16_mask(h, hd), t_diag);
+            if (local) t = _mm512_mask_mov_epi16(t, _mm512_cmpeq_epi16_mask(h, zero), zero);
+            t = _mm512_mask_add_epi16(t, _mm512_cmpeq_epi16_mask(ev, e_ext), t, t_eext);
+            t = _mm512_mask_add_epi16(t, _mm512_cmpeq_epi16_mask(f, f_ext), t, t_fext);
+            _mm256_storeu_si256((__m256i *)(tcol + (i + 1) * PAIR_LANES), _mm512_cvtepi16_epi8(t));
+            _mm512_storeu_si512((void *)(hbuf + i * PAIR_LANES), h);
+            hup = h;
+            /* candidate end cells, in the scalar scan order */
+            const __mmask32 ivalid = _mm512_cmpgt_epi16_mask(ql, i0);
+            const __mmask32 cand = local ? (ivalid & jvalid)
+                                         : ((_mm512_cmpeq_epi16_mask(ql, iv) & jvalid) | (jlast & ivalid));
+            const __mmask32 up = _mm512_cmpgt_epi16_mask(h, best) & cand;
+            best = _mm512_mask_mov_epi16(best, up, h);
+            bi = _mm512_mask_mov_epi16(bi, up, iv);
+            bj = _mm512_mask_mov_epi16(bj, up, jv);
+        }
+    }
+    _mm512_storeu_si512((void *)best_out, best);
+    _mm512_storeu_si512((void *)bi_out, bi);
+    _mm512_storeu_si512((void *)bj_out, bj);
+}
+
+typedef struct {
+    size_t k, ql, rl;
+} pair_ord;
+
+static int pair_ord_cmp(const void *a, const void *b) {
+    const pair_ord *x = (const pair_ord *)a, *y = (const pair_ord *)b;
+    if (x->ql != y->ql) return x->ql < y->ql ? -1 : 1;
+    if (x->rl != y->rl) return x->rl < y->rl ? -1 : 1;
+    return x->k < y->k ? -1 : (x->k > y->k ? 1 : 0); /* stable */
+}
+
+/* A worker's buffers, grown as needed and kept between calls. */
+struct eo_pair_scratch {
+    int16_t *hbuf, *ebuf, *qt, *rt;
+    uint8_t *tb;
+    pair_ord *ord;
+    size_t cap_h, cap_tb, cap_q, cap_r, cap_ord;
+    int simd;
+};
+
+eo_pair_scratch *eo_pair_scratch_new(void) {
+    eo_pair_scratch *s = (eo_pair_scratch *)calloc(1, sizeof(eo_pair_scratch));
+    s->simd = (eo_simd_supported() & 2) != 0;
+    return s;
+}
+
+void eo_pair_scratch_free(eo_pair_scratch *s) {
+    if (!s) return;
+    free(s->hbuf);
+    free(s->ebuf);
+    free(s->qt);
+    free(s->rt);
+    free(s->tb);
+    free(s->ord);
+    free(s);
+}
+
+static void *grow(void *p, size_t *cap, size_t need, size_t elem) {
+    if (need <= *cap) return p;
+    free(p);
+    *cap = need;
+    p = malloc(need * elem);
+    if (!p) abort();
+    return p;
+}
+
+/* simd::align_pairs (lib/simd/mod.rs:405-499) over one call's pairs, on the
+ * calling thread: pair k is q[k] (ql[k] codes) against r[k] (rl[k]); its
+ * alignment goes to al[k], its runs to ops + ops_off[k] (ql + rl + 1 slots)
+ * and their count to runs[k]. The SIMD-eligible pairs (non-empty, both
+ * under 32,000, fits_i16) are sorted by (read length, reference length) and
+ * cut into groups of 32 lanes; the others, groups over 16 MiB of
+ * traceback, and every pair without AVX-512BW take eo_align. */
+void eo_simd_align_chunk(eo_pair_scratch *s, const uint8_t *const *q, const size_t *ql, const uint8_t *const *r,
+                         const size_t *rl, size_t cnt, int32_t ma, int32_t mi, int32_t o, int32_t e, int local,
+                         eo_alignment *al, int *runs, uint32_t *ops, const size_t *ops_off) {
+    s->ord = (pair_ord *)grow(s->ord, &s->cap_ord, cnt ? cnt : 1, sizeof(pair_ord));
+    size_t no = 0;
+    for (size_t k = 0; k < cnt; k++) {
+        const int ok = s->simd && ql[k] > 0 && rl[k] > 0 && ql[k] < 32000 && rl[k] < 32000 &&
+                       eo_fits_i16(ql[k], rl[k], ma, mi, o, e);
+        if (ok) {
+            pair_ord x = {k, ql[k], rl[k]};
+            s->ord[no++] = x;
+        } else {
+            runs[k] = eo_align(q[k], ql[k], r[k], rl[k], ma, mi, o, e, local, &al[k], ops + ops_off[k],
+                               ql[k] + rl[k] + 1);
+        }
+    }
+    qsort(s->ord, no, sizeof(pair_ord), pair_ord_cmp);
+    for (size_t g0 = 0; g0 < no; g0 += PAIR_LANES) {
+        const size_t gn = no - g0 < PAIR_LANES ? no - g0 : PAIR_LANES;
+        const pair_ord *g = s->ord + g0;
+        size_t rows = 0, cols = 0;
+        for (size_t l = 0; l < gn; l++) {
+            rows = g[l].ql > rows ? g[l].ql : rows;
+            cols = g[l].rl > cols ? g[l].rl : cols;
+        }
+        const size_t need = (rows + 1) * (cols + 1) * PAIR_LANES;
+        if (need > MAX_TRACE_BYTES) {
+            for (size_t l = 0; l < gn; l++) {
+                const size_t k = g[l].k;
+                runs[k] = eo_align(q[k], ql[k], r[k], rl[k], ma, mi, o, e, local, &al[k], ops + ops_off[k],
+                                   ql[k] + rl[k] + 1);
+            }
+            continue;
+        }
+        size_t ch = s->cap_h;
+        s->hbuf = (int16_t *)grow(s->hbuf, &s->cap_h, rows * PAIR_LANES, sizeof(int16_t));
+        if (s->cap_h != ch) {
+            free(s->ebuf);
+            s->ebuf = (int16_t *)malloc(s->cap_h * sizeof(int16_t));
+        }
+        s->tb = (uint8_t *)grow(s->tb, &s->cap_tb, need, 1);
+        s->qt = (int16_t *)grow(s->qt, &s->cap_q, rows * PAIR_LANES, sizeof(int16_t));
+        s->rt = (int16_t *)grow(s->rt, &s->cap_r, cols * PAIR_LANES, sizeof(int16_t));
+        memset(s->qt, 0, rows * PAIR_LANES * sizeof(int16_t));
+        memset(s->rt, 0, cols * PAIR_LANES * sizeof(int16_t));
+        int16_t qlens[PAIR_LANES] = {0}, rlens[PAIR_LANES] = {0};
+        for (size_t l = 0; l < gn; l++) {
+            const size_t k = g[l].k;
+            qlens[l] = (int16_t)ql[k];
+            rlens[l] = (int16_t)rl[k];
+            for (size_t i = 0; i < ql[k]; i++) s->qt[i * PAIR_LANES + l] = q[k][i];
+            for (size_t c = 0; c < rl[k]; c++) s->rt[c * PAIR_LANES + l] = r[k][c];
+        }
+        int16_t best[PAIR_LANES], bi[PAIR_LANES], bj[PAIR_LANES];
+        pairs512(local, rows, cols, qlens, rlens, s->qt, s->rt, (int16_t)ma, (int16_t)mi, (int16_t)o, (int16_t)e,
+                 s->hbuf, s->ebuf, s->tb, best, bi, bj);
+        for (size_t l = 0; l < gn; l++) {
+            const size_t k = g[l].k;
+            runs[k] = eo_walk_trace(s->tb, rows + 1, PAIR_LANES, l, best[l], (size_t)bi[l], (size_t)bj[l], &al[k],
+                                    ops + ops_off[k], ql[k] + rl[k] + 1);
+        }
+    }
+}
+
+typedef struct {
+    const uint8_t *qcodes, *rcodes;
+    const uint64_t *qoff, *roff, *ops_off;
+    const int32_t *qlen, *rlen, *refs;
+    size_t n;
+    int32_t ma, mi, o, e;
+    int local;
+    int32_t *recs;
+    uint32_t *ops;
+    size_t *next;
+    pthread_mutex_t *mu;
+} pair_job;
+
+static void *pair_worker(void *arg) {
+    const pair_job *j = (const pair_job *)arg;
+    eo_pair_scratch *s = eo_pair_scratch_new();
+    const uint8_t *q[64], *r[64];
+    size_t ql[64], rl[64], off[64];
+    eo_alignment al[64];
+    int runs[64];
+    uint32_t *ops = NULL;
+    size_t cap_ops = 0;
+    for (;;) {
+        pthread_mutex_lock(j->mu);
+        const size_t start = *j->next;
+        *j->next += 64; /* a worker's chunk: 64 reads' winners (crates/eschalign-cli/src/align.rs:1470) */
+        pthread_mutex_unlock(j->mu);
+        if (start >= j->n) break;
+        const size_t end = start + 64 < j->n ? start + 64 : j->n, cnt = end - start;
+        size_t room = 0;
+        for (size_t t = 0; t < cnt; t++) {
+            const size_t k = start + t;
+            q[t] = j->qcodes + j->qoff[k];
+            ql[t] = (size_t)j->qlen[k];
+            r[t] = j->rcodes + j->roff[j->refs[k]];
+            rl[t] = (size_t)j->rlen[j->refs[k]];
+            off[t] = room;
+            room += ql[t] + rl[t] + 1;
+        }
+        ops = (uint32_t *)grow(ops, &cap_ops, room, sizeof(uint32_t));
+        eo_simd_align_chunk(s, q, ql, r, rl, cnt, j->ma, j->mi, j->o, j->e, j->local, al, runs, ops, off);
+        for (size_t t = 0; t < cnt; t++) {
+            const size_t k = start + t;
+            int32_t *o = j->recs + k * 6;
+            o[0] = al[t].score;
+            o[1] = al[t].query_start;
+            o[2] = al[t].query_end;
+            o[3] = al[t].ref_start;
+            o[4] = al[t].ref_end;
+            o[5] = runs[t];
+            if (j->ops && runs[t] > 0) memcpy(j->ops + j->ops_off[k], ops + off[t], (size_t)runs[t] * sizeof(uint32_t));
+        }
+    }
+    free(ops);
+    eo_pair_scratch_free(s);
+    return NULL;
+}
+
+void eo_simd_align_pairs(const uint8_t *qcodes, const uint64_t *qoff, const int32_t *qlen,
+                         const uint8_t *rcodes, const uint64_t *roff, const int32_t *rlen,
+                         const int32_t *refs, size_t n, int32_t ma, int32_t mi, int32_t o, int32_t e,
+                         int local, int threads, int32_t *recs, uint32_t *ops, const uint64_t *ops_off) {
+    if (threads < 1) threads = 1;
+    size_t next = 0;
+    pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)threads);
+    pair_job *jobs = (pair_job *)malloc(sizeof(pair_job) * (size_t)threads);
+    for (int t = 0; t < threads; t++) {
+        pair_job j = {qcodes, rcodes, qoff, roff, ops_off, qlen, rlen, refs, n, ma, mi, o, e, local,
+                      recs, ops, &next, &mu};
+        jobs[t] = j;
+        pthread_create(&th[t], NULL, pair_worker, &jobs[t]);
     }
     for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
     free(th);

@@ -44,6 +44,9 @@ def lib():
         L.eo_align.restype = C.c_int
         L.eo_align_batch.argtypes = [U8P, U64P, I32P, U8P, U64P, I32P, I32P, C.c_size_t] + \
             [C.c_int32] * 4 + [C.c_int, C.c_int, I32P]
+        L.eo_simd_align_pairs.argtypes = [U8P, U64P, I32P, U8P, U64P, I32P, I32P, C.c_size_t] + \
+            [C.c_int32] * 4 + [C.c_int, C.c_int, I32P, C.c_void_p, C.c_void_p]
+        L.eo_simd_align_pairs.restype = None
         L.eo_fits_i16.restype = C.c_int
         L.eo_score_matrix.argtypes = [U8P, U64P, I32P, C.c_size_t, U8P, U64P, I32P,
                                       C.c_size_t, C.c_int32, C.c_int32, C.c_int32,
@@ -165,6 +168,36 @@ def align_batch_concat(qc, qo, ql, refs_codes, ref_idx, scoring, local: bool, th
     if n:
         lib().eo_align_batch(qc, qo, ql, rc, ro, rl, idx, n, *scoring, int(local), threads,
                              out.reshape(-1))
+    return out[:n]
+
+
+def simd_align_batch_concat(qc, qo, ql, refs_codes, ref_idx, scoring, local: bool, threads: int | None = None,
+                            with_ops=False):
+    """simd::align_pairs with the AVX-512BW pair kernel (the reference's CPU
+    traceback of winners) for queries qc[qo[k]:qo[k] + ql[k]] against
+    refs_codes[ref_idx[k]]: int32 [n, 6] as align_batch_concat, and with
+    with_ops the runs of pair k at ops[off[k]:off[k] + runs] as (recs, ops,
+    off)."""
+    threads = threads or min(8, os.cpu_count() or 1)
+    qc = np.ascontiguousarray(qc, np.uint8)
+    qo = np.ascontiguousarray(qo, np.uint64)
+    ql = np.ascontiguousarray(ql, np.int32)
+    n = len(ql)
+    rc, ro, rl = concat(refs_codes)
+    idx = np.ascontiguousarray(ref_idx, np.int32)
+    out = np.zeros((max(n, 1), 6), np.int32)
+    ops = off = None
+    if with_ops:
+        room = ql.astype(np.int64) + rl[idx].astype(np.int64) + 1 if n else np.zeros(0, np.int64)
+        off = np.zeros(max(n, 1), np.uint64)
+        if n > 1:
+            off[1:n] = np.cumsum(room[:-1]).astype(np.uint64)
+        ops = np.zeros(max(int(room.sum()) if n else 1, 1), np.uint32)
+    if n:
+        lib().eo_simd_align_pairs(qc, qo, ql, rc, ro, rl, idx, n, *scoring, int(local), threads, out.reshape(-1),
+                                  None if ops is None else ops.ctypes.data, None if off is None else off.ctypes.data)
+    if with_ops:
+        return out[:n], ops, off[:n]
     return out[:n]
 
 

@@ -17,6 +17,7 @@ import itertools
 import random
 
 import numpy as np
+import pytest
 
 from oracle import pyoracle as O
 import refgen as G
@@ -217,3 +218,48 @@ def test_matrix_restatement_catches_planted_mistakes():
             if all(found.values()):
                 break
     assert found == {"del": True, "open": True}
+
+
+def _simd_vs_scalar(queries, panel, refs, scoring, local):
+    qc, qo, ql = O.concat(queries)
+    recs, ops, off = O.simd_align_batch_concat(qc, qo, ql, panel, np.array(refs, np.int32), scoring, local, 4,
+                                               with_ops=True)
+    for k, (q, r) in enumerate(zip(queries, refs)):
+        want = O.align(q, panel[r], scoring, local)
+        runs = [(O.OPS[int(x) >> 30], int(x) & 0x3FFFFFFF) for x in ops[int(off[k]):int(off[k]) + recs[k][5]]]
+        assert tuple(int(v) for v in recs[k][:5]) + (runs,) == tuple(want[:5]) + (want[5],), (k, scoring, local)
+
+
+def test_simd_pair_traceback_equals_scalar_align():
+    """The AVX-512BW pair traceback (simd::align_pairs, lib/simd/avx512.rs:
+    120-247; the CPU baseline of the device tracebacks) equals eo_align
+    pair for pair, score, coordinates and every run, as the reference's own
+    lib/simd/mod.rs:869-912 checks: random pairs over five schemes and both
+    modes (wildcards, reads 1-700 nt, references 1-500 nt, chunks that mix
+    lane groups), empty reads (the scalar path), and a pair outside the
+    i16 bound (the scalar path)."""
+    if not O.simd_supported() & 2:
+        pytest.skip("no AVX-512BW on this host: every pair takes the scalar path")
+    rng = G.Rng(0x5EED_0C01)
+    panel = [G.encode(G.random_seq(rng, 1 + rng.below(500), 0.03)) for _ in range(30)]
+    queries = [G.encode(G.random_seq(rng, 1 + rng.below(700), 0.03)) for _ in range(300)]
+    queries += [np.zeros(0, np.uint8), G.encode(G.random_seq(rng, 5, 0.0))]
+    refs = [rng.below(len(panel)) for _ in queries]
+    for scoring in G.SCHEMES_GPU:
+        for local in (True, False):
+            _simd_vs_scalar(queries, panel, refs, scoring, local)
+    # outside fits_i16 at 1000,-1000,-1000,-1000: every pair on the scalar path
+    _simd_vs_scalar(queries[:20], panel, refs[:20], (1000, -1000, -1000, -1000), True)
+
+
+def test_simd_pair_traceback_on_the_golden_pairs():
+    """The 192 parasail results through the pair kernel (both schemes, both
+    modes): equal to eo_align, which they pin."""
+    if not O.simd_supported() & 2:
+        pytest.skip("no AVX-512BW on this host")
+    g = G.golden()
+    for scoring in ((2, -1, -10, -1), (1, -1, -2, -1)):
+        qs = [G.encode(p["query"].encode()) for p in g["pairs"]]
+        rs = [G.encode(p["reference"].encode()) for p in g["pairs"]]
+        for local in (True, False):
+            _simd_vs_scalar(qs, rs, list(range(len(rs))), scoring, local)
