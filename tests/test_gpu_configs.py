@@ -112,6 +112,41 @@ def test_config3_large_panel_near_ties():
     assert ties_seen > 0, "the paralog panel should produce ties"
 
 
+def test_config3_at_scale_10k_references():
+    """The large-panel configuration's own panel (10,000 references of
+    200-1000 nt: 2,000 seeds and 4 copies of each at 2% substitutions) with
+    400 of its reads (100-1000 nt), local 2,-1,-10,-1: 2,500 reference
+    groups per chunk and the longest-first reference order at scale. The
+    ordinary path (swapping off), the swapped-roles path a batch of at most
+    one row per reference takes by default, and the autotuner's tuned tile
+    shape and plan all give the CPU kernel's scores; the fused tie sets
+    equal the oracle's selection."""
+    panel = synth.panel_with_copies(seed=0xC0F3, n_seeds=2000, copies=4, lo=200, hi=1000)
+    assert len(panel) == 10_000
+    codes, off, lens = synth.read_arrays(panel, seed=0xC0F3 + 1, n=400, length=("uniform", 100, 1000))
+    reads = [codes[int(off[k]):int(off[k]) + int(lens[k])] for k in range(400)]
+    scoring = (2, -1, -10, -1)
+    want = simd_rows(panel, reads, scoring, True)
+    gpu = ea.GpuScorer(ea.Panel(panel), ea.Scoring(*scoring), ea.Mode.LOCAL)
+    assert gpu.swap_rows == 10_000
+    got, ok = gpu.score_batch_i32(reads)            # 400 rows: the roles swapped
+    assert ok.all() and np.array_equal(got, want)
+    gpu.swap_rows = 0
+    got, ok = gpu.score_batch_i32(reads)            # the ordinary path
+    assert ok.all() and np.array_equal(got, want)
+    chosen, ms = gpu.autotune(reads[:200], both_strands=False)
+    assert ms[chosen] > 0
+    got, ok = gpu.score_batch_i32(reads)
+    assert ok.all() and np.array_equal(got, want), (chosen, gpu.diag_plan)
+    sel = gpu.select_batch(reads, both_strands=False, tie_cap=64)
+    tied = 0
+    for k in range(len(reads)):
+        b, nt, so, take = O.select(want[k], None, 0, None)
+        tied += nt > 1
+        assert (sel[k].best, sel[k].n_tied, sel[k].suboptimal, sel[k].take) == (b, nt, so, take[:64]), k
+    assert tied > 0
+
+
 def test_config4_paired_with_adapter_read_through():
     """500 refs of 70-95 nt, mates of 150 nt from inserts N(120, 30) reading
     into the TruSeq adapters: mates trimmed by the library's adapter
