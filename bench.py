@@ -51,10 +51,27 @@ OPS_PER_CELL = {("semiglobal", False, False): 8, ("local", False, False): 10,
                 ("semiglobal", True, False): 7, ("local", True, False): 8.5,
                 ("semiglobal", False, True): 7, ("local", False, True): 9,
                 ("semiglobal", True, True): 6, ("local", True, True): 7.5}
-F16_VARIANTS = (4, 5, 7, 8, 9, 10, 12, 13, 15, 16)  # k_dp16 variants that compute in fp16
-SHIFTED_VARIANTS = (6, 7, 8, 9, 10, 11, 12, 13, 14)  # and those with a shifted column
-# variant -> rows per DP stripe (9-11, 15, 16: 24 rows, 12-14: 32 rows, else 16)
-STRIPE_ROWS = {9: 24, 10: 24, 11: 24, 12: 32, 13: 32, 14: 32, 15: 24, 16: 24}
+
+
+def vinfo(v):
+    """A k_dp16 variant's schedule, rows per stripe, fp16 arithmetic and
+    shifted column, as the library reports them."""
+    import eschalign_amd as ea
+    return ea.variant_info(v)
+
+
+def int16_counterpart(v):
+    """The int16 variant with v's schedule family and stripe height: the
+    windowed int16 schedule for a windowed fp16 one, the row-shifted int16
+    schedule 6 for the other shifted ones, 3 for the unshifted."""
+    import eschalign_amd as ea
+    info = vinfo(v)
+    if not info.shifted:
+        return 3
+    sched = 11 if info.schedule >= 9 else 6
+    return next(u for u, x in ea.variants().items() if x.schedule == sched and x.stripe_rows == info.stripe_rows)
+
+
 # SQ_INSTS_VALU x 128 / cells of one k_dp16 launch of the default workload,
 # from the counter summary in profiles/ (mode, variant) -> lane-ops per cell
 VALU_OPS_PER_CELL_ISSUED = {("semiglobal", 3): round(344_021_737_036 * 128 / 5_303_272_453_076, 3),
@@ -350,8 +367,9 @@ def _timed_steps(batch, what, steps, tie_cap=1):
     return (time.perf_counter() - t) / steps, float(np.mean(dps))
 
 
-def _roof(cells, dp_ms, mode, variant):
-    ops = OPS_PER_CELL[(mode, variant in F16_VARIANTS, variant in SHIFTED_VARIANTS)]
+def _roof(cells, dp_ms, mode, v):
+    info = vinfo(v)
+    ops = OPS_PER_CELL[(mode, info.f16, info.shifted)]
     achieved = cells * ops / (dp_ms / 1e3)
     return {"bound": "valu", "achieved": round(achieved / 1e12, 3), "peak": round(PEAK_I16 / 1e12, 3),
             "unit": "Tlane-op/s (packed 16-bit)", "frac": round(achieved / PEAK_I16, 4), "ops_per_cell": ops}
@@ -451,7 +469,7 @@ def config3_leg(n_reads, cpu, cpu_reads=2000, variant=-1):
     out = {"workload": f"10k refs 200-1000 nt (2k seeds + 4 copies at 2% substitutions), {n_reads} reads "
                        f"100-1000 nt, local 2,-1,-10,-1, forward strand",
            "reads": n_reads, "n_refs": len(panel), "panel_bases": int(sum(len(p) for p in panel)),
-           "cells": int(cells), "kernel_variant": chosen, "stripe_rows": STRIPE_ROWS.get(chosen, 16),
+           "cells": int(cells), "kernel_variant": chosen, "stripe_rows": vinfo(chosen).stripe_rows,
            "diag_ratio": gpu.diag_ratio, "diag_plan_x16": gpu.diag_plan,
            "autotune_ms": [round(x, 3) for x in tune_ms], "autotune_s": round(tune_s, 2),
            "reads_per_s": round(n_reads / wall, 1), "wall_ms": round(wall * 1e3, 1), "dp_ms": round(dp, 3),
@@ -774,8 +792,8 @@ def main():
     value = args.reads / (wall / args.steps)
     dp = float(np.mean(dp_ms)) / 1e3
     gcups = cells / dp / 1e9
-    f16 = chosen in F16_VARIANTS
-    shifted = chosen in SHIFTED_VARIANTS
+    f16 = vinfo(chosen).f16
+    shifted = vinfo(chosen).shifted
     ops = OPS_PER_CELL[(args.mode, f16, shifted)]
     achieved = cells * ops / dp
     log(f"[bench] {value:,.0f} reads/s, {ms_step:.1f} ms/step (graph; eager "
@@ -783,12 +801,10 @@ def main():
         f"{n_diag} anti-diagonal / {n32} int32 queries; from host buffers {xfer * 1e3:.1f} ms/step")
 
     # With an fp16 variant chosen, the same steps again with its int16
-    # counterpart (variant 6 for the row-shifted 7, else 3), so both
-    # arithmetics' rates are on record.
+    # counterpart on the same stripes, so both arithmetics' rates are on
+    # record.
     int16 = None
-    # the int16 schedule on the same stripes: 6 (or 11 / 14 on 24 / 32 rows)
-    v16 = {16: 6, 24: 11, 32: 14}[STRIPE_ROWS.get(chosen, 16)] if shifted else 3
-    # (the unshifted fp16 schedules' int16 counterpart is 3, on 16 rows)
+    v16 = int16_counterpart(chosen)
     if f16:
         gpu.variant = v16
         batch.run(eager, tie_cap=tie_cap)
@@ -945,7 +961,7 @@ def main():
                                "scope": "reads from host buffers (gather into pinned staging, copy), the "
                                         "graph, selection records and tie lists copied back"},
         "kernel_variant": chosen,
-        "stripe_rows": STRIPE_ROWS.get(chosen, 16),
+        "stripe_rows": vinfo(chosen).stripe_rows,
         "int16_variant": int16,
         "diag_ratio": gpu_ratio,
         "diag_plan": {"classes_nt": ["<128", "128-255", "256-511", "512-1023", "1024-2047", "2048-4095",

@@ -64,7 +64,7 @@ EXPORTS = [
     "esch_batch_create", "esch_batch_free", "esch_batch_footprint", "esch_batch_load",
     "esch_batch_load_concat", "esch_batch_run", "esch_batch_wait", "esch_batch_timing",
     "esch_batch_stats", "esch_batch_read_scores32", "esch_batch_read_scores16",
-    "esch_batch_read_select", "esch_batch_read_pairs", "esch_n_variants", "esch_get_variant",
+    "esch_batch_read_select", "esch_batch_read_pairs", "esch_n_variants", "esch_variant_info", "esch_get_variant",
     "esch_set_variant", "esch_autotune", "esch_adapter_find", "esch_adapter_detect_pair",
     "esch_batch_ready", "esch_server_new", "esch_server_free", "esch_server_add_panel",
     "esch_server_scorer", "esch_server_submit", "esch_server_wait", "esch_server_result",
@@ -184,6 +184,7 @@ def load_library():
     L.esch_batch_read_select.argtypes = [vp, vp, vp]
     L.esch_batch_read_pairs.argtypes = [vp, vp, vp]
     L.esch_n_variants.restype = C.c_int
+    L.esch_variant_info.argtypes = [C.c_int] + [C.POINTER(C.c_int)] * 4
     L.esch_get_variant.argtypes = [vp]
     L.esch_set_variant.argtypes = [vp, C.c_int]
     L.esch_autotune.argtypes = [vp, vp, vp, sz, C.c_int, vp, C.POINTER(C.c_int)]
@@ -443,6 +444,27 @@ def valu_peak16(ordinal: int = 0) -> dict:
     v, cu, clk = C.c_double(), C.c_int(), C.c_int()
     _check(load_library().esch_valu_peak16(ordinal, C.byref(v), C.byref(cu), C.byref(clk)))
     return {"lane_ops_per_s": v.value, "n_cu": cu.value, "clock_khz": clk.value}
+
+
+@dataclass(frozen=True)
+class VariantInfo:
+    schedule: int
+    stripe_rows: int
+    f16: bool
+    shifted: bool
+
+
+def variant_info(variant: int) -> VariantInfo:
+    """esch_variant_info: a batched-kernel variant's schedule, rows per
+    stripe, fp16 arithmetic (the 2048 condition) and shifted column."""
+    out = [C.c_int() for _ in range(4)]
+    _check(load_library().esch_variant_info(variant, *[C.byref(x) for x in out]))
+    return VariantInfo(out[0].value, out[1].value, bool(out[2].value), bool(out[3].value))
+
+
+def variants() -> dict:
+    """Every batched-kernel variant's VariantInfo, by number."""
+    return {v: variant_info(v) for v in range(load_library().esch_n_variants())}
 
 
 def device_visible() -> bool:

@@ -3,6 +3,9 @@
 // (kernels_sg.hip, kernels_lo.hip), which instantiate its variants, and by
 // kernels.hip for the helpers.
 #pragma once
+#include <type_traits>
+#include <utility>
+
 #include "esch_internal.h"
 
 namespace esch {
@@ -347,6 +350,119 @@ __device__ __forceinline__ void column16c_pair(typename Pk<F16>::T (&scA)[RR], t
     for (int q = (RR - 1) / 4; q < NP; ++q) plane_part<RR, F16>(nB, q, scB);
 }
 
+// Calls f(std::integral_constant<int, I>) for I = B .. E - 1 in order: a
+// loop unrolled by construction, whatever its body's size, so that row
+// indices stay compile-time constants (a loop the unroller declines would
+// index the register arrays at run time).
+template <int B, typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, B + I>{}), ...);
+}
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (E > B) static_for_impl<B>(f, std::make_integer_sequence<int, E - B>{});
+}
+
+// The profile reloads of a windowed column pair (column16s_pairw,
+// column16c_pairw), issued once a column has computed row r: its own quad
+// k + LA once quad k is done (LA quads ahead), and the next pair's first LA
+// quads as its last LA quads finish, into the registers those first quads
+// left. A column's plane then occupies about LA + 1 quads of registers
+// rather than all of its rows, which leaves room for taller stripes or one
+// more resident wave.
+template <int RR, int LA, bool F16, int r>
+__device__ __forceinline__ void window_loads(const uint4 *cur, const uint4 *nxt, typename Pk<F16>::T (&sc)[RR]) {
+    constexpr int NP = (RR + 3) / 4;
+    constexpr int SKIP = NP > LA ? NP - LA : 0;
+    constexpr int k = r == RR - 1 ? NP - 1 : r / 4;  // the quad row r ends, if it ends one
+    if constexpr (r == RR - 1 || r % 4 == 3) {
+        if constexpr (k + LA < NP) plane_part<RR, F16>(cur, k + LA, sc);
+        if constexpr (k >= SKIP && k - SKIP < LA) plane_part<RR, F16>(nxt, k - SKIP, sc);
+    }
+}
+
+// column16s_pair with the windowed profile: on entry quads 0 .. LA - 1 of
+// both columns' planes (cA, cB) are loaded; on exit those of the next pair's
+// (nA, nB). Each cell's arithmetic is column16s'.
+template <int RR, bool ADJ, bool F16, int LA>
+__device__ __forceinline__ void column16s_pairw(typename Pk<F16>::T (&scA)[RR], typename Pk<F16>::T (&scB)[RR],
+                                                typename Pk<F16>::T (&Pv)[RR], typename Pk<F16>::T (&R)[RR],
+                                                typename Pk<F16>::T &puA, typename Pk<F16>::T &qA,
+                                                typename Pk<F16>::T pdA, typename Pk<F16>::T &puB,
+                                                typename Pk<F16>::T &qB, typename Pk<F16>::T pdB,
+                                                typename Pk<F16>::T voe, typename Pk<F16>::T ve,
+                                                typename Pk<F16>::T vc, const uint4 *cA, const uint4 *cB,
+                                                const uint4 *nA, const uint4 *nB) {
+    using P = Pk<F16>;
+    using T = typename P::T;
+    auto row = [&](auto rc, const T (&sc)[RR], T &pu, T &q, T &pd) {
+        constexpr int r = decltype(rc)::value;
+        R[r] = P::add(P::max(Pv[r], R[r]), ve);
+        q = P::max(pu, q);
+        if constexpr (ADJ && r == 0) q = P::add(q, vc);
+        const T h = P::max3(P::add(pd, sc[r]), R[r], q);
+        pd = Pv[r];
+        Pv[r] = P::add(h, voe);
+        pu = Pv[r];
+    };
+    // step r: A's row r, then B's row r - 1 (one row behind)
+    static_for<0, RR + 1>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        if constexpr (r < RR) {
+            row(std::integral_constant<int, r>{}, scA, puA, qA, pdA);
+            window_loads<RR, LA, F16, r>(cA, nA, scA);
+        }
+        if constexpr (r >= 1) {
+            row(std::integral_constant<int, r - 1>{}, scB, puB, qB, pdB);
+            window_loads<RR, LA, F16, r - 1>(cB, nB, scB);
+        }
+    });
+}
+
+// column16c_pair with the windowed profile (as column16s_pairw).
+template <int RR, bool F16, int LA>
+__device__ __forceinline__ void column16c_pairw(typename Pk<F16>::T (&scA)[RR], typename Pk<F16>::T (&scB)[RR],
+                                                typename Pk<F16>::T (&Pv)[RR], typename Pk<F16>::T (&R)[RR],
+                                                typename Pk<F16>::T &puA, typename Pk<F16>::T &qA,
+                                                typename Pk<F16>::T pdA, typename Pk<F16>::T &puB,
+                                                typename Pk<F16>::T &qB, typename Pk<F16>::T pdB,
+                                                typename Pk<F16>::T voe, typename Pk<F16>::T ve,
+                                                typename Pk<F16>::T zA, typename Pk<F16>::T zB,
+                                                typename Pk<F16>::T &tA, typename Pk<F16>::T &tB,
+                                                const uint4 *cA, const uint4 *cB, const uint4 *nA,
+                                                const uint4 *nB) {
+    using P = Pk<F16>;
+    using T = typename P::T;
+    auto row = [&](auto rc, const T (&sc)[RR], T &pu, T &q, T &pd, T z) {
+        constexpr int r = decltype(rc)::value;
+        R[r] = P::max(Pv[r], R[r]);
+        q = P::add(P::max(pu, q), ve);
+        const T h = P::max(P::max3(P::add(pd, sc[r]), R[r], q), z);
+        pd = Pv[r];
+        Pv[r] = P::add(h, voe);
+        pu = Pv[r];
+    };
+    // rows r - 1 and r (r odd) into a column's running best
+    auto fold = [&](auto rc, T &t) {
+        constexpr int r = decltype(rc)::value;
+        if constexpr (r == 1) t = P::max(Pv[0], Pv[1]);
+        else t = P::max3(t, Pv[r - 1], Pv[r]);
+    };
+    static_for<0, RR + 1>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        if constexpr (r < RR) {
+            row(std::integral_constant<int, r>{}, scA, puA, qA, pdA, zA);
+            window_loads<RR, LA, F16, r>(cA, nA, scA);
+            if constexpr (r & 1) fold(std::integral_constant<int, r>{}, tA);  // before B overwrites row r - 1
+        }
+        if constexpr (r >= 1) {
+            row(std::integral_constant<int, r - 1>{}, scB, puB, qB, pdB, zB);
+            window_loads<RR, LA, F16, r - 1>(cB, nB, scB);
+            if constexpr ((r - 1) & 1) fold(std::integral_constant<int, r - 1>{}, tB);
+        }
+    });
+}
+
 // One stripe of RR rows (RR = 16, or 4/8/12 for a chunk's first stripe)
 // against the wave's reference, columns 0 .. m-1. row0 is the stripe's first
 // row in the chunk's padded row numbering, pA / pB the padding rows of the
@@ -452,7 +568,7 @@ __device__ __forceinline__ void stripe16(const uint4 *__restrict__ prof, uint2 *
 // stripe's space); CS (local only) the column-shifted one (column16c), with
 // `best` carried in column j's space through the stripe.
 template <int RR, bool LOCAL, bool FIRST, bool LAST, bool QUAD, bool F16, bool OCT = false, bool SH = false,
-          bool CS = false, int PQ = 4>
+          bool CS = false, int PQ = 4, int LA = 0>
 __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 *__restrict__ scr,
                                           const int lane, const cwords w2, const cwords wn,
                                           const int m, const int row0, const int pA, const int pB,
@@ -546,8 +662,22 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
     };
 
     v2s sa[RR], sb[RR];
-    plane(code(0), sa);
-    if constexpr (XS) plane(code(1 < m ? 1 : m), sb);
+    // LA > 0 (shifted columns): the windowed profile, each column's plane
+    // read LA quads ahead of its rows (column16s_pairw / column16c_pairw);
+    // pa / pb are the planes of the pair about to run
+    static_assert(LA == 0 || XS, "the windowed profile runs the shifted column pairs");
+    const uint4 *pa = prof + code(0) * PQ * 64 + lane;
+    const uint4 *pb = prof + code(1 < m ? 1 : m) * PQ * 64 + lane;
+    if constexpr (LA > 0) {
+#pragma unroll
+        for (int q = 0; q < LA && q < (RR + 3) / 4; ++q) {
+            plane_part<RR, F16>(pa, q, sa);
+            plane_part<RR, F16>(pb, q, sb);
+        }
+    } else {
+        plane(code(0), sa);
+        if constexpr (XS) plane(code(1 < m ? 1 : m), sb);
+    }
     uint2 o0 = border, o1 = border;
     int j = 0;
     // Columns jj (plane sa) and jj + 1 into o0 and o1. SH: both planes are
@@ -566,9 +696,17 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
             const v2s zA = z, zB = P::add(z, vne);
             z = P::add(zB, vne);
             const int ja = jj + 2 < m ? jj + 2 : m, jb = jj + 3 < m ? jj + 3 : m;
+            const uint4 *nA = prof + code(ja) * PQ * 64 + lane, *nB = prof + code(jb) * PQ * 64 + lane;
             v2s tA, tB;
-            column16c_pair<RR, F16>(sa, sb, Ho, E, puA, qA, pdA, puB, qB, pdB, voe, vse, zA, zB, tA, tB,
-                                    prof + code(ja) * PQ * 64 + lane, prof + code(jb) * PQ * 64 + lane);
+            if constexpr (LA > 0) {
+                column16c_pairw<RR, F16, LA>(sa, sb, Ho, E, puA, qA, pdA, puB, qB, pdB, voe, vse, zA, zB, tA,
+                                             tB, pa, pb, nA, nB);
+                pa = nA;
+                pb = nB;
+            } else {
+                column16c_pair<RR, F16>(sa, sb, Ho, E, puA, qA, pdA, puB, qB, pdB, voe, vse, zA, zB, tA, tB, nA,
+                                        nB);
+            }
             // best from column jj - 1's space to jj + 1's
             best = P::max3(P::add(best, vn2e), P::add(tA, vne), tB);
             o0 = make_uint2(P::bits(puA), P::bits(qA));
@@ -578,8 +716,15 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
             const v2s pdA = hdt, pdB = puA;
             hdt = puB;
             const int ja = jj + 2 < m ? jj + 2 : m, jb = jj + 3 < m ? jj + 3 : m;
-            column16s_pair<RR, !FIRST, F16>(sa, sb, Ho, E, puA, qA, pdA, puB, qB, pdB, voe, vse, vc,
-                                            prof + code(ja) * PQ * 64 + lane, prof + code(jb) * PQ * 64 + lane);
+            const uint4 *nA = prof + code(ja) * PQ * 64 + lane, *nB = prof + code(jb) * PQ * 64 + lane;
+            if constexpr (LA > 0) {
+                column16s_pairw<RR, !FIRST, F16, LA>(sa, sb, Ho, E, puA, qA, pdA, puB, qB, pdB, voe, vse, vc, pa,
+                                                     pb, nA, nB);
+                pa = nA;
+                pb = nB;
+            } else {
+                column16s_pair<RR, !FIRST, F16>(sa, sb, Ho, E, puA, qA, pdA, puB, qB, pdB, voe, vse, vc, nA, nB);
+            }
             if constexpr (LAST) bestL = P::max3(bestL, puA, puB);
             o0 = make_uint2(P::bits(puA), P::bits(qA));
             o1 = make_uint2(P::bits(puB), P::bits(qB));
@@ -652,6 +797,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
         s0[(ptrdiff_t)(j - 1) * 64] = o1;
     }
     if (j < m) {
+        if constexpr (LA > 0) plane(code(j), sa);  // the window held its first LA quads only
         o0 = col(sa, a0);
         if constexpr (!LAST) s0[(size_t)j * 64] = o0;
     }
@@ -675,6 +821,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
         s0[(ptrdiff_t)(j - 1) * 64] = o1;
     }
     if (j < m) {
+        if constexpr (LA > 0) plane(code(j), sa);
         o0 = col(sa, r0);
         if constexpr (!LAST) s0[(size_t)j * 64] = o0;
     }
@@ -720,6 +867,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
         s0[(ptrdiff_t)(j - 1) * 64] = o1;
     }
     if (j < m) {
+        if constexpr (LA > 0) plane(code(j), sa);
         o0 = col(sa, r0);
         if constexpr (!LAST) s0[(size_t)j * 64] = o0;
     }
@@ -746,15 +894,16 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
     }
 }
 
-template <int RR, bool LOCAL, bool FIRST, bool QUAD, bool F16, bool OCT, bool SH = false, bool CS = false, int PQ = 4>
+template <int RR, bool LOCAL, bool FIRST, bool QUAD, bool F16, bool OCT, bool SH = false, bool CS = false, int PQ = 4,
+          int LA = 0>
 __device__ __forceinline__ void stripe16d_any(const uint4 *prof, uint2 *scr, const int lane,
                                               const cwords w2, const cwords wn, const int m,
                                               const bool last, const int row0, const int pA,
                                               const int pB, const typename Pk<F16>::T vo,
                                               const typename Pk<F16>::T ve, typename Pk<F16>::T &best,
                                               const int c = 0) {
-    if (last) stripe16d<RR, LOCAL, FIRST, true, QUAD, F16, OCT, SH, CS, PQ>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best, c);
-    else stripe16d<RR, LOCAL, FIRST, false, QUAD, F16, OCT, SH, CS, PQ>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best, c);
+    if (last) stripe16d<RR, LOCAL, FIRST, true, QUAD, F16, OCT, SH, CS, PQ, LA>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best, c);
+    else stripe16d<RR, LOCAL, FIRST, false, QUAD, F16, OCT, SH, CS, PQ, LA>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best, c);
 }
 
 template <bool LOCAL, int PF, int ST = kStripe>
@@ -778,8 +927,12 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
     // iteration with the boundary four columns ahead, in fp16 arithmetic;
     // 6, 7 and 8: variants 3, 5 and 4 with the row-shifted semiglobal
     // column (column16s), in local mode the column-shifted one (column16c)
-    constexpr bool F16 = PF == 4 || PF == 5 || PF == 7 || PF == 8;
-    constexpr bool OCT = PF == 5 || PF == 7;
+    // 9, 10 and 11: schedules 8, 7 and 6 with the windowed profile (each
+    // column's plane read one quad ahead of its rows instead of a pair
+    // ahead: fewer registers, so taller stripes or one more wave)
+    constexpr bool F16 = PF == 4 || PF == 5 || PF == 7 || PF == 8 || PF == 9 || PF == 10;
+    constexpr bool OCT = PF == 5 || PF == 7 || PF == 10;
+    constexpr int LA = PF >= 9 ? 1 : 0;
     constexpr bool SH = !LOCAL && PF >= 6;
     constexpr bool CS = LOCAL && PF >= 6;  // local mode: the column shift (column16c)
     using PK = Pk<F16>;
@@ -872,22 +1025,22 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
             if (!active) continue;
             const bool first = s == 0, last = s + 1 == S;
             if constexpr (PF >= 2) {
-                constexpr bool Q = PF == 3 || PF == 4 || PF == 6 || PF == 8;
+                constexpr bool Q = PF == 3 || PF == 4 || PF == 6 || PF == 8 || PF == 9 || PF == 11;
                 if (s > 0) {
-                    stripe16d_any<ST, LOCAL, false, Q, F16, OCT, SH, CS, PQ>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best, cadj);
+                    stripe16d_any<ST, LOCAL, false, Q, F16, OCT, SH, CS, PQ, LA>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best, cadj);
                 } else {
                     switch (RR) {
 #define ESCH_FIRST(H) \
-    case H: stripe16d_any<(H <= ST ? H : 2), LOCAL, true, Q, F16, OCT, SH, CS, PQ>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best); break;
+    case H: stripe16d_any<(H <= ST ? H : 2), LOCAL, true, Q, F16, OCT, SH, CS, PQ, LA>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best); break;
                         ESCH_FIRST(16) ESCH_FIRST(14) ESCH_FIRST(12) ESCH_FIRST(10)
                         ESCH_FIRST(8) ESCH_FIRST(6) ESCH_FIRST(4)
 #undef ESCH_FIRST
 #define ESCH_FIRST(H) \
-    case H: if constexpr (ST >= H) stripe16d_any<(H <= ST ? H : 2), LOCAL, true, Q, F16, OCT, SH, CS, PQ>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best); break;
+    case H: if constexpr (ST >= H) stripe16d_any<(H <= ST ? H : 2), LOCAL, true, Q, F16, OCT, SH, CS, PQ, LA>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best); break;
                         ESCH_FIRST(32) ESCH_FIRST(30) ESCH_FIRST(28) ESCH_FIRST(26) ESCH_FIRST(24)
                         ESCH_FIRST(22) ESCH_FIRST(20) ESCH_FIRST(18)
 #undef ESCH_FIRST
-                        default: stripe16d_any<2, LOCAL, true, Q, F16, OCT, SH, CS, PQ>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
+                        default: stripe16d_any<2, LOCAL, true, Q, F16, OCT, SH, CS, PQ, LA>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
                     }
                 }
             } else {
@@ -924,9 +1077,17 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
 // The variants of one mode and stripe height: the kernel's host handle and
 // its launch. Each (mode, stripe height) has a translation unit of its own
 // (kernels_{sg,lo}{,24,32}.hip), so they compile in parallel; the 24-row
-// units instantiate the schedules 4-8, the 32-row ones 6, 7 and 8.
+// units instantiate the schedules 4-11, the 32-row ones 6-11.
 template <bool LOCAL, int ST>
 inline const void *dp16_fn_impl(int variant) {
+    if constexpr (ST >= 24) {
+        switch (dp16_schedule(variant)) {
+            case 11: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 11, ST>);
+            case 10: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 10, ST>);
+            case 9: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 9, ST>);
+            default: break;
+        }
+    }
     switch (dp16_schedule(variant)) {
         case 8: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 8, ST>);
         case 7: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 7, ST>);
@@ -954,6 +1115,14 @@ inline const void *dp16_fn_impl(int variant) {
 template <bool LOCAL, int ST>
 inline hipError_t launch_dp16_impl(const KArgs &a, const Launch &g, hipStream_t st) {
     const dim3 grid(g.grid16), block(64 * kWaves16);
+    if constexpr (ST >= 24) {
+        switch (dp16_schedule(g.variant16)) {
+            case 11: hipLaunchKernelGGL((k_dp16<LOCAL, 11, ST>), grid, block, 0, st, a); return hipGetLastError();
+            case 10: hipLaunchKernelGGL((k_dp16<LOCAL, 10, ST>), grid, block, 0, st, a); return hipGetLastError();
+            case 9: hipLaunchKernelGGL((k_dp16<LOCAL, 9, ST>), grid, block, 0, st, a); return hipGetLastError();
+            default: break;
+        }
+    }
     switch (dp16_schedule(g.variant16)) {
         case 8: hipLaunchKernelGGL((k_dp16<LOCAL, 8, ST>), grid, block, 0, st, a); return hipGetLastError();
         case 7: hipLaunchKernelGGL((k_dp16<LOCAL, 7, ST>), grid, block, 0, st, a); return hipGetLastError();

@@ -176,17 +176,24 @@ struct Launch {
 // 12, 13, 14 = 7, 8, 6 on 32 rows; 15, 16 = 4, 5 on 24 rows. A stripe
 // boundary per 24 or 32 rows instead of 16 means less HBM traffic and the
 // per-stripe work spread over more rows, held in more registers (fewer
-// resident waves where they pass 168).
-constexpr int kDp16Variants = 17;
+// resident waves where they pass 168). 17-22 are the windowed-profile
+// schedules 9, 10, 11 (8, 7 and 6 with each column's plane read one quad
+// ahead of its rows, not a column pair ahead), on 24 and 32 rows: the
+// plane no longer holds a register per row, so 24-row stripes fit 4 waves
+// per SIMD and 32-row ones 3.
+constexpr int kDp16Variants = 23;
 constexpr int kDp16Default = 3;  // the int16 default; panels within kBoundF16 take kDp16F16
 constexpr int kDp16F16 = 4;      // the first variant computing in fp16
-constexpr int kDp16Sched[kDp16Variants] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 7, 8, 6, 7, 8, 6, 4, 5};
-constexpr int kDp16Rows[kDp16Variants] = {16, 16, 16, 16, 16, 16, 16, 16, 16, 24, 24, 24, 32, 32, 32, 24, 24};
+constexpr int kDp16Sched[kDp16Variants] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 7, 8, 6, 7, 8, 6, 4, 5, 9, 9, 10, 10, 11, 11};
+constexpr int kDp16Rows[kDp16Variants] = {16, 16, 16, 16, 16, 16, 16, 16, 16, 24, 24, 24,
+                                          32, 32, 32, 24, 24, 24, 32, 24, 32, 24, 32};
 constexpr int dp16_schedule(int v) { return kDp16Sched[v]; }
 constexpr int dp16_stripe(int v) { return kDp16Rows[v]; }
-// variants computing in fp16, and those with the row-shifted semiglobal column
+// variants computing in fp16, and those with a shifted column (row-shifted
+// in semiglobal mode, column-shifted in local mode)
 constexpr bool dp16_is_f16(int v) {
-    return dp16_schedule(v) == 4 || dp16_schedule(v) == 5 || dp16_schedule(v) == 7 || dp16_schedule(v) == 8;
+    return dp16_schedule(v) == 4 || dp16_schedule(v) == 5 || dp16_schedule(v) == 7 || dp16_schedule(v) == 8 ||
+           dp16_schedule(v) == 9 || dp16_schedule(v) == 10;
 }
 constexpr bool dp16_is_shifted(int v) { return dp16_schedule(v) >= 6; }
 // The variant with schedule `schedule` on stripes of `stripe` rows (-1: none).

@@ -1262,6 +1262,16 @@ int32_t esch_get_diag_ratio(const esch_scorer *s) {
 }
 int esch_n_variants(void) { return kDp16Variants; }
 
+esch_status esch_variant_info(int variant, int *schedule, int *stripe_rows, int *f16, int *shifted) {
+    if (variant < 0 || variant >= kDp16Variants)
+        return fail(ESCH_INVALID_ARG, "variant %d outside 0..%d", variant, kDp16Variants - 1);
+    if (schedule) *schedule = esch::dp16_schedule(variant);
+    if (stripe_rows) *stripe_rows = esch::dp16_stripe(variant);
+    if (f16) *f16 = esch::dp16_is_f16(variant) ? 1 : 0;
+    if (shifted) *shifted = esch::dp16_is_shifted(variant) ? 1 : 0;
+    return ESCH_OK;
+}
+
 esch_status esch_autotune(esch_scorer *s, const uint8_t *const *reads, const size_t *lens, size_t n,
                           int both, float *ms, int *chosen) {
     if (!s || (n && (!reads || !lens))) return fail(ESCH_INVALID_ARG, "null argument");

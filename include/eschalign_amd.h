@@ -477,8 +477,19 @@ esch_status esch_bam_build_records(const uint8_t *in, size_t in_bytes, size_t n,
  *      through device memory, so taller stripes move fewer boundary bytes
  *      and spread each stripe's fixed work over more rows, in more
  *      registers. The row-shifted ones need (rows + 1) |extend| in place of
- *      17 |extend| in the condition of 6 / 7. */
+ *      17 |extend| in the condition of 6 / 7.
+ *   17-22  schedules 8, 7 and 6 with a windowed profile: each column reads
+ *      its plane a quad of rows ahead of use rather than a column pair
+ *      ahead, so the plane holds a few registers instead of one per row;
+ *      17, 18 = 8 on 24 / 32 rows; 19, 20 = 7 on 24 / 32; 21, 22 = 6 (int16)
+ *      on 24 / 32. The conditions of the schedules they window. */
 int esch_n_variants(void);
+/* A variant's shape: its schedule (variants 0-8 are schedules 0-8 on 16
+ * rows; 9, 10 and 11 are the windowed 8, 7 and 6), rows per stripe, whether
+ * it computes in fp16 (the 2048 condition) and whether it runs the shifted
+ * column. Returns ESCH_INVALID_ARG for a variant outside
+ * 0 .. esch_n_variants() - 1; any output pointer may be NULL. */
+esch_status esch_variant_info(int variant, int *schedule, int *stripe_rows, int *f16, int *shifted);
 int esch_get_variant(const esch_scorer *s);
 esch_status esch_set_variant(esch_scorer *s, int variant);
 /* ms[v] (may be NULL, esch_n_variants() entries) receives each variant's DP

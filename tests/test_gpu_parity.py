@@ -398,14 +398,15 @@ def test_fp16_variant_at_its_bound():
         assert chosen not in F16_VARIANTS and all(ms[v] == -1.0 for v in F16_VARIANTS)
 
 
-F16_VARIANTS = (4, 5, 7, 8)
+F16_VARIANTS = (4, 5, 7, 8, 9, 10, 12, 13, 15, 16, 17, 18, 19, 20)
 
 
 def test_row_shifted_variants_at_their_bound():
     """Variants 6 (int16), 7 and 8 (fp16) hold every value of a stripe row r
     shifted by |e| r: exact while longest reference x M + |o| + 17 |e| stays
     within the arithmetic's bound (2048 for fp16, 32000 for int16); the same
-    schedules on 24- and 32-row stripes (9-14) need 25 |e| and 33 |e|. Panels
+    schedules on 24- and 32-row stripes (9-14, and the windowed 17-22) need
+    25 |e| and 33 |e|. Panels
     exactly at it, with reads that reach both ends of the value range (a
     reference read back whole, runs of one base against a reference of
     another, reads several stripes long), equal the oracle in semiglobal
@@ -416,10 +417,13 @@ def test_row_shifted_variants_at_their_bound():
     # equals the bound; the tall-stripe variants 9-14 on 24 and 32 rows
     for scoring, L, others, st in (((1, -1, -2, -1), 2029, (7, 8), 16), ((2, -1, -11, -3), 993, (7, 8), 16),
                                    ((100, -100, -200, -100), 301, (6,), 16),
-                                   ((1, -1, -2, -1), 2021, (9, 10), 24), ((100, -100, -200, -100), 293, (11,), 24),
-                                   ((2, -1, -11, -3), 969, (12, 13), 32), ((100, -100, -200, -100), 285, (14,), 32)):
+                                   ((1, -1, -2, -1), 2021, (9, 10, 17, 19), 24),
+                                   ((100, -100, -200, -100), 293, (11, 21), 24),
+                                   ((2, -1, -11, -3), 969, (12, 13, 18, 20), 32),
+                                   ((100, -100, -200, -100), 285, (14, 22), 32)):
         M = max(abs(scoring[0]), abs(scoring[1]))
-        f16 = others[0] not in (6, 11, 14)
+        f16 = ea.variant_info(others[0]).f16
+        assert all(ea.variant_info(w).stripe_rows == st and ea.variant_info(w).f16 == f16 for w in others)
         assert L * M + abs(scoring[2]) + (st + 1) * abs(scoring[3]) == (2048 if f16 else 32000)
         refs = [G.random_seq(rng, L, 0.0), b"C" * L, G.random_seq(rng, L // 2, 0.02),
                 G.random_seq(rng, 40, 0.0)]
@@ -456,8 +460,9 @@ def test_column_shifted_local_variants_at_their_bound():
     refuses the variant while semiglobal mode (its row-shift bound) keeps
     it."""
     rng = G.Rng(0x5EED_0A05)
-    # the schedules on 24- and 32-row stripes (9-14) shift columns the same way
-    f16, i16 = (7, 8, 9, 10, 12, 13), (6, 11, 14)
+    # the schedules on 24- and 32-row stripes (9-14, windowed 17-22) shift
+    # columns the same way
+    f16, i16 = (7, 8, 9, 10, 12, 13, 17, 18, 19, 20), (6, 11, 14, 21, 22)
     for scoring, L, vs in (((1, -1, -2, -1), 1021, f16), ((2, -1, -12, -3), 405, f16),
                            ((100, -100, -200, -100), 157, i16)):
         M = max(abs(scoring[0]), abs(scoring[1]))
@@ -896,9 +901,9 @@ def test_short_reads_take_fp16_on_a_panel_of_long_references():
         # the autotuner's pick (among the int16 variants this panel admits)
         # is not the caller's: a batch of short reads still takes fp16
         chosen, _ = gpu.autotune(codes_of(short), both_strands=False)
-        assert chosen in (0, 1, 2, 3, 6, 11, 14)  # the int16 variants
+        assert not ea.variant_info(chosen).f16  # an int16 variant
         # semiglobal: the row-shifted fp16 schedule on the pick's stripes
-        rows = {11: 24, 14: 32}.get(chosen, 16)
+        rows = ea.variant_info(chosen).stripe_rows
         up = {16: 7, 24: 9, 32: 12}[rows] if mode == ea.Mode.SEMIGLOBAL else 4
         q = codes_of(short)
         b = gpu.batch(len(q), sum(len(x) for x in q))
