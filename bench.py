@@ -42,15 +42,16 @@ PEAK_I16 = N_CU * CLOCK_HZ * 128           # lane-ops/s
 PEAK_I32 = N_CU * CLOCK_HZ * 64
 # Minimum packed 16-bit lane-ops per DP cell of the recurrence: 4 adds and
 # 4 two-input maxima in semiglobal mode, plus the zero floor and the running
-# best in local mode; with the fp16 variants' three-input maximum (4, 5, 7, 8)
-# H takes one max instead of two, and the column best a three-way tree. The
-# shifted variants (6, 7, 8) drop one gap chain's extension add: rows shifted
-# in semiglobal mode (one op less), columns in local mode (one less again).
-# Keyed (mode, fp16, shifted).
-OPS_PER_CELL = {("semiglobal", False, False): 8, ("local", False, False): 10,
-                ("semiglobal", True, False): 7, ("local", True, False): 8.5,
-                ("semiglobal", False, True): 7, ("local", False, True): 9,
-                ("semiglobal", True, True): 6, ("local", True, True): 7.5}
+# best in local mode; with the fp16 variants' three-input maximum H takes
+# one max instead of two, and the column best a three-way tree. A shift
+# along one axis (shifted 1: rows in semiglobal mode, columns in local
+# mode) drops one gap chain's extension add; the diagonal shift (shifted 2,
+# semiglobal mode only) drops both. Keyed (mode, fp16, shifted).
+OPS_PER_CELL = {("semiglobal", False, 0): 8, ("local", False, 0): 10,
+                ("semiglobal", True, 0): 7, ("local", True, 0): 8.5,
+                ("semiglobal", False, 1): 7, ("local", False, 1): 9,
+                ("semiglobal", True, 1): 6, ("local", True, 1): 7.5,
+                ("semiglobal", False, 2): 6, ("semiglobal", True, 2): 5}
 
 
 def vinfo(v):
@@ -62,13 +63,14 @@ def vinfo(v):
 
 def int16_counterpart(v):
     """The int16 variant with v's schedule family and stripe height: the
-    windowed int16 schedule for a windowed fp16 one, the row-shifted int16
+    diagonal-shift int16 schedule 14 for a diagonal-shift fp16 one, the
+    windowed int16 schedule 11 for a windowed fp16 one, the row-shifted int16
     schedule 6 for the other shifted ones, 3 for the unshifted."""
     import eschalign_amd as ea
     info = vinfo(v)
     if not info.shifted:
         return 3
-    sched = 11 if info.schedule >= 9 else 6
+    sched = 14 if info.diag else 11 if info.schedule >= 9 else 6
     return next(u for u, x in ea.variants().items() if x.schedule == sched and x.stripe_rows == info.stripe_rows)
 
 
@@ -917,7 +919,8 @@ def main():
             "frac_of_measured": round(achieved / peak["lane_ops_per_s"], 4),
             "ops_per_cell": ops,
             "ops_basis": "the minimum packed lane-ops per cell of the formulation the kernel runs "
-                         "(6: fp16 with the row shift; 7: int16 shifted or fp16 unshifted; 8: int16)",
+                         "(5: fp16 with the diagonal shift; 6: fp16 with the row shift or int16 with the "
+                         "diagonal shift; 7: int16 row-shifted or fp16 unshifted; 8: int16)",
             "valu_lane_ops_per_cell_issued": VALU_OPS_PER_CELL_ISSUED.get((args.mode, chosen)),
             "traffic_unit": "B per k_dp16 launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE)",
             "traffic_source": traffic_src}

@@ -64,7 +64,7 @@ EXPORTS = [
     "esch_batch_create", "esch_batch_free", "esch_batch_footprint", "esch_batch_load",
     "esch_batch_load_concat", "esch_batch_run", "esch_batch_wait", "esch_batch_timing",
     "esch_batch_stats", "esch_batch_read_scores32", "esch_batch_read_scores16",
-    "esch_batch_read_select", "esch_batch_read_pairs", "esch_n_variants", "esch_variant_info", "esch_get_variant",
+    "esch_batch_read_select", "esch_batch_read_pairs", "esch_n_variants", "esch_variant_info", "esch_variant_usable", "esch_get_variant",
     "esch_set_variant", "esch_autotune", "esch_adapter_find", "esch_adapter_detect_pair",
     "esch_batch_ready", "esch_server_new", "esch_server_free", "esch_server_add_panel",
     "esch_server_scorer", "esch_server_submit", "esch_server_wait", "esch_server_result",
@@ -185,6 +185,7 @@ def load_library():
     L.esch_batch_read_pairs.argtypes = [vp, vp, vp]
     L.esch_n_variants.restype = C.c_int
     L.esch_variant_info.argtypes = [C.c_int] + [C.POINTER(C.c_int)] * 4
+    L.esch_variant_usable.argtypes = [vp, C.c_int]
     L.esch_get_variant.argtypes = [vp]
     L.esch_set_variant.argtypes = [vp, C.c_int]
     L.esch_autotune.argtypes = [vp, vp, vp, sz, C.c_int, vp, C.POINTER(C.c_int)]
@@ -451,15 +452,20 @@ class VariantInfo:
     schedule: int
     stripe_rows: int
     f16: bool
-    shifted: bool
+    shifted: int  # 0 none, 1 one axis (rows / columns), 2 both (the diagonal shift)
+
+    @property
+    def diag(self) -> bool:
+        """The diagonal shift (semiglobal mode only)."""
+        return self.shifted == 2
 
 
 def variant_info(variant: int) -> VariantInfo:
     """esch_variant_info: a batched-kernel variant's schedule, rows per
-    stripe, fp16 arithmetic (the 2048 condition) and shifted column."""
+    stripe, fp16 arithmetic (the 2048 condition) and shifted values."""
     out = [C.c_int() for _ in range(4)]
     _check(load_library().esch_variant_info(variant, *[C.byref(x) for x in out]))
-    return VariantInfo(out[0].value, out[1].value, bool(out[2].value), bool(out[3].value))
+    return VariantInfo(out[0].value, out[1].value, bool(out[2].value), out[3].value)
 
 
 def variants() -> dict:
@@ -851,6 +857,11 @@ class GpuScorer:
     @variant.setter
     def variant(self, v: int):
         _check(_lib.esch_set_variant(self._h, int(v)))
+
+    def usable_variants(self) -> list:
+        """The batched-kernel variants this scorer admits (its mode, and the
+        panel within each variant's exactness condition)."""
+        return [v for v in range(_lib.esch_n_variants()) if _lib.esch_variant_usable(self._h, v)]
 
     @property
     def diag_min_len(self) -> int:

@@ -189,7 +189,18 @@ __device__ __forceinline__ void column16(const typename Pk<F16>::T (&sc)[RR], ty
 // space, whose bottom row r' = h - 1 (h its height) is this stripe's row -1:
 // values there are lower by c = e h here. ADJ adds c to Q at row 0, and
 // the profile's row 0 carries + c for the diagonal term.
-template <int RR, bool ADJ, bool F16>
+//
+// DS (the diagonal shift): every value of stripe row r in reference column
+// j is shifted by -e (r + j) instead, which takes the extension out of both
+// gap chains:
+//   P = H - e (r + j) + o,  R = E - e (r + j) + e,  Q = F - e (r + j) + e
+//   R[j]    = max(P[j-1], R[j-1])
+//   Q[r]    = max(P[r-1], Q[r-1])
+//   P[r][j] = max(P[r-1][j-1] + (s - o - e), R, Q) + (o - e)
+// 5 packed ops per row in fp16 (6 in int16). The column part of the shift
+// is the same in every stripe, so the rebase by c is the row shift's; the
+// profile carries s - o - e.
+template <int RR, bool ADJ, bool F16, bool DS = false>
 __device__ __forceinline__ void column16s(const typename Pk<F16>::T (&sc)[RR], typename Pk<F16>::T (&Pv)[RR],
                                           typename Pk<F16>::T (&R)[RR], typename Pk<F16>::T &pu,
                                           typename Pk<F16>::T &q, typename Pk<F16>::T pd,
@@ -198,7 +209,7 @@ __device__ __forceinline__ void column16s(const typename Pk<F16>::T (&sc)[RR], t
     using P = Pk<F16>;
 #pragma unroll
     for (int r = 0; r < RR; ++r) {
-        R[r] = P::add(P::max(Pv[r], R[r]), ve);
+        R[r] = DS ? P::max(Pv[r], R[r]) : P::add(P::max(Pv[r], R[r]), ve);
         q = P::max(pu, q);
         if constexpr (ADJ) {
             if (r == 0) q = P::add(q, vc);
@@ -383,8 +394,8 @@ __device__ __forceinline__ void window_loads(const uint4 *cur, const uint4 *nxt,
 
 // column16s_pair with the windowed profile: on entry quads 0 .. LA - 1 of
 // both columns' planes (cA, cB) are loaded; on exit those of the next pair's
-// (nA, nB). Each cell's arithmetic is column16s'.
-template <int RR, bool ADJ, bool F16, int LA>
+// (nA, nB). Each cell's arithmetic is column16s' (DS: the diagonal shift).
+template <int RR, bool ADJ, bool F16, int LA, bool DS = false>
 __device__ __forceinline__ void column16s_pairw(typename Pk<F16>::T (&scA)[RR], typename Pk<F16>::T (&scB)[RR],
                                                 typename Pk<F16>::T (&Pv)[RR], typename Pk<F16>::T (&R)[RR],
                                                 typename Pk<F16>::T &puA, typename Pk<F16>::T &qA,
@@ -397,7 +408,8 @@ __device__ __forceinline__ void column16s_pairw(typename Pk<F16>::T (&scA)[RR], 
     using T = typename P::T;
     auto row = [&](auto rc, const T (&sc)[RR], T &pu, T &q, T &pd) {
         constexpr int r = decltype(rc)::value;
-        R[r] = P::add(P::max(Pv[r], R[r]), ve);
+        if constexpr (DS) R[r] = P::max(Pv[r], R[r]);
+        else R[r] = P::add(P::max(Pv[r], R[r]), ve);
         q = P::max(pu, q);
         if constexpr (ADJ && r == 0) q = P::add(q, vc);
         const T h = P::max3(P::add(pd, sc[r]), R[r], q);
@@ -566,9 +578,13 @@ __device__ __forceinline__ void stripe16(const uint4 *__restrict__ prof, uint2 *
 // only) runs the row-shifted column (column16s), c being the shift offset
 // of the row above (0 for a first stripe, whose border is given in this
 // stripe's space); CS (local only) the column-shifted one (column16c), with
-// `best` carried in column j's space through the stripe.
+// `best` carried in column j's space through the stripe. DS (with SH and
+// the windowed profile) shifts by row and column at once (column16s' DS):
+// the first stripe's top border P = o + e - e j is advanced per column as
+// CS's, the last stripe's bottom-row best is carried in column j's space,
+// and the last column's rows come back to H + o through + e (r + m - 1).
 template <int RR, bool LOCAL, bool FIRST, bool LAST, bool QUAD, bool F16, bool OCT = false, bool SH = false,
-          bool CS = false, int PQ = 4, int LA = 0>
+          bool CS = false, int PQ = 4, int LA = 0, bool DS = false>
 __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 *__restrict__ scr,
                                           const int lane, const cwords w2, const cwords wn,
                                           const int m, const int row0, const int pA, const int pB,
@@ -577,6 +593,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
     static_assert(RR % 2 == 0 && RR >= 2 && RR <= 4 * PQ, "an even number of rows, at most the plane's");
     static_assert(!(SH && LOCAL), "the row shift is a semiglobal form");
     static_assert(!(CS && !LOCAL), "the column shift is a local form");
+    static_assert(!DS || (SH && LA > 0), "the diagonal shift extends the windowed row-shifted schedules");
     constexpr bool XS = SH || CS;  // a shifted column: pairs run interleaved
     using P = Pk<F16>;
     using v2s = typename P::T;  // the packed value type of this instance
@@ -595,15 +612,16 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
     v2s bestL = NEG;
     // CS: the top border's P = o - e j (first stripe), the floor z_j =
     // e (1 - j), both advanced by -e a column, and best moved from H + o
-    // (column 0's space) to column -1's
-    v2s vtop = P::splat(so), z = P::splat(se);
+    // (column 0's space) to column -1's. DS: the top border's P = o + e - e j.
+    v2s vtop = P::splat(DS ? so + se : so), z = P::splat(se);
     const v2s vn2e = us(P::splat(-2 * se));
     if constexpr (CS) best = P::add(best, us(P::splat(se)));
     v2s Ho[RR], E[RR];
-    // SH: the left border's o - e r as a chain of adds from a base the
-    // compiler cannot see through, so the RR constants are not hoisted out
-    // of the stripe loop into registers held for the whole kernel
-    v2s t = vo;
+    // SH: the left border's o - e r (DS: o + e - e r, column -1) as a chain
+    // of adds from a base the compiler cannot see through, so the RR
+    // constants are not hoisted out of the stripe loop into registers held
+    // for the whole kernel
+    v2s t = DS ? P::splat(so + se) : vo;
     if constexpr (SH) asm volatile("" : "+v"(t));
 #pragma unroll
     for (int r = 0; r < RR; ++r) {
@@ -611,7 +629,9 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
         if constexpr (SH) t = P::add(t, vne);
         E[r] = NEG;
     }
-    v2s hdt = SH ? P::splat(so + se - c) : CS ? us(P::splat(so + se)) : vo;
+    // the corner above column 0's row 0, in the space of the row above
+    // (lower by c after a first stripe)
+    v2s hdt = DS ? P::splat(so + 2 * se - c) : SH ? P::splat(so + se - c) : CS ? us(P::splat(so + se)) : vo;
     auto code = [&](int j) -> int {
         const uint32_t c2 = (w2[j >> 4] >> (2 * (j & 15))) & 3u;
         return ((wn[j >> 5] >> (j & 31)) & 1u) ? 4 : (int)c2;
@@ -641,7 +661,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
     };
     auto col = [&](const v2s (&sc)[RR], uint2 b) -> uint2 {
         v2s hou = P::from(b.x), f = P::from(b.y);
-        if constexpr (CS && FIRST) {
+        if constexpr ((CS || DS) && FIRST) {
             hou = vtop;
             vtop = P::add(vtop, vne);
         }
@@ -652,8 +672,8 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
             z = P::add(z, vne);
             best = P::max(P::add(best, vne), tcol);
         } else if constexpr (SH) {
-            column16s<RR, !FIRST, F16>(sc, Ho, E, hou, f, hod, voe, vse, vc);
-            if constexpr (LAST) bestL = P::max(bestL, hou);
+            column16s<RR, !FIRST, F16, DS>(sc, Ho, E, hou, f, hod, voe, vse, vc);
+            if constexpr (LAST) bestL = DS ? P::max(P::add(bestL, vne), hou) : P::max(bestL, hou);
         } else {
             column16<RR, LOCAL, F16>(sc, Ho, E, hou, f, hod, vo, ve, best);
             if constexpr (!LOCAL && LAST) best = P::max(best, hou);
@@ -713,19 +733,26 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
             o1 = make_uint2(P::bits(puB), P::bits(qB));
         } else if constexpr (SH) {
             v2s puA = P::from(x0.x), qA = P::from(x0.y), puB = P::from(x1.x), qB = P::from(x1.y);
+            if constexpr (DS && FIRST) {
+                puA = vtop;
+                puB = P::add(vtop, vne);
+                vtop = P::add(puB, vne);
+            }
             const v2s pdA = hdt, pdB = puA;
             hdt = puB;
             const int ja = jj + 2 < m ? jj + 2 : m, jb = jj + 3 < m ? jj + 3 : m;
             const uint4 *nA = prof + code(ja) * PQ * 64 + lane, *nB = prof + code(jb) * PQ * 64 + lane;
             if constexpr (LA > 0) {
-                column16s_pairw<RR, !FIRST, F16, LA>(sa, sb, Ho, E, puA, qA, pdA, puB, qB, pdB, voe, vse, vc, pa,
-                                                     pb, nA, nB);
+                column16s_pairw<RR, !FIRST, F16, LA, DS>(sa, sb, Ho, E, puA, qA, pdA, puB, qB, pdB, voe, vse, vc,
+                                                         pa, pb, nA, nB);
                 pa = nA;
                 pb = nB;
             } else {
                 column16s_pair<RR, !FIRST, F16>(sa, sb, Ho, E, puA, qA, pdA, puB, qB, pdB, voe, vse, vc, nA, nB);
             }
-            if constexpr (LAST) bestL = P::max3(bestL, puA, puB);
+            // DS: bestL from column jj - 1's space to jj + 1's
+            if constexpr (LAST && DS) bestL = P::max3(P::add(bestL, vn2e), P::add(puA, vne), puB);
+            else if constexpr (LAST) bestL = P::max3(bestL, puA, puB);
             o0 = make_uint2(P::bits(puA), P::bits(qA));
             o1 = make_uint2(P::bits(puB), P::bits(qB));
         } else {
@@ -877,8 +904,8 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
     if constexpr (CS) best = P::add(best, us(P::splat(se * (m - 1))));  // column m - 1's space to H + o
     if constexpr (!LOCAL) {
         // the last column's real rows, in Ho = H + o space (SH: P + e r,
-        // the e r again a chain from an opaque zero)
-        v2s k = P::splat(0);
+        // the e r again a chain from an opaque zero; DS: from e (m - 1))
+        v2s k = P::splat(DS ? se * (m - 1) : 0);
         if constexpr (SH) asm volatile("" : "+v"(k));
 #pragma unroll
         for (int r = 0; r < RR; ++r) {
@@ -890,20 +917,21 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
             if (row0 + r < pB) P::neg_hi(v);
             best = P::max(best, v);
         }
-        if constexpr (SH && LAST) best = P::max(best, P::add(bestL, k));  // k = e (RR - 1)
+        // k = e (RR - 1) (DS: e (RR - 1 + m - 1), bestL being in column m - 1's space)
+        if constexpr (SH && LAST) best = P::max(best, P::add(bestL, k));
     }
 }
 
 template <int RR, bool LOCAL, bool FIRST, bool QUAD, bool F16, bool OCT, bool SH = false, bool CS = false, int PQ = 4,
-          int LA = 0>
+          int LA = 0, bool DS = false>
 __device__ __forceinline__ void stripe16d_any(const uint4 *prof, uint2 *scr, const int lane,
                                               const cwords w2, const cwords wn, const int m,
                                               const bool last, const int row0, const int pA,
                                               const int pB, const typename Pk<F16>::T vo,
                                               const typename Pk<F16>::T ve, typename Pk<F16>::T &best,
                                               const int c = 0) {
-    if (last) stripe16d<RR, LOCAL, FIRST, true, QUAD, F16, OCT, SH, CS, PQ, LA>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best, c);
-    else stripe16d<RR, LOCAL, FIRST, false, QUAD, F16, OCT, SH, CS, PQ, LA>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best, c);
+    if (last) stripe16d<RR, LOCAL, FIRST, true, QUAD, F16, OCT, SH, CS, PQ, LA, DS>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best, c);
+    else stripe16d<RR, LOCAL, FIRST, false, QUAD, F16, OCT, SH, CS, PQ, LA, DS>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best, c);
 }
 
 template <bool LOCAL, int PF, int ST = kStripe>
@@ -930,18 +958,24 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
     // 9, 10 and 11: schedules 8, 7 and 6 with the windowed profile (each
     // column's plane read one quad ahead of its rows instead of a pair
     // ahead: fewer registers, so taller stripes or one more wave)
-    constexpr bool F16 = PF == 4 || PF == 5 || PF == 7 || PF == 8 || PF == 9 || PF == 10;
-    constexpr bool OCT = PF == 5 || PF == 7 || PF == 10;
+    // 12, 13 and 14 (semiglobal only): 9, 10 and 11 with the diagonal shift
+    // (column16s' DS: both gap chains without their extension add)
+    static_assert(!(LOCAL && PF >= 12), "the diagonal shift is a semiglobal form");
+    constexpr bool F16 = PF == 4 || PF == 5 || PF == 7 || PF == 8 || PF == 9 || PF == 10 || PF == 12 || PF == 13;
+    constexpr bool OCT = PF == 5 || PF == 7 || PF == 10 || PF == 13;
     constexpr int LA = PF >= 9 ? 1 : 0;
     constexpr bool SH = !LOCAL && PF >= 6;
     constexpr bool CS = LOCAL && PF >= 6;  // local mode: the column shift (column16c)
+    constexpr bool DS = !LOCAL && PF >= 12;
     using PK = Pk<F16>;
     using V = typename PK::T;
     const V vo = PK::splat(a.sc.o), ve = PK::splat(a.sc.e), NEG = PK::neg();
     // The profile's values, encoded once: s - o for a match and for a
-    // mismatch, and -o for the top padding rows (the values prof_val gives).
-    const uint32_t eMa = PK::enc(a.sc.ma - a.sc.o), eMi = PK::enc(a.sc.mi - a.sc.o);
-    const uint32_t ePre = PK::enc(-a.sc.o);
+    // mismatch, and -o for the top padding rows (the values prof_val gives);
+    // DS: each less e.
+    const int dse = DS ? a.sc.e : 0;
+    const uint32_t eMa = PK::enc(a.sc.ma - a.sc.o - dse), eMi = PK::enc(a.sc.mi - a.sc.o - dse);
+    const uint32_t ePre = PK::enc(-a.sc.o - dse);
 
     for (;;) {
         __syncthreads();
@@ -1003,9 +1037,9 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
                     const int xb = (i < pB || i >= T) ? kPreCode : qB[i - pB];
                     // the stripe's row 0 carries the shift offset (SH)
                     const bool top = SH && cadj != 0 && i == row0;
-                    const uint32_t rMa = top ? PK::enc(a.sc.ma - a.sc.o + cadj) : eMa;
-                    const uint32_t rMi = top ? PK::enc(a.sc.mi - a.sc.o + cadj) : eMi;
-                    const uint32_t rPre = top ? PK::enc(-a.sc.o + cadj) : ePre;
+                    const uint32_t rMa = top ? PK::enc(a.sc.ma - a.sc.o - dse + cadj) : eMa;
+                    const uint32_t rMi = top ? PK::enc(a.sc.mi - a.sc.o - dse + cadj) : eMi;
+                    const uint32_t rPre = top ? PK::enc(-a.sc.o - dse + cadj) : ePre;
 #pragma unroll
                     for (int c = 0; c < 4; ++c) {
                         const uint32_t ea = xa == kPreCode ? rPre : (xa == c || xa == 4) ? rMa : rMi;
@@ -1025,22 +1059,22 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
             if (!active) continue;
             const bool first = s == 0, last = s + 1 == S;
             if constexpr (PF >= 2) {
-                constexpr bool Q = PF == 3 || PF == 4 || PF == 6 || PF == 8 || PF == 9 || PF == 11;
+                constexpr bool Q = PF == 3 || PF == 4 || PF == 6 || PF == 8 || PF == 9 || PF == 11 || PF == 12 || PF == 14;
                 if (s > 0) {
-                    stripe16d_any<ST, LOCAL, false, Q, F16, OCT, SH, CS, PQ, LA>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best, cadj);
+                    stripe16d_any<ST, LOCAL, false, Q, F16, OCT, SH, CS, PQ, LA, DS>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best, cadj);
                 } else {
                     switch (RR) {
 #define ESCH_FIRST(H) \
-    case H: stripe16d_any<(H <= ST ? H : 2), LOCAL, true, Q, F16, OCT, SH, CS, PQ, LA>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best); break;
+    case H: stripe16d_any<(H <= ST ? H : 2), LOCAL, true, Q, F16, OCT, SH, CS, PQ, LA, DS>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best); break;
                         ESCH_FIRST(16) ESCH_FIRST(14) ESCH_FIRST(12) ESCH_FIRST(10)
                         ESCH_FIRST(8) ESCH_FIRST(6) ESCH_FIRST(4)
 #undef ESCH_FIRST
 #define ESCH_FIRST(H) \
-    case H: if constexpr (ST >= H) stripe16d_any<(H <= ST ? H : 2), LOCAL, true, Q, F16, OCT, SH, CS, PQ, LA>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best); break;
+    case H: if constexpr (ST >= H) stripe16d_any<(H <= ST ? H : 2), LOCAL, true, Q, F16, OCT, SH, CS, PQ, LA, DS>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best); break;
                         ESCH_FIRST(32) ESCH_FIRST(30) ESCH_FIRST(28) ESCH_FIRST(26) ESCH_FIRST(24)
                         ESCH_FIRST(22) ESCH_FIRST(20) ESCH_FIRST(18)
 #undef ESCH_FIRST
-                        default: stripe16d_any<2, LOCAL, true, Q, F16, OCT, SH, CS, PQ, LA>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
+                        default: stripe16d_any<2, LOCAL, true, Q, F16, OCT, SH, CS, PQ, LA, DS>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
                     }
                 }
             } else {
@@ -1077,75 +1111,81 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
 // The variants of one mode and stripe height: the kernel's host handle and
 // its launch. Each (mode, stripe height) has a translation unit of its own
 // (kernels_{sg,lo}{,24,32}.hip), so they compile in parallel; the 24-row
-// units instantiate the schedules 4-11, the 32-row ones 6-11.
-template <bool LOCAL, int ST>
-inline const void *dp16_fn_impl(int variant) {
-    if constexpr (ST >= 24) {
-        switch (dp16_schedule(variant)) {
-            case 11: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 11, ST>);
-            case 10: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 10, ST>);
-            case 9: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 9, ST>);
+// units instantiate the schedules 4-14 (local mode 4-11), the 32-row ones
+// 6-14 (local 6-11). The diagonal-shift schedules 12-14 are semiglobal
+// forms: in local mode, where the scorer refuses them, their variants name
+// the schedules they extend (9-11), so every variant has a code object.
+template <bool LOCAL>
+constexpr int dp16_mode_schedule(int variant) {
+    const int sc = dp16_schedule(variant);
+    return LOCAL && sc >= 12 ? sc - 3 : sc;
+}
+
+template <bool LOCAL, int ST, int PF>
+inline const void *dp16_handle() {
+    return reinterpret_cast<const void *>(&k_dp16<LOCAL, PF, ST>);
+}
+
+// Calls f(std::integral_constant<int, PF>) for the schedule PF that
+// `variant` runs in this mode on ST-row stripes; false when this unit has
+// no such instance.
+template <bool LOCAL, int ST, typename F>
+inline bool dp16_dispatch(int variant, F &&f) {
+    const int sc = ST == 16 ? variant : dp16_mode_schedule<LOCAL>(variant);
+    if constexpr (ST >= 24 && !LOCAL) {
+        switch (sc) {
+            case 14: f(std::integral_constant<int, 14>{}); return true;
+            case 13: f(std::integral_constant<int, 13>{}); return true;
+            case 12: f(std::integral_constant<int, 12>{}); return true;
             default: break;
         }
     }
-    switch (dp16_schedule(variant)) {
-        case 8: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 8, ST>);
-        case 7: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 7, ST>);
-        case 6: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 6, ST>);
+    if constexpr (ST >= 24) {
+        switch (sc) {
+            case 11: f(std::integral_constant<int, 11>{}); return true;
+            case 10: f(std::integral_constant<int, 10>{}); return true;
+            case 9: f(std::integral_constant<int, 9>{}); return true;
+            default: break;
+        }
+    }
+    switch (sc) {
+        case 8: f(std::integral_constant<int, 8>{}); return true;
+        case 7: f(std::integral_constant<int, 7>{}); return true;
+        case 6: f(std::integral_constant<int, 6>{}); return true;
         default: break;
     }
     if constexpr (ST <= 24) {
-        switch (dp16_schedule(variant)) {
-            case 5: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 5, ST>);
-            case 4: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 4, ST>);
+        switch (sc) {
+            case 5: f(std::integral_constant<int, 5>{}); return true;
+            case 4: f(std::integral_constant<int, 4>{}); return true;
             default: break;
         }
     }
     if constexpr (ST == 16) {
-        switch (variant) {
-            case 3: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 3>);
-            case 2: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 2>);
-            case 1: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 1>);
-            default: return reinterpret_cast<const void *>(&k_dp16<LOCAL, 0>);
+        switch (sc) {
+            case 3: f(std::integral_constant<int, 3>{}); return true;
+            case 2: f(std::integral_constant<int, 2>{}); return true;
+            case 1: f(std::integral_constant<int, 1>{}); return true;
+            default: f(std::integral_constant<int, 0>{}); return true;
         }
     }
-    return nullptr;
+    return false;
+}
+
+template <bool LOCAL, int ST>
+inline const void *dp16_fn_impl(int variant) {
+    const void *fn = nullptr;
+    dp16_dispatch<LOCAL, ST>(variant, [&](auto pf) { fn = dp16_handle<LOCAL, ST, decltype(pf)::value>(); });
+    return fn;
 }
 
 template <bool LOCAL, int ST>
 inline hipError_t launch_dp16_impl(const KArgs &a, const Launch &g, hipStream_t st) {
     const dim3 grid(g.grid16), block(64 * kWaves16);
-    if constexpr (ST >= 24) {
-        switch (dp16_schedule(g.variant16)) {
-            case 11: hipLaunchKernelGGL((k_dp16<LOCAL, 11, ST>), grid, block, 0, st, a); return hipGetLastError();
-            case 10: hipLaunchKernelGGL((k_dp16<LOCAL, 10, ST>), grid, block, 0, st, a); return hipGetLastError();
-            case 9: hipLaunchKernelGGL((k_dp16<LOCAL, 9, ST>), grid, block, 0, st, a); return hipGetLastError();
-            default: break;
-        }
-    }
-    switch (dp16_schedule(g.variant16)) {
-        case 8: hipLaunchKernelGGL((k_dp16<LOCAL, 8, ST>), grid, block, 0, st, a); return hipGetLastError();
-        case 7: hipLaunchKernelGGL((k_dp16<LOCAL, 7, ST>), grid, block, 0, st, a); return hipGetLastError();
-        case 6: hipLaunchKernelGGL((k_dp16<LOCAL, 6, ST>), grid, block, 0, st, a); return hipGetLastError();
-        default: break;
-    }
-    if constexpr (ST <= 24) {
-        switch (dp16_schedule(g.variant16)) {
-            case 5: hipLaunchKernelGGL((k_dp16<LOCAL, 5, ST>), grid, block, 0, st, a); return hipGetLastError();
-            case 4: hipLaunchKernelGGL((k_dp16<LOCAL, 4, ST>), grid, block, 0, st, a); return hipGetLastError();
-            default: break;
-        }
-    }
-    if constexpr (ST == 16) {
-        switch (g.variant16) {
-            case 3: hipLaunchKernelGGL((k_dp16<LOCAL, 3>), grid, block, 0, st, a); break;
-            case 2: hipLaunchKernelGGL((k_dp16<LOCAL, 2>), grid, block, 0, st, a); break;
-            case 1: hipLaunchKernelGGL((k_dp16<LOCAL, 1>), grid, block, 0, st, a); break;
-            default: hipLaunchKernelGGL((k_dp16<LOCAL, 0>), grid, block, 0, st, a); break;
-        }
-        return hipGetLastError();
-    }
-    return hipErrorInvalidValue;
+    const bool ok = dp16_dispatch<LOCAL, ST>(g.variant16, [&](auto pf) {
+        hipLaunchKernelGGL((k_dp16<LOCAL, decltype(pf)::value, ST>), grid, block, 0, st, a);
+    });
+    return ok ? hipGetLastError() : hipErrorInvalidValue;
 }
 
 // Defined in kernels_{sg,lo}{,24,32}.hip.

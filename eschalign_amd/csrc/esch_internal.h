@@ -54,6 +54,20 @@ __host__ __device__ inline int64_t colshift_bound(int64_t max_ref_len, int32_t m
     const int64_t MM = M > Mi ? M : Mi;
     return max_ref_len * (MM + Ee) + O + 3 * Ee + MM;
 }
+// The diagonal-shift semiglobal schedules (12-14) hold a value of stripe
+// row r in reference column j shifted by |e| (r + j) (r = -1 .. ST - 1,
+// j = -1 .. L - 1) and offset by o or e: every value lies within
+// [-(kM + |o| + M + (ST + 1)|e|), kM + |e| (ST + L - 2)], the profile's
+// stripe-top entries s - o - e + e h included, so the bound grows with the
+// longest reference L as the column shift's does, and with the stripe as
+// the row shift's does.
+__host__ __device__ inline int64_t diagshift_bound(int64_t max_ref_len, int32_t ma, int32_t mi, int32_t o, int32_t e,
+                                                   int stripe) {
+    const int64_t M = ma < 0 ? -(int64_t)ma : ma, Mi = mi < 0 ? -(int64_t)mi : mi;
+    const int64_t O = o < 0 ? -(int64_t)o : o, Ee = e < 0 ? -(int64_t)e : e;
+    const int64_t MM = M > Mi ? M : Mi;
+    return max_ref_len * (MM + Ee) + O + (stripe + 2) * Ee + MM;
+}
 // -inf of the int32 kernel, the oracle's i32::MIN / 4 (lib/scalar.rs:42).
 constexpr int32_t kNeg32 = INT32_MIN / 4;
 
@@ -180,22 +194,28 @@ struct Launch {
 // schedules 9, 10, 11 (8, 7 and 6 with each column's plane read one quad
 // ahead of its rows, not a column pair ahead), on 24 and 32 rows: the
 // plane no longer holds a register per row, so 24-row stripes fit 4 waves
-// per SIMD and 32-row ones 3.
-constexpr int kDp16Variants = 23;
+// per SIMD and 32-row ones 3. 23-28 are the diagonal-shift schedules 12,
+// 13, 14 (9, 10, 11 with every value shifted by row and column, so neither
+// gap chain adds its extension: 5 packed ops per cell in fp16, 6 in int16;
+// semiglobal mode only, diagshift_bound) on 24 and 32 rows.
+constexpr int kDp16Variants = 29;
 constexpr int kDp16Default = 3;  // the int16 default; panels within kBoundF16 take kDp16F16
 constexpr int kDp16F16 = 4;      // the first variant computing in fp16
-constexpr int kDp16Sched[kDp16Variants] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 7, 8, 6, 7, 8, 6, 4, 5, 9, 9, 10, 10, 11, 11};
-constexpr int kDp16Rows[kDp16Variants] = {16, 16, 16, 16, 16, 16, 16, 16, 16, 24, 24, 24,
-                                          32, 32, 32, 24, 24, 24, 32, 24, 32, 24, 32};
+constexpr int kDp16Sched[kDp16Variants] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 7, 8, 6, 7, 8, 6, 4, 5, 9, 9, 10, 10, 11, 11,
+                                           12, 12, 13, 13, 14, 14};
+constexpr int kDp16Rows[kDp16Variants] = {16, 16, 16, 16, 16, 16, 16, 16, 16, 24, 24, 24, 32, 32, 32,
+                                          24, 24, 24, 32, 24, 32, 24, 32, 24, 32, 24, 32, 24, 32};
 constexpr int dp16_schedule(int v) { return kDp16Sched[v]; }
 constexpr int dp16_stripe(int v) { return kDp16Rows[v]; }
 // variants computing in fp16, and those with a shifted column (row-shifted
 // in semiglobal mode, column-shifted in local mode)
 constexpr bool dp16_is_f16(int v) {
     return dp16_schedule(v) == 4 || dp16_schedule(v) == 5 || dp16_schedule(v) == 7 || dp16_schedule(v) == 8 ||
-           dp16_schedule(v) == 9 || dp16_schedule(v) == 10;
+           dp16_schedule(v) == 9 || dp16_schedule(v) == 10 || dp16_schedule(v) == 12 || dp16_schedule(v) == 13;
 }
 constexpr bool dp16_is_shifted(int v) { return dp16_schedule(v) >= 6; }
+// the diagonal-shift schedules (semiglobal mode only)
+constexpr bool dp16_is_diag(int v) { return dp16_schedule(v) >= 12; }
 // The variant with schedule `schedule` on stripes of `stripe` rows (-1: none).
 constexpr int dp16_variant(int schedule, int stripe) {
     for (int v = 0; v < kDp16Variants; ++v)

@@ -234,12 +234,26 @@ esch_status esch_scorer_new(int ordinal, const uint8_t *const *refs, const size_
             const int64_t sb = local ? esch::colshift_bound((int64_t)max_len, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e)
                                      : esch::shifted_bound((int64_t)max_len, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e,
                                                            esch::dp16_stripe(v));
-            const bool shift_ok = !esch::dp16_is_shifted(v) || sb <= (f16 ? esch::kBoundF16 : esch::kBound16);
-            s->usable16[v] = (!f16 || s->f16_ok) && shift_ok;
+            const int64_t lim = f16 ? esch::kBoundF16 : esch::kBound16;
+            const bool shift_ok = !esch::dp16_is_shifted(v) || sb <= lim;
+            // the diagonal shift: semiglobal mode, within its own bound
+            const bool diag_ok =
+                !esch::dp16_is_diag(v) ||
+                (!local && esch::diagshift_bound((int64_t)max_len, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e,
+                                                 esch::dp16_stripe(v)) <= lim);
+            s->usable16[v] = (!f16 || s->f16_ok) && shift_ok && diag_ok;
         }
     }
-    // the shifted variants where their bound holds
-    const int v0 = s->usable16[7] ? 7 : s->f16_ok ? kDp16F16 : s->usable16[6] ? 6 : kDp16Default;
+    // semiglobal mode: the diagonal shift on 32-row stripes (fp16, else
+    // int16) where its bound holds; then the shifted variants where theirs
+    // does
+    const int vd16 = esch::dp16_variant(12, 32), vd = esch::dp16_variant(14, 32);
+    const int v0 = s->usable16[vd16] ? vd16
+                   : s->usable16[vd] ? vd
+                   : s->usable16[7]  ? 7
+                   : s->f16_ok       ? kDp16F16
+                   : s->usable16[6]  ? 6
+                                     : kDp16Default;
     s->geo.variant16 = v0;
     s->geo.grid16 = s->grid16_v[v0];
     s->regs16 = s->regs16_v[v0];
@@ -644,19 +658,22 @@ esch_status enqueue(esch_batch *b, int what, bool timed, const Launch &g) {
 // bounded through k = min(read, reference), so the batch's longest read
 // caps k as well as the panel's longest reference does: short reads on a
 // panel of long references still get fp16 (semiglobal: the row-shifted 7,
-// 6 ops per cell, local: 4). The column shift's range grows with the
-// reference itself, so local mode does not take it from the reads.
+// or 10 for the windowed int16 schedule 11, 6 ops per cell; local: 4). The
+// column shift's range grows with the reference itself, so local mode does
+// not take it from the reads, and neither does the diagonal shift: its
+// int16 schedule (6 ops per cell) stays, the row-shifted fp16 one being no
+// fewer ops.
 int batch_variant(const esch_batch *b, int v, bool fixed) {
     const esch_scorer *s = b->s;
-    if (fixed || esch::dp16_is_f16(v)) return v;
+    if (fixed || esch::dp16_is_f16(v) || esch::dp16_is_diag(v)) return v;
     const int64_t k = (int64_t)std::min<size_t>(s->max_len, b->longest);
     if (!fits_f16(k, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e)) return v;
     if (s->sc.local) return kDp16F16;
-    // the row-shifted fp16 schedule on the same stripe height
+    // the row-shifted fp16 schedule on the same stripe height (the windowed
+    // one for a windowed pick)
     const int st = esch::dp16_stripe(v);
-    return esch::shifted_bound(k, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e, st) <= esch::kBoundF16
-               ? esch::dp16_variant(7, st)
-               : kDp16F16;
+    const int up = esch::dp16_variant(esch::dp16_schedule(v) == 11 ? 10 : 7, st);
+    return esch::shifted_bound(k, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e, st) <= esch::kBoundF16 ? up : kDp16F16;
 }
 
 // Rows up to which a batch is scored with the roles swapped. With the
@@ -1160,6 +1177,12 @@ esch_status set_variant(esch_scorer *s, int variant, bool fixed) {
                     "variant %d (fp16) needs longest reference x max(|match|, |mismatch|) + |open| + "
                     "|extend| <= %lld",
                     variant, (long long)esch::kBoundF16);
+    if (!s->usable16[variant] && esch::dp16_is_diag(variant))
+        return fail(ESCH_INVALID_ARG,
+                    s->sc.local ? "variant %d (diagonal shift) is a semiglobal form; local mode refuses it"
+                                : "variant %d (diagonal shift) needs longest reference x (max(|match|, |mismatch|) + "
+                                  "|extend|) + |open| + (stripe rows + 2) |extend| + max(|match|, |mismatch|) <= %lld",
+                    variant, (long long)(esch::dp16_is_f16(variant) ? esch::kBoundF16 : esch::kBound16));
     if (!s->usable16[variant])
         return fail(ESCH_INVALID_ARG,
                     s->sc.local ? "variant %d (column-shifted in local mode) needs longest reference x "
@@ -1268,8 +1291,13 @@ esch_status esch_variant_info(int variant, int *schedule, int *stripe_rows, int 
     if (schedule) *schedule = esch::dp16_schedule(variant);
     if (stripe_rows) *stripe_rows = esch::dp16_stripe(variant);
     if (f16) *f16 = esch::dp16_is_f16(variant) ? 1 : 0;
-    if (shifted) *shifted = esch::dp16_is_shifted(variant) ? 1 : 0;
+    if (shifted) *shifted = esch::dp16_is_diag(variant) ? 2 : esch::dp16_is_shifted(variant) ? 1 : 0;
     return ESCH_OK;
+}
+
+int esch_variant_usable(const esch_scorer *s, int variant) {
+    if (!s || variant < 0 || variant >= kDp16Variants) return 0;
+    return s->usable16[variant] ? 1 : 0;
 }
 
 esch_status esch_autotune(esch_scorer *s, const uint8_t *const *reads, const size_t *lens, size_t n,

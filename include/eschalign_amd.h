@@ -482,14 +482,27 @@ esch_status esch_bam_build_records(const uint8_t *in, size_t in_bytes, size_t n,
  *      its plane a quad of rows ahead of use rather than a column pair
  *      ahead, so the plane holds a few registers instead of one per row;
  *      17, 18 = 8 on 24 / 32 rows; 19, 20 = 7 on 24 / 32; 21, 22 = 6 (int16)
- *      on 24 / 32. The conditions of the schedules they window. */
+ *      on 24 / 32. The conditions of the schedules they window.
+ *   23-28  schedules 12, 13 and 14: the windowed 8, 7 and 6 with every value
+ *      of stripe row r in reference column j shifted by |e| (r + j), so
+ *      that neither gap chain adds its extension: 5 packed ops per cell in
+ *      fp16 (12, 13), 6 in int16 (14). Semiglobal mode only (local mode
+ *      refuses them); they need longest reference x (M + |extend|) +
+ *      |open| + (stripe rows + 2) |extend| + M within 2048 (fp16) or 32000
+ *      (int16). 23, 24 = 12 on 24 / 32 rows; 25, 26 = 13; 27, 28 = 14. */
 int esch_n_variants(void);
 /* A variant's shape: its schedule (variants 0-8 are schedules 0-8 on 16
- * rows; 9, 10 and 11 are the windowed 8, 7 and 6), rows per stripe, whether
- * it computes in fp16 (the 2048 condition) and whether it runs the shifted
- * column. Returns ESCH_INVALID_ARG for a variant outside
+ * rows; 9, 10 and 11 are the windowed 8, 7 and 6; 12, 13 and 14 those with
+ * the diagonal shift), rows per stripe, whether it computes in fp16 (the
+ * 2048 condition) and how its values are shifted: 0 not at all, 1 along
+ * one axis (rows in semiglobal mode, columns in local mode), 2 along both
+ * (the diagonal shift). Returns ESCH_INVALID_ARG for a variant outside
  * 0 .. esch_n_variants() - 1; any output pointer may be NULL. */
 esch_status esch_variant_info(int variant, int *schedule, int *stripe_rows, int *f16, int *shifted);
+/* 1 when the scorer admits `variant` (its mode, and the panel within the
+ * variant's exactness condition: esch_set_variant accepts it and the
+ * autotuner times it), else 0. */
+int esch_variant_usable(const esch_scorer *s, int variant);
 int esch_get_variant(const esch_scorer *s);
 esch_status esch_set_variant(esch_scorer *s, int variant);
 /* ms[v] (may be NULL, esch_n_variants() entries) receives each variant's DP

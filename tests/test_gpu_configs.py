@@ -48,8 +48,10 @@ def test_config2_batched_uneven_lengths():
     seed 0x5EED_2002 and lognormal(90, 0.6) lengths clipped 20-1000, both
     strands in one device batch, semiglobal 1,-1,-2,-1. The timed batch's
     own int32 rows (as bench.py reads them) equal the CPU kernel for the
-    row-shifted fp16 variant 7, the int16 variant 6 and the plain
-    one-thread-per-pair kernel, eagerly and over three graph replays; the
+    diagonal-shift variants 24, 25 (fp16, the default and the autotuner's
+    picks here) and 28 (int16), the row-shifted fp16 variant 7, the int16
+    variant 6 and the plain one-thread-per-pair kernel, eagerly and over
+    three graph replays; the
     fused selection equals the oracle's on those rows; a sample equals the
     scalar oracle."""
     panel = synth.panel(seed=0x0123456789ABCDEF, n_refs=164, lo=140, hi=160)
@@ -82,7 +84,10 @@ def test_config2_batched_uneven_lengths():
             assert (s.best, s.n_tied, s.suboptimal, s.take) == (b, nt, so, take[:tie_cap]), f"{label}: read {k}"
 
     base = ea.RUN_SCORES32 | ea.RUN_SELECT
-    for v in (7, 6):
+    # the diagonal-shift variants the headline runs (fp16 24, 25 and their
+    # int16 counterpart 28), and the row-shifted 7 (fp16) and 6 (int16)
+    assert gpu.variant == 24  # the default on this panel
+    for v in (24, 25, 28, 7, 6):
         gpu.variant = v
         check(base | ea.RUN_NO_GRAPH, f"variant {v} eager")
         assert batch.variant() == v

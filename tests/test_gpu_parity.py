@@ -350,12 +350,22 @@ def test_every_kernel_variant_equals_oracle_and_autotune_picks_one():
         for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
             gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
             want = oracle(queries, panel.codes, scoring, mode)
-            for v in range(ea.load_library().esch_n_variants()):
+            usable = gpu.usable_variants()
+            # the diagonal-shift variants are semiglobal forms
+            diag = [v for v, x in ea.variants().items() if x.diag]
+            assert set(diag) <= set(usable) if mode == ea.Mode.SEMIGLOBAL else not set(diag) & set(usable)
+            for v in usable:
                 gpu.variant = v
                 s32, ok = gpu.score_batch_i32(queries)
                 assert ok.all() and np.array_equal(s32, want), (scoring, mode, v)
+            for v in set(range(ea.load_library().esch_n_variants())) - set(usable):
+                with pytest.raises(ea.GpuError):
+                    gpu.variant = v
             chosen, ms = gpu.autotune(queries[:120])
-            assert gpu.variant == chosen and all(t > 0 for t in ms)
+            assert gpu.variant == chosen and chosen in usable
+            # the autotuner times what the scorer admits and marks the rest -1
+            assert all((ms[v] > 0) == (v in usable) for v in range(len(ms)))
+            assert all(ms[v] == -1.0 for v in range(len(ms)) if v not in usable)
 
 
 def test_fp16_variant_at_its_bound():
@@ -379,12 +389,14 @@ def test_fp16_variant_at_its_bound():
         panel = ea.Panel(refs)
         for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
             gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
-            assert gpu.variant == 4, (scoring, mode)
+            # semiglobal mode defaults to the diagonal shift in int16 (its
+            # 32000 bound holds, its fp16 one does not)
+            assert gpu.variant == (4 if mode == ea.Mode.LOCAL else 28), (scoring, mode)
             want = oracle(queries, panel.codes, scoring, mode)
             s32, ok = gpu.score_batch_i32(queries)
             assert ok.all() and np.array_equal(s32, want), (scoring, mode)
             assert want.max() >= L - 20, want.max()
-            for v in (5, 3):
+            for v in (4, 5, 3):
                 gpu.variant = v
                 sv, okv = gpu.score_batch_i32(queries)
                 assert okv.all() and np.array_equal(sv, want), (scoring, mode, v)
@@ -398,7 +410,7 @@ def test_fp16_variant_at_its_bound():
         assert chosen not in F16_VARIANTS and all(ms[v] == -1.0 for v in F16_VARIANTS)
 
 
-F16_VARIANTS = (4, 5, 7, 8, 9, 10, 12, 13, 15, 16, 17, 18, 19, 20)
+F16_VARIANTS = (4, 5, 7, 8, 9, 10, 12, 13, 15, 16, 17, 18, 19, 20, 23, 24, 25, 26)
 
 
 def test_row_shifted_variants_at_their_bound():
@@ -446,6 +458,54 @@ def test_row_shifted_variants_at_their_bound():
         longer = ea.Panel(refs + [G.random_seq(rng, L + 1, 0.0)])
         gpu = ea.GpuScorer(longer, ea.Scoring(*scoring), ea.Mode.SEMIGLOBAL)
         for w in others:
+            with pytest.raises(ea.GpuError):
+                gpu.variant = w
+
+
+def test_diagonal_shifted_variants_at_their_bound():
+    """Variants 23-28 hold every value of stripe row r in reference column j
+    shifted by |e| (r + j) (semiglobal mode): exact while longest reference
+    x (M + |e|) + |o| + (rows + 2) |e| + M stays within 2048 (fp16: 23-26)
+    or 32000 (int16: 27, 28). Panels exactly at it, with reads that reach
+    both ends of the value range (a reference read back whole, runs of one
+    base against a reference of another, reads several stripes long), equal
+    the oracle; one base longer, the panel refuses the variant; local mode
+    refuses them on any panel."""
+    rng = G.Rng(0x5EED_0A07)
+    for scoring, L, vs, st in (((1, -1, -3, -1), 1009, (23, 25), 24), ((1, -1, -3, -1), 1005, (24, 26), 32),
+                               ((2, -1, -9, -3), 387, (24, 26), 32),
+                               ((100, -100, -300, -100), 145, (27,), 24),
+                               ((100, -100, -500, -100), 140, (28,), 32)):
+        M = max(abs(scoring[0]), abs(scoring[1]))
+        e = abs(scoring[3])
+        f16 = ea.variant_info(vs[0]).f16
+        assert all(ea.variant_info(w).stripe_rows == st and ea.variant_info(w).diag for w in vs)
+        assert L * (M + e) + abs(scoring[2]) + (st + 2) * e + M == (2048 if f16 else 32000)
+        refs = [G.random_seq(rng, L, 0.0), b"C" * L, G.random_seq(rng, L // 2, 0.02),
+                G.random_seq(rng, 40, 0.0)]
+        reads = [refs[0], b"A" * L, b"A" * (L // 3), b"G" * 5, refs[0][L // 9:L - L // 9],
+                 G.random_seq(rng, L + 40, 0.03), refs[0] + refs[0][:300], b"C" * 3 + refs[0]]
+        reads += [G.random_seq(rng, 1 + rng.below(L), 0.02) for _ in range(40)]
+        queries = codes_of(reads)
+        queries += [ea.reverse_complement_codes(q) for q in queries]
+        panel = ea.Panel(refs)
+        gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), ea.Mode.SEMIGLOBAL)
+        gpu.diag_ratio = 0
+        want = oracle(queries, panel.codes, scoring, ea.Mode.SEMIGLOBAL)
+        for w in vs:
+            gpu.variant = w
+            s32, ok = gpu.score_batch_i32(queries)
+            assert ok.all()
+            bad = np.argwhere(s32 != want)
+            assert bad.size == 0, (scoring, w, bad[:4])
+        assert want.max() >= L * scoring[0] - 20 * abs(scoring[0]), want.max()
+        longer = ea.Panel(refs + [G.random_seq(rng, L + 1, 0.0)])
+        gpu = ea.GpuScorer(longer, ea.Scoring(*scoring), ea.Mode.SEMIGLOBAL)
+        for w in vs:
+            with pytest.raises(ea.GpuError):
+                gpu.variant = w
+        gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), ea.Mode.LOCAL)
+        for w in vs:
             with pytest.raises(ea.GpuError):
                 gpu.variant = w
 
@@ -511,7 +571,7 @@ def test_every_kernel_variant_on_short_references():
             gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
             gpu.diag_ratio = 0
             want = oracle(queries, panel.codes, scoring, mode)
-            for v in range(ea.load_library().esch_n_variants()):
+            for v in gpu.usable_variants():
                 gpu.variant = v
                 s32, ok = gpu.score_batch_i32(queries)
                 assert ok.all()
@@ -810,7 +870,8 @@ def test_autotune_sweeps_the_routing_ratio():
     panel = ea.Panel(panel_l)
     gpu = ea.GpuScorer(panel, ea.Scoring(1, -1, -2, -1), ea.Mode.LOCAL)
     chosen, ms = gpu.autotune(queries, both_strands=False)
-    assert chosen in range(len(ms)) and all(m > 0 for m in ms)
+    usable = gpu.usable_variants()
+    assert chosen in usable and all((ms[v] > 0) == (v in usable) for v in range(len(ms)))
     assert gpu.diag_ratio in (0.0, 1.5, 2.0, 3.0, 4.0)
     gpu.diag_ratio = 3.0
     gpu.autotune(queries, both_strands=False)
@@ -871,46 +932,61 @@ def test_short_reads_take_fp16_on_a_panel_of_long_references():
     """Every fp16 and row-shifted value is bounded through min(read,
     reference), so a batch's longest read caps it as the panel's longest
     reference does. On a panel of 2,100-nt references (no fp16 variant
-    panel-wide: 2,100 + 2 + 1 > 2,048), a batch of reads up to 1,500 nt runs
-    variant 7 in semiglobal mode and 4 in local mode; a batch holding a
-    2,100-nt read (the top score, 2,100, is beyond fp16's exact integers)
-    stays on the scorer's int16 variant 6, and so does every batch once the
-    caller fixes a variant (an autotuned scorer still upgrades). Scores equal the oracle's in every case (the
+    panel-wide: 2,100 + |o| + |e| > 2,048), a batch of reads up to 1,500 nt
+    runs the row-shifted fp16 variant 7 in semiglobal mode (scoring
+    1,-1,-20,-15, whose diagonal-shift bound, 34,131, is over 32,000, so the
+    scorer's default is the int16 variant 6) and 4 in local mode
+    (1,-1,-2,-1); a batch holding a 2,100-nt read (the top score, 2,100, is
+    beyond fp16's exact integers) stays on the scorer's int16 variant 6, and
+    so does every batch once the caller fixes a variant (an autotuned scorer
+    still upgrades). The diagonal shift's range grows with the reference
+    whatever the reads, so a scorer on the diagonal-shift int16 variant 28
+    (semiglobal 1,-1,-2,-1) keeps it for every batch: the row-shifted fp16
+    variant takes no fewer ops. Scores equal the oracle's in every case (the
     captured graphs are keyed by the variant they launch)."""
     rng = G.Rng(0x5EED_0A06)
-    scoring = (1, -1, -2, -1)
     L = 2100
     refs = [G.random_seq(rng, L, 0.0), G.random_seq(rng, L - 300, 0.01), b"C" * 900]
     short = [refs[0][100:1600], refs[1][:700], b"A" * 400, G.random_seq(rng, 1500, 0.03)]
     short += [G.random_seq(rng, 1 + rng.below(1500), 0.02) for _ in range(30)]
     longer = short + [refs[0]]
     panel = ea.Panel(refs)
-    for mode, fast in ((ea.Mode.SEMIGLOBAL, 7), (ea.Mode.LOCAL, 4)):
+    for mode, scoring, default, fast in ((ea.Mode.SEMIGLOBAL, (1, -1, -20, -15), 6, 7),
+                                         (ea.Mode.SEMIGLOBAL, (1, -1, -2, -1), 28, 28),
+                                         (ea.Mode.LOCAL, (1, -1, -2, -1), 6, 4)):
         gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
-        assert gpu.variant == 6
-        for reads, want_v in ((short, fast), (longer, 6), (short, fast)):
+        assert gpu.variant == default, (mode, scoring)
+        for reads, want_v in ((short, fast), (longer, default), (short, fast)):
             q = codes_of(reads)
             b = gpu.batch(len(q), sum(len(x) for x in q))
             b.load(q)
             b.run(ea.RUN_SCORES32)
             b.wait()
             s32, ok = b.scores32()
-            assert b.variant() == want_v, (mode, len(reads), b.variant())
+            assert b.variant() == want_v, (mode, scoring, len(reads), b.variant())
             want = oracle(q, panel.codes, scoring, mode)
-            assert ok.all() and np.array_equal(s32, want), (mode, want_v)
+            assert ok.all() and np.array_equal(s32, want), (mode, scoring, want_v)
         # the autotuner's pick (among the int16 variants this panel admits)
         # is not the caller's: a batch of short reads still takes fp16
         chosen, _ = gpu.autotune(codes_of(short), both_strands=False)
-        assert not ea.variant_info(chosen).f16  # an int16 variant
+        info = ea.variant_info(chosen)
+        assert not info.f16  # an int16 variant
         # semiglobal: the row-shifted fp16 schedule on the pick's stripes
-        rows = ea.variant_info(chosen).stripe_rows
-        up = {16: 7, 24: 9, 32: 12}[rows] if mode == ea.Mode.SEMIGLOBAL else 4
+        # (the windowed one for a windowed pick); a diagonal-shift pick stays
+        rows = info.stripe_rows
+        if mode == ea.Mode.LOCAL:
+            up = 4
+        elif info.diag:
+            up = chosen
+        else:
+            sched = 10 if info.schedule == 11 else 7
+            up = next(v for v, x in ea.variants().items() if x.schedule == sched and x.stripe_rows == rows)
         q = codes_of(short)
         b = gpu.batch(len(q), sum(len(x) for x in q))
         b.load(q)
         b.run(ea.RUN_SCORES32)
         b.wait()
-        assert b.variant() == up, (mode, chosen, b.variant())
+        assert b.variant() == up, (mode, scoring, chosen, b.variant())
         assert np.array_equal(b.scores32()[0], oracle(q, panel.codes, scoring, mode))
         gpu.variant = 6
         q = codes_of(short)

@@ -87,20 +87,23 @@ def test_refusals_before_any_device_work():
 
 def test_variant_shapes():
     """esch_variant_info (host-only): 0-8 are the schedules on 16 rows, 9-16
-    some of them on 24 / 32 rows, 17-22 the windowed schedules 9-11 on 24 and
-    32 rows; fp16 and shifted follow the schedule; out of range refused."""
+    some of them on 24 / 32 rows, 17-22 the windowed schedules 9-11 and
+    23-28 the diagonal-shift schedules 12-14 on 24 and 32 rows; fp16 and
+    the shift follow the schedule; out of range refused."""
     vs = ea.variants()
-    assert len(vs) == 23
+    assert len(vs) == 29
     assert [vs[v].schedule for v in range(9)] == list(range(9))
     assert all(vs[v].stripe_rows == 16 for v in range(9))
-    assert [(vs[v].schedule, vs[v].stripe_rows) for v in range(17, 23)] == [
-        (9, 24), (9, 32), (10, 24), (10, 32), (11, 24), (11, 32)]
+    assert [(vs[v].schedule, vs[v].stripe_rows) for v in range(17, 29)] == [
+        (9, 24), (9, 32), (10, 24), (10, 32), (11, 24), (11, 32),
+        (12, 24), (12, 32), (13, 24), (13, 32), (14, 24), (14, 32)]
     for v, x in vs.items():
-        assert x.f16 == (x.schedule in (4, 5, 7, 8, 9, 10)), v
-        assert x.shifted == (x.schedule >= 6), v
+        assert x.f16 == (x.schedule in (4, 5, 7, 8, 9, 10, 12, 13)), v
+        assert x.shifted == (2 if x.schedule >= 12 else 1 if x.schedule >= 6 else 0), v
+        assert x.diag == (x.schedule >= 12), v
     # each (schedule, stripe height) once
     assert len({(x.schedule, x.stripe_rows) for x in vs.values()}) == len(vs)
-    for bad in (-1, 23):
+    for bad in (-1, 29):
         with pytest.raises(ea.GpuError):
             ea.variant_info(bad)
 

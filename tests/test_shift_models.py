@@ -1,7 +1,8 @@
-"""The shifted value spaces of k_dp16's variants 6-14 (eschalign_amd/csrc/
-kernels.hip: column16s / column16s_pair with the stripe rebase in semiglobal
-mode, column16c / column16c_pair in local mode), modelled step for step in
-Python on the CPU.
+"""The shifted value spaces of k_dp16's shifted variants (eschalign_amd/csrc/
+dp16.h: column16s / column16s_pair with the stripe rebase in semiglobal
+mode, column16c / column16c_pair in local mode, and column16s' diagonal
+shift, by row and column at once, in semiglobal mode), modelled step for
+step in Python on the CPU.
 
 Each model walks a read the way the kernel does. The read is padded at the
 top to its chunk's row plan (a first stripe of RR rows, then stripes of 16),
@@ -13,7 +14,8 @@ the reference's vectors), so a wrong shift, rebase, floor or best
 conversion fails.
 
 They also record every value they form, against the ranges the exactness
-bounds in esch_internal.h claim (shifted_bound, colshift_bound). A value
+bounds in esch_internal.h claim (shifted_bound, colshift_bound,
+diagshift_bound). A value
 outside them would let the fp16 or int16 arithmetic round or clamp.
 """
 import random
@@ -113,6 +115,78 @@ def walk_semiglobal_rows(q, r, scoring, nmax, rng_vals, st=16):
                 best = mx(best, rng_vals.see(Pv[k] + e * k))
         if s == S - 1:
             best = mx(best, rng_vals.see(add(bestL, e * (h - 1))))
+        bottom = newb
+        row0 += h
+    return best - o
+
+
+def walk_semiglobal_diag(q, r, scoring, nmax, rng_vals, st=16, advance_top=True):
+    """The diagonal shift's semiglobal walk (variants 23-28): row r of a
+    stripe in reference column j holds P = H - e (r + j) + o,
+    R = E - e (r + j) + e, Q = F - e (r + j) + e, so neither gap chain adds
+    its extension and the profile carries s - o - e. The row above arrives
+    in the previous stripe's space, lower by c = e h (the column part is the
+    same in every stripe); the first stripe's top border P = o + e - e j is
+    advanced per column; the last stripe's bottom-row best is carried in
+    column j's space (a pair: best - 2e against the pair's A - e and B) and
+    the last column's rows come back to H + o through + e (r + m - 1).
+    advance_top=False leaves the top border at column 0's value (a planted
+    mistake)."""
+    ma, mi, o, e = scoring
+    S, RR, T = plan(nmax, st)
+    rows = [None] * (T - len(q)) + list(q)
+    pad = T - len(q)
+    m = len(r)
+    best = None
+    bottom = None
+    row0 = 0
+    for s in range(S):
+        h = RR if s == 0 else st
+        c = 0 if s == 0 else e * (RR if s == 1 else st)
+        Pv = [o + e - e * k for k in range(h)]  # the left border, column -1
+        R = [NEG] * h
+        hdt = o + 2 * e - c
+        bestL = NEG
+        vtop = o + e
+        newb = []
+
+        def column(j, pu, qq, pd):
+            for k in range(h):
+                sc = subst(rows[row0 + k], r[j], ma, mi) - o - e + (c if k == 0 else 0)
+                R[k] = rng_vals.see(mx(Pv[k], R[k]))
+                qq = rng_vals.see(mx(pu, qq))
+                if k == 0 and s > 0:
+                    qq = rng_vals.see(add(qq, c))
+                hh = rng_vals.see(mx(rng_vals.see(pd + sc), R[k], qq))
+                pd = Pv[k]
+                Pv[k] = rng_vals.see(hh + (o - e))
+                pu = Pv[k]
+            return pu, qq
+
+        j = 0
+        while j < m:
+            cols = (j, j + 1) if j + 1 < m else (j,)
+            for jj in cols:
+                if s == 0:
+                    pu, qq = rng_vals.see(vtop), NEG
+                    if advance_top:
+                        vtop -= e
+                else:
+                    pu, qq = bottom[jj]
+                pd = hdt
+                hdt = pu
+                pu, qq = column(jj, pu, qq, pd)
+                newb.append((pu, qq))
+            if len(cols) == 2:
+                bestL = rng_vals.see(mx(add(bestL, -2 * e), rng_vals.see(newb[-2][0] - e), newb[-1][0]))
+            else:
+                bestL = rng_vals.see(mx(add(bestL, -e), newb[-1][0]))
+            j += 2
+        for k in range(h):
+            if row0 + k >= pad:
+                best = mx(best, rng_vals.see(Pv[k] + e * k + e * (m - 1)))
+        if s == S - 1:
+            best = mx(best, rng_vals.see(add(bestL, e * (h - 1) + e * (m - 1))))
         bottom = newb
         row0 += h
     return best - o
@@ -229,6 +303,40 @@ def test_column_shifted_local_walk_equals_oracle(seed, st):
         k = min(len(q), len(ref))
         lo, hi = -(O_ + 3 * E + M), k * M + E * (len(ref) + 2) + M
         assert lo <= vals.lo and vals.hi <= hi, (sc, vals.lo, vals.hi, lo, hi)
+
+
+@pytest.mark.parametrize("seed,st", [(11, 16), (12, 24), (13, 32), (14, 32)])
+def test_diagonal_shifted_semiglobal_walk_equals_oracle(seed, st):
+    """The diagonal shift (semiglobal, rows and columns at once), on 16-,
+    24- and 32-row stripes: scores equal the oracle's, and every value lies
+    in [-(kM + |o| + M + (st + 1)|e|), kM + |e| (st + m - 2)], m the
+    reference length: the range diagshift_bound(..., st) rests on."""
+    for sc, q, ref, nmax in cases(seed, 200, n_max=4 * st):
+        vals = Range()
+        got = walk_semiglobal_diag(q, ref, sc, nmax, vals, st)
+        want = O.score(np.array(q, np.uint8), np.array(ref, np.uint8), sc, False)
+        assert got == want, (sc, q, ref, nmax, st)
+        M, O_, E = bounds(sc)
+        k = min(len(q), len(ref))
+        lo, hi = -(k * M + O_ + M + (st + 1) * E), k * M + E * (st + len(ref) - 2)
+        assert lo <= vals.lo and vals.hi <= hi, (sc, vals.lo, vals.hi, lo, hi, st)
+        # diagshift_bound covers both ends
+        assert max(-lo, hi) <= len(ref) * (M + E) + O_ + (st + 2) * E + M
+
+
+def test_a_wrong_diagonal_top_border_is_caught():
+    """The diagonal-shift model with the first stripe's top border left at
+    column 0's value (not advanced by -e per column) disagrees with the
+    oracle, so the equality above sees that mistake."""
+    wrong = tried = 0
+    for sc, q, ref, nmax in cases(15, 120):
+        if sc[3] == 0 or len(ref) < 3:
+            continue
+        tried += 1
+        got = walk_semiglobal_diag(q, ref, sc, nmax, Range(), 16, advance_top=False)
+        want = O.score(np.array(q, np.uint8), np.array(ref, np.uint8), sc, False)
+        wrong += got != want
+    assert tried and wrong > 0
 
 
 def test_a_wrong_rebase_is_caught():
