@@ -10,8 +10,8 @@ both strands, semiglobal mode, scoring 1,-1,-2,-1. Synthetic data.
 A step is one pass of the device pipeline over the batch, already resident
 in HBM: length bucketing, the batched packed 16-bit DP (in fp16 on panels
 whose value bound fits its exact integers, as this one does, with every
-stripe row's values shifted so that a cell takes 6 packed ops; int16 and
-int32 rescue where needed) and the fused best / tie selection. value =
+value shifted by row and column so that a cell takes 5 packed ops; int16
+and int32 rescue where needed) and the fused best / tie selection. value =
 reads per second.
 
 Also reported: the DP kernel's time from HIP events on its stream, GCUPS and
@@ -83,6 +83,8 @@ VALU_OPS_PER_CELL_ISSUED = {("semiglobal", 3): round(344_021_737_036 * 128 / 5_3
                             ("semiglobal", 7): round(263_781_725_920 * 128 / 5_303_272_453_076, 3),
                             ("semiglobal", 10): round(262_088_845_368 * 128 / 5_303_272_453_076, 3),
                             ("semiglobal", 11): round(303_721_597_544 * 128 / 5_303_272_453_076, 3),
+                            ("semiglobal", 24): round(220_033_994_246 * 128 / 5_303_272_453_076, 3),
+                            ("semiglobal", 28): round(261_701_617_397 * 128 / 5_303_272_453_076, 3),
                             ("local", 6): round(389_701_369_198 * 128 / 5_303_272_453_076, 3),
                             ("local", 7): round(327_569_070_244 * 128 / 5_303_272_453_076, 3)}
 
