@@ -45,12 +45,14 @@ PEAK_I32 = N_CU * CLOCK_HZ * 64
 # best in local mode; with the fp16 variants' three-input maximum H takes
 # one max instead of two, and the column best a three-way tree. A shift
 # along one axis (shifted 1: rows in semiglobal mode, columns in local
-# mode) drops one gap chain's extension add; the diagonal shift (shifted 2,
-# semiglobal mode only) drops both. Keyed (mode, fp16, shifted).
+# mode) drops one gap chain's extension add, and in local mode lets the
+# zero floor ride on the horizontal chain's maximum (free in fp16, whose
+# three-input maximum is one op); the diagonal shift (shifted 2,
+# semiglobal mode only) drops both adds. Keyed (mode, fp16, shifted).
 OPS_PER_CELL = {("semiglobal", False, 0): 8, ("local", False, 0): 10,
                 ("semiglobal", True, 0): 7, ("local", True, 0): 8.5,
                 ("semiglobal", False, 1): 7, ("local", False, 1): 9,
-                ("semiglobal", True, 1): 6, ("local", True, 1): 7.5,
+                ("semiglobal", True, 1): 6, ("local", True, 1): 6.5,
                 ("semiglobal", False, 2): 6, ("semiglobal", True, 2): 5}
 
 
@@ -84,9 +86,7 @@ VALU_OPS_PER_CELL_ISSUED = {("semiglobal", 3): round(344_021_737_036 * 128 / 5_3
                             ("semiglobal", 10): round(262_088_845_368 * 128 / 5_303_272_453_076, 3),
                             ("semiglobal", 11): round(303_721_597_544 * 128 / 5_303_272_453_076, 3),
                             ("semiglobal", 24): round(220_033_994_246 * 128 / 5_303_272_453_076, 3),
-                            ("semiglobal", 28): round(261_701_617_397 * 128 / 5_303_272_453_076, 3),
-                            ("local", 6): round(389_701_369_198 * 128 / 5_303_272_453_076, 3),
-                            ("local", 7): round(327_569_070_244 * 128 / 5_303_272_453_076, 3)}
+                            ("semiglobal", 28): round(261_701_617_397 * 128 / 5_303_272_453_076, 3)}
 
 
 def log(*a):

@@ -64,7 +64,7 @@ EXPORTS = [
     "esch_batch_create", "esch_batch_free", "esch_batch_footprint", "esch_batch_load",
     "esch_batch_load_concat", "esch_batch_run", "esch_batch_wait", "esch_batch_timing",
     "esch_batch_stats", "esch_batch_read_scores32", "esch_batch_read_scores16",
-    "esch_batch_read_select", "esch_batch_read_pairs", "esch_n_variants", "esch_variant_info", "esch_variant_usable", "esch_get_variant",
+    "esch_batch_read_select", "esch_batch_read_pairs", "esch_n_variants", "esch_variant_info", "esch_variant_usable", "esch_column_block", "esch_get_variant",
     "esch_set_variant", "esch_autotune", "esch_adapter_find", "esch_adapter_detect_pair",
     "esch_batch_ready", "esch_server_new", "esch_server_free", "esch_server_add_panel",
     "esch_server_scorer", "esch_server_submit", "esch_server_wait", "esch_server_result",
@@ -186,6 +186,7 @@ def load_library():
     L.esch_n_variants.restype = C.c_int
     L.esch_variant_info.argtypes = [C.c_int] + [C.POINTER(C.c_int)] * 4
     L.esch_variant_usable.argtypes = [vp, C.c_int]
+    L.esch_column_block.argtypes = [vp, C.c_int]
     L.esch_get_variant.argtypes = [vp]
     L.esch_set_variant.argtypes = [vp, C.c_int]
     L.esch_autotune.argtypes = [vp, vp, vp, sz, C.c_int, vp, C.POINTER(C.c_int)]
@@ -857,6 +858,11 @@ class GpuScorer:
     @variant.setter
     def variant(self, v: int):
         _check(_lib.esch_set_variant(self._h, int(v)))
+
+    def column_block(self, variant: int) -> int:
+        """esch_column_block: the columns after which `variant`'s column
+        shift restarts on this panel (0: never; -1: refused)."""
+        return int(_lib.esch_column_block(self._h, int(variant)))
 
     def usable_variants(self) -> list:
         """The batched-kernel variants this scorer admits (its mode, and the

@@ -289,13 +289,17 @@ __device__ __forceinline__ void column16s_pair(typename Pk<F16>::T (&scA)[RR], t
 // The local column with every value of reference column j shifted by -e j,
 // which takes the extension out of the horizontal gap chain instead:
 //   P = H - e j + o,  R = E - e j + e,  Q = F - e j + e
-//   R[j]    = max(P[j-1], R[j-1])
+//   R[j]    = max(P[j-1], R[j-1], z_j)
 //   Q[r]    = max(P[r-1], Q[r-1]) + e
-//   P[r][j] = max(P[r-1][j-1] + (s - o), R, Q, z_j) + (o - e)
+//   P[r][j] = max(P[r-1][j-1] + (s - o), R, Q) + (o - e)
 // where z_j = e (1 - j) is the zero floor H >= 0 in this space: one value
 // per column, shared by every row (a row shift would need one per row, and a
-// best per row). 7 packed ops per row where the unshifted local column takes
-// 8, and a column's rows share one shift, so their best folds by a tree of
+// best per row). The floor rides on the horizontal chain: carrying
+// E' = max(E, 0) changes no H (by induction E'[j] = max(H[j-1] + o,
+// E'[j-1] + e, 0) = max(E[j], 0), e <= 0), and the cell's maximum then
+// includes it. 6 packed ops per row in fp16 (R's three-input maximum is one
+// op), 8 in int16, where the unshifted local column takes 7.5 / 10 with its
+// tree; a column's rows share one shift, so their best folds by a tree of
 // three-way maxima as before. Returns the column's best P (column j space).
 template <int RR, bool F16>
 __device__ __forceinline__ typename Pk<F16>::T column16c(const typename Pk<F16>::T (&sc)[RR],
@@ -307,9 +311,9 @@ __device__ __forceinline__ typename Pk<F16>::T column16c(const typename Pk<F16>:
     using P = Pk<F16>;
 #pragma unroll
     for (int r = 0; r < RR; ++r) {
-        R[r] = P::max(Pv[r], R[r]);
+        R[r] = P::max3(Pv[r], R[r], z);
         q = P::add(P::max(pu, q), ve);
-        const typename P::T h = P::max(P::max3(P::add(pd, sc[r]), R[r], q), z);
+        const typename P::T h = P::max3(P::add(pd, sc[r]), R[r], q);
         pd = Pv[r];
         Pv[r] = P::add(h, voe);
         pu = Pv[r];
@@ -334,9 +338,9 @@ __device__ __forceinline__ void column16c_pair(typename Pk<F16>::T (&scA)[RR], t
     using T = typename P::T;
     constexpr int NP = (RR + 3) / 4;
     auto row = [&](int r, const T (&sc)[RR], T &pu, T &q, T &pd, T z) {
-        R[r] = P::max(Pv[r], R[r]);
+        R[r] = P::max3(Pv[r], R[r], z);  // the floor on the horizontal chain
         q = P::add(P::max(pu, q), ve);
-        const T h = P::max(P::max3(P::add(pd, sc[r]), R[r], q), z);
+        const T h = P::max3(P::add(pd, sc[r]), R[r], q);
         pd = Pv[r];
         Pv[r] = P::add(h, voe);
         pu = Pv[r];
@@ -447,9 +451,9 @@ __device__ __forceinline__ void column16c_pairw(typename Pk<F16>::T (&scA)[RR], 
     using T = typename P::T;
     auto row = [&](auto rc, const T (&sc)[RR], T &pu, T &q, T &pd, T z) {
         constexpr int r = decltype(rc)::value;
-        R[r] = P::max(Pv[r], R[r]);
+        R[r] = P::max3(Pv[r], R[r], z);  // the floor on the horizontal chain
         q = P::add(P::max(pu, q), ve);
-        const T h = P::max(P::max3(P::add(pd, sc[r]), R[r], q), z);
+        const T h = P::max3(P::add(pd, sc[r]), R[r], q);
         pd = Pv[r];
         Pv[r] = P::add(h, voe);
         pu = Pv[r];
@@ -583,13 +587,20 @@ __device__ __forceinline__ void stripe16(const uint4 *__restrict__ prof, uint2 *
 // the first stripe's top border P = o + e - e j is advanced per column as
 // CS's, the last stripe's bottom-row best is carried in column j's space,
 // and the last column's rows come back to H + o through + e (r + m - 1).
+// CS and DS restart the column part of the shift every cm + 1 columns (cm
+// = -1: never): at a column j with j & cm == 0 every value carried in from
+// column j - 1 (the rows' P and R, the corner, the top border, the floor
+// and the carried bests) moves by + e (cm + 1) into the new block's space,
+// and the stripe's bests return to H + o through column m - 1's place in
+// its block. Every stripe restarts at the same columns, so the boundary a
+// stripe leaves for column j is in the space the next one reads it in.
 template <int RR, bool LOCAL, bool FIRST, bool LAST, bool QUAD, bool F16, bool OCT = false, bool SH = false,
           bool CS = false, int PQ = 4, int LA = 0, bool DS = false>
 __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 *__restrict__ scr,
                                           const int lane, const cwords w2, const cwords wn,
                                           const int m, const int row0, const int pA, const int pB,
                                           const typename Pk<F16>::T vo, const typename Pk<F16>::T ve,
-                                          typename Pk<F16>::T &best, const int c = 0) {
+                                          typename Pk<F16>::T &best, const int c = 0, const int cm = -1) {
     static_assert(RR % 2 == 0 && RR >= 2 && RR <= 4 * PQ, "an even number of rows, at most the plane's");
     static_assert(!(SH && LOCAL), "the row shift is a semiglobal form");
     static_assert(!(CS && !LOCAL), "the column shift is a local form");
@@ -632,6 +643,25 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
     // the corner above column 0's row 0, in the space of the row above
     // (lower by c after a first stripe)
     v2s hdt = DS ? P::splat(so + 2 * se - c) : SH ? P::splat(so + se - c) : CS ? us(P::splat(so + se)) : vo;
+    // CS / DS: the start of a column block (see above); wave-uniform
+    auto rebase = [&](int j) {
+        if constexpr (CS || DS) {
+            if (j == 0 || (j & cm) != 0) return;
+            const v2s vk = us(P::splat(se * (cm + 1)));
+#pragma unroll
+            for (int r = 0; r < RR; ++r) {
+                Ho[r] = P::add(Ho[r], vk);
+                E[r] = P::add(E[r], vk);
+            }
+            hdt = P::add(hdt, vk);
+            if constexpr (FIRST) vtop = P::add(vtop, vk);
+            if constexpr (CS) {
+                z = P::add(z, vk);
+                best = P::add(best, vk);
+            }
+            if constexpr (DS && LAST) bestL = P::add(bestL, vk);
+        }
+    };
     auto code = [&](int j) -> int {
         const uint32_t c2 = (w2[j >> 4] >> (2 * (j & 15))) & 3u;
         return ((wn[j >> 5] >> (j & 31)) & 1u) ? 4 : (int)c2;
@@ -704,6 +734,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
     // loaded on entry, and the pair loads those of columns jj + 2 and jj + 3
     // (at most the padding column m) for the next one as it goes.
     auto pair = [&](int jj, uint2 x0, uint2 x1) {
+        rebase(jj);
         if constexpr (CS) {
             v2s puA = P::from(x0.x), qA = P::from(x0.y), puB = P::from(x1.x), qB = P::from(x1.y);
             if constexpr (FIRST) {
@@ -825,6 +856,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
     }
     if (j < m) {
         if constexpr (LA > 0) plane(code(j), sa);  // the window held its first LA quads only
+        rebase(j);
         o0 = col(sa, a0);
         if constexpr (!LAST) s0[(size_t)j * 64] = o0;
     }
@@ -849,6 +881,7 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
     }
     if (j < m) {
         if constexpr (LA > 0) plane(code(j), sa);
+        rebase(j);
         o0 = col(sa, r0);
         if constexpr (!LAST) s0[(size_t)j * 64] = o0;
     }
@@ -895,17 +928,19 @@ __device__ __forceinline__ void stripe16d(const uint4 *__restrict__ prof, uint2 
     }
     if (j < m) {
         if constexpr (LA > 0) plane(code(j), sa);
+        rebase(j);
         o0 = col(sa, r0);
         if constexpr (!LAST) s0[(size_t)j * 64] = o0;
     }
 
     }
 
-    if constexpr (CS) best = P::add(best, us(P::splat(se * (m - 1))));  // column m - 1's space to H + o
+    // column m - 1's space (its place in its block) to H + o
+    if constexpr (CS) best = P::add(best, us(P::splat(se * ((m - 1) & cm))));
     if constexpr (!LOCAL) {
         // the last column's real rows, in Ho = H + o space (SH: P + e r,
         // the e r again a chain from an opaque zero; DS: from e (m - 1))
-        v2s k = P::splat(DS ? se * (m - 1) : 0);
+        v2s k = P::splat(DS ? se * ((m - 1) & cm) : 0);
         if constexpr (SH) asm volatile("" : "+v"(k));
 #pragma unroll
         for (int r = 0; r < RR; ++r) {
@@ -929,9 +964,9 @@ __device__ __forceinline__ void stripe16d_any(const uint4 *prof, uint2 *scr, con
                                               const bool last, const int row0, const int pA,
                                               const int pB, const typename Pk<F16>::T vo,
                                               const typename Pk<F16>::T ve, typename Pk<F16>::T &best,
-                                              const int c = 0) {
-    if (last) stripe16d<RR, LOCAL, FIRST, true, QUAD, F16, OCT, SH, CS, PQ, LA, DS>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best, c);
-    else stripe16d<RR, LOCAL, FIRST, false, QUAD, F16, OCT, SH, CS, PQ, LA, DS>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best, c);
+                                              const int c = 0, const int cm = -1) {
+    if (last) stripe16d<RR, LOCAL, FIRST, true, QUAD, F16, OCT, SH, CS, PQ, LA, DS>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best, c, cm);
+    else stripe16d<RR, LOCAL, FIRST, false, QUAD, F16, OCT, SH, CS, PQ, LA, DS>(prof, scr, lane, w2, wn, m, row0, pA, pB, vo, ve, best, c, cm);
 }
 
 template <bool LOCAL, int PF, int ST = kStripe>
@@ -974,6 +1009,8 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
     // mismatch, and -o for the top padding rows (the values prof_val gives);
     // DS: each less e.
     const int dse = DS ? a.sc.e : 0;
+    // the column-shift block mask (CS, DS): restart every 2^cshift columns, -1: never
+    const int cm = a.sc.cshift > 0 ? (1 << a.sc.cshift) - 1 : -1;
     const uint32_t eMa = PK::enc(a.sc.ma - a.sc.o - dse), eMi = PK::enc(a.sc.mi - a.sc.o - dse);
     const uint32_t ePre = PK::enc(-a.sc.o - dse);
 
@@ -1061,20 +1098,20 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
             if constexpr (PF >= 2) {
                 constexpr bool Q = PF == 3 || PF == 4 || PF == 6 || PF == 8 || PF == 9 || PF == 11 || PF == 12 || PF == 14;
                 if (s > 0) {
-                    stripe16d_any<ST, LOCAL, false, Q, F16, OCT, SH, CS, PQ, LA, DS>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best, cadj);
+                    stripe16d_any<ST, LOCAL, false, Q, F16, OCT, SH, CS, PQ, LA, DS>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best, cadj, cm);
                 } else {
                     switch (RR) {
 #define ESCH_FIRST(H) \
-    case H: stripe16d_any<(H <= ST ? H : 2), LOCAL, true, Q, F16, OCT, SH, CS, PQ, LA, DS>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best); break;
+    case H: stripe16d_any<(H <= ST ? H : 2), LOCAL, true, Q, F16, OCT, SH, CS, PQ, LA, DS>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best, 0, cm); break;
                         ESCH_FIRST(16) ESCH_FIRST(14) ESCH_FIRST(12) ESCH_FIRST(10)
                         ESCH_FIRST(8) ESCH_FIRST(6) ESCH_FIRST(4)
 #undef ESCH_FIRST
 #define ESCH_FIRST(H) \
-    case H: if constexpr (ST >= H) stripe16d_any<(H <= ST ? H : 2), LOCAL, true, Q, F16, OCT, SH, CS, PQ, LA, DS>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best); break;
+    case H: if constexpr (ST >= H) stripe16d_any<(H <= ST ? H : 2), LOCAL, true, Q, F16, OCT, SH, CS, PQ, LA, DS>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best, 0, cm); break;
                         ESCH_FIRST(32) ESCH_FIRST(30) ESCH_FIRST(28) ESCH_FIRST(26) ESCH_FIRST(24)
                         ESCH_FIRST(22) ESCH_FIRST(20) ESCH_FIRST(18)
 #undef ESCH_FIRST
-                        default: stripe16d_any<2, LOCAL, true, Q, F16, OCT, SH, CS, PQ, LA, DS>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best);
+                        default: stripe16d_any<2, LOCAL, true, Q, F16, OCT, SH, CS, PQ, LA, DS>(prof, scr, lane, w2, wn, m, last, row0, pA, pB, vo, ve, best, 0, cm);
                     }
                 }
             } else {

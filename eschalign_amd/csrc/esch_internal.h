@@ -47,12 +47,22 @@ __host__ __device__ inline int64_t shifted_bound(int64_t max_ref_len, int32_t ma
 // In local mode those variants hold reference column j's values shifted by
 // |e| j (j = -1 .. L - 1): every value lies within
 // [-(|o| + 3|e| + M), kM + |e| (L + 2) + M], the floor and the carried best
-// included, so the bound grows with the longest reference L.
-__host__ __device__ inline int64_t colshift_bound(int64_t max_ref_len, int32_t ma, int32_t mi, int32_t o, int32_t e) {
+// included, so the bound grows with the longest reference L. Restarted
+// every `block` columns (a power of two, 0: never), the shift spans at most
+// a block, which takes L's place in the |e| term.
+// The two lengths apart: k bounds the read-reference overlap (min(read,
+// reference) over a batch), span the columns the shift spans.
+__host__ __device__ inline int64_t colshift_bound_ks(int64_t k, int64_t span, int32_t ma, int32_t mi, int32_t o,
+                                                     int32_t e) {
     const int64_t M = ma < 0 ? -(int64_t)ma : ma, Mi = mi < 0 ? -(int64_t)mi : mi;
     const int64_t O = o < 0 ? -(int64_t)o : o, Ee = e < 0 ? -(int64_t)e : e;
     const int64_t MM = M > Mi ? M : Mi;
-    return max_ref_len * (MM + Ee) + O + 3 * Ee + MM;
+    return k * MM + span * Ee + O + 3 * Ee + MM;
+}
+__host__ __device__ inline int64_t colshift_bound(int64_t max_ref_len, int32_t ma, int32_t mi, int32_t o, int32_t e,
+                                                  int64_t block = 0) {
+    const int64_t span = block > 0 && block < max_ref_len ? block : max_ref_len;
+    return colshift_bound_ks(max_ref_len, span, ma, mi, o, e);
 }
 // The diagonal-shift semiglobal schedules (12-14) hold a value of stripe
 // row r in reference column j shifted by |e| (r + j) (r = -1 .. ST - 1,
@@ -60,13 +70,36 @@ __host__ __device__ inline int64_t colshift_bound(int64_t max_ref_len, int32_t m
 // [-(kM + |o| + M + (ST + 1)|e|), kM + |e| (ST + L - 2)], the profile's
 // stripe-top entries s - o - e + e h included, so the bound grows with the
 // longest reference L as the column shift's does, and with the stripe as
-// the row shift's does.
-__host__ __device__ inline int64_t diagshift_bound(int64_t max_ref_len, int32_t ma, int32_t mi, int32_t o, int32_t e,
-                                                   int stripe) {
+// the row shift's does; a column part restarted every `block` columns (0:
+// never) spans at most a block, which takes L's place in the |e| term.
+__host__ __device__ inline int64_t diagshift_bound_ks(int64_t k, int64_t span, int32_t ma, int32_t mi, int32_t o,
+                                                      int32_t e, int stripe) {
     const int64_t M = ma < 0 ? -(int64_t)ma : ma, Mi = mi < 0 ? -(int64_t)mi : mi;
     const int64_t O = o < 0 ? -(int64_t)o : o, Ee = e < 0 ? -(int64_t)e : e;
     const int64_t MM = M > Mi ? M : Mi;
-    return max_ref_len * (MM + Ee) + O + (stripe + 2) * Ee + MM;
+    return k * MM + span * Ee + O + (stripe + 2) * Ee + MM;
+}
+__host__ __device__ inline int64_t diagshift_bound(int64_t max_ref_len, int32_t ma, int32_t mi, int32_t o, int32_t e,
+                                                   int stripe, int64_t block = 0) {
+    const int64_t span = block > 0 && block < max_ref_len ? block : max_ref_len;
+    return diagshift_bound_ks(max_ref_len, span, ma, mi, o, e, stripe);
+}
+// The column-shift block (log2) of the column-shifted (local) or
+// diagonal-shift (semiglobal) variants under `lim`, for overlaps of at
+// most k on references of at most L: 0 (never restarted) when the whole
+// reference fits, else the widest power of two from 4096 down to 8 that
+// does; -1 when none does (the variants are refused). Restarting costs two
+// packed adds per stripe row each block (1/4 op per cell at 8 columns,
+// 1/16 at 32).
+__host__ __device__ inline int column_block_log2(bool local, int64_t k, int64_t L, int32_t ma, int32_t mi, int32_t o,
+                                                 int32_t e, int stripe, int64_t lim) {
+    auto bound = [&](int64_t span) {
+        return local ? colshift_bound_ks(k, span, ma, mi, o, e) : diagshift_bound_ks(k, span, ma, mi, o, e, stripe);
+    };
+    if (bound(L) <= lim) return 0;
+    for (int b = 12; b >= 3; --b)
+        if (((int64_t)1 << b) < L && bound((int64_t)1 << b) <= lim) return b;
+    return -1;
 }
 // -inf of the int32 kernel, the oracle's i32::MIN / 4 (lib/scalar.rs:42).
 constexpr int32_t kNeg32 = INT32_MIN / 4;
@@ -74,6 +107,10 @@ constexpr int32_t kNeg32 = INT32_MIN / 4;
 struct Scoring {
     int32_t ma, mi, o, e;
     int32_t local;
+    // k_dp16's column-shifted (local) and diagonal-shift (semiglobal)
+    // variants restart the column part of their shift every 2^cshift
+    // columns (0: never); set per launch from the variant's arithmetic
+    int32_t cshift;
 };
 
 // A panel resident on the device: 2-bit codes (16 per word, A0 C1 G2 T3,
@@ -184,6 +221,10 @@ struct Launch {
                     // in local mode the column-shifted one (colshift_bound)
     int grid_diag;  // persistent waves of the anti-diagonal kernel
     size_t diag_lds;  // k_diag64's LDS boundary bytes; 0: k_diag16 (global scratch)
+    // the column-shift block (column_block_log2) of the fp16 and the int16
+    // shifted-column variants on this panel; launch_dp16 passes the one of
+    // the variant it launches in Scoring::cshift
+    int cshift_f16, cshift_i16;
 };
 // Variants 0-8 are k_dp16's schedules (above) on 16-row stripes; 9-16 run
 // some of them on taller stripes: 9, 10, 11 = schedules 7, 8, 6 on 24 rows;

@@ -1029,7 +1029,11 @@ hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_dp16(const KArgs &a, const Launch &g, bool local, hipStream_t st) {
+hipError_t launch_dp16(const KArgs &a0, const Launch &g, bool local, hipStream_t st) {
+    // the column-shift block of the launched variant's arithmetic
+    KArgs a = a0;
+    a.sc.cshift = dp16_is_f16(g.variant16) ? g.cshift_f16 : g.cshift_i16;
+    if (a.sc.cshift < 0) a.sc.cshift = 0;  // a variant the scorer refuses never runs shifted columns
     switch (dp16_stripe(g.variant16)) {
         case 24: return local ? launch_dp16_lo24(a, g, st) : launch_dp16_sg24(a, g, st);
         case 32: return local ? launch_dp16_lo32(a, g, st) : launch_dp16_sg32(a, g, st);

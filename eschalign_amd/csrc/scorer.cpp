@@ -228,20 +228,30 @@ esch_status esch_scorer_new(int ordinal, const uint8_t *const *refs, const size_
     // workloads); esch_autotune re-decides on the caller's reads
     s->f16_ok = fits_f16((int64_t)max_len, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e);
     {
-        // the shifted variants: rows shifted in semiglobal mode, columns in local
+        // The shifted variants: rows shifted in semiglobal mode, columns in
+        // local mode, both in the diagonal shift (semiglobal mode only). The
+        // column part restarts every 2^cshift columns where the whole
+        // reference does not fit the arithmetic's bound (column_block_log2;
+        // the diagonal shift's taken at 32 rows, which covers 24); -1: no
+        // block fits and the variants are refused.
+        const auto block = [&](int64_t lim) {
+            return esch::column_block_log2(local, (int64_t)max_len, (int64_t)max_len, s->sc.ma, s->sc.mi, s->sc.o,
+                                           s->sc.e, 32, lim);
+        };
+        s->geo.cshift_f16 = block(esch::kBoundF16);
+        s->geo.cshift_i16 = block(esch::kBound16);
         for (int v = 0; v < kDp16Variants; ++v) {
             const bool f16 = esch::dp16_is_f16(v);
-            const int64_t sb = local ? esch::colshift_bound((int64_t)max_len, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e)
-                                     : esch::shifted_bound((int64_t)max_len, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e,
-                                                           esch::dp16_stripe(v));
             const int64_t lim = f16 ? esch::kBoundF16 : esch::kBound16;
-            const bool shift_ok = !esch::dp16_is_shifted(v) || sb <= lim;
-            // the diagonal shift: semiglobal mode, within its own bound
-            const bool diag_ok =
-                !esch::dp16_is_diag(v) ||
-                (!local && esch::diagshift_bound((int64_t)max_len, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e,
-                                                 esch::dp16_stripe(v)) <= lim);
-            s->usable16[v] = (!f16 || s->f16_ok) && shift_ok && diag_ok;
+            const int cs = f16 ? s->geo.cshift_f16 : s->geo.cshift_i16;
+            bool ok = !f16 || s->f16_ok;
+            if (esch::dp16_is_diag(v))
+                ok = ok && !local && cs >= 0;
+            else if (esch::dp16_is_shifted(v))
+                ok = ok && (local ? cs >= 0
+                                  : esch::shifted_bound((int64_t)max_len, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e,
+                                                        esch::dp16_stripe(v)) <= lim);
+            s->usable16[v] = ok;
         }
     }
     // semiglobal mode: the diagonal shift on 32-row stripes (fp16, else
@@ -652,28 +662,42 @@ esch_status enqueue(esch_batch *b, int what, bool timed, const Launch &g) {
     return ESCH_OK;
 }
 
-// The k_dp16 variant a batch runs: the scorer's, unless the caller fixed
-// none (the default or the autotuner's pick) and the panel admits no fp16
-// variant while this batch's reads do. Every value of the fp16 and row-shifted recurrences is
-// bounded through k = min(read, reference), so the batch's longest read
-// caps k as well as the panel's longest reference does: short reads on a
-// panel of long references still get fp16 (semiglobal: the row-shifted 7,
-// or 10 for the windowed int16 schedule 11, 6 ops per cell; local: 4). The
-// column shift's range grows with the reference itself, so local mode does
-// not take it from the reads, and neither does the diagonal shift: its
-// int16 schedule (6 ops per cell) stays, the row-shifted fp16 one being no
-// fewer ops.
-int batch_variant(const esch_batch *b, int v, bool fixed) {
+// The k_dp16 variant a batch runs, and the column block of its fp16
+// shifted-column form (*cs_f16: the panel's unless the batch upgrades):
+// the scorer's variant, unless the caller fixed none (the default or the
+// autotuner's pick) and the panel admits no fp16 variant while this batch's
+// reads do. Every value of the fp16 recurrences is bounded through
+// k = min(read, reference), the shifted columns' span aside, and a column
+// shift restarted often enough spans a block whatever the references. So
+// the batch's longest read caps k as the panel's longest reference does,
+// and short reads on a panel of long references still get fp16:
+//  - local mode: the column-shifted fp16 schedule (7 on 16-row stripes,
+//    the windowed 10 on taller ones; 6.5 ops per cell) with the widest
+//    block the batch admits, else the unshifted 4 (8.5 ops);
+//  - semiglobal mode, an int16 diagonal-shift pick: the diagonal shift in
+//    fp16 (schedule 12, 5 ops) with its block; where no block fits, the
+//    pick stays (the row-shifted fp16 form is no fewer ops);
+//  - semiglobal mode, another int16 pick: the row-shifted fp16 schedule
+//    (7, or 10 for a windowed pick; 6 ops), else 4 (7 ops).
+int batch_variant(const esch_batch *b, int v, bool fixed, int *cs_f16) {
     const esch_scorer *s = b->s;
-    if (fixed || esch::dp16_is_f16(v) || esch::dp16_is_diag(v)) return v;
+    if (fixed || esch::dp16_is_f16(v)) return v;
+    const Scoring &sc = s->sc;
+    const int64_t L = (int64_t)s->max_len;
     const int64_t k = (int64_t)std::min<size_t>(s->max_len, b->longest);
-    if (!fits_f16(k, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e)) return v;
-    if (s->sc.local) return kDp16F16;
-    // the row-shifted fp16 schedule on the same stripe height (the windowed
-    // one for a windowed pick)
+    if (!fits_f16(k, sc.ma, sc.mi, sc.o, sc.e)) return v;
     const int st = esch::dp16_stripe(v);
+    const bool local = sc.local != 0;
+    if (local || esch::dp16_is_diag(v)) {
+        const int kb = esch::column_block_log2(local, k, L, sc.ma, sc.mi, sc.o, sc.e, 32, esch::kBoundF16);
+        if (kb >= 0) {
+            *cs_f16 = kb;
+            return local ? esch::dp16_variant(st == 16 ? 7 : 10, st) : esch::dp16_variant(12, st);
+        }
+        return local ? kDp16F16 : v;
+    }
     const int up = esch::dp16_variant(esch::dp16_schedule(v) == 11 ? 10 : 7, st);
-    return esch::shifted_bound(k, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e, st) <= esch::kBoundF16 ? up : kDp16F16;
+    return esch::shifted_bound(k, sc.ma, sc.mi, sc.o, sc.e, st) <= esch::kBoundF16 ? up : kDp16F16;
 }
 
 // Rows up to which a batch is scored with the roles swapped. With the
@@ -889,16 +913,19 @@ esch_status esch_batch_run(esch_batch *b, int what, int32_t min_score, int64_t m
     }
     d.diag_ratio = (what & ESCH_RUN_PLAIN) ? 0 : ratio;
     const Launch geo_scorer = geo;  // the swapped path runs the scorer's own variant
-    const int v16 = batch_variant(b, geo.variant16, fixed);
+    int cs_f16 = geo.cshift_f16;
+    const int v16 = batch_variant(b, geo.variant16, fixed, &cs_f16);
     geo.variant16 = v16;
+    geo.cshift_f16 = cs_f16;
     geo.grid16 = s->grid16_v[v16];
     d.blocks16 = (int32_t)geo.grid16;
     HIPC(hipMemcpyAsync(b->d_desc, b->h_desc, sizeof(DevDesc), hipMemcpyHostToDevice, b->stream));
     b->last_what = what;
     const bool graph = !(what & ESCH_RUN_NO_GRAPH);
     // the template-strand count is read from the descriptor, not baked in;
-    // the kernel variant is baked into the captured launches
-    const int key = (what & ~(ESCH_RUN_NO_GRAPH | ESCH_RUN_PAIR_BOTH)) | (v16 << 16);
+    // the kernel variant and its fp16 column block are baked into the
+    // captured launches (the run flags take bits 0-6, the variant 16-20)
+    const int key = (what & ~(ESCH_RUN_NO_GRAPH | ESCH_RUN_PAIR_BOTH)) | (v16 << 16) | (((cs_f16 + 1) & 31) << 24);
     HIPC(hipEventRecord(b->ev0, b->stream));
     b->last_swapped = swap_eligible(b, what);
     b->last_variant = b->last_swapped ? -1 : v16;
@@ -1180,14 +1207,16 @@ esch_status set_variant(esch_scorer *s, int variant, bool fixed) {
     if (!s->usable16[variant] && esch::dp16_is_diag(variant))
         return fail(ESCH_INVALID_ARG,
                     s->sc.local ? "variant %d (diagonal shift) is a semiglobal form; local mode refuses it"
-                                : "variant %d (diagonal shift) needs longest reference x (max(|match|, |mismatch|) + "
-                                  "|extend|) + |open| + (stripe rows + 2) |extend| + max(|match|, |mismatch|) <= %lld",
+                                : "variant %d (diagonal shift) needs longest reference x max(|match|, |mismatch|) + "
+                                  "8 |extend| + |open| + 34 |extend| + max(|match|, |mismatch|) <= %lld (its "
+                                  "column shift restarted every 8 columns at the least)",
                     variant, (long long)(esch::dp16_is_f16(variant) ? esch::kBoundF16 : esch::kBound16));
     if (!s->usable16[variant])
         return fail(ESCH_INVALID_ARG,
                     s->sc.local ? "variant %d (column-shifted in local mode) needs longest reference x "
-                                  "(max(|match|, |mismatch|) + |extend|) + |open| + 3 |extend| + "
-                                  "max(|match|, |mismatch|) <= %lld"
+                                  "max(|match|, |mismatch|) + 8 |extend| + |open| + 3 |extend| + "
+                                  "max(|match|, |mismatch|) <= %lld (its shift restarted every 8 columns at the "
+                                  "least)"
                                 : "variant %d (row-shifted) needs longest reference x max(|match|, |mismatch|) + "
                                   "|open| + (stripe rows + 1) |extend| <= %lld",
                     variant, (long long)(esch::dp16_is_f16(variant) ? esch::kBoundF16 : esch::kBound16));
@@ -1298,6 +1327,14 @@ esch_status esch_variant_info(int variant, int *schedule, int *stripe_rows, int 
 int esch_variant_usable(const esch_scorer *s, int variant) {
     if (!s || variant < 0 || variant >= kDp16Variants) return 0;
     return s->usable16[variant] ? 1 : 0;
+}
+
+int esch_column_block(const esch_scorer *s, int variant) {
+    if (!s || variant < 0 || variant >= kDp16Variants || !s->usable16[variant]) return -1;
+    const bool cols = esch::dp16_is_diag(variant) || (s->sc.local && esch::dp16_is_shifted(variant));
+    if (!cols) return 0;
+    const int k = esch::dp16_is_f16(variant) ? s->geo.cshift_f16 : s->geo.cshift_i16;
+    return k > 0 ? 1 << k : 0;
 }
 
 esch_status esch_autotune(esch_scorer *s, const uint8_t *const *reads, const size_t *lens, size_t n,

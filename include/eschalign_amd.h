@@ -489,7 +489,12 @@ esch_status esch_bam_build_records(const uint8_t *in, size_t in_bytes, size_t n,
  *      fp16 (12, 13), 6 in int16 (14). Semiglobal mode only (local mode
  *      refuses them); they need longest reference x (M + |extend|) +
  *      |open| + (stripe rows + 2) |extend| + M within 2048 (fp16) or 32000
- *      (int16). 23, 24 = 12 on 24 / 32 rows; 25, 26 = 13; 27, 28 = 14. */
+ *      (int16). 23, 24 = 12 on 24 / 32 rows; 25, 26 = 13; 27, 28 = 14.
+ *   The column-shifted (local) and diagonal-shift variants also run where
+ *   the longest reference is too long for those sums: their column shift
+ *   then restarts every K columns (esch_column_block), and K |extend|
+ *   takes the place of longest reference x |extend| in the condition, K at
+ *   least 8. */
 int esch_n_variants(void);
 /* A variant's shape: its schedule (variants 0-8 are schedules 0-8 on 16
  * rows; 9, 10 and 11 are the windowed 8, 7 and 6; 12, 13 and 14 those with
@@ -503,6 +508,14 @@ esch_status esch_variant_info(int variant, int *schedule, int *stripe_rows, int 
  * variant's exactness condition: esch_set_variant accepts it and the
  * autotuner times it), else 0. */
 int esch_variant_usable(const esch_scorer *s, int variant);
+/* The column block of `variant` on this scorer: the column-shifted (local)
+ * and diagonal-shift (semiglobal) variants restart the column part of
+ * their shift every that many columns where the panel's longest reference
+ * does not fit the arithmetic's bound whole (a power of two from 8 to
+ * 4096; each restart costs two packed adds per stripe row). 0: never
+ * restarted, or a variant without a column shift; -1: a variant the
+ * scorer refuses. */
+int esch_column_block(const esch_scorer *s, int variant);
 int esch_get_variant(const esch_scorer *s);
 esch_status esch_set_variant(esch_scorer *s, int variant);
 /* ms[v] (may be NULL, esch_n_variants() entries) receives each variant's DP

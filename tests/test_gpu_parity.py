@@ -462,98 +462,121 @@ def test_row_shifted_variants_at_their_bound():
                 gpu.variant = w
 
 
+def _bound_case(rng, L, extra=()):
+    """Panel and reads that reach both ends of a shifted variant's value
+    range: a reference read back whole (the top score), runs of one base
+    against a reference of another (the low end), reads several stripes
+    long, and reads longer than the references."""
+    refs = [G.random_seq(rng, L, 0.0), b"C" * L, G.random_seq(rng, L // 2, 0.02),
+            G.random_seq(rng, 40, 0.0)]
+    reads = [refs[0], b"A" * L, b"A" * (L // 3), b"G" * 5, refs[0][L // 9:L - L // 9],
+             G.random_seq(rng, L + 40, 0.03), refs[0] + refs[0][:300], b"C" * 3 + refs[0]]
+    reads += [G.random_seq(rng, 1 + rng.below(L), 0.02) for _ in range(40)] + list(extra)
+    queries = codes_of(reads)
+    queries += [ea.reverse_complement_codes(q) for q in queries]
+    return refs, queries
+
+
+def _check_variants(refs, queries, scoring, mode, vs, block):
+    """The variants `vs` on a panel of `refs` run with the column block
+    `block` (0: the shift never restarts) and equal the oracle."""
+    panel = ea.Panel(refs)
+    gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+    gpu.diag_ratio = 0
+    want = oracle(queries, panel.codes, scoring, mode)
+    for w in vs:
+        assert gpu.column_block(w) == block, (scoring, mode, w, gpu.column_block(w), block)
+        gpu.variant = w
+        s32, ok = gpu.score_batch_i32(queries)
+        assert ok.all()
+        bad = np.argwhere(s32 != want)
+        assert bad.size == 0, (scoring, mode, w, block, bad[:4])
+    return want
+
+
 def test_diagonal_shifted_variants_at_their_bound():
     """Variants 23-28 hold every value of stripe row r in reference column j
     shifted by |e| (r + j) (semiglobal mode): exact while longest reference
-    x (M + |e|) + |o| + (rows + 2) |e| + M stays within 2048 (fp16: 23-26)
-    or 32000 (int16: 27, 28). Panels exactly at it, with reads that reach
-    both ends of the value range (a reference read back whole, runs of one
-    base against a reference of another, reads several stripes long), equal
-    the oracle; one base longer, the panel refuses the variant; local mode
-    refuses them on any panel."""
+    L x (M + |e|) + |o| + (rows + 2) |e| + M stays within 2048 (fp16: 23-26)
+    or 32000 (int16: 27, 28), the block the scorer takes at 32 rows. Past
+    it the column part of the shift restarts every K columns, K |e| in
+    place of L |e|, down to K = 8. Panels exactly at the whole-reference
+    bound (no restart), one base over (a restart), exactly at the K = 8
+    bound (a restart every 8 columns), with reads that reach both ends of
+    the value range, equal the oracle; one base past that the panel refuses
+    the variants; local mode refuses them on any panel."""
     rng = G.Rng(0x5EED_0A07)
-    for scoring, L, vs, st in (((1, -1, -3, -1), 1009, (23, 25), 24), ((1, -1, -3, -1), 1005, (24, 26), 32),
-                               ((2, -1, -9, -3), 387, (24, 26), 32),
-                               ((100, -100, -300, -100), 145, (27,), 24),
-                               ((100, -100, -500, -100), 140, (28,), 32)):
-        M = max(abs(scoring[0]), abs(scoring[1]))
-        e = abs(scoring[3])
-        f16 = ea.variant_info(vs[0]).f16
-        assert all(ea.variant_info(w).stripe_rows == st and ea.variant_info(w).diag for w in vs)
-        assert L * (M + e) + abs(scoring[2]) + (st + 2) * e + M == (2048 if f16 else 32000)
-        refs = [G.random_seq(rng, L, 0.0), b"C" * L, G.random_seq(rng, L // 2, 0.02),
-                G.random_seq(rng, 40, 0.0)]
-        reads = [refs[0], b"A" * L, b"A" * (L // 3), b"G" * 5, refs[0][L // 9:L - L // 9],
-                 G.random_seq(rng, L + 40, 0.03), refs[0] + refs[0][:300], b"C" * 3 + refs[0]]
-        reads += [G.random_seq(rng, 1 + rng.below(L), 0.02) for _ in range(40)]
-        queries = codes_of(reads)
-        queries += [ea.reverse_complement_codes(q) for q in queries]
-        panel = ea.Panel(refs)
-        gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), ea.Mode.SEMIGLOBAL)
-        gpu.diag_ratio = 0
-        want = oracle(queries, panel.codes, scoring, ea.Mode.SEMIGLOBAL)
-        for w in vs:
-            gpu.variant = w
-            s32, ok = gpu.score_batch_i32(queries)
-            assert ok.all()
-            bad = np.argwhere(s32 != want)
-            assert bad.size == 0, (scoring, w, bad[:4])
-        assert want.max() >= L * scoring[0] - 20 * abs(scoring[0]), want.max()
-        longer = ea.Panel(refs + [G.random_seq(rng, L + 1, 0.0)])
+    # (scoring, L at the whole-reference bound (32 rows), L at the K = 8 bound, variants)
+    for scoring, L0, L8, vs in (((1, -1, -3, -1), 1005, 2002, (23, 24, 25, 26)),
+                                ((2, -1, -9, -3), 387, 955, (24, 26)),
+                                ((100, -100, -500, -100), 140, 272, (27, 28))):
+        M, O_, E = max(abs(scoring[0]), abs(scoring[1])), abs(scoring[2]), abs(scoring[3])
+        lim = 2048 if ea.variant_info(vs[0]).f16 else 32000
+        assert L0 * (M + E) + O_ + 34 * E + M == lim
+        assert L8 * M + 8 * E + O_ + 34 * E + M in (lim, lim - 1) and (L8 + 1) * M + 42 * E + O_ + M > lim
+        refs, queries = _bound_case(rng, L0)
+        want = _check_variants(refs, queries, scoring, ea.Mode.SEMIGLOBAL, vs, 0)
+        assert want.max() >= L0 * scoring[0] - 20 * abs(scoring[0]), want.max()
+        # one base over: the widest power of two that fits
+        K1 = next(K for K in (4096, 2048, 1024, 512, 256, 128, 64, 32, 16, 8)
+                  if (L0 + 1) * M + K * E + O_ + 34 * E + M <= lim)
+        refs1 = refs + [G.random_seq(rng, L0 + 1, 0.0)]
+        _check_variants(refs1, queries, scoring, ea.Mode.SEMIGLOBAL, vs, K1)
+        refs8, q8 = _bound_case(rng, L8)
+        want8 = _check_variants(refs8, q8[:20] + q8[-20:], scoring, ea.Mode.SEMIGLOBAL, vs, 8)
+        assert want8.max() >= L8 * scoring[0] - 20 * abs(scoring[0]), want8.max()
+        longer = ea.Panel(refs8 + [G.random_seq(rng, L8 + 1, 0.0)])
         gpu = ea.GpuScorer(longer, ea.Scoring(*scoring), ea.Mode.SEMIGLOBAL)
         for w in vs:
+            assert gpu.column_block(w) == -1
             with pytest.raises(ea.GpuError):
                 gpu.variant = w
-        gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), ea.Mode.LOCAL)
+        gpu = ea.GpuScorer(ea.Panel(refs), ea.Scoring(*scoring), ea.Mode.LOCAL)
         for w in vs:
             with pytest.raises(ea.GpuError):
                 gpu.variant = w
 
 
 def test_column_shifted_local_variants_at_their_bound():
-    """In local mode variants 6, 7 and 8 (and 9-14) hold every value of reference
-    column j shifted by |e| j: exact while longest reference x (M + |e|) +
-    |o| + 3 |e| + M stays within 2048 (fp16) or 32000 (int16). Panels
-    exactly at it, with reads reaching the top score (a reference read back
-    whole) and the floor (runs of one base against a reference of another),
-    several stripes long, equal the oracle; one base longer, local mode
-    refuses the variant while semiglobal mode (its row-shift bound) keeps
-    it."""
+    """In local mode variants 6, 7 and 8 (and their tall and windowed forms)
+    hold every value of reference column j shifted by |e| j: exact while
+    longest reference L x (M + |e|) + |o| + 3 |e| + M stays within 2048
+    (fp16) or 32000 (int16). Past it the shift restarts every K columns,
+    K |e| in place of L |e|, down to K = 8. Panels exactly at the
+    whole-reference bound (no restart), one base over (a restart) and
+    exactly at the K = 8 bound (a restart every 8 columns), with reads
+    reaching the top score (a reference read back whole) and the floor
+    (runs of one base against a reference of another), several stripes
+    long, equal the oracle; one base past that local mode refuses the
+    variants while semiglobal mode (its row-shift bound) keeps them."""
     rng = G.Rng(0x5EED_0A05)
-    # the schedules on 24- and 32-row stripes (9-14, windowed 17-22) shift
-    # columns the same way
     f16, i16 = (7, 8, 9, 10, 12, 13, 17, 18, 19, 20), (6, 11, 14, 21, 22)
-    for scoring, L, vs in (((1, -1, -2, -1), 1021, f16), ((2, -1, -12, -3), 405, f16),
-                           ((100, -100, -200, -100), 157, i16)):
-        M = max(abs(scoring[0]), abs(scoring[1]))
-        e = abs(scoring[3])
-        assert L * (M + e) + abs(scoring[2]) + 3 * e + M == (32000 if vs == i16 else 2048)
-        refs = [G.random_seq(rng, L, 0.0), b"C" * L, G.random_seq(rng, L // 2, 0.02),
-                G.random_seq(rng, 40, 0.0)]
-        reads = [refs[0], b"A" * L, b"A" * (L // 3), b"G" * 5, refs[0][L // 9:L - L // 9],
-                 G.random_seq(rng, L + 40, 0.03), refs[0] + refs[0][:100], b"C" * 3 + refs[0]]
-        reads += [G.random_seq(rng, 1 + rng.below(L), 0.02) for _ in range(40)]
-        queries = codes_of(reads)
-        queries += [ea.reverse_complement_codes(q) for q in queries]
-        panel = ea.Panel(refs)
-        gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), ea.Mode.LOCAL)
-        gpu.diag_ratio = 0
-        want = oracle(queries, panel.codes, scoring, ea.Mode.LOCAL)
-        for w in vs:
-            gpu.variant = w
-            s32, ok = gpu.score_batch_i32(queries)
-            assert ok.all()
-            bad = np.argwhere(s32 != want)
-            assert bad.size == 0, (scoring, w, bad[:4])
-        assert want.max() == L * scoring[0], want.max()
-        longer = ea.Panel(refs + [G.random_seq(rng, L + 1, 0.0)])
+    # (scoring, L at the whole-reference bound, L at the K = 8 bound, variants)
+    for scoring, L0, L8, vs in (((1, -1, -2, -1), 1021, 2034, f16), ((2, -1, -12, -3), 405, 1000, f16),
+                                ((100, -100, -200, -100), 157, 306, i16)):
+        M, O_, E = max(abs(scoring[0]), abs(scoring[1])), abs(scoring[2]), abs(scoring[3])
+        lim = 32000 if vs == i16 else 2048
+        assert L0 * (M + E) + O_ + 3 * E + M == lim
+        assert L8 * M + 8 * E + O_ + 3 * E + M in (lim, lim - 1) and (L8 + 1) * M + 11 * E + O_ + M > lim
+        refs, queries = _bound_case(rng, L0, [b"C" * 3 + G.random_seq(rng, L0, 0.0)])
+        want = _check_variants(refs, queries, scoring, ea.Mode.LOCAL, vs, 0)
+        assert want.max() == L0 * scoring[0], want.max()
+        K1 = next(K for K in (4096, 2048, 1024, 512, 256, 128, 64, 32, 16, 8)
+                  if (L0 + 1) * M + K * E + O_ + 3 * E + M <= lim)
+        refs1 = refs + [G.random_seq(rng, L0 + 1, 0.0)]
+        _check_variants(refs1, queries, scoring, ea.Mode.LOCAL, vs, K1)
+        refs8, q8 = _bound_case(rng, L8)
+        want8 = _check_variants(refs8, q8[:20] + q8[-20:], scoring, ea.Mode.LOCAL, vs[:4], 8)
+        assert want8.max() == L8 * scoring[0], want8.max()
+        longer = ea.Panel(refs8 + [G.random_seq(rng, L8 + 1, 0.0)])
         gpu = ea.GpuScorer(longer, ea.Scoring(*scoring), ea.Mode.LOCAL)
         for w in vs:
             with pytest.raises(ea.GpuError):
                 gpu.variant = w
         gpu = ea.GpuScorer(longer, ea.Scoring(*scoring), ea.Mode.SEMIGLOBAL)
         for w in vs:
-            gpu.variant = w
+            if gpu.column_block(w) >= 0:  # the row shift's bound is its own
+                gpu.variant = w
 
 
 def test_every_kernel_variant_on_short_references():
@@ -929,21 +952,21 @@ def test_autotune_per_length_bucket_plan():
 
 @pytest.mark.gpu
 def test_short_reads_take_fp16_on_a_panel_of_long_references():
-    """Every fp16 and row-shifted value is bounded through min(read,
-    reference), so a batch's longest read caps it as the panel's longest
-    reference does. On a panel of 2,100-nt references (no fp16 variant
-    panel-wide: 2,100 + |o| + |e| > 2,048), a batch of reads up to 1,500 nt
-    runs the row-shifted fp16 variant 7 in semiglobal mode (scoring
-    1,-1,-20,-15, whose diagonal-shift bound, 34,131, is over 32,000, so the
-    scorer's default is the int16 variant 6) and 4 in local mode
-    (1,-1,-2,-1); a batch holding a 2,100-nt read (the top score, 2,100, is
-    beyond fp16's exact integers) stays on the scorer's int16 variant 6, and
-    so does every batch once the caller fixes a variant (an autotuned scorer
-    still upgrades). The diagonal shift's range grows with the reference
-    whatever the reads, so a scorer on the diagonal-shift int16 variant 28
-    (semiglobal 1,-1,-2,-1) keeps it for every batch: the row-shifted fp16
-    variant takes no fewer ops. Scores equal the oracle's in every case (the
-    captured graphs are keyed by the variant they launch)."""
+    """Every value of the fp16 recurrences is bounded through min(read,
+    reference), the shifted columns' span aside, and a column shift
+    restarted every K columns spans K whatever the references. So on a
+    panel of 2,100-nt references (no fp16 variant panel-wide: 2,100 +
+    |o| + |e| > 2,048), a batch of reads up to 1,500 nt takes the fp16 form
+    of the scorer's int16 variant: the diagonal shift in semiglobal mode
+    (default 28 -> 24, with a 256-column block) and the column shift in
+    local mode (default 6 -> 7, a 512-column block). With scoring
+    1,-1,-20,-15 no block fits fp16 for those reads and the semiglobal
+    scorer keeps the diagonal-shift int16 variant 28 (the row-shifted fp16
+    form takes as many ops). A batch holding a 2,100-nt read (the top score,
+    2,100, is beyond fp16's exact integers) stays on the scorer's int16
+    variant, and so does every batch once the caller fixes a variant (an
+    autotuned scorer still upgrades). Scores equal the oracle's in every
+    case (the captured graphs are keyed by the variant and its block)."""
     rng = G.Rng(0x5EED_0A06)
     L = 2100
     refs = [G.random_seq(rng, L, 0.0), G.random_seq(rng, L - 300, 0.01), b"C" * 900]
@@ -951,9 +974,9 @@ def test_short_reads_take_fp16_on_a_panel_of_long_references():
     short += [G.random_seq(rng, 1 + rng.below(1500), 0.02) for _ in range(30)]
     longer = short + [refs[0]]
     panel = ea.Panel(refs)
-    for mode, scoring, default, fast in ((ea.Mode.SEMIGLOBAL, (1, -1, -20, -15), 6, 7),
-                                         (ea.Mode.SEMIGLOBAL, (1, -1, -2, -1), 28, 28),
-                                         (ea.Mode.LOCAL, (1, -1, -2, -1), 6, 4)):
+    for mode, scoring, default, fast in ((ea.Mode.SEMIGLOBAL, (1, -1, -2, -1), 28, 24),
+                                         (ea.Mode.SEMIGLOBAL, (1, -1, -20, -15), 28, 28),
+                                         (ea.Mode.LOCAL, (1, -1, -2, -1), 6, 7)):
         gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
         assert gpu.variant == default, (mode, scoring)
         for reads, want_v in ((short, fast), (longer, default), (short, fast)):
@@ -971,16 +994,17 @@ def test_short_reads_take_fp16_on_a_panel_of_long_references():
         chosen, _ = gpu.autotune(codes_of(short), both_strands=False)
         info = ea.variant_info(chosen)
         assert not info.f16  # an int16 variant
-        # semiglobal: the row-shifted fp16 schedule on the pick's stripes
-        # (the windowed one for a windowed pick); a diagonal-shift pick stays
-        rows = info.stripe_rows
+        rows = ea.variant_info(chosen).stripe_rows
+
+        def on_rows(sched):
+            return next(v for v, x in ea.variants().items() if x.schedule == sched and x.stripe_rows == rows)
+
         if mode == ea.Mode.LOCAL:
-            up = 4
+            up = on_rows(7 if rows == 16 else 10)
         elif info.diag:
-            up = chosen
+            up = on_rows(12) if scoring == (1, -1, -2, -1) else chosen
         else:
-            sched = 10 if info.schedule == 11 else 7
-            up = next(v for v, x in ea.variants().items() if x.schedule == sched and x.stripe_rows == rows)
+            up = on_rows(10 if info.schedule == 11 else 7)
         q = codes_of(short)
         b = gpu.batch(len(q), sum(len(x) for x in q))
         b.load(q)

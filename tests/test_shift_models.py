@@ -120,7 +120,7 @@ def walk_semiglobal_rows(q, r, scoring, nmax, rng_vals, st=16):
     return best - o
 
 
-def walk_semiglobal_diag(q, r, scoring, nmax, rng_vals, st=16, advance_top=True):
+def walk_semiglobal_diag(q, r, scoring, nmax, rng_vals, st=16, advance_top=True, rebase=0):
     """The diagonal shift's semiglobal walk (variants 23-28): row r of a
     stripe in reference column j holds P = H - e (r + j) + o,
     R = E - e (r + j) + e, Q = F - e (r + j) + e, so neither gap chain adds
@@ -131,7 +131,10 @@ def walk_semiglobal_diag(q, r, scoring, nmax, rng_vals, st=16, advance_top=True)
     column j's space (a pair: best - 2e against the pair's A - e and B) and
     the last column's rows come back to H + o through + e (r + m - 1).
     advance_top=False leaves the top border at column 0's value (a planted
-    mistake)."""
+    mistake). rebase = K (even) restarts the column part of the shift every
+    K columns, as walk_local_columns does: the rows' P and R, the corner,
+    the top border and the bottom-row best move by + e K at each block
+    start, and the last column comes back through + e ((m - 1) mod K)."""
     ma, mi, o, e = scoring
     S, RR, T = plan(nmax, st)
     rows = [None] * (T - len(q)) + list(q)
@@ -165,6 +168,13 @@ def walk_semiglobal_diag(q, r, scoring, nmax, rng_vals, st=16, advance_top=True)
 
         j = 0
         while j < m:
+            if rebase and j and j % rebase == 0:
+                for k in range(h):
+                    Pv[k] += e * rebase
+                    R[k] = add(R[k], e * rebase)
+                hdt += e * rebase
+                vtop += e * rebase
+                bestL = add(bestL, e * rebase)
             cols = (j, j + 1) if j + 1 < m else (j,)
             for jj in cols:
                 if s == 0:
@@ -182,22 +192,31 @@ def walk_semiglobal_diag(q, r, scoring, nmax, rng_vals, st=16, advance_top=True)
             else:
                 bestL = rng_vals.see(mx(add(bestL, -e), newb[-1][0]))
             j += 2
+        jl = (m - 1) % rebase if rebase else m - 1  # column m - 1's place in its block
         for k in range(h):
             if row0 + k >= pad:
-                best = mx(best, rng_vals.see(Pv[k] + e * k + e * (m - 1)))
+                best = mx(best, rng_vals.see(Pv[k] + e * k + e * jl))
         if s == S - 1:
-            best = mx(best, rng_vals.see(add(bestL, e * (h - 1) + e * (m - 1))))
+            best = mx(best, rng_vals.see(add(bestL, e * (h - 1) + e * jl)))
         bottom = newb
         row0 += h
     return best - o
 
 
-def walk_local_columns(q, r, scoring, nmax, rng_vals, st=16):
+def walk_local_columns(q, r, scoring, nmax, rng_vals, st=16, floor_on_chain=True, floor_shift=0, rebase=0):
     """Variant 7's local walk: column j holds P = H - e j + o,
     R = E - e j + e, Q = F - e j + e, the zero floor is z_j = e (1 - j), and
     the best is carried in the space of the last column done (two columns a
     pair: best - 2e against the pair's tA - e and tB), back to H + o at the
-    stripe's end."""
+    stripe's end. The floor rides on the horizontal chain, R = max(P, R, z_j)
+    (E' = max(E, 0), which changes no H), so the cell's own maximum needs no
+    fourth operand; floor_on_chain=False applies it to the cell instead (the
+    earlier form, equal). floor_shift moves the floor by that many extension
+    steps (a planted mistake when not 0). rebase = K (even) restarts the
+    shift every K columns: at column j = K, 2K, ... every value carried in
+    from column j - 1 (the rows' P and R, the corner, the top border, the
+    floor and the best) moves into the new block's space by + e K, and the
+    best returns to H + o through + e ((m - 1) mod K)."""
     ma, mi, o, e = scoring
     S, RR, T = plan(nmax, st)
     rows = [None] * (T - len(q)) + list(q)
@@ -218,9 +237,14 @@ def walk_local_columns(q, r, scoring, nmax, rng_vals, st=16):
         def column(j, pu, qq, pd, zf):
             for k in range(h):
                 sc = subst(rows[row0 + k], r[j], ma, mi) - o
-                R[k] = rng_vals.see(mx(Pv[k], R[k]))
-                qq = rng_vals.see(add(mx(pu, qq), e))
-                hh = rng_vals.see(mx(mx(rng_vals.see(pd + sc), R[k], qq), zf))
+                if floor_on_chain:
+                    R[k] = rng_vals.see(mx(Pv[k], R[k], zf + floor_shift * e))
+                    qq = rng_vals.see(add(mx(pu, qq), e))
+                    hh = rng_vals.see(mx(rng_vals.see(pd + sc), R[k], qq))
+                else:
+                    R[k] = rng_vals.see(mx(Pv[k], R[k]))
+                    qq = rng_vals.see(add(mx(pu, qq), e))
+                    hh = rng_vals.see(mx(mx(rng_vals.see(pd + sc), R[k], qq), zf))
                 pd = Pv[k]
                 Pv[k] = rng_vals.see(hh + (o - e))
                 pu = Pv[k]
@@ -228,6 +252,14 @@ def walk_local_columns(q, r, scoring, nmax, rng_vals, st=16):
 
         j = 0
         while j < m:
+            if rebase and j and j % rebase == 0:
+                for k in range(h):
+                    Pv[k] += e * rebase
+                    R[k] = add(R[k], e * rebase)
+                hdt += e * rebase
+                vtop += e * rebase
+                z += e * rebase
+                b += e * rebase
             cols = (j, j + 1) if j + 1 < m else (j,)
             ts = []
             for jj in cols:
@@ -247,7 +279,7 @@ def walk_local_columns(q, r, scoring, nmax, rng_vals, st=16):
             else:
                 b = rng_vals.see(max(rng_vals.see(b - e), ts[0]))
             j += 2
-        best = rng_vals.see(b + e * (m - 1))
+        best = rng_vals.see(b + e * ((m - 1) % rebase if rebase else m - 1))
         bottom = newb
         row0 += h
     return best - o
@@ -290,8 +322,9 @@ def test_row_shifted_semiglobal_walk_equals_oracle(seed, st):
 
 @pytest.mark.parametrize("seed,st", [(3, 16), (4, 16), (8, 24), (9, 32)])
 def test_column_shifted_local_walk_equals_oracle(seed, st):
-    """The column shift (local), on 16-, 24- and 32-row stripes: scores
-    equal the oracle's, and every value lies in [-(|o| + 3|e| + M),
+    """The column shift (local), on 16-, 24- and 32-row stripes, with the
+    zero floor on the horizontal chain (the kernel's form) and on the cell:
+    scores equal the oracle's, and every value lies in [-(|o| + 3|e| + M),
     kM + |e| (m + 2) + M], m the reference length, the range colshift_bound
     rests on (whatever the stripe height)."""
     for sc, q, ref, nmax in cases(seed, 150, n_max=4 * st):
@@ -299,6 +332,7 @@ def test_column_shifted_local_walk_equals_oracle(seed, st):
         got = walk_local_columns(q, ref, sc, nmax, vals, st)
         want = O.score(np.array(q, np.uint8), np.array(ref, np.uint8), sc, True)
         assert got == want, (sc, q, ref, nmax)
+        assert walk_local_columns(q, ref, sc, nmax, vals, st, floor_on_chain=False) == want
         M, O_, E = bounds(sc)
         k = min(len(q), len(ref))
         lo, hi = -(O_ + 3 * E + M), k * M + E * (len(ref) + 2) + M
@@ -324,6 +358,25 @@ def test_diagonal_shifted_semiglobal_walk_equals_oracle(seed, st):
         assert max(-lo, hi) <= len(ref) * (M + E) + O_ + (st + 2) * E + M
 
 
+@pytest.mark.parametrize("seed,st,K", [(31, 16, 8), (32, 24, 16), (33, 32, 8), (34, 32, 2)])
+def test_diagonal_shift_rebased_every_k_columns_equals_oracle(seed, st, K):
+    """The diagonal shift with its column part restarted every K columns:
+    scores equal the oracle's on references several blocks long, and every
+    value lies in [-(kM + |o| + M + (st + 1)|e|), kM + |e| (st + K - 2)]:
+    the range diagshift_bound rests on with the block width in place of the
+    reference length."""
+    for sc, q, ref, nmax in cases(seed, 150, n_max=4 * st):
+        ref = (ref * 3)[: len(ref) * 3]
+        vals = Range()
+        got = walk_semiglobal_diag(q, ref, sc, nmax, vals, st, rebase=K)
+        want = O.score(np.array(q, np.uint8), np.array(ref, np.uint8), sc, False)
+        assert got == want, (sc, q, ref, nmax, st, K)
+        M, O_, E = bounds(sc)
+        k = min(len(q), len(ref))
+        lo, hi = -(k * M + O_ + M + (st + 1) * E), k * M + E * (st + K - 2)
+        assert lo <= vals.lo and vals.hi <= hi, (sc, vals.lo, vals.hi, lo, hi, st, K)
+
+
 def test_a_wrong_diagonal_top_border_is_caught():
     """The diagonal-shift model with the first stripe's top border left at
     column 0's value (not advanced by -e per column) disagrees with the
@@ -335,6 +388,102 @@ def test_a_wrong_diagonal_top_border_is_caught():
         tried += 1
         got = walk_semiglobal_diag(q, ref, sc, nmax, Range(), 16, advance_top=False)
         want = O.score(np.array(q, np.uint8), np.array(ref, np.uint8), sc, False)
+        wrong += got != want
+    assert tried and wrong > 0
+
+
+@pytest.mark.parametrize("seed,st,K", [(21, 16, 8), (22, 24, 16), (23, 32, 8), (24, 16, 2)])
+def test_column_shift_rebased_every_k_columns_equals_oracle(seed, st, K):
+    """The column shift restarted every K columns (local mode on panels
+    whose references are too long for the shift's range): scores equal the
+    oracle's on references several blocks long, and every value lies in
+    [-(|o| + 3|e| + M), kM + |e| (K + 2) + M]: the range colshift_bound
+    rests on with the block width in place of the reference length."""
+    for sc, q, ref, nmax in cases(seed, 150, n_max=4 * st):
+        ref = (ref * 3)[: len(ref) * 3]  # 3 to 72 columns: several blocks
+        vals = Range()
+        got = walk_local_columns(q, ref, sc, nmax, vals, st, rebase=K)
+        want = O.score(np.array(q, np.uint8), np.array(ref, np.uint8), sc, True)
+        assert got == want, (sc, q, ref, nmax, K)
+        M, O_, E = bounds(sc)
+        k = min(len(q), len(ref))
+        lo, hi = -(O_ + 3 * E + M), k * M + E * (K + 2) + M
+        assert lo <= vals.lo and vals.hi <= hi, (sc, vals.lo, vals.hi, lo, hi, K)
+
+
+def test_a_missing_block_rebase_is_caught():
+    """The rebased column shift with the best left in the old block's space
+    at a block boundary (its + e K dropped) disagrees with the oracle."""
+    wrong = tried = 0
+    for sc, q, ref, nmax in cases(25, 120):
+        if sc[3] == 0:
+            continue
+        ref = (ref * 3)[: len(ref) * 3]
+        tried += 1
+        want = O.score(np.array(q, np.uint8), np.array(ref, np.uint8), sc, True)
+        assert walk_local_columns(q, ref, sc, nmax, Range(), 16, rebase=4) == want
+        wrong += walk_local_columns_rebase_without_best(q, ref, sc, nmax, 4) != want
+    assert tried and wrong > 0
+
+
+def walk_local_columns_rebase_without_best(q, r, scoring, nmax, K):
+    """walk_local_columns(rebase=K) with the best's block conversion left
+    out (the mistake)."""
+    ma, mi, o, e = scoring
+    S, RR, T = plan(nmax, 16)
+    rows = [None] * (T - len(q)) + list(q)
+    m = len(r)
+    best = o
+    bottom = None
+    row0 = 0
+    for s in range(S):
+        h = RR if s == 0 else 16
+        Pv = [o + e] * h
+        R = [NEG] * h
+        hdt, vtop, z, b = o + e, o, e, best + e
+        newb = []
+        for j in range(m):
+            if j and j % K == 0:
+                Pv = [x + e * K for x in Pv]
+                R = [add(x, e * K) for x in R]
+                hdt += e * K
+                vtop += e * K
+                z += e * K
+            if s == 0:
+                pu, qq = vtop, NEG
+                vtop -= e
+            else:
+                pu, qq = bottom[j]
+            pd = hdt
+            hdt = pu
+            for k in range(h):
+                sc = subst(rows[row0 + k], r[j], ma, mi) - o
+                R[k] = mx(Pv[k], R[k], z)
+                qq = add(mx(pu, qq), e)
+                hh = mx(pd + sc, R[k], qq)
+                pd = Pv[k]
+                Pv[k] = hh + (o - e)
+                pu = Pv[k]
+            z -= e
+            b = max(b - e, max(Pv))
+            newb.append((pu, qq))
+        best = b + e * ((m - 1) % K)
+        bottom = newb
+        row0 += h
+    return best - o
+
+
+def test_a_wrong_chain_floor_is_caught():
+    """The local model with the chain's floor one extension low (H held at
+    or above e instead of 0) disagrees with the oracle, so the equality
+    above sees a floor in the wrong column's space."""
+    wrong = tried = 0
+    for sc, q, ref, nmax in cases(16, 120):
+        if sc[3] == 0:
+            continue
+        tried += 1
+        got = walk_local_columns(q, ref, sc, nmax, Range(), 16, floor_shift=1)
+        want = O.score(np.array(q, np.uint8), np.array(ref, np.uint8), sc, True)
         wrong += got != want
     assert tried and wrong > 0
 
