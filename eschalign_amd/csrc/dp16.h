@@ -298,9 +298,10 @@ __device__ __forceinline__ void column16s_pair(typename Pk<F16>::T (&scA)[RR], t
 // E' = max(E, 0) changes no H (by induction E'[j] = max(H[j-1] + o,
 // E'[j-1] + e, 0) = max(E[j], 0), e <= 0), and the cell's maximum then
 // includes it. 6 packed ops per row in fp16 (R's three-input maximum is one
-// op), 8 in int16, where the unshifted local column takes 7.5 / 10 with its
-// tree; a column's rows share one shift, so their best folds by a tree of
-// three-way maxima as before. Returns the column's best P (column j space).
+// op) and 8 in int16; a column's rows share one shift, so their best folds
+// by a tree of three-way maxima as before (1/2 op a row in fp16, 1 in
+// int16): 6.5 and 9 per cell where the unshifted local column takes 8.5 and
+// 10. Returns the column's best P (column j space).
 template <int RR, bool F16>
 __device__ __forceinline__ typename Pk<F16>::T column16c(const typename Pk<F16>::T (&sc)[RR],
                                                          typename Pk<F16>::T (&Pv)[RR],

@@ -1353,9 +1353,14 @@ esch_status esch_autotune(esch_scorer *s, const uint8_t *const *reads, const siz
         }
         return esch_batch_timing(b, out, nullptr);
     };
-    // 1. the inter-sequence kernel's variants, routing as configured
-    int best_v = esch_get_variant(s);
-    float best_ms = 0.0f;
+    // 1. the inter-sequence kernel's variants, routing as configured; the
+    // scorer's current variant stays unless another is faster by more than
+    // 1% (near-ties, such as schedules 12 and 13 on 32 rows, would
+    // otherwise flip between runs)
+    const int v_in = esch_get_variant(s);
+    int best_v = v_in;
+    float best_ms = 0.0f, in_ms = -1.0f;
+    bool any = false;
     for (int v = 0; !st && v < kDp16Variants; ++v) {
         if (!s->usable16[v]) {
             if (ms) ms[v] = -1.0f;  // not applicable to this panel
@@ -1366,11 +1371,14 @@ esch_status esch_autotune(esch_scorer *s, const uint8_t *const *reads, const siz
         if (!st) st = timed(&dp);
         if (st) break;
         if (ms) ms[v] = dp;
-        if (v == 0 || dp < best_ms) {
+        if (v == v_in) in_ms = dp;
+        if (!any || dp < best_ms) {
             best_ms = dp;
             best_v = v;
+            any = true;
         }
     }
+    if (in_ms > 0.0f && best_ms >= 0.99f * in_ms) best_v = v_in;
     // the pick is the autotuner's, not the caller's: a batch whose reads
     // admit an fp16 variant the panel does not still takes it
     if (!st) st = set_variant(s, best_v, false);

@@ -465,7 +465,8 @@ esch_status esch_bam_build_records(const uint8_t *in, size_t in_bytes, size_t n,
  *      column j by |e| j (9 instead of 10; needs longest reference x (M +
  *      |extend|) + |open| + 3 |extend| + M <= 32000);
  *   7  (default where it applies) variant 5 with those shifts: 6 packed ops
- *      instead of 7 in semiglobal mode, 7.5 instead of 8.5 in local mode;
+ *      instead of 7 in semiglobal mode, 6.5 instead of 8.5 in local mode
+ *      (where the zero floor rides on the shifted chain's maximum);
  *      the same sums at most 2048;
  *   8  variant 4 with those shifts (fewer registers than 7, one more
  *      resident wave, a shallower prefetch); the condition of 7.
