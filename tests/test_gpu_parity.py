@@ -859,6 +859,38 @@ def test_swapped_roles_equal_the_oracle():
 
 
 
+def test_column_restart_through_the_swapped_path():
+    """A panel with references up to 1,900 nt under 1,-1,-2,-1: the
+    default variants shift columns with a restart (local mode: the column
+    shift every 128 columns; semiglobal mode: the diagonal shift every 64),
+    and the swapped-roles path runs them on the reads as a panel with the
+    scorer's block. Ordinary and swapped scores equal the oracle in both
+    modes, and so does the fused selection on both strands."""
+    rng = G.Rng(0x5EED_0B08)
+    panel_l = [G.random_seq(rng, n, 0.01) for n in (1900, 1700, 1500)] + G.random_panel(rng, 130, 40, 400)
+    reads = [panel_l[0][200:1400], panel_l[1][:900], G.random_seq(rng, 1700, 0.02)]
+    reads += [G.chimeric_read(rng, panel_l, 60 + rng.below(500)) for _ in range(12)]
+    queries = codes_of(reads)
+    panel = ea.Panel(panel_l)
+    L = max(map(len, panel_l))
+    for mode, block in ((ea.Mode.LOCAL, 128), (ea.Mode.SEMIGLOBAL, 64)):
+        gpu = ea.GpuScorer(panel, ea.Scoring(1, -1, -2, -1), mode)
+        v = gpu.variant
+        assert ea.variant_info(v).f16 and gpu.column_block(v) == block, (mode, v, gpu.column_block(v), L)
+        want = oracle(queries, panel.codes, (1, -1, -2, -1), mode)
+        gpu.swap_rows = 0
+        got0, ok0, swapped0, _ = _scores(gpu, queries)
+        assert not swapped0 and ok0.all() and np.array_equal(got0, want), mode
+        gpu.swap_rows = 127
+        got, ok, swapped, _ = _scores(gpu, queries)
+        assert swapped and ok.all(), mode
+        bad = np.argwhere(got != want)
+        assert bad.size == 0, (mode, bad[:4])
+        sel = gpu.select_batch(queries, both_strands=True, max_ties=3)
+        gpu.swap_rows = 0
+        assert gpu.select_batch(queries, both_strands=True, max_ties=3) == sel, mode
+
+
 def test_swapped_roles_by_the_rule_on_a_wide_panel():
     """By the default rule a panel of at least 4,096 references swaps roles
     for batches of up to one row per reference: 40 reads on both strands

@@ -105,6 +105,26 @@ def test_noalloc_equals_score_batch_i32_two_calls():
         assert np.array_equal(got2[:100], O.score_matrix(small[:100], panel, scoring, local, THREADS))
 
 
+def test_noalloc_with_a_restarted_column_shift():
+    """References up to 1,900 nt under 1,-1,-2,-1: the default variants
+    restart their column shift (local every 128 columns, semiglobal every
+    64); the allocation-free entry point runs them with the scorer's block
+    and equals esch_score_batch_i32 and, on a sample, the scalar oracle, over
+    two calls, the second smaller."""
+    rng = G.Rng(0x5EED_7006)
+    panel_l = [G.random_seq(rng, n, 0.01) for n in (1900, 1700, 1500)] + G.random_panel(rng, 60, 40, 400)
+    reads = [panel_l[0][200:1400], G.random_seq(rng, 1700, 0.02)]
+    reads += [G.random_seq(rng, 20 + rng.below(600), 0.02) for _ in range(40)]
+    q = [G.encode(r) for r in reads]
+    for mode, block in ((ea.Mode.LOCAL, 128), (ea.Mode.SEMIGLOBAL, 64)):
+        gpu = ea.GpuScorer(ea.Panel(panel_l), ea.Scoring(1, -1, -2, -1), mode)
+        assert gpu.column_block(gpu.variant) == block, (mode, gpu.variant)
+        got = check_equal(gpu, q, f"{ea.Mode.name(mode)} restart call 1")
+        check_equal(gpu, q[:7], f"{ea.Mode.name(mode)} restart call 2")
+        want = O.score_matrix(q[:6], ea.Panel(panel_l).codes, (1, -1, -2, -1), mode == ea.Mode.LOCAL, THREADS)
+        assert np.array_equal(got[:6], want), mode
+
+
 def test_noalloc_int32_rescue_and_long_references():
     """Every query outside the int16 bound (1000,-1000,-1000,-1000) takes
     the int32 kernel; a panel of 5,000-nt references (the 16-lane
