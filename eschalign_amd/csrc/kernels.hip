@@ -840,6 +840,10 @@ __device__ __forceinline__ int wave_sum(int v) {
     return v;
 }
 
+// Candidates a lane of k_select holds in registers: a read's whole
+// candidate list (both strands) up to 64 x this many is loaded once.
+constexpr int kSelectRegs = 8;
+
 __global__ __launch_bounds__(256) void k_select(KArgs a) {
     const DevDesc &d = *a.desc;
     const int n_reads = d.n_reads, both = d.both, nr = a.panel.n_refs;
@@ -862,18 +866,14 @@ __global__ __launch_bounds__(256) void k_select(KArgs a) {
             if (!both) return fwd[k];
             return (k & 1) ? rev[k >> 1] : fwd[k >> 1];
         };
-        int b = INT32_MIN;
-        for (int k = lane; k < nc; k += 64) b = max(b, cand(k));
-        b = wave_max(b);
         const int64_t take_lim = max_ties < 0 ? INT64_MAX : max_ties + 1;
         const int lim = (int)min((int64_t)cap, take_lim);
         int nt = 0, so = INT32_MIN, hs = 0, base = 0;
-        for (int c0 = 0; c0 < nc; c0 += 64) {
+        // the tie pass over candidates c0 .. c0 + 63, lane k holding s
+        auto tie_pass = [&](int c0, int s, int b) {
             const int k = c0 + lane;
-            int s = 0;
             bool tie = false;
             if (k < nc) {
-                s = cand(k);
                 tie = s == b;
                 if (!tie) {
                     so = max(so, s);
@@ -888,6 +888,26 @@ __global__ __launch_bounds__(256) void k_select(KArgs a) {
                 a.ties[(size_t)rd * cap + base + rank] = (r << 1) | rv;
             }
             base += __popcll(mask);
+        };
+        int b = INT32_MIN;
+        if (nc <= 64 * kSelectRegs) {
+            // every candidate loaded once, into registers, all loads in
+            // flight together; the tie pass reads the registers
+            int sv[kSelectRegs];
+#pragma unroll
+            for (int i = 0; i < kSelectRegs; ++i) {
+                const int k = 64 * i + lane;
+                sv[i] = k < nc ? cand(k) : INT32_MIN;
+                b = max(b, sv[i]);
+            }
+            b = wave_max(b);
+#pragma unroll
+            for (int i = 0; i < kSelectRegs; ++i)
+                if (64 * i < nc) tie_pass(64 * i, sv[i], b);
+        } else {
+            for (int k = lane; k < nc; k += 64) b = max(b, cand(k));
+            b = wave_max(b);
+            for (int c0 = 0; c0 < nc; c0 += 64) tie_pass(c0, c0 + lane < nc ? cand(c0 + lane) : 0, b);
         }
         nt = base;
         so = wave_max(so);
@@ -1022,7 +1042,8 @@ hipError_t launch_peak16(uint32_t *out, int blocks, int iters, hipStream_t st) {
 
 hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st) {
     hipLaunchKernelGGL(k_zero, dim3((kMaxLen16 + 1 + 255) / 256), dim3(256), 0, st, a);
-    hipLaunchKernelGGL(k_revcomp, dim3(g.grid_aux), dim3(256), 0, st, a);
+    // one wave per read, latency-bound: twice the auxiliary grid's waves
+    hipLaunchKernelGGL(k_revcomp, dim3(2 * g.grid_aux), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_classify, dim3(g.grid_aux), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, a);
     hipLaunchKernelGGL(k_scatter, dim3(g.grid_aux), dim3(256), 0, st, a);
@@ -1075,7 +1096,9 @@ hipError_t diag_occupancy(bool local, size_t lds, int *blocks) {
 }
 
 hipError_t launch_select(const KArgs &a, const Launch &g, hipStream_t st) {
-    hipLaunchKernelGGL(k_select, dim3(g.grid_aux), dim3(256), 0, st, a);
+    // two waves per read slot of the auxiliary grid: a wave's reads are
+    // latency-bound loads, so more resident waves hide more of it
+    hipLaunchKernelGGL(k_select, dim3(2 * g.grid_aux), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
