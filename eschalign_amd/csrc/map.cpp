@@ -166,6 +166,15 @@ struct Ctx {
     };
     esch_status select_trace(const std::vector<int32_t> &rows, int both, int32_t min_score, int64_t max_ties,
                              Selected &S) {
+        std::vector<int32_t> jrows, jrefs;
+        if (esch_status e = select(rows, both, min_score, max_ties, S, jrows, jrefs)) return e;
+        return trace(jrows, jrefs, S.t);
+    }
+
+    // The selection half of select_trace: the records and tie lists, and the
+    // (row, reference) job of every taken tie for `trace`.
+    esch_status select(const std::vector<int32_t> &rows, int both, int32_t min_score, int64_t max_ties,
+                       Selected &S, std::vector<int32_t> &jrows, std::vector<int32_t> &jrefs) {
         const size_t n = rows.size();
         if (!n) return ESCH_OK;
         hipStream_t st = b->stream;
@@ -207,7 +216,8 @@ struct Ctx {
         S.tie_at.assign(n, -1);  // into ties_over for the overflowed
         for (size_t i = 0; i < over.size(); ++i) S.tie_at[over[i]] = over_off[i];
         // every taken tie traced (lib/mapper.rs:486-496)
-        std::vector<int32_t> jrows, jrefs;
+        jrows.clear();
+        jrefs.clear();
         S.first.assign(n, 0);
         for (size_t e = 0; e < n; ++e) {
             S.first[e] = jrows.size();
@@ -217,7 +227,7 @@ struct Ctx {
                 jrefs.push_back(c >> 1);
             }
         }
-        return trace(jrows, jrefs, S.t);
+        return ESCH_OK;
     }
 
     // The host half: records and hits (lib/mapper.rs:498-550); no device use.
@@ -355,68 +365,96 @@ esch_status map_reads_core(esch_scorer *s, size_t n, const Getter &get, int both
     if (esch_status st = scorer_ready(s)) return st;
     std::lock_guard<std::mutex> call(s->map->mu);
     // reads per chunk: what the rows' memory allows, and by default about a
-    // quarter of the call (at least 2^17 reads). A chunk is staged while the
-    // one before scores, and its records are built while the next one
-    // scores; its selection and tracebacks run between the two scorings
-    // (beside the persistent scoring kernel they would wait for it).
-    // Measured on 1M reads of the 164-reference panel: 1.64 M reads/s in
-    // one chunk, 1.75 M in four, 1.67 M in sixteen (smaller traceback and
-    // scoring launches).
+    // quarter of the call (at least 2^17 reads). Chunk ci, once scored:
+    //  - its selection (resolve.hip) on its resident rows;
+    //  - the next chunk's scoring launched, and the chunk after that staged
+    //    (gathered and uploaded) beside it;
+    //  - its tracebacks: their host half (ordering, staging) beside the
+    //    scoring, their kernels after it (beside the persistent scoring
+    //    kernel they would only wait for it);
+    //  - its records built on a worker thread, beside the next chunk.
+    // So the device idles only for a chunk's selection round trip. Measured
+    // on 1M reads of the 164-reference panel with the tracebacks between
+    // the scorings instead: 1.64 M reads/s in one chunk, 1.75 M in four,
+    // 1.67 M in sixteen.
     const size_t cap = std::max<size_t>(1, chunk_rows(s) / 2);
     const size_t chunk = s->map_chunk ? cap : std::min(cap, std::max<size_t>((size_t)1 << 17, (n + 3) / 4));
     const size_t n_chunks = (n + chunk - 1) / chunk;
+    // chunks ci, ci + 1 and ci + 2, and the records' worker; joined and
+    // handed back on every exit (before M goes)
     struct Held {
         esch_scorer *s;
-        esch_batch *cur = nullptr, *next = nullptr;
+        esch_batch *b[3] = {nullptr, nullptr, nullptr};
+        std::thread builder;
+        double build_ms = 0.0;  // the last builder's time, read after its join
+        void join(esch_map_stats &stats) {
+            if (!builder.joinable()) return;
+            builder.join();
+            stats.assemble_ms += build_ms;
+        }
         ~Held() {
-            if (cur) give_batch(s, cur);
-            if (next) give_batch(s, next);
+            if (builder.joinable()) builder.join();
+            for (esch_batch *x : b)
+                if (x) give_batch(s, x);
         }
     } held{s};
+    auto range = [&](size_t ci) { return std::make_pair(ci * chunk, std::min(n, ci * chunk + chunk)); };
     if (n_chunks) {
-        if (esch_status st = load_chunk(s, get, 0, std::min(n, chunk), both, 0, &held.cur, M->stats)) return st;
-        if (esch_status st = score_chunk(held.cur)) return st;
+        if (esch_status st = load_chunk(s, get, 0, range(0).second, both, 0, &held.b[0], M->stats)) return st;
+        if (esch_status st = score_chunk(held.b[0])) return st;
     }
+    if (n_chunks > 1)
+        if (esch_status st = load_chunk(s, get, range(1).first, range(1).second, both, 0, &held.b[1], M->stats))
+            return st;
     for (size_t ci = 0; ci < n_chunks; ++ci) {
-        const size_t k0 = ci * chunk, k1 = std::min(n, k0 + chunk);
-        if (ci + 1 < n_chunks)
-            if (esch_status st = load_chunk(s, get, k1, std::min(n, k1 + chunk), both, 0, &held.next, M->stats))
-                return st;
-        esch_batch *b = held.cur;
+        const size_t k0 = range(ci).first, k1 = range(ci).second;
+        esch_batch *b = held.b[0];
         if (esch_status st = finish_chunk(b, M->stats)) return st;
         Ctx cx{s, b, *s->map, M->stats};
         std::vector<int32_t> rows;  // non-empty reads; empty ones have best None
         for (size_t k = k0; k < k1; ++k)
             if (b->h_qlen[k - k0]) rows.push_back((int32_t)(k - k0));
-        Ctx::Selected S;
-        if (esch_status st = cx.select_trace(rows, both, min_score, max_ties, S)) return st;
-        // this chunk's device work is done: the next chunk scores meanwhile
-        if (held.next)
-            if (esch_status st = score_chunk(held.next)) return st;
-        // The chunk's hits go straight after the earlier chunks' (the
-        // result's arrays serve as the pool: offsets come out absolute), and
-        // a read without hits points where the next read's hits start.
-        const auto ta = Clock::now();
-        std::vector<esch_read_mapping> maps;
-        Pool pool;
-        pool.hits.swap(M->hits);
-        pool.ops.swap(M->ops);
-        Ctx::build(rows.size(), S, maps, pool);
-        M->hits.swap(pool.hits);
-        M->ops.swap(pool.ops);
-        for (size_t k = k0; k < k1; ++k) M->reads[k] = esch_read_mapping{};
-        for (size_t e = 0; e < rows.size(); ++e) M->reads[k0 + (size_t)rows[e]] = maps[e];
-        int64_t next = (int64_t)M->hits.size();  // reads without hits, from the back
-        for (size_t k = k1; k-- > k0;) {
-            esch_read_mapping &m = M->reads[k];
-            if (m.n_hits) next = m.hit_offset;
-            else m.hit_offset = next;
-        }
-        M->stats.assemble_ms += ms_since(ta);
-        give_batch(s, held.cur);
-        held.cur = held.next;
-        held.next = nullptr;
+        std::unique_ptr<Ctx::Selected> S(new Ctx::Selected);
+        std::vector<int32_t> jrows, jrefs;
+        if (esch_status st = cx.select(rows, both, min_score, max_ties, *S, jrows, jrefs)) return st;
+        if (held.b[1])
+            if (esch_status st = score_chunk(held.b[1])) return st;
+        if (ci + 2 < n_chunks)
+            if (esch_status st = load_chunk(s, get, range(ci + 2).first, range(ci + 2).second, both, 0, &held.b[2],
+                                            M->stats))
+                return st;
+        if (esch_status st = cx.trace(jrows, jrefs, S->t)) return st;
+        // The chunk's records on the worker. Its hits go straight after the
+        // earlier chunks' (the result's arrays serve as the pool: offsets
+        // come out absolute), and a read without hits points where the next
+        // read's hits start; one builder at a time keeps that order.
+        held.join(M->stats);
+        esch_mapped *m = M.get();
+        held.builder = std::thread([m, k0, k1, rows = std::move(rows), S = std::move(S), &held]() {
+            const auto ta = Clock::now();
+            std::vector<esch_read_mapping> maps;
+            Pool pool;
+            pool.hits.swap(m->hits);
+            pool.ops.swap(m->ops);
+            Ctx::build(rows.size(), *S, maps, pool);
+            m->hits.swap(pool.hits);
+            m->ops.swap(pool.ops);
+            for (size_t k = k0; k < k1; ++k) m->reads[k] = esch_read_mapping{};
+            for (size_t e = 0; e < rows.size(); ++e) m->reads[k0 + (size_t)rows[e]] = maps[e];
+            int64_t next = (int64_t)m->hits.size();  // reads without hits, from the back
+            for (size_t k = k1; k-- > k0;) {
+                esch_read_mapping &r = m->reads[k];
+                if (r.n_hits) next = r.hit_offset;
+                else r.hit_offset = next;
+            }
+            held.build_ms = ms_since(ta);
+        });
+        give_batch(s, held.b[0]);
+        held.b[0] = held.b[1];
+        held.b[1] = held.b[2];
+        held.b[2] = nullptr;
     }
+    held.join(M->stats);
     M->stats.total_ms = ms_since(t0);
     *out = M.release();
     return ESCH_OK;
