@@ -738,9 +738,10 @@ class GpuScorer:
     """lib/cuda/mod.rs:328-709: a panel resident on the device under one
     scoring and one mode."""
 
-    def __init__(self, panel: Panel, scoring: Scoring = Scoring(), mode: int = Mode.LOCAL,
+    def __init__(self, panel: Panel, scoring: Optional[Scoring] = None, mode: int = Mode.LOCAL,
                  ordinal: int = 0):
         L = load_library()
+        scoring = Scoring() if scoring is None else scoring
         self.panel = panel
         self.scoring = scoring
         self.mode = mode
@@ -1220,7 +1221,8 @@ class Server:
 
     __del__ = close
 
-    def add_panel(self, panel: Panel, scoring: Scoring = Scoring(), mode: int = Mode.LOCAL) -> int:
+    def add_panel(self, panel: Panel, scoring: Optional[Scoring] = None, mode: int = Mode.LOCAL) -> int:
+        scoring = Scoring() if scoring is None else scoring
         keep, ptrs, lens = _ptr_array(panel.codes)
         pid = C.c_int()
         _check(_lib.esch_server_add_panel(self._h, ptrs, lens, len(keep), *scoring.tuple(), mode,
