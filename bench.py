@@ -566,7 +566,7 @@ def serving(rate, duration, seed=0x5E4E):
     for `duration` s, each request 1-50k reads (log-uniform) of the batched
     workload's lengths against a uniformly chosen panel, both strands,
     semiglobal 1,-1,-2,-1, fused selection. Requests are cut into device
-    batches of at most 2^38 DP cells, two batches in flight. Reports latency
+    batches of at most 2^37 DP cells, two batches in flight. Reports latency
     from each request's scheduled arrival to its result (the server's
     submit-to-result time plus any lag of the submitting thread) and reads/s
     over the run."""
@@ -579,7 +579,7 @@ def serving(rate, duration, seed=0x5E4E):
               for k, n in enumerate((164, 1000, 10000))]
     scoring = ea.Scoring(1, -1, -2, -1)
     server = ea.Server(hbm_budget=96 << 30, max_batch_reads=50_000, max_batch_bases=8_000_000,
-                       max_slots=2, max_batch_cells=1 << 38)
+                       max_slots=2, max_batch_cells=1 << 37)
     ids = [server.add_panel(ea.Panel(p), scoring, ea.Mode.SEMIGLOBAL) for p in panels]
     for pid, p in zip(ids, panels):
         server.autotune(pid, synth.reads(p, seed=seed + 7, n=5000))

@@ -1204,7 +1204,7 @@ class Server:
     (include/eschalign_amd.h, esch_server_*)."""
 
     def __init__(self, hbm_budget=32 << 30, max_batch_reads=50_000, max_batch_bases=8_000_000,
-                 max_slots=2, ordinal=0, max_batch_cells=1 << 38):
+                 max_slots=2, ordinal=0, max_batch_cells=1 << 37):
         L = load_library()
         cfg = ServerConfig(ordinal, hbm_budget, max_batch_reads, max_batch_bases, max_slots,
                            max_batch_cells)
