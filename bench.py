@@ -86,6 +86,7 @@ VALU_OPS_PER_CELL_ISSUED = {("semiglobal", 3): round(344_021_737_036 * 128 / 5_3
                             ("semiglobal", 10): round(262_088_845_368 * 128 / 5_303_272_453_076, 3),
                             ("semiglobal", 11): round(303_721_597_544 * 128 / 5_303_272_453_076, 3),
                             ("semiglobal", 24): round(220_033_994_246 * 128 / 5_303_272_453_076, 3),
+                            ("semiglobal", 26): round(219_282_137_274 * 128 / 5_303_272_453_076, 3),
                             ("semiglobal", 28): round(261_701_617_397 * 128 / 5_303_272_453_076, 3)}
 
 
