@@ -1015,12 +1015,21 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
     const uint32_t eMa = PK::enc(a.sc.ma - a.sc.o - dse), eMi = PK::enc(a.sc.mi - a.sc.o - dse);
     const uint32_t ePre = PK::enc(-a.sc.o - dse);
 
+    // the items this launch takes: all of them, or one slice of a sliced
+    // run (items [lo, hi) from the slice's own cursor)
+    uint32_t lo = 0, hi = n_items;
+    uint32_t *cursor = &a.desc->next16;
+    if (a.n_slices > 1) {
+        lo = (uint32_t)((uint64_t)n_items * (uint32_t)a.slice / (uint32_t)a.n_slices);
+        hi = (uint32_t)((uint64_t)n_items * (uint32_t)(a.slice + 1) / (uint32_t)a.n_slices);
+        cursor = &a.desc->next_slice[a.slice];
+    }
     for (;;) {
         __syncthreads();
-        if (threadIdx.x == 0) s_item = atomicAdd(&a.desc->next16, 1u);
+        if (threadIdx.x == 0) s_item = lo + atomicAdd(cursor, 1u);
         __syncthreads();
         const uint32_t t = s_item;
-        if (t >= n_items) break;
+        if (t >= hi) break;
         const int k = (int)(t / (uint32_t)n_groups);
         const int g = (int)(t - (uint32_t)k * (uint32_t)n_groups);
         const int slot = g * kWaves16 + wv;

@@ -141,6 +141,9 @@ __host__ __device__ inline int len_class(int L) {
 
 // Device-resident batch descriptor: every kernel reads its sizes from here,
 // so one captured graph replays for any batch that fits the buffers.
+// Most launches a sliced k_dp16 run is cut into.
+constexpr int kMaxSlices = 16;
+
 struct DevDesc {
     int32_t n_queries;   // rows scored (reads, then reverse complements)
     int32_t n_reads;     // reads as loaded (mates interleaved when pairs)
@@ -153,6 +156,7 @@ struct DevDesc {
     int64_t max_ties;
     int32_t n16, n32;    // queries on the int16 and int32 kernels
     uint32_t next16, next32a, next32b;  // work-queue cursors
+    uint32_t next_slice[kMaxSlices];     // k_dp16's cursors in a sliced run, one per slice
     int32_t tstrands;    // pairs: template strands considered (1 or 2)
     int32_t diag_min_len;  // reads this long go to the anti-diagonal kernel (0: none)
     int32_t nd;          // queries on the anti-diagonal kernel
@@ -195,6 +199,10 @@ struct KArgs {
     // query q against reference r is the read r's against panel reference
     // q: out[r * n_queries + q], the layout of an ordinary run.
     int32_t swap;
+    // k_dp16 in a sliced run (n_slices > 1): this launch scores slice
+    // `slice` of the run's work items, [n_items slice / n_slices,
+    // n_items (slice + 1) / n_slices), from its own cursor; 0 or 1: all
+    int32_t slice, n_slices;
 };
 
 // Where a kernel writes the score of query q against reference r.

@@ -131,6 +131,14 @@ struct esch_batch {
     std::map<int, hipGraphExec_t> graphs;
     SwapBufs *swap = nullptr;   // the swapped path's buffers (allocated on first use)
     bool last_swapped = false;  // the last run scored with the roles swapped
+    // a sliced run (esch::batch_run_sliced): k_dp16 in n_slices launches,
+    // slices_out of them queued so far; the kernel variant and run flags
+    // they launch with, and a start / end event per slice
+    int n_slices = 0, slices_out = 0;
+    esch::Launch slice_geo{};
+    int slice_what = 0;
+    hipEvent_t sev[2 * esch::kMaxSlices] = {};
+    hipEvent_t pump_ev = nullptr;
 };
 
 namespace esch {
@@ -191,8 +199,26 @@ void give_batch(esch_scorer *s, esch_batch *b);
 // dev_codes (reads already resident, e.g. a batch's codes; the caller has
 // synchronised the stream that wrote them) and nothing is staged. With
 // ops_vec set, the runs go there (resized to fit) instead of ops / ops_cap.
+// With pump set, each wait for the device first queues pump's next slice
+// (batch_pump) after the traceback's work.
 esch_status align_core(esch_scorer *s, size_t n, const std::function<void(size_t, const uint8_t *&, size_t &)> &get,
                        const int32_t *refs, esch_alignment *out, uint32_t *ops, size_t ops_cap, size_t *ops_used,
-                       const uint8_t *dev_codes = nullptr, std::vector<uint32_t> *ops_vec = nullptr);
+                       const uint8_t *dev_codes = nullptr, std::vector<uint32_t> *ops_vec = nullptr,
+                       esch_batch *pump = nullptr);
+
+// Sliced runs. k_dp16 is persistent: once launched it holds every CU until
+// its last work item, so a short kernel of another stream waits for the
+// whole scoring. A sliced run cuts k_dp16's work items into n_slices
+// launches (at most kMaxSlices): the run queues the bucketing and slice 0;
+// batch_pump queues the next slice behind the work that stream `after` has
+// queued so far (a mapping call's selection or tracebacks of the previous
+// chunk), which so runs between two slices; the last slice brings the rest
+// of the run (the anti-diagonal and int32 kernels). Every wait for the
+// batch, read of its results, reload or new run first queues the slices
+// still out (batch_drain). The swapped-roles path and the plain kernel run
+// unsliced (the pump is then a no-op). Not captured in a graph.
+esch_status batch_run_sliced(esch_batch *b, int what, int n_slices);
+esch_status batch_pump(esch_batch *b, hipStream_t after);
+esch_status batch_drain(esch_batch *b);
 
 }  // namespace esch

@@ -543,6 +543,7 @@ __global__ void k_zero(KArgs a) {
         a.desc->next32b = 0;
         a.desc->max16 = 0;
         a.desc->cells = 0;
+        for (int k = 0; k < kMaxSlices; ++k) a.desc->next_slice[k] = 0;
     }
 }
 

@@ -127,6 +127,12 @@ struct Ctx {
     esch_batch *b;
     MapCache &C;
     esch_map_stats &stats;
+    // the next chunk's sliced scoring (or null): each wait for the device
+    // below first queues its next slice behind the work just queued, so
+    // that work runs between two slices and the device stays busy while
+    // the host waits and then uses the results
+    esch_batch *next = nullptr;
+    esch_status pump(hipStream_t st) const { return batch_pump(next, st); }
     // row x of the batch: its codes on the device and its length
     const uint8_t *row_ptr(int32_t x) const { return b->d_codes + b->h_qoff[x]; }
     size_t row_len(int32_t x) const { return (size_t)b->h_qlen[x]; }
@@ -143,7 +149,7 @@ struct Ctx {
                 p = row_ptr(rows[k]);
                 len = row_len(rows[k]);
             },
-            refs.data(), t.al.data(), nullptr, 0, nullptr, b->d_codes, &t.ops);
+            refs.data(), t.al.data(), nullptr, 0, nullptr, b->d_codes, &t.ops, next);
         stats.trace_ms += s->trace_ms;
         stats.trace_wall_ms += ms_since(tw);
         stats.traced += (int64_t)rows.size();
@@ -185,6 +191,7 @@ struct Ctx {
         MateSelArgs a{b->d_out32, (int32_t)b->n_reads, (int32_t)s->n_refs, both, min_score, max_ties,
                       C.list.as<int32_t>(), (int32_t)n, kSelCap, nullptr, C.recs.as<int32_t>(), C.cands.as<int32_t>()};
         HIPC(launch_mate_select(a, kResolveBlocks, st));
+        if (esch_status e = pump(st)) return e;
         if (esch_status e = download(S.recs, C.recs, n * kSelRec, st)) return e;
         if (esch_status e = download(S.ties, C.cands, n * kSelCap, st)) return e;
         const std::vector<int32_t> &recs = S.recs;
@@ -210,6 +217,7 @@ struct Ctx {
             o.recs = C.set.as<int32_t>();
             o.ties = C.out.as<int32_t>();
             HIPC(launch_mate_select(o, kResolveBlocks, st));
+            if (esch_status e = pump(st)) return e;
             if (esch_status e = download(S.ties_over, C.out, (size_t)total, st)) return e;
         }
         stats.select_ms += ms_since(t0);
@@ -330,6 +338,33 @@ esch_status load_chunk(esch_scorer *s, const Getter &get, size_t k0, size_t k1, 
 // Launch the scoring of a loaded batch into resident int32 rows.
 esch_status score_chunk(esch_batch *b) { return esch_batch_run(b, ESCH_RUN_SCORES32, 0, -1, 1); }
 
+// Chunk boundaries of a mapping call of n reads (or pairs) in chunks of
+// `chunk`: whole chunks and, with taper, the last chunk's worth cut in
+// halves down to a quarter chunk (.., chunk / 2, chunk / 4, chunk / 4):
+// the work after the last scoring (its selection, tracebacks and records)
+// has no scoring left to run beside, so the last chunk is kept small.
+std::vector<size_t> chunk_cuts(size_t n, size_t chunk, bool taper) {
+    std::vector<size_t> cuts{0};
+    for (size_t at = 0; at < n;) {
+        const size_t left = n - at;
+        size_t take = std::min(left, chunk);
+        if (taper && left <= chunk && 4 * left > chunk) take = (left + 1) / 2;
+        at += take;
+        cuts.push_back(at);
+    }
+    return cuts;
+}
+
+// The same in slices (batch_run_sliced), for a chunk scored while the
+// previous chunk's selection and tracebacks run: enough slices that those
+// kernels wait at most a slice for the device, few enough that the slices'
+// drains stay a small part of the scoring. Measured on 1M reads of the
+// 164-reference panel and 2M pairs of 500 references (reads / pairs per
+// s): 4 slices 2.281 M / 1.255 M, 6 2.300 M / 1.255 M, 8 2.293 M / 1.250 M,
+// 12 2.270 M / 1.236 M.
+constexpr int kMapSlices = 6;
+esch_status score_chunk_sliced(esch_batch *b) { return batch_run_sliced(b, ESCH_RUN_SCORES32, kMapSlices); }
+
 // Wait for a batch's scoring; count its device time and cells.
 esch_status finish_chunk(esch_batch *b, esch_map_stats &stats) {
     if (esch_status st = esch_batch_wait(b)) return st;
@@ -366,20 +401,20 @@ esch_status map_reads_core(esch_scorer *s, size_t n, const Getter &get, int both
     std::lock_guard<std::mutex> call(s->map->mu);
     // reads per chunk: what the rows' memory allows, and by default about a
     // quarter of the call (at least 2^17 reads). Chunk ci, once scored:
-    //  - its selection (resolve.hip) on its resident rows;
-    //  - the next chunk's scoring launched, and the chunk after that staged
-    //    (gathered and uploaded) beside it;
-    //  - its tracebacks: their host half (ordering, staging) beside the
-    //    scoring, their kernels after it (beside the persistent scoring
-    //    kernel they would only wait for it);
+    //  - the next chunk's scoring launched in slices (score_chunk_sliced);
+    //  - its selection (resolve.hip) on its resident rows, between two of
+    //    those slices;
+    //  - the chunk after that staged (gathered and uploaded);
+    //  - its tracebacks, between two more slices;
     //  - its records built on a worker thread, beside the next chunk.
-    // So the device idles only for a chunk's selection round trip. Measured
-    // on 1M reads of the 164-reference panel with the tracebacks between
-    // the scorings instead: 1.64 M reads/s in one chunk, 1.75 M in four,
-    // 1.67 M in sixteen.
+    // So the device idles only between the last chunk's scoring and its
+    // records. Measured on 1M reads of the 164-reference panel with the
+    // tracebacks between the scorings instead: 1.64 M reads/s in one chunk,
+    // 1.75 M in four, 1.67 M in sixteen.
     const size_t cap = std::max<size_t>(1, chunk_rows(s) / 2);
     const size_t chunk = s->map_chunk ? cap : std::min(cap, std::max<size_t>((size_t)1 << 17, (n + 3) / 4));
-    const size_t n_chunks = (n + chunk - 1) / chunk;
+    const std::vector<size_t> cuts = chunk_cuts(n, chunk, !s->map_chunk && n > chunk);
+    const size_t n_chunks = cuts.size() - 1;
     // chunks ci, ci + 1 and ci + 2, and the records' worker; joined and
     // handed back on every exit (before M goes)
     struct Held {
@@ -398,7 +433,7 @@ esch_status map_reads_core(esch_scorer *s, size_t n, const Getter &get, int both
                 if (x) give_batch(s, x);
         }
     } held{s};
-    auto range = [&](size_t ci) { return std::make_pair(ci * chunk, std::min(n, ci * chunk + chunk)); };
+    auto range = [&](size_t ci) { return std::make_pair(cuts[ci], cuts[ci + 1]); };
     if (n_chunks) {
         if (esch_status st = load_chunk(s, get, 0, range(0).second, both, 0, &held.b[0], M->stats)) return st;
         if (esch_status st = score_chunk(held.b[0])) return st;
@@ -410,15 +445,15 @@ esch_status map_reads_core(esch_scorer *s, size_t n, const Getter &get, int both
         const size_t k0 = range(ci).first, k1 = range(ci).second;
         esch_batch *b = held.b[0];
         if (esch_status st = finish_chunk(b, M->stats)) return st;
-        Ctx cx{s, b, *s->map, M->stats};
+        if (held.b[1])
+            if (esch_status st = score_chunk_sliced(held.b[1])) return st;
+        Ctx cx{s, b, *s->map, M->stats, held.b[1]};
         std::vector<int32_t> rows;  // non-empty reads; empty ones have best None
         for (size_t k = k0; k < k1; ++k)
             if (b->h_qlen[k - k0]) rows.push_back((int32_t)(k - k0));
         std::unique_ptr<Ctx::Selected> S(new Ctx::Selected);
         std::vector<int32_t> jrows, jrefs;
         if (esch_status st = cx.select(rows, both, min_score, max_ties, *S, jrows, jrefs)) return st;
-        if (held.b[1])
-            if (esch_status st = score_chunk(held.b[1])) return st;
         if (ci + 2 < n_chunks)
             if (esch_status st = load_chunk(s, get, range(ci + 2).first, range(ci + 2).second, both, 0, &held.b[2],
                                             M->stats))
@@ -537,17 +572,17 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
     // 944 k in eight, 959 k in sixteen.
     const size_t cap = std::max<size_t>(1, chunk_rows(s) / (half ? 2 : 4));
     const size_t chunk = s->map_chunk ? cap : std::min(cap, std::max<size_t>((size_t)1 << 16, (n_pairs + 15) / 16));
-    const size_t n_chunks = (n_pairs + chunk - 1) / chunk;
+    const std::vector<size_t> cuts = chunk_cuts(n_pairs, chunk, !s->map_chunk && n_pairs > chunk);
+    const size_t n_chunks = cuts.size() - 1;
     std::vector<uint8_t> &rc2 = s->map->rc2;  // grow-only across calls
     std::vector<size_t> &rc2_at = s->map->rc2_at;
     // Stage chunk q into a batch: mate 2's reverse complements (half rows),
     // the gather into pinned staging and the upload. Its scoring is launched
-    // separately, so the next chunk is staged while this one scores, and a
-    // chunk's records are assembled while the next one scores. (The tier
-    // loop's own kernels run between the two scorings: queued beside the
-    // persistent scoring kernel they would wait for it.)
+    // separately, so the next chunk is staged while this one scores; once a
+    // chunk is scored, the next one scores in slices, the chunk's tier loop
+    // runs between them and its records are assembled beside the last ones.
     auto stage = [&](size_t q, esch_batch **out) -> esch_status {
-        const size_t p0 = q * chunk, p1 = std::min(n_pairs, p0 + chunk), c = p1 - p0;
+        const size_t p0 = cuts[q], p1 = cuts[q + 1], c = p1 - p0;
         if (!half) return load_chunk(s, mate, 2 * p0, 2 * p1, 1, 1, out, M->stats);
         const auto t = Clock::now();
         size_t bad = 0;
@@ -578,11 +613,14 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
         if (esch_status st = score_chunk(held.cur)) return st;
     }
     for (size_t ci = 0; ci < n_chunks; ++ci) {
-        const size_t p0 = ci * chunk, p1 = std::min(n_pairs, p0 + chunk), c = p1 - p0;
+        const size_t p0 = cuts[ci], p1 = cuts[ci + 1], c = p1 - p0;
         if (ci + 1 < n_chunks)
             if (esch_status st = stage(ci + 1, &held.next)) return st;
         esch_batch *b = held.cur;
         if (esch_status st = finish_chunk(b, M->stats)) return st;
+        // the next chunk scores in slices, this chunk's tier loop between them
+        if (held.next)
+            if (esch_status e = score_chunk_sliced(held.next)) return e;
         struct Guard {
             esch_scorer *s;
             esch_batch *b;
@@ -590,7 +628,7 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
                 if (b) give_batch(s, b);
             }
         } guard_sep{s, nullptr};
-        Ctx cx{s, b, *s->map, M->stats};
+        Ctx cx{s, b, *s->map, M->stats, held.next};
         MapCache &C = *s->map;
         hipStream_t st = b->stream;
         const int32_t nrd = (int32_t)b->n_reads;  // 2c
@@ -624,6 +662,7 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
                        C.cands.as<int32_t>()};
             HIPC(launch_pair_tier(a, kResolveBlocks, st));
             std::vector<int32_t> recs, cands;
+            if (esch_status e = cx.pump(st)) return e;
             if (esch_status e = download(recs, C.recs, n * kTierRec, st)) return e;
             if (esch_status e = download(cands, C.cands, n * kTierCap, st)) return e;
             // tiers wider than the inline capacity: those pairs again
@@ -653,6 +692,7 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
                 w.recs = C.set.as<int32_t>();
                 w.cands = C.out.as<int32_t>();
                 HIPC(launch_pair_tier(w, kResolveBlocks, st));
+                if (esch_status e = cx.pump(st)) return e;
                 if (esch_status e = download(wide, C.out, (size_t)total, st)) return e;
             }
             M->stats.select_ms += ms_since(ts);
@@ -746,6 +786,7 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
                          C.out.as<int32_t>()};
             HIPC(launch_pair_subopt(a, kResolveBlocks, st));
             std::vector<int32_t> so;
+            if (esch_status e = cx.pump(st)) return e;
             if (esch_status e = download(so, C.out, pl.size() * kSuboptRec, st)) return e;
             M->stats.select_ms += ms_since(ts);
             for (size_t q = 0; q < pl.size(); ++q) {
@@ -769,7 +810,7 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
             esch_batch *bs = nullptr;
             if (esch_status e = load_and_score(s, sm, 0, mates_of.size(), 1, 0, &bs, M->stats)) return e;
             guard_sep.b = bs;
-            Ctx cs{s, bs, *s->map, M->stats};
+            Ctx cs{s, bs, *s->map, M->stats, held.next};
             std::vector<int32_t> rows(mates_of.size());
             for (size_t q = 0; q < rows.size(); ++q) rows[q] = (int32_t)q;
             if (esch_status e = cs.map_rows(rows, 1, min_score, max_ties, mm, pool)) return e;
@@ -780,9 +821,9 @@ esch_status map_pairs_core(esch_scorer *s, size_t n_pairs, const Getter &mate, i
             const int32_t x = mates_of[e];
             res[(size_t)(x / 2)].separate[x % 2] = mm[e];
         }
-        // this chunk's device work is done: the next chunk scores meanwhile
-        if (held.next)
-            if (esch_status e = score_chunk(held.next)) return e;
+        // this chunk's device work is done: the next chunk's slices still
+        // out run beside the assembly
+        if (esch_status e = batch_drain(held.next)) return e;
         // into the result, in pair order
         const auto ta = Clock::now();
         reserve_more(M->hits, pool.hits.size());

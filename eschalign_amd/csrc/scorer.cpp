@@ -517,6 +517,9 @@ void esch_batch_free(esch_batch *b) {
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->ev2) (void)hipEventDestroy(b->ev2);
     if (b->ev3) (void)hipEventDestroy(b->ev3);
+    for (hipEvent_t e : b->sev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->pump_ev) (void)hipEventDestroy(b->pump_ev);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     if (b->swap) {
         swap_release(b->swap);
@@ -543,6 +546,7 @@ esch_status batch_stage(esch_batch *b, size_t n, int both, int pairs,
     if (n > b->max_reads || total > b->max_bases)
         return fail(ESCH_CAPACITY, "batch holds %zu reads / %zu bases; got %zu / %zu", b->max_reads,
                     b->max_bases, n, total);
+    b->n_slices = b->slices_out = 0;        // a sliced run's slices still out: the new reads replace it
     HIPC(hipStreamSynchronize(b->stream));  // staging buffers may still feed a copy
     // offsets first, then the gather into pinned staging on up to 8 host
     // threads (1M short reads are a million small copies), then one
@@ -641,7 +645,40 @@ KArgs make_args(esch_batch *b, int what) {
     return a;
 }
 
-esch_status enqueue(esch_batch *b, int what, bool timed, const Launch &g) {
+// What follows k_dp16 in a run: the anti-diagonal and int32 kernels, then
+// the fused selection.
+esch_status enqueue_tail(esch_batch *b, int what, bool timed, const Launch &g) {
+    const KArgs a = make_args(b, what);
+    const bool local = b->s->sc.local != 0;
+    hipStream_t st = b->stream;
+    HIPC(launch_diag(a, g, local, st));
+    HIPC(launch_dp32(a, g, local, 1, st));
+    if (timed) HIPC(hipEventRecord(b->ev2, st));
+    if (what & ESCH_RUN_SELECT) HIPC(launch_select(a, g, st));
+    if (what & ESCH_RUN_PAIRS) HIPC(launch_pairs(a, g, st));
+    return ESCH_OK;
+}
+
+// Slice k of a sliced run, with its events; the last one brings the tail.
+esch_status launch_slice(esch_batch *b, int k) {
+    KArgs a = make_args(b, b->slice_what);
+    a.slice = k;
+    a.n_slices = b->n_slices;
+    hipStream_t st = b->stream;
+    HIPC(hipEventRecord(b->sev[2 * k], st));
+    HIPC(launch_dp16(a, b->slice_geo, b->s->sc.local != 0, st));
+    HIPC(hipEventRecord(b->sev[2 * k + 1], st));
+    b->slices_out = k + 1;
+    if (k + 1 == b->n_slices) {
+        if (esch_status e = enqueue_tail(b, b->slice_what, true, b->slice_geo)) return e;
+        HIPC(hipEventRecord(b->ev3, st));
+    }
+    return ESCH_OK;
+}
+
+// A run's launches: bucketing, then k_dp16 (a sliced run: its slice 0, the
+// rest from batch_pump) and the tail, or the plain kernel.
+esch_status enqueue(esch_batch *b, int what, bool timed, const Launch &g, int n_slices = 0) {
     esch_scorer *s = b->s;
     const KArgs a = make_args(b, what);
     const bool local = s->sc.local != 0;
@@ -651,15 +688,19 @@ esch_status enqueue(esch_batch *b, int what, bool timed, const Launch &g) {
     if (what & ESCH_RUN_PLAIN) {
         HIPC(launch_dp32(a, g, local, 0, st));
         HIPC(launch_dp32(a, g, local, 1, st));
-    } else {
-        HIPC(launch_dp16(a, g, local, st));
-        HIPC(launch_diag(a, g, local, st));
-        HIPC(launch_dp32(a, g, local, 1, st));
+        if (timed) HIPC(hipEventRecord(b->ev2, st));
+        if (what & ESCH_RUN_SELECT) HIPC(launch_select(a, g, st));
+        if (what & ESCH_RUN_PAIRS) HIPC(launch_pairs(a, g, st));
+        return ESCH_OK;
     }
-    if (timed) HIPC(hipEventRecord(b->ev2, st));
-    if (what & ESCH_RUN_SELECT) HIPC(launch_select(a, g, st));
-    if (what & ESCH_RUN_PAIRS) HIPC(launch_pairs(a, g, st));
-    return ESCH_OK;
+    if (n_slices > 1) {
+        b->n_slices = n_slices;
+        b->slice_geo = g;
+        b->slice_what = what;
+        return launch_slice(b, 0);
+    }
+    HIPC(launch_dp16(a, g, local, st));
+    return enqueue_tail(b, what, timed, g);
 }
 
 // The k_dp16 variant a batch runs, and the column block of its fp16
@@ -847,32 +888,10 @@ esch_status run_swapped(esch_batch *b, int what, const Launch &geo) {
     return ESCH_OK;
 }
 
-}  // namespace
-
-esch_status esch_batch_load(esch_batch *b, const uint8_t *const *reads, const size_t *lens, size_t n,
-                            int both, int pairs) {
-    if (!b || (n && (!reads || !lens))) return fail(ESCH_INVALID_ARG, "null argument");
-    HIPC(hipSetDevice(b->s->ordinal));
-    return batch_stage(b, n, both ? 1 : 0, pairs ? 1 : 0, [&](size_t k, const uint8_t *&p, size_t &len) {
-        p = reads[k];
-        len = lens[k];
-    });
-}
-
-esch_status esch_batch_load_concat(esch_batch *b, const uint8_t *codes, const uint64_t *offs,
-                                   const int32_t *lens, size_t n, int both, int pairs) {
-    if (!b || (n && (!codes || !offs || !lens))) return fail(ESCH_INVALID_ARG, "null argument");
-    HIPC(hipSetDevice(b->s->ordinal));
-    return batch_stage(b, n, both ? 1 : 0, pairs ? 1 : 0, [&](size_t k, const uint8_t *&p, size_t &len) {
-        p = codes + offs[k];
-        len = (size_t)(lens[k] < 0 ? 0 : lens[k]);
-    });
-}
-
-esch_status esch_batch_run(esch_batch *b, int what, int32_t min_score, int64_t max_ties, size_t tie_cap) {
-    if (!b) return fail(ESCH_INVALID_ARG, "null batch");
+esch_status run_impl(esch_batch *b, int what, int32_t min_score, int64_t max_ties, size_t tie_cap, int n_slices) {
     esch_scorer *s = b->s;
     HIPC(hipSetDevice(s->ordinal));
+    b->n_slices = b->slices_out = 0;  // an earlier sliced run's slices still out are dropped: this run replaces it
     if ((what & ESCH_RUN_PAIRS) && !b->pairs) return fail(ESCH_INVALID_ARG, "batch was not loaded as pairs");
     if ((what & (ESCH_RUN_SELECT | ESCH_RUN_PAIRS)) && (what & ESCH_RUN_SCORES16))
         return fail(ESCH_INVALID_ARG, "selection reads the int32 matrix");
@@ -932,6 +951,14 @@ esch_status esch_batch_run(esch_batch *b, int what, int32_t min_score, int64_t m
     if (b->last_swapped) {
         b->timed_dp = true;
         if (esch_status st = run_swapped(b, what, geo_scorer)) return st;
+    } else if (n_slices > 1 && !(what & ESCH_RUN_PLAIN)) {
+        b->timed_dp = true;
+        n_slices = std::min(n_slices, esch::kMaxSlices);
+        for (int k = 0; k < 2 * n_slices; ++k)
+            if (!b->sev[k]) HIPC(hipEventCreate(&b->sev[k]));
+        if (!b->pump_ev) HIPC(hipEventCreateWithFlags(&b->pump_ev, hipEventDisableTiming));
+        // ev3 comes with the last slice
+        return enqueue(b, key, true, geo, n_slices);
     } else if (!graph) {
         b->timed_dp = true;
         if (esch_status st = enqueue(b, key, true, geo)) return st;
@@ -957,20 +984,69 @@ esch_status esch_batch_run(esch_batch *b, int what, int32_t min_score, int64_t m
     return ESCH_OK;
 }
 
+}  // namespace
+
+esch_status esch_batch_load(esch_batch *b, const uint8_t *const *reads, const size_t *lens, size_t n,
+                            int both, int pairs) {
+    if (!b || (n && (!reads || !lens))) return fail(ESCH_INVALID_ARG, "null argument");
+    HIPC(hipSetDevice(b->s->ordinal));
+    return batch_stage(b, n, both ? 1 : 0, pairs ? 1 : 0, [&](size_t k, const uint8_t *&p, size_t &len) {
+        p = reads[k];
+        len = lens[k];
+    });
+}
+
+esch_status esch_batch_load_concat(esch_batch *b, const uint8_t *codes, const uint64_t *offs,
+                                   const int32_t *lens, size_t n, int both, int pairs) {
+    if (!b || (n && (!codes || !offs || !lens))) return fail(ESCH_INVALID_ARG, "null argument");
+    HIPC(hipSetDevice(b->s->ordinal));
+    return batch_stage(b, n, both ? 1 : 0, pairs ? 1 : 0, [&](size_t k, const uint8_t *&p, size_t &len) {
+        p = codes + offs[k];
+        len = (size_t)(lens[k] < 0 ? 0 : lens[k]);
+    });
+}
+
+esch_status esch_batch_run(esch_batch *b, int what, int32_t min_score, int64_t max_ties, size_t tie_cap) {
+    if (!b) return fail(ESCH_INVALID_ARG, "null batch");
+    return run_impl(b, what, min_score, max_ties, tie_cap, 0);
+}
+
 esch_status esch_batch_wait(esch_batch *b) {
     if (!b) return fail(ESCH_INVALID_ARG, "null batch");
     HIPC(hipSetDevice(b->s->ordinal));
+    if (esch_status e = esch::batch_drain(b)) return e;
     HIPC(hipStreamSynchronize(b->stream));
     return ESCH_OK;
 }
 
 int esch_batch_ready(esch_batch *b) {
     if (!b) return 0;
+    if (esch::batch_drain(b)) return 0;
     return hipStreamQuery(b->stream) == hipSuccess ? 1 : 0;
 }
 
 esch_status esch_batch_timing(const esch_batch *b, float *dp_ms, float *total_ms) {
     if (!b) return fail(ESCH_INVALID_ARG, "null batch");
+    if (b->slices_out < b->n_slices) return fail(ESCH_INVALID_ARG, "the run has slices still to launch");
+    if (b->n_slices > 1) {
+        // the slices alone (other streams' work ran between them), then the
+        // tail; the bucketing before and the selection after for the total
+        float dp = 0.0f, x = 0.0f;
+        for (int k = 0; k < b->n_slices; ++k) {
+            HIPC(hipEventElapsedTime(&x, b->sev[2 * k], b->sev[2 * k + 1]));
+            dp += x;
+        }
+        HIPC(hipEventElapsedTime(&x, b->sev[2 * b->n_slices - 1], b->ev2));
+        dp += x;
+        if (dp_ms) *dp_ms = dp;
+        if (total_ms) {
+            float pre = 0.0f, post = 0.0f;
+            HIPC(hipEventElapsedTime(&pre, b->ev0, b->ev1));
+            HIPC(hipEventElapsedTime(&post, b->ev2, b->ev3));
+            *total_ms = pre + dp + post;
+        }
+        return ESCH_OK;
+    }
     if (dp_ms) {
         *dp_ms = -1.0f;
         if (b->timed_dp) HIPC(hipEventElapsedTime(dp_ms, b->ev1, b->ev2));
@@ -1006,6 +1082,7 @@ esch_status esch_batch_read_scores32(esch_batch *b, int32_t *scores, uint8_t *sc
     if (!b) return fail(ESCH_INVALID_ARG, "null batch");
     if (b->last_what & ESCH_RUN_SCORES16) return fail(ESCH_INVALID_ARG, "last run wrote the int16 matrix");
     HIPC(hipSetDevice(b->s->ordinal));
+    if (esch_status e = esch::batch_drain(b)) return e;
     HIPC(hipStreamSynchronize(b->stream));
     const size_t rows = b->n_rows, nr = b->s->n_refs;
     if (scores && rows) HIPC(hipMemcpy(scores, b->d_out32, rows * nr * 4, hipMemcpyDeviceToHost));
@@ -1020,6 +1097,7 @@ esch_status esch_batch_read_scores16(esch_batch *b, int16_t *scores, uint8_t *sc
     if (!b) return fail(ESCH_INVALID_ARG, "null batch");
     if (!(b->last_what & ESCH_RUN_SCORES16)) return fail(ESCH_INVALID_ARG, "last run wrote the int32 matrix");
     HIPC(hipSetDevice(b->s->ordinal));
+    if (esch_status e = esch::batch_drain(b)) return e;
     HIPC(hipStreamSynchronize(b->stream));
     const size_t rows = b->n_rows, nr = b->s->n_refs;
     if (scores && rows) HIPC(hipMemcpy(scores, b->d_out16, rows * nr * 2, hipMemcpyDeviceToHost));
@@ -1135,6 +1213,29 @@ void give_batch(esch_scorer *s, esch_batch *b) {
         s->pool.erase(s->pool.begin());
         esch_batch_free(old);
     }
+}
+
+esch_status batch_run_sliced(esch_batch *b, int what, int n_slices) {
+    if (!b) return fail(ESCH_INVALID_ARG, "null batch");
+    if (what & (ESCH_RUN_SELECT | ESCH_RUN_PAIRS | ESCH_RUN_PLAIN))
+        return fail(ESCH_INVALID_ARG, "a sliced run scores only");
+    return run_impl(b, what | ESCH_RUN_NO_GRAPH, 0, -1, 1, n_slices);
+}
+
+esch_status batch_pump(esch_batch *b, hipStream_t after) {
+    if (!b || b->slices_out >= b->n_slices) return ESCH_OK;
+    HIPC(hipSetDevice(b->s->ordinal));
+    if (after) {
+        HIPC(hipEventRecord(b->pump_ev, after));
+        HIPC(hipStreamWaitEvent(b->stream, b->pump_ev, 0));
+    }
+    return launch_slice(b, b->slices_out);
+}
+
+esch_status batch_drain(esch_batch *b) {
+    while (b && b->slices_out < b->n_slices)
+        if (esch_status e = launch_slice(b, b->slices_out)) return e;
+    return ESCH_OK;
 }
 
 }  // namespace esch
@@ -1563,7 +1664,8 @@ void trace_cache_free(esch_scorer *s) {
 esch_status esch::align_core(esch_scorer *s, size_t n,
                              const std::function<void(size_t, const uint8_t *&, size_t &)> &get,
                              const int32_t *refs, esch_alignment *out, uint32_t *ops, size_t ops_cap,
-                             size_t *ops_used, const uint8_t *dev_codes, std::vector<uint32_t> *ops_vec) {
+                             size_t *ops_used, const uint8_t *dev_codes, std::vector<uint32_t> *ops_vec,
+                             esch_batch *pump) {
     if (ops_used) *ops_used = 0;
     std::unique_ptr<size_t[]> lens(new size_t[std::max<size_t>(n, 1)]);
     std::unique_ptr<const uint8_t *[]> ptr(new const uint8_t *[std::max<size_t>(n, 1)]);
@@ -1899,6 +2001,7 @@ esch_status esch::align_core(esch_scorer *s, size_t n,
     HIPC(hipEventRecord(T.e1, st));
     int32_t *const recs = T.h_recs.as<int32_t>();
     HIPC(hipMemcpyAsync(recs, T.recs.p, n * kTraceRec * 4, hipMemcpyDeviceToHost, st));
+    if (esch_status e = esch::batch_pump(pump, st)) return e;
     HIPC(hipStreamSynchronize(st));
     HIPC(hipEventElapsedTime(&s->trace_ms, T.e0, T.e1));
     if (T.arena.cap > kTraceArenaKeep) T.arena.release();
@@ -1923,6 +2026,7 @@ esch_status esch::align_core(esch_scorer *s, size_t n,
         HIPC(launch_trace_compact(T.ops_wc.as<uint32_t>(), T.ops_off.as<uint64_t>(), T.recs.as<int32_t>(),
                                   T.out_off.as<uint64_t>(), T.out.as<uint32_t>(), (int32_t)n, st));
         HIPC(hipMemcpyAsync(T.h_ops.p, T.out.p, runs * 4, hipMemcpyDeviceToHost, st));
+        if (esch_status e = esch::batch_pump(pump, st)) return e;
         HIPC(hipStreamSynchronize(st));
         // pinned staging to the caller's memory
         const uint32_t *h = T.h_ops.as<uint32_t>();

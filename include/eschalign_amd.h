@@ -378,9 +378,11 @@ const esch_pair_mapping *esch_mapped_pairs(const esch_mapped *m); /* NULL for re
 const esch_hit *esch_mapped_hits(const esch_mapped *m, size_t *n);
 const uint32_t *esch_mapped_ops(const esch_mapped *m, size_t *n);
 /* Where the time of the call went (diagnostics; no counterpart in the
- * reference), in ms: */
+ * reference), in ms. A chunk's selection and tracebacks run between two
+ * launches of the next chunk's scoring, so select_ms and trace_ms include
+ * the wait for the launch they run behind: */
 typedef struct {
-    double score_ms;      /* device scoring pipelines (HIP events) */
+    double score_ms;      /* device scoring pipelines (HIP events; their launches alone) */
     double select_ms;     /* tier / suboptimal / selection kernels with their copies */
     double trace_ms;      /* device traceback kernels (HIP events) */
     double trace_wall_ms; /* the traceback calls, host staging and copies included */

@@ -274,3 +274,39 @@ def test_map_reads_longer_than_the_drop_in_bound():
         for k, (g, w) in enumerate(zip(got, want)):
             assert g == w, (k, len(reads[k]))
         assert all(g[3] for g in got[:16])  # every long read maps
+
+
+def test_default_chunking_equals_one_chunk():
+    """Calls over more than the default chunk (2^17 reads, 2^16 pairs): each
+    chunk after the first scores in slices with the previous chunk's
+    selection, rounds and tracebacks between them, and the last chunk's
+    worth is cut in halves. Every record, hit and run equals the same call
+    in one chunk (scored whole, unsliced), and the last reads' records equal
+    the oracle's."""
+    panel_l = synth.panel(seed=0x0123456789ABCDEF, n_refs=164, lo=140, hi=160)
+    codes, off, lens = synth.read_arrays(panel_l, seed=0xC0F6, n=300_000)
+    scoring = (1, -1, -2, -1)
+    gpu = ea.GpuScorer(ea.Panel(panel_l), ea.Scoring(*scoring), ea.Mode.SEMIGLOBAL)
+    cut = gpu.map_reads_concat(codes, off, lens, both_strands=True)[:3]
+    gpu.set_map_chunk(len(lens))
+    whole = gpu.map_reads_concat(codes, off, lens, both_strands=True)[:3]
+    gpu.set_map_chunk(0)
+    for a, b in zip(cut, whole):
+        assert len(a) == len(b) and np.array_equal(a, b)
+    tail = range(len(lens) - 400, len(lens))
+    reads = [codes[int(off[k]):int(off[k]) + int(lens[k])] for k in tail]
+    want = O.map_reads(reads, panel_l, scoring, False, both_strands=True)
+    recs, hits, ops = cut
+    for k, w in zip(tail, want):
+        assert O.read_form(recs[k], hits, ops) == w, k
+
+    ppanel = synth.panel(seed=0xC0F4, n_refs=500, lo=70, hi=95)
+    pc, po, pl = synth.pair_arrays(ppanel, seed=0xC0F7, n_pairs=100_000)
+    pg = ea.GpuScorer(ea.Panel(ppanel), ea.Scoring(*DEFAULT), ea.Mode.SEMIGLOBAL)
+    for kw in (dict(), dict(both_strands=True)):
+        cut = pg.map_pairs_concat(pc, po, pl, **kw)[:3]
+        pg.set_map_chunk(2 * len(pl))
+        whole = pg.map_pairs_concat(pc, po, pl, **kw)[:3]
+        pg.set_map_chunk(0)
+        for a, b in zip(cut, whole):
+            assert len(a) == len(b) and np.array_equal(a, b), kw
