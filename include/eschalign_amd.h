@@ -585,12 +585,16 @@ esch_status esch_batch_read_select(esch_batch *b, esch_read_select *out, int32_t
 int esch_batch_swapped(const esch_batch *b);
 /* The k_dp16 variant the batch's last run used (-1 before any run, or when
  * it ran with the roles swapped). It is the scorer's variant, except when
- * neither the caller nor the autotuner fixed one (esch_set_variant), the
- * panel admits no fp16 variant, and the batch's reads do: every value is
- * bounded through min(read, reference), so a batch whose longest read n
- * gives min(n, longest reference) x max(|match|, |mismatch|) + |open| +
- * 17 |extend| <= 2048 runs variant 7 in semiglobal mode (4 in local mode,
- * from the same sum with |extend| once). No counterpart in the reference. */
+ * the caller fixed none (esch_set_variant; the autotuner's pick does not
+ * count), the scorer's variant computes in int16, and the batch's reads
+ * admit fp16: every value is bounded through min(read, reference), so a
+ * batch whose longest read n keeps min(n, longest reference) x
+ * max(|match|, |mismatch|) + |open| + |extend| within 2048 upgrades to
+ * an fp16 form (semiglobal: the diagonal shift for a diagonal-shift pick,
+ * else the row shift; local: the column shift), its column shift restarted
+ * as often as the batch needs (esch_column_block); where the shifted form
+ * does not fit, to the unshifted fp16 schedule (a diagonal-shift pick
+ * stays). No counterpart in the reference. */
 int esch_batch_variant(const esch_batch *b);
 esch_status esch_batch_read_pairs(esch_batch *b, esch_pair_select *out, int32_t *top);
 
