@@ -448,9 +448,8 @@ esch_status esch_bam_build_records(const uint8_t *in, size_t in_bytes, size_t n,
  *   2  every memory access of a column pair issued at its start, stores
  *      deferred by one pair, no data-dependent branch around a memory
  *      access;
- *   3  (default where no other applies) columns in quads: two pairs whose
- *      boundary registers swap roles;
- *   4  (default where 7 does not apply and 2048 holds) variant 3 in packed
+ *   3  columns in quads: two pairs whose boundary registers swap roles;
+ *   4  variant 3 in packed
  *      fp16 arithmetic, whose three-way maximum is one instruction: 7
  *      packed ops per cell pair in semiglobal mode instead of 8. Exact only
  *      while every value is an integer fp16 holds exactly, so only for
@@ -459,14 +458,13 @@ esch_status esch_bam_build_records(const uint8_t *in, size_t in_bytes, size_t n,
  *      elsewhere;
  *   5  the fp16 arithmetic of variant 4 over eight columns per iteration,
  *      the stripe boundary loaded four columns ahead; the same condition;
- *   6  (default where 7 does not apply and 4 neither) variant 3 with
- *      shifted values, which take one gap chain's extension add out:
+ *   6  variant 3 with shifted values, which take one gap chain's extension add out:
  *      semiglobal mode shifts every value of stripe row r by |e| r (7 packed
  *      ops instead of 8; needs longest reference x M + |open| + 17 |extend|
  *      <= 32000, M = max(|match|, |mismatch|)); local mode shifts reference
  *      column j by |e| j (9 instead of 10; needs longest reference x (M +
  *      |extend|) + |open| + 3 |extend| + M <= 32000);
- *   7  (default where it applies) variant 5 with those shifts: 6 packed ops
+ *   7  variant 5 with those shifts: 6 packed ops
  *      instead of 7 in semiglobal mode, 6.5 instead of 8.5 in local mode
  *      (where the zero floor rides on the shifted chain's maximum);
  *      the same sums at most 2048;
@@ -497,7 +495,9 @@ esch_status esch_bam_build_records(const uint8_t *in, size_t in_bytes, size_t n,
  *   the longest reference is too long for those sums: their column shift
  *   then restarts every K columns (esch_column_block), and K |extend|
  *   takes the place of longest reference x |extend| in the condition, K at
- *   least 8. */
+ *   least 8.
+ * A new scorer runs the first of 24, 28, 7, 4, 6 and 3 its mode and panel
+ * admit (local mode: 7, 4, 6, 3). */
 int esch_n_variants(void);
 /* A variant's shape: its schedule (variants 0-8 are schedules 0-8 on 16
  * rows; 9, 10 and 11 are the windowed 8, 7 and 6; 12, 13 and 14 those with
@@ -523,7 +523,8 @@ int esch_get_variant(const esch_scorer *s);
 esch_status esch_set_variant(esch_scorer *s, int variant);
 /* ms[v] (may be NULL, esch_n_variants() entries) receives each variant's DP
  * time on the sample (-1 for a variant the panel does not admit); the
- * fastest is selected and returned in *chosen.
+ * fastest is selected, unless the scorer's current variant is within 1% of
+ * it (which then stays), and returned in *chosen.
  * Then, unless esch_set_diag_ratio or esch_set_diag_plan fixed it, the
  * routing ratio is swept over {off, 1.5, 2, 3, 4} on the same sample
  * (another one replaces the default 2.0 only when more than 2% faster), and
