@@ -226,8 +226,10 @@ def pairs_leg(n_pairs, cpu, both_strands=False):
     (0.5% substitutions, 0.05% indels); each mate less its adapter
     (esch_adapter_trim_pairs); semi-global 2,-1,-10,-1, --max-insert 500.
     One esch_map_pairs_concat call end to end from host buffers: staging and
-    upload, both orientations of every mate scored, the tier loop, the
-    tracebacks, the per-mate suboptimal scores, the Separate fallback and the
+    upload, the orientations the tiers read scored (forward template strand:
+    mate 1 as sequenced and mate 2 reverse-complemented; both strands: all
+    four), the tier loop, the tracebacks, the per-mate suboptimal scores, the
+    Separate fallback (its mates' other orientations scored then) and the
     records back. On a bounded sample, the oracle on the host cores (panel
     rows by the SIMD restatement of the reference's CPU kernel, then the
     restated map_pairs_scored with its scalar tracebacks), equal on every
@@ -905,6 +907,11 @@ def main():
     if not args.no_pairs:
         pairs = pairs_leg(args.pairs, not args.no_cpu, args.pairs_both)
         log(f"[bench] paired mapping: {pairs}")
+        if not args.pairs_both:
+            # both template strands: every orientation of every mate scored
+            # (the reference's GPU thread scores all four for a paired run)
+            pairs["both_strands"] = pairs_leg(args.pairs, False, True)
+            log(f"[bench] paired mapping, both strands: {pairs['both_strands']}")
 
     serve = None
     if not args.no_serving:
