@@ -1024,6 +1024,7 @@ __global__ __launch_bounds__(64 * kWaves16) void k_dp16(KArgs a) {
         hi = (uint32_t)((uint64_t)n_items * (uint32_t)(a.slice + 1) / (uint32_t)a.n_slices);
         cursor = &a.desc->next_slice[a.slice];
     }
+    if (lo >= hi) return;  // no work items (uniform: before any barrier)
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) s_item = lo + atomicAdd(cursor, 1u);

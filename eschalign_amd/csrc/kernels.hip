@@ -55,6 +55,7 @@ __global__ __launch_bounds__(64) void k_diag16(KArgs a) {
     const int n_rg = (n_refs + kDiagRefs - 1) / kDiagRefs;
     const int nd = uni(a.desc->nd);
     const uint32_t n_items = (uint32_t)nd * (uint32_t)n_rg;
+    if (n_items == 0) return;  // nothing routed here (uniform: before any barrier)
     uint2 *scr = a.scratchD + ((size_t)blockIdx.x * 4 + grp) * (size_t)P.max_len;
     const v2s vo = splat(a.sc.o), ve = splat(a.sc.e), NEG = splat(-32768);
     const int nval_real = a.sc.ma - a.sc.o, nval_pre = -a.sc.o;
@@ -382,6 +383,7 @@ __global__ __launch_bounds__(64) void k_diag64(KArgs a) {
     const int n_pairs = (n_refs + 1) / 2;
     const int nd = uni(a.desc->nd);
     const uint32_t n_items = (uint32_t)nd * (uint32_t)n_pairs;
+    if (n_items == 0) return;  // nothing routed here (uniform: before any barrier)
     for (;;) {
         // the item index goes through LDS between two barriers (as in
         // k_dp16): every lane reads it after lane 0 fetched it, whatever
@@ -425,6 +427,7 @@ __global__ __launch_bounds__(256) void k_dp32(KArgs a, int which) {
     const int *list = which == 0 ? a.sorted16 : a.list32;
     uint32_t *cursor = which == 0 ? &a.desc->next32a : &a.desc->next32b;
     const uint32_t n_items = (uint32_t)count * (uint32_t)groups;
+    if (n_items == 0) return;  // an empty list (uniform: before any barrier)
     const size_t total = (size_t)gridDim.x * G;
     const size_t gtid = (size_t)blockIdx.x * G + threadIdx.x;
     int2 *scr = a.scratch32 + gtid;
