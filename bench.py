@@ -795,6 +795,30 @@ def main():
         xr, xt = batch.select_raw()
     xfer = (time.perf_counter() - t) / n_xfer
     assert np.array_equal(xr, timed_recs) and np.array_equal(xt, timed_ties)
+    # 4. the same with two batches, as a caller keeps one batch scoring
+    # while it packs the next: step k + 1 gathered and uploaded while step k
+    # scores, launched as step k ends (queued earlier, its persistent DP
+    # kernel would hold the CUs that step k's selection waits for), and step
+    # k's records copied back while step k + 1 scores
+    other = gpu.batch(args.reads, int(lens.sum()))
+    other.load_concat(codes, off, lens, both_strands=True)
+    other.run(graph, tie_cap=tie_cap)  # its own graph captured
+    other.wait()
+    two = (batch, other)
+    n_pipe = 6  # the first load and the last copy back amortised over six steps
+    t = time.perf_counter()
+    two[0].load_concat(codes, off, lens, both_strands=True)
+    two[0].run(graph, tie_cap=tie_cap)
+    for k in range(1, n_pipe + 1):
+        if k < n_pipe:
+            two[k % 2].load_concat(codes, off, lens, both_strands=True)
+        two[(k - 1) % 2].wait()
+        if k < n_pipe:
+            two[k % 2].run(graph, tie_cap=tie_cap)
+        pr, pt = two[(k - 1) % 2].select_raw()
+        assert np.array_equal(pr, timed_recs) and np.array_equal(pt, timed_ties)
+    pipe = (time.perf_counter() - t) / n_pipe
+    del other, two, pr, pt
     ms_step = wall / args.steps * 1e3
     value = args.reads / (wall / args.steps)
     dp = float(np.mean(dp_ms)) / 1e3
@@ -972,7 +996,12 @@ def main():
                                "h2d_bytes": int(lens.sum()) + 12 * args.reads,
                                "d2h_bytes": int(xr.nbytes + xt.nbytes), "load_ms": round(load_s * 1e3, 3),
                                "scope": "reads from host buffers (gather into pinned staging, copy), the "
-                                        "graph, selection records and tie lists copied back"},
+                                        "graph, selection records and tie lists copied back",
+                               "two_batches_reads_per_s": round(args.reads / pipe, 1),
+                               "two_batches_ms_per_step": round(pipe * 1e3, 3),
+                               "two_batches_scope": "the same steps with two batches alternating: step k + 1 "
+                                                    "loaded while step k scores, launched as it ends, step k's "
+                                                    "records copied back while step k + 1 scores"},
         "kernel_variant": chosen,
         "stripe_rows": vinfo(chosen).stripe_rows,
         "int16_variant": int16,
