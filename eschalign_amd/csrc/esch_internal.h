@@ -272,6 +272,9 @@ constexpr int dp16_variant(int schedule, int stripe) {
     return -1;
 }
 hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st);
+// k_revcomp alone: the reverse-complement rows of a both-strand batch
+// written into its codes (launch_prep does it for the ordinary path).
+hipError_t launch_revcomp(const KArgs &a, const Launch &g, hipStream_t st);
 hipError_t launch_dp16(const KArgs &a, const Launch &g, bool local, hipStream_t st);
 hipError_t launch_dp32(const KArgs &a, const Launch &g, bool local, int which, hipStream_t st);
 hipError_t launch_select(const KArgs &a, const Launch &g, hipStream_t st);

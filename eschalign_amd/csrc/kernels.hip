@@ -1054,6 +1054,11 @@ hipError_t launch_prep(const KArgs &a, const Launch &g, hipStream_t st) {
     return hipGetLastError();
 }
 
+hipError_t launch_revcomp(const KArgs &a, const Launch &g, hipStream_t st) {
+    hipLaunchKernelGGL(k_revcomp, dim3(2 * g.grid_aux), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_dp16(const KArgs &a0, const Launch &g, bool local, hipStream_t st) {
     // the column-shift block of the launched variant's arithmetic
     KArgs a = a0;

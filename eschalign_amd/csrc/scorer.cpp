@@ -859,6 +859,10 @@ esch_status run_swapped(esch_batch *b, int what, const Launch &geo) {
     HIPC(hipMemcpyAsync(d + w.off_desc, h + w.off_desc, sizeof(DevDesc), hipMemcpyHostToDevice, st));
 
     const KArgs a0 = make_args(b, what);
+    // the batch's own reverse-complement rows, as the ordinary path writes
+    // them: whatever reads the batch's codes afterwards (a mapping call's
+    // tracebacks) finds every row there
+    if (b->both) HIPC(launch_revcomp(a0, geo, st));
     KArgs a = a0;
     a.panel = DevPanel{reinterpret_cast<uint32_t *>(d + w.off_w2), reinterpret_cast<uint32_t *>(d + w.off_wn),
                        reinterpret_cast<uint32_t *>(d), reinterpret_cast<uint32_t *>(d + w.off_nm),

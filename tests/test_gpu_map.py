@@ -310,3 +310,38 @@ def test_default_chunking_equals_one_chunk():
         pg.set_map_chunk(0)
         for a, b in zip(cut, whole):
             assert len(a) == len(b) and np.array_equal(a, b), kw
+
+
+def test_map_reads_with_swapped_chunks():
+    """Chunks scored with the roles swapped (at most swap_rows rows each, so
+    every chunk takes that path; a swapped run is never sliced): every
+    record equals the oracle's and the same call unchunked and unswapped,
+    the reverse strand's alignments included (the tracebacks read the
+    reverse complements from the batch's codes); pairs on one and both
+    template strands equal the oracle the same way."""
+    rng = G.Rng(0x5EED_0C0A)
+    panel_l = G.random_panel(rng, 200, 60, 180)
+    L = max(map(len, panel_l))
+    # swapped, a read is a panel reference: none longer than the panel's longest
+    reads = [G.encode(r) for r in G.reads_for(rng, panel_l, 1000) if len(r) <= L]
+    panel = ea.Panel(panel_l)
+    scoring = (1, -1, -2, -1)
+    for mode in (ea.Mode.SEMIGLOBAL, ea.Mode.LOCAL):
+        gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
+        plain = dev_reads(gpu, reads, both_strands=True, max_ties=3)
+        gpu.swap_rows = 500
+        gpu.set_map_chunk(200)  # 400 rows a chunk on both strands
+        swapped = dev_reads(gpu, reads, both_strands=True, max_ties=3)
+        gpu.set_map_chunk(0)
+        gpu.swap_rows = 0
+        assert swapped == plain, mode
+        want = O.map_reads(reads, panel.codes, scoring, mode == ea.Mode.LOCAL, both_strands=True, max_ties=3)
+        for k, (g, w) in enumerate(zip(swapped, want)):
+            assert g == w, (mode, k)
+        pairs = [(reads[2 * k], reads[2 * k + 1]) for k in range(len(reads) // 2)]
+        gpu.swap_rows = 500
+        gpu.set_map_chunk(100)  # 200 rows a chunk: 100 pairs on one strand, 50 on both
+        for kw in (dict(), dict(both_strands=True)):
+            compare_pairs(gpu, panel.codes, pairs, scoring, mode == ea.Mode.LOCAL, **kw)
+        gpu.set_map_chunk(0)
+        gpu.swap_rows = 0
