@@ -261,19 +261,33 @@ def test_map_reads_longer_than_the_drop_in_bound():
     cli/align.rs:1751-1774 keeps them on the CPU) mapped whole on the
     device: scored by the int16 path at that length, their winners traced
     by the anti-diagonal kernel; both modes, both strands, equal to the
-    oracle."""
+    oracle. Under 40,-20,-60,-20 their scores pass the int16 bound and take
+    the int32 kernel. Cut into chunks of 8 reads, the later ones scored in
+    slices with those kernels after the last, the records are the same."""
     rng = G.Rng(0x5EED0207)
     panel_l = [G.random_seq(rng, 200 + rng.below(800), 0.01) for _ in range(8)]
     reads = [G.encode(G.chimeric_read(rng, panel_l, 4500 + rng.below(1500))) for _ in range(16)]
     reads += [G.encode(G.mutate(rng, panel_l[k % 8])) for k in range(40)]
     panel = ea.Panel(panel_l)
-    for scoring, mode in ((DEFAULT, ea.Mode.LOCAL), ((1, -1, -2, -1), ea.Mode.SEMIGLOBAL)):
+    for scoring, mode in ((DEFAULT, ea.Mode.LOCAL), ((1, -1, -2, -1), ea.Mode.SEMIGLOBAL),
+                          ((40, -20, -60, -20), ea.Mode.SEMIGLOBAL)):
         gpu = ea.GpuScorer(panel, ea.Scoring(*scoring), mode)
         got = dev_reads(gpu, reads, both_strands=True)
         want = O.map_reads(reads, panel.codes, scoring, mode == ea.Mode.LOCAL, both_strands=True)
         for k, (g, w) in enumerate(zip(got, want)):
             assert g == w, (k, len(reads[k]))
         assert all(g[3] for g in got[:16])  # every long read maps
+        if scoring[0] == 40:  # every long read over the int16 bound
+            b = gpu.batch(16, sum(len(r) for r in reads[:16]))
+            b.load(reads[:16], both_strands=True)
+            b.run(ea.RUN_SCORES32)
+            b.wait()
+            _, n16, n32 = b.stats()
+            del b
+            assert n32 == 32 and n16 == 0, (n16, n32)
+        gpu.set_map_chunk(8)
+        assert dev_reads(gpu, reads, both_strands=True) == got, scoring
+        gpu.set_map_chunk(0)
 
 
 def test_default_chunking_equals_one_chunk():
