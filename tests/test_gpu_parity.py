@@ -891,6 +891,25 @@ def test_column_restart_through_the_swapped_path():
         assert gpu.select_batch(queries, both_strands=True, max_ties=3) == sel, mode
 
 
+def test_drop_in_matrix_on_a_panel_that_restarts_its_shift():
+    """References up to 1,900 nt under 1,-1,-2,-1, where the default
+    variants restart their column shift (local mode every 128 columns,
+    semiglobal mode every 64): the drop-in int16 matrix and the int32
+    extension, every query and its reverse complement, both modes, equal
+    to the oracle; reads of 1-1,700 nt, then a batch of short reads only."""
+    rng = G.Rng(0x5EED_0B09)
+    panel_l = [G.random_seq(rng, n, 0.01) for n in (1900, 1700, 1500)] + G.random_panel(rng, 60, 40, 400)
+    reads = [panel_l[0][200:1400], panel_l[1][:900], G.random_seq(rng, 1700, 0.02)]
+    reads += G.reads_for(rng, panel_l, 40)
+    short = G.reads_for(rng, panel_l, 1)[:40]  # lengths 1-40
+    for mode in (ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL):
+        gpu = ea.GpuScorer(ea.Panel(panel_l), ea.Scoring(1, -1, -2, -1), mode)
+        assert gpu.column_block(gpu.variant) > 0, mode
+        del gpu
+        check(panel_l, reads, (1, -1, -2, -1), mode, "long-reference panel")
+        check(panel_l, short, (1, -1, -2, -1), mode, "short reads on a long-reference panel")
+
+
 def test_swapped_roles_by_the_rule_on_a_wide_panel():
     """By the default rule a panel of at least 4,096 references swaps roles
     for batches of up to one row per reference: 40 reads on both strands
