@@ -166,3 +166,53 @@ def test_config5_real_panels_and_swapped_roles():
             assert (r.best, r.n_tied, r.suboptimal, r.take) == (b, nt, so, take[:8]), (k, i)
         break
     server.close()
+
+
+def test_serving_local_mode_and_panels_that_restart_their_shift():
+    """Two resident panels of references up to 1,900 nt, one in local mode
+    and one in semiglobal mode, under 1,-1,-2,-1: their default variants
+    restart the column shift (local every 128 columns, semiglobal every
+    64), and a batch of short reads may take another block. Requests of
+    1-2,000 reads arriving while others run: every result equals the same
+    request alone, and a sample of each panel's equals the CPU oracle."""
+    scoring = ea.Scoring(1, -1, -2, -1)
+    rng = random.Random(4321)
+    base = synth.panel(seed=77, n_refs=60, lo=140, hi=1000)
+    long_refs = synth.panel(seed=78, n_refs=3, lo=1500, hi=1900)
+    panels = [long_refs + base, long_refs[::-1] + base[::-1]]
+    modes = [ea.Mode.LOCAL, ea.Mode.SEMIGLOBAL]
+    server = ea.Server(hbm_budget=16 << 30, max_batch_reads=4000, max_batch_bases=2_000_000, max_slots=3,
+                       max_batch_cells=2_000_000_000)
+    ids = [server.add_panel(ea.Panel(p), scoring, m) for p, m in zip(panels, modes)]
+    alone = [ea.GpuScorer(ea.Panel(p), scoring, m) for p, m in zip(panels, modes)]
+    for g in alone:
+        assert g.column_block(g.variant) > 0
+    requests = []
+    for r in range(24):
+        pid = r % 2
+        n = int(round(np.exp(rng.uniform(0, np.log(2000)))))
+        reads = synth.reads(panels[pid], seed=2000 + r, n=n, length=("lognormal", 90, 0.8, 20, 1500))
+        requests.append((pid, reads))
+    submitted = []
+    for pid, reads in requests:
+        time.sleep(rng.expovariate(1 / 0.002))
+        submitted.append(server.submit(ids[pid], reads, both_strands=True, tie_cap=8))
+    results = []
+    for rid in submitted:
+        assert server.wait(rid, timeout_ms=120_000)
+        results.append(server.result(rid)[0])
+    assert server.stats()["completed"] == len(requests)
+    server.close()
+    for (pid, reads), res in zip(requests, results):
+        assert res == alone[pid].select_batch(reads, both_strands=True, tie_cap=8)
+    for pid in (0, 1):
+        k0 = next(k for k, (p, reads) in enumerate(requests) if p == pid and len(reads) >= 30)
+        reads = requests[k0][1][:30]
+        rows = reads + [ea.reverse_complement_codes(q) for q in reads]
+        sm = O.score_matrix(rows, panels[pid], scoring.tuple(), modes[pid] == ea.Mode.LOCAL,
+                            min(16, os.cpu_count() or 1))
+        n = len(reads)
+        for k in range(n):
+            b, nt, so, take = O.select(sm[k], sm[n + k], 0, None)
+            r = results[k0][k]
+            assert (r.best, r.n_tied, r.suboptimal, r.take) == (b, nt, so, take[:8]), (pid, k)
