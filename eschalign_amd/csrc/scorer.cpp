@@ -224,8 +224,8 @@ esch_status esch_scorer_new(int ordinal, const uint8_t *const *refs, const size_
     int b32 = 0;
     int block32 = (int)std::min<size_t>(256, ((n_refs + 63) / 64) * 64);
     HIPC(dp32_occupancy(local, block32, &b32));
-    // default: the deep-prefetch variant (the fastest on the measured
-    // workloads); esch_autotune re-decides on the caller's reads
+    // the default variant (chosen below: the fewest ops per cell the panel
+    // admits); esch_autotune re-decides on the caller's reads
     s->f16_ok = fits_f16((int64_t)max_len, s->sc.ma, s->sc.mi, s->sc.o, s->sc.e);
     {
         // The shifted variants: rows shifted in semiglobal mode, columns in
