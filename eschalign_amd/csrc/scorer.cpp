@@ -1673,16 +1673,19 @@ esch_status esch::align_core(esch_scorer *s, size_t n,
     if (ops_used) *ops_used = 0;
     std::unique_ptr<size_t[]> lens(new size_t[std::max<size_t>(n, 1)]);
     std::unique_ptr<const uint8_t *[]> ptr(new const uint8_t *[std::max<size_t>(n, 1)]);
-    // The caller's pairs, checked on up to 8 host threads (a mapping call
-    // traces millions); an error names the first bad pair, raised here on
-    // the calling thread (the message is per thread).
+    // The caller's pairs, checked on up to 16 host threads (a mapping call
+    // traces millions; from 2^16 pairs on, the check and the staging below
+    // are threaded: 210k winner pairs traced from host buffers in 12.8-13.8
+    // ms against 14.7-15.4 on one thread and 8); an error names the first
+    // bad pair, raised here on the calling thread (the message is per
+    // thread).
     struct Part {
         size_t np = 0, bases = 0, max_len = 0, bad = SIZE_MAX;
         uint64_t W = 0, max_cells = 0;
         int why = 0;
     };
     const size_t hw = std::max(1u, std::thread::hardware_concurrency());
-    const size_t nt = n < ((size_t)1 << 18) ? 1 : std::min<size_t>(8, hw);
+    const size_t nt = n < ((size_t)1 << 16) ? 1 : std::min<size_t>(16, hw);
     std::vector<Part> part(nt);
     auto check = [&](size_t t) {
         Part &P = part[t];
@@ -1847,7 +1850,7 @@ esch_status esch::align_core(esch_scorer *s, size_t n,
                 hslot[i] = k;
             }
         };
-        const size_t ft = np < ((size_t)1 << 18) ? 1 : std::min<size_t>(8, hw);
+        const size_t ft = np < ((size_t)1 << 16) ? 1 : std::min<size_t>(16, hw);
         std::vector<std::thread> pool;
         for (size_t t = 1; t < ft; ++t) pool.emplace_back(fill, np * t / ft, np * (t + 1) / ft);
         fill(0, np / ft);
@@ -1860,10 +1863,10 @@ esch_status esch::align_core(esch_scorer *s, size_t n,
             }
         }
     }
-    // the gather into staging and the code check on up to 8 host threads
+    // the gather into staging and the code check on up to 16 host threads
     // (resident reads were checked when they were loaded)
     if (!dev_codes) {
-        const size_t gt = bases < ((size_t)8 << 20) ? 1 : std::min<size_t>(8, hw);
+        const size_t gt = bases < ((size_t)8 << 20) ? 1 : std::min<size_t>(16, hw);
         std::vector<uint8_t> bad(gt, 0);
         auto gather = [&](size_t t) {
             const size_t i0 = np * t / gt, i1 = np * (t + 1) / gt;
