@@ -191,10 +191,41 @@ def trace_leg(gpu, batch, panel_codes, codes, off, lens, scoring, local, n_reads
     gpu.align_pairs_concat(big, qoff, qlen, refs)  # warm: code objects, the call's buffers (steady state)
     t = time.perf_counter()
     al, _ = gpu.align_pairs_concat(big, qoff, qlen, refs)
-    wall = time.perf_counter() - t
+    py_wall = time.perf_counter() - t
     kms = gpu.align_last_ms()
+    # the C ABI call alone, host buffers in and the caller's records and runs
+    # out, those allocated once as a caller reusing them would (the Python
+    # mirror's call above adds its allocations and the conversion of the
+    # records to an int64 table)
+    import ctypes as C
+    import eschalign_amd as ea
+    lib = ea.load_library()
+    n = len(qlen)
+    cap = int(qlen.astype(np.int64).sum()) + int(rlen[refs].sum()) + 1
+    recs_buf = np.empty((n, 4), np.int64)
+    ops_buf = np.empty(cap, np.uint32)
+    used = C.c_size_t()
+    qoff_c, qlen_c, refs_c = (np.ascontiguousarray(qoff, np.uint64), np.ascontiguousarray(qlen, np.int32),
+                              np.ascontiguousarray(refs, np.int32))
+
+    def abi_call():
+        st = lib.esch_align_pairs_concat(gpu._h, big.ctypes.data, qoff_c.ctypes.data, qlen_c.ctypes.data,
+                                         refs_c.ctypes.data, n, recs_buf.ctypes.data, ops_buf.ctypes.data, cap,
+                                         C.byref(used))
+        assert st == 0, "esch_align_pairs_concat failed"
+    abi_call()  # warm: the outputs' pages
+    walls = []
+    for _ in range(3):
+        t = time.perf_counter()
+        abi_call()
+        walls.append(time.perf_counter() - t)
+    wall = float(np.median(walls))
+    assert np.array_equal(recs_buf.view(np.int32).reshape(-1, 8)[:n, :6], al[:, :6])
     out = {"reads": int(nr), "pairs": int(len(refs)), "cells": cells,
            "pairs_per_s": round(len(refs) / wall, 1), "wall_ms": round(wall * 1e3, 2),
+           "scope": "one esch_align_pairs_concat call from host buffers to the caller's records and "
+                    "runs (median of 3); python_* is the Python mirror's call with its own allocations",
+           "python_wall_ms": round(py_wall * 1e3, 2), "python_pairs_per_s": round(len(refs) / py_wall, 1),
            "kernel_ms": round(kms, 3), "kernel_gcups": round(cells / (kms / 1e3) / 1e9, 1),
            "wall_gcups": round(cells / wall / 1e9, 1)}
     if cpu:
