@@ -111,7 +111,12 @@ esch_status esch_scorer_new(int ordinal, const uint8_t *const *refs, const size_
                             esch_mode mode, esch_scorer **out) {
     if (!out) return fail(ESCH_INVALID_ARG, "out is null");
     *out = nullptr;
-    if (mode != ESCH_LOCAL && mode != ESCH_SEMIGLOBAL) return fail(ESCH_INVALID_ARG, "unknown mode %d", (int)mode);
+    // the mode's integer, read as stored: a caller across the C ABI may pass
+    // any int, and a value outside the enumeration is not one C++ may load
+    int mode_i = 0;
+    static_assert(sizeof mode_i == sizeof mode, "esch_mode is an int");
+    memcpy(&mode_i, &mode, sizeof mode_i);
+    if (mode_i != ESCH_LOCAL && mode_i != ESCH_SEMIGLOBAL) return fail(ESCH_INVALID_ARG, "unknown mode %d", mode_i);
     if (esch_status st = validate(ma, mi, o, e)) return st;
     if (n_refs == 0) return fail(ESCH_EMPTY_PANEL, "the reference panel is empty");
     if (!refs || !ref_lens) return fail(ESCH_INVALID_ARG, "refs or ref_lens is null");
