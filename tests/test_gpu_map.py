@@ -263,7 +263,8 @@ def test_map_reads_longer_than_the_drop_in_bound():
     by the anti-diagonal kernel; both modes, both strands, equal to the
     oracle. Under 40,-20,-60,-20 their scores pass the int16 bound and take
     the int32 kernel. Cut into chunks of 8 reads, the later ones scored in
-    slices with those kernels after the last, the records are the same."""
+    slices with those kernels after the last, and with every long read sent
+    to the anti-diagonal kernel by length, the records are the same."""
     rng = G.Rng(0x5EED0207)
     panel_l = [G.random_seq(rng, 200 + rng.below(800), 0.01) for _ in range(8)]
     reads = [G.encode(G.chimeric_read(rng, panel_l, 4500 + rng.below(1500))) for _ in range(16)]
@@ -288,6 +289,9 @@ def test_map_reads_longer_than_the_drop_in_bound():
         gpu.set_map_chunk(8)
         assert dev_reads(gpu, reads, both_strands=True) == got, scoring
         gpu.set_map_chunk(0)
+        gpu.diag_min_len = 1000  # every long read on the anti-diagonal kernel by length
+        assert dev_reads(gpu, reads, both_strands=True) == got, scoring
+        gpu.diag_min_len = 0
 
 
 def test_default_chunking_equals_one_chunk():
