@@ -339,16 +339,20 @@ esch_status load_chunk(esch_scorer *s, const Getter &get, size_t k0, size_t k1, 
 esch_status score_chunk(esch_batch *b) { return esch_batch_run(b, ESCH_RUN_SCORES32, 0, -1, 1); }
 
 // Chunk boundaries of a mapping call of n reads (or pairs) in chunks of
-// `chunk`: whole chunks and, with taper, the last chunk's worth cut in
+// `chunk`: whole chunks and, with taper, a first chunk of a quarter (its
+// staging comes before any scoring) and the last chunk's worth cut in
 // halves down to a quarter chunk (.., chunk / 2, chunk / 4, chunk / 4):
 // the work after the last scoring (its selection, tracebacks and records)
 // has no scoring left to run beside, so the last chunk is kept small.
+// Measured on 1M reads: 439-442 ms in the library with a whole first
+// chunk, 434-435 with a quarter (pairs: no change).
 std::vector<size_t> chunk_cuts(size_t n, size_t chunk, bool taper) {
     std::vector<size_t> cuts{0};
     for (size_t at = 0; at < n;) {
         const size_t left = n - at;
         size_t take = std::min(left, chunk);
-        if (taper && left <= chunk && 4 * left > chunk) take = (left + 1) / 2;
+        if (taper && at == 0) take = std::max<size_t>(1, chunk / 4);  // the first scoring starts sooner
+        else if (taper && left <= chunk && 4 * left > chunk) take = (left + 1) / 2;
         at += take;
         cuts.push_back(at);
     }
