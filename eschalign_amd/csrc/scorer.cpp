@@ -1877,6 +1877,8 @@ esch_status esch::align_core(esch_scorer *s, size_t n,
             const size_t i0 = np * t / gt, i1 = np * (t + 1) / gt;
             uint8_t mx = 0;
             for (size_t i = i0; i < i1; ++i) {
+                // the reads sit anywhere in the caller's memory: fetch ahead
+                if (i + 16 < i1) __builtin_prefetch(ptr[order[i + 16]]);
                 const uint8_t *q = ptr[order[i]];
                 uint8_t *d = hcodes + hqoff[i];
                 memcpy(d, q, (size_t)hqlen[i]);
