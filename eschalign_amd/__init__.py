@@ -377,33 +377,50 @@ def _pair_mapping(p, hits, ops):
                       tuple(phs))
 
 
-def _copy_out(ptr, n: int, dt: np.dtype) -> np.ndarray:
-    """n records of dtype dt at a library pointer, copied once."""
-    out = np.empty(n, dt)
-    if n:
-        C.memmove(out.ctypes.data, ptr, n * dt.itemsize)
-    return out
+class _MappedOwner:
+    """Owns an esch_mapped whose buffers arrays view in place; frees it when
+    the last such array is gone."""
+
+    __slots__ = ("m",)
+
+    def __init__(self, m):
+        self.m = m
+
+    def __del__(self):
+        m, self.m = self.m, None
+        if m is not None and m.value and _lib is not None:
+            _lib.esch_mapped_free(m)
+
+
+def _view(owner: _MappedOwner, ptr, n: int, dt: np.dtype) -> np.ndarray:
+    """n records of dtype dt at a pointer into owner's esch_mapped, viewed
+    without a copy (writable); the view keeps owner, and so the buffer,
+    alive."""
+    if not n:
+        return np.empty(0, dt)
+    buf = (C.c_char * (n * dt.itemsize)).from_address(ptr)
+    buf._owner = owner
+    return np.frombuffer(buf, dt)
 
 
 def _take_mapped(m, pairs: bool):
-    """Copy an esch_mapped's records out and free it: (records, hits, ops,
-    stats dict)."""
+    """An esch_mapped's records, hits and operation runs as arrays viewing
+    its buffers (freed with the last of them), and its stats: (records,
+    hits, ops, stats dict)."""
     L = _lib
-    try:
-        n = int(L.esch_mapped_len(m))
-        dt = PAIR_MAPPING_DTYPE if pairs else READ_MAPPING_DTYPE
-        rp = L.esch_mapped_pairs(m) if pairs else L.esch_mapped_reads(m)
-        nh, no = C.c_size_t(), C.c_size_t()
-        hp = L.esch_mapped_hits(m, C.byref(nh))
-        op = L.esch_mapped_ops(m, C.byref(no))
-        recs = _copy_out(rp, n, dt)
-        hits = _copy_out(hp, nh.value, HIT_DTYPE)
-        ops = _copy_out(op, no.value, np.dtype(np.uint32))
-        st = MapStats()
-        L.esch_mapped_stats(m, C.byref(st))
-        stats = {f: getattr(st, f) for f, _ in MapStats._fields_ if f != "reserved"}
-    finally:
-        L.esch_mapped_free(m)
+    owner = _MappedOwner(m)
+    n = int(L.esch_mapped_len(m))
+    dt = PAIR_MAPPING_DTYPE if pairs else READ_MAPPING_DTYPE
+    rp = L.esch_mapped_pairs(m) if pairs else L.esch_mapped_reads(m)
+    nh, no = C.c_size_t(), C.c_size_t()
+    hp = L.esch_mapped_hits(m, C.byref(nh))
+    op = L.esch_mapped_ops(m, C.byref(no))
+    recs = _view(owner, rp, n, dt)
+    hits = _view(owner, hp, nh.value, HIT_DTYPE)
+    ops = _view(owner, op, no.value, np.dtype(np.uint32))
+    st = MapStats()
+    L.esch_mapped_stats(m, C.byref(st))
+    stats = {f: getattr(st, f) for f, _ in MapStats._fields_ if f != "reserved"}
     return recs, hits, ops, stats
 
 
