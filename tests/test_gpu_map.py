@@ -363,3 +363,34 @@ def test_map_reads_with_swapped_chunks():
             compare_pairs(gpu, panel.codes, pairs, scoring, mode == ea.Mode.LOCAL, **kw)
         gpu.set_map_chunk(0)
         gpu.swap_rows = 0
+
+
+def test_mapped_arrays_outlive_the_scorer_and_each_other():
+    """The mirror's arrays view the library's result buffers: they stay whole
+    after the scorer is freed and a collection runs, two calls share no
+    memory, and writing one call's arrays changes nothing in the other's."""
+    import gc
+
+    rng = G.Rng(0x5EED0213)
+    panel_l = G.random_panel(rng, 12, 40, 120)
+    reads = [G.encode(r) for r in G.reads_for(rng, panel_l, 300)]
+    panel = ea.Panel(panel_l)
+    gpu = ea.GpuScorer(panel, ea.Scoring(*DEFAULT), ea.Mode.LOCAL)
+    a = gpu.map_reads_raw(reads, both_strands=True)
+    b = gpu.map_reads_raw(reads, both_strands=True)
+    del gpu
+    gc.collect()
+    want = O.map_reads(reads, panel.codes, DEFAULT, True, both_strands=True)
+    for recs, hits, ops, _ in (a, b):
+        assert recs.flags.writeable and hits.flags.writeable
+        got = [O.read_form(r, hits, ops) for r in recs]
+        assert got == want
+    for x, y in zip(a[:3], b[:3]):
+        assert not np.shares_memory(x, y)
+    recs_a, hits_a = a[0], a[1]
+    del a
+    gc.collect()
+    assert [O.read_form(r, b[1], b[2]) for r in b[0]] == want
+    recs_a[:] = np.zeros(1, recs_a.dtype)
+    hits_a[:] = np.zeros(1, hits_a.dtype)
+    assert [O.read_form(r, b[1], b[2]) for r in b[0]] == want
