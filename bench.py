@@ -354,10 +354,16 @@ def map_reads_leg(gpu, panel_codes, codes, off, lens, scoring, local, n_reads, c
     from oracle import pyoracle as O
 
     gpu.map_reads_concat(codes, off[:n_reads], lens[:n_reads], both_strands=True)  # warm: the call's buffers
-    t = time.perf_counter()
-    recs, hits, ops, st = gpu.map_reads_concat(codes, off[:n_reads], lens[:n_reads], both_strands=True)
-    wall = time.perf_counter() - t
-    out = {"reads": n_reads, "reads_per_s": round(n_reads / wall, 1), "wall_ms": round(wall * 1e3, 1),
+    calls = []
+    for _ in range(3):
+        t = time.perf_counter()
+        res = gpu.map_reads_concat(codes, off[:n_reads], lens[:n_reads], both_strands=True)
+        calls.append((time.perf_counter() - t, res))
+        del res
+    calls.sort(key=lambda c: c[0])
+    wall, (recs, hits, ops, st) = calls[1]  # the median of three calls
+    del calls
+    out = {"reads": n_reads, "calls": "median of 3", "reads_per_s": round(n_reads / wall, 1), "wall_ms": round(wall * 1e3, 1),
            "lib_ms": round(st["total_ms"], 1), "load_ms": round(st["load_ms"], 1),
            "score_ms": round(st["score_ms"], 1), "select_ms": round(st["select_ms"], 1),
            "trace_ms": round(st["trace_ms"], 1), "trace_wall_ms": round(st["trace_wall_ms"], 1),
