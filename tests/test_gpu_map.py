@@ -394,3 +394,18 @@ def test_mapped_arrays_outlive_the_scorer_and_each_other():
     recs_a[:] = np.zeros(1, recs_a.dtype)
     hits_a[:] = np.zeros(1, hits_a.dtype)
     assert [O.read_form(r, b[1], b[2]) for r in b[0]] == want
+
+
+def test_mapping_calls_with_no_reads():
+    """A call with no reads (no pairs) returns empty arrays and leaves the
+    scorer ready for the next call."""
+    rng = G.Rng(0x5EED0214)
+    panel_l = G.random_panel(rng, 6, 40, 80)
+    panel = ea.Panel(panel_l)
+    gpu = ea.GpuScorer(panel, ea.Scoring(*DEFAULT), ea.Mode.SEMIGLOBAL)
+    recs, hits, ops, st = gpu.map_reads_raw([], both_strands=True)
+    assert len(recs) == 0 and len(hits) == 0 and int(st["traced"]) == 0
+    precs, phits, pops, pst = gpu.map_pairs_raw([], [], both_strands=True)
+    assert len(precs) == 0 and len(phits) == 0
+    reads = [G.encode(r) for r in G.reads_for(rng, panel_l, 40)]
+    assert dev_reads(gpu, reads) == O.map_reads(reads, panel.codes, DEFAULT, False)
